@@ -1,0 +1,9 @@
+"""gaia_amd: MI355X-native Vertex Block Descent hot path.
+
+Host interface (mirrors the reference's VBDDynamics framework API) over the
+C ABI of libgaia_vbd.so (include/gaia_vbd.h), whose device work runs in
+hand-written HIP kernels for gfx950.
+"""
+from . import scenes  # noqa: F401
+from ._lib import LIB_PATH, VBDError, header_functions, lib  # noqa: F401
+from .vbd import Batch, Context, VBDPhysics, color_graph, color_tet_mesh, physics_params  # noqa: F401

@@ -1,0 +1,168 @@
+"""ctypes binding of include/gaia_vbd.h (libgaia_vbd.so).
+
+The library must exist in-tree (gaia_amd/libgaia_vbd.so, built by
+gaia_amd/build.py); a missing library is an error, never a fallback.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libgaia_vbd.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "gaia_vbd.h")
+
+GVBD_OK = 0
+GVBD_COLOR_MCS = 0
+GVBD_COLOR_GREEDY = 1
+
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class PhysicsParams(ctypes.Structure):
+    """gvbd_physics_params."""
+
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("time_step", ctypes.c_float),
+        ("num_substeps", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("world_lower", ctypes.c_float * 3),
+        ("world_upper", ctypes.c_float * 3),
+        ("use_plane_ground", ctypes.c_int32),
+        ("boundary_collision_stiffness", ctypes.c_float),
+        ("boundary_friction_dynamic", ctypes.c_float),
+        ("boundary_friction_eps_v", ctypes.c_float),
+        ("step_size", ctypes.c_float),
+        ("step_size_gd", ctypes.c_float),
+        ("friction_start_iter", ctypes.c_int32),
+        ("associate_gravity_with_inertia", ctypes.c_int32),
+    ]
+
+
+class TetMeshDesc(ctypes.Structure):
+    """gvbd_tet_mesh_desc."""
+
+    _fields_ = [
+        ("num_vertices", ctypes.c_int32),
+        ("positions", c_float_p),
+        ("velocities", c_float_p),
+        ("initial_velocity", ctypes.c_float * 3),
+        ("num_tets", ctypes.c_int32),
+        ("tets", c_int32_p),
+        ("density", ctypes.c_float),
+        ("mu", ctypes.c_float),
+        ("lmbd", ctypes.c_float),
+        ("damping_hydrostatic", ctypes.c_float),
+        ("damping_deviatoric", ctypes.c_float),
+        ("exponential_vel_damping", ctypes.c_float),
+        ("constant_vel_damping", ctypes.c_float),
+        ("initialization_type", ctypes.c_int32),
+        ("init_ratio_g", ctypes.c_float),
+        ("num_fixed", ctypes.c_int32),
+        ("fixed_points", c_int32_p),
+        ("vertex_colors", c_int32_p),
+        ("coloring_algorithm", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+# name -> argtypes (all return int unless listed in _RESTYPES)
+_SIGNATURES = {
+    "gvbd_abi_struct_sizes": [c_int32_p, c_int32_p],
+    "gvbd_context_create": [_I, ctypes.POINTER(_P)],
+    "gvbd_context_destroy": [_P],
+    "gvbd_last_error": [_P],
+    "gvbd_synchronize": [_P],
+    "gvbd_context_memory": [_P, c_int64_p],
+    "gvbd_scene_create": [_P, ctypes.POINTER(PhysicsParams), c_int32_p],
+    "gvbd_scene_destroy": [_P, _I],
+    "gvbd_scene_add_tet_mesh": [_P, _I, ctypes.POINTER(TetMeshDesc), c_int32_p],
+    "gvbd_scene_initialize": [_P, _I],
+    "gvbd_scene_load_json": [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_int32_p],
+    "gvbd_scene_num_meshes": [_P, _I, c_int32_p],
+    "gvbd_scene_mesh_info": [_P, _I, _I, c_int32_p, c_int32_p],
+    "gvbd_scene_num_parallel_groups": [_P, _I, c_int32_p],
+    "gvbd_scene_parallel_group": [_P, _I, _I, c_int32_p, c_int32_p],
+    "gvbd_scene_mesh_constants": [_P, _I, _I, c_float_p, c_float_p, c_float_p],
+    "gvbd_scene_set_schedule": [_P, _I, _I, _I],
+    "gvbd_scene_run_step": [_P, _I],
+    "gvbd_scene_run_steps": [_P, _I, _I],
+    "gvbd_scene_apply_initial_step": [_P, _I],
+    "gvbd_scene_solve_parallel_group": [_P, _I, _I, _I],
+    "gvbd_scene_update_velocities": [_P, _I],
+    "gvbd_scene_record_graph": [_P, _I, _I],
+    "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
+    "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
+    "gvbd_batch_create": [_P, c_int32_p, _I, c_int32_p],
+    "gvbd_batch_destroy": [_P, _I],
+    "gvbd_batch_set_schedule": [_P, _I, _I, _I],
+    "gvbd_batch_record_graph": [_P, _I, _I],
+    "gvbd_batch_run_steps": [_P, _I, _I],
+    "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
+    "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
+    "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],
+    "gvbd_event_record": [_P, _I],
+    "gvbd_event_elapsed_ms": [_P, _I, _I, c_float_p],
+    "gvbd_tet_mesh_constants": [_I, c_float_p, _I, c_int32_p, ctypes.c_float, c_float_p, c_float_p, c_float_p],
+    "gvbd_color_graph": [_I, _I, c_int32_p, _I, c_int32_p, c_int32_p],
+    "gvbd_color_tet_mesh": [_I, _I, c_int32_p, _I, c_int32_p, c_int32_p],
+}
+_RESTYPES = {"gvbd_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+class VBDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gaia_vbd error {code}: {msg}")
+        self.code = code
+
+
+def header_functions():
+    """Names of every function declared in include/gaia_vbd.h."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(gvbd_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    """Loads libgaia_vbd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc != GVBD_OK:
+        msg = lib().gvbd_last_error(ctx)
+        raise VBDError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def fptr(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def iptr(a):
+    return a.ctypes.data_as(c_int32_p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -1,0 +1,21 @@
+// Host-callable launchers of the HIP kernels (all asynchronous on `stream`).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_types.h"
+
+namespace gvbd {
+
+// Lanes per vertex (1, 2, 4, 8, 16) and block size (64, 128, 256) of the group solve.
+bool valid_schedule(int lpv, int block);
+
+// One parallel group: item i covers `count` consecutive colour-major slots of one scene.
+// Friction applies when iteration >= SolveParams::frictionStartIter (VBDPhysics.cpp:1610).
+hipError_t launch_tet_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
+                                  int lpv, int block, hipStream_t stream);
+// Items cover vertex ranges [slotBegin, slotBegin + count), count <= 256.
+hipError_t launch_tet_initial_step(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
+hipError_t launch_tet_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems,
+                                      hipStream_t stream);
+
+}  // namespace gvbd

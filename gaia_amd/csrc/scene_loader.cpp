@@ -1,0 +1,212 @@
+// Scene JSON loading; see scene_loader.h.
+#include "scene_loader.h"
+
+#include <cmath>
+#include <fstream>
+#include <sstream>
+
+#include "json_lite.h"
+
+namespace gvbd {
+
+namespace {
+
+bool read_file(const std::string& path, std::string& out, std::string& err) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << in.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+bool read_json(const std::string& path, json::Value& v, std::string& err) {
+    std::string text;
+    if (!read_file(path, text, err)) return false;
+    json::Parser p(text);
+    if (!p.parse(v, err)) {
+        err = path + ": " + err;
+        return false;
+    }
+    return true;
+}
+
+std::string substitute_root(std::string s, const std::string& root) {
+    const std::string key = "${REPO_ROOT}";
+    size_t at;
+    while ((at = s.find(key)) != std::string::npos) s.replace(at, key.size(), root);
+    for (char& c : s)
+        if (c == '\\') c = '/';
+    return s;
+}
+
+void get_num(const json::Value& o, const char* k, float& dst) {
+    const json::Value* v = o.get(k);
+    if (v && v->type == json::Value::Number) dst = (float)v->num;
+}
+void get_int(const json::Value& o, const char* k, int32_t& dst) {
+    const json::Value* v = o.get(k);
+    if (v && v->type == json::Value::Number) dst = (int32_t)v->num;
+    if (v && v->type == json::Value::Bool) dst = v->b ? 1 : 0;
+}
+void get_vec3(const json::Value& o, const char* k, float* dst) {
+    const json::Value* v = o.get(k);
+    if (v && v->type == json::Value::Array && v->arr.size() == 3)
+        for (int d = 0; d < 3; ++d) dst[d] = (float)v->arr[d].num;
+}
+std::string get_str(const json::Value& o, const char* k) {
+    const json::Value* v = o.get(k);
+    return v && v->type == json::Value::String ? v->str : std::string();
+}
+
+}  // namespace
+
+gvbd_physics_params default_physics_params() {
+    gvbd_physics_params p{};
+    p.gravity[0] = 0.f;
+    p.gravity[1] = -10.f;
+    p.gravity[2] = 0.f;
+    p.time_step = 0.01666666666666666666f;
+    p.num_substeps = 20;
+    p.iterations = 1;
+    for (int d = 0; d < 3; ++d) {
+        p.world_lower[d] = -1e7f;
+        p.world_upper[d] = 1e7f;
+    }
+    p.use_plane_ground = 1;
+    p.boundary_collision_stiffness = 1e5f;
+    p.boundary_friction_dynamic = 0.1f;
+    p.boundary_friction_eps_v = 1.0f;
+    p.step_size = 1.0f;
+    p.step_size_gd = 1e-3f;
+    p.friction_start_iter = 0;
+    p.associate_gravity_with_inertia = 1;
+    return p;
+}
+
+bool load_scene_json(const std::string& modelsPath, const std::string& paramsPath, const std::string& repoRoot,
+                     LoadedScene& out, std::string& err) {
+    json::Value models, params;
+    if (!read_json(modelsPath, models, err) || !read_json(paramsPath, params, err)) return false;
+    out.params = default_physics_params();
+    if (const json::Value* pp = params.get("PhysicsParams")) {
+        gvbd_physics_params& p = out.params;
+        get_vec3(*pp, "gravity", p.gravity);
+        get_num(*pp, "timeStep", p.time_step);
+        get_int(*pp, "numSubsteps", p.num_substeps);
+        get_int(*pp, "iterations", p.iterations);
+        if (const json::Value* wb = pp->get("worldBounds")) {
+            if (wb->type == json::Value::Array && wb->arr.size() == 2 && wb->arr[0].arr.size() == 3 &&
+                wb->arr[1].arr.size() == 3) {
+                for (int d = 0; d < 3; ++d) {
+                    p.world_lower[d] = (float)wb->arr[0].arr[d].num;
+                    p.world_upper[d] = (float)wb->arr[1].arr[d].num;
+                }
+            }
+        }
+        get_int(*pp, "usePlaneGround", p.use_plane_ground);
+        get_num(*pp, "boundaryCollisionStiffness", p.boundary_collision_stiffness);
+        get_num(*pp, "boundaryFrictionDynamic", p.boundary_friction_dynamic);
+        get_num(*pp, "boundaryFrictionEpsV", p.boundary_friction_eps_v);
+        get_num(*pp, "stepSize", p.step_size);
+        get_num(*pp, "stepSizeGD", p.step_size_gd);
+        get_int(*pp, "frictionStartIter", p.friction_start_iter);
+        get_int(*pp, "associateGravityWithInertia", p.associate_gravity_with_inertia);
+        int32_t useBowl = 0;
+        get_int(*pp, "useBowlGround", useBowl);
+        if (useBowl) {
+            err = "useBowlGround is not supported by this build (plane ground only)";
+            return false;
+        }
+    }
+    const json::Value* list = models.get("Models");
+    if (!list || list->type != json::Value::Array) {
+        err = modelsPath + ": missing \"Models\" array";
+        return false;
+    }
+    for (const json::Value& mj : list->arr) {
+        std::string material = get_str(mj, "materialName");
+        if (material != "NeoHookean") {
+            err = "materialName \"" + material + "\" is not supported for tet meshes (NeoHookean only)";
+            return false;
+        }
+        TetMeshHost m;
+        const std::string path = substitute_root(get_str(mj, "path"), repoRoot);
+        if (!load_t_file(path, m.pos, m.tets, err)) return false;
+        m.nv = (int32_t)(m.pos.size() / 3);
+        m.nt = (int32_t)(m.tets.size() / 4);
+        float rot[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, trans[3] = {0, 0, 0}, vel0[3] = {0, 0, 0};
+        get_vec3(mj, "rotation", rot);
+        get_vec3(mj, "scale", scale);
+        get_vec3(mj, "translation", trans);
+        get_vec3(mj, "initialVelocity", vel0);
+        const float angle = std::sqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2]);
+        float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
+        if (angle > 0.f) {
+            const float ax = rot[0] / angle, ay = rot[1] / angle, az = rot[2] / angle;
+            const float c = std::cos(angle), s = std::sin(angle), t = 1.f - c;
+            const float Rr[9] = {t * ax * ax + c,      t * ax * ay - s * az, t * ax * az + s * ay,
+                                 t * ax * ay + s * az, t * ay * ay + c,      t * ay * az - s * ax,
+                                 t * ax * az - s * ay, t * ay * az + s * ax, t * az * az + c};
+            for (int i = 0; i < 9; ++i) R[i] = Rr[i];
+        }
+        for (int32_t i = 0; i < m.nv; ++i) {
+            float* x = &m.pos[3 * i];
+            const float p0 = x[0], p1 = x[1], p2 = x[2];
+            for (int r = 0; r < 3; ++r) x[r] = R[3 * r] * p0 + R[3 * r + 1] * p1 + R[3 * r + 2] * p2;
+            for (int r = 0; r < 3; ++r) x[r] = x[r] * scale[r] + trans[r];
+        }
+        m.vel.resize(3 * (size_t)m.nv);
+        for (int32_t i = 0; i < m.nv; ++i)
+            for (int d = 0; d < 3; ++d) m.vel[3 * i + d] = vel0[d];
+        get_num(mj, "density", m.mat.density);
+        get_num(mj, "miu", m.mat.mu);
+        get_num(mj, "lmbd", m.mat.lambda);
+        get_num(mj, "dampingHydrostatic", m.mat.dampingHydrostatic);
+        get_num(mj, "dampingDeviatoric", m.mat.dampingDeviatoric);
+        get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
+        get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
+        get_int(mj, "initializationType", m.mat.initializationType);
+        if (m.mat.initializationType < 0 || m.mat.initializationType > 4) {
+            err = "initializationType must be 0..4 in this build";
+            return false;
+        }
+        if (const json::Value* fp = mj.get("fixedPoints"))
+            for (const json::Value& f : fp->arr) m.fixed.push_back((int32_t)f.num);
+        const std::string colorPath = substitute_root(get_str(mj, "verticesColoringCategoriesPath"), repoRoot);
+        if (!colorPath.empty()) {
+            json::Value cats;
+            if (!read_json(colorPath, cats, err)) return false;
+            m.colors.assign(m.nv, -1);
+            for (size_t c = 0; c < cats.arr.size(); ++c)
+                for (const json::Value& v : cats.arr[c].arr) {
+                    const int32_t id = (int32_t)v.num;
+                    if (id < 0 || id >= m.nv) {
+                        err = colorPath + ": vertex id out of range";
+                        return false;
+                    }
+                    m.colors[id] = (int32_t)c;
+                }
+            for (int32_t c : m.colors)
+                if (c < 0) {
+                    err = colorPath + ": colouring does not cover every vertex";
+                    return false;
+                }
+            Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
+            if (!coloring_valid(g, m.colors)) {
+                err = colorPath + ": colouring is not valid for " + path;
+                return false;
+            }
+        } else {
+            Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
+            m.colors = color_mcs(g);
+        }
+        out.meshes.push_back(std::move(m));
+    }
+    return true;
+}
+
+}  // namespace gvbd

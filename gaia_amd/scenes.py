@@ -1,0 +1,146 @@
+"""Seeded synthetic scene generators for the benchmark configurations.
+
+Every generator is deterministic given its arguments (and seed). Outputs are
+numpy arrays: positions (N, 3) float32/float64, tets (T, 4) int32.
+"""
+import json
+import os
+
+import numpy as np
+
+
+def cube_tet_mesh(n=10, h=0.1, origin=(0.0, 0.0, 0.0), dtype=np.float64):
+    """Regular n^3 hex grid with spacing h, each hex split into 5 tets.
+
+    Vertex (i, j, k) has index i + (n+1) * (j + (n+1) * k). In hex (i, j, k) the
+    central tet joins the four corners of even global parity (i+j+k+a+b+c even),
+    and each odd corner forms a corner tet with its three edge neighbours. Using
+    global parity makes the split alternate from hex to hex, so shared faces get
+    the same diagonal (a conforming mesh). Tets are oriented to positive volume.
+    At n = 10: 1331 vertices, 5000 tets, incidence 15.03 on average, 32 at most.
+    """
+    n1 = n + 1
+    kk, jj, ii = np.meshgrid(np.arange(n1), np.arange(n1), np.arange(n1), indexing="ij")
+    pos = np.stack([origin[0] + ii.ravel() * h, origin[1] + jj.ravel() * h, origin[2] + kk.ravel() * h],
+                   axis=1).astype(dtype)
+    # hexes in (k, j, i) order, i fastest; per hex: central tet, then the four corner tets
+    hk, hj, hi = [a.ravel() for a in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")]
+    par = (hi + hj + hk) % 2
+
+    def corner(a, b, c):
+        return (hi + a) + n1 * ((hj + b) + n1 * (hk + c))
+
+    def templates(p):
+        cs = [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+        even = sorted(x for x in cs if (p + sum(x)) % 2 == 0)
+        odd = sorted(x for x in cs if (p + sum(x)) % 2 == 1)
+        out = [even]
+        for o in odd:
+            out.append([o, (o[0] ^ 1, o[1], o[2]), (o[0], o[1] ^ 1, o[2]), (o[0], o[1], o[2] ^ 1)])
+        return out
+
+    tets = np.zeros((n ** 3, 5, 4), dtype=np.int64)
+    for p in (0, 1):
+        sel = par == p
+        for t, tmpl in enumerate(templates(p)):
+            for v, (a, b, c) in enumerate(tmpl):
+                tets[sel, t, v] = corner(a, b, c)[sel]
+    tets = tets.reshape(-1, 4).astype(np.int32)
+    # orient to positive volume
+    p = pos.astype(np.float64)
+    d = np.einsum("ij,ij->i", np.cross(p[tets[:, 1]] - p[tets[:, 0]], p[tets[:, 2]] - p[tets[:, 0]]),
+                  p[tets[:, 3]] - p[tets[:, 0]])
+    neg = d < 0
+    tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+    return pos, tets
+
+
+def c1_soft_cube():
+    """Plain VBD soft cube: 10x10x10 cells of 0.1 m (1331 vertices, 5000 tets), bottom at y = 0.5."""
+    pos, tets = cube_tet_mesh(10, 0.1, origin=(-0.5, 0.5, -0.5))
+    physics = dict(
+        gravity=(0.0, -9.8, 0.0),
+        time_step=1.0 / 60.0,
+        num_substeps=1,
+        iterations=10,
+        world_lower=(-1e7, 0.0, -1e7),
+        world_upper=(1e7, 1e7, 1e7),
+    )
+    material = dict(density=1000.0, mu=1e5, lmbd=1e6)
+    return dict(positions=pos, tets=tets, physics=physics, material=material, coloring="greedy", steps=300)
+
+
+def c4_tet_scene_sizes(seed=0, num_scenes=256, vmin=500, vmax=50000):
+    """Vertex-count targets drawn log-uniform in [vmin, vmax] (seeded)."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(vmin), np.log(vmax), size=num_scenes))
+
+
+def c4_tet_scenes(seed=0, num_scenes=256):
+    """Batched-VBD workload, tet-cube part: per scene a cube of n^3 cells with
+    (n+1)^3 close to a log-uniform vertex target in [500, 50k], 5 cm cells, resting
+    10 cm above the ground, stable neo-Hookean mu=1e5, lambda=1e6, rho=1000,
+    dt = 1/60 s, 10 iterations, greedy colouring."""
+    out = []
+    for i, target in enumerate(c4_tet_scene_sizes(seed, num_scenes)):
+        n = max(2, int(round(target ** (1.0 / 3.0))) - 1)
+        pos, tets = cube_tet_mesh(n, 0.05, origin=(0.0, 0.1, 0.0), dtype=np.float32)
+        physics = dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=10,
+                       world_lower=(-1e7, 0.0, -1e7), world_upper=(1e7, 1e7, 1e7))
+        out.append(dict(kind="tet", positions=pos, tets=tets, physics=physics,
+                        material=dict(density=1000.0, mu=1e5, lmbd=1e6), coloring="greedy"))
+    return out
+
+
+def write_t_file(path, pos, tets):
+    """MeshFrame .t text format ("Vertex i x y z", "Tet i a b c d")."""
+    with open(path, "w") as f:
+        for i, p in enumerate(pos):
+            f.write(f"Vertex {i} {p[0]:.9g} {p[1]:.9g} {p[2]:.9g}\n")
+        for i, t in enumerate(tets):
+            f.write(f"Tet {i} {t[0]} {t[1]} {t[2]} {t[3]}\n")
+
+
+def write_scene_json(folder, name, pos, tets, physics, material, colors=None, translation=(0.0, 0.0, 0.0)):
+    """Writes <name>.t, optional colouring, Models.json and Parameters.json in the reference schema
+    (Materials/Materials.h:55-75, VBD/VBD_NeoHookean.cpp:1613-1625, Parameters/PhysicsParameters.h:107-178)."""
+    os.makedirs(folder, exist_ok=True)
+    tpath = os.path.join(folder, name + ".t")
+    write_t_file(tpath, pos, tets)
+    model = {
+        "path": "${REPO_ROOT}/" + name + ".t",
+        "materialName": "NeoHookean",
+        "density": material.get("density", 1.0),
+        "miu": material.get("mu", 1e4),
+        "lmbd": material.get("lmbd", 1e4),
+        "translation": list(translation),
+        "rotation": [0.0, 0.0, 0.0],
+        "scale": [1.0, 1.0, 1.0],
+        "initialVelocity": [0.0, 0.0, 0.0],
+    }
+    if colors is not None:
+        cats = [np.flatnonzero(colors == c).tolist() for c in range(int(colors.max()) + 1)]
+        cpath = tpath + ".vertexColoring.json"
+        with open(cpath, "w") as f:
+            json.dump(cats, f)
+        model["verticesColoringCategoriesPath"] = model["path"] + ".vertexColoring.json"
+    p = physics
+    params = {
+        "PhysicsParams": {
+            "gravity": list(p.get("gravity", (0.0, -10.0, 0.0))),
+            "timeStep": p.get("time_step", 1.0 / 60.0),
+            "numSubsteps": p.get("num_substeps", 20),
+            "iterations": p.get("iterations", 1),
+            "worldBounds": [list(p.get("world_lower", (-1e7,) * 3)), list(p.get("world_upper", (1e7,) * 3))],
+            "usePlaneGround": True,
+            "boundaryFrictionDynamic": p.get("boundary_friction_dynamic", 0.1),
+        },
+        "CollisionParams": {"allowDCD": False, "allowCCD": False},
+    }
+    mpath = os.path.join(folder, "Models.json")
+    ppath = os.path.join(folder, "Parameters.json")
+    with open(mpath, "w") as f:
+        json.dump({"Models": [model]}, f, indent=1)
+    with open(ppath, "w") as f:
+        json.dump(params, f, indent=1)
+    return mpath, ppath

@@ -1,0 +1,347 @@
+"""Python host interface over the C ABI, mirroring the reference's VBD framework API.
+
+`VBDPhysics` follows GAIA::VBDPhysics (Simulator/Modules/VBD/VBDPhysics.h:36-254):
+loadRunningparameters -> initialize -> runStep, with the per-iteration units
+(applyInitialStep, solveParallelGroup, updateVelocities) and
+recordVBDSolveGraph. Every call goes through libgaia_vbd.so; the device work
+runs in hand-written HIP kernels.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import PhysicsParams, TetMeshDesc, check, fptr, f32, i32, iptr, lib
+
+# Reference defaults (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
+DEFAULT_PHYSICS = dict(
+    gravity=(0.0, -10.0, 0.0),
+    time_step=1.0 / 60.0,
+    num_substeps=20,
+    iterations=1,
+    world_lower=(-1e7, -1e7, -1e7),
+    world_upper=(1e7, 1e7, 1e7),
+    use_plane_ground=1,
+    boundary_collision_stiffness=1e5,
+    boundary_friction_dynamic=0.1,
+    boundary_friction_eps_v=1.0,
+    step_size=1.0,
+    step_size_gd=1e-3,
+    friction_start_iter=0,
+    associate_gravity_with_inertia=1,
+)
+
+# ObjectParametersVBDNeoHookean defaults (VBD/VBD_NeoHookean.h:25-29, VBD/VBD_BaseMaterial.h:29-45,
+# Materials/Materials.h:32).
+DEFAULT_TET_MATERIAL = dict(
+    density=1.0,
+    mu=1e4,
+    lmbd=1e4,
+    damping_hydrostatic=0.0,
+    damping_deviatoric=0.0,
+    exponential_vel_damping=1.0,
+    constant_vel_damping=0.0,
+    initialization_type=0,
+    init_ratio_g=0.5,
+)
+
+
+def physics_params(**overrides):
+    d = dict(DEFAULT_PHYSICS)
+    unknown = set(overrides) - set(d)
+    if unknown:
+        raise KeyError(f"unknown physics parameters {sorted(unknown)}")
+    d.update(overrides)
+    p = PhysicsParams()
+    for k, v in d.items():
+        if isinstance(v, (tuple, list)):
+            getattr(p, k)[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def physics_dict(**overrides):
+    d = dict(DEFAULT_PHYSICS)
+    d.update(overrides)
+    return d
+
+
+class Context:
+    """One device context (one HIP stream)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(lib().gvbd_context_create(device, ctypes.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("context closed")
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().gvbd_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, rc):
+        return check(rc, self._h)
+
+    def synchronize(self):
+        self.check(lib().gvbd_synchronize(self.handle))
+
+    def memory_bytes(self):
+        b = ctypes.c_int64()
+        self.check(lib().gvbd_context_memory(self.handle, ctypes.byref(b)))
+        return b.value
+
+    def event_record(self, slot):
+        self.check(lib().gvbd_event_record(self.handle, slot))
+
+    def event_elapsed_ms(self, a, b):
+        ms = ctypes.c_float()
+        self.check(lib().gvbd_event_elapsed_ms(self.handle, a, b, ctypes.byref(ms)))
+        return ms.value
+
+
+def color_tet_mesh(num_vertices, tets, algorithm="mcs"):
+    """Vertex colouring of a tet mesh's vertex graph (GraphColoring mcs/greedy)."""
+    tets = i32(tets).reshape(-1)
+    colors = np.zeros(num_vertices, dtype=np.int32)
+    nc = ctypes.c_int32()
+    alg = _lib.GVBD_COLOR_GREEDY if algorithm == "greedy" else _lib.GVBD_COLOR_MCS
+    check(lib().gvbd_color_tet_mesh(num_vertices, tets.size // 4, iptr(tets), alg, iptr(colors), ctypes.byref(nc)))
+    return colors
+
+
+def tet_mesh_constants(positions, tets, density):
+    """Rest-pose DmInv (T, 9 column-major), signed volume (T,) and lumped mass (N,)."""
+    pos = f32(positions).reshape(-1)
+    tets = i32(tets).reshape(-1)
+    nv, nt = pos.size // 3, tets.size // 4
+    dminv = np.zeros(9 * nt, np.float32)
+    vol = np.zeros(nt, np.float32)
+    mass = np.zeros(nv, np.float32)
+    check(lib().gvbd_tet_mesh_constants(nv, fptr(pos), nt, iptr(tets), float(density), fptr(dminv), fptr(vol),
+                                        fptr(mass)))
+    return dminv.reshape(nt, 9), vol, mass
+
+
+def color_graph(num_vertices, edges, algorithm="mcs"):
+    edges = i32(edges).reshape(-1)
+    colors = np.zeros(num_vertices, dtype=np.int32)
+    nc = ctypes.c_int32()
+    alg = _lib.GVBD_COLOR_GREEDY if algorithm == "greedy" else _lib.GVBD_COLOR_MCS
+    check(lib().gvbd_color_graph(num_vertices, edges.size // 2, iptr(edges), alg, iptr(colors), ctypes.byref(nc)))
+    return colors
+
+
+class VBDPhysics:
+    """One scene on one context (GAIA::VBDPhysics counterpart)."""
+
+    def __init__(self, ctx=None, device=0):
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.scene = None
+        self._pending = []
+        self._params = None
+        self._json = None
+
+    # ---- set-up (BasePhysicFramework::loadRunningparameters / initialize) ----
+    def loadRunningparameters(self, models_json, params_json, out_folder=None, repo_root=None):
+        self._json = (os.fspath(models_json), os.fspath(params_json), repo_root)
+        self.outFolder = out_folder
+
+    def setPhysicsParams(self, **overrides):
+        self._params = physics_params(**overrides)
+
+    def addTetMesh(self, positions, tets, fixed_points=(), vertex_colors=None, coloring="mcs", velocities=None,
+                   initial_velocity=(0.0, 0.0, 0.0), **material):
+        self._pending.append((positions, tets, fixed_points, vertex_colors, coloring, velocities, initial_velocity,
+                              material))
+
+    def initialize(self):
+        h = self.ctx.handle
+        sid = ctypes.c_int32()
+        if self._json is not None:
+            models, params, root = self._json
+            self.ctx.check(lib().gvbd_scene_load_json(h, models.encode(), params.encode(),
+                                                      (root or "").encode(), ctypes.byref(sid)))
+            self.scene = sid.value
+            return
+        params = self._params if self._params is not None else physics_params()
+        self.ctx.check(lib().gvbd_scene_create(h, ctypes.byref(params), ctypes.byref(sid)))
+        self.scene = sid.value
+        try:
+            for (pos, tets, fixed, colors, coloring, vel, v0, material) in self._pending:
+                self._add_tet(pos, tets, fixed, colors, coloring, vel, v0, material)
+            self.ctx.check(lib().gvbd_scene_initialize(h, self.scene))
+        except Exception:
+            lib().gvbd_scene_destroy(h, self.scene)
+            self.scene = None
+            raise
+        self._pending = []
+
+    def _add_tet(self, pos, tets, fixed, colors, coloring, vel, v0, material):
+        mat = dict(DEFAULT_TET_MATERIAL)
+        unknown = set(material) - set(mat)
+        if unknown:
+            raise KeyError(f"unknown material parameters {sorted(unknown)}")
+        mat.update(material)
+        pos = f32(pos).reshape(-1)
+        tets = i32(tets).reshape(-1)
+        fixed = i32(np.asarray(fixed, dtype=np.int32).reshape(-1))
+        d = TetMeshDesc()
+        d.num_vertices = pos.size // 3
+        d.positions = fptr(pos)
+        keep = [pos, tets, fixed]
+        if vel is not None:
+            vel = f32(vel).reshape(-1)
+            keep.append(vel)
+            d.velocities = fptr(vel)
+        d.initial_velocity[:] = [float(x) for x in v0]
+        d.num_tets = tets.size // 4
+        d.tets = iptr(tets)
+        for k in ("density", "mu", "lmbd", "damping_hydrostatic", "damping_deviatoric", "exponential_vel_damping",
+                  "constant_vel_damping", "init_ratio_g"):
+            setattr(d, k, float(mat[k]))
+        d.initialization_type = int(mat["initialization_type"])
+        d.num_fixed = fixed.size
+        d.fixed_points = iptr(fixed) if fixed.size else None
+        if colors is not None:
+            colors = i32(colors).reshape(-1)
+            keep.append(colors)
+            d.vertex_colors = iptr(colors)
+        d.coloring_algorithm = _lib.GVBD_COLOR_GREEDY if coloring == "greedy" else _lib.GVBD_COLOR_MCS
+        mid = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_add_tet_mesh(self.ctx.handle, self.scene, ctypes.byref(d), ctypes.byref(mid)))
+        del keep
+        return mid.value
+
+    # ---- stepping (VBDPhysics::runStep and its per-iteration units) ----
+    def runStep(self):
+        self.ctx.check(lib().gvbd_scene_run_step(self.ctx.handle, self.scene))
+
+    def runSteps(self, n):
+        self.ctx.check(lib().gvbd_scene_run_steps(self.ctx.handle, self.scene, n))
+
+    def applyInitialStep(self):
+        self.ctx.check(lib().gvbd_scene_apply_initial_step(self.ctx.handle, self.scene))
+
+    def solveParallelGroup(self, group, iIter=0):
+        self.ctx.check(lib().gvbd_scene_solve_parallel_group(self.ctx.handle, self.scene, group, iIter))
+
+    def updateVelocities(self):
+        self.ctx.check(lib().gvbd_scene_update_velocities(self.ctx.handle, self.scene))
+
+    def recordVBDSolveGraph(self, enable=True):
+        self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))
+
+    def setSchedule(self, lanes_per_vertex, block):
+        self.ctx.check(lib().gvbd_scene_set_schedule(self.ctx.handle, self.scene, lanes_per_vertex, block))
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
+    # ---- queries / state ----
+    def numMeshes(self):
+        n = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_num_meshes(self.ctx.handle, self.scene, ctypes.byref(n)))
+        return n.value
+
+    def meshInfo(self, mesh):
+        nv, nt = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_mesh_info(self.ctx.handle, self.scene, mesh, ctypes.byref(nv), ctypes.byref(nt)))
+        return nv.value, nt.value
+
+    def parallelGroups(self):
+        n = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_num_parallel_groups(self.ctx.handle, self.scene, ctypes.byref(n)))
+        out = []
+        for g in range(n.value):
+            cnt = ctypes.c_int32()
+            self.ctx.check(lib().gvbd_scene_parallel_group(self.ctx.handle, self.scene, g, None, ctypes.byref(cnt)))
+            buf = np.zeros(2 * cnt.value, dtype=np.int32)
+            self.ctx.check(lib().gvbd_scene_parallel_group(self.ctx.handle, self.scene, g, iptr(buf), ctypes.byref(cnt)))
+            out.append(buf.reshape(-1, 2))
+        return out
+
+    def meshConstants(self, mesh):
+        nv, nt = self.meshInfo(mesh)
+        dminv = np.zeros(9 * nt, dtype=np.float32)
+        vol = np.zeros(nt, dtype=np.float32)
+        mass = np.zeros(nv, dtype=np.float32)
+        self.ctx.check(lib().gvbd_scene_mesh_constants(self.ctx.handle, self.scene, mesh, fptr(dminv), fptr(vol),
+                                                       fptr(mass)))
+        return dminv.reshape(nt, 9), vol, mass
+
+    def getState(self, mesh=0):
+        nv, _ = self.meshInfo(mesh)
+        x = np.zeros(3 * nv, dtype=np.float32)
+        v = np.zeros(3 * nv, dtype=np.float32)
+        self.ctx.check(lib().gvbd_scene_get_mesh_state(self.ctx.handle, self.scene, mesh, fptr(x), fptr(v)))
+        return x.reshape(nv, 3), v.reshape(nv, 3)
+
+    def setState(self, mesh, positions=None, velocities=None):
+        x = f32(positions).reshape(-1) if positions is not None else None
+        v = f32(velocities).reshape(-1) if velocities is not None else None
+        self.ctx.check(lib().gvbd_scene_set_mesh_state(self.ctx.handle, self.scene, mesh,
+                                                       fptr(x) if x is not None else None,
+                                                       fptr(v) if v is not None else None))
+
+    def destroy(self):
+        if self.scene is not None and self.ctx._h:
+            lib().gvbd_scene_destroy(self.ctx.handle, self.scene)
+        self.scene = None
+
+
+class Batch:
+    """Many initialised scenes advanced by one launch per parallel group."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        ids = i32([s.scene if isinstance(s, VBDPhysics) else int(s) for s in scenes])
+        b = ctypes.c_int32()
+        ctx.check(lib().gvbd_batch_create(ctx.handle, iptr(ids), ids.size, ctypes.byref(b)))
+        self.batch = b.value
+
+    def set_schedule(self, lanes_per_vertex, block):
+        self.ctx.check(lib().gvbd_batch_set_schedule(self.ctx.handle, self.batch, lanes_per_vertex, block))
+
+    def record_graph(self, enable=True):
+        self.ctx.check(lib().gvbd_batch_record_graph(self.ctx.handle, self.batch, 1 if enable else 0))
+
+    def run_steps(self, n):
+        self.ctx.check(lib().gvbd_batch_run_steps(self.ctx.handle, self.batch, n))
+
+    def solve_group(self, group, iteration=0):
+        self.ctx.check(lib().gvbd_batch_solve_parallel_group(self.ctx.handle, self.batch, group, iteration))
+
+    def num_groups(self):
+        n = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_batch_num_parallel_groups(self.ctx.handle, self.batch, ctypes.byref(n)))
+        return n.value
+
+    def group_work(self, group):
+        b, v = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx.check(lib().gvbd_batch_group_work(self.ctx.handle, self.batch, group, ctypes.byref(b), ctypes.byref(v)))
+        return b.value, v.value
+
+    def destroy(self):
+        if self.batch is not None and self.ctx._h:
+            lib().gvbd_batch_destroy(self.ctx.handle, self.batch)
+        self.batch = None

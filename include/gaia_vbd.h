@@ -1,0 +1,178 @@
+/*
+ * gaia_vbd.h -- C ABI of the MI355X-native Vertex Block Descent hot path.
+ *
+ * This is the drop-in boundary between a VBD host framework (scene loading,
+ * frame loop, output) and the device hot path. Every entry point replaces a
+ * reference interface, cited as Simulator/Modules/<file>:<line>:
+ *
+ *   gvbd_scene_load_json        VBDPhysics::loadRunningparameters + initialize   (VBD/VBDPhysics.h:50-52)
+ *   gvbd_scene_add_tet_mesh     BasePhysicFramework::initialize mesh load +
+ *                               TetMeshFEM::initialize                           (Framework/BasePhysicsFramework.cpp:101-202,
+ *                                                                                 TetMesh/TetMeshFEM.cpp:332-443)
+ *   gvbd_scene_initialize       VBDPhysics::initialize (parallel groups, GPU upload) (VBD/VBDPhysics.cpp:208-454,563-651)
+ *   gvbd_scene_run_step         VBDPhysics::runStep / runStepGPU                 (VBD/VBDPhysics.cpp:809-969)
+ *   gvbd_scene_apply_initial_step VBDBaseTetMesh::applyInitialStep              (VBD/VBD_BaseMaterial.h:244-259)
+ *   gvbd_scene_solve_parallel_group VBDSolveParallelGroup_vertexSweep_2hierarchiesGPU +
+ *                               ..._updateVertexPositionGPU                      (VBD/VBDPhysicsCompute.h:10-32)
+ *   gvbd_scene_update_velocities VBDUpdateVelocityGPU / updateVelocities         (VBD/VBDPhysicsCompute.h:30,
+ *                                                                                 VBD/VBDPhysics.cpp:2486-2561)
+ *   gvbd_scene_record_graph     VBDPhysics::recordVBDSolveGraph                  (VBD/VBDPhysics.cpp:1546-1570)
+ *   gvbd_scene_get/set_mesh_state PhysicsState positions/velocities (checkpoint) (Framework/BasePhysicsFramework.h:145-319)
+ *   gvbd_color_*                GraphColoring Mcs / OrderedGreedy                (GraphColoring/mcs.cpp:14-168,
+ *                                                                                 GraphColoring/gready.cpp:15-117)
+ *   gvbd_batch_*                (no reference counterpart: many scenes per launch; the reference's
+ *                                nearest form is the multi-mesh parallel-group merge, VBD/VBDPhysics.cpp:235-274)
+ *
+ * Conventions: every function returns GVBD_OK (0) or a negative status; the
+ * message is available from gvbd_last_error(ctx) (or gvbd_last_error(NULL) for
+ * calls without a context). No exceptions cross the ABI. The caller owns host
+ * arrays; the library owns device memory. One context owns one HIP stream;
+ * calls on one context must be externally synchronised. Stepping calls are
+ * asynchronous on the context's stream; gvbd_synchronize waits for them.
+ * Positions/velocities cross the ABI as float xyz triples (3 * n floats).
+ */
+#ifndef GAIA_VBD_H
+#define GAIA_VBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVBD_OK 0
+#define GVBD_ERR_INVALID (-1)
+#define GVBD_ERR_HIP (-2)
+#define GVBD_ERR_IO (-3)
+#define GVBD_ERR_STATE (-4)
+#define GVBD_ERR_NOMEM (-5)
+#define GVBD_ERR_INTERNAL (-6)
+
+#define GVBD_COLOR_MCS 0
+#define GVBD_COLOR_GREEDY 1
+
+typedef struct gvbd_context gvbd_context;
+
+/* VBDPhysicsParameters + BasePhysicsParams fields on the hot path
+ * (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
+ * dt = time_step / num_substeps (Framework/BasePhysicsFramework.cpp:245-249). */
+typedef struct gvbd_physics_params {
+    float gravity[3];
+    float time_step;
+    int32_t num_substeps;
+    int32_t iterations;
+    float world_lower[3]; /* worldBounds[0] */
+    float world_upper[3]; /* worldBounds[1] */
+    int32_t use_plane_ground;
+    float boundary_collision_stiffness;
+    float boundary_friction_dynamic;
+    float boundary_friction_eps_v;
+    float step_size;
+    float step_size_gd;
+    int32_t friction_start_iter;
+    int32_t associate_gravity_with_inertia;
+} gvbd_physics_params;
+
+/* One tet mesh (ObjectParametersVBDNeoHookean, VBD/VBD_NeoHookean.cpp:1613-1625,
+ * VBD/VBD_BaseMaterial.h:21-46, Materials/Materials.h:12-75). Positions are
+ * world-space (rotation/scale/translation already applied). */
+typedef struct gvbd_tet_mesh_desc {
+    int32_t num_vertices;
+    const float* positions;   /* 3 * num_vertices */
+    const float* velocities;  /* 3 * num_vertices, or NULL for initial_velocity */
+    float initial_velocity[3];
+    int32_t num_tets;
+    const int32_t* tets;      /* 4 * num_tets */
+    float density;
+    float mu;                 /* "miu" */
+    float lambda;             /* "lmbd" */
+    float damping_hydrostatic;
+    float damping_deviatoric;
+    float exponential_vel_damping; /* raw value; pow(., dt) is applied (VBD/VBDPhysics.cpp:794-807) */
+    float constant_vel_damping;    /* raw value; multiplied by dt */
+    int32_t initialization_type;   /* 0..4 (VBD/VBD_BaseMaterial.cpp:7-50) */
+    float init_ratio_g;
+    int32_t num_fixed;
+    const int32_t* fixed_points;
+    const int32_t* vertex_colors;  /* colour per vertex, or NULL to colour here */
+    int32_t coloring_algorithm;    /* GVBD_COLOR_MCS / GVBD_COLOR_GREEDY when vertex_colors is NULL */
+} gvbd_tet_mesh_desc;
+
+/* sizes of the ABI structs as compiled into the library (binding self-check) */
+int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes);
+
+/* ---- context ---- */
+int gvbd_context_create(int32_t device, gvbd_context** out);
+int gvbd_context_destroy(gvbd_context* ctx);
+const char* gvbd_last_error(const gvbd_context* ctx);
+int gvbd_synchronize(gvbd_context* ctx);
+/* device bytes currently held by scenes and plans of this context */
+int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
+
+/* ---- scenes ---- */
+int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
+int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene);
+int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mesh_desc* desc, int32_t* mesh_out);
+int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene);
+/* Models.json + Parameters.json; "${REPO_ROOT}" in paths is replaced by repo_root (may be NULL). */
+int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char* params_json,
+                         const char* repo_root, int32_t* scene_out);
+
+int gvbd_scene_num_meshes(gvbd_context* ctx, int32_t scene, int32_t* n);
+int gvbd_scene_mesh_info(gvbd_context* ctx, int32_t scene, int32_t mesh, int32_t* num_vertices,
+                         int32_t* num_tets);
+int gvbd_scene_num_parallel_groups(gvbd_context* ctx, int32_t scene, int32_t* n);
+/* interleaved (mesh, vertex) pairs; pairs may be NULL to query the count */
+int gvbd_scene_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t* pairs, int32_t* count);
+/* rest-pose constants of one mesh: dminv 9*T (column-major), volume T, mass N (any may be NULL) */
+int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, float* dminv, float* volume,
+                              float* mass);
+/* lanes per vertex (1,2,4,8,16) and block size (64,128,256) of the vertex solve */
+int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_vertex, int32_t block);
+
+/* ---- stepping (asynchronous on the context stream) ---- */
+int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene);               /* all substeps of one frame */
+int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n);
+int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene);
+int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t iteration);
+int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
+/* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
+ * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
+int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);
+
+/* ---- state I/O (synchronous) ---- */
+int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, float* positions, float* velocities);
+int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, const float* positions,
+                              const float* velocities);
+
+/* ---- batched stepping: many scenes per launch ---- */
+int gvbd_batch_create(gvbd_context* ctx, const int32_t* scenes, int32_t n, int32_t* batch_out);
+int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch);
+int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_vertex, int32_t block);
+int gvbd_batch_record_graph(gvbd_context* ctx, int32_t batch, int32_t enable);
+int gvbd_batch_run_steps(gvbd_context* ctx, int32_t batch, int32_t n);
+/* launch only the vertex solve of parallel group `group` (timing of the dominant kernel) */
+int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
+int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);
+/* algorithmic bytes of one solve launch of `group` (gather model; see DESIGN) and its vertex count */
+int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
+
+/* ---- timing on the context stream ---- */
+int gvbd_event_record(gvbd_context* ctx, int32_t slot);  /* slot 0..63 */
+int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, float* ms);
+
+/* ---- rest-pose precompute (host; TetMeshFEM::initialize, TetMesh/TetMeshFEM.cpp:409-429) ----
+ * dminv 9*T column-major DmInv, volume T signed det(Dm)/6, mass N lumped vol*density/4 */
+int gvbd_tet_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_tets, const int32_t* tets,
+                            float density, float* dminv, float* volume, float* mass);
+
+/* ---- colouring (host) ---- */
+int gvbd_color_graph(int32_t num_vertices, int32_t num_edges, const int32_t* edges, int32_t algorithm,
+                     int32_t* colors, int32_t* num_colors);
+int gvbd_color_tet_mesh(int32_t num_vertices, int32_t num_tets, const int32_t* tets, int32_t algorithm,
+                        int32_t* colors, int32_t* num_colors);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GAIA_VBD_H */

@@ -1,0 +1,91 @@
+/*
+ * CPU oracle for the Vertex Block Descent hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is a plain C++ restatement of the
+ * reference's CPU (TBB) VBD path, in FP32 (parity) and FP64 (error yardstick).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only as the checker or as the timed CPU baseline; the product
+ * path (gaia_amd) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates. Parts that no
+ * fixture, closed form or invariant pins are marked "parity unpinned".
+ *
+ * Flat parameter vector layout (double[OP_COUNT]) shared with the tests.
+ */
+#pragma once
+#include <stdint.h>
+
+enum OraclePhysicsParam {
+    OP_GRAVITY_X = 0, OP_GRAVITY_Y, OP_GRAVITY_Z,
+    OP_DT,                       /* already timeStep / numSubsteps */
+    OP_NUM_SUBSTEPS,
+    OP_ITERATIONS,
+    OP_LOWER_X, OP_LOWER_Y, OP_LOWER_Z,
+    OP_UPPER_X, OP_UPPER_Y, OP_UPPER_Z,
+    OP_USE_PLANE_GROUND,
+    OP_BOUNDARY_STIFFNESS,
+    OP_BOUNDARY_FRICTION_MU,
+    OP_BOUNDARY_FRICTION_EPS_V,
+    OP_STEP_SIZE,
+    OP_STEP_SIZE_GD,
+    OP_FRICTION_START_ITER,
+    OP_ASSOCIATE_GRAVITY,
+    OP_INIT_TYPE_UNUSED,         /* per mesh: OM_INIT_TYPE */
+    OP_INIT_RATIO_G,
+    OP_COUNT = 32
+};
+
+enum OracleMaterialParam {
+    OM_DENSITY = 0,
+    OM_MU, OM_LAMBDA,
+    OM_DAMP_HYDRO, OM_DAMP_DEV,
+    OM_EXP_VEL_DAMPING,          /* raw JSON value; the oracle applies pow(., dt) */
+    OM_CONST_VEL_DAMPING,        /* raw JSON value; the oracle multiplies by dt */
+    OM_INIT_TYPE,                /* ObjectParamsVBD::initializationType */
+    OM_COUNT = 8
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- mesh precompute (TetMeshFEM.cpp:409-429) --- */
+int oracle_tet_prepare_f64(int nv, const double* pos, int nt, const int32_t* tets, double density,
+                           double* dminv, double* vol, double* mass);
+int oracle_tet_prepare_f32(int nv, const float* pos, int nt, const int32_t* tets, float density,
+                           float* dminv, float* vol, float* mass);
+
+/* --- per-tet material force / Hessian for one vertex (VBD_NeoHookean.cpp:379-581) --- */
+void oracle_neohookean_vertex_f64(const double* x4 /*12: tet vertex positions*/, const double* dminv /*9 col-major*/,
+                                  double vol, double mu, double lmbd, int order,
+                                  double* force3, double* hess9);
+
+/* --- 3x3 solve (own closed form; CuMatrix::solve3x3_psd_stable is absent: parity unpinned) --- */
+int oracle_solve3x3_f64(const double* h9, const double* b3, double* x3);
+int oracle_solve3x3_f32(const float* h9, const float* b3, float* x3);
+
+/* --- coloring (mcs.cpp:14-168, gready.cpp:15-117, TetMeshVertexGraph.h:15-41) --- */
+int oracle_color_tet_mesh(int nv, int nt, const int32_t* tets, int algorithm /*0 mcs, 1 greedy*/, int32_t* colors);
+int oracle_color_graph(int nv, int ne, const int32_t* edges, int algorithm, int32_t* colors);
+
+/* --- tet VBD scene (VBDPhysics.cpp runStep_serialCollisionHandling) --- */
+typedef struct OracleTetScene OracleTetScene;
+
+/* precision: 32 or 64 */
+OracleTetScene* oracle_tet_scene_create(int precision, const double* physics /*OP_COUNT*/);
+void oracle_tet_scene_destroy(OracleTetScene*);
+/* colors: per-vertex color id (categories = vertices per color, ascending id) */
+int oracle_tet_scene_add_mesh(OracleTetScene*, int nv, const double* pos, const double* vel, int nt,
+                              const int32_t* tets, const double* material /*OM_COUNT*/,
+                              int nfixed, const int32_t* fixed, const int32_t* colors);
+int oracle_tet_scene_finalize(OracleTetScene*);
+int oracle_tet_scene_num_groups(OracleTetScene*);
+/* group g as interleaved (mesh, vertex) pairs; returns pair count, writes when out != NULL */
+int oracle_tet_scene_group(OracleTetScene*, int g, int32_t* out);
+int oracle_tet_scene_step(OracleTetScene*, int nsteps, int nthreads);
+int oracle_tet_scene_get_state(OracleTetScene*, int mesh, double* pos, double* vel);
+int oracle_tet_scene_set_state(OracleTetScene*, int mesh, const double* pos, const double* vel);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+# flat parameter indices (oracle.h)
+OP = dict(GRAVITY_X=0, GRAVITY_Y=1, GRAVITY_Z=2, DT=3, NUM_SUBSTEPS=4, ITERATIONS=5, LOWER_X=6, LOWER_Y=7,
+          LOWER_Z=8, UPPER_X=9, UPPER_Y=10, UPPER_Z=11, USE_PLANE_GROUND=12, BOUNDARY_STIFFNESS=13,
+          BOUNDARY_FRICTION_MU=14, BOUNDARY_FRICTION_EPS_V=15, STEP_SIZE=16, STEP_SIZE_GD=17,
+          FRICTION_START_ITER=18, ASSOCIATE_GRAVITY=19, INIT_RATIO_G=21)
+OP_COUNT = 32
+OM = dict(DENSITY=0, MU=1, LAMBDA=2, DAMP_HYDRO=3, DAMP_DEV=4, EXP_VEL_DAMPING=5, CONST_VEL_DAMPING=6, INIT_TYPE=7)
+OM_COUNT = 8
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: build with `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_tet_prepare_f64.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_tet_prepare_f32.argtypes = [ctypes.c_int, _fp, ctypes.c_int, _ip, ctypes.c_float, _fp, _fp, _fp]
+        L.oracle_neohookean_vertex_f64.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_int, _dp, _dp]
+        L.oracle_neohookean_vertex_f64.restype = None
+        L.oracle_solve3x3_f64.argtypes = [_dp, _dp, _dp]
+        L.oracle_solve3x3_f32.argtypes = [_fp, _fp, _fp]
+        L.oracle_color_tet_mesh.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip]
+        L.oracle_color_graph.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, _ip]
+        L.oracle_tet_scene_create.argtypes = [ctypes.c_int, _dp]
+        L.oracle_tet_scene_create.restype = ctypes.c_void_p
+        L.oracle_tet_scene_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_tet_scene_destroy.restype = None
+        L.oracle_tet_scene_add_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                                ctypes.c_int, _ip, _ip]
+        L.oracle_tet_scene_finalize.argtypes = [ctypes.c_void_p]
+        L.oracle_tet_scene_num_groups.argtypes = [ctypes.c_void_p]
+        L.oracle_tet_scene_group.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+        L.oracle_tet_scene_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_tet_scene_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_tet_scene_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def physics_vector(p):
+    """Flat oracle parameter vector from a physics dict (gaia_amd.vbd.physics_dict keys)."""
+    v = np.zeros(OP_COUNT)
+    g = p["gravity"]
+    v[0:3] = g
+    v[OP["DT"]] = np.float32(np.float32(p["time_step"]) / np.float32(p["num_substeps"]))
+    v[OP["NUM_SUBSTEPS"]] = p["num_substeps"]
+    v[OP["ITERATIONS"]] = p["iterations"]
+    v[6:9] = p["world_lower"]
+    v[9:12] = p["world_upper"]
+    v[OP["USE_PLANE_GROUND"]] = p["use_plane_ground"]
+    v[OP["BOUNDARY_STIFFNESS"]] = p["boundary_collision_stiffness"]
+    v[OP["BOUNDARY_FRICTION_MU"]] = p["boundary_friction_dynamic"]
+    v[OP["BOUNDARY_FRICTION_EPS_V"]] = p["boundary_friction_eps_v"]
+    v[OP["STEP_SIZE"]] = p["step_size"]
+    v[OP["STEP_SIZE_GD"]] = p["step_size_gd"]
+    v[OP["FRICTION_START_ITER"]] = p["friction_start_iter"]
+    v[OP["ASSOCIATE_GRAVITY"]] = p["associate_gravity_with_inertia"]
+    v[OP["INIT_RATIO_G"]] = p.get("init_ratio_g", 0.5)
+    return v
+
+
+def material_vector(m):
+    v = np.zeros(OM_COUNT)
+    v[OM["DENSITY"]] = m.get("density", 1.0)
+    v[OM["MU"]] = m.get("mu", 1e4)
+    v[OM["LAMBDA"]] = m.get("lmbd", 1e4)
+    v[OM["DAMP_HYDRO"]] = m.get("damping_hydrostatic", 0.0)
+    v[OM["DAMP_DEV"]] = m.get("damping_deviatoric", 0.0)
+    v[OM["EXP_VEL_DAMPING"]] = m.get("exponential_vel_damping", 1.0)
+    v[OM["CONST_VEL_DAMPING"]] = m.get("constant_vel_damping", 0.0)
+    v[OM["INIT_TYPE"]] = m.get("initialization_type", 0)
+    return v
+
+
+def tet_prepare(pos, tets, density, precision=64):
+    pos = np.asarray(pos).reshape(-1, 3)
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    nv, nt = len(pos), len(tets)
+    if precision == 64:
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        dminv, vol, mass = np.zeros(9 * nt), np.zeros(nt), np.zeros(nv)
+        rc = lib().oracle_tet_prepare_f64(nv, p.ctypes.data_as(_dp), nt, tets.ctypes.data_as(_ip), density,
+                                          dminv.ctypes.data_as(_dp), vol.ctypes.data_as(_dp), mass.ctypes.data_as(_dp))
+    else:
+        p = np.ascontiguousarray(pos, dtype=np.float32)
+        dminv = np.zeros(9 * nt, np.float32)
+        vol = np.zeros(nt, np.float32)
+        mass = np.zeros(nv, np.float32)
+        rc = lib().oracle_tet_prepare_f32(nv, p.ctypes.data_as(_fp), nt, tets.ctypes.data_as(_ip), density,
+                                          dminv.ctypes.data_as(_fp), vol.ctypes.data_as(_fp), mass.ctypes.data_as(_fp))
+    if rc != 0:
+        raise ValueError("degenerate tet")
+    return dminv.reshape(nt, 9), vol, mass
+
+
+def neohookean_vertex(x4, dminv, vol, mu, lmbd, order):
+    x4, px = _d(np.asarray(x4).reshape(-1))
+    dm, pd = _d(np.asarray(dminv).reshape(-1))
+    f = np.zeros(3)
+    h = np.zeros(9)
+    lib().oracle_neohookean_vertex_f64(px, pd, vol, mu, lmbd, order, f.ctypes.data_as(_dp), h.ctypes.data_as(_dp))
+    return f, h.reshape(3, 3, order="F")
+
+
+def solve3x3(h, b, precision=64):
+    if precision == 64:
+        h = np.asfortranarray(h, dtype=np.float64).reshape(-1, order="F").copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(3)
+        ok = lib().oracle_solve3x3_f64(h.ctypes.data_as(_dp), b.ctypes.data_as(_dp), x.ctypes.data_as(_dp))
+    else:
+        h = np.asarray(h, dtype=np.float32).reshape(-1, order="F").copy()
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.zeros(3, np.float32)
+        ok = lib().oracle_solve3x3_f32(h.ctypes.data_as(_fp), b.ctypes.data_as(_fp), x.ctypes.data_as(_fp))
+    return bool(ok), x
+
+
+def color_tet_mesh(nv, tets, algorithm="mcs"):
+    tets, pt = _i(np.asarray(tets).reshape(-1))
+    colors = np.zeros(nv, np.int32)
+    lib().oracle_color_tet_mesh(nv, tets.size // 4, pt, 0 if algorithm == "mcs" else 1, colors.ctypes.data_as(_ip))
+    return colors
+
+
+def color_graph(nv, edges, algorithm="mcs"):
+    edges, pe = _i(np.asarray(edges).reshape(-1))
+    colors = np.zeros(nv, np.int32)
+    lib().oracle_color_graph(nv, edges.size // 2, pe, 0 if algorithm == "mcs" else 1, colors.ctypes.data_as(_ip))
+    return colors
+
+
+class TetScene:
+    """Oracle tet VBD scene (VBDPhysics::runStep_serialCollisionHandling restated)."""
+
+    def __init__(self, physics, precision=32):
+        pv, pp = _d(physics_vector(physics))
+        self._h = lib().oracle_tet_scene_create(precision, pp)
+        self.nmesh = 0
+        self.nv = []
+
+    def add_mesh(self, pos, tets, material, colors, fixed=(), vel=None):
+        pos = np.asarray(pos).reshape(-1, 3)
+        nv = len(pos)
+        p, pp = _d(pos.reshape(-1))
+        v = None
+        pv = None
+        if vel is not None:
+            v, pv = _d(np.asarray(vel).reshape(-1))
+        t, pt = _i(np.asarray(tets).reshape(-1))
+        m, pm = _d(material_vector(material))
+        f, pf = _i(np.asarray(fixed, dtype=np.int32).reshape(-1))
+        c, pc = _i(colors)
+        rc = lib().oracle_tet_scene_add_mesh(self._h, nv, pp, pv, t.size // 4, pt, pm, f.size, pf, pc)
+        if rc < 0:
+            raise ValueError("oracle: bad mesh")
+        self.nmesh += 1
+        self.nv.append(nv)
+        return rc
+
+    def finalize(self):
+        lib().oracle_tet_scene_finalize(self._h)
+
+    def groups(self):
+        out = []
+        for g in range(lib().oracle_tet_scene_num_groups(self._h)):
+            n = lib().oracle_tet_scene_group(self._h, g, None)
+            buf = np.zeros(2 * n, np.int32)
+            lib().oracle_tet_scene_group(self._h, g, buf.ctypes.data_as(_ip))
+            out.append(buf.reshape(-1, 2))
+        return out
+
+    def step(self, n=1, threads=1):
+        lib().oracle_tet_scene_step(self._h, n, threads)
+
+    def get_state(self, mesh=0):
+        nv = self.nv[mesh]
+        x = np.zeros(3 * nv)
+        v = np.zeros(3 * nv)
+        lib().oracle_tet_scene_get_state(self._h, mesh, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp))
+        return x.reshape(nv, 3), v.reshape(nv, 3)
+
+    def set_state(self, mesh, x=None, v=None):
+        px = pv = None
+        if x is not None:
+            xa, px = _d(np.asarray(x).reshape(-1))
+        if v is not None:
+            va, pv = _d(np.asarray(v).reshape(-1))
+        lib().oracle_tet_scene_set_state(self._h, mesh, px, pv)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().oracle_tet_scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

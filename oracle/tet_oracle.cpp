@@ -1,0 +1,584 @@
+// CPU oracle: restatement of the reference's CPU tet VBD path.
+// TEST INFRASTRUCTURE ONLY (see oracle.h). Never linked by the product.
+//
+// Reference files restated here (paths relative to Simulator/Modules/):
+//   TetMesh/TetMeshFEM.cpp:332-443        rest-pose precompute (DmInv, volume, lumped mass)
+//   TetMesh/TetMeshFEM.cpp:275-325        vertex -> tet incidence (order here: ascending tet id;
+//                                         MeshFrame's V_TV order is unavailable: parity unpinned)
+//   TetMesh/TetMeshFEM.cpp:122-133        colour categories sorted by size, descending
+//   VBD/VBDPhysics.cpp:99-122,235-274     parallel-group merge of per-mesh colours
+//   VBD/VBDPhysics.cpp:1580-1632          runStep_serialCollisionHandling (no DCD/CCD)
+//   VBD/VBD_BaseMaterial.h:201-259        applyInitialStep / computeInertia
+//   VBD/VBD_BaseMaterial.cpp:7-50         forwardStep (initialisation types 0-4)
+//   VBD/VBDPhysics.cpp:2402-2483          VBDStepWithCollision
+//   VBD/VBD_NeoHookean.cpp:126-152,379-581 neo-Hookean force/Hessian with full 9x9 assembly
+//   VBD/VBD_BaseMaterial.h:359-366        inertia force/Hessian
+//   VBD/VBDPhysics.cpp:2821-2954,2563-2592 plane-ground boundary penalty + IPC friction
+//   VBD/VBDPhysics.cpp:2486-2561,794-807  velocity update + damping (step-invariant scaling)
+#include "oracle.h"
+#include "pool.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+template <class R> struct V3 {
+    R x, y, z;
+};
+
+// ---------------------------------------------------------------------------
+// 3x3 helpers, column-major like the reference's Eigen Mat3 (Types.h:76-127).
+// m[r + 3*c]
+// ---------------------------------------------------------------------------
+template <class R> inline R det3(const R* m) {
+    // cofactor expansion along the first column
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
+           m[2] * (m[3] * m[7] - m[6] * m[4]);
+}
+
+template <class R> inline bool inv3(const R* m, R* out) {
+    R c00 = m[4] * m[8] - m[7] * m[5];
+    R c01 = -(m[1] * m[8] - m[7] * m[2]);
+    R c02 = m[1] * m[5] - m[4] * m[2];
+    R det = m[0] * c00 + m[3] * c01 + m[6] * c02;
+    if (det == R(0)) return false;
+    R inv = R(1) / det;
+    // out = adj(m)/det, adj = transpose of cofactor matrix
+    out[0] = c00 * inv;
+    out[1] = c01 * inv;
+    out[2] = c02 * inv;
+    out[3] = -(m[3] * m[8] - m[6] * m[5]) * inv;
+    out[4] = (m[0] * m[8] - m[6] * m[2]) * inv;
+    out[5] = -(m[0] * m[5] - m[3] * m[2]) * inv;
+    out[6] = (m[3] * m[7] - m[6] * m[4]) * inv;
+    out[7] = -(m[0] * m[7] - m[6] * m[1]) * inv;
+    out[8] = (m[0] * m[4] - m[3] * m[1]) * inv;
+    return true;
+}
+
+// Own closed-form 3x3 solve used in place of CuMatrix::solve3x3_psd_stable
+// (VBDPhysics.cpp:2449). CuMatrix is an empty submodule, so its rounding and
+// failure threshold are unknown: parity unpinned. Definition used here and in
+// the HIP kernels: adjugate / determinant; failure iff det is not finite or
+// |det| <= 1e-12 * |h00 h11 h22|.
+template <class R> inline bool solve3(const R* h, const R* b, R* x) {
+    // a_ij = h[i + 3j]; C_ij = cofactor of a_ij
+    const R a00 = h[0], a10 = h[1], a20 = h[2];
+    const R a01 = h[3], a11 = h[4], a21 = h[5];
+    const R a02 = h[6], a12 = h[7], a22 = h[8];
+    const R C00 = a11 * a22 - a12 * a21;
+    const R C01 = a12 * a20 - a10 * a22;
+    const R C02 = a10 * a21 - a11 * a20;
+    const R C10 = a02 * a21 - a01 * a22;
+    const R C11 = a00 * a22 - a02 * a20;
+    const R C12 = a01 * a20 - a00 * a21;
+    const R C20 = a01 * a12 - a02 * a11;
+    const R C21 = a02 * a10 - a00 * a12;
+    const R C22 = a00 * a11 - a01 * a10;
+    const R det = a00 * C00 + a01 * C01 + a02 * C02;
+    const R scale = std::fabs(a00 * a11 * a22);
+    if (!std::isfinite(det) || !(std::fabs(det) > R(1e-12) * scale)) return false;
+    // (A^-1)_ij = C_ji / det
+    const R inv = R(1) / det;
+    x[0] = (C00 * b[0] + C10 * b[1] + C20 * b[2]) * inv;
+    x[1] = (C01 * b[0] + C11 * b[1] + C21 * b[2]) * inv;
+    x[2] = (C02 * b[0] + C12 * b[1] + C22 * b[2]) * inv;
+    return std::isfinite(x[0]) && std::isfinite(x[1]) && std::isfinite(x[2]);
+}
+
+// TetMeshFEM.cpp:409-429: Dm = [x1-x0, x2-x0, x3-x0]; DmInv = Dm^-1; vol = det(Dm)/6;
+// every tet vertex gets 0.25*vol*density of lumped mass.
+template <class R>
+int tet_prepare(int nv, const R* pos, int nt, const int32_t* tets, R density, R* dminv, R* vol, R* mass) {
+    for (int i = 0; i < nv; ++i) mass[i] = R(0);
+    for (int t = 0; t < nt; ++t) {
+        const int32_t* tv = tets + 4 * t;
+        R Dm[9];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) Dm[r + 3 * c] = pos[3 * tv[c + 1] + r] - pos[3 * tv[0] + r];
+        if (!inv3(Dm, dminv + 9 * t)) return -1;
+        R v = det3(Dm) / R(6);
+        vol[t] = v;
+        // `vertexMass += 0.25 * vol * density` evaluates in double, then rounds
+        for (int k = 0; k < 4; ++k) mass[tv[k]] = (R)((double)mass[tv[k]] + 0.25 * (double)v * (double)density);
+    }
+    return 0;
+}
+
+// VBD_NeoHookean.cpp:126-152 assembleVertexVForceAndHessian.
+template <class R>
+inline void assemble_vertex(const R* dEdF, const R* d2E, R m1, R m2, R m3, R* f, R* h) {
+    f[0] = dEdF[0] * m1 + dEdF[3] * m2 + dEdF[6] * m3;
+    f[1] = dEdF[1] * m1 + dEdF[4] * m2 + dEdF[7] * m3;
+    f[2] = dEdF[2] * m1 + dEdF[5] * m2 + dEdF[8] * m3;
+    R HL[3][9];
+    for (int c = 0; c < 9; ++c) {
+        for (int r = 0; r < 3; ++r)
+            HL[r][c] = d2E[r + 9 * c] * m1 + d2E[(r + 3) + 9 * c] * m2 + d2E[(r + 6) + 9 * c] * m3;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) h[r + 3 * c] = HL[r][c] * m1 + HL[r][c + 3] * m2 + HL[r][c + 6] * m3;
+}
+
+// VBD_NeoHookean.cpp:379-581 accumlateMaterialForceAndHessian2, per incident tet:
+// the full 9x9 d2E/dF2 of the stable neo-Hookean energy, contracted with the
+// vertex's row m of the shape matrix, plus damping. Adds into force/hess.
+template <class R>
+void neohookean_vertex(const R* x4, const R* DmInv, R A, R miu, R lmbd, int order, R dampH, R dampD, R dt,
+                       const R* disp, R* force, R* hess) {
+    const R a = R(1) + miu / lmbd;
+    R Ds[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Ds[r + 3 * c] = x4[3 * (c + 1) + r] - x4[r];
+    R F[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            F[r + 3 * c] = Ds[r + 0] * DmInv[0 + 3 * c] + Ds[r + 3] * DmInv[1 + 3 * c] + Ds[r + 6] * DmInv[2 + 3 * c];
+    const R detF = det3(F);
+    const R F1_1 = F[0], F2_1 = F[1], F3_1 = F[2];
+    const R F1_2 = F[3], F2_2 = F[4], F3_2 = F[5];
+    const R F1_3 = F[6], F2_3 = F[7], F3_3 = F[8];
+    R g[9] = {F2_2 * F3_3 - F2_3 * F3_2, F1_3 * F3_2 - F1_2 * F3_3, F1_2 * F2_3 - F1_3 * F2_2,
+              F2_3 * F3_1 - F2_1 * F3_3, F1_1 * F3_3 - F1_3 * F3_1, F1_3 * F2_1 - F1_1 * F2_3,
+              F2_1 * F3_2 - F2_2 * F3_1, F1_2 * F3_1 - F1_1 * F3_2, F1_1 * F2_2 - F1_2 * F2_1};
+    R d2E[81];
+    for (int c = 0; c < 9; ++c)
+        for (int r = 0; r < 9; ++r) d2E[r + 9 * c] = g[r] * g[c];
+    const R k = detF - a;
+    auto sym = [&](int i, int j, R v) {
+        d2E[i + 9 * j] += v;
+        d2E[j + 9 * i] += v;
+    };
+    sym(0, 4, k * F3_3);  sym(0, 5, k * -F2_3); sym(0, 7, k * -F3_2); sym(0, 8, k * F2_2);
+    sym(1, 3, k * -F3_3); sym(1, 5, k * F1_3);  sym(1, 6, k * F3_2);  sym(1, 8, k * -F1_2);
+    sym(2, 3, k * F2_3);  sym(2, 4, k * -F1_3); sym(2, 6, k * -F2_2); sym(2, 7, k * F1_2);
+    sym(3, 7, k * F3_1);  sym(3, 8, k * -F2_1);
+    sym(4, 6, k * -F3_1); sym(4, 8, k * F1_1);
+    sym(5, 6, k * F2_1);  sym(5, 7, k * -F1_1);
+    for (int i = 0; i < 81; ++i) d2E[i] *= lmbd;
+    for (int i = 0; i < 81; ++i) d2E[i] *= A;
+    R dEdF[9];
+    const R s = lmbd * (detF - a);
+    for (int i = 0; i < 9; ++i) dEdF[i] = A * (miu * F[i] + s * g[i]);
+
+    R m1, m2, m3;
+    switch (order) {
+    case 0:
+        m1 = -DmInv[0] - DmInv[1] - DmInv[2];
+        m2 = -DmInv[3] - DmInv[4] - DmInv[5];
+        m3 = -DmInv[6] - DmInv[7] - DmInv[8];
+        break;
+    case 1: m1 = DmInv[0]; m2 = DmInv[3]; m3 = DmInv[6]; break;
+    case 2: m1 = DmInv[1]; m2 = DmInv[4]; m3 = DmInv[7]; break;
+    default: m1 = DmInv[2]; m2 = DmInv[5]; m3 = DmInv[8]; break;
+    }
+    R dEdx[3], h[9];
+    assemble_vertex(dEdF, d2E, m1, m2, m3, dEdx, h);
+    R dampingH[9];
+    for (int i = 0; i < 9; ++i) dampingH[i] = h[i] * dampH;
+    R tmp = (m1 * m1 + m2 * m2 + m3 * m3) * miu * A;
+    h[0] += tmp; h[4] += tmp; h[8] += tmp;
+    tmp *= dampD;
+    dampingH[0] += tmp; dampingH[4] += tmp; dampingH[8] += tmp;
+    for (int i = 0; i < 9; ++i) dampingH[i] /= dt;
+    R df[3];
+    for (int r = 0; r < 3; ++r) df[r] = dampingH[r] * disp[0] + dampingH[r + 3] * disp[1] + dampingH[r + 6] * disp[2];
+    for (int r = 0; r < 3; ++r) force[r] -= dEdx[r] + df[r];
+    for (int i = 0; i < 9; ++i) hess[i] += h[i] + dampingH[i];
+}
+
+// VBDPhysics.cpp:2563-2592 computeVertexFriction (IPC mollifier; the u u^T
+// term is dropped in the reference's Hessian, and here).
+// T is 3x2 column-major, u = T^T dx.
+template <class R>
+inline void vertex_friction(R mu, R lambda, const R* T, const R* u, R epsU, R* f, R* H) {
+    R uNorm = std::sqrt(u[0] * u[0] + u[1] * u[1]);
+    if (uNorm > 0) {
+        R f1_SF_over_x;
+        if (uNorm > epsU) {
+            f1_SF_over_x = R(1) / uNorm;
+        } else {
+            f1_SF_over_x = (-uNorm / epsU + R(2)) / epsU;
+        }
+        for (int r = 0; r < 3; ++r) f[r] = -mu * lambda * (T[r] * u[0] + T[r + 3] * u[1]) * f1_SF_over_x;
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r)
+                H[r + 3 * c] = mu * lambda * f1_SF_over_x * (T[r] * T[c] + T[r + 3] * T[c + 3]);
+    } else {
+        for (int i = 0; i < 3; ++i) f[i] = 0;
+        for (int i = 0; i < 9; ++i) H[i] = 0;
+    }
+}
+
+template <class R> struct Mesh {
+    int nv = 0, nt = 0;
+    std::vector<R> x, xprev, v, vprev, y, dminv, vol, mass;
+    std::vector<int32_t> tets;
+    std::vector<uint8_t> fixed;
+    std::vector<int32_t> fixedList;
+    std::vector<int32_t> incBegin, incTet, incOrder;
+    std::vector<std::vector<int32_t>> categories;
+    R mu, lmbd, dampH, dampD, expVelDamp, constVelDamp, density;
+    int initType = 0;
+};
+
+template <class R> struct Scene {
+    double p[OP_COUNT];
+    R dt, dtSqrRecip;
+    std::vector<Mesh<R>> meshes;
+    std::vector<std::vector<int32_t>> groups;  // interleaved (mesh, vertex)
+    oracle::ThreadPool* pool = nullptr;
+};
+
+// TetMeshFEM.cpp:275-325 (incidence; ascending tet id within each vertex).
+template <class R> void build_incidence(Mesh<R>& m) {
+    std::vector<int32_t> cnt(m.nv + 1, 0);
+    for (int t = 0; t < m.nt; ++t)
+        for (int k = 0; k < 4; ++k) cnt[m.tets[4 * t + k] + 1]++;
+    m.incBegin.assign(m.nv + 1, 0);
+    for (int i = 0; i < m.nv; ++i) m.incBegin[i + 1] = m.incBegin[i] + cnt[i + 1];
+    m.incTet.resize(m.incBegin[m.nv]);
+    m.incOrder.resize(m.incBegin[m.nv]);
+    std::vector<int32_t> fill(m.incBegin.begin(), m.incBegin.end() - 1);
+    for (int t = 0; t < m.nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+            int v = m.tets[4 * t + k];
+            m.incTet[fill[v]] = t;
+            // first k with tetVIds(k) == v (TetMeshFEM.cpp:297-305)
+            int ord = k;
+            for (int kk = 0; kk < 4; ++kk)
+                if (m.tets[4 * t + kk] == v) { ord = kk; break; }
+            m.incOrder[fill[v]] = ord;
+            fill[v]++;
+        }
+}
+
+// VBD_BaseMaterial.h:244-259, 201-221; VBD_BaseMaterial.cpp:7-50.
+template <class R> void apply_initial_step(Scene<R>& s, Mesh<R>& m) {
+    const R dt = s.dt;
+    const R g[3] = {(R)s.p[OP_GRAVITY_X], (R)s.p[OP_GRAVITY_Y], (R)s.p[OP_GRAVITY_Z]};
+    const int initType = m.initType;
+    m.xprev = m.x;
+    m.vprev = m.v;
+    // external force is zero (evaluateExternalForce sets it to 0)
+    if (s.p[OP_ASSOCIATE_GRAVITY] != 0) {
+        R dg[3] = {dt * g[0], dt * g[1], dt * g[2]};
+        for (int i = 0; i < m.nv; ++i)
+            for (int d = 0; d < 3; ++d) m.v[3 * i + d] += dg[d];
+    }
+    for (int f : m.fixedList)
+        for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
+    for (int i = 0; i < 3 * m.nv; ++i) m.y[i] = m.x[i] + dt * m.v[i];
+    switch (initType) {
+    case 0: m.x = m.y; break;
+    case 1: break;
+    case 2:
+        for (int i = 0; i < 3 * m.nv; ++i) m.x[i] = m.xprev[i] + dt * m.vprev[i];
+        break;
+    case 3:
+        for (int i = 0; i < 3 * m.nv; ++i) m.x[i] = R(0.5) * m.x[i] + R(0.5) * m.y[i];
+        break;
+    case 4: {
+        const R r = (R)s.p[OP_INIT_RATIO_G];
+        for (int i = 0; i < m.nv; ++i)
+            for (int d = 0; d < 3; ++d) m.x[3 * i + d] = m.x[3 * i + d] + dt * m.vprev[3 * i + d] + dt * dt * r * g[d];
+        break;
+    }
+    default: break;
+    }
+}
+
+// VBDPhysics.cpp:2402-2483 VBDStepWithCollision (no contacts, no line search).
+template <class R> void vbd_vertex(Scene<R>& s, Mesh<R>& m, int vid, bool applyFriction) {
+    R h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    R f[3] = {0, 0, 0};
+    R* xv = &m.x[3 * vid];
+    const R disp[3] = {xv[0] - m.xprev[3 * vid], xv[1] - m.xprev[3 * vid + 1], xv[2] - m.xprev[3 * vid + 2]};
+    for (int k = m.incBegin[vid]; k < m.incBegin[vid + 1]; ++k) {
+        const int t = m.incTet[k];
+        R x4[12];
+        for (int j = 0; j < 4; ++j)
+            for (int d = 0; d < 3; ++d) x4[3 * j + d] = m.x[3 * m.tets[4 * t + j] + d];
+        neohookean_vertex<R>(x4, &m.dminv[9 * t], m.vol[t], m.mu, m.lmbd, m.incOrder[k], m.dampH, m.dampD, s.dt,
+                             disp, f, h);
+    }
+    // inertia (VBD_BaseMaterial.h:359-366)
+    const R mass = m.mass[vid];
+    for (int d = 0; d < 3; ++d) f[d] += mass * (m.y[3 * vid + d] - xv[d]) * s.dtSqrRecip;
+    R hi = mass * s.dtSqrRecip;
+    h[0] += hi; h[4] += hi; h[8] += hi;
+    // plane-ground boundary (VBDPhysics.cpp:2877-2952)
+    if (s.p[OP_USE_PLANE_GROUND] != 0) {
+        const R k = (R)s.p[OP_BOUNDARY_STIFFNESS];
+        for (int dim = 0; dim < 3; ++dim) {
+            const R lower = (R)s.p[OP_LOWER_X + dim];
+            const R upper = (R)s.p[OP_UPPER_X + dim];
+            R pen;
+            if (xv[dim] < lower) {
+                pen = lower - xv[dim];
+                f[dim] += pen * k;
+            } else if (xv[dim] > upper) {
+                pen = xv[dim] - upper;
+                f[dim] -= pen * k;
+            } else {
+                continue;
+            }
+            h[dim + 3 * dim] += k;
+            if (applyFriction) {
+                R dx[3] = {xv[0] - m.xprev[3 * vid], xv[1] - m.xprev[3 * vid + 1], xv[2] - m.xprev[3 * vid + 2]};
+                R T[6] = {0, 0, 0, 0, 0, 0};
+                T[(dim + 1) % 3] = 1;
+                T[3 + (dim + 2) % 3] = 1;
+                R u[2] = {T[0] * dx[0] + T[1] * dx[1] + T[2] * dx[2], T[3] * dx[0] + T[4] * dx[1] + T[5] * dx[2]};
+                R lambda = pen * k;
+                R mu = (R)s.p[OP_BOUNDARY_FRICTION_MU];
+                R epsU = (R)s.p[OP_BOUNDARY_FRICTION_EPS_V] * s.dt;
+                R ff[3], fh[9];
+                vertex_friction<R>(mu, lambda, T, u, epsU, ff, fh);
+                for (int d = 0; d < 3; ++d) f[d] += ff[d];
+                for (int i = 0; i < 9; ++i) h[i] += fh[i];
+            }
+        }
+    }
+    if (f[0] * f[0] + f[1] * f[1] + f[2] * f[2] > R(1e-10)) {
+        R dx[3];
+        R step = (R)s.p[OP_STEP_SIZE];
+        if (!solve3<R>(h, f, dx)) {
+            step = (R)s.p[OP_STEP_SIZE_GD];
+            dx[0] = f[0]; dx[1] = f[1]; dx[2] = f[2];
+        }
+        for (int d = 0; d < 3; ++d) xv[d] += step * dx[d];
+    }
+}
+
+// VBDPhysics.cpp:2486-2561 updateVelocities + dampVelocities (no-gravity-zone branch omitted).
+template <class R> void update_velocities(Scene<R>& s, Mesh<R>& m) {
+    for (int i = 0; i < m.nv; ++i) {
+        R* v = &m.v[3 * i];
+        for (int d = 0; d < 3; ++d) v[d] = (m.x[3 * i + d] - m.xprev[3 * i + d]) / s.dt;
+        R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (vMag > R(1e-6)) {
+            R vNew = vMag * m.expVelDamp - m.constVelDamp;
+            vNew = vNew > R(1e-6) ? vNew : R(0);
+            for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
+        } else {
+            for (int d = 0; d < 3; ++d) v[d] *= R(0);
+        }
+    }
+    for (int f : m.fixedList)
+        for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
+}
+
+template <class R> void scene_step(Scene<R>& s, int nthreads) {
+    const int nsub = (int)s.p[OP_NUM_SUBSTEPS];
+    const int iters = (int)s.p[OP_ITERATIONS];
+    const int fsi = (int)s.p[OP_FRICTION_START_ITER];
+    for (int sub = 0; sub < nsub; ++sub) {
+        for (auto& m : s.meshes) apply_initial_step(s, m);
+        for (int it = 0; it < iters; ++it) {
+            const bool fr = it >= fsi;
+            for (auto& grp : s.groups) {
+                const int n = (int)grp.size() / 2;
+                auto body = [&](int i) {
+                    Mesh<R>& m = s.meshes[grp[2 * i]];
+                    int vid = grp[2 * i + 1];
+                    if (!m.fixed[vid]) vbd_vertex(s, m, vid, fr);
+                };
+                if (s.pool && nthreads > 1)
+                    s.pool->parallel_for(0, n, body);
+                else
+                    for (int i = 0; i < n; ++i) body(i);
+            }
+        }
+        for (auto& m : s.meshes) update_velocities(s, m);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scene wrapper: one struct holds either precision.
+// ---------------------------------------------------------------------------
+struct OracleTetScene {
+    int precision;
+    Scene<float> s32;
+    Scene<double> s64;
+    std::unique_ptr<oracle::ThreadPool> pool;
+};
+
+template <class R> static int add_mesh(Scene<R>& s, int nv, const double* pos, const double* vel, int nt,
+                                       const int32_t* tets, const double* mat, int nfixed, const int32_t* fixed,
+                                       const int32_t* colors) {
+    Mesh<R> m;
+    m.nv = nv;
+    m.nt = nt;
+    m.x.resize(3 * nv);
+    m.v.assign(3 * nv, R(0));
+    for (int i = 0; i < 3 * nv; ++i) m.x[i] = (R)pos[i];
+    if (vel)
+        for (int i = 0; i < 3 * nv; ++i) m.v[i] = (R)vel[i];
+    m.xprev = m.x;
+    m.vprev = m.v;
+    m.y = m.x;
+    m.tets.assign(tets, tets + 4 * nt);
+    m.density = (R)mat[OM_DENSITY];
+    m.mu = (R)mat[OM_MU];
+    m.lmbd = (R)mat[OM_LAMBDA];
+    m.dampH = (R)mat[OM_DAMP_HYDRO];
+    m.dampD = (R)mat[OM_DAMP_DEV];
+    m.initType = (int)mat[OM_INIT_TYPE];
+    // VBDPhysics.cpp:794-807 timeStepEqualize
+    m.expVelDamp = (R)std::pow((R)mat[OM_EXP_VEL_DAMPING], s.dt);
+    m.constVelDamp = (R)mat[OM_CONST_VEL_DAMPING] * s.dt;
+    m.dminv.resize(9 * nt);
+    m.vol.resize(nt);
+    m.mass.resize(nv);
+    if (tet_prepare<R>(nv, m.x.data(), nt, tets, m.density, m.dminv.data(), m.vol.data(), m.mass.data())) return -1;
+    m.fixed.assign(nv, 0);
+    for (int i = 0; i < nfixed; ++i) {
+        m.fixed[fixed[i]] = 1;
+        m.fixedList.push_back(fixed[i]);
+    }
+    build_incidence(m);
+    // categories: vertices per colour, ascending id (ColoringAlgorithms.cpp:69-80), then
+    // sorted by size descending (TetMeshFEM.cpp:122-133; stable here, std::sort there).
+    int nc = 0;
+    for (int i = 0; i < nv; ++i) nc = std::max(nc, colors[i] + 1);
+    m.categories.assign(nc, {});
+    for (int i = 0; i < nv; ++i) m.categories[colors[i]].push_back(i);
+    std::stable_sort(m.categories.begin(), m.categories.end(),
+                     [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) { return a.size() > b.size(); });
+    s.meshes.push_back(std::move(m));
+    return (int)s.meshes.size() - 1;
+}
+
+// VBDPhysics.cpp:99-122 findSmallestParallelGroup and :235-274 group merge.
+template <class R> static void build_groups(Scene<R>& s) {
+    size_t ng = 0;
+    for (auto& m : s.meshes) ng = std::max(ng, m.categories.size());
+    s.groups.assign(ng, {});
+    for (size_t im = 0; im < s.meshes.size(); ++im) {
+        auto& m = s.meshes[im];
+        std::vector<int> avail(m.categories.size());
+        for (size_t c = 0; c < avail.size(); ++c) avail[c] = (int)c;
+        for (size_t c = 0; c < m.categories.size(); ++c) {
+            size_t best = s.groups[avail[0]].size();
+            int bestId = avail[0];
+            size_t bestI = 0;
+            for (size_t i = 1; i < avail.size(); ++i)
+                if (s.groups[avail[i]].size() < best) {
+                    best = s.groups[avail[i]].size();
+                    bestId = avail[i];
+                    bestI = i;
+                }
+            avail.erase(avail.begin() + bestI);
+            for (int v : m.categories[c]) {
+                s.groups[bestId].push_back((int32_t)im);
+                s.groups[bestId].push_back(v);
+            }
+        }
+    }
+}
+
+template <class R> static int get_state(Scene<R>& s, int mesh, double* pos, double* vel) {
+    if (mesh < 0 || mesh >= (int)s.meshes.size()) return -1;
+    auto& m = s.meshes[mesh];
+    for (int i = 0; i < 3 * m.nv; ++i) {
+        if (pos) pos[i] = (double)m.x[i];
+        if (vel) vel[i] = (double)m.v[i];
+    }
+    return 0;
+}
+template <class R> static int set_state(Scene<R>& s, int mesh, const double* pos, const double* vel) {
+    if (mesh < 0 || mesh >= (int)s.meshes.size()) return -1;
+    auto& m = s.meshes[mesh];
+    for (int i = 0; i < 3 * m.nv; ++i) {
+        if (pos) m.x[i] = (R)pos[i];
+        if (vel) m.v[i] = (R)vel[i];
+    }
+    return 0;
+}
+
+extern "C" {
+
+int oracle_tet_prepare_f64(int nv, const double* pos, int nt, const int32_t* tets, double density, double* dminv,
+                           double* vol, double* mass) {
+    return tet_prepare<double>(nv, pos, nt, tets, density, dminv, vol, mass);
+}
+int oracle_tet_prepare_f32(int nv, const float* pos, int nt, const int32_t* tets, float density, float* dminv,
+                           float* vol, float* mass) {
+    return tet_prepare<float>(nv, pos, nt, tets, density, dminv, vol, mass);
+}
+
+void oracle_neohookean_vertex_f64(const double* x4, const double* dminv, double vol, double mu, double lmbd,
+                                  int order, double* force3, double* hess9) {
+    const double disp[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) force3[i] = 0;
+    for (int i = 0; i < 9; ++i) hess9[i] = 0;
+    neohookean_vertex<double>(x4, dminv, vol, mu, lmbd, order, 0.0, 0.0, 1.0, disp, force3, hess9);
+}
+
+int oracle_solve3x3_f64(const double* h9, const double* b3, double* x3) { return solve3<double>(h9, b3, x3) ? 1 : 0; }
+int oracle_solve3x3_f32(const float* h9, const float* b3, float* x3) { return solve3<float>(h9, b3, x3) ? 1 : 0; }
+
+OracleTetScene* oracle_tet_scene_create(int precision, const double* physics) {
+    auto* sc = new OracleTetScene();
+    sc->precision = precision == 64 ? 64 : 32;
+    std::memcpy(sc->s32.p, physics, sizeof(double) * OP_COUNT);
+    std::memcpy(sc->s64.p, physics, sizeof(double) * OP_COUNT);
+    // BasePhysicsFramework.cpp:245-249: dt in FloatingType, dtSqrReciprocal = 1/(dt*dt)
+    sc->s32.dt = (float)physics[OP_DT];
+    sc->s32.dtSqrRecip = 1.f / (sc->s32.dt * sc->s32.dt);
+    sc->s64.dt = physics[OP_DT];
+    sc->s64.dtSqrRecip = 1.0 / (sc->s64.dt * sc->s64.dt);
+    return sc;
+}
+
+void oracle_tet_scene_destroy(OracleTetScene* sc) { delete sc; }
+
+int oracle_tet_scene_add_mesh(OracleTetScene* sc, int nv, const double* pos, const double* vel, int nt,
+                              const int32_t* tets, const double* material, int nfixed, const int32_t* fixed,
+                              const int32_t* colors) {
+    if (sc->precision == 64) return add_mesh<double>(sc->s64, nv, pos, vel, nt, tets, material, nfixed, fixed, colors);
+    return add_mesh<float>(sc->s32, nv, pos, vel, nt, tets, material, nfixed, fixed, colors);
+}
+
+int oracle_tet_scene_finalize(OracleTetScene* sc) {
+    if (sc->precision == 64) build_groups(sc->s64);
+    else build_groups(sc->s32);
+    return 0;
+}
+
+int oracle_tet_scene_num_groups(OracleTetScene* sc) {
+    return sc->precision == 64 ? (int)sc->s64.groups.size() : (int)sc->s32.groups.size();
+}
+
+int oracle_tet_scene_group(OracleTetScene* sc, int g, int32_t* out) {
+    auto& grp = sc->precision == 64 ? sc->s64.groups : sc->s32.groups;
+    if (g < 0 || g >= (int)grp.size()) return -1;
+    if (out) std::memcpy(out, grp[g].data(), grp[g].size() * sizeof(int32_t));
+    return (int)grp[g].size() / 2;
+}
+
+int oracle_tet_scene_step(OracleTetScene* sc, int nsteps, int nthreads) {
+    if (nthreads > 1 && (!sc->pool || sc->pool->size() != nthreads)) sc->pool.reset(new oracle::ThreadPool(nthreads));
+    sc->s32.pool = sc->s64.pool = nthreads > 1 ? sc->pool.get() : nullptr;
+    for (int i = 0; i < nsteps; ++i) {
+        if (sc->precision == 64) scene_step(sc->s64, nthreads);
+        else scene_step(sc->s32, nthreads);
+    }
+    return 0;
+}
+
+int oracle_tet_scene_get_state(OracleTetScene* sc, int mesh, double* pos, double* vel) {
+    return sc->precision == 64 ? get_state(sc->s64, mesh, pos, vel) : get_state(sc->s32, mesh, pos, vel);
+}
+int oracle_tet_scene_set_state(OracleTetScene* sc, int mesh, const double* pos, const double* vel) {
+    return sc->precision == 64 ? set_state(sc->s64, mesh, pos, vel) : set_state(sc->s32, mesh, pos, vel);
+}
+
+}  // extern "C"

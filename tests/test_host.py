@@ -1,0 +1,90 @@
+"""CPU-side checks of the product library: the C ABI loads and exports every
+declared symbol, and its host-side preparation (colouring, rest-pose constants)
+matches the reference fixtures and the oracle. No GPU calls."""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gaia_amd
+from gaia_amd import scenes
+from gaia_amd.vbd import tet_mesh_constants
+from conftest import GOLDEN, load_t
+
+FIXTURES = ["Beam_lowRes", "Beam", "Cube", "bunny_small"]
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gaia_amd.lib()
+    declared = gaia_amd.header_functions()
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(lib, name), name
+    # and the dynamic symbol table has them as text symbols
+    out = subprocess.run(["nm", "-D", "--defined-only", gaia_amd.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(declared) <= exported
+    # ctypes binding covers every declared function
+    assert sorted(declared) == sorted(gaia_amd._lib._SIGNATURES)
+
+
+def test_abi_struct_layout_matches_header():
+    from gaia_amd._lib import PhysicsParams, TetMeshDesc
+
+    a, b = ctypes.c_int32(), ctypes.c_int32()
+    assert gaia_amd.lib().gvbd_abi_struct_sizes(ctypes.byref(a), ctypes.byref(b)) == 0
+    assert ctypes.sizeof(PhysicsParams) == a.value == 80
+    assert ctypes.sizeof(TetMeshDesc) == b.value
+    assert TetMeshDesc.positions.offset == 8 and TetMeshDesc.tets.offset == 40
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_product_mcs_matches_reference_fixture(name):
+    pos, tets = load_t(os.path.join(GOLDEN, "coloring", name + ".t"))
+    want = json.load(open(os.path.join(GOLDEN, "coloring", name + ".t.vertexColoring.json")))
+    c = gaia_amd.color_tet_mesh(len(pos), tets, "mcs")
+    got = [sorted(np.flatnonzero(c == k).tolist()) for k in range(c.max() + 1)]
+    assert got == [sorted(x) for x in want]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_product_greedy_matches_oracle(oracle, name):
+    pos, tets = load_t(os.path.join(GOLDEN, "coloring", name + ".t"))
+    assert np.array_equal(gaia_amd.color_tet_mesh(len(pos), tets, "greedy"),
+                          oracle.color_tet_mesh(len(pos), tets, "greedy"))
+
+
+def test_soft_cube_colouring_sizes(oracle):
+    """10^3-cell cube with the alternating 5-tet split: greedy colours of sizes
+    665, 216, 126, 138, 110, 49, 19, 8 and MCS 341, 330, 330, 330."""
+    pos, tets = scenes.cube_tet_mesh(10, 0.1)
+    g = gaia_amd.color_tet_mesh(len(pos), tets, "greedy")
+    m = gaia_amd.color_tet_mesh(len(pos), tets, "mcs")
+    assert np.bincount(g).tolist() == [665, 216, 126, 138, 110, 49, 19, 8]
+    assert np.bincount(m).tolist() == [341, 330, 330, 330]
+    assert np.array_equal(g, oracle.color_tet_mesh(len(pos), tets, "greedy"))
+    assert np.array_equal(m, oracle.color_tet_mesh(len(pos), tets, "mcs"))
+    inc = np.bincount(tets.ravel())
+    assert inc.max() == 32 and abs(inc.mean() - 15.03) < 0.01
+
+
+def test_rest_pose_constants_match_oracle(oracle):
+    pos, tets = scenes.cube_tet_mesh(4, 0.1)
+    rng = np.random.default_rng(0)
+    pos = pos + rng.normal(scale=0.005, size=pos.shape)
+    dminv, vol, mass = tet_mesh_constants(pos, tets, 1000.0)
+    odm, ovol, omass = oracle.tet_prepare(pos.astype(np.float32), tets, 1000.0, precision=32)
+    assert np.allclose(dminv, odm, rtol=1e-6, atol=1e-6 * np.abs(odm).max())
+    assert np.allclose(vol, ovol, rtol=1e-6)
+    assert np.allclose(mass, omass, rtol=1e-6)
+    assert np.all(vol > 0)
+
+
+def test_errors_are_reported_not_raised_in_c():
+    with pytest.raises(gaia_amd.VBDError, match="out of range"):
+        gaia_amd.color_tet_mesh(3, np.array([[0, 1, 2, 5]]))
+    with pytest.raises(gaia_amd.VBDError, match="degenerate"):
+        tet_mesh_constants(np.zeros((4, 3)), np.array([[0, 1, 2, 3]]), 1.0)

@@ -1,0 +1,64 @@
+"""Shared scene specs for the parity tests: the same spec builds a HIP scene
+(through the C ABI) and an oracle scene."""
+import numpy as np
+
+from gaia_amd import VBDPhysics, scenes
+from gaia_amd.vbd import physics_dict, physics_params
+
+
+def rel_err(x, ref):
+    """Normwise relative position error: max |x - ref| / max |ref|."""
+    return float(np.abs(np.asarray(x, np.float64) - ref).max() / max(np.abs(ref).max(), 1e-30))
+
+
+def c1_spec():
+    c = scenes.c1_soft_cube()
+    mesh = dict(pos=c["positions"], tets=c["tets"], material=c["material"], fixed=(), vel=None,
+                coloring=c["coloring"])
+    return dict(physics=physics_dict(**c["physics"]), meshes=[mesh], steps=c["steps"])
+
+
+def multi_spec():
+    """Two cubes of different materials, one with fixed points, damping and a
+    non-default initialisation type; ground at y = 0 with friction."""
+    p1, t1 = scenes.cube_tet_mesh(4, 0.1, origin=(-0.6, 0.05, -0.2))
+    p2, t2 = scenes.cube_tet_mesh(3, 0.12, origin=(0.3, 0.3, -0.2))
+    rng = np.random.default_rng(4)
+    v1 = rng.normal(scale=0.2, size=p1.shape)
+    m1 = dict(density=800.0, mu=5e4, lmbd=4e5, damping_hydrostatic=0.01, damping_deviatoric=0.02,
+              exponential_vel_damping=0.9, constant_vel_damping=0.01)
+    m2 = dict(density=1200.0, mu=2e5, lmbd=1e6, initialization_type=2)
+    physics = physics_dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=2, iterations=6,
+                           world_lower=(-1e7, 0.0, -1e7), world_upper=(1e7, 1e7, 1e7),
+                           boundary_friction_dynamic=0.3, friction_start_iter=1)
+    top = [i for i in range(len(p2)) if p2[i, 1] > p2[:, 1].max() - 1e-9][:3]
+    meshes = [dict(pos=p1, tets=t1, material=m1, fixed=(), vel=v1, coloring="mcs"),
+              dict(pos=p2, tets=t2, material=m2, fixed=top, vel=None, coloring="greedy")]
+    return dict(physics=physics, meshes=meshes, steps=20)
+
+
+def build_gpu(ctx, spec):
+    import gaia_amd
+
+    sim = VBDPhysics(ctx)
+    sim.setPhysicsParams(**spec["physics"])
+    for m in spec["meshes"]:
+        colors = gaia_amd.color_tet_mesh(len(m["pos"]), m["tets"], m["coloring"])
+        sim.addTetMesh(m["pos"], m["tets"], fixed_points=m["fixed"], vertex_colors=colors, velocities=m["vel"],
+                       **m["material"])
+    sim.initialize()
+    return sim
+
+
+def build_oracle(oracle, spec, precision=32):
+    import gaia_amd
+
+    sc = oracle.TetScene(spec["physics"], precision=precision)
+    for m in spec["meshes"]:
+        colors = gaia_amd.color_tet_mesh(len(m["pos"]), m["tets"], m["coloring"])
+        pos = np.asarray(m["pos"], np.float32).astype(np.float64) if precision == 32 else m["pos"]
+        vel = None if m["vel"] is None else (np.asarray(m["vel"], np.float32).astype(np.float64)
+                                             if precision == 32 else m["vel"])
+        sc.add_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
+    sc.finalize()
+    return sc
