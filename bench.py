@@ -38,13 +38,13 @@ def build_scene(ctx, spec):
     return sim
 
 
-def autotune(ctx, batch, candidates, reps=2):
-    """Times one graph-captured frame per schedule and keeps the fastest."""
+def autotune(ctx, batch, candidates, reps=2, use_graph=True):
+    """Times one (graph-captured) frame per schedule and keeps the fastest."""
     best = None
     results = {}
     for lpv, block in candidates:
         batch.set_schedule(lpv, block)
-        batch.record_graph(True)
+        batch.record_graph(use_graph)
         batch.run_steps(1)
         ctx.synchronize()
         ctx.event_record(0)
@@ -55,7 +55,7 @@ def autotune(ctx, batch, candidates, reps=2):
         if best is None or ms < best[0]:
             best = (ms, lpv, block)
     batch.set_schedule(best[1], best[2])
-    batch.record_graph(True)
+    batch.record_graph(use_graph)
     return (best[1], best[2]), results
 
 
@@ -124,7 +124,12 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--schedule", default=None, help="fixed LPVxBLOCK (e.g. 8x256) instead of autotuning")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth calibration")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels directly instead of replaying HIP graphs (profiler runs)")
     args = ap.parse_args()
+    use_graph = not args.no_graph
     if args.gpus != 1:
         print("bench.py measures one GPU: every scene of the workload runs on one device; "
               "multi-GPU runs are not part of this build (scenes would be sharded whole per GPU).",
@@ -147,7 +152,7 @@ def main():
     plain_rate = None
     if not args.no_plain:
         for s in sims:
-            s.recordVBDSolveGraph(True)
+            s.recordVBDSolveGraph(use_graph)
         for _ in range(args.warmup):
             for s in sims:
                 s.runStep()
@@ -163,7 +168,14 @@ def main():
 
     # ---- batched: all scenes, one launch per parallel group, one graph per frame
     batch = Batch(ctx, sims)
-    sched, tune = autotune(ctx, batch, [(1, 256), (2, 256), (4, 256), (8, 256), (4, 128), (8, 128)])
+    if args.schedule:
+        lpv, block = (int(x) for x in args.schedule.split("x"))
+        batch.set_schedule(lpv, block)
+        batch.record_graph(use_graph)
+        sched, tune = (lpv, block), {}
+    else:
+        sched, tune = autotune(ctx, batch, [(1, 256), (2, 256), (4, 256), (8, 256), (4, 128), (8, 128)],
+                               use_graph=use_graph)
     log(f"autotune (ms/frame): {tune} -> lpv {sched[0]}, block {sched[1]}")
     for _ in range(args.warmup):
         batch.run_steps(1)
@@ -176,6 +188,15 @@ def main():
     log(f"batched: {batched_s / args.steps * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s")
 
     roof = solve_kernel_roofline(ctx, batch)
+    # measured HBM bytes per launch (scaled PMC FETCH_SIZE + WRITE_SIZE) for this kernel,
+    # workload and schedule, from the committed profile summary
+    side = os.path.join(REPO, "profiles", "tet_group_solve_batched_256_tet_scenes.traffic.json")
+    if os.path.exists(side) and args.scenes == 256 and args.seed == 0:
+        t = json.load(open(side))
+        if t.get("schedule") == f"{sched[0]}x{sched[1]}":
+            roof["traffic"] = t["hbm_bytes_per_launch"]
+    copy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5)
+    roof["measured_copy_gbs"] = round(2 * args.copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
     log(f"roofline: {roof}")
     cpu = None if args.no_cpu_baseline else cpu_baseline(specs)
     if cpu:

@@ -107,6 +107,7 @@ _SIGNATURES = {
     "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
     "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],
+    "gvbd_stream_copy": [_P, ctypes.c_int64, _I, c_float_p],
     "gvbd_event_record": [_P, _I],
     "gvbd_event_elapsed_ms": [_P, _I, _I, c_float_p],
     "gvbd_tet_mesh_constants": [_I, c_float_p, _I, c_int32_p, ctypes.c_float, c_float_p, c_float_p, c_float_p],

@@ -18,4 +18,7 @@ hipError_t launch_tet_initial_step(const SceneView* scenes, const WorkItem* item
 hipError_t launch_tet_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems,
                                       hipStream_t stream);
 
+// dst[i] = src[i] for n float4 elements (bandwidth / profiler calibration).
+hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream);
+
 }  // namespace gvbd

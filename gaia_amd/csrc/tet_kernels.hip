@@ -305,6 +305,22 @@ hipError_t launch_tet_update_velocity(const SceneView* scenes, const WorkItem* i
     return hipGetLastError();
 }
 
+// Plain streaming copy (float4 per lane, grid-stride): the reference kernel
+// whose HBM counters calibrate the profiler and whose rate is the measured
+// copy bandwidth of the device.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                          int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 bool valid_schedule(int lpv, int block) {
     const bool l = lpv == 1 || lpv == 2 || lpv == 4 || lpv == 8 || lpv == 16;
     const bool b = block == 64 || block == 128 || block == 256;

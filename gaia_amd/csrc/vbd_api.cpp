@@ -878,6 +878,37 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
     });
 }
 
+int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms) {
+    return guard(ctx, [&] {
+        require(ctx && avg_ms && bytes > 0 && bytes % 16 == 0 && reps > 0, "bad argument");
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipFree(a);
+            fail(GVBD_ERR_NOMEM, "hipMalloc for stream copy failed");
+        }
+        try {
+            hip_check(hipMemsetAsync(a, 0, bytes, ctx->stream), "memset");
+            const int64_t n = bytes / 16;
+            hip_check(launch_stream_copy((const float4*)a, (float4*)b, n, ctx->stream), "copy launch");
+            hip_check(hipEventRecord(ctx->events[62], ctx->stream), "event");
+            for (int32_t r = 0; r < reps; ++r)
+                hip_check(launch_stream_copy((const float4*)a, (float4*)b, n, ctx->stream), "copy launch");
+            hip_check(hipEventRecord(ctx->events[63], ctx->stream), "event");
+            hip_check(hipEventSynchronize(ctx->events[63]), "event sync");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ctx->events[62], ctx->events[63]), "elapsed");
+            *avg_ms = ms / (float)reps;
+        } catch (...) {
+            (void)hipFree(a);
+            (void)hipFree(b);
+            throw;
+        }
+        hip_check(hipFree(a), "hipFree");
+        hip_check(hipFree(b), "hipFree");
+    });
+}
+
 int gvbd_event_record(gvbd_context* ctx, int32_t slot) {
     return guard(ctx, [&] {
         require(ctx != nullptr && slot >= 0 && slot < 64, "bad argument");

@@ -110,6 +110,12 @@ class Context:
         self.check(lib().gvbd_context_memory(self.handle, ctypes.byref(b)))
         return b.value
 
+    def stream_copy_ms(self, nbytes, reps=5):
+        """Average time of a plain device copy of nbytes (measured copy bandwidth)."""
+        ms = ctypes.c_float()
+        self.check(lib().gvbd_stream_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)))
+        return ms.value
+
     def event_record(self, slot):
         self.check(lib().gvbd_event_record(self.handle, slot))
 

@@ -158,6 +158,10 @@ int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n)
 int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
 
 /* ---- timing on the context stream ---- */
+/* copies `bytes` (multiple of 16) between two device buffers `reps` times with a plain
+ * streaming kernel and returns the average kernel time: the measured copy bandwidth and
+ * the calibration point for profiler byte counters (no reference counterpart) */
+int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms);
 int gvbd_event_record(gvbd_context* ctx, int32_t slot);  /* slot 0..63 */
 int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, float* ms);
 
