@@ -32,10 +32,19 @@ def build_scene(ctx, spec):
 
     sim = VBDPhysics(ctx)
     sim.setPhysicsParams(**spec["physics"])
-    colors = gaia_amd.color_tet_mesh(len(spec["positions"]), spec["tets"], spec["coloring"])
-    sim.addTetMesh(spec["positions"], spec["tets"], vertex_colors=colors, **spec["material"])
+    if spec["kind"] == "cloth":
+        colors = gaia_amd.color_tri_mesh(len(spec["positions"]), spec["faces"], spec["coloring"])
+        sim.addClothMesh(spec["positions"], spec["faces"], fixed_points=spec["fixed"], vertex_colors=colors,
+                         **spec["material"])
+    else:
+        colors = gaia_amd.color_tet_mesh(len(spec["positions"]), spec["tets"], spec["coloring"])
+        sim.addTetMesh(spec["positions"], spec["tets"], vertex_colors=colors, **spec["material"])
     sim.initialize()
     return sim
+
+
+def free_vertices(spec):
+    return len(spec["positions"]) - len(spec.get("fixed", []))
 
 
 def autotune(ctx, batch, candidates, reps=2, use_graph=True):
@@ -82,7 +91,7 @@ def solve_kernel_roofline(ctx, batch, reps=5):
     gbs = avg_bytes / (avg_ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="tet_group_solve_kernel", avg_launch_us=round(avg_ms * 1e3, 2),
+                kernel="group_solve_kernel", avg_launch_us=round(avg_ms * 1e3, 2),
                 algorithmic_bytes_per_launch=int(avg_bytes))
 
 
@@ -100,14 +109,20 @@ def cpu_baseline(specs, budget_s=12.0):
     elapsed = 0.0
     used = 0
     for spec in specs:
-        sc = pyoracle.TetScene(physics_dict(**spec["physics"]), precision=32)
-        colors = gaia_amd.color_tet_mesh(len(spec["positions"]), spec["tets"], spec["coloring"])
-        sc.add_mesh(spec["positions"].astype(np.float64), spec["tets"], spec["material"], colors)
+        if spec["kind"] == "cloth":
+            sc = pyoracle.ClothScene(physics_dict(**spec["physics"]), precision=32)
+            colors = gaia_amd.color_tri_mesh(len(spec["positions"]), spec["faces"], spec["coloring"])
+            sc.add_mesh(spec["positions"].astype(np.float64), spec["faces"], spec["material"], colors,
+                        fixed=spec["fixed"])
+        else:
+            sc = pyoracle.TetScene(physics_dict(**spec["physics"]), precision=32)
+            colors = gaia_amd.color_tet_mesh(len(spec["positions"]), spec["tets"], spec["coloring"])
+            sc.add_mesh(spec["positions"].astype(np.float64), spec["tets"], spec["material"], colors)
         sc.finalize()
         t0 = time.perf_counter()
         sc.step(1, threads=threads)
         elapsed += time.perf_counter() - t0
-        solves += len(spec["positions"]) * spec["physics"]["iterations"]
+        solves += free_vertices(spec) * spec["physics"]["iterations"]
         used += 1
         if elapsed > budget_s:
             break
@@ -140,13 +155,15 @@ def main():
     from gaia_amd import Batch, scenes
 
     t0 = time.perf_counter()
-    specs = scenes.c4_tet_scenes(args.seed, args.scenes)
+    specs = scenes.c4_scenes(args.seed, args.scenes)
     ctx = gaia_amd.Context(0)
     sims = [build_scene(ctx, s) for s in specs]
-    nverts = sum(len(s["positions"]) for s in specs)
+    # vertex-solves count free (solved) vertices only
+    nverts = sum(free_vertices(s) for s in specs)
+    ncloth = sum(1 for s in specs if s["kind"] == "cloth")
     iters = specs[0]["physics"]["iterations"]
-    log(f"built {len(sims)} scenes, {nverts} vertices, in {time.perf_counter() - t0:.1f}s; "
-        f"device memory {ctx.memory_bytes() / 2**30:.2f} GiB")
+    log(f"built {len(sims)} scenes ({len(sims) - ncloth} tet, {ncloth} cloth), {nverts} free vertices, in "
+        f"{time.perf_counter() - t0:.1f}s; device memory {ctx.memory_bytes() / 2**30:.2f} GiB")
 
     # ---- plain: one scene per launch sequence (each scene replays its own graph)
     plain_rate = None
@@ -190,7 +207,7 @@ def main():
     roof = solve_kernel_roofline(ctx, batch)
     # measured HBM bytes per launch (scaled PMC FETCH_SIZE + WRITE_SIZE) for this kernel,
     # workload and schedule, from the committed profile summary
-    side = os.path.join(REPO, "profiles", "tet_group_solve_batched_256_tet_scenes.traffic.json")
+    side = os.path.join(REPO, "profiles", "group_solve_batched_256_scenes.traffic.json")
     if os.path.exists(side) and args.scenes == 256 and args.seed == 0:
         t = json.load(open(side))
         if t.get("schedule") == f"{sched[0]}x{sched[1]}":
@@ -216,9 +233,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched VBD, {len(specs)} tet-cube scenes, log-uniform 500-50k vertices "
-                        f"(seed {args.seed}), dt=1/60, 10 iterations, HIP graph per frame",
+            "workload": f"batched VBD: {len(specs)} scenes of log-uniform 500-50k vertices (seed {args.seed}), "
+                        f"{len(specs) - ncloth} neo-Hookean tet cubes + {ncloth} StVK+bending cloth sheets pinned "
+                        f"at two corners, dt=1/60, 10 iterations, one launch per parallel group, HIP graph "
+                        f"per frame",
             "scenes": len(specs),
+            "cloth_scenes": ncloth,
             "vertices": nverts,
             "iterations": iters,
             "schedule": {"lanes_per_vertex": sched[0], "block": sched[1]},

@@ -6,4 +6,14 @@ hand-written HIP kernels for gfx950.
 """
 from . import scenes  # noqa: F401
 from ._lib import LIB_PATH, VBDError, header_functions, lib  # noqa: F401
-from .vbd import Batch, Context, VBDPhysics, color_graph, color_tet_mesh, physics_params  # noqa: F401
+from .vbd import (  # noqa: F401
+    Batch,
+    Context,
+    VBDPhysics,
+    cloth_mesh_constants,
+    color_graph,
+    color_tet_mesh,
+    color_tri_mesh,
+    physics_params,
+    tet_mesh_constants,
+)

@@ -41,6 +41,16 @@ class PhysicsParams(ctypes.Structure):
         ("step_size_gd", ctypes.c_float),
         ("friction_start_iter", ctypes.c_int32),
         ("associate_gravity_with_inertia", ctypes.c_int32),
+        ("cloth_boundary_contact_stiffness", ctypes.c_float),
+        ("cloth_boundary_friction_eps", ctypes.c_float),
+        ("degenerate_triangle_thres", ctypes.c_float),
+        ("cloth_step_size", ctypes.c_float),
+        ("handle_collision", ctypes.c_int32),
+        ("contact_stiffness", ctypes.c_float),
+        ("contact_radius", ctypes.c_float),
+        ("thickness", ctypes.c_float),
+        ("conservative_step_relaxation", ctypes.c_float),
+        ("contact_energy_type", ctypes.c_int32),
     ]
 
 
@@ -70,11 +80,37 @@ class TetMeshDesc(ctypes.Structure):
     ]
 
 
+class ClothMeshDesc(ctypes.Structure):
+    """gvbd_cloth_mesh_desc."""
+
+    _fields_ = [
+        ("num_vertices", ctypes.c_int32),
+        ("positions", c_float_p),
+        ("velocities", c_float_p),
+        ("initial_velocity", ctypes.c_float * 3),
+        ("num_faces", ctypes.c_int32),
+        ("faces", c_int32_p),
+        ("density", ctypes.c_float),
+        ("mu", ctypes.c_float),
+        ("lmbd", ctypes.c_float),
+        ("bending_stiffness", ctypes.c_float),
+        ("damping_stvk", ctypes.c_float),
+        ("damping_bending", ctypes.c_float),
+        ("exponential_vel_damping", ctypes.c_float),
+        ("constant_vel_damping", ctypes.c_float),
+        ("initialization_type", ctypes.c_int32),
+        ("num_fixed", ctypes.c_int32),
+        ("fixed_points", c_int32_p),
+        ("vertex_colors", c_int32_p),
+        ("coloring_algorithm", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 # name -> argtypes (all return int unless listed in _RESTYPES)
 _SIGNATURES = {
-    "gvbd_abi_struct_sizes": [c_int32_p, c_int32_p],
+    "gvbd_abi_struct_sizes": [c_int32_p, c_int32_p, c_int32_p],
     "gvbd_context_create": [_I, ctypes.POINTER(_P)],
     "gvbd_context_destroy": [_P],
     "gvbd_last_error": [_P],
@@ -83,6 +119,7 @@ _SIGNATURES = {
     "gvbd_scene_create": [_P, ctypes.POINTER(PhysicsParams), c_int32_p],
     "gvbd_scene_destroy": [_P, _I],
     "gvbd_scene_add_tet_mesh": [_P, _I, ctypes.POINTER(TetMeshDesc), c_int32_p],
+    "gvbd_scene_add_cloth_mesh": [_P, _I, ctypes.POINTER(ClothMeshDesc), c_int32_p],
     "gvbd_scene_initialize": [_P, _I],
     "gvbd_scene_load_json": [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_int32_p],
     "gvbd_scene_num_meshes": [_P, _I, c_int32_p],
@@ -113,6 +150,9 @@ _SIGNATURES = {
     "gvbd_tet_mesh_constants": [_I, c_float_p, _I, c_int32_p, ctypes.c_float, c_float_p, c_float_p, c_float_p],
     "gvbd_color_graph": [_I, _I, c_int32_p, _I, c_int32_p, c_int32_p],
     "gvbd_color_tet_mesh": [_I, _I, c_int32_p, _I, c_int32_p, c_int32_p],
+    "gvbd_cloth_mesh_constants": [_I, c_float_p, _I, c_int32_p, ctypes.c_float, c_float_p, c_float_p, c_float_p,
+                                  c_int32_p, c_int32_p, c_float_p],
+    "gvbd_color_tri_mesh": [_I, _I, c_int32_p, _I, _I, c_int32_p, c_int32_p],
 }
 _RESTYPES = {"gvbd_last_error": ctypes.c_char_p}
 

@@ -36,6 +36,39 @@ enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
 };
 
+// One incident triangle of a cloth vertex v, relative to v:
+//   F (3x2) = (x_nb0 - x_v) m0^T + (x_nb1 - x_v) m1^T,  m_v = -(m0 + m1)
+// with m0, m1 the rows of the face's 2x2 DmInv (or minus their sum for face vertex 0).
+struct alignas(16) FaceInc {
+    int32_t nb[2];
+    float Amu;    // rest area * mu
+    float Alm;    // rest area * lambda
+    float m[4];   // m0 (2), m1 (2)
+};
+static_assert(sizeof(FaceInc) == 32, "FaceInc must be 32 bytes");
+
+// One relevant bending stencil of a cloth vertex v (VBDTriMeshStVK.cpp:913-982): v plays role
+// `order` in (eV1, eV2, eV12Next, eV21Next); nb are the other three in role order; q holds
+// bendingStiffness * Q(order, role of nb[k]) and qself bendingStiffness * Q(order, order).
+struct alignas(16) BendInc {
+    int32_t nb[3];
+    int32_t order;
+    float q[3];
+    float qself;
+};
+static_assert(sizeof(BendInc) == 32, "BendInc must be 32 bytes");
+
+// Cloth work slot: (vertex, face begin, face end, flags), (bending begin, bending end, -, -).
+struct alignas(16) ClothSlot {
+    int4 a;
+    int4 b;
+};
+
+enum WorkKind : int32_t {
+    WORK_TET = 0,
+    WORK_CLOTH = 1,
+};
+
 // Scene-wide constants of the per-vertex solve (VBDPhysicsParameters / BasePhysicsParams).
 struct SolveParams {
     float dt;
@@ -53,6 +86,11 @@ struct SolveParams {
     int32_t associateGravity;
     int32_t frictionStartIter;
     int32_t pad;
+    // cloth (VBDClothPhysicsParameters)
+    float clothBoundaryStiffness;  // boundaryContactStiffness
+    float clothFrictionEpsU;       // boundaryFrictionEpsValue * dt
+    float degenerateThresSq;       // degenerateTriangleThres^2
+    float clothStepSize;
 };
 
 // Device view of one scene, referenced by work items of a (batched) launch.
@@ -65,17 +103,23 @@ struct SceneView {
     const uint8_t* vflag;
     const int4* slots;
     const TetInc* tinc;
+    // cloth
+    float4* pnext;  // Jacobi target of cloth solves (positionsNext)
+    const ClothSlot* cslots;
+    const FaceInc* cface;
+    const BendInc* cbend;
     int32_t nv;
     int32_t pad;
     SolveParams sp;
 };
 
-// One block's share of a parallel-group launch: slots [slotBegin, slotBegin + count) of scene `scene`.
+// One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`
+// (tet slots or cloth slots by `kind`; vertex ranges for the per-vertex kernels).
 struct WorkItem {
     int32_t scene;
     int32_t slotBegin;
     int32_t count;
-    int32_t pad;
+    int32_t kind;
 };
 
 }  // namespace gvbd

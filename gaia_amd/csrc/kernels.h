@@ -9,14 +9,16 @@ namespace gvbd {
 // Lanes per vertex (1, 2, 4, 8, 16) and block size (64, 128, 256) of the group solve.
 bool valid_schedule(int lpv, int block);
 
-// One parallel group: item i covers `count` consecutive colour-major slots of one scene.
-// Friction applies when iteration >= SolveParams::frictionStartIter (VBDPhysics.cpp:1610).
-hipError_t launch_tet_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
-                                  int lpv, int block, hipStream_t stream);
+// One parallel group: item i covers `count` consecutive colour-major slots of one scene,
+// tet or cloth by item kind. Tet friction applies when iteration >= frictionStartIter
+// (VBDPhysics.cpp:1610).
+hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
+                              int block, hipStream_t stream);
+// Copies pnext -> pos for the cloth slots of `items` (after the group's solves).
+hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 // Items cover vertex ranges [slotBegin, slotBegin + count), count <= 256.
-hipError_t launch_tet_initial_step(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
-hipError_t launch_tet_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems,
-                                      hipStream_t stream);
+hipError_t launch_initial_step(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
+hipError_t launch_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 
 // dst[i] = src[i] for n float4 elements (bandwidth / profiler calibration).
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream);

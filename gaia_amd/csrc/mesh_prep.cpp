@@ -253,17 +253,16 @@ std::vector<std::vector<int32_t>> sorted_categories(const std::vector<int32_t>& 
 }
 
 std::vector<std::vector<int32_t>> merge_parallel_groups(
-    const std::vector<const std::vector<std::vector<int32_t>>*>& meshCategories, bool removeUsed) {
+    const std::vector<const std::vector<std::vector<int32_t>>*>& meshCategories, const std::vector<bool>& removeUsed) {
     size_t ng = 0;
     for (auto* c : meshCategories) ng = std::max(ng, c->size());
     std::vector<std::vector<int32_t>> groups(ng);
     for (size_t im = 0; im < meshCategories.size(); ++im) {
         const auto& cats = *meshCategories[im];
-        std::vector<int32_t> avail(ng);
-        for (size_t i = 0; i < ng; ++i) avail[i] = (int32_t)i;
-        // tet meshes: the available list starts with the mesh's own colour count
-        // (VBDPhysics.cpp:245-252); groups beyond it are never chosen.
-        if (removeUsed) avail.resize(cats.size());
+        // the available list holds the mesh's own colour count of groups (VBDPhysics.cpp:245-252,
+        // VBDClothPhysics.cpp:88-94); groups beyond it are never chosen for this mesh
+        std::vector<int32_t> avail(cats.size());
+        for (size_t i = 0; i < cats.size(); ++i) avail[i] = (int32_t)i;
         for (size_t c = 0; c < cats.size(); ++c) {
             size_t best = groups[avail[0]].size();
             size_t bestI = 0;
@@ -273,7 +272,7 @@ std::vector<std::vector<int32_t>> merge_parallel_groups(
                     bestI = i;
                 }
             const int32_t gid = avail[bestI];
-            if (removeUsed) avail.erase(avail.begin() + bestI);
+            if (removeUsed[im]) avail.erase(avail.begin() + bestI);
             for (int32_t v : cats[c]) {
                 groups[gid].push_back((int32_t)im);
                 groups[gid].push_back(v);
@@ -281,6 +280,271 @@ std::vector<std::vector<int32_t>> merge_parallel_groups(
         }
     }
     return groups;
+}
+
+namespace {
+
+inline void cross3f(const float* a, const float* b, float* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline float dot3f(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Edge key -> id via a sorted list of (min, max, first index).
+struct EdgeTable {
+    std::vector<std::pair<uint64_t, int32_t>> keys;
+    static uint64_t key(int32_t a, int32_t b) {
+        const uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+        return ((uint64_t)lo << 32) | hi;
+    }
+    int32_t find(int32_t a, int32_t b) const {
+        const uint64_t k = key(a, b);
+        auto it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(k, (int32_t)INT32_MIN));
+        return (it != keys.end() && it->first == k) ? it->second : -1;
+    }
+};
+
+}  // namespace
+
+bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err) {
+    if ((int64_t)m.pos.size() != 3ll * m.nv || (int64_t)m.faces.size() != 3ll * m.nf) {
+        err = "cloth mesh: array sizes do not match vertex/face counts";
+        return false;
+    }
+    for (int32_t v : m.faces)
+        if (v < 0 || v >= m.nv) {
+            err = "cloth mesh: vertex index out of range";
+            return false;
+        }
+    for (int32_t f : m.fixed)
+        if (f < 0 || f >= m.nv) {
+            err = "cloth mesh: fixed point index out of range";
+            return false;
+        }
+    // rest pose
+    m.dminv.assign(4 * (size_t)m.nf, 0.f);
+    m.area.assign(m.nf, 0.f);
+    m.mass.assign(m.nv, 0.f);
+    for (int32_t f = 0; f < m.nf; ++f) {
+        const int32_t* fv = &m.faces[3 * (size_t)f];
+        float e1[3], e2[3], n[3];
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = m.pos[3 * (size_t)fv[1] + d] - m.pos[3 * (size_t)fv[0] + d];
+            e2[d] = m.pos[3 * (size_t)fv[2] + d] - m.pos[3 * (size_t)fv[0] + d];
+        }
+        cross3f(e1, e2, n);
+        const float nn = std::sqrt(dot3f(n, n)), l1 = std::sqrt(dot3f(e1, e1));
+        if (!(nn > 0.f) || !(l1 > 0.f)) {
+            err = "cloth mesh: degenerate face " + std::to_string(f);
+            return false;
+        }
+        const float a1[3] = {e1[0] / l1, e1[1] / l1, e1[2] / l1};
+        const float un[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+        float a2[3];
+        cross3f(un, a1, a2);
+        const float Dm[4] = {dot3f(a1, e1), dot3f(a2, e1), dot3f(a1, e2), dot3f(a2, e2)};
+        const float det = Dm[0] * Dm[3] - Dm[2] * Dm[1];
+        m.area[f] = det * 0.5f;
+        const float inv = 1.f / det;
+        float* Di = &m.dminv[4 * (size_t)f];
+        Di[0] = Dm[3] * inv;
+        Di[1] = -Dm[1] * inv;
+        Di[2] = -Dm[2] * inv;
+        Di[3] = Dm[0] * inv;
+        for (int k = 0; k < 3; ++k) m.mass[fv[k]] += m.mat.density * m.area[f] / 3.f;
+    }
+    // edges, numbered by first appearance
+    EdgeTable table;
+    {
+        std::vector<std::pair<uint64_t, int32_t>> occ;
+        occ.reserve(3 * (size_t)m.nf);
+        for (int32_t f = 0; f < m.nf; ++f)
+            for (int k = 0; k < 3; ++k)
+                occ.push_back({EdgeTable::key(m.faces[3 * (size_t)f + k], m.faces[3 * (size_t)f + (k + 1) % 3]),
+                               3 * f + k});
+        std::stable_sort(occ.begin(), occ.end(),
+                         [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
+                             return a.first < b.first;
+                         });
+        // first occurrence of each key, in occurrence order
+        std::vector<int32_t> firstOcc;
+        for (size_t i = 0; i < occ.size(); ++i)
+            if (i == 0 || occ[i].first != occ[i - 1].first) firstOcc.push_back(occ[i].second);
+        std::sort(firstOcc.begin(), firstOcc.end());
+        m.edges.clear();
+        m.edges.reserve(firstOcc.size());
+        for (int32_t o : firstOcc) {
+            const int32_t f = o / 3, k = o % 3;
+            const int32_t a = m.faces[3 * (size_t)f + k], b = m.faces[3 * (size_t)f + (k + 1) % 3],
+                          c = m.faces[3 * (size_t)f + (k + 2) % 3];
+            table.keys.push_back({EdgeTable::key(a, b), (int32_t)m.edges.size()});
+            m.edges.push_back({a, b, c, -1, f, -1});
+        }
+        std::sort(table.keys.begin(), table.keys.end());
+        for (size_t i = 0; i < occ.size(); ++i) {
+            const int32_t f = occ[i].second / 3, k = occ[i].second % 3;
+            const int32_t a = m.faces[3 * (size_t)f + k], b = m.faces[3 * (size_t)f + (k + 1) % 3],
+                          c = m.faces[3 * (size_t)f + (k + 2) % 3];
+            TriEdge& e = m.edges[table.find(a, b)];
+            if (e.f1 == f) continue;
+            if (e.f2 != -1 || e.v1 != b || e.v2 != a) {
+                err = "cloth mesh: non-manifold or inconsistently oriented edge (" + std::to_string(a) + ", " +
+                      std::to_string(b) + ")";
+                return false;
+            }
+            e.f2 = f;
+            e.v21Next = c;
+        }
+    }
+    // vertex -> faces
+    m.vfBegin.assign(m.nv + 1, 0);
+    for (int32_t v : m.faces) m.vfBegin[v + 1]++;
+    for (int32_t i = 0; i < m.nv; ++i) m.vfBegin[i + 1] += m.vfBegin[i];
+    m.vfFace.assign(m.vfBegin[m.nv], 0);
+    m.vfOrder.assign(m.vfBegin[m.nv], 0);
+    {
+        std::vector<int32_t> fill(m.vfBegin.begin(), m.vfBegin.end() - 1);
+        for (int32_t f = 0; f < m.nf; ++f)
+            for (int k = 0; k < 3; ++k) {
+                const int32_t v = m.faces[3 * (size_t)f + k];
+                m.vfFace[fill[v]] = f;
+                m.vfOrder[fill[v]] = k;
+                fill[v]++;
+            }
+    }
+    // vertex -> incident edges (ascending edge id)
+    std::vector<int32_t> veBegin(m.nv + 1, 0), veEdge(2 * m.edges.size());
+    for (const TriEdge& e : m.edges) {
+        veBegin[e.v1 + 1]++;
+        veBegin[e.v2 + 1]++;
+    }
+    for (int32_t i = 0; i < m.nv; ++i) veBegin[i + 1] += veBegin[i];
+    {
+        std::vector<int32_t> fill(veBegin.begin(), veBegin.end() - 1);
+        for (int32_t id = 0; id < (int32_t)m.edges.size(); ++id) {
+            veEdge[fill[m.edges[id].v1]++] = id;
+            veEdge[fill[m.edges[id].v2]++] = id;
+        }
+    }
+    m.vbBegin.assign(m.nv + 1, 0);
+    m.vbEdge.clear();
+    m.vbOrder.clear();
+    for (int32_t v = 0; v < m.nv; ++v) {
+        for (int32_t k = veBegin[v]; k < veBegin[v + 1]; ++k) {
+            const TriEdge& e = m.edges[veEdge[k]];
+            if (e.f2 == -1) continue;
+            m.vbEdge.push_back(veEdge[k]);
+            m.vbOrder.push_back(e.v1 == v ? 0 : 1);
+        }
+        for (int32_t k = m.vfBegin[v]; k < m.vfBegin[v + 1]; ++k) {
+            const int32_t f = m.vfFace[k], o = m.vfOrder[k];
+            const int32_t id = table.find(m.faces[3 * (size_t)f + (o + 1) % 3], m.faces[3 * (size_t)f + (o + 2) % 3]);
+            const TriEdge& e = m.edges[id];
+            if (e.f2 == -1) continue;
+            m.vbEdge.push_back(id);
+            m.vbOrder.push_back(e.f1 == f ? 2 : 3);
+        }
+        m.vbBegin[v + 1] = (int32_t)m.vbEdge.size();
+    }
+    // bending quadratic forms
+    m.Q.assign(16 * m.edges.size(), 0.f);
+    auto cot = [](const float* a, const float* b) {
+        float c[3];
+        cross3f(a, b, c);
+        return dot3f(a, b) / std::sqrt(dot3f(c, c));
+    };
+    auto face_normal_len = [&](int32_t f) {
+        const int32_t* fv = &m.faces[3 * (size_t)f];
+        float a[3], b[3], n[3];
+        for (int d = 0; d < 3; ++d) {
+            a[d] = m.pos[3 * (size_t)fv[1] + d] - m.pos[3 * (size_t)fv[0] + d];
+            b[d] = m.pos[3 * (size_t)fv[2] + d] - m.pos[3 * (size_t)fv[0] + d];
+        }
+        cross3f(a, b, n);
+        return std::sqrt(dot3f(n, n));
+    };
+    for (size_t id = 0; id < m.edges.size(); ++id) {
+        const TriEdge& e = m.edges[id];
+        if (e.f2 == -1) continue;
+        const float* X0 = &m.pos[3 * (size_t)e.v1];
+        const float* X1 = &m.pos[3 * (size_t)e.v2];
+        const float* X2 = &m.pos[3 * (size_t)e.v12Next];
+        const float* X3 = &m.pos[3 * (size_t)e.v21Next];
+        float e01[3], e03[3], e02[3], e12[3], e13[3], me01[3];
+        for (int d = 0; d < 3; ++d) {
+            e01[d] = X1[d] - X0[d];
+            e03[d] = X3[d] - X0[d];
+            e02[d] = X2[d] - X0[d];
+            e12[d] = X2[d] - X1[d];
+            e13[d] = X3[d] - X1[d];
+            me01[d] = -e01[d];
+        }
+        const float c01 = cot(e02, e01), c02 = cot(e01, e03), c03 = cot(me01, e12), c04 = cot(me01, e13);
+        const float K0[4] = {c03 + c04, c01 + c02, -c01 - c03, -c02 - c04};
+        const float A = face_normal_len(e.f1) + face_normal_len(e.f2);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) m.Q[16 * id + 4 * i + j] = 3.f * K0[i] * K0[j] / A;
+    }
+    return true;
+}
+
+Graph tri_vertex_graph(int32_t nv, int32_t nf, const int32_t* faces, bool withBending) {
+    std::vector<std::pair<uint64_t, int32_t>> occ;  // (edge key, opposite vertex)
+    occ.reserve(3 * (size_t)nf);
+    for (int32_t f = 0; f < nf; ++f)
+        for (int k = 0; k < 3; ++k)
+            occ.push_back({EdgeTable::key(faces[3 * (size_t)f + k], faces[3 * (size_t)f + (k + 1) % 3]),
+                           faces[3 * (size_t)f + (k + 2) % 3]});
+    std::sort(occ.begin(), occ.end());
+    std::vector<int32_t> edges;
+    for (size_t i = 0; i < occ.size();) {
+        size_t j = i;
+        while (j < occ.size() && occ[j].first == occ[i].first) ++j;
+        edges.push_back((int32_t)(occ[i].first >> 32));
+        edges.push_back((int32_t)(occ[i].first & 0xffffffffu));
+        if (withBending && j - i == 2) {
+            edges.push_back(occ[i].second);
+            edges.push_back(occ[i + 1].second);
+        }
+        i = j;
+    }
+    return graph_from_edges(nv, edges.data(), (int32_t)(edges.size() / 2));
+}
+
+bool load_obj_file(const std::string& path, std::vector<float>& pos, std::vector<int32_t>& faces, std::string& err) {
+    std::ifstream in(path);
+    if (!in) {
+        err = "cannot open " + path;
+        return false;
+    }
+    pos.clear();
+    faces.clear();
+    std::string line, tag;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        if (!(ss >> tag)) continue;
+        if (tag == "v") {
+            double x, y, z;
+            if (!(ss >> x >> y >> z)) {
+                err = "bad v line in " + path;
+                return false;
+            }
+            pos.push_back((float)x);
+            pos.push_back((float)y);
+            pos.push_back((float)z);
+        } else if (tag == "f") {
+            std::vector<int32_t> ids;
+            std::string tok;
+            while (ss >> tok) ids.push_back((int32_t)std::strtol(tok.c_str(), nullptr, 10) - 1);
+            if (ids.size() != 3) {
+                err = "only triangle faces are supported in " + path;
+                return false;
+            }
+            faces.insert(faces.end(), ids.begin(), ids.end());
+        }
+    }
+    return true;
 }
 
 bool load_t_file(const std::string& path, std::vector<float>& pos, std::vector<int32_t>& tets, std::string& err) {

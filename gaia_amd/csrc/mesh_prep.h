@@ -62,11 +62,62 @@ bool coloring_valid(const Graph& g, const std::vector<int32_t>& colors);
 // std::sort; this build uses a stable sort (equal-size colours keep colour order).
 std::vector<std::vector<int32_t>> sorted_categories(const std::vector<int32_t>& colors);
 
-// VBDPhysics.cpp:99-122,235-274: colour c of mesh k goes to the currently smallest
-// parallel group; with `removeUsed` a mesh uses each group at most once (tet meshes).
-// Groups hold interleaved (mesh, vertex) pairs.
-std::vector<std::vector<int32_t>> merge_parallel_groups(const std::vector<const std::vector<std::vector<int32_t>>*>& meshCategories,
-                                                        bool removeUsed);
+// VBDPhysics.cpp:99-122,235-274 and VBDClothPhysics.cpp:72-111: colour c of mesh k goes to
+// the currently smallest of the first (number of colours of k) parallel groups; with
+// removeUsed[k] a mesh uses each group at most once (tet meshes), without it (cloth meshes)
+// two colours of one mesh may share a group. Groups hold interleaved (mesh, vertex) pairs.
+std::vector<std::vector<int32_t>> merge_parallel_groups(
+    const std::vector<const std::vector<std::vector<int32_t>>*>& meshCategories, const std::vector<bool>& removeUsed);
+
+struct ClothMaterial {
+    float density = 1.f;
+    float mu = 1e4f;
+    float lambda = 1e4f;
+    float bendingStiffness = 0.f;
+    float dampingStVK = 0.f;
+    float dampingBending = 0.f;
+    float exponentialVelDamping = 1.f;  // applied per substep as given (cloth does not rescale by dt)
+    float constantVelDamping = 0.f;
+    int32_t initializationType = 2;     // VBDObjectParamsTriMesh default: previous position
+};
+
+// Edge of a triangle mesh (TriMesh/TriMesh.h:51-60). Edges are numbered in order of first
+// appearance when scanning faces (edges v0v1, v1v2, v2v0 of each face); eV1 -> eV2 follows the
+// first face that contains the edge (fId1), eV12Next is that face's third vertex; fId2 /
+// eV21Next are the second face's (-1 on boundary edges).
+struct TriEdge {
+    int32_t v1, v2, v12Next, v21Next, f1, f2;
+};
+
+struct ClothMeshHost {
+    int32_t nv = 0, nf = 0;
+    std::vector<float> pos, vel;
+    std::vector<int32_t> faces;  // 3 * nf
+    ClothMaterial mat;
+    std::vector<int32_t> fixed;
+    std::vector<int32_t> colors;
+    // derived
+    std::vector<float> dminv, area, mass;              // 4 * nf (col-major 2x2), nf, nv
+    std::vector<TriEdge> edges;
+    std::vector<float> Q;                               // 16 per edge (row-major), zero on boundary edges
+    std::vector<int32_t> vfBegin, vfFace, vfOrder;      // vertex -> faces (ascending)
+    std::vector<int32_t> vbBegin, vbEdge, vbOrder;      // vertex -> relevant bendings (order 0..3)
+    std::vector<std::vector<int32_t>> categories;
+};
+
+// TriMesh.cpp:213-244 (rest pose from 3D: right-handed in-plane frame, Dm, area = det(Dm)/2,
+// mass += rho * area / 3), TriMesh.cpp:481-753 (edges, vertex->faces, relevant bendings:
+// incident interior edges ascending with order 0/1, then per incident face (ascending) its
+// opposite interior edge with order 2 (face is fId1) or 3) and VBDTriMeshStVK.cpp:50-88
+// (Q = 3 K0 K0^T / (|n1| + |n2|)). Fails on degenerate faces or non-manifold / inconsistently
+// oriented meshes.
+bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err);
+// TriMeshVertexGraph.h:20-45: mesh edges; with bending also the two vertices opposite each
+// interior edge.
+Graph tri_vertex_graph(int32_t nv, int32_t nf, const int32_t* faces, bool withBending);
+
+// Reads the vertices ("v x y z") and faces ("f a b c", 1-based, "a/b/c" forms allowed) of an .obj file.
+bool load_obj_file(const std::string& path, std::vector<float>& pos, std::vector<int32_t>& faces, std::string& err);
 
 // Reads a MeshFrame ".t" file ("Vertex i x y z" / "Tet i a b c d" lines).
 bool load_t_file(const std::string& path, std::vector<float>& pos, std::vector<int32_t>& tets, std::string& err);

@@ -62,6 +62,51 @@ std::string get_str(const json::Value& o, const char* k) {
     return v && v->type == json::Value::String ? v->str : std::string();
 }
 
+// Rotation (axis-angle vector), per-axis scale, translation, in that order.
+void transform_positions(const json::Value& mj, std::vector<float>& pos, float rotEps) {
+    float rot[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, trans[3] = {0, 0, 0};
+    get_vec3(mj, "rotation", rot);
+    get_vec3(mj, "scale", scale);
+    get_vec3(mj, "translation", trans);
+    const float angle = std::sqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2]);
+    float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
+    if (angle > rotEps) {
+        const float ax = rot[0] / angle, ay = rot[1] / angle, az = rot[2] / angle;
+        const float c = std::cos(angle), s = std::sin(angle), t = 1.f - c;
+        const float Rr[9] = {t * ax * ax + c,      t * ax * ay - s * az, t * ax * az + s * ay,
+                             t * ax * ay + s * az, t * ay * ay + c,      t * ay * az - s * ax,
+                             t * ax * az - s * ay, t * ay * az + s * ax, t * az * az + c};
+        for (int i = 0; i < 9; ++i) R[i] = Rr[i];
+    }
+    for (size_t i = 0; i + 2 < pos.size(); i += 3) {
+        float* x = &pos[i];
+        const float p0 = x[0], p1 = x[1], p2 = x[2];
+        for (int r = 0; r < 3; ++r) x[r] = R[3 * r] * p0 + R[3 * r + 1] * p1 + R[3 * r + 2] * p2;
+        for (int r = 0; r < 3; ++r) x[r] = x[r] * scale[r] + trans[r];
+    }
+}
+
+bool read_colors(const std::string& path, int32_t nv, std::vector<int32_t>& colors, std::string& err) {
+    json::Value cats;
+    if (!read_json(path, cats, err)) return false;
+    colors.assign(nv, -1);
+    for (size_t c = 0; c < cats.arr.size(); ++c)
+        for (const json::Value& v : cats.arr[c].arr) {
+            const int32_t id = (int32_t)v.num;
+            if (id < 0 || id >= nv) {
+                err = path + ": vertex id out of range";
+                return false;
+            }
+            colors[id] = (int32_t)c;
+        }
+    for (int32_t c : colors)
+        if (c < 0) {
+            err = path + ": colouring does not cover every vertex";
+            return false;
+        }
+    return true;
+}
+
 }  // namespace
 
 gvbd_physics_params default_physics_params() {
@@ -84,6 +129,16 @@ gvbd_physics_params default_physics_params() {
     p.step_size_gd = 1e-3f;
     p.friction_start_iter = 0;
     p.associate_gravity_with_inertia = 1;
+    p.cloth_boundary_contact_stiffness = 1e4f;
+    p.cloth_boundary_friction_eps = 1e-2f;
+    p.degenerate_triangle_thres = 1e-6f;
+    p.cloth_step_size = 1.0f;
+    p.handle_collision = 1;
+    p.contact_stiffness = 1e4f;
+    p.contact_radius = 1.0f;
+    p.thickness = 0.f;
+    p.conservative_step_relaxation = 0.45f;
+    p.contact_energy_type = 1;
     return p;
 }
 
@@ -92,8 +147,8 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
     json::Value models, params;
     if (!read_json(modelsPath, models, err) || !read_json(paramsPath, params, err)) return false;
     out.params = default_physics_params();
+    gvbd_physics_params& p = out.params;
     if (const json::Value* pp = params.get("PhysicsParams")) {
-        gvbd_physics_params& p = out.params;
         get_vec3(*pp, "gravity", p.gravity);
         get_num(*pp, "timeStep", p.time_step);
         get_int(*pp, "numSubsteps", p.num_substeps);
@@ -115,6 +170,14 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         get_num(*pp, "stepSizeGD", p.step_size_gd);
         get_int(*pp, "frictionStartIter", p.friction_start_iter);
         get_int(*pp, "associateGravityWithInertia", p.associate_gravity_with_inertia);
+        // cloth (VBDClothPhysicsParameters::fromJson, VBDCloth/VBDClothPhysicsParameters.h:83-130)
+        get_num(*pp, "stepSize", p.cloth_step_size);
+        get_num(*pp, "boundaryFrictionEpsValue", p.cloth_boundary_friction_eps);
+        get_int(*pp, "handleCollision", p.handle_collision);
+        get_num(*pp, "contactStiffness", p.contact_stiffness);
+        get_num(*pp, "contactRadius", p.contact_radius);
+        get_num(*pp, "thickness", p.thickness);
+        get_num(*pp, "conservativeStepRelaxation", p.conservative_step_relaxation);
         int32_t useBowl = 0;
         get_int(*pp, "useBowlGround", useBowl);
         if (useBowl) {
@@ -128,83 +191,83 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         return false;
     }
     for (const json::Value& mj : list->arr) {
-        std::string material = get_str(mj, "materialName");
-        if (material != "NeoHookean") {
-            err = "materialName \"" + material + "\" is not supported for tet meshes (NeoHookean only)";
-            return false;
-        }
-        TetMeshHost m;
+        const std::string material = get_str(mj, "materialName");
         const std::string path = substitute_root(get_str(mj, "path"), repoRoot);
-        if (!load_t_file(path, m.pos, m.tets, err)) return false;
-        m.nv = (int32_t)(m.pos.size() / 3);
-        m.nt = (int32_t)(m.tets.size() / 4);
-        float rot[3] = {0, 0, 0}, scale[3] = {1, 1, 1}, trans[3] = {0, 0, 0}, vel0[3] = {0, 0, 0};
-        get_vec3(mj, "rotation", rot);
-        get_vec3(mj, "scale", scale);
-        get_vec3(mj, "translation", trans);
-        get_vec3(mj, "initialVelocity", vel0);
-        const float angle = std::sqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2]);
-        float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major
-        if (angle > 0.f) {
-            const float ax = rot[0] / angle, ay = rot[1] / angle, az = rot[2] / angle;
-            const float c = std::cos(angle), s = std::sin(angle), t = 1.f - c;
-            const float Rr[9] = {t * ax * ax + c,      t * ax * ay - s * az, t * ax * az + s * ay,
-                                 t * ax * ay + s * az, t * ay * ay + c,      t * ay * az - s * ax,
-                                 t * ax * az - s * ay, t * ay * az + s * ax, t * az * az + c};
-            for (int i = 0; i < 9; ++i) R[i] = Rr[i];
-        }
-        for (int32_t i = 0; i < m.nv; ++i) {
-            float* x = &m.pos[3 * i];
-            const float p0 = x[0], p1 = x[1], p2 = x[2];
-            for (int r = 0; r < 3; ++r) x[r] = R[3 * r] * p0 + R[3 * r + 1] * p1 + R[3 * r + 2] * p2;
-            for (int r = 0; r < 3; ++r) x[r] = x[r] * scale[r] + trans[r];
-        }
-        m.vel.resize(3 * (size_t)m.nv);
-        for (int32_t i = 0; i < m.nv; ++i)
-            for (int d = 0; d < 3; ++d) m.vel[3 * i + d] = vel0[d];
-        get_num(mj, "density", m.mat.density);
-        get_num(mj, "miu", m.mat.mu);
-        get_num(mj, "lmbd", m.mat.lambda);
-        get_num(mj, "dampingHydrostatic", m.mat.dampingHydrostatic);
-        get_num(mj, "dampingDeviatoric", m.mat.dampingDeviatoric);
-        get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
-        get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
-        get_int(mj, "initializationType", m.mat.initializationType);
-        if (m.mat.initializationType < 0 || m.mat.initializationType > 4) {
-            err = "initializationType must be 0..4 in this build";
-            return false;
-        }
-        if (const json::Value* fp = mj.get("fixedPoints"))
-            for (const json::Value& f : fp->arr) m.fixed.push_back((int32_t)f.num);
         const std::string colorPath = substitute_root(get_str(mj, "verticesColoringCategoriesPath"), repoRoot);
-        if (!colorPath.empty()) {
-            json::Value cats;
-            if (!read_json(colorPath, cats, err)) return false;
-            m.colors.assign(m.nv, -1);
-            for (size_t c = 0; c < cats.arr.size(); ++c)
-                for (const json::Value& v : cats.arr[c].arr) {
-                    const int32_t id = (int32_t)v.num;
-                    if (id < 0 || id >= m.nv) {
-                        err = colorPath + ": vertex id out of range";
-                        return false;
-                    }
-                    m.colors[id] = (int32_t)c;
-                }
-            for (int32_t c : m.colors)
-                if (c < 0) {
-                    err = colorPath + ": colouring does not cover every vertex";
-                    return false;
-                }
-            Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
-            if (!coloring_valid(g, m.colors)) {
-                err = colorPath + ": colouring is not valid for " + path;
+        float vel0[3] = {0, 0, 0};
+        get_vec3(mj, "initialVelocity", vel0);
+        std::vector<int32_t> fixed;
+        if (const json::Value* fp = mj.get("fixedPoints"))
+            for (const json::Value& f : fp->arr) fixed.push_back((int32_t)f.num);
+        if (material == "NeoHookean") {
+            TetMeshHost m;
+            if (!load_t_file(path, m.pos, m.tets, err)) return false;
+            m.nv = (int32_t)(m.pos.size() / 3);
+            m.nt = (int32_t)(m.tets.size() / 4);
+            transform_positions(mj, m.pos, 0.f);
+            m.vel.resize(3 * (size_t)m.nv);
+            for (int32_t i = 0; i < m.nv; ++i)
+                for (int d = 0; d < 3; ++d) m.vel[3 * i + d] = vel0[d];
+            get_num(mj, "density", m.mat.density);
+            get_num(mj, "miu", m.mat.mu);
+            get_num(mj, "lmbd", m.mat.lambda);
+            get_num(mj, "dampingHydrostatic", m.mat.dampingHydrostatic);
+            get_num(mj, "dampingDeviatoric", m.mat.dampingDeviatoric);
+            get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
+            get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
+            get_int(mj, "initializationType", m.mat.initializationType);
+            if (m.mat.initializationType < 0 || m.mat.initializationType > 4) {
+                err = "initializationType must be 0..4 for tet meshes in this build";
                 return false;
             }
-        } else {
+            m.fixed = fixed;
             Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
-            m.colors = color_mcs(g);
+            if (!colorPath.empty()) {
+                if (!read_colors(colorPath, m.nv, m.colors, err)) return false;
+                if (!coloring_valid(g, m.colors)) {
+                    err = colorPath + ": colouring is not valid for " + path;
+                    return false;
+                }
+            } else {
+                m.colors = color_mcs(g);
+            }
+            out.order.push_back({0, (int)out.tets.size()});
+            out.tets.push_back(std::move(m));
+        } else if (material == "StVK_triMesh") {
+            ClothMeshHost m;
+            if (!load_obj_file(path, m.pos, m.faces, err)) return false;
+            m.nv = (int32_t)(m.pos.size() / 3);
+            m.nf = (int32_t)(m.faces.size() / 3);
+            transform_positions(mj, m.pos, 1e-5f);
+            m.vel.resize(3 * (size_t)m.nv);
+            for (int32_t i = 0; i < m.nv; ++i)
+                for (int d = 0; d < 3; ++d) m.vel[3 * i + d] = vel0[d];
+            get_num(mj, "density", m.mat.density);
+            get_num(mj, "miu", m.mat.mu);
+            get_num(mj, "lambda", m.mat.lambda);
+            get_num(mj, "bendingStiffness", m.mat.bendingStiffness);
+            get_num(mj, "dampingStVK", m.mat.dampingStVK);
+            get_num(mj, "dampingBending", m.mat.dampingBending);
+            get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
+            get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
+            get_int(mj, "initializationType", m.mat.initializationType);
+            const int t = m.mat.initializationType;
+            if (!(t == 0 || t == 2 || t == 3 || t == 4)) {
+                err = "cloth initializationType must be 0, 2, 3 or 4 in this build";
+                return false;
+            }
+            m.fixed = fixed;
+            if (!colorPath.empty()) {
+                if (!read_colors(colorPath, m.nv, m.colors, err)) return false;
+            } else {
+                m.colors = color_mcs(tri_vertex_graph(m.nv, m.nf, m.faces.data(), true));
+            }
+            out.order.push_back({1, (int)out.cloths.size()});
+            out.cloths.push_back(std::move(m));
+        } else {
+            err = "materialName \"" + material + "\" is not supported (NeoHookean, StVK_triMesh)";
+            return false;
         }
-        out.meshes.push_back(std::move(m));
     }
     return true;
 }

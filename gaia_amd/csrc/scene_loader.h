@@ -1,7 +1,8 @@
 // Models.json / Parameters.json scene loading (VBDPhysics::loadRunningparameters
-// + the mesh part of BasePhysicFramework::initialize).
+// / VBDClothSimulationFramework parameter parsing + the mesh part of initialize).
 #pragma once
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gaia_vbd.h"
@@ -11,18 +12,21 @@ namespace gvbd {
 
 struct LoadedScene {
     gvbd_physics_params params{};
-    std::vector<TetMeshHost> meshes;
+    std::vector<TetMeshHost> tets;
+    std::vector<ClothMeshHost> cloths;
+    std::vector<std::pair<int, int>> order;  // (0 tet / 1 cloth, index) in Models order
 };
 
 // Reference defaults of the hot-path parameters (VBD/VBDPhysicsParameters.h:8-81,
-// Parameters/PhysicsParameters.h:18-120).
+// Parameters/PhysicsParameters.h:18-120, VBDCloth/VBDClothPhysicsParameters.h:15-81).
 gvbd_physics_params default_physics_params();
 
 // Parses both files. Paths containing "${REPO_ROOT}" get repoRoot substituted
-// (Parser/InputHandler.h:128-184). Meshes are loaded from ".t" files and
-// transformed by rotation (axis-angle), scale and translation
-// (TetMesh/TetMeshFEM.cpp:358-375); vertex colours come from
-// verticesColoringCategoriesPath when given, otherwise from MCS colouring.
+// (Parser/InputHandler.h:128-184). "NeoHookean" models load ".t" files, "StVK_triMesh"
+// models load ".obj" files; both are transformed by rotation (axis-angle), scale and
+// translation (TetMesh/TetMeshFEM.cpp:358-375, TriMesh/TriMesh.cpp:115-160). Vertex colours
+// come from verticesColoringCategoriesPath when given, otherwise from MCS colouring (of the
+// vertex graph with bending edges for cloth).
 bool load_scene_json(const std::string& modelsPath, const std::string& paramsPath, const std::string& repoRoot,
                      LoadedScene& out, std::string& err);
 

@@ -33,31 +33,48 @@ void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) fail(GVBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void require(bool cond, const char* msg) {
+    if (!cond) fail(GVBD_ERR_INVALID, msg);
+}
+
 constexpr int kVertexChunk = 256;
 
 // All device arrays of one scene live in one allocation.
 struct DeviceScene {
     void* arena = nullptr;
     size_t bytes = 0;
-    float4 *pos = nullptr, *prev = nullptr, *inertia = nullptr, *vel = nullptr, *vparam = nullptr;
+    float4 *pos = nullptr, *prev = nullptr, *inertia = nullptr, *vel = nullptr, *vparam = nullptr, *pnext = nullptr;
     uint8_t* vflag = nullptr;
     int4* slots = nullptr;
     TetInc* tinc = nullptr;
+    ClothSlot* cslots = nullptr;
+    FaceInc* cface = nullptr;
+    BendInc* cbend = nullptr;
 };
+
+enum MeshKind { MESH_TET = 0, MESH_CLOTH = 1 };
 
 struct SceneObj {
     gvbd_physics_params params{};
     SolveParams sp{};
-    std::vector<TetMeshHost> meshes;
-    std::vector<int32_t> vertexOffset;  // per mesh, into the scene's vertex space
+    std::vector<TetMeshHost> tets;
+    std::vector<ClothMeshHost> cloths;
+    std::vector<std::pair<int, int>> meshes;  // (kind, index into tets / cloths), in addition order
+    std::vector<int32_t> vertexOffset;        // per mesh, into the scene's vertex space
     int32_t nv = 0;
     bool initialized = false;
     std::vector<std::vector<int32_t>> groups;  // interleaved (mesh, vertex)
-    std::vector<int32_t> groupSlotOffset;      // G + 1
-    std::vector<int32_t> slotIncCount;         // incidence count per slot (work model)
+    std::vector<int32_t> tetGroupOffset;       // G + 1, into tet slots
+    std::vector<int32_t> clothGroupOffset;     // G + 1, into cloth slots
+    std::vector<int64_t> slotWorkBytes;        // algorithmic bytes per tet slot then per cloth slot
     DeviceScene dev;
     SceneView view{};
     int32_t plan = -1;  // the scene's own single-scene plan
+
+    int32_t meshVertices(int32_t m) const {
+        return meshes[m].first == MESH_TET ? tets[meshes[m].second].nv : cloths[meshes[m].second].nv;
+    }
+    size_t numTetSlots() const { return tetGroupOffset.empty() ? 0 : (size_t)tetGroupOffset.back(); }
 };
 
 struct PlanObj {
@@ -65,9 +82,11 @@ struct PlanObj {
     int lpv = 4, block = 256;
     int numGroups = 0, iterations = 0, substeps = 0;
     SceneView* dViews = nullptr;
-    WorkItem* dItems = nullptr;
-    WorkItem* dVItems = nullptr;
-    std::vector<int32_t> groupItemBegin;  // G + 1
+    WorkItem* dItems = nullptr;   // per group: solve items (tet and cloth)
+    WorkItem* dCommit = nullptr;  // per group: cloth commit items
+    WorkItem* dVItems = nullptr;  // per-vertex items
+    std::vector<int32_t> groupItemBegin;    // G + 1
+    std::vector<int32_t> groupCommitBegin;  // G + 1
     int32_t nVItems = 0;
     size_t bytes = 0;
     bool wantGraph = false;
@@ -111,7 +130,7 @@ PlanObj& batch_plan(gvbd_context* ctx, int32_t batch) {
     return plan_of(ctx, it->second);
 }
 
-SolveParams make_solve_params(const gvbd_physics_params& p, const std::vector<TetMeshHost>& meshes) {
+SolveParams make_solve_params(const gvbd_physics_params& p, const SceneObj& s) {
     SolveParams sp{};
     if (p.num_substeps <= 0) fail(GVBD_ERR_INVALID, "num_substeps must be positive");
     sp.dt = p.time_step / (float)p.num_substeps;
@@ -129,7 +148,11 @@ SolveParams make_solve_params(const gvbd_physics_params& p, const std::vector<Te
     sp.usePlaneGround = p.use_plane_ground;
     sp.associateGravity = p.associate_gravity_with_inertia;
     sp.frictionStartIter = p.friction_start_iter;
-    sp.initRatioG = meshes.empty() ? 0.5f : meshes[0].mat.initRatioG;
+    sp.initRatioG = s.tets.empty() ? 0.5f : s.tets[0].mat.initRatioG;
+    sp.clothBoundaryStiffness = p.cloth_boundary_contact_stiffness;
+    sp.clothFrictionEpsU = p.cloth_boundary_friction_eps * sp.dt;
+    sp.degenerateThresSq = p.degenerate_triangle_thres * p.degenerate_triangle_thres;
+    sp.clothStepSize = p.cloth_step_size;
     return sp;
 }
 
@@ -142,86 +165,189 @@ void free_plan_graph(PlanObj& p) {
 
 void free_plan(PlanObj& p) {
     free_plan_graph(p);
-    if (p.dViews) hipFree(p.dViews);
-    if (p.dItems) hipFree(p.dItems);
-    if (p.dVItems) hipFree(p.dVItems);
+    for (void* ptr : {(void*)p.dViews, (void*)p.dItems, (void*)p.dCommit, (void*)p.dVItems})
+        if (ptr) (void)hipFree(ptr);
     p.dViews = nullptr;
     p.dItems = nullptr;
+    p.dCommit = nullptr;
     p.dVItems = nullptr;
     p.bytes = 0;
+    p.dirty = true;
 }
 
-// Builds the device arrays of an initialised scene (colour-major slots and incidence records).
+// Colour-major tet slot and incidence records of one vertex.
+void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<int4>& slots,
+                   std::vector<TetInc>& tinc, std::vector<int64_t>& work) {
+    int4 slot;
+    slot.x = off + lv;
+    slot.y = (int32_t)tinc.size();
+    for (int32_t k = m.incBegin[lv]; k < m.incBegin[lv + 1]; ++k) {
+        const int32_t t = m.incTet[k];
+        const int32_t ord = m.incOrder[k];
+        const float* D = &m.dminv[9 * (size_t)t];
+        // shape-matrix row j of tet vertex j: j = 0 -> minus column sums, j >= 1 -> row j-1 of DmInv
+        float B[4][3];
+        for (int c = 0; c < 3; ++c) {
+            B[0][c] = -D[0 + 3 * c] - D[1 + 3 * c] - D[2 + 3 * c];
+            for (int j = 1; j < 4; ++j) B[j][c] = D[(j - 1) + 3 * c];
+        }
+        TetInc rec{};
+        int n = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (j == ord) continue;
+            rec.nb[n] = off + m.tets[4 * (size_t)t + j];
+            for (int c = 0; c < 3; ++c) rec.m[3 * n + c] = B[j][c];
+            ++n;
+        }
+        rec.Amu = m.vol[t] * m.mat.mu;
+        rec.Alm = m.vol[t] * m.mat.lambda;
+        rec.a = 1.f + m.mat.mu / m.mat.lambda;
+        rec.pad = 0.f;
+        tinc.push_back(rec);
+    }
+    slot.z = (int32_t)tinc.size();
+    const bool damping = m.mat.dampingHydrostatic != 0.f || m.mat.dampingDeviatoric != 0.f;
+    slot.w = damping ? SLOT_DAMPING : 0;
+    slots.push_back(slot);
+    // gather model: 60 B per vertex + 96 B per incident tet
+    work.push_back(60 + 96 * (int64_t)(slot.z - slot.y));
+}
+
+// Colour-major cloth slot, face and bending records of one vertex.
+void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, std::vector<ClothSlot>& slots,
+                     std::vector<FaceInc>& faces, std::vector<BendInc>& bends, std::vector<int64_t>& work) {
+    ClothSlot cs{};
+    cs.a.x = off + lv;
+    cs.a.y = (int32_t)faces.size();
+    for (int32_t k = m.vfBegin[lv]; k < m.vfBegin[lv + 1]; ++k) {
+        const int32_t f = m.vfFace[k], o = m.vfOrder[k];
+        const float* D = &m.dminv[4 * (size_t)f];  // (0,0) (1,0) (0,1) (1,1)
+        // shape row of face vertex j: j = 0 -> minus the sum of DmInv rows, j >= 1 -> row j-1
+        float B[3][2];
+        for (int c = 0; c < 2; ++c) {
+            B[0][c] = -D[0 + 2 * c] - D[1 + 2 * c];
+            B[1][c] = D[0 + 2 * c];
+            B[2][c] = D[1 + 2 * c];
+        }
+        FaceInc rec{};
+        int n = 0;
+        for (int j = 0; j < 3; ++j) {
+            if (j == o) continue;
+            rec.nb[n] = off + m.faces[3 * (size_t)f + j];
+            rec.m[2 * n] = B[j][0];
+            rec.m[2 * n + 1] = B[j][1];
+            ++n;
+        }
+        rec.Amu = m.area[f] * m.mat.mu;
+        rec.Alm = m.area[f] * m.mat.lambda;
+        faces.push_back(rec);
+    }
+    cs.a.z = (int32_t)faces.size();
+    cs.a.w = 0;
+    cs.b.x = (int32_t)bends.size();
+    const float kb = m.mat.bendingStiffness;
+    for (int32_t k = m.vbBegin[lv]; k < m.vbBegin[lv + 1]; ++k) {
+        const int32_t id = m.vbEdge[k], o = m.vbOrder[k];
+        const TriEdge& e = m.edges[id];
+        const int32_t roles[4] = {e.v1, e.v2, e.v12Next, e.v21Next};
+        const float* Q = &m.Q[16 * (size_t)id];
+        BendInc rec{};
+        int n = 0;
+        for (int j = 0; j < 4; ++j) {
+            if (j == o) continue;
+            rec.nb[n] = off + roles[j];
+            rec.q[n] = kb * Q[4 * o + j];
+            ++n;
+        }
+        rec.order = o;
+        rec.qself = kb * Q[4 * o + o];
+        bends.push_back(rec);
+    }
+    cs.b.y = (int32_t)bends.size();
+    slots.push_back(cs);
+    // gather model: 60 B per vertex + 56 B per face + 116 B per bending stencil
+    work.push_back(60 + 56 * (int64_t)(cs.a.z - cs.a.y) + 116 * (int64_t)(cs.b.y - cs.b.x));
+}
+
+// Builds the device arrays of an initialised scene.
 void upload_scene(gvbd_context* ctx, SceneObj& s) {
     const int32_t G = (int32_t)s.groups.size();
-    // slots and incidence in colour-major order; fixed vertices are not solved (VBDPhysics.cpp:1617)
     std::vector<int4> slots;
     std::vector<TetInc> tinc;
-    s.groupSlotOffset.assign(G + 1, 0);
-    s.slotIncCount.clear();
+    std::vector<ClothSlot> cslots;
+    std::vector<FaceInc> cface;
+    std::vector<BendInc> cbend;
+    std::vector<int64_t> tetWork, clothWork;
+    s.tetGroupOffset.assign(G + 1, 0);
+    s.clothGroupOffset.assign(G + 1, 0);
+    // fixed vertices are not solved (VBDPhysics.cpp:1617, VBDClothPhysics.cpp:470)
+    std::vector<std::vector<uint8_t>> fixedMask(s.meshes.size());
+    for (size_t im = 0; im < s.meshes.size(); ++im) {
+        fixedMask[im].assign(s.meshVertices((int32_t)im), 0);
+        const auto& fx = s.meshes[im].first == MESH_TET ? s.tets[s.meshes[im].second].fixed
+                                                        : s.cloths[s.meshes[im].second].fixed;
+        for (int32_t f : fx) fixedMask[im][f] = 1;
+    }
     for (int32_t g = 0; g < G; ++g) {
         const auto& grp = s.groups[g];
         for (size_t i = 0; i + 1 < grp.size(); i += 2) {
-            const TetMeshHost& m = s.meshes[grp[i]];
-            const int32_t lv = grp[i + 1];
-            bool fixed = false;
-            for (int32_t f : m.fixed) fixed = fixed || f == lv;
-            if (fixed) continue;
-            const int32_t off = s.vertexOffset[grp[i]];
-            int4 slot;
-            slot.x = off + lv;
-            slot.y = (int32_t)tinc.size();
-            const bool damping = m.mat.dampingHydrostatic != 0.f || m.mat.dampingDeviatoric != 0.f;
-            for (int32_t k = m.incBegin[lv]; k < m.incBegin[lv + 1]; ++k) {
-                const int32_t t = m.incTet[k];
-                const int32_t ord = m.incOrder[k];
-                const float* D = &m.dminv[9 * (size_t)t];
-                // shape-matrix row j of tet vertex j: j = 0 -> minus column sums, j >= 1 -> row j-1 of DmInv
-                float B[4][3];
-                for (int c = 0; c < 3; ++c) {
-                    B[0][c] = -D[0 + 3 * c] - D[1 + 3 * c] - D[2 + 3 * c];
-                    for (int j = 1; j < 4; ++j) B[j][c] = D[(j - 1) + 3 * c];
-                }
-                TetInc rec{};
-                int n = 0;
-                for (int j = 0; j < 4; ++j) {
-                    if (j == ord) continue;
-                    rec.nb[n] = off + m.tets[4 * (size_t)t + j];
-                    for (int c = 0; c < 3; ++c) rec.m[3 * n + c] = B[j][c];
-                    ++n;
-                }
-                rec.Amu = m.vol[t] * m.mat.mu;
-                rec.Alm = m.vol[t] * m.mat.lambda;
-                rec.a = 1.f + m.mat.mu / m.mat.lambda;
-                rec.pad = 0.f;
-                tinc.push_back(rec);
-            }
-            slot.z = (int32_t)tinc.size();
-            slot.w = damping ? SLOT_DAMPING : 0;
-            slots.push_back(slot);
-            s.slotIncCount.push_back(slot.z - slot.y);
+            const int32_t im = grp[i], lv = grp[i + 1];
+            if (fixedMask[im][lv]) continue;
+            const int32_t off = s.vertexOffset[im];
+            if (s.meshes[im].first == MESH_TET)
+                emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork);
+            else
+                emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, cslots, cface, cbend, clothWork);
         }
-        s.groupSlotOffset[g + 1] = (int32_t)slots.size();
+        s.tetGroupOffset[g + 1] = (int32_t)slots.size();
+        s.clothGroupOffset[g + 1] = (int32_t)cslots.size();
     }
+    s.slotWorkBytes = tetWork;
+    s.slotWorkBytes.insert(s.slotWorkBytes.end(), clothWork.begin(), clothWork.end());
     // per-vertex arrays
     const int32_t nv = s.nv;
-    std::vector<float4> pos(nv), prev(nv), inertia(nv), vel(nv), vparam(nv);
+    std::vector<float4> pos(nv), inertia(nv), vel(nv), vparam(nv);
     std::vector<uint8_t> vflag(nv, 0);
     for (size_t im = 0; im < s.meshes.size(); ++im) {
-        const TetMeshHost& m = s.meshes[im];
         const int32_t off = s.vertexOffset[im];
-        const float expD = std::pow(m.mat.exponentialVelDamping, s.sp.dt);
-        const float constD = m.mat.constantVelDamping * s.sp.dt;
-        for (int32_t i = 0; i < m.nv; ++i) {
-            const float4 p = make_float4(m.pos[3 * i], m.pos[3 * i + 1], m.pos[3 * i + 2], 0.f);
-            pos[off + i] = p;
-            prev[off + i] = p;
-            inertia[off + i] = make_float4(p.x, p.y, p.z, m.mass[i]);
-            vel[off + i] = make_float4(m.vel[3 * i], m.vel[3 * i + 1], m.vel[3 * i + 2], 0.f);
-            vparam[off + i] = make_float4(expD, constD, m.mat.dampingHydrostatic, m.mat.dampingDeviatoric);
-            vflag[off + i] = (uint8_t)((m.mat.initializationType & 7) << 1);
+        const std::vector<float>* mp;
+        const std::vector<float>* mv;
+        const std::vector<float>* mm;
+        float4 vp;
+        uint8_t code;
+        if (s.meshes[im].first == MESH_TET) {
+            const TetMeshHost& m = s.tets[s.meshes[im].second];
+            mp = &m.pos;
+            mv = &m.vel;
+            mm = &m.mass;
+            // step-invariant damping (VBDPhysics.cpp:794-807)
+            vp = make_float4(std::pow(m.mat.exponentialVelDamping, s.sp.dt), m.mat.constantVelDamping * s.sp.dt,
+                             m.mat.dampingHydrostatic, m.mat.dampingDeviatoric);
+            code = (uint8_t)(m.mat.initializationType & 7);
+        } else {
+            const ClothMeshHost& m = s.cloths[s.meshes[im].second];
+            mp = &m.pos;
+            mv = &m.vel;
+            mm = &m.mass;
+            vp = make_float4(m.mat.exponentialVelDamping, m.mat.constantVelDamping, m.mat.dampingStVK,
+                             m.mat.dampingBending);
+            // cloth initialisation types (VBDBaseTriMesh.cpp:72-153) as per-vertex codes:
+            // 0 inertia -> x = y; 2 previous position, 4 none -> keep x;
+            // 3 inertia within conservative bounds -> x = y (bounds are infinite without contact)
+            const int t = m.mat.initializationType;
+            code = (uint8_t)((t == 0 || t == 3) ? 0 : 1);
         }
-        for (int32_t f : m.fixed) vflag[off + f] |= 1;
+        const int32_t n = s.meshVertices((int32_t)im);
+        for (int32_t i = 0; i < n; ++i) {
+            const float4 p = make_float4((*mp)[3 * i], (*mp)[3 * i + 1], (*mp)[3 * i + 2], 0.f);
+            pos[off + i] = p;
+            inertia[off + i] = make_float4(p.x, p.y, p.z, (*mm)[i]);
+            vel[off + i] = make_float4((*mv)[3 * i], (*mv)[3 * i + 1], (*mv)[3 * i + 2], 0.f);
+            vparam[off + i] = vp;
+            vflag[off + i] = (uint8_t)(code << 1);
+        }
+        for (int32_t i = 0; i < n; ++i)
+            if (fixedMask[im][i]) vflag[off + i] |= 1;
     }
     // one arena
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -229,7 +355,10 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     const size_t bF = align((size_t)std::max(nv, 1));
     const size_t bS = align(sizeof(int4) * std::max<size_t>(slots.size(), 1));
     const size_t bI = align(sizeof(TetInc) * std::max<size_t>(tinc.size(), 1));
-    const size_t total = 5 * bV + bF + bS + bI;
+    const size_t bCS = align(sizeof(ClothSlot) * std::max<size_t>(cslots.size(), 1));
+    const size_t bCF = align(sizeof(FaceInc) * std::max<size_t>(cface.size(), 1));
+    const size_t bCB = align(sizeof(BendInc) * std::max<size_t>(cbend.size(), 1));
+    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB;
     DeviceScene& d = s.dev;
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (hipMalloc(&d.arena, total) != hipSuccess) {
@@ -239,36 +368,55 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     }
     d.bytes = total;
     char* base = (char*)d.arena;
-    d.pos = (float4*)base;
-    d.prev = (float4*)(base + bV);
-    d.inertia = (float4*)(base + 2 * bV);
-    d.vel = (float4*)(base + 3 * bV);
-    d.vparam = (float4*)(base + 4 * bV);
-    d.vflag = (uint8_t*)(base + 5 * bV);
-    d.slots = (int4*)(base + 5 * bV + bF);
-    d.tinc = (TetInc*)(base + 5 * bV + bF + bS);
+    size_t at = 0;
+    auto take = [&](size_t n) {
+        char* p = base + at;
+        at += n;
+        return p;
+    };
+    d.pos = (float4*)take(bV);
+    d.prev = (float4*)take(bV);
+    d.inertia = (float4*)take(bV);
+    d.vel = (float4*)take(bV);
+    d.vparam = (float4*)take(bV);
+    d.pnext = (float4*)take(bV);
+    d.vflag = (uint8_t*)take(bF);
+    d.slots = (int4*)take(bS);
+    d.tinc = (TetInc*)take(bI);
+    d.cslots = (ClothSlot*)take(bCS);
+    d.cface = (FaceInc*)take(bCF);
+    d.cbend = (BendInc*)take(bCB);
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "upload");
     };
     up(d.pos, pos.data(), sizeof(float4) * nv);
-    up(d.prev, prev.data(), sizeof(float4) * nv);
+    up(d.prev, pos.data(), sizeof(float4) * nv);
+    up(d.pnext, pos.data(), sizeof(float4) * nv);
     up(d.inertia, inertia.data(), sizeof(float4) * nv);
     up(d.vel, vel.data(), sizeof(float4) * nv);
     up(d.vparam, vparam.data(), sizeof(float4) * nv);
     up(d.vflag, vflag.data(), nv);
     up(d.slots, slots.data(), sizeof(int4) * slots.size());
     up(d.tinc, tinc.data(), sizeof(TetInc) * tinc.size());
+    up(d.cslots, cslots.data(), sizeof(ClothSlot) * cslots.size());
+    up(d.cface, cface.data(), sizeof(FaceInc) * cface.size());
+    up(d.cbend, cbend.data(), sizeof(BendInc) * cbend.size());
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
-    s.view.pos = d.pos;
-    s.view.prev = d.prev;
-    s.view.inertia = d.inertia;
-    s.view.vel = d.vel;
-    s.view.vparam = d.vparam;
-    s.view.vflag = d.vflag;
-    s.view.slots = d.slots;
-    s.view.tinc = d.tinc;
-    s.view.nv = nv;
-    s.view.sp = s.sp;
+    SceneView& v = s.view;
+    v.pos = d.pos;
+    v.prev = d.prev;
+    v.inertia = d.inertia;
+    v.vel = d.vel;
+    v.vparam = d.vparam;
+    v.vflag = d.vflag;
+    v.slots = d.slots;
+    v.tinc = d.tinc;
+    v.pnext = d.pnext;
+    v.cslots = d.cslots;
+    v.cface = d.cface;
+    v.cbend = d.cbend;
+    v.nv = nv;
+    v.sp = s.sp;
 }
 
 // (Re)builds the work items of a plan for its current schedule.
@@ -283,16 +431,22 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
         views.push_back(s.view);
         p.numGroups = std::max(p.numGroups, (int)s.groups.size());
     }
-    std::vector<WorkItem> items, vitems;
+    std::vector<WorkItem> items, commits, vitems;
     p.groupItemBegin.assign(p.numGroups + 1, 0);
+    p.groupCommitBegin.assign(p.numGroups + 1, 0);
     for (int g = 0; g < p.numGroups; ++g) {
         for (size_t si = 0; si < p.scenes.size(); ++si) {
             SceneObj& s = scene_of(ctx, p.scenes[si]);
             if (g >= (int)s.groups.size()) continue;
-            const int32_t b = s.groupSlotOffset[g], e = s.groupSlotOffset[g + 1];
-            for (int32_t k = b; k < e; k += vpb) items.push_back({(int32_t)si, k, std::min(vpb, e - k), 0});
+            for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += vpb)
+                items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_TET});
+            for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += vpb)
+                items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
+            for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += kVertexChunk)
+                commits.push_back({(int32_t)si, k, std::min(kVertexChunk, e - k), WORK_CLOTH});
         }
         p.groupItemBegin[g + 1] = (int32_t)items.size();
+        p.groupCommitBegin[g + 1] = (int32_t)commits.size();
     }
     for (size_t si = 0; si < p.scenes.size(); ++si) {
         SceneObj& s = scene_of(ctx, p.scenes[si]);
@@ -311,27 +465,30 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     };
     alloc_up((void**)&p.dViews, views.data(), sizeof(SceneView) * views.size());
     alloc_up((void**)&p.dItems, items.data(), sizeof(WorkItem) * items.size());
+    alloc_up((void**)&p.dCommit, commits.data(), sizeof(WorkItem) * commits.size());
     alloc_up((void**)&p.dVItems, vitems.data(), sizeof(WorkItem) * vitems.size());
     hip_check(hipStreamSynchronize(ctx->stream), "plan upload sync");
     p.dirty = false;
 }
 
 void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
-    hip_check(launch_tet_initial_step(p.dViews, p.dVItems, p.nVItems, ctx->stream), "initial step launch");
+    hip_check(launch_initial_step(p.dViews, p.dVItems, p.nVItems, ctx->stream), "initial step launch");
 }
 
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
     const int b = p.groupItemBegin[g], e = p.groupItemBegin[g + 1];
-    hip_check(launch_tet_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, ctx->stream),
+    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, ctx->stream),
               "group solve launch");
+    const int cb = p.groupCommitBegin[g], ce = p.groupCommitBegin[g + 1];
+    if (ce > cb) hip_check(launch_cloth_commit(p.dViews, p.dCommit + cb, ce - cb, ctx->stream), "commit launch");
 }
 
 void plan_velocity(gvbd_context* ctx, PlanObj& p) {
-    hip_check(launch_tet_update_velocity(p.dViews, p.dVItems, p.nVItems, ctx->stream), "velocity launch");
+    hip_check(launch_update_velocity(p.dViews, p.dVItems, p.nVItems, ctx->stream), "velocity launch");
 }
 
 // One frame: substeps x (initial step, iterations x groups, velocity update)
-// (VBDPhysics.cpp:839-969 structure, all on the device).
+// (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
     for (int sub = 0; sub < p.substeps; ++sub) {
         plan_initial_step(ctx, p);
@@ -387,10 +544,21 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
 void initialize_scene(gvbd_context* ctx, SceneObj& s) {
     if (s.initialized) fail(GVBD_ERR_STATE, "scene already initialised");
     if (s.meshes.empty()) fail(GVBD_ERR_STATE, "scene has no meshes");
-    s.sp = make_solve_params(s.params, s.meshes);
+    if (!s.cloths.empty() && s.params.handle_collision)
+        fail(GVBD_ERR_INVALID, "handle_collision (cloth self-contact) is not supported by this build; set it to 0");
+    s.sp = make_solve_params(s.params, s);
     std::vector<const std::vector<std::vector<int32_t>>*> cats;
-    for (auto& m : s.meshes) cats.push_back(&m.categories);
-    s.groups = merge_parallel_groups(cats, true);
+    std::vector<bool> removeUsed;
+    for (auto& mk : s.meshes) {
+        if (mk.first == MESH_TET) {
+            cats.push_back(&s.tets[mk.second].categories);
+            removeUsed.push_back(true);
+        } else {
+            cats.push_back(&s.cloths[mk.second].categories);
+            removeUsed.push_back(false);
+        }
+    }
+    s.groups = merge_parallel_groups(cats, removeUsed);
     upload_scene(ctx, s);
     s.initialized = true;
 }
@@ -414,25 +582,37 @@ template <class F> int guard(gvbd_context* ctx, F&& f) {
     }
 }
 
-void require(bool cond, const char* msg) {
-    if (!cond) fail(GVBD_ERR_INVALID, msg);
+SceneObj& uninitialized_scene(gvbd_context* ctx, int32_t scene) {
+    SceneObj& s = scene_of(ctx, scene);
+    if (s.initialized) fail(GVBD_ERR_STATE, "cannot add meshes after initialisation");
+    return s;
 }
 
 }  // namespace
 
-// Used by the JSON scene loader.
 int32_t add_tet_mesh_host(gvbd_context* ctx, int32_t scene, TetMeshHost&& m) {
-    SceneObj& s = scene_of(ctx, scene);
-    if (s.initialized) fail(GVBD_ERR_STATE, "cannot add meshes after initialisation");
+    SceneObj& s = uninitialized_scene(ctx, scene);
     std::string err;
     if (!prepare_tet_mesh(m, err)) fail(GVBD_ERR_INVALID, err);
-    if (m.colors.empty()) {
-        fail(GVBD_ERR_INTERNAL, "mesh colours missing");
-    }
+    if ((int32_t)m.colors.size() != m.nv) fail(GVBD_ERR_INTERNAL, "mesh colours missing");
     m.categories = sorted_categories(m.colors);
     s.vertexOffset.push_back(s.nv);
     s.nv += m.nv;
-    s.meshes.push_back(std::move(m));
+    s.tets.push_back(std::move(m));
+    s.meshes.push_back({MESH_TET, (int)s.tets.size() - 1});
+    return (int32_t)s.meshes.size() - 1;
+}
+
+int32_t add_cloth_mesh_host(gvbd_context* ctx, int32_t scene, ClothMeshHost&& m) {
+    SceneObj& s = uninitialized_scene(ctx, scene);
+    std::string err;
+    if (!prepare_cloth_mesh(m, err)) fail(GVBD_ERR_INVALID, err);
+    if ((int32_t)m.colors.size() != m.nv) fail(GVBD_ERR_INTERNAL, "mesh colours missing");
+    m.categories = sorted_categories(m.colors);
+    s.vertexOffset.push_back(s.nv);
+    s.nv += m.nv;
+    s.cloths.push_back(std::move(m));
+    s.meshes.push_back({MESH_CLOTH, (int)s.cloths.size() - 1});
     return (int32_t)s.meshes.size() - 1;
 }
 
@@ -442,9 +622,11 @@ using namespace gvbd;
 
 extern "C" {
 
-int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes) {
+int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
+                          int32_t* cloth_mesh_desc_bytes) {
     if (physics_params_bytes) *physics_params_bytes = (int32_t)sizeof(gvbd_physics_params);
     if (tet_mesh_desc_bytes) *tet_mesh_desc_bytes = (int32_t)sizeof(gvbd_tet_mesh_desc);
+    if (cloth_mesh_desc_bytes) *cloth_mesh_desc_bytes = (int32_t)sizeof(gvbd_cloth_mesh_desc);
     return GVBD_OK;
 }
 
@@ -477,9 +659,7 @@ int gvbd_context_destroy(gvbd_context* ctx) {
     return GVBD_OK;
 }
 
-const char* gvbd_last_error(const gvbd_context* ctx) {
-    return ctx ? ctx->lastError.c_str() : g_lastError.c_str();
-}
+const char* gvbd_last_error(const gvbd_context* ctx) { return ctx ? ctx->lastError.c_str() : g_lastError.c_str(); }
 
 int gvbd_synchronize(gvbd_context* ctx) {
     return guard(ctx, [&] {
@@ -559,16 +739,59 @@ int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mes
         m.mat.initRatioG = desc->init_ratio_g;
         m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
         for (int32_t t : m.tets) require(t >= 0 && t < m.nv, "tet vertex index out of range");
+        Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
         if (desc->vertex_colors) {
             m.colors.assign(desc->vertex_colors, desc->vertex_colors + m.nv);
             for (int32_t c : m.colors) require(c >= 0, "negative vertex colour");
-            Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
             require(coloring_valid(g, m.colors), "vertex colouring is not valid for this mesh");
         } else {
-            Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
             m.colors = desc->coloring_algorithm == GVBD_COLOR_GREEDY ? color_greedy(g) : color_mcs(g);
         }
         const int32_t id = add_tet_mesh_host(ctx, scene, std::move(m));
+        if (mesh_out) *mesh_out = id;
+    });
+}
+
+int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth_mesh_desc* desc,
+                              int32_t* mesh_out) {
+    return guard(ctx, [&] {
+        require(ctx && desc, "NULL argument");
+        require(desc->num_vertices > 0 && desc->num_faces > 0, "empty cloth mesh");
+        require(desc->positions && desc->faces, "positions/faces are NULL");
+        require(desc->num_fixed == 0 || desc->fixed_points, "fixed_points is NULL");
+        const int t = desc->initialization_type;
+        require(t == 0 || t == 2 || t == 3 || t == 4, "cloth initialization_type must be 0, 2, 3 or 4");
+        ClothMeshHost m;
+        m.nv = desc->num_vertices;
+        m.nf = desc->num_faces;
+        m.pos.assign(desc->positions, desc->positions + 3 * (size_t)m.nv);
+        if (desc->velocities) {
+            m.vel.assign(desc->velocities, desc->velocities + 3 * (size_t)m.nv);
+        } else {
+            m.vel.resize(3 * (size_t)m.nv);
+            for (int32_t i = 0; i < m.nv; ++i)
+                for (int d = 0; d < 3; ++d) m.vel[3 * i + d] = desc->initial_velocity[d];
+        }
+        m.faces.assign(desc->faces, desc->faces + 3 * (size_t)m.nf);
+        m.mat.density = desc->density;
+        m.mat.mu = desc->mu;
+        m.mat.lambda = desc->lambda;
+        m.mat.bendingStiffness = desc->bending_stiffness;
+        m.mat.dampingStVK = desc->damping_stvk;
+        m.mat.dampingBending = desc->damping_bending;
+        m.mat.exponentialVelDamping = desc->exponential_vel_damping;
+        m.mat.constantVelDamping = desc->constant_vel_damping;
+        m.mat.initializationType = t;
+        m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
+        for (int32_t v : m.faces) require(v >= 0 && v < m.nv, "face vertex index out of range");
+        if (desc->vertex_colors) {
+            m.colors.assign(desc->vertex_colors, desc->vertex_colors + m.nv);
+            for (int32_t c : m.colors) require(c >= 0, "negative vertex colour");
+        } else {
+            Graph g = tri_vertex_graph(m.nv, m.nf, m.faces.data(), true);
+            m.colors = desc->coloring_algorithm == GVBD_COLOR_GREEDY ? color_greedy(g) : color_mcs(g);
+        }
+        const int32_t id = add_cloth_mesh_host(ctx, scene, std::move(m));
         if (mesh_out) *mesh_out = id;
     });
 }
@@ -593,7 +816,12 @@ int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char*
         int rc = gvbd_scene_create(ctx, &ls.params, &sid);
         if (rc != GVBD_OK) fail(rc, ctx->lastError);
         try {
-            for (auto& m : ls.meshes) add_tet_mesh_host(ctx, sid, std::move(m));
+            for (auto& mk : ls.order) {
+                if (mk.first == MESH_TET)
+                    add_tet_mesh_host(ctx, sid, std::move(ls.tets[mk.second]));
+                else
+                    add_cloth_mesh_host(ctx, sid, std::move(ls.cloths[mk.second]));
+            }
             SceneObj& s = scene_of(ctx, sid);
             initialize_scene(ctx, s);
             s.plan = new_plan(ctx, {sid});
@@ -612,13 +840,15 @@ int gvbd_scene_num_meshes(gvbd_context* ctx, int32_t scene, int32_t* n) {
     });
 }
 
-int gvbd_scene_mesh_info(gvbd_context* ctx, int32_t scene, int32_t mesh, int32_t* num_vertices, int32_t* num_tets) {
+int gvbd_scene_mesh_info(gvbd_context* ctx, int32_t scene, int32_t mesh, int32_t* num_vertices, int32_t* num_elements) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
         SceneObj& s = scene_of(ctx, scene);
         require(mesh >= 0 && mesh < (int32_t)s.meshes.size(), "mesh index out of range");
-        if (num_vertices) *num_vertices = s.meshes[mesh].nv;
-        if (num_tets) *num_tets = s.meshes[mesh].nt;
+        if (num_vertices) *num_vertices = s.meshVertices(mesh);
+        if (num_elements)
+            *num_elements = s.meshes[mesh].first == MESH_TET ? s.tets[s.meshes[mesh].second].nt
+                                                            : s.cloths[s.meshes[mesh].second].nf;
     });
 }
 
@@ -649,10 +879,21 @@ int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, fl
         require(ctx != nullptr, "ctx is NULL");
         SceneObj& s = scene_of(ctx, scene);
         require(mesh >= 0 && mesh < (int32_t)s.meshes.size(), "mesh index out of range");
-        const TetMeshHost& m = s.meshes[mesh];
-        if (dminv) std::memcpy(dminv, m.dminv.data(), m.dminv.size() * sizeof(float));
-        if (volume) std::memcpy(volume, m.vol.data(), m.vol.size() * sizeof(float));
-        if (mass) std::memcpy(mass, m.mass.data(), m.mass.size() * sizeof(float));
+        const std::vector<float>*D, *V, *M;
+        if (s.meshes[mesh].first == MESH_TET) {
+            const TetMeshHost& m = s.tets[s.meshes[mesh].second];
+            D = &m.dminv;
+            V = &m.vol;
+            M = &m.mass;
+        } else {
+            const ClothMeshHost& m = s.cloths[s.meshes[mesh].second];
+            D = &m.dminv;
+            V = &m.area;
+            M = &m.mass;
+        }
+        if (dminv) std::memcpy(dminv, D->data(), D->size() * sizeof(float));
+        if (volume) std::memcpy(volume, V->data(), V->size() * sizeof(float));
+        if (mass) std::memcpy(mass, M->data(), M->size() * sizeof(float));
     });
 }
 
@@ -665,8 +906,7 @@ int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_
         PlanObj& p = plan_of(ctx, s.plan);
         p.lpv = lanes_per_vertex;
         p.block = block;
-        p.dirty = true;
-        free_plan_graph(p);
+        free_plan(p);
     });
 }
 
@@ -730,54 +970,40 @@ int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable) {
     });
 }
 
+static void mesh_state_io(gvbd_context* ctx, int32_t scene, int32_t mesh, float* pos, float* vel, bool get) {
+    require(ctx != nullptr, "ctx is NULL");
+    SceneObj& s = scene_of(ctx, scene);
+    if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
+    require(mesh >= 0 && mesh < (int32_t)s.meshes.size(), "mesh index out of range");
+    const int32_t off = s.vertexOffset[mesh], n = s.meshVertices(mesh);
+    std::vector<float4> tmp(n);
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    float4* dev[2] = {s.dev.pos + off, s.dev.vel + off};
+    float* host[2] = {pos, vel};
+    for (int a = 0; a < 2; ++a) {
+        if (!host[a]) continue;
+        if (get) {
+            hip_check(hipMemcpy(tmp.data(), dev[a], sizeof(float4) * n, hipMemcpyDeviceToHost), "download");
+            for (int32_t i = 0; i < n; ++i) {
+                host[a][3 * i] = tmp[i].x;
+                host[a][3 * i + 1] = tmp[i].y;
+                host[a][3 * i + 2] = tmp[i].z;
+            }
+        } else {
+            for (int32_t i = 0; i < n; ++i) tmp[i] = make_float4(host[a][3 * i], host[a][3 * i + 1], host[a][3 * i + 2], 0.f);
+            hip_check(hipMemcpy(dev[a], tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
+        }
+    }
+}
+
 int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, float* positions, float* velocities) {
-    return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
-        require(mesh >= 0 && mesh < (int32_t)s.meshes.size(), "mesh index out of range");
-        const int32_t off = s.vertexOffset[mesh], n = s.meshes[mesh].nv;
-        std::vector<float4> tmp(n);
-        hip_check(hipStreamSynchronize(ctx->stream), "sync");
-        if (positions) {
-            hip_check(hipMemcpy(tmp.data(), s.dev.pos + off, sizeof(float4) * n, hipMemcpyDeviceToHost), "download");
-            for (int32_t i = 0; i < n; ++i) {
-                positions[3 * i] = tmp[i].x;
-                positions[3 * i + 1] = tmp[i].y;
-                positions[3 * i + 2] = tmp[i].z;
-            }
-        }
-        if (velocities) {
-            hip_check(hipMemcpy(tmp.data(), s.dev.vel + off, sizeof(float4) * n, hipMemcpyDeviceToHost), "download");
-            for (int32_t i = 0; i < n; ++i) {
-                velocities[3 * i] = tmp[i].x;
-                velocities[3 * i + 1] = tmp[i].y;
-                velocities[3 * i + 2] = tmp[i].z;
-            }
-        }
-    });
+    return guard(ctx, [&] { mesh_state_io(ctx, scene, mesh, positions, velocities, true); });
 }
 
 int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, const float* positions,
                               const float* velocities) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
-        require(mesh >= 0 && mesh < (int32_t)s.meshes.size(), "mesh index out of range");
-        const int32_t off = s.vertexOffset[mesh], n = s.meshes[mesh].nv;
-        std::vector<float4> tmp(n);
-        hip_check(hipStreamSynchronize(ctx->stream), "sync");
-        if (positions) {
-            for (int32_t i = 0; i < n; ++i)
-                tmp[i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.f);
-            hip_check(hipMemcpy(s.dev.pos + off, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
-        }
-        if (velocities) {
-            for (int32_t i = 0; i < n; ++i)
-                tmp[i] = make_float4(velocities[3 * i], velocities[3 * i + 1], velocities[3 * i + 2], 0.f);
-            hip_check(hipMemcpy(s.dev.vel + off, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
-        }
+        mesh_state_io(ctx, scene, mesh, const_cast<float*>(positions), const_cast<float*>(velocities), false);
     });
 }
 
@@ -813,8 +1039,7 @@ int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_
         PlanObj& p = batch_plan(ctx, batch);
         p.lpv = lanes_per_vertex;
         p.block = block;
-        p.dirty = true;
-        free_plan_graph(p);
+        free_plan(p);
     });
 }
 
@@ -844,7 +1069,9 @@ int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t gr
         PlanObj& p = batch_plan(ctx, batch);
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
-        plan_group(ctx, p, group, iteration);
+        const int b = p.groupItemBegin[group], e = p.groupItemBegin[group + 1];
+        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, ctx->stream),
+                  "group solve launch");
     });
 }
 
@@ -867,11 +1094,11 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
         for (int32_t sid : p.scenes) {
             SceneObj& s = scene_of(ctx, sid);
             if (group >= (int32_t)s.groups.size()) continue;
-            for (int32_t k = s.groupSlotOffset[group]; k < s.groupSlotOffset[group + 1]; ++k) {
-                // tet vertex-solve gather model: 60 B per vertex + 96 B per incident tet
-                b += 60 + 96 * (int64_t)s.slotIncCount[k];
-                ++nvx;
-            }
+            const size_t nt = s.numTetSlots();
+            for (int32_t k = s.tetGroupOffset[group]; k < s.tetGroupOffset[group + 1]; ++k, ++nvx)
+                b += s.slotWorkBytes[k];
+            for (int32_t k = s.clothGroupOffset[group]; k < s.clothGroupOffset[group + 1]; ++k, ++nvx)
+                b += s.slotWorkBytes[nt + k];
         }
         if (bytes) *bytes = b;
         if (vertices) *vertices = nvx;
@@ -942,12 +1169,40 @@ int gvbd_tet_mesh_constants(int32_t num_vertices, const float* positions, int32_
     });
 }
 
+int gvbd_cloth_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_faces, const int32_t* faces,
+                              float density, float* dminv, float* area, float* mass, int32_t* num_edges,
+                              int32_t* edges, float* Q) {
+    return guard(nullptr, [&] {
+        require(num_vertices > 0 && num_faces > 0 && positions && faces, "bad argument");
+        ClothMeshHost m;
+        m.nv = num_vertices;
+        m.nf = num_faces;
+        m.pos.assign(positions, positions + 3 * (size_t)num_vertices);
+        m.faces.assign(faces, faces + 3 * (size_t)num_faces);
+        m.mat.density = density;
+        std::string err;
+        if (!prepare_cloth_mesh(m, err)) fail(GVBD_ERR_INVALID, err);
+        if (dminv) std::memcpy(dminv, m.dminv.data(), m.dminv.size() * sizeof(float));
+        if (area) std::memcpy(area, m.area.data(), m.area.size() * sizeof(float));
+        if (mass) std::memcpy(mass, m.mass.data(), m.mass.size() * sizeof(float));
+        if (num_edges) *num_edges = (int32_t)m.edges.size();
+        if (edges)
+            for (size_t i = 0; i < m.edges.size(); ++i) {
+                const TriEdge& e = m.edges[i];
+                const int32_t row[6] = {e.v1, e.v2, e.v12Next, e.v21Next, e.f1, e.f2};
+                std::memcpy(edges + 6 * i, row, sizeof(row));
+            }
+        if (Q) std::memcpy(Q, m.Q.data(), m.Q.size() * sizeof(float));
+    });
+}
+
 int gvbd_color_graph(int32_t num_vertices, int32_t num_edges, const int32_t* edges, int32_t algorithm,
                      int32_t* colors, int32_t* num_colors_out) {
     return guard(nullptr, [&] {
         require(num_vertices >= 0 && num_edges >= 0 && colors, "bad argument");
         require(num_edges == 0 || edges, "edges is NULL");
-        for (int32_t i = 0; i < 2 * num_edges; ++i) require(edges[i] >= 0 && edges[i] < num_vertices, "edge index out of range");
+        for (int32_t i = 0; i < 2 * num_edges; ++i)
+            require(edges[i] >= 0 && edges[i] < num_vertices, "edge index out of range");
         Graph g = graph_from_edges(num_vertices, edges, num_edges);
         std::vector<int32_t> c = algorithm == GVBD_COLOR_GREEDY ? color_greedy(g) : color_mcs(g);
         std::memcpy(colors, c.data(), c.size() * sizeof(int32_t));
@@ -959,8 +1214,22 @@ int gvbd_color_tet_mesh(int32_t num_vertices, int32_t num_tets, const int32_t* t
                         int32_t* colors, int32_t* num_colors_out) {
     return guard(nullptr, [&] {
         require(num_vertices >= 0 && num_tets >= 0 && colors && (num_tets == 0 || tets), "bad argument");
-        for (int32_t i = 0; i < 4 * num_tets; ++i) require(tets[i] >= 0 && tets[i] < num_vertices, "tet index out of range");
+        for (int32_t i = 0; i < 4 * num_tets; ++i)
+            require(tets[i] >= 0 && tets[i] < num_vertices, "tet index out of range");
         Graph g = tet_vertex_graph(num_vertices, num_tets, tets);
+        std::vector<int32_t> c = algorithm == GVBD_COLOR_GREEDY ? color_greedy(g) : color_mcs(g);
+        std::memcpy(colors, c.data(), c.size() * sizeof(int32_t));
+        if (num_colors_out) *num_colors_out = num_colors(c);
+    });
+}
+
+int gvbd_color_tri_mesh(int32_t num_vertices, int32_t num_faces, const int32_t* faces, int32_t with_bending,
+                        int32_t algorithm, int32_t* colors, int32_t* num_colors_out) {
+    return guard(nullptr, [&] {
+        require(num_vertices >= 0 && num_faces >= 0 && colors && (num_faces == 0 || faces), "bad argument");
+        for (int32_t i = 0; i < 3 * num_faces; ++i)
+            require(faces[i] >= 0 && faces[i] < num_vertices, "face index out of range");
+        Graph g = tri_vertex_graph(num_vertices, num_faces, faces, with_bending != 0);
         std::vector<int32_t> c = algorithm == GVBD_COLOR_GREEDY ? color_greedy(g) : color_mcs(g);
         std::memcpy(colors, c.data(), c.size() * sizeof(int32_t));
         if (num_colors_out) *num_colors_out = num_colors(c);

@@ -70,6 +70,72 @@ def c1_soft_cube():
     return dict(positions=pos, tets=tets, physics=physics, material=material, coloring="greedy", steps=300)
 
 
+def cloth_sheet(n, size=1.0, height=1.0, origin=(0.0, 0.0), dtype=np.float64):
+    """n x n vertex grid in the horizontal plane y = height, tessellated as the reference's
+    grid generator does (Simulator/VBDCloth/ParameterGen/DataGen/S01_GenCloth.py, writeOBj):
+    vertex (i, j) = j + i n with x along j and z along i; each cell (i, j) gets faces
+    (v, v+1, v+n+1) and (v, v+n+1, v+n)."""
+    xs = np.linspace(origin[0], origin[0] + size, n)
+    zs = np.linspace(origin[1], origin[1] + size, n)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    pos = np.stack([xs[jj.ravel()], np.full(n * n, height), zs[ii.ravel()]], axis=1).astype(dtype)
+    ci, cj = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    v = (cj + ci * n).ravel()
+    f1 = np.stack([v, v + 1, v + n + 1], axis=1)
+    f2 = np.stack([v, v + n + 1, v + n], axis=1)
+    faces = np.stack([f1, f2], axis=1).reshape(-1, 3).astype(np.int32)
+    return pos, faces
+
+
+def cloth_curtain(n, size=1.0, top=2.0, dtype=np.float64):
+    """n x n vertical sheet in the plane z = 0 hanging from its top row (vertices 0..n-1 at
+    y = top), same tessellation as cloth_sheet; returns (positions, faces, top_row)."""
+    pos, faces = cloth_sheet(n, size, 0.0, dtype=np.float64)
+    pos = np.stack([pos[:, 0], top - pos[:, 2], np.zeros(len(pos))], axis=1).astype(dtype)
+    return pos, faces[:, [0, 2, 1]].copy(), list(range(n))
+
+
+def stvk_lame(young, poisson):
+    """Membrane (plane-stress) Lame parameters of a sheet from Young's modulus and Poisson ratio:
+    mu = E / (2 (1 + nu)), lambda = E nu / (1 - nu^2)."""
+    return young / (2.0 * (1.0 + poisson)), young * poisson / (1.0 - poisson * poisson)
+
+
+C4_PHYSICS = dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=10,
+                  world_lower=(-1e7, 0.0, -1e7), world_upper=(1e7, 1e7, 1e7))
+
+
+def c4_scenes(seed=0, num_scenes=256, vmin=500, vmax=50000, cloth_fraction=0.5, poisson=0.3):
+    """Batched-VBD workload: num_scenes independent scenes, vertex counts log-uniform in
+    [vmin, vmax], each (seeded) either a tet cube or a cloth sheet.
+
+    Tet cube: n^3 cells of 5 cm, (n+1)^3 close to the target, 10 cm above the ground,
+    stable neo-Hookean mu = 1e5, lambda = 1e6, rho = 1000, greedy colouring.
+    Cloth sheet: N x N vertices (N^2 close to the target), 1 m square at 1.5 m height, pinned
+    at two corners, StVK from E = 1e5 with nu = 0.3 (plane-stress Lame parameters), bending
+    stiffness 1e-4, rho = 0.2 kg/m^2, MCS colouring of the vertex graph with bending edges.
+    dt = 1/60 s, 10 iterations."""
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(vmin), np.log(vmax), size=num_scenes))
+    kinds = rng.random(num_scenes) < cloth_fraction
+    mu_c, lm_c = stvk_lame(1e5, poisson)
+    out = []
+    for target, is_cloth in zip(targets, kinds):
+        if is_cloth:
+            n = max(3, int(round(np.sqrt(target))))
+            pos, faces = cloth_sheet(n, 1.0, 1.5, dtype=np.float32)
+            out.append(dict(kind="cloth", positions=pos, faces=faces, physics=dict(C4_PHYSICS),
+                            fixed=[0, n - 1],
+                            material=dict(density=0.2, mu=mu_c, lmbd=lm_c, bending_stiffness=1e-4),
+                            coloring="mcs"))
+        else:
+            n = max(2, int(round(target ** (1.0 / 3.0))) - 1)
+            pos, tets = cube_tet_mesh(n, 0.05, origin=(0.0, 0.1, 0.0), dtype=np.float32)
+            out.append(dict(kind="tet", positions=pos, tets=tets, physics=dict(C4_PHYSICS), fixed=[],
+                            material=dict(density=1000.0, mu=1e5, lmbd=1e6), coloring="greedy"))
+    return out
+
+
 def c4_tet_scene_sizes(seed=0, num_scenes=256, vmin=500, vmax=50000):
     """Vertex-count targets drawn log-uniform in [vmin, vmax] (seeded)."""
     rng = np.random.default_rng(seed)

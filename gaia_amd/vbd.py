@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import PhysicsParams, TetMeshDesc, check, fptr, f32, i32, iptr, lib
+from ._lib import ClothMeshDesc, PhysicsParams, TetMeshDesc, check, fptr, f32, i32, iptr, lib
 
 # Reference defaults (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
 DEFAULT_PHYSICS = dict(
@@ -30,6 +30,32 @@ DEFAULT_PHYSICS = dict(
     step_size_gd=1e-3,
     friction_start_iter=0,
     associate_gravity_with_inertia=1,
+    # cloth (VBDCloth/VBDClothPhysicsParameters.h:15-81); self-contact is not supported yet, so
+    # handle_collision defaults to 0 here (the reference's JSON default is 1, see gvbd_scene_load_json)
+    cloth_boundary_contact_stiffness=1e4,
+    cloth_boundary_friction_eps=1e-2,
+    degenerate_triangle_thres=1e-6,
+    cloth_step_size=1.0,
+    handle_collision=0,
+    contact_stiffness=1e4,
+    contact_radius=1.0,
+    thickness=0.0,
+    conservative_step_relaxation=0.45,
+    contact_energy_type=1,
+)
+
+# VBDObjectParamsTriMeshStVk defaults (VBDCloth/VBDTriMeshStVK.h:17-20, VBDCloth/VBDBaseTriMesh.h:12-16,
+# TriMesh/TriMesh.h:33-36).
+DEFAULT_CLOTH_MATERIAL = dict(
+    density=1.0,
+    mu=1e4,
+    lmbd=1e4,
+    bending_stiffness=0.0,
+    damping_stvk=0.0,
+    damping_bending=0.0,
+    exponential_vel_damping=1.0,
+    constant_vel_damping=0.0,
+    initialization_type=2,
 )
 
 # ObjectParametersVBDNeoHookean defaults (VBD/VBD_NeoHookean.h:25-29, VBD/VBD_BaseMaterial.h:29-45,
@@ -148,6 +174,35 @@ def tet_mesh_constants(positions, tets, density):
     return dminv.reshape(nt, 9), vol, mass
 
 
+def cloth_mesh_constants(positions, faces, density):
+    """Rest-pose DmInv (F, 4 col-major 2x2), area (F,), mass (N,), edges (E, 6) and Q (E, 4, 4)."""
+    pos = f32(positions).reshape(-1)
+    faces = i32(faces).reshape(-1)
+    nv, nf = pos.size // 3, faces.size // 3
+    ne = ctypes.c_int32()
+    check(lib().gvbd_cloth_mesh_constants(nv, fptr(pos), nf, iptr(faces), float(density), None, None, None,
+                                          ctypes.byref(ne), None, None))
+    dminv = np.zeros(4 * nf, np.float32)
+    area = np.zeros(nf, np.float32)
+    mass = np.zeros(nv, np.float32)
+    edges = np.zeros(6 * ne.value, np.int32)
+    Q = np.zeros(16 * ne.value, np.float32)
+    check(lib().gvbd_cloth_mesh_constants(nv, fptr(pos), nf, iptr(faces), float(density), fptr(dminv), fptr(area),
+                                          fptr(mass), ctypes.byref(ne), iptr(edges), fptr(Q)))
+    return dminv.reshape(nf, 4), area, mass, edges.reshape(-1, 6), Q.reshape(-1, 4, 4)
+
+
+def color_tri_mesh(num_vertices, faces, algorithm="mcs", with_bending=True):
+    """Vertex colouring of a triangle mesh's vertex graph (optionally with bending cross edges)."""
+    faces = i32(faces).reshape(-1)
+    colors = np.zeros(num_vertices, dtype=np.int32)
+    nc = ctypes.c_int32()
+    alg = _lib.GVBD_COLOR_GREEDY if algorithm == "greedy" else _lib.GVBD_COLOR_MCS
+    check(lib().gvbd_color_tri_mesh(num_vertices, faces.size // 3, iptr(faces), 1 if with_bending else 0, alg,
+                                    iptr(colors), ctypes.byref(nc)))
+    return colors
+
+
 def color_graph(num_vertices, edges, algorithm="mcs"):
     edges = i32(edges).reshape(-1)
     colors = np.zeros(num_vertices, dtype=np.int32)
@@ -177,8 +232,14 @@ class VBDPhysics:
 
     def addTetMesh(self, positions, tets, fixed_points=(), vertex_colors=None, coloring="mcs", velocities=None,
                    initial_velocity=(0.0, 0.0, 0.0), **material):
-        self._pending.append((positions, tets, fixed_points, vertex_colors, coloring, velocities, initial_velocity,
-                              material))
+        self._pending.append(("tet", positions, tets, fixed_points, vertex_colors, coloring, velocities,
+                              initial_velocity, material))
+
+    def addClothMesh(self, positions, faces, fixed_points=(), vertex_colors=None, coloring="mcs", velocities=None,
+                     initial_velocity=(0.0, 0.0, 0.0), **material):
+        """StVK + bending triangle mesh (VBDClothSimulationFramework mesh)."""
+        self._pending.append(("cloth", positions, faces, fixed_points, vertex_colors, coloring, velocities,
+                              initial_velocity, material))
 
     def initialize(self):
         h = self.ctx.handle
@@ -193,8 +254,11 @@ class VBDPhysics:
         self.ctx.check(lib().gvbd_scene_create(h, ctypes.byref(params), ctypes.byref(sid)))
         self.scene = sid.value
         try:
-            for (pos, tets, fixed, colors, coloring, vel, v0, material) in self._pending:
-                self._add_tet(pos, tets, fixed, colors, coloring, vel, v0, material)
+            for (kind, pos, elems, fixed, colors, coloring, vel, v0, material) in self._pending:
+                if kind == "tet":
+                    self._add_tet(pos, elems, fixed, colors, coloring, vel, v0, material)
+                else:
+                    self._add_cloth(pos, elems, fixed, colors, coloring, vel, v0, material)
             self.ctx.check(lib().gvbd_scene_initialize(h, self.scene))
         except Exception:
             lib().gvbd_scene_destroy(h, self.scene)
@@ -235,6 +299,43 @@ class VBDPhysics:
         d.coloring_algorithm = _lib.GVBD_COLOR_GREEDY if coloring == "greedy" else _lib.GVBD_COLOR_MCS
         mid = ctypes.c_int32()
         self.ctx.check(lib().gvbd_scene_add_tet_mesh(self.ctx.handle, self.scene, ctypes.byref(d), ctypes.byref(mid)))
+        del keep
+        return mid.value
+
+    def _add_cloth(self, pos, faces, fixed, colors, coloring, vel, v0, material):
+        mat = dict(DEFAULT_CLOTH_MATERIAL)
+        unknown = set(material) - set(mat)
+        if unknown:
+            raise KeyError(f"unknown cloth material parameters {sorted(unknown)}")
+        mat.update(material)
+        pos = f32(pos).reshape(-1)
+        faces = i32(faces).reshape(-1)
+        fixed = i32(np.asarray(fixed, dtype=np.int32).reshape(-1))
+        d = ClothMeshDesc()
+        d.num_vertices = pos.size // 3
+        d.positions = fptr(pos)
+        keep = [pos, faces, fixed]
+        if vel is not None:
+            vel = f32(vel).reshape(-1)
+            keep.append(vel)
+            d.velocities = fptr(vel)
+        d.initial_velocity[:] = [float(x) for x in v0]
+        d.num_faces = faces.size // 3
+        d.faces = iptr(faces)
+        for k in ("density", "mu", "lmbd", "bending_stiffness", "damping_stvk", "damping_bending",
+                  "exponential_vel_damping", "constant_vel_damping"):
+            setattr(d, k, float(mat[k]))
+        d.initialization_type = int(mat["initialization_type"])
+        d.num_fixed = fixed.size
+        d.fixed_points = iptr(fixed) if fixed.size else None
+        if colors is not None:
+            colors = i32(colors).reshape(-1)
+            keep.append(colors)
+            d.vertex_colors = iptr(colors)
+        d.coloring_algorithm = _lib.GVBD_COLOR_GREEDY if coloring == "greedy" else _lib.GVBD_COLOR_MCS
+        mid = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_add_cloth_mesh(self.ctx.handle, self.scene, ctypes.byref(d),
+                                                       ctypes.byref(mid)))
         del keep
         return mid.value
 

@@ -71,6 +71,17 @@ typedef struct gvbd_physics_params {
     float step_size_gd;
     int32_t friction_start_iter;
     int32_t associate_gravity_with_inertia;
+    /* cloth meshes (VBDClothPhysicsParameters, VBDCloth/VBDClothPhysicsParameters.h:15-81) */
+    float cloth_boundary_contact_stiffness; /* boundaryContactStiffness (1e4; the reference never reads it from JSON) */
+    float cloth_boundary_friction_eps;      /* boundaryFrictionEpsValue (1e-2) */
+    float degenerate_triangle_thres;        /* degenerateTriangleThres (1e-6) */
+    float cloth_step_size;                  /* stepSize (1) */
+    int32_t handle_collision;               /* VF/EE self-contact of cloth; must be 0 in this build */
+    float contact_stiffness;                /* contactStiffness (1e4) */
+    float contact_radius;                   /* contactRadius (1) */
+    float thickness;                        /* thickness (0) */
+    float conservative_step_relaxation;     /* conservativeStepRelaxation (0.45) */
+    int32_t contact_energy_type;            /* contactEnergyTypes (1) */
 } gvbd_physics_params;
 
 /* One tet mesh (ObjectParametersVBDNeoHookean, VBD/VBD_NeoHookean.cpp:1613-1625,
@@ -98,8 +109,34 @@ typedef struct gvbd_tet_mesh_desc {
     int32_t coloring_algorithm;    /* GVBD_COLOR_MCS / GVBD_COLOR_GREEDY when vertex_colors is NULL */
 } gvbd_tet_mesh_desc;
 
+/* One cloth (triangle) mesh with StVK membrane + quadratic bending (VBDObjectParamsTriMeshStVk,
+ * VBDCloth/VBDTriMeshStVK.cpp:1171-1180, VBDCloth/VBDBaseTriMesh.h:8-18, TriMesh/TriMesh.h:21-40).
+ * The rest pose is the given positions (use3DRestpose). */
+typedef struct gvbd_cloth_mesh_desc {
+    int32_t num_vertices;
+    const float* positions;   /* 3 * num_vertices */
+    const float* velocities;  /* 3 * num_vertices, or NULL for initial_velocity */
+    float initial_velocity[3];
+    int32_t num_faces;
+    const int32_t* faces;     /* 3 * num_faces, consistently oriented */
+    float density;
+    float mu;                 /* "miu" */
+    float lambda;
+    float bending_stiffness;
+    float damping_stvk;
+    float damping_bending;
+    float exponential_vel_damping; /* applied per substep as given (cloth does not rescale by dt) */
+    float constant_vel_damping;
+    int32_t initialization_type;   /* 0 inertia, 2 previous position (default), 3 (= 0 without contact), 4 none */
+    int32_t num_fixed;
+    const int32_t* fixed_points;
+    const int32_t* vertex_colors;  /* colour per vertex, or NULL to colour the vertex graph with bending edges */
+    int32_t coloring_algorithm;
+} gvbd_cloth_mesh_desc;
+
 /* sizes of the ABI structs as compiled into the library (binding self-check) */
-int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes);
+int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
+                          int32_t* cloth_mesh_desc_bytes);
 
 /* ---- context ---- */
 int gvbd_context_create(int32_t device, gvbd_context** out);
@@ -113,6 +150,9 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
 int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
 int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene);
 int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mesh_desc* desc, int32_t* mesh_out);
+/* VBDClothSimulationFramework mesh load + TriMeshFEM::initialize (Framework/BaseClothSimPhsicsFramework.cpp,
+ * TriMesh/TriMesh.cpp:15-111) */
+int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth_mesh_desc* desc, int32_t* mesh_out);
 int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene);
 /* Models.json + Parameters.json; "${REPO_ROOT}" in paths is replaced by repo_root (may be NULL). */
 int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char* params_json,
@@ -170,7 +210,19 @@ int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, flo
 int gvbd_tet_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_tets, const int32_t* tets,
                             float density, float* dminv, float* volume, float* mass);
 
+/* ---- cloth rest pose, topology and bending forms (host; TriMesh/TriMesh.cpp:213-244, 481-753,
+ * VBDCloth/VBDTriMeshStVK.cpp:50-88). dminv 4*F (column-major 2x2), area F, mass N; edges 6*E as
+ * (eV1, eV2, eV12Next, eV21Next, fId1, fId2) with -1 on boundary edges; Q 16*E (row-major 4x4).
+ * Pass NULL for outputs not needed; *num_edges receives E (call once with edges = Q = NULL to size). */
+int gvbd_cloth_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_faces, const int32_t* faces,
+                              float density, float* dminv, float* area, float* mass, int32_t* num_edges,
+                              int32_t* edges, float* Q);
+
 /* ---- colouring (host) ---- */
+/* cloth vertex graph: mesh edges plus (with_bending) the cross pair of every interior edge
+ * (GraphColoring/TriMeshVertexGraph.h:20-45) */
+int gvbd_color_tri_mesh(int32_t num_vertices, int32_t num_faces, const int32_t* faces, int32_t with_bending,
+                        int32_t algorithm, int32_t* colors, int32_t* num_colors);
 int gvbd_color_graph(int32_t num_vertices, int32_t num_edges, const int32_t* edges, int32_t algorithm,
                      int32_t* colors, int32_t* num_colors);
 int gvbd_color_tet_mesh(int32_t num_vertices, int32_t num_tets, const int32_t* tets, int32_t algorithm,

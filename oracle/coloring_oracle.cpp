@@ -13,6 +13,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <map>
 #include <set>
 #include <utility>
 #include <vector>
@@ -125,6 +126,27 @@ int oracle_color_tet_mesh(int nv, int nt, const int32_t* tets, int algorithm, in
         flat.push_back(e.second);
     }
     return oracle_color_graph(nv, (int)es.size(), flat.data(), algorithm, colors);
+}
+
+// TriMeshVertexGraph.h:20-45: every mesh edge, and with bending the pair of vertices
+// opposite each interior edge.
+int oracle_color_tri_mesh(int nv, int nf, const int32_t* faces, int with_bending, int algorithm, int32_t* colors) {
+    std::map<std::pair<int, int>, std::vector<int>> opposite;
+    for (int f = 0; f < nf; ++f)
+        for (int k = 0; k < 3; ++k) {
+            int a = faces[3 * f + k], b = faces[3 * f + (k + 1) % 3], c = faces[3 * f + (k + 2) % 3];
+            opposite[{std::min(a, b), std::max(a, b)}].push_back(c);
+        }
+    std::vector<int32_t> flat;
+    for (auto& kv : opposite) {
+        flat.push_back(kv.first.first);
+        flat.push_back(kv.first.second);
+        if (with_bending && kv.second.size() == 2) {
+            flat.push_back(kv.second[0]);
+            flat.push_back(kv.second[1]);
+        }
+    }
+    return oracle_color_graph(nv, (int)flat.size() / 2, flat.data(), algorithm, colors);
 }
 
 }  // extern "C"

@@ -32,7 +32,20 @@ enum OraclePhysicsParam {
     OP_ASSOCIATE_GRAVITY,
     OP_INIT_TYPE_UNUSED,         /* per mesh: OM_INIT_TYPE */
     OP_INIT_RATIO_G,
+    OP_CLOTH_BOUNDARY_STIFFNESS, /* VBDClothPhysicsParameters::boundaryContactStiffness */
+    OP_CLOTH_BOUNDARY_FRICTION_EPS, /* boundaryFrictionEpsValue */
+    OP_DEGENERATE_THRES,         /* degenerateTriangleThres */
     OP_COUNT = 32
+};
+
+enum OracleClothMaterialParam {
+    OMC_DENSITY = 0,
+    OMC_MU, OMC_LAMBDA, OMC_BENDING,
+    OMC_DAMP_STVK, OMC_DAMP_BENDING,
+    OMC_EXP_VEL_DAMPING,         /* applied per substep as given (no pow(., dt) for cloth) */
+    OMC_CONST_VEL_DAMPING,
+    OMC_INIT_TYPE,               /* VBDObjectParamsTriMesh::initializationType: 0, 2, 3 (no contact), 4 */
+    OMC_COUNT = 16
 };
 
 enum OracleMaterialParam {
@@ -85,6 +98,32 @@ int oracle_tet_scene_group(OracleTetScene*, int g, int32_t* out);
 int oracle_tet_scene_step(OracleTetScene*, int nsteps, int nthreads);
 int oracle_tet_scene_get_state(OracleTetScene*, int mesh, double* pos, double* vel);
 int oracle_tet_scene_set_state(OracleTetScene*, int mesh, const double* pos, const double* vel);
+
+/* --- cloth: TriMesh precompute + topology + bending forms (TriMesh.cpp, VBDTriMeshStVK.cpp:50-88) ---
+ * dminv 4F col-major, area F, mass N, edges 6E (eV1 eV2 eV12Next eV21Next fId1 fId2), Q 16E row-major;
+ * pass NULL to skip an output; num_edges receives E */
+int oracle_cloth_prepare(int precision, int nv, const double* pos, int nf, const int32_t* faces, double density,
+                         double* dminv, double* area, double* mass, int32_t* num_edges, int32_t* edges, double* Q);
+/* StVK membrane force/Hessian of one triangle for vertex `order` (VBDTriMeshStVK.cpp:721-911) */
+void oracle_stvk_vertex_f64(const double* x3, const double* dminv4, double area, double mu, double lmbd, int order,
+                            double* force3, double* hess9);
+
+typedef struct OracleClothScene OracleClothScene;
+OracleClothScene* oracle_cloth_scene_create(int precision, const double* physics /*OP_COUNT*/);
+void oracle_cloth_scene_destroy(OracleClothScene*);
+int oracle_cloth_scene_add_mesh(OracleClothScene*, int nv, const double* pos, const double* vel, int nf,
+                                const int32_t* faces, const double* material /*OMC_COUNT*/, int nfixed,
+                                const int32_t* fixed, const int32_t* colors);
+int oracle_cloth_scene_finalize(OracleClothScene*);
+int oracle_cloth_scene_num_groups(OracleClothScene*);
+int oracle_cloth_scene_group(OracleClothScene*, int g, int32_t* out);
+int oracle_cloth_scene_step(OracleClothScene*, int nsteps, int nthreads);
+int oracle_cloth_scene_get_state(OracleClothScene*, int mesh, double* pos, double* vel);
+int oracle_cloth_scene_set_state(OracleClothScene*, int mesh, const double* pos, const double* vel);
+
+/* --- cloth vertex graph: mesh edges plus, with bending, the cross edge of every interior edge
+ * (TriMeshVertexGraph.h:20-45) --- */
+int oracle_color_tri_mesh(int nv, int nf, const int32_t* faces, int with_bending, int algorithm, int32_t* colors);
 
 #ifdef __cplusplus
 }

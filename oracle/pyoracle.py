@@ -225,3 +225,162 @@ class TetScene:
                 self._h = None
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- cloth
+OP.update(CLOTH_BOUNDARY_STIFFNESS=22, CLOTH_BOUNDARY_FRICTION_EPS=23, DEGENERATE_THRES=24)
+OMC = dict(DENSITY=0, MU=1, LAMBDA=2, BENDING=3, DAMP_STVK=4, DAMP_BENDING=5, EXP_VEL_DAMPING=6,
+           CONST_VEL_DAMPING=7, INIT_TYPE=8)
+OMC_COUNT = 16
+
+
+def _cloth_lib():
+    L = lib()
+    if not getattr(L, "_cloth_bound", False):
+        L.oracle_cloth_prepare.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_double, _dp,
+                                           _dp, _dp, _ip, _ip, _dp]
+        L.oracle_stvk_vertex_f64.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _dp, _dp]
+        L.oracle_stvk_vertex_f64.restype = None
+        L.oracle_cloth_scene_create.argtypes = [ctypes.c_int, _dp]
+        L.oracle_cloth_scene_create.restype = ctypes.c_void_p
+        L.oracle_cloth_scene_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_cloth_scene_destroy.restype = None
+        L.oracle_cloth_scene_add_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                                  ctypes.c_int, _ip, _ip]
+        L.oracle_cloth_scene_finalize.argtypes = [ctypes.c_void_p]
+        L.oracle_cloth_scene_num_groups.argtypes = [ctypes.c_void_p]
+        L.oracle_cloth_scene_group.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+        L.oracle_cloth_scene_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_cloth_scene_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_cloth_scene_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_color_tri_mesh.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip]
+        L._cloth_bound = True
+    return L
+
+
+def cloth_physics_vector(p):
+    v = physics_vector(p)
+    v[OP["CLOTH_BOUNDARY_STIFFNESS"]] = p.get("cloth_boundary_contact_stiffness", 1e4)
+    v[OP["CLOTH_BOUNDARY_FRICTION_EPS"]] = p.get("cloth_boundary_friction_eps", 1e-2)
+    v[OP["DEGENERATE_THRES"]] = p.get("degenerate_triangle_thres", 1e-6)
+    v[OP["STEP_SIZE"]] = p.get("cloth_step_size", 1.0)
+    return v
+
+
+def cloth_material_vector(m):
+    v = np.zeros(OMC_COUNT)
+    v[OMC["DENSITY"]] = m.get("density", 1.0)
+    v[OMC["MU"]] = m.get("mu", 1e4)
+    v[OMC["LAMBDA"]] = m.get("lmbd", 1e4)
+    v[OMC["BENDING"]] = m.get("bending_stiffness", 0.0)
+    v[OMC["DAMP_STVK"]] = m.get("damping_stvk", 0.0)
+    v[OMC["DAMP_BENDING"]] = m.get("damping_bending", 0.0)
+    v[OMC["EXP_VEL_DAMPING"]] = m.get("exponential_vel_damping", 1.0)
+    v[OMC["CONST_VEL_DAMPING"]] = m.get("constant_vel_damping", 0.0)
+    v[OMC["INIT_TYPE"]] = m.get("initialization_type", 2)
+    return v
+
+
+def cloth_prepare(pos, faces, density, precision=64):
+    L = _cloth_lib()
+    p, pp = _d(np.asarray(pos).reshape(-1))
+    f, pf = _i(np.asarray(faces).reshape(-1))
+    nv, nf = p.size // 3, f.size // 3
+    ne = ctypes.c_int32()
+    rc = L.oracle_cloth_prepare(precision, nv, pp, nf, pf, density, None, None, None, ctypes.byref(ne), None, None)
+    if rc != 0:
+        raise ValueError("oracle: bad cloth mesh")
+    dminv, area, mass = np.zeros(4 * nf), np.zeros(nf), np.zeros(nv)
+    edges = np.zeros(6 * ne.value, np.int32)
+    Q = np.zeros(16 * ne.value)
+    L.oracle_cloth_prepare(precision, nv, pp, nf, pf, density, dminv.ctypes.data_as(_dp), area.ctypes.data_as(_dp),
+                           mass.ctypes.data_as(_dp), ctypes.byref(ne), edges.ctypes.data_as(_ip),
+                           Q.ctypes.data_as(_dp))
+    return dminv.reshape(nf, 4), area, mass, edges.reshape(-1, 6), Q.reshape(-1, 4, 4)
+
+
+def stvk_vertex(x3, dminv4, area, mu, lmbd, order):
+    L = _cloth_lib()
+    x, px = _d(np.asarray(x3).reshape(-1))
+    dm, pd = _d(np.asarray(dminv4).reshape(-1))
+    f = np.zeros(3)
+    h = np.zeros(9)
+    L.oracle_stvk_vertex_f64(px, pd, area, mu, lmbd, order, f.ctypes.data_as(_dp), h.ctypes.data_as(_dp))
+    return f, h.reshape(3, 3, order="F")
+
+
+def color_tri_mesh(nv, faces, algorithm="mcs", with_bending=True):
+    L = _cloth_lib()
+    f, pf = _i(np.asarray(faces).reshape(-1))
+    colors = np.zeros(nv, np.int32)
+    L.oracle_color_tri_mesh(nv, f.size // 3, pf, 1 if with_bending else 0, 0 if algorithm == "mcs" else 1,
+                            colors.ctypes.data_as(_ip))
+    return colors
+
+
+class ClothScene:
+    """Oracle cloth VBD scene (VBDClothSimulationFramework::runStep_CDEveryIter, no contact)."""
+
+    def __init__(self, physics, precision=32):
+        L = _cloth_lib()
+        pv, pp = _d(cloth_physics_vector(physics))
+        self._h = L.oracle_cloth_scene_create(precision, pp)
+        self.nv = []
+
+    def add_mesh(self, pos, faces, material, colors, fixed=(), vel=None):
+        L = _cloth_lib()
+        pos = np.asarray(pos).reshape(-1, 3)
+        nv = len(pos)
+        p, pp = _d(pos.reshape(-1))
+        pv = None
+        if vel is not None:
+            v, pv = _d(np.asarray(vel).reshape(-1))
+        f, pf = _i(np.asarray(faces).reshape(-1))
+        m, pm = _d(cloth_material_vector(material))
+        fx, pfx = _i(np.asarray(fixed, dtype=np.int32).reshape(-1))
+        c, pc = _i(colors)
+        rc = L.oracle_cloth_scene_add_mesh(self._h, nv, pp, pv, f.size // 3, pf, pm, fx.size, pfx, pc)
+        if rc < 0:
+            raise ValueError("oracle: bad cloth mesh")
+        self.nv.append(nv)
+        return rc
+
+    def finalize(self):
+        _cloth_lib().oracle_cloth_scene_finalize(self._h)
+
+    def groups(self):
+        L = _cloth_lib()
+        out = []
+        for g in range(L.oracle_cloth_scene_num_groups(self._h)):
+            n = L.oracle_cloth_scene_group(self._h, g, None)
+            buf = np.zeros(2 * n, np.int32)
+            L.oracle_cloth_scene_group(self._h, g, buf.ctypes.data_as(_ip))
+            out.append(buf.reshape(-1, 2))
+        return out
+
+    def step(self, n=1, threads=1):
+        _cloth_lib().oracle_cloth_scene_step(self._h, n, threads)
+
+    def get_state(self, mesh=0):
+        nv = self.nv[mesh]
+        x = np.zeros(3 * nv)
+        v = np.zeros(3 * nv)
+        _cloth_lib().oracle_cloth_scene_get_state(self._h, mesh, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp))
+        return x.reshape(nv, 3), v.reshape(nv, 3)
+
+    def set_state(self, mesh, x=None, v=None):
+        px = pv = None
+        if x is not None:
+            xa, px = _d(np.asarray(x).reshape(-1))
+        if v is not None:
+            va, pv = _d(np.asarray(v).reshape(-1))
+        _cloth_lib().oracle_cloth_scene_set_state(self._h, mesh, px, pv)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _cloth_lib().oracle_cloth_scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
