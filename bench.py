@@ -2,11 +2,13 @@
 
 One step = one frame of VBD (initial step, 10 iterations x parallel groups,
 velocity update) over every scene of the workload, with all scene data
-resident in HBM. `value` is the batched rate (all scenes advanced by one launch
-per parallel group, captured in a HIP graph); the plain rate (one scene per
-launch sequence, each scene its own graph) is reported beside it.
+resident in HBM. `value` is the batched rate: all scenes advanced together, either
+by one persistent work-queue launch per iteration or by one launch per parallel
+group (the autotuner times both and keeps the faster), the frame captured in a HIP
+graph. The plain rate (one scene per launch sequence, each scene its own graph)
+is reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 256] [--seed 0]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 256] [--seed 0] [--schedule KEY]
 """
 import argparse
 import json
@@ -47,51 +49,42 @@ def free_vertices(spec):
     return len(spec["positions"]) - len(spec.get("fixed", []))
 
 
-def autotune(ctx, batch, candidates, reps=2, use_graph=True):
-    """Times one (graph-captured) frame per schedule and keeps the fastest."""
-    best = None
-    results = {}
-    for lpv, block in candidates:
-        batch.set_schedule(lpv, block)
-        batch.record_graph(use_graph)
-        batch.run_steps(1)
-        ctx.synchronize()
-        ctx.event_record(0)
-        batch.run_steps(reps)
-        ctx.event_record(1)
-        ms = ctx.event_elapsed_ms(0, 1) / reps
-        results[f"{lpv}x{block}"] = round(ms, 3)
-        if best is None or ms < best[0]:
-            best = (ms, lpv, block)
-    batch.set_schedule(best[1], best[2])
-    batch.record_graph(use_graph)
-    return (best[1], best[2]), results
+def solve_kernel_roofline(ctx, batch, schedule, reps=5):
+    """Average duration of the dominant vertex-solve launches (HIP events on the library's
+    stream) and their algorithmic bytes (gather model).
 
-
-def solve_kernel_roofline(ctx, batch, reps=5):
-    """Average duration of the vertex-solve launches (HIP events on the library's
-    stream) and their algorithmic bytes (gather model)."""
+    group mode: every parallel group's launch, `reps` times each;
+    queue mode: `reps` persistent launches, each one whole iteration over every group."""
     groups = batch.num_groups()
-    tot_bytes = 0
-    tot_ms = 0.0
-    launches = 0
-    for g in range(groups):
-        b, _ = batch.group_work(g)
-        batch.solve_group(g, 0)
+    work = [batch.group_work(g)[0] for g in range(groups)]
+    if schedule["mode"] == "queue":
+        batch.apply_initial_step()
+        batch.run_iteration(0)
         ctx.synchronize()
         ctx.event_record(2)
-        for _ in range(reps):
-            batch.solve_group(g, 0)
+        for it in range(1, reps + 1):
+            batch.run_iteration(it)
         ctx.event_record(3)
-        tot_ms += ctx.event_elapsed_ms(2, 3)
-        tot_bytes += b * reps
-        launches += reps
-    avg_ms = tot_ms / launches
-    avg_bytes = tot_bytes / launches
+        avg_ms = ctx.event_elapsed_ms(2, 3) / reps
+        avg_bytes = float(sum(work))
+        kernel = "queue_solve_kernel"
+    else:
+        tot_ms = 0.0
+        for g in range(groups):
+            batch.solve_group(g, 0)
+            ctx.synchronize()
+            ctx.event_record(2)
+            for _ in range(reps):
+                batch.solve_group(g, 0)
+            ctx.event_record(3)
+            tot_ms += ctx.event_elapsed_ms(2, 3)
+        avg_ms = tot_ms / (groups * reps)
+        avg_bytes = sum(work) / groups
+        kernel = "group_solve_kernel"
     gbs = avg_bytes / (avg_ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="group_solve_kernel", avg_launch_us=round(avg_ms * 1e3, 2),
+                kernel=kernel, avg_launch_us=round(avg_ms * 1e3, 2),
                 algorithmic_bytes_per_launch=int(avg_bytes))
 
 
@@ -139,7 +132,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--schedule", default=None, help="fixed LPVxBLOCK (e.g. 8x256) instead of autotuning")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed schedule instead of autotuning: group:LxB or queue:LxB:eN[:gM] "
+                         "(lanes per vertex x block, N vertices per queue entry, M blocks per CU)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth calibration")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (profiler runs)")
@@ -153,6 +148,7 @@ def main():
 
     import gaia_amd
     from gaia_amd import Batch, scenes
+    from gaia_amd.autotune import apply_schedule, autotune, default_candidates, parse_schedule_key, schedule_key
 
     t0 = time.perf_counter()
     specs = scenes.c4_scenes(args.seed, args.scenes)
@@ -165,10 +161,24 @@ def main():
     log(f"built {len(sims)} scenes ({len(sims) - ncloth} tet, {ncloth} cloth), {nverts} free vertices, in "
         f"{time.perf_counter() - t0:.1f}s; device memory {ctx.memory_bytes() / 2**30:.2f} GiB")
 
-    # ---- plain: one scene per launch sequence (each scene replays its own graph)
+    # ---- batched schedule: autotuned (or given) for the batch of all scenes
+    batch = Batch(ctx, sims)
+    if args.schedule:
+        apply_schedule(batch, parse_schedule_key(args.schedule), use_graph)
+        sched, tune = batch.get_schedule(), {}
+    else:
+        sched, tune = autotune(ctx, batch, default_candidates(), use_graph=use_graph)
+    key = schedule_key(sched)
+    entries, grid = batch.queue_info()
+    log(f"autotune (ms/frame): {tune} -> {key}" + (f" ({entries} queue entries per iteration, grid {grid})"
+                                                     if sched["mode"] == "queue" else ""))
+
+    # ---- plain: one scene per launch sequence (each scene replays its own graph), with the
+    # batched schedule's lanes per vertex and block size
     plain_rate = None
     if not args.no_plain:
         for s in sims:
+            s.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
             s.recordVBDSolveGraph(use_graph)
         for _ in range(args.warmup):
             for s in sims:
@@ -183,17 +193,7 @@ def main():
         plain_rate = nverts * iters * args.steps / plain_s
         log(f"plain: {plain_s / args.steps * 1e3:.2f} ms/step, {plain_rate:.3e} vertex-solves/s")
 
-    # ---- batched: all scenes, one launch per parallel group, one graph per frame
-    batch = Batch(ctx, sims)
-    if args.schedule:
-        lpv, block = (int(x) for x in args.schedule.split("x"))
-        batch.set_schedule(lpv, block)
-        batch.record_graph(use_graph)
-        sched, tune = (lpv, block), {}
-    else:
-        sched, tune = autotune(ctx, batch, [(1, 256), (2, 256), (4, 256), (8, 256), (4, 128), (8, 128)],
-                               use_graph=use_graph)
-    log(f"autotune (ms/frame): {tune} -> lpv {sched[0]}, block {sched[1]}")
+    # ---- batched: all scenes together, one graph per frame
     for _ in range(args.warmup):
         batch.run_steps(1)
     ctx.synchronize()
@@ -204,13 +204,13 @@ def main():
     rate = nverts * iters * args.steps / batched_s
     log(f"batched: {batched_s / args.steps * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s")
 
-    roof = solve_kernel_roofline(ctx, batch)
+    roof = solve_kernel_roofline(ctx, batch, sched)
     # measured HBM bytes per launch (scaled PMC FETCH_SIZE + WRITE_SIZE) for this kernel,
     # workload and schedule, from the committed profile summary
-    side = os.path.join(REPO, "profiles", "group_solve_batched_256_scenes.traffic.json")
+    side = os.path.join(REPO, "profiles", f"{roof['kernel']}_batched_256_scenes.traffic.json")
     if os.path.exists(side) and args.scenes == 256 and args.seed == 0:
         t = json.load(open(side))
-        if t.get("schedule") == f"{sched[0]}x{sched[1]}":
+        if t.get("schedule") == key:
             roof["traffic"] = t["hbm_bytes_per_launch"]
     copy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5)
     roof["measured_copy_gbs"] = round(2 * args.copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
@@ -235,13 +235,16 @@ def main():
         "config": {
             "workload": f"batched VBD: {len(specs)} scenes of log-uniform 500-50k vertices (seed {args.seed}), "
                         f"{len(specs) - ncloth} neo-Hookean tet cubes + {ncloth} StVK+bending cloth sheets pinned "
-                        f"at two corners, dt=1/60, 10 iterations, one launch per parallel group, HIP graph "
-                        f"per frame",
+                        f"at two corners, dt=1/60, 10 iterations, "
+                        + ("one persistent work-queue launch per iteration" if sched["mode"] == "queue"
+                           else "one launch per parallel group") + ", HIP graph per frame",
             "scenes": len(specs),
             "cloth_scenes": ncloth,
             "vertices": nverts,
             "iterations": iters,
-            "schedule": {"lanes_per_vertex": sched[0], "block": sched[1]},
+            "schedule": dict(sched, key=key, queue_entries=entries if sched["mode"] == "queue" else None,
+                             grid=grid if sched["mode"] == "queue" else None),
+            "autotune_ms_per_frame": tune,
         },
         "plain_value": None if plain_rate is None else round(plain_rate, 1),
         "batched_over_plain": None if plain_rate is None else round(rate / plain_rate, 2),

@@ -106,11 +106,29 @@ class ClothMeshDesc(ctypes.Structure):
     ]
 
 
+MODE_GROUP_LAUNCHES = 0
+MODE_WORK_QUEUE = 1
+
+
+class Schedule(ctypes.Structure):
+    """gvbd_schedule."""
+    _fields_ = [
+        ("lanes_per_vertex", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("vertices_per_entry", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("spin_limit", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
+_SCHED = ctypes.POINTER(Schedule)
 # name -> argtypes (all return int unless listed in _RESTYPES)
 _SIGNATURES = {
-    "gvbd_abi_struct_sizes": [c_int32_p, c_int32_p, c_int32_p],
+    "gvbd_abi_struct_sizes": [c_int32_p, c_int32_p, c_int32_p, c_int32_p],
     "gvbd_context_create": [_I, ctypes.POINTER(_P)],
     "gvbd_context_destroy": [_P],
     "gvbd_last_error": [_P],
@@ -128,10 +146,13 @@ _SIGNATURES = {
     "gvbd_scene_parallel_group": [_P, _I, _I, c_int32_p, c_int32_p],
     "gvbd_scene_mesh_constants": [_P, _I, _I, c_float_p, c_float_p, c_float_p],
     "gvbd_scene_set_schedule": [_P, _I, _I, _I],
+    "gvbd_scene_set_schedule_ex": [_P, _I, _SCHED],
+    "gvbd_scene_get_schedule": [_P, _I, _SCHED],
     "gvbd_scene_run_step": [_P, _I],
     "gvbd_scene_run_steps": [_P, _I, _I],
     "gvbd_scene_apply_initial_step": [_P, _I],
     "gvbd_scene_solve_parallel_group": [_P, _I, _I, _I],
+    "gvbd_scene_run_iteration": [_P, _I, _I],
     "gvbd_scene_update_velocities": [_P, _I],
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
@@ -139,8 +160,14 @@ _SIGNATURES = {
     "gvbd_batch_create": [_P, c_int32_p, _I, c_int32_p],
     "gvbd_batch_destroy": [_P, _I],
     "gvbd_batch_set_schedule": [_P, _I, _I, _I],
+    "gvbd_batch_set_schedule_ex": [_P, _I, _SCHED],
+    "gvbd_batch_get_schedule": [_P, _I, _SCHED],
+    "gvbd_batch_queue_info": [_P, _I, c_int32_p, c_int32_p],
     "gvbd_batch_record_graph": [_P, _I, _I],
     "gvbd_batch_run_steps": [_P, _I, _I],
+    "gvbd_batch_apply_initial_step": [_P, _I],
+    "gvbd_batch_run_iteration": [_P, _I, _I],
+    "gvbd_batch_update_velocities": [_P, _I],
     "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
     "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],

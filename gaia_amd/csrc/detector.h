@@ -1,0 +1,143 @@
+// GPU contact detector for triangle meshes (the Embree-based ClothContactDetector replaced by
+// linear BVHs): Morton-sorted LBVH build, per-step refit, VF and EE discrete proximity queries,
+// per-primitive minimum distances and per-vertex conservative bounds.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace gvbd {
+
+// One BVH node: box lo/hi; internal nodes keep their children in lo.w / hi.w (int bits),
+// leaves their primitive id in lo.w. Internal nodes occupy [0, n-1), leaves [n-1, 2n-1).
+struct alignas(16) BvhNode {
+    float4 lo;
+    float4 hi;
+};
+
+// A VF contact: the face whose closest point to the vertex lies on primitive (type, id)
+// (type 0 vertex, 1 edge, 2 face interior), d = distance.
+struct alignas(16) VFContactRec {
+    int32_t face;
+    int32_t type;
+    int32_t prim;
+    float d;
+};
+
+// An EE contact: the other edge, the closest-point parameters on both edges and the distance.
+struct alignas(16) EEContactRec {
+    int32_t edge;
+    float mu1;
+    float mu2;
+    float d;
+};
+
+struct DeviceBvh {
+    int n = 0;                       // primitives
+    BvhNode* nodes = nullptr;        // 2n - 1
+    int* parent = nullptr;           // 2n - 1
+    int* levelNodes = nullptr;       // internal nodes ordered by depth, deepest first
+    std::vector<int> levelBegin;     // host: ranges of levelNodes per depth (deepest first)
+    uint64_t* keys = nullptr;        // n (sorted Morton keys)
+    uint64_t* keysAlt = nullptr;     // n
+    uint64_t* depthKeys = nullptr;   // n - 1
+    uint64_t* depthKeysAlt = nullptr;
+    int* depth = nullptr;            // n - 1
+    void* sortTemp = nullptr;
+    size_t sortTempBytes = 0;
+};
+
+struct DetectorResultCounts {
+    int64_t vfPairs, vfContacts, eeContacts, vfOverflow, eeOverflow;
+};
+
+class ContactDetector {
+  public:
+    ContactDetector(int device, hipStream_t stream, float maxQueryDis, int vfCap, int eeCap);
+    ~ContactDetector();
+    ContactDetector(const ContactDetector&) = delete;
+    ContactDetector& operator=(const ContactDetector&) = delete;
+
+    int addMesh(int nv, const float* pos, int nf, const int* faces);
+    void finalize();
+    bool finalized() const { return ready_; }
+
+    void setPositions(int mesh, const float* pos);
+    void getPositions(int mesh, float* pos) const;
+    // x = rest + step * drift[mesh] + sigma * N(0, 1) (counter-based, seeded per (seed, step, vertex))
+    void perturb(uint64_t seed, int step, float sigma, const float* drift);
+    void updateBvh(bool rebuild);
+    void query(bool record);
+    void bounds(float thickness, float relaxation);
+
+    DetectorResultCounts counts();
+    // record-mode outputs (valid after query(true)); sizes from counts()
+    void vfPairs(int32_t* vf, float* d);
+    void vfContactsAll(int32_t* rec, float* d);
+    void eeContactsAll(int32_t* ee, float* mu, float* d);
+    // capped outputs (every query): per vertex vfCap, per edge eeCap records and the stored count
+    void vfContactsCapped(int32_t* count, VFContactRec* rec);
+    void eeContactsCapped(int32_t* count, EEContactRec* rec);
+    void minDistances(float* vf, float* face, float* edge);
+    void boundsOut(float* b);
+
+    int numVertices() const { return nv_; }
+    int numFaces() const { return nf_; }
+    int numEdges() const { return ne_; }
+    int numMeshes() const { return (int)meshV_.size(); }
+    void meshOffsets(int mesh, int* v, int* f, int* e) const;
+    void topology(int32_t* edges, int32_t* faceEdges) const;
+    size_t deviceBytes() const { return bytes_; }
+    hipStream_t stream() const { return stream_; }
+
+  private:
+    void buildBvh(DeviceBvh& b, int kind);
+    void refitBvh(DeviceBvh& b, int kind);
+    void freeAll();
+    template <class T> T* dalloc(size_t n);
+
+    int device_;
+    hipStream_t stream_;
+    float r_;
+    int vfCap_, eeCap_;
+    bool ready_ = false;
+    size_t bytes_ = 0;
+    std::vector<void*> allocs_;
+    // host topology (global ids)
+    int nv_ = 0, nf_ = 0, ne_ = 0;
+    std::vector<int> meshV_, meshF_, meshE_;  // offsets per mesh (size M + 1 after finalize)
+    std::vector<float> hostPos_;
+    std::vector<int> hFaces_, hFaceEdges_, hEdges_, hVMesh_, hEMesh_;
+    // device
+    float4 *pos_ = nullptr, *rest_ = nullptr;
+    int *vMesh_ = nullptr, *eMesh_ = nullptr;
+    int4 *faces_ = nullptr, *faceEdges_ = nullptr, *edges_ = nullptr;
+    int *vvBegin_ = nullptr, *vvNei_ = nullptr, *vfBegin_ = nullptr, *vfFace_ = nullptr, *veBegin_ = nullptr,
+        *veEdge_ = nullptr;
+    float* drift_ = nullptr;
+    DeviceBvh fb_, eb_;
+    float* sceneBox_ = nullptr;  // 6 floats (as ordered ints during reduction)
+    // outputs
+    float *vfMin_ = nullptr, *eeMin_ = nullptr, *bounds_ = nullptr;
+    int* faceMin_ = nullptr;  // float bits
+    int *vfPairCount_ = nullptr, *vfStored_ = nullptr, *eeCount_ = nullptr, *eeStored_ = nullptr;
+    int *vfOverflow_ = nullptr, *eeOverflow_ = nullptr;
+    VFContactRec* vfRec_ = nullptr;
+    EEContactRec* eeRec_ = nullptr;
+    int64_t *vfOffset_ = nullptr, *eeOffset_ = nullptr;
+    void* scanTemp_ = nullptr;
+    size_t scanTempBytes_ = 0;
+    // record mode buffers (grown on demand)
+    int64_t nRecPairs_ = 0, nRecEE_ = 0, capPairs_ = 0, capEE_ = 0;
+    int2* recVF_ = nullptr;
+    float* recVFd_ = nullptr;
+    int4* recKeys_ = nullptr;  // (v, type, prim, face), type -1 = unused slot
+    float* recKeysD_ = nullptr;
+    int2* recEE_ = nullptr;
+    float2* recEEmu_ = nullptr;
+    float* recEEd_ = nullptr;
+    bool recorded_ = false;
+};
+
+}  // namespace gvbd

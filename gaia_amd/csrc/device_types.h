@@ -34,6 +34,7 @@ static_assert(sizeof(TetInc) == 64, "TetInc must be 64 bytes");
 
 enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
+    SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)
 };
 
 // One incident triangle of a cloth vertex v, relative to v:
@@ -121,5 +122,22 @@ struct WorkItem {
     int32_t count;
     int32_t kind;
 };
+
+// Entry of the flattened (scene, colour, vertex-chunk) work queue of one iteration.
+// kind: WORK_TET / WORK_CLOTH solve chunk, or WORK_COMMIT (pnext -> pos for the cloth
+// slots of one scene colour). The entry may start once `need` earlier entries of its
+// scene have completed in this iteration; entries are ordered by (colour level, scene),
+// so every dependency lies earlier in the queue.
+struct QueueEntry {
+    int32_t scene;
+    int32_t slotBegin;
+    int32_t count;
+    int32_t kind;
+    int32_t need;
+    int32_t perIteration;  // entries of this scene per iteration (progress counters accumulate)
+    int32_t pad[2];
+};
+
+enum : int32_t { WORK_COMMIT = 2 };
 
 }  // namespace gvbd

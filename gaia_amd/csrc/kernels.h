@@ -14,6 +14,24 @@ bool valid_schedule(int lpv, int block);
 // (VBDPhysics.cpp:1610).
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
                               int block, hipStream_t stream);
+// Device arguments of the persistent work-queue solve of a plan.
+struct QueueArgs {
+    const SceneView* scenes;
+    const QueueEntry* queue;
+    int nq;
+    int* tickets;   // one ticket counter per launch (epoch) between counter resets
+    int* progress;  // one completion counter per scene
+    int* status;    // bit 0: a dependency wait exceeded spinLimit
+    int spinLimit;
+};
+// One iteration of every scene of the plan (queue_solve_kernel) on `grid` persistent blocks;
+// `epoch` counts the queue launches since the counters were last reset.
+hipError_t launch_queue_solve(const QueueArgs& a, int iteration, int epoch, int grid, int lpv, int block,
+                              hipStream_t stream);
+// Resident blocks per CU of the queue kernel for a schedule (grid sizing).
+int queue_blocks_per_cu(int lpv, int block);
+// counters[0..n) = 0.
+hipError_t launch_counter_reset(int* counters, int n, hipStream_t stream);
 // Copies pnext -> pos for the cloth slots of `items` (after the group's solves).
 hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 // Items cover vertex ranges [slotBegin, slotBegin + count), count <= 256.

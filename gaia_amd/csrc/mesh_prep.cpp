@@ -307,6 +307,59 @@ struct EdgeTable {
 
 }  // namespace
 
+bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<TriEdge>& edges,
+                     std::vector<int32_t>* faceEdges, std::string& err) {
+    EdgeTable table;
+    std::vector<std::pair<uint64_t, int32_t>> occ;
+    occ.reserve(3 * (size_t)nf);
+    for (int32_t f = 0; f < nf; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const int32_t a = faces[3 * (size_t)f + k], b = faces[3 * (size_t)f + (k + 1) % 3];
+            if (a < 0 || a >= nv || b < 0 || b >= nv) {
+                err = "triangle mesh: vertex index out of range";
+                return false;
+            }
+            occ.push_back({EdgeTable::key(a, b), 3 * f + k});
+        }
+    std::stable_sort(occ.begin(), occ.end(),
+                     [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
+                         return a.first < b.first;
+                     });
+    // first occurrence of each key, in occurrence order
+    std::vector<int32_t> firstOcc;
+    for (size_t i = 0; i < occ.size(); ++i)
+        if (i == 0 || occ[i].first != occ[i - 1].first) firstOcc.push_back(occ[i].second);
+    std::sort(firstOcc.begin(), firstOcc.end());
+    edges.clear();
+    edges.reserve(firstOcc.size());
+    for (int32_t o : firstOcc) {
+        const int32_t f = o / 3, k = o % 3;
+        const int32_t a = faces[3 * (size_t)f + k], b = faces[3 * (size_t)f + (k + 1) % 3],
+                      c = faces[3 * (size_t)f + (k + 2) % 3];
+        table.keys.push_back({EdgeTable::key(a, b), (int32_t)edges.size()});
+        edges.push_back({a, b, c, -1, f, -1});
+    }
+    std::sort(table.keys.begin(), table.keys.end());
+    if (faceEdges) faceEdges->assign(3 * (size_t)nf, -1);
+    for (size_t i = 0; i < occ.size(); ++i) {
+        const int32_t f = occ[i].second / 3, k = occ[i].second % 3;
+        const int32_t a = faces[3 * (size_t)f + k], b = faces[3 * (size_t)f + (k + 1) % 3],
+                      c = faces[3 * (size_t)f + (k + 2) % 3];
+        const int32_t id = table.find(a, b);
+        if (faceEdges) (*faceEdges)[3 * (size_t)f + k] = id;
+        TriEdge& e = edges[id];
+        if (e.f1 == f) continue;
+        if (e.f2 != -1 || e.v1 != b || e.v2 != a) {
+            err = "triangle mesh: non-manifold or inconsistently oriented edge (" + std::to_string(a) + ", " +
+                  std::to_string(b) + ")";
+            return false;
+        }
+        e.f2 = f;
+        e.v21Next = c;
+    }
+    return true;
+}
+
 bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err) {
     if ((int64_t)m.pos.size() != 3ll * m.nv || (int64_t)m.faces.size() != 3ll * m.nf) {
         err = "cloth mesh: array sizes do not match vertex/face counts";
@@ -355,48 +408,8 @@ bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err) {
         for (int k = 0; k < 3; ++k) m.mass[fv[k]] += m.mat.density * m.area[f] / 3.f;
     }
     // edges, numbered by first appearance
-    EdgeTable table;
-    {
-        std::vector<std::pair<uint64_t, int32_t>> occ;
-        occ.reserve(3 * (size_t)m.nf);
-        for (int32_t f = 0; f < m.nf; ++f)
-            for (int k = 0; k < 3; ++k)
-                occ.push_back({EdgeTable::key(m.faces[3 * (size_t)f + k], m.faces[3 * (size_t)f + (k + 1) % 3]),
-                               3 * f + k});
-        std::stable_sort(occ.begin(), occ.end(),
-                         [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
-                             return a.first < b.first;
-                         });
-        // first occurrence of each key, in occurrence order
-        std::vector<int32_t> firstOcc;
-        for (size_t i = 0; i < occ.size(); ++i)
-            if (i == 0 || occ[i].first != occ[i - 1].first) firstOcc.push_back(occ[i].second);
-        std::sort(firstOcc.begin(), firstOcc.end());
-        m.edges.clear();
-        m.edges.reserve(firstOcc.size());
-        for (int32_t o : firstOcc) {
-            const int32_t f = o / 3, k = o % 3;
-            const int32_t a = m.faces[3 * (size_t)f + k], b = m.faces[3 * (size_t)f + (k + 1) % 3],
-                          c = m.faces[3 * (size_t)f + (k + 2) % 3];
-            table.keys.push_back({EdgeTable::key(a, b), (int32_t)m.edges.size()});
-            m.edges.push_back({a, b, c, -1, f, -1});
-        }
-        std::sort(table.keys.begin(), table.keys.end());
-        for (size_t i = 0; i < occ.size(); ++i) {
-            const int32_t f = occ[i].second / 3, k = occ[i].second % 3;
-            const int32_t a = m.faces[3 * (size_t)f + k], b = m.faces[3 * (size_t)f + (k + 1) % 3],
-                          c = m.faces[3 * (size_t)f + (k + 2) % 3];
-            TriEdge& e = m.edges[table.find(a, b)];
-            if (e.f1 == f) continue;
-            if (e.f2 != -1 || e.v1 != b || e.v2 != a) {
-                err = "cloth mesh: non-manifold or inconsistently oriented edge (" + std::to_string(a) + ", " +
-                      std::to_string(b) + ")";
-                return false;
-            }
-            e.f2 = f;
-            e.v21Next = c;
-        }
-    }
+    std::vector<int32_t> faceEdges;
+    if (!build_tri_edges(m.nv, m.nf, m.faces.data(), m.edges, &faceEdges, err)) return false;
     // vertex -> faces
     m.vfBegin.assign(m.nv + 1, 0);
     for (int32_t v : m.faces) m.vfBegin[v + 1]++;
@@ -439,7 +452,7 @@ bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err) {
         }
         for (int32_t k = m.vfBegin[v]; k < m.vfBegin[v + 1]; ++k) {
             const int32_t f = m.vfFace[k], o = m.vfOrder[k];
-            const int32_t id = table.find(m.faces[3 * (size_t)f + (o + 1) % 3], m.faces[3 * (size_t)f + (o + 2) % 3]);
+            const int32_t id = faceEdges[3 * (size_t)f + (o + 1) % 3];  // edge (o+1, o+2) of the face
             const TriEdge& e = m.edges[id];
             if (e.f2 == -1) continue;
             m.vbEdge.push_back(id);

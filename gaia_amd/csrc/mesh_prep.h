@@ -112,6 +112,14 @@ struct ClothMeshHost {
 // (Q = 3 K0 K0^T / (|n1| + |n2|)). Fails on degenerate faces or non-manifold / inconsistently
 // oriented meshes.
 bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err);
+
+// Edges of a triangle mesh numbered by first appearance over the faces (TriMesh.cpp:481-560):
+// v1 -> v2 is the edge's direction in its first face f1, v12Next that face's third vertex,
+// v21Next / f2 the other face (-1 on boundary edges). faceEdges (optional, 3 per face) receives
+// the ids of edges (v0 v1), (v1 v2), (v2 v0) of each face (faces3NeighborEdges order).
+// Fails on out-of-range indices and non-manifold or inconsistently oriented meshes.
+bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<TriEdge>& edges,
+                     std::vector<int32_t>* faceEdges, std::string& err);
 // TriMeshVertexGraph.h:20-45: mesh edges; with bending also the two vertices opposite each
 // interior edge.
 Graph tri_vertex_graph(int32_t nv, int32_t nf, const int32_t* faces, bool withBending);

@@ -332,7 +332,8 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     f3 dx;
     f3 xn = x;
     if (solve_sym3(h, f, dx)) xn = x + sp.clothStepSize * dx;
-    S.pnext[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
+    float4* dst = (sa.w & SLOT_INPLACE) ? S.pos : S.pnext;
+    dst[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
 }
 
 }  // namespace
@@ -360,6 +361,99 @@ __global__ __launch_bounds__(256) void cloth_commit_kernel(const SceneView* __re
         const int v = S.cslots[it.slotBegin + i].a.x;
         S.pos[v] = S.pnext[v];
     }
+}
+
+// One whole VBD iteration of every scene of a plan in one persistent launch.
+//
+// The queue holds (scene, colour level, slot chunk) entries ordered by (level, scene); an
+// entry may run once `need + epoch * perIteration` entries of its scene have completed
+// (per-scene counters accumulate over the launches (epochs) since counter_reset_kernel last
+// cleared them; `iteration` is the VBD iteration index, used for the friction start). Blocks
+// take entries in queue order from the epoch's ticket counter, so every entry a block waits for
+// was taken earlier by a block that is
+// already running: the lowest unfinished entry never waits, and the grid drains without
+// co-residency assumptions. A wait longer than `spinLimit` polls sets *status and proceeds, so
+// no wave can spin forever; the host reports the flag as an error.
+//
+// Within a scene this keeps the reference's order exactly (colour g of iteration i sees colours
+// < g of iteration i and colours > g of iteration i - 1), so results are bitwise those of one
+// launch per parallel group; across scenes there is no barrier, so the tail of one colour
+// overlaps the head of the next.
+template <int LPV, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void queue_solve_kernel(const SceneView* __restrict__ scenes,
+                                                           const QueueEntry* __restrict__ queue, int nq,
+                                                           int* __restrict__ ticket, int* __restrict__ progress,
+                                                           int iteration, int epoch, int* __restrict__ status,
+                                                           int spinLimit) {
+    constexpr int VPB = BLOCK / LPV;
+    __shared__ int sTicket;
+    // The scheduling work (count the previous entry, take a ticket, wait for its dependencies)
+    // is done by the whole first wave, with lane 0 alone issuing the counter atomics: every
+    // loop and branch around a barrier is then uniform within each wave, so no lane can be
+    // parked in a loop that other lanes of its wave keep executing.
+    const bool wave0 = threadIdx.x < 64;
+    const int lane = threadIdx.x & 63;
+    int prevScene = -1;
+    for (;;) {
+        if (wave0) {
+            if (prevScene >= 0 && lane == 0)
+                __hip_atomic_fetch_add(progress + prevScene, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int t = 0;
+            if (lane == 0) t = atomicAdd(ticket, 1);
+            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+            if (t < nq) {
+                const int scene = queue[t].scene;
+                const int target = queue[t].need + epoch * queue[t].perIteration;
+                for (int spins = 0;; ++spins) {
+                    const int done = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(progress + scene, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    if (done >= target) break;
+                    if (spins >= spinLimit) {
+                        if (lane == 0) atomicOr(status, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            if (lane == 0) sTicket = t;
+        }
+        // wave 0's agent-scope acquire happens before every thread's loads below through this
+        // barrier (workgroup-scope synchronisation)
+        __syncthreads();
+        const int t = sTicket;
+        if (t >= nq) break;
+        const QueueEntry e = queue[t];
+        const SceneView& S = scenes[e.scene];
+        if (e.kind == WORK_COMMIT) {
+            for (int i = threadIdx.x; i < e.count; i += BLOCK) {
+                const int v = S.cslots[e.slotBegin + i].a.x;
+                S.pos[v] = S.pnext[v];
+            }
+        } else {
+            for (int p = 0; p < e.count; p += VPB) {
+                const WorkItem it{e.scene, e.slotBegin + p, min(VPB, e.count - p), e.kind};
+                if (e.kind == WORK_TET)
+                    tet_block<LPV>(S, it, iteration);
+                else
+                    cloth_block<LPV>(S, it);
+            }
+        }
+        // every thread's stores happen before lane 0's agent-scope release (the completion count
+        // at the top of the next round) through this barrier: each wave first waits until its own
+        // stores are acknowledged by L2 (s_waitcnt vmcnt(0); expcnt and lgkmcnt left free), so the
+        // L2 write-back of that release covers them. The barrier also keeps sTicket until every
+        // thread has read it.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        prevScene = e.scene;
+    }
+}
+
+// Clears the work-queue counters (a kernel rather than a memset so that it can sit in a
+// captured graph).
+__global__ __launch_bounds__(256) void counter_reset_kernel(int* __restrict__ counters, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counters[i] = 0;
 }
 
 // VBD_BaseMaterial.h:244-259 applyInitialStep (computeInertia + forwardStep) and
@@ -452,6 +546,54 @@ hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, in
     GVBD_CASE(16, 64) GVBD_CASE(16, 128) GVBD_CASE(16, 256)
 #undef GVBD_CASE
     return hipErrorInvalidValue;
+}
+
+template <int LPV, int BLOCK>
+static hipError_t launch_queue(const QueueArgs& a, int iteration, int epoch, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL((queue_solve_kernel<LPV, BLOCK>), dim3(grid), dim3(BLOCK), 0, stream, a.scenes, a.queue,
+                       a.nq, a.tickets + epoch, a.progress, iteration, epoch, a.status, a.spinLimit);
+    return hipGetLastError();
+}
+
+template <int LPV, int BLOCK>
+static int queue_occupancy() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, queue_solve_kernel<LPV, BLOCK>, BLOCK, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+
+#define GVBD_SCHEDULES(X)                                        \
+    X(1, 64) X(1, 128) X(1, 256) X(2, 64) X(2, 128) X(2, 256)    \
+    X(4, 64) X(4, 128) X(4, 256) X(8, 64) X(8, 128) X(8, 256)    \
+    X(16, 64) X(16, 128) X(16, 256)
+
+hipError_t launch_queue_solve(const QueueArgs& a, int iteration, int epoch, int grid, int lpv, int block,
+                              hipStream_t stream) {
+    if (a.nq <= 0) return hipSuccess;
+    if (grid <= 0) return hipErrorInvalidValue;
+#define GVBD_CASE(L, B) \
+    if (lpv == L && block == B) return launch_queue<L, B>(a, iteration, epoch, grid, stream);
+    GVBD_SCHEDULES(GVBD_CASE)
+#undef GVBD_CASE
+    return hipErrorInvalidValue;
+}
+
+int queue_blocks_per_cu(int lpv, int block) {
+#define GVBD_CASE(L, B) \
+    if (lpv == L && block == B) return queue_occupancy<L, B>();
+    GVBD_SCHEDULES(GVBD_CASE)
+#undef GVBD_CASE
+    return 1;
+}
+
+hipError_t launch_counter_reset(int* counters, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int blocks = std::min((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(counter_reset_kernel, dim3(blocks), dim3(256), 0, stream, counters, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream) {

@@ -11,7 +11,9 @@
 #include <vector>
 
 #include "../../include/gaia_vbd.h"
+#include "detector.h"
 #include "device_types.h"
+#include "errors.h"
 #include "kernels.h"
 #include "mesh_prep.h"
 #include "scene_loader.h"
@@ -21,21 +23,6 @@ namespace gvbd {
 namespace {
 
 thread_local std::string g_lastError;
-
-struct Failure {
-    int code;
-    std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw Failure{code, msg}; }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) fail(GVBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-void require(bool cond, const char* msg) {
-    if (!cond) fail(GVBD_ERR_INVALID, msg);
-}
 
 constexpr int kVertexChunk = 256;
 
@@ -67,6 +54,7 @@ struct SceneObj {
     std::vector<int32_t> tetGroupOffset;       // G + 1, into tet slots
     std::vector<int32_t> clothGroupOffset;     // G + 1, into cloth slots
     std::vector<int64_t> slotWorkBytes;        // algorithmic bytes per tet slot then per cloth slot
+    std::vector<uint8_t> clothSlotInplace;     // per cloth slot: solved in place (no commit pass)
     DeviceScene dev;
     SceneView view{};
     int32_t plan = -1;  // the scene's own single-scene plan
@@ -77,9 +65,16 @@ struct SceneObj {
     size_t numTetSlots() const { return tetGroupOffset.empty() ? 0 : (size_t)tetGroupOffset.back(); }
 };
 
+constexpr int kCommitChunk = 1024;
+constexpr int kDefaultSpinLimit = 1 << 20;
+
 struct PlanObj {
     std::vector<int32_t> scenes;
     int lpv = 4, block = 256;
+    int mode = GVBD_MODE_GROUP_LAUNCHES;
+    int vertsPerEntry = 0;  // work-queue entry size (0: 4 passes of block / lpv vertices)
+    int blocksPerCu = 0;    // persistent grid per CU (0: resident maximum)
+    int spinLimit = kDefaultSpinLimit;
     int numGroups = 0, iterations = 0, substeps = 0;
     SceneView* dViews = nullptr;
     WorkItem* dItems = nullptr;   // per group: solve items (tet and cloth)
@@ -88,6 +83,12 @@ struct PlanObj {
     std::vector<int32_t> groupItemBegin;    // G + 1
     std::vector<int32_t> groupCommitBegin;  // G + 1
     int32_t nVItems = 0;
+    // work queue (mode GVBD_MODE_WORK_QUEUE)
+    QueueEntry* dQueue = nullptr;
+    int* dCounters = nullptr;  // tickets[iterations] then progress[scenes]
+    int32_t nQueue = 0, nCounters = 0, queueGrid = 0;
+    int32_t ticketSlots = 0;  // queue launches allowed between counter resets
+    int32_t epoch = 0;        // queue launches since the last reset (host-issued iterations)
     size_t bytes = 0;
     bool wantGraph = false;
     hipGraph_t graph = nullptr;
@@ -100,7 +101,9 @@ struct PlanObj {
 
 struct gvbd_context {
     int device = 0;
+    int numCUs = 0;
     hipStream_t stream = nullptr;
+    int* dStatus = nullptr;  // set by work-queue kernels when a dependency wait exceeds its limit
     hipEvent_t events[64] = {};
     std::string lastError;
     std::map<int32_t, std::unique_ptr<gvbd::SceneObj>> scenes;
@@ -157,22 +160,36 @@ SolveParams make_solve_params(const gvbd_physics_params& p, const SceneObj& s) {
 }
 
 void free_plan_graph(PlanObj& p) {
-    if (p.exec) hipGraphExecDestroy(p.exec);
-    if (p.graph) hipGraphDestroy(p.graph);
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
     p.exec = nullptr;
     p.graph = nullptr;
 }
 
 void free_plan(PlanObj& p) {
     free_plan_graph(p);
-    for (void* ptr : {(void*)p.dViews, (void*)p.dItems, (void*)p.dCommit, (void*)p.dVItems})
+    for (void* ptr : {(void*)p.dViews, (void*)p.dItems, (void*)p.dCommit, (void*)p.dVItems, (void*)p.dQueue,
+                      (void*)p.dCounters})
         if (ptr) (void)hipFree(ptr);
     p.dViews = nullptr;
     p.dItems = nullptr;
     p.dCommit = nullptr;
     p.dVItems = nullptr;
+    p.dQueue = nullptr;
+    p.dCounters = nullptr;
+    p.nQueue = p.nCounters = p.queueGrid = p.ticketSlots = p.epoch = 0;
     p.bytes = 0;
     p.dirty = true;
+}
+
+// Work-queue status: reports (and clears) a dependency wait that hit its spin limit.
+void check_queue_status(gvbd_context* ctx) {
+    int st = 0;
+    hip_check(hipMemcpy(&st, ctx->dStatus, sizeof(int), hipMemcpyDeviceToHost), "status read");
+    if (st != 0) {
+        hip_check(hipMemset(ctx->dStatus, 0, sizeof(int)), "status clear");
+        fail(GVBD_ERR_HIP, "work-queue dependency wait exceeded its spin limit; the affected steps are invalid");
+    }
 }
 
 // Colour-major tet slot and incidence records of one vertex.
@@ -214,7 +231,7 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
 }
 
 // Colour-major cloth slot, face and bending records of one vertex.
-void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, std::vector<ClothSlot>& slots,
+void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, bool inplace, std::vector<ClothSlot>& slots,
                      std::vector<FaceInc>& faces, std::vector<BendInc>& bends, std::vector<int64_t>& work) {
     ClothSlot cs{};
     cs.a.x = off + lv;
@@ -243,7 +260,7 @@ void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, std::vecto
         faces.push_back(rec);
     }
     cs.a.z = (int32_t)faces.size();
-    cs.a.w = 0;
+    cs.a.w = inplace ? SLOT_INPLACE : 0;
     cs.b.x = (int32_t)bends.size();
     const float kb = m.mat.bendingStiffness;
     for (int32_t k = m.vbBegin[lv]; k < m.vbBegin[lv + 1]; ++k) {
@@ -288,16 +305,32 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
                                                         : s.cloths[s.meshes[im].second].fixed;
         for (int32_t f : fx) fixedMask[im][f] = 1;
     }
+    s.clothSlotInplace.clear();
     for (int32_t g = 0; g < G; ++g) {
         const auto& grp = s.groups[g];
+        // A cloth mesh with a single colour in this group can be updated in place: its solved
+        // vertices share no element, so the reference's positionsNext + copy (Jacobi over the
+        // group) gives the same result. Without contact, meshes do not interact.
+        std::vector<int32_t> firstColor(s.meshes.size(), -1);
+        std::vector<uint8_t> multiColor(s.meshes.size(), 0);
+        for (size_t i = 0; i + 1 < grp.size(); i += 2) {
+            const int32_t im = grp[i], lv = grp[i + 1];
+            if (s.meshes[im].first != MESH_CLOTH) continue;
+            const int32_t c = s.cloths[s.meshes[im].second].colors[lv];
+            if (firstColor[im] < 0) firstColor[im] = c;
+            else if (firstColor[im] != c) multiColor[im] = 1;
+        }
         for (size_t i = 0; i + 1 < grp.size(); i += 2) {
             const int32_t im = grp[i], lv = grp[i + 1];
             if (fixedMask[im][lv]) continue;
             const int32_t off = s.vertexOffset[im];
-            if (s.meshes[im].first == MESH_TET)
+            if (s.meshes[im].first == MESH_TET) {
                 emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork);
-            else
-                emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, cslots, cface, cbend, clothWork);
+            } else {
+                const bool inplace = !multiColor[im] && !s.params.handle_collision;
+                emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, inplace, cslots, cface, cbend, clothWork);
+                s.clothSlotInplace.push_back(inplace ? 1 : 0);
+            }
         }
         s.tetGroupOffset[g + 1] = (int32_t)slots.size();
         s.clothGroupOffset[g + 1] = (int32_t)cslots.size();
@@ -434,6 +467,20 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     std::vector<WorkItem> items, commits, vitems;
     p.groupItemBegin.assign(p.numGroups + 1, 0);
     p.groupCommitBegin.assign(p.numGroups + 1, 0);
+    // runs of consecutive cloth slots of group g that need the positionsNext copy
+    auto commit_runs = [](const SceneObj& s, int g, int chunk, auto&& emit) {
+        const int32_t e = s.clothGroupOffset[g + 1];
+        for (int32_t k = s.clothGroupOffset[g]; k < e;) {
+            if (s.clothSlotInplace[k]) {
+                ++k;
+                continue;
+            }
+            int32_t r = k;
+            while (r < e && !s.clothSlotInplace[r] && r - k < chunk) ++r;
+            emit(k, r - k);
+            k = r;
+        }
+    };
     for (int g = 0; g < p.numGroups; ++g) {
         for (size_t si = 0; si < p.scenes.size(); ++si) {
             SceneObj& s = scene_of(ctx, p.scenes[si]);
@@ -442,11 +489,51 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
                 items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_TET});
             for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += vpb)
                 items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
-            for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += kVertexChunk)
-                commits.push_back({(int32_t)si, k, std::min(kVertexChunk, e - k), WORK_CLOTH});
+            commit_runs(s, g, kVertexChunk,
+                        [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
         }
         p.groupItemBegin[g + 1] = (int32_t)items.size();
         p.groupCommitBegin[g + 1] = (int32_t)commits.size();
+    }
+    // Work queue of one iteration, ordered by (level, scene): solve entries of level g of every
+    // scene, then the level's commit entries. Solve entries of (s, g) need every entry of scene s
+    // in levels < g; commits of (s, g) also need the level's solves of s.
+    std::vector<QueueEntry> queue;
+    if (p.mode == GVBD_MODE_WORK_QUEUE) {
+        const int per = p.vertsPerEntry > 0 ? p.vertsPerEntry : 4 * vpb;
+        const size_t nS = p.scenes.size();
+        std::vector<int32_t> done(nS, 0), levelSolves(nS, 0);
+        for (int g = 0; g < p.numGroups; ++g) {
+            for (size_t si = 0; si < nS; ++si) {
+                SceneObj& s = scene_of(ctx, p.scenes[si]);
+                levelSolves[si] = 0;
+                if (g >= (int)s.groups.size()) continue;
+                auto solves = [&](int32_t b, int32_t e, int32_t kind) {
+                    for (int32_t k = b; k < e; k += per) {
+                        queue.push_back({(int32_t)si, k, std::min(per, e - k), kind, done[si], 0, {0, 0}});
+                        ++levelSolves[si];
+                    }
+                };
+                solves(s.tetGroupOffset[g], s.tetGroupOffset[g + 1], WORK_TET);
+                solves(s.clothGroupOffset[g], s.clothGroupOffset[g + 1], WORK_CLOTH);
+            }
+            std::vector<int32_t> levelCommits(nS, 0);
+            for (size_t si = 0; si < nS; ++si) {
+                SceneObj& s = scene_of(ctx, p.scenes[si]);
+                if (g >= (int)s.groups.size()) continue;
+                commit_runs(s, g, kCommitChunk, [&](int32_t k, int32_t n) {
+                    queue.push_back({(int32_t)si, k, n, WORK_COMMIT, done[si] + levelSolves[si], 0, {0, 0}});
+                    ++levelCommits[si];
+                });
+            }
+            for (size_t si = 0; si < nS; ++si) done[si] += levelSolves[si] + levelCommits[si];
+        }
+        for (QueueEntry& q : queue) q.perIteration = done[q.scene];
+        p.nQueue = (int32_t)queue.size();
+        p.ticketSlots = std::max(p.iterations, 64);
+        p.nCounters = p.ticketSlots + (int32_t)nS;
+        const int perCu = p.blocksPerCu > 0 ? p.blocksPerCu : queue_blocks_per_cu(p.lpv, p.block);
+        p.queueGrid = std::max(1, std::min(p.nQueue, perCu * std::max(ctx->numCUs, 1)));
     }
     for (size_t si = 0; si < p.scenes.size(); ++si) {
         SceneObj& s = scene_of(ctx, p.scenes[si]);
@@ -467,12 +554,21 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     alloc_up((void**)&p.dItems, items.data(), sizeof(WorkItem) * items.size());
     alloc_up((void**)&p.dCommit, commits.data(), sizeof(WorkItem) * commits.size());
     alloc_up((void**)&p.dVItems, vitems.data(), sizeof(WorkItem) * vitems.size());
+    if (p.mode == GVBD_MODE_WORK_QUEUE) {
+        alloc_up((void**)&p.dQueue, queue.data(), sizeof(QueueEntry) * queue.size());
+        std::vector<int> zeros(p.nCounters, 0);
+        alloc_up((void**)&p.dCounters, zeros.data(), sizeof(int) * zeros.size());
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "plan upload sync");
     p.dirty = false;
 }
 
+void plan_reset_counters(gvbd_context* ctx, PlanObj& p);
+
+// Initial step of a substep; in work-queue mode it also starts a new counter epoch.
 void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
     hip_check(launch_initial_step(p.dViews, p.dVItems, p.nVItems, ctx->stream), "initial step launch");
+    if (p.mode == GVBD_MODE_WORK_QUEUE) plan_reset_counters(ctx, p);
 }
 
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
@@ -487,15 +583,36 @@ void plan_velocity(gvbd_context* ctx, PlanObj& p) {
     hip_check(launch_update_velocity(p.dViews, p.dVItems, p.nVItems, ctx->stream), "velocity launch");
 }
 
+QueueArgs queue_args(gvbd_context* ctx, PlanObj& p) {
+    return QueueArgs{p.dViews, p.dQueue, p.nQueue, p.dCounters, p.dCounters + p.ticketSlots, ctx->dStatus,
+                     p.spinLimit};
+}
+
+void plan_reset_counters(gvbd_context* ctx, PlanObj& p) {
+    hip_check(launch_counter_reset(p.dCounters, p.nCounters, ctx->stream), "counter reset launch");
+    p.epoch = 0;
+}
+
+// One iteration of every scene: one persistent launch (work queue, the `epoch`-th launch since
+// the counters were reset) or one launch per parallel group.
+void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
+    if (p.mode == GVBD_MODE_WORK_QUEUE) {
+        hip_check(launch_queue_solve(queue_args(ctx, p), it, epoch, p.queueGrid, p.lpv, p.block, ctx->stream),
+                  "work-queue solve launch");
+    } else {
+        for (int g = 0; g < p.numGroups; ++g) plan_group(ctx, p, g, it);
+    }
+}
+
 // One frame: substeps x (initial step, iterations x groups, velocity update)
 // (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
     for (int sub = 0; sub < p.substeps; ++sub) {
         plan_initial_step(ctx, p);
-        for (int it = 0; it < p.iterations; ++it)
-            for (int g = 0; g < p.numGroups; ++g) plan_group(ctx, p, g, it);
+        for (int it = 0; it < p.iterations; ++it) plan_iteration(ctx, p, it, it);
         plan_velocity(ctx, p);
     }
+    p.epoch = p.iterations;  // device counters after the last substep
 }
 
 void plan_capture(gvbd_context* ctx, PlanObj& p) {
@@ -506,7 +623,7 @@ void plan_capture(gvbd_context* ctx, PlanObj& p) {
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(ctx->stream, &g);
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         throw;
     }
     hip_check(hipStreamEndCapture(ctx->stream, &p.graph), "end capture");
@@ -522,6 +639,7 @@ void plan_run(gvbd_context* ctx, PlanObj& p, int n) {
         else
             plan_frame_launches(ctx, p);
     }
+    p.epoch = p.iterations;
 }
 
 int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
@@ -623,10 +741,11 @@ using namespace gvbd;
 extern "C" {
 
 int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
-                          int32_t* cloth_mesh_desc_bytes) {
+                          int32_t* cloth_mesh_desc_bytes, int32_t* schedule_bytes) {
     if (physics_params_bytes) *physics_params_bytes = (int32_t)sizeof(gvbd_physics_params);
     if (tet_mesh_desc_bytes) *tet_mesh_desc_bytes = (int32_t)sizeof(gvbd_tet_mesh_desc);
     if (cloth_mesh_desc_bytes) *cloth_mesh_desc_bytes = (int32_t)sizeof(gvbd_cloth_mesh_desc);
+    if (schedule_bytes) *schedule_bytes = (int32_t)sizeof(gvbd_schedule);
     return GVBD_OK;
 }
 
@@ -639,8 +758,12 @@ int gvbd_context_create(int32_t device, gvbd_context** out) {
         auto ctx = std::make_unique<gvbd_context>();
         ctx->device = device;
         hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device),
+                  "hipDeviceGetAttribute");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ctx->events) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipMalloc((void**)&ctx->dStatus, 256), "hipMalloc status");
+        hip_check(hipMemset(ctx->dStatus, 0, 256), "status clear");
         *out = ctx.release();
     });
 }
@@ -654,6 +777,7 @@ int gvbd_context_destroy(gvbd_context* ctx) {
         if (kv.second->dev.arena) (void)hipFree(kv.second->dev.arena);
     for (auto& e : ctx->events)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->dStatus) (void)hipFree(ctx->dStatus);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GVBD_OK;
@@ -665,6 +789,7 @@ int gvbd_synchronize(gvbd_context* ctx) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        check_queue_status(ctx);
     });
 }
 
@@ -897,17 +1022,79 @@ int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, fl
     });
 }
 
+static PlanObj& scene_plan(gvbd_context* ctx, int32_t scene) {
+    require(ctx != nullptr, "ctx is NULL");
+    SceneObj& s = scene_of(ctx, scene);
+    if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
+    return plan_of(ctx, s.plan);
+}
+
+static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
+    require(valid_schedule(sc.lanes_per_vertex, sc.block),
+            "schedule must be lanes_per_vertex in {1,2,4,8,16}, block in {64,128,256}");
+    require(sc.mode == GVBD_MODE_GROUP_LAUNCHES || sc.mode == GVBD_MODE_WORK_QUEUE, "unknown schedule mode");
+    const int vpb = sc.block / sc.lanes_per_vertex;
+    require(sc.vertices_per_entry >= 0 && sc.vertices_per_entry <= 65536 && sc.vertices_per_entry % vpb == 0,
+            "vertices_per_entry must be a multiple of block / lanes_per_vertex (at most 65536)");
+    require(sc.blocks_per_cu >= 0 && sc.blocks_per_cu <= 32, "blocks_per_cu must be in [0, 32]");
+    require(sc.spin_limit >= 0, "spin_limit must be non-negative");
+    p.lpv = sc.lanes_per_vertex;
+    p.block = sc.block;
+    p.mode = sc.mode;
+    p.vertsPerEntry = sc.vertices_per_entry;
+    p.blocksPerCu = sc.blocks_per_cu;
+    p.spinLimit = sc.spin_limit > 0 ? sc.spin_limit : kDefaultSpinLimit;
+    free_plan(p);
+}
+
+static void get_schedule(const PlanObj& p, gvbd_schedule* sc) {
+    require(sc != nullptr, "schedule is NULL");
+    *sc = gvbd_schedule{};
+    sc->lanes_per_vertex = p.lpv;
+    sc->block = p.block;
+    sc->mode = p.mode;
+    sc->vertices_per_entry = p.vertsPerEntry;
+    sc->blocks_per_cu = p.blocksPerCu;
+    sc->spin_limit = p.spinLimit;
+}
+
+// One VBD iteration issued by the host: in work-queue mode it is the next epoch of the
+// counters (a fresh epoch range is started when the ticket slots run out), so it is exactly
+// one queue launch.
+static void run_single_iteration(gvbd_context* ctx, PlanObj& p, int32_t iteration) {
+    require(iteration >= 0, "iteration must be non-negative");
+    build_plan(ctx, p);
+    if (p.mode == GVBD_MODE_WORK_QUEUE && p.epoch >= p.ticketSlots) plan_reset_counters(ctx, p);
+    const int epoch = p.epoch;
+    plan_iteration(ctx, p, iteration, epoch);
+    if (p.mode == GVBD_MODE_WORK_QUEUE) p.epoch = epoch + 1;
+}
+
 int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_vertex, int32_t block) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        require(valid_schedule(lanes_per_vertex, block), "schedule must be lpv in {1,2,4,8,16}, block in {64,128,256}");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        PlanObj& p = plan_of(ctx, s.plan);
-        p.lpv = lanes_per_vertex;
-        p.block = block;
-        free_plan(p);
+        PlanObj& p = scene_plan(ctx, scene);
+        gvbd_schedule sc;
+        get_schedule(p, &sc);
+        sc.lanes_per_vertex = lanes_per_vertex;
+        sc.block = block;
+        sc.vertices_per_entry = 0;
+        set_schedule(p, sc);
     });
+}
+
+int gvbd_scene_set_schedule_ex(gvbd_context* ctx, int32_t scene, const gvbd_schedule* schedule) {
+    return guard(ctx, [&] {
+        require(schedule != nullptr, "schedule is NULL");
+        set_schedule(scene_plan(ctx, scene), *schedule);
+    });
+}
+
+int gvbd_scene_get_schedule(gvbd_context* ctx, int32_t scene, gvbd_schedule* schedule) {
+    return guard(ctx, [&] { get_schedule(scene_plan(ctx, scene), schedule); });
+}
+
+int gvbd_scene_run_iteration(gvbd_context* ctx, int32_t scene, int32_t iteration) {
+    return guard(ctx, [&] { run_single_iteration(ctx, scene_plan(ctx, scene), iteration); });
 }
 
 int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene) { return gvbd_scene_run_steps(ctx, scene, 1); }
@@ -923,10 +1110,7 @@ int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n) {
 
 int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        PlanObj& p = plan_of(ctx, s.plan);
+        PlanObj& p = scene_plan(ctx, scene);
         build_plan(ctx, p);
         plan_initial_step(ctx, p);
     });
@@ -978,6 +1162,7 @@ static void mesh_state_io(gvbd_context* ctx, int32_t scene, int32_t mesh, float*
     const int32_t off = s.vertexOffset[mesh], n = s.meshVertices(mesh);
     std::vector<float4> tmp(n);
     hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    check_queue_status(ctx);
     float4* dev[2] = {s.dev.pos + off, s.dev.vel + off};
     float* host[2] = {pos, vel};
     for (int a = 0; a < 2; ++a) {
@@ -1035,11 +1220,62 @@ int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch) {
 int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_vertex, int32_t block) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
-        require(valid_schedule(lanes_per_vertex, block), "schedule must be lpv in {1,2,4,8,16}, block in {64,128,256}");
         PlanObj& p = batch_plan(ctx, batch);
-        p.lpv = lanes_per_vertex;
-        p.block = block;
-        free_plan(p);
+        gvbd_schedule sc;
+        get_schedule(p, &sc);
+        sc.lanes_per_vertex = lanes_per_vertex;
+        sc.block = block;
+        sc.vertices_per_entry = 0;
+        set_schedule(p, sc);
+    });
+}
+
+int gvbd_batch_set_schedule_ex(gvbd_context* ctx, int32_t batch, const gvbd_schedule* schedule) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr && schedule != nullptr, "NULL argument");
+        set_schedule(batch_plan(ctx, batch), *schedule);
+    });
+}
+
+int gvbd_batch_get_schedule(gvbd_context* ctx, int32_t batch, gvbd_schedule* schedule) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        get_schedule(batch_plan(ctx, batch), schedule);
+    });
+}
+
+int gvbd_batch_queue_info(gvbd_context* ctx, int32_t batch, int32_t* entries, int32_t* grid) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        PlanObj& p = batch_plan(ctx, batch);
+        build_plan(ctx, p);
+        if (entries) *entries = p.nQueue;
+        if (grid) *grid = p.queueGrid;
+    });
+}
+
+int gvbd_batch_apply_initial_step(gvbd_context* ctx, int32_t batch) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        PlanObj& p = batch_plan(ctx, batch);
+        build_plan(ctx, p);
+        plan_initial_step(ctx, p);
+    });
+}
+
+int gvbd_batch_run_iteration(gvbd_context* ctx, int32_t batch, int32_t iteration) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        run_single_iteration(ctx, batch_plan(ctx, batch), iteration);
+    });
+}
+
+int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        PlanObj& p = batch_plan(ctx, batch);
+        build_plan(ctx, p);
+        plan_velocity(ctx, p);
     });
 }
 

@@ -210,3 +210,51 @@ def write_scene_json(folder, name, pos, tets, physics, material, colors=None, tr
     with open(ppath, "w") as f:
         json.dump(params, f, indent=1)
     return mpath, ppath
+
+
+def torus(nu=128, nv=64, major=0.08, minor=0.03, dtype=np.float64):
+    """Closed torus triangle mesh: vertex (i, j) = i nv + j at angles u_i = 2 pi i / nu (around
+    the axis z) and v_j = 2 pi j / nv (around the tube); every quad (i, j) (i+1, j) (i+1, j+1)
+    (i, j+1) is split into two consistently oriented triangles. nu nv vertices, 2 nu nv faces,
+    3 nu nv edges."""
+    u = 2.0 * np.pi * np.arange(nu) / nu
+    v = 2.0 * np.pi * np.arange(nv) / nv
+    U, V = np.meshgrid(u, v, indexing="ij")
+    ring = major + minor * np.cos(V)
+    pos = np.stack([ring * np.cos(U), ring * np.sin(U), minor * np.sin(V)], axis=-1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    faces = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], axis=2).reshape(-1, 3)
+    return pos.astype(dtype), faces.astype(np.int32)
+
+
+def random_rotation(rng):
+    """Rotation from three seeded angles, each uniform in [0, 2 pi) (z-y-x Euler)."""
+    a, b, c = rng.uniform(0.0, 2.0 * np.pi, 3)
+    rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    rx = np.array([[1, 0, 0], [0, np.cos(c), -np.sin(c)], [0, np.sin(c), np.cos(c)]])
+    return rz @ ry @ rx
+
+
+def c3_tori(seed=0, grid=4, nu=128, nv=64, spacing=0.3, major=0.12, minor=0.04, drift=0.005):
+    """Collision-detection workload: grid^3 tori (default 64 x 16,384 triangles, edges of about
+    4 mm) on a lattice with `spacing` between centres (neighbouring tori of outer radius 0.16 m
+    interpenetrate),
+    each with a seeded random rotation and a seeded random drift direction of `drift` metres
+    per step. Returns (meshes [(positions, faces)], drift vectors (M, 3))."""
+    rng = np.random.default_rng(seed)
+    base, faces = torus(nu, nv, major, minor)
+    meshes, drifts = [], []
+    for ix in range(grid):
+        for iy in range(grid):
+            for iz in range(grid):
+                R = random_rotation(rng)
+                centre = spacing * np.array([ix, iy, iz], np.float64)
+                meshes.append(((base @ R.T + centre).astype(np.float32), faces))
+                d = rng.normal(size=3)
+                drifts.append(drift * d / np.linalg.norm(d))
+    return meshes, np.asarray(drifts, np.float32)

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import ClothMeshDesc, PhysicsParams, TetMeshDesc, check, fptr, f32, i32, iptr, lib
+from ._lib import (MODE_GROUP_LAUNCHES, MODE_WORK_QUEUE, ClothMeshDesc, PhysicsParams, Schedule, TetMeshDesc, check,
+                   fptr, f32, i32, iptr, lib)
 
 # Reference defaults (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
 DEFAULT_PHYSICS = dict(
@@ -358,8 +359,23 @@ class VBDPhysics:
     def recordVBDSolveGraph(self, enable=True):
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))
 
-    def setSchedule(self, lanes_per_vertex, block):
-        self.ctx.check(lib().gvbd_scene_set_schedule(self.ctx.handle, self.scene, lanes_per_vertex, block))
+    def setSchedule(self, lanes_per_vertex, block, mode=None, vertices_per_entry=0, blocks_per_cu=0, spin_limit=0):
+        """Vertex-solve schedule; `mode` "group" (one launch per parallel group) or "queue" (one
+        persistent work-queue launch per iteration); None keeps the current mode."""
+        if mode is None:
+            self.ctx.check(lib().gvbd_scene_set_schedule(self.ctx.handle, self.scene, lanes_per_vertex, block))
+            return
+        sc = make_schedule(lanes_per_vertex, block, mode, vertices_per_entry, blocks_per_cu, spin_limit)
+        self.ctx.check(lib().gvbd_scene_set_schedule_ex(self.ctx.handle, self.scene, ctypes.byref(sc)))
+
+    def getSchedule(self):
+        sc = Schedule()
+        self.ctx.check(lib().gvbd_scene_get_schedule(self.ctx.handle, self.scene, ctypes.byref(sc)))
+        return schedule_dict(sc)
+
+    def runIteration(self, iIter):
+        """One VBD iteration (every parallel group in order) after applyInitialStep."""
+        self.ctx.check(lib().gvbd_scene_run_iteration(self.ctx.handle, self.scene, iIter))
 
     def synchronize(self):
         self.ctx.synchronize()
@@ -416,8 +432,24 @@ class VBDPhysics:
         self.scene = None
 
 
+_MODES = {"group": MODE_GROUP_LAUNCHES, "queue": MODE_WORK_QUEUE}
+
+
+def make_schedule(lanes_per_vertex, block, mode="group", vertices_per_entry=0, blocks_per_cu=0, spin_limit=0):
+    if mode not in _MODES:
+        raise ValueError(f"mode must be one of {sorted(_MODES)}")
+    return Schedule(lanes_per_vertex, block, _MODES[mode], vertices_per_entry, blocks_per_cu, spin_limit)
+
+
+def schedule_dict(sc):
+    mode = {v: k for k, v in _MODES.items()}[sc.mode]
+    return dict(lanes_per_vertex=sc.lanes_per_vertex, block=sc.block, mode=mode,
+                vertices_per_entry=sc.vertices_per_entry, blocks_per_cu=sc.blocks_per_cu, spin_limit=sc.spin_limit)
+
+
 class Batch:
-    """Many initialised scenes advanced by one launch per parallel group."""
+    """Many initialised scenes advanced together: one launch per parallel group, or one
+    persistent work-queue launch per iteration."""
 
     def __init__(self, ctx, scenes):
         self.ctx = ctx
@@ -426,14 +458,38 @@ class Batch:
         ctx.check(lib().gvbd_batch_create(ctx.handle, iptr(ids), ids.size, ctypes.byref(b)))
         self.batch = b.value
 
-    def set_schedule(self, lanes_per_vertex, block):
-        self.ctx.check(lib().gvbd_batch_set_schedule(self.ctx.handle, self.batch, lanes_per_vertex, block))
+    def set_schedule(self, lanes_per_vertex, block, mode=None, vertices_per_entry=0, blocks_per_cu=0, spin_limit=0):
+        if mode is None:
+            self.ctx.check(lib().gvbd_batch_set_schedule(self.ctx.handle, self.batch, lanes_per_vertex, block))
+            return
+        sc = make_schedule(lanes_per_vertex, block, mode, vertices_per_entry, blocks_per_cu, spin_limit)
+        self.ctx.check(lib().gvbd_batch_set_schedule_ex(self.ctx.handle, self.batch, ctypes.byref(sc)))
+
+    def get_schedule(self):
+        sc = Schedule()
+        self.ctx.check(lib().gvbd_batch_get_schedule(self.ctx.handle, self.batch, ctypes.byref(sc)))
+        return schedule_dict(sc)
+
+    def queue_info(self):
+        """(entries of one iteration's work queue, persistent grid blocks) of the current schedule."""
+        e, g = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(lib().gvbd_batch_queue_info(self.ctx.handle, self.batch, ctypes.byref(e), ctypes.byref(g)))
+        return e.value, g.value
 
     def record_graph(self, enable=True):
         self.ctx.check(lib().gvbd_batch_record_graph(self.ctx.handle, self.batch, 1 if enable else 0))
 
     def run_steps(self, n):
         self.ctx.check(lib().gvbd_batch_run_steps(self.ctx.handle, self.batch, n))
+
+    def apply_initial_step(self):
+        self.ctx.check(lib().gvbd_batch_apply_initial_step(self.ctx.handle, self.batch))
+
+    def run_iteration(self, iteration):
+        self.ctx.check(lib().gvbd_batch_run_iteration(self.ctx.handle, self.batch, iteration))
+
+    def update_velocities(self):
+        self.ctx.check(lib().gvbd_batch_update_velocities(self.ctx.handle, self.batch))
 
     def solve_group(self, group, iteration=0):
         self.ctx.check(lib().gvbd_batch_solve_parallel_group(self.ctx.handle, self.batch, group, iteration))

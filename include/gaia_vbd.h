@@ -134,9 +134,24 @@ typedef struct gvbd_cloth_mesh_desc {
     int32_t coloring_algorithm;
 } gvbd_cloth_mesh_desc;
 
+#define GVBD_MODE_GROUP_LAUNCHES 0 /* one kernel launch per parallel group (the reference's structure) */
+#define GVBD_MODE_WORK_QUEUE 1     /* one persistent launch per iteration over a (scene, colour, vertex) queue */
+
+/* Launch schedule of the vertex solve (no reference counterpart: the reference fixes one block
+ * per vertex, VBD/VBDPhysicsCompute.cu:803-968). */
+typedef struct gvbd_schedule {
+    int32_t lanes_per_vertex;   /* 1, 2, 4, 8, 16: lanes of one wave64 sharing a vertex's incident elements */
+    int32_t block;              /* 64, 128, 256 threads */
+    int32_t mode;               /* GVBD_MODE_* */
+    int32_t vertices_per_entry; /* work-queue entry size, a multiple of block / lanes_per_vertex (0: 4x that) */
+    int32_t blocks_per_cu;      /* persistent grid per compute unit (0: the resident maximum) */
+    int32_t spin_limit;         /* polls before a dependency wait is reported as an error (0: default) */
+    int32_t reserved[2];
+} gvbd_schedule;
+
 /* sizes of the ABI structs as compiled into the library (binding self-check) */
 int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
-                          int32_t* cloth_mesh_desc_bytes);
+                          int32_t* cloth_mesh_desc_bytes, int32_t* schedule_bytes);
 
 /* ---- context ---- */
 int gvbd_context_create(int32_t device, gvbd_context** out);
@@ -169,12 +184,18 @@ int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, fl
                               float* mass);
 /* lanes per vertex (1,2,4,8,16) and block size (64,128,256) of the vertex solve */
 int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_vertex, int32_t block);
+/* full schedule (mode, work-queue entry size, persistent grid); drops a recorded graph */
+int gvbd_scene_set_schedule_ex(gvbd_context* ctx, int32_t scene, const gvbd_schedule* schedule);
+int gvbd_scene_get_schedule(gvbd_context* ctx, int32_t scene, gvbd_schedule* schedule);
 
 /* ---- stepping (asynchronous on the context stream) ---- */
 int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene);               /* all substeps of one frame */
 int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n);
 int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene);
 int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t iteration);
+/* one VBD iteration: every parallel group in order, in the scene's schedule mode
+ * (the iteration body of VBDPhysics::runStepGPU / runStep, VBD/VBDPhysics.cpp:879-945, 1605-1627) */
+int gvbd_scene_run_iteration(gvbd_context* ctx, int32_t scene, int32_t iteration);
 int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
@@ -189,8 +210,16 @@ int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, co
 int gvbd_batch_create(gvbd_context* ctx, const int32_t* scenes, int32_t n, int32_t* batch_out);
 int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch);
 int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_vertex, int32_t block);
+int gvbd_batch_set_schedule_ex(gvbd_context* ctx, int32_t batch, const gvbd_schedule* schedule);
+int gvbd_batch_get_schedule(gvbd_context* ctx, int32_t batch, gvbd_schedule* schedule);
+/* work-queue size of one iteration and the persistent grid (blocks) of the current schedule */
+int gvbd_batch_queue_info(gvbd_context* ctx, int32_t batch, int32_t* entries, int32_t* grid);
 int gvbd_batch_record_graph(gvbd_context* ctx, int32_t batch, int32_t enable);
 int gvbd_batch_run_steps(gvbd_context* ctx, int32_t batch, int32_t n);
+/* per-iteration hooks over every scene of the batch (as the gvbd_scene_ forms) */
+int gvbd_batch_apply_initial_step(gvbd_context* ctx, int32_t batch);
+int gvbd_batch_run_iteration(gvbd_context* ctx, int32_t batch, int32_t iteration);
+int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch);
 /* launch only the vertex solve of parallel group `group` (timing of the dominant kernel) */
 int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
 int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);

@@ -384,3 +384,129 @@ class ClothScene:
                 self._h = None
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# contact detector (detector_oracle.cpp)
+# ---------------------------------------------------------------------------
+_cd = None
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def _cd_lib():
+    global _cd
+    if _cd is None:
+        L = lib()
+        L.oracle_cd_create.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        L.oracle_cd_create.restype = ctypes.c_void_p
+        L.oracle_cd_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_cd_destroy.restype = None
+        L.oracle_cd_sizes.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.oracle_cd_sizes.restype = None
+        L.oracle_cd_topology.argtypes = [ctypes.c_void_p, _ip, _ip]
+        L.oracle_cd_topology.restype = None
+        L.oracle_cd_detect.argtypes = [ctypes.c_void_p, _fp, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        L.oracle_cd_detect.restype = None
+        L.oracle_cd_counts.argtypes = [ctypes.c_void_p, _i64p]
+        L.oracle_cd_counts.restype = None
+        L.oracle_cd_vf_pairs.argtypes = [ctypes.c_void_p, _ip, _fp]
+        L.oracle_cd_vf_pairs.restype = None
+        L.oracle_cd_vf_contacts.argtypes = [ctypes.c_void_p, _ip, _fp]
+        L.oracle_cd_vf_contacts.restype = None
+        L.oracle_cd_ee_contacts.argtypes = [ctypes.c_void_p, _ip, _fp, _fp]
+        L.oracle_cd_ee_contacts.restype = None
+        L.oracle_cd_min_dis.argtypes = [ctypes.c_void_p, _fp, _fp, _fp]
+        L.oracle_cd_min_dis.restype = None
+        L.oracle_cd_bounds.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, _fp]
+        L.oracle_cd_bounds.restype = None
+        L.oracle_closest_point_triangle.argtypes = [_fp, _fp, _fp, _fp, _fp, _fp]
+        L.oracle_segment_closest_points.argtypes = [_fp, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.oracle_segment_closest_points.restype = None
+        _cd = L
+    return _cd
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def closest_point_triangle(p, a, b, c):
+    """(closest point, barycentrics, type) with type 0..6 = A, B, C, AB, BC, AC, interior."""
+    args = [_f(x)[0] for x in (p, a, b, c)]
+    out = np.zeros(3, np.float32)
+    bary = np.zeros(3, np.float32)
+    t = _cd_lib().oracle_closest_point_triangle(*[x.ctypes.data_as(_fp) for x in args], out.ctypes.data_as(_fp),
+                                               bary.ctypes.data_as(_fp))
+    return out, bary, t
+
+
+def segment_closest_points(p0, p1, q0, q1):
+    args = [_f(x)[0] for x in (p0, p1, q0, q1)]
+    cp, cq, st = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(2, np.float32)
+    _cd_lib().oracle_segment_closest_points(*[x.ctypes.data_as(_fp) for x in args], cp.ctypes.data_as(_fp),
+                                           cq.ctypes.data_as(_fp), st.ctypes.data_as(_fp))
+    return cp, cq, float(st[0]), float(st[1])
+
+
+class Detector:
+    """VF / EE proximity queries over triangle meshes (global vertex / face / edge ids,
+    meshes concatenated in order). mode "grid" (uniform grid) or "brute" (all pairs)."""
+
+    def __init__(self, meshes):
+        L = _cd_lib()
+        nv = np.array([len(p) for p, _ in meshes], np.int32)
+        nf = np.array([len(f) for _, f in meshes], np.int32)
+        faces = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32).reshape(-1) for _, f in meshes]))
+        self._h = L.oracle_cd_create(len(meshes), nv.ctypes.data_as(_ip), nf.ctypes.data_as(_ip),
+                                     faces.ctypes.data_as(_ip))
+        if not self._h:
+            raise ValueError("invalid mesh (index out of range, non-manifold or inconsistent orientation)")
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.oracle_cd_sizes(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self.nv, self.nf, self.ne = a.value, b.value, c.value
+
+    def topology(self):
+        edges = np.zeros((self.ne, 4), np.int32)
+        fe = np.zeros((self.nf, 3), np.int32)
+        _cd_lib().oracle_cd_topology(self._h, edges.ctypes.data_as(_ip), fe.ctypes.data_as(_ip))
+        return edges, fe
+
+    def detect(self, positions, radius, mode="grid", threads=8):
+        L = _cd_lib()
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        self.radius = np.float32(radius)
+        L.oracle_cd_detect(self._h, p.ctypes.data_as(_fp), ctypes.c_float(radius), 1 if mode == "grid" else 0,
+                           threads)
+        cnt = np.zeros(3, np.int64)
+        L.oracle_cd_counts(self._h, cnt.ctypes.data_as(_i64p))
+        n_pairs, n_vf, n_ee = (int(x) for x in cnt)
+        out = {}
+        vf = np.zeros((n_pairs, 2), np.int32)
+        d = np.zeros(n_pairs, np.float32)
+        L.oracle_cd_vf_pairs(self._h, vf.ctypes.data_as(_ip), d.ctypes.data_as(_fp))
+        out["vf_pairs"], out["vf_pair_d"] = vf, d
+        rec = np.zeros((n_vf, 4), np.int32)
+        d = np.zeros(n_vf, np.float32)
+        L.oracle_cd_vf_contacts(self._h, rec.ctypes.data_as(_ip), d.ctypes.data_as(_fp))
+        out["vf_contacts"], out["vf_contact_d"] = rec, d
+        ee = np.zeros((n_ee, 2), np.int32)
+        mu = np.zeros((n_ee, 2), np.float32)
+        d = np.zeros(n_ee, np.float32)
+        L.oracle_cd_ee_contacts(self._h, ee.ctypes.data_as(_ip), mu.ctypes.data_as(_fp), d.ctypes.data_as(_fp))
+        out["ee_contacts"], out["ee_mu"], out["ee_d"] = ee, mu, d
+        a, b, c = np.zeros(self.nv, np.float32), np.zeros(self.nf, np.float32), np.zeros(self.ne, np.float32)
+        L.oracle_cd_min_dis(self._h, a.ctypes.data_as(_fp), b.ctypes.data_as(_fp), c.ctypes.data_as(_fp))
+        out["vf_min"], out["face_min"], out["edge_min"] = a, b, c
+        return out
+
+    def bounds(self, thickness, relaxation):
+        b = np.zeros(self.nv, np.float32)
+        _cd_lib().oracle_cd_bounds(self._h, ctypes.c_float(thickness), ctypes.c_float(relaxation),
+                                   b.ctypes.data_as(_fp))
+        return b
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _cd_lib().oracle_cd_destroy(self._h)
+            self._h = None

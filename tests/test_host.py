@@ -32,13 +32,15 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_struct_layout_matches_header():
-    from gaia_amd._lib import ClothMeshDesc, PhysicsParams, TetMeshDesc
+    from gaia_amd._lib import ClothMeshDesc, PhysicsParams, Schedule, TetMeshDesc
 
-    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    assert gaia_amd.lib().gvbd_abi_struct_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+    a, b, c, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert gaia_amd.lib().gvbd_abi_struct_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                ctypes.byref(d)) == 0
     assert ctypes.sizeof(PhysicsParams) == a.value == 120
     assert ctypes.sizeof(TetMeshDesc) == b.value
     assert ctypes.sizeof(ClothMeshDesc) == c.value
+    assert ctypes.sizeof(Schedule) == d.value == 32
     assert TetMeshDesc.positions.offset == 8 and TetMeshDesc.tets.offset == 40
 
 
