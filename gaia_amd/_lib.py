@@ -171,6 +171,26 @@ _SIGNATURES = {
     "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
     "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],
+    "gvbd_cd_create": [_P, ctypes.c_float, _I, _I, c_int32_p],
+    "gvbd_cd_destroy": [_P, _I],
+    "gvbd_cd_add_mesh": [_P, _I, _I, c_float_p, _I, c_int32_p, c_int32_p],
+    "gvbd_cd_finalize": [_P, _I],
+    "gvbd_cd_sizes": [_P, _I, c_int32_p, c_int32_p, c_int32_p],
+    "gvbd_cd_mesh_offsets": [_P, _I, _I, c_int32_p, c_int32_p, c_int32_p],
+    "gvbd_cd_topology": [_P, _I, c_int32_p, c_int32_p],
+    "gvbd_cd_set_positions": [_P, _I, _I, c_float_p],
+    "gvbd_cd_get_positions": [_P, _I, _I, c_float_p],
+    "gvbd_cd_perturb": [_P, _I, ctypes.c_uint64, _I, ctypes.c_float, c_float_p],
+    "gvbd_cd_update_bvh": [_P, _I, _I],
+    "gvbd_cd_query": [_P, _I, _I],
+    "gvbd_cd_bounds": [_P, _I, ctypes.c_float, ctypes.c_float, c_float_p],
+    "gvbd_cd_counts": [_P, _I, c_int64_p],
+    "gvbd_cd_vf_pairs": [_P, _I, c_int32_p, c_float_p],
+    "gvbd_cd_vf_contacts": [_P, _I, c_int32_p, c_float_p],
+    "gvbd_cd_ee_contacts": [_P, _I, c_int32_p, c_float_p, c_float_p],
+    "gvbd_cd_vf_results": [_P, _I, c_int32_p, _P],
+    "gvbd_cd_ee_results": [_P, _I, c_int32_p, _P],
+    "gvbd_cd_min_distances": [_P, _I, c_float_p, c_float_p, c_float_p],
     "gvbd_stream_copy": [_P, ctypes.c_int64, _I, c_float_p],
     "gvbd_event_record": [_P, _I],
     "gvbd_event_elapsed_ms": [_P, _I, _I, c_float_p],

@@ -85,7 +85,7 @@ class ContactDetector {
     int numVertices() const { return nv_; }
     int numFaces() const { return nf_; }
     int numEdges() const { return ne_; }
-    int numMeshes() const { return (int)meshV_.size(); }
+    int numMeshes() const { return (int)meshV_.size() - 1; }
     void meshOffsets(int mesh, int* v, int* f, int* e) const;
     void topology(int32_t* edges, int32_t* faceEdges) const;
     size_t deviceBytes() const { return bytes_; }
@@ -138,6 +138,8 @@ class ContactDetector {
     float2* recEEmu_ = nullptr;
     float* recEEd_ = nullptr;
     bool recorded_ = false;
+    int* error_ = nullptr;      // bit 0: traversal stack overflow, bit 1: tree too deep
+    int* depthHist_ = nullptr;  // per-depth internal node counts (build)
 };
 
 }  // namespace gvbd

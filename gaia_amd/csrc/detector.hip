@@ -1,0 +1,1226 @@
+// GPU contact detector (CDNA4 / gfx950): linear BVHs over faces and edges replace the
+// reference's Embree scenes (CollisionDetector/ClothContactDetector.cpp:944-1181).
+//
+//   build   scene box of primitive centroids -> 30-bit Morton codes in the box (the 10-bit-per-
+//           axis interleave of BVH/MortonCode.h:9-31) with the primitive id in the low 32 bits ->
+//           device radix sort -> binary radix tree from the sorted keys (Karras 2012, HPG) ->
+//           node depths -> internal nodes sorted by depth (the refit schedule)
+//   refit   leaf boxes from the current positions, then one launch per depth level, deepest
+//           first (no atomics, no inter-block waits)
+//   query   one thread per vertex (VF, query box of half-size r around the vertex) or per edge
+//           (EE, half-size r + |e|/2 around the midpoint), stack traversal, exact tests with the
+//           reference's predicates (ClothContactDetector.cpp:144-288, 731-813; CollisionGeometry.h:
+//           90-267; TriMeshCollisionGeometry.cpp:7-186)
+//
+// The exact tests use the same float operations in the same order as the CPU restatement
+// (dot = (x + y) + z; no contraction into fused multiply-adds; IEEE division and square root),
+// so candidate sets match it bit for bit. Outputs that the reference takes from its Embree
+// traversal order are made order-free: the representative face of a VF contact is the lowest
+// face id with that closest primitive; per-primitive minima are exact minima.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "detector.h"
+#include "errors.h"
+#include "mesh_prep.h"
+
+#pragma clang fp contract(off)
+
+namespace gvbd {
+namespace {
+
+constexpr int kMaxCap = 16;       // largest per-query contact capacity
+constexpr int kStack = 64;        // traversal stack (tree depth <= 62 for 62-bit unique keys)
+constexpr int kMaxDepth = 128;    // depth histogram size
+constexpr float kCmpEpsilon = 0.00001f;
+constexpr float kCmpEpsilon2 = kCmpEpsilon * kCmpEpsilon;
+constexpr float kAbsoluteRelaxation = 1e-6f;
+constexpr float kFeasibleEpsilon = 1e-3f;
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float dist(V3 a, V3 b) {
+    const V3 d = a - b;
+    return sqrtf(dot(d, d));
+}
+__device__ __forceinline__ V3 ld(const float4* p, int i) {
+    const float4 v = p[i];
+    return {v.x, v.y, v.z};
+}
+
+enum PointType { AT_A = 0, AT_B, AT_C, AT_AB, AT_BC, AT_AC, AT_INTERIOR };
+
+__device__ V3 closest_point_triangle(V3 p, V3 a, V3 b, V3 c, int& type) {
+    const V3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.f && d2 <= 0.f) {
+        type = AT_A;
+        return a;
+    }
+    const V3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.f && d4 <= d3) {
+        type = AT_B;
+        return b;
+    }
+    const V3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.f && d5 <= d6) {
+        type = AT_C;
+        return c;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+        const float v = d1 / (d1 - d3);
+        type = AT_AB;
+        return a + v * ab;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+        const float v = d2 / (d2 - d6);
+        type = AT_AC;
+        return a + v * ac;
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+        type = AT_BC;
+        const float v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return b + v * (c - b);
+    }
+    const float denom = 1.f / (va + vb + vc);
+    const float v = vb * denom, w = vc * denom;
+    type = AT_INTERIOR;
+    return (a + v * ab) + w * ac;
+}
+
+__device__ void segment_closest_points(V3 p0, V3 p1, V3 q0, V3 q1, V3& cp, V3& cq, float& s, float& t) {
+    const V3 p = p1 - p0, q = q1 - q0, r = p0 - q0;
+    const float a = dot(p, p), b = dot(p, q), c = dot(q, q), d = dot(p, r), e = dot(q, r);
+    s = 0.0f;
+    t = 0.0f;
+    const float det = a * c - b * b;
+    if (det > kCmpEpsilon) {
+        const float bte = b * e, ctd = c * d;
+        if (bte <= ctd) {
+            if (e <= 0.0f) {
+                s = (-d >= a ? 1.f : (-d > 0.0f ? -d / a : 0.0f));
+                t = 0.0f;
+            } else if (e < c) {
+                s = 0.0f;
+                t = e / c;
+            } else {
+                s = (b - d >= a ? 1.f : (b - d > 0.0f ? (b - d) / a : 0.0f));
+                t = 1.f;
+            }
+        } else {
+            s = bte - ctd;
+            if (s >= det) {
+                if (b + e <= 0.0f) {
+                    s = (-d <= 0.0f ? 0.0f : (-d < a ? -d / a : 1.f));
+                    t = 0.0f;
+                } else if (b + e < c) {
+                    s = 1.f;
+                    t = (b + e) / c;
+                } else {
+                    s = (b - d <= 0.0f ? 0.0f : (b - d < a ? (b - d) / a : 1.f));
+                    t = 1.f;
+                }
+            } else {
+                const float ate = a * e, btd = b * d;
+                if (ate <= btd) {
+                    s = (-d <= 0.0f ? 0.0f : (-d >= a ? 1.f : -d / a));
+                    t = 0.0f;
+                } else {
+                    t = ate - btd;
+                    if (t >= det) {
+                        s = (b - d <= 0.0f ? 0.0f : (b - d >= a ? 1.f : (b - d) / a));
+                        t = 1.f;
+                    } else {
+                        s /= det;
+                        t /= det;
+                    }
+                }
+            }
+        }
+    } else {
+        if (e <= 0.0f) {
+            s = (-d <= 0.0f ? 0.0f : (-d >= a ? 1.f : -d / a));
+            t = 0.0f;
+        } else if (e >= c) {
+            s = (b - d <= 0.0f ? 0.0f : (b - d >= a ? 1.f : (b - d) / a));
+            t = 1.f;
+        } else {
+            s = 0.0f;
+            t = e / c;
+        }
+    }
+    cp = (1.f - s) * p0 + s * p1;
+    cq = (1.f - t) * q0 + t * q1;
+}
+
+// Device view of a detector's topology and outputs.
+struct CDView {
+    const float4* pos;
+    int nv, nf, ne;
+    const int* vMesh;
+    const int4* faces;      // v0, v1, v2, mesh
+    const int4* faceEdges;  // edges (v0 v1), (v1 v2), (v2 v0)
+    const int4* edges;      // eV1, eV2, eV12Next, eV21Next
+    const int* eMesh;
+    const int* vvBegin;
+    const int* vvNei;
+    const BvhNode* fNodes;
+    int fN;
+    const BvhNode* eNodes;
+    int eN;
+    float r;
+    int vfCap, eeCap;
+    float* vfMin;
+    int* faceMin;
+    float* eeMin;
+    int* vfPairCount;
+    int* vfStored;
+    int* vfOverflow;
+    int* eeCount;
+    int* eeStored;
+    int* eeOverflow;
+    VFContactRec* vfRec;
+    EEContactRec* eeRec;
+    // record mode
+    const int64_t* vfOffset;
+    const int64_t* eeOffset;
+    int2* recVF;
+    float* recVFd;
+    int4* recKeys;
+    float* recKeysD;
+    int2* recEE;
+    float2* recEEmu;
+    float* recEEd;
+    int* error;
+};
+
+__device__ __forceinline__ bool box_overlap(V3 qlo, V3 qhi, const BvhNode& n) {
+    return qlo.x <= n.hi.x && qhi.x >= n.lo.x && qlo.y <= n.hi.y && qhi.y >= n.lo.y && qlo.z <= n.hi.z &&
+           qhi.z >= n.lo.z;
+}
+
+__device__ bool vertex_feasible(const CDView& S, V3 p, int v) {
+    const V3 A = ld(S.pos, v);
+    const V3 AP = p - A;
+    const float relaxed = -kFeasibleEpsilon - kAbsoluteRelaxation;
+    for (int k = S.vvBegin[v]; k < S.vvBegin[v + 1]; ++k) {
+        const V3 BA = A - ld(S.pos, S.vvNei[k]);
+        if (dot(AP, BA) < relaxed) return false;
+    }
+    return true;
+}
+
+__device__ bool edge_feasible(const CDView& S, V3 p, int e) {
+    const int4 ei = S.edges[e];
+    const V3 v1 = ld(S.pos, ei.x), v2 = ld(S.pos, ei.y);
+    const V3 AP = p - v1, BP = p - v2, AB = v2 - v1;
+    const float relaxed = kFeasibleEpsilon + kAbsoluteRelaxation;
+    if (dot(AP, AB) < -relaxed) return false;
+    if (dot(BP, AB) > relaxed) return false;
+    const float ab2 = dot(AB, AB);
+    if (ab2 < kCmpEpsilon2) return false;
+    const int nx[2] = {ei.z, ei.w};
+    for (int k = 0; k < 2; ++k) {
+        if (nx[k] == -1) continue;
+        const V3 vn = ld(S.pos, nx[k]);
+        const float t = dot(AB, vn - v1) / ab2;
+        const V3 foot = v1 + t * AB;
+        if (dot(vn - foot, p - foot) > relaxed) return false;
+    }
+    return true;
+}
+
+// VF query of one vertex (ClothContactDetector.cpp:144-288 per candidate face)
+template <bool RECORD>
+__global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= S.nv) return;
+    const V3 p = ld(S.pos, v);
+    const int mesh = S.vMesh[v];
+    const float r = S.r;
+    const float rq = r * 1.0001f + 1e-7f;
+    const V3 qlo = {p.x - rq, p.y - rq, p.z - rq}, qhi = {p.x + rq, p.y + rq, p.z + rq};
+    float vmin = r;
+    int pairs = 0, stored = 0, overflow = 0;
+    int kType[kMaxCap], kPrim[kMaxCap], kFace[kMaxCap];
+    float kD[kMaxCap];
+    int64_t pairBase = 0, keyBase = 0;
+    int recKeys = 0, pairCap = 0;
+    if (RECORD) {
+        pairCap = S.vfPairCount[v];
+        pairBase = S.vfOffset[v];
+        keyBase = pairBase;  // the key segment has the pair segment's size (an upper bound)
+    }
+    int stack[kStack];
+    int sp = 0;
+    if (S.fN > 0) stack[sp++] = 0;
+    const int firstLeaf = S.fN - 1;
+    while (sp > 0) {
+        const int node = stack[--sp];
+        const BvhNode nd = S.fNodes[node];
+        if (!box_overlap(qlo, qhi, nd)) continue;
+        if (node < firstLeaf) {
+            if (sp + 2 > kStack) {
+                atomicOr(S.error, 1);
+                break;
+            }
+            stack[sp++] = __float_as_int(nd.hi.w);
+            stack[sp++] = __float_as_int(nd.lo.w);
+            continue;
+        }
+        const int f = __float_as_int(nd.lo.w);
+        const int4 fv = S.faces[f];
+        if (fv.w == mesh && (fv.x == v || fv.y == v || fv.z == v)) continue;
+        int type;
+        const V3 cp = closest_point_triangle(p, ld(S.pos, fv.x), ld(S.pos, fv.y), ld(S.pos, fv.z), type);
+        const float d = dist(p, cp);
+        if (!(d < r)) continue;
+        vmin = fminf(d, vmin);
+        atomicMin(S.faceMin + f, __float_as_int(d));
+        if (RECORD) {
+            // the record pass repeats the counting pass's traversal; never write past its count
+            if (pairs >= pairCap) {
+                atomicOr(S.error, 4);
+                break;
+            }
+            S.recVF[pairBase + pairs] = make_int2(v, f);
+            S.recVFd[pairBase + pairs] = d;
+        }
+        ++pairs;
+        int prim, ptype;
+        if (type <= AT_C) {
+            prim = type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z);
+            ptype = 0;
+        } else if (type == AT_INTERIOR) {
+            prim = f;
+            ptype = 2;
+        } else {
+            const int4 fe = S.faceEdges[f];
+            prim = type == AT_AB ? fe.x : (type == AT_BC ? fe.y : fe.z);
+            ptype = 1;
+        }
+        // dedup on (type, primitive): keep the lowest face id as the representative
+        bool dup = false;
+        if (!RECORD) {
+            for (int k = 0; k < stored; ++k)
+                if (kType[k] == ptype && kPrim[k] == prim) {
+                    if (f < kFace[k]) {
+                        kFace[k] = f;
+                        kD[k] = d;
+                    }
+                    dup = true;
+                    break;
+                }
+        } else {
+            for (int k = 0; k < recKeys; ++k) {
+                int4& rk = S.recKeys[keyBase + k];
+                if (rk.y == ptype && rk.z == prim) {
+                    if (f < rk.w) {
+                        rk.w = f;
+                        S.recKeysD[keyBase + k] = d;
+                    }
+                    dup = true;
+                    break;
+                }
+            }
+        }
+        if (dup) continue;
+        bool ok = true;
+        if (ptype == 0)
+            ok = vertex_feasible(S, p, prim);
+        else if (ptype == 1)
+            ok = edge_feasible(S, p, prim);
+        if (!ok) continue;
+        if (RECORD) {
+            S.recKeys[keyBase + recKeys] = make_int4(v, ptype, prim, f);
+            S.recKeysD[keyBase + recKeys] = d;
+            ++recKeys;
+        } else {
+            // keep the vfCap keys with the lowest representative faces (order-free selection)
+            int slot = stored;
+            if (stored == S.vfCap) {
+                overflow = 1;
+                slot = 0;
+                for (int k = 1; k < stored; ++k)
+                    if (kFace[k] > kFace[slot]) slot = k;
+                if (f > kFace[slot]) slot = -1;
+            } else {
+                ++stored;
+            }
+            if (slot >= 0) {
+                kType[slot] = ptype;
+                kPrim[slot] = prim;
+                kFace[slot] = f;
+                kD[slot] = d;
+            }
+        }
+    }
+    if (RECORD) {
+        for (int k = recKeys; k < pairs; ++k) S.recKeys[keyBase + k] = make_int4(v, -1, -1, -1);
+        return;
+    }
+    S.vfMin[v] = vmin;
+    S.vfPairCount[v] = pairs;
+    S.vfStored[v] = stored;
+    S.vfOverflow[v] = overflow;
+    for (int k = 0; k < stored; ++k) S.vfRec[(int64_t)v * S.vfCap + k] = VFContactRec{kFace[k], kType[k], kPrim[k], kD[k]};
+}
+
+// EE query of one edge (ClothContactDetector.cpp:731-813 per candidate edge, :1106-1135 setup)
+template <bool RECORD>
+__global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
+    const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e1 >= S.ne) return;
+    const int4 a = S.edges[e1];
+    const int m1 = S.eMesh[e1];
+    const V3 p1 = ld(S.pos, a.x), p2 = ld(S.pos, a.y);
+    const float r = S.r;
+    const float len = dist(p1, p2);
+    const V3 mid = 0.5f * (p1 + p2);
+    const float rq = (r + 0.5f * len) * 1.0001f + 1e-7f;
+    const V3 qlo = {mid.x - rq, mid.y - rq, mid.z - rq}, qhi = {mid.x + rq, mid.y + rq, mid.z + rq};
+    float emin = r;
+    int count = 0, stored = 0, overflow = 0;
+    int cE[kMaxCap];
+    float cMu1[kMaxCap], cMu2[kMaxCap], cD[kMaxCap];
+    const int64_t base = RECORD ? S.eeOffset[e1] : 0;
+    const int countCap = RECORD ? S.eeCount[e1] : 0;
+    int stack[kStack];
+    int sp = 0;
+    if (S.eN > 0) stack[sp++] = 0;
+    const int firstLeaf = S.eN - 1;
+    const V3 ev1 = p2 - p1;
+    while (sp > 0) {
+        const int node = stack[--sp];
+        const BvhNode nd = S.eNodes[node];
+        if (!box_overlap(qlo, qhi, nd)) continue;
+        if (node < firstLeaf) {
+            if (sp + 2 > kStack) {
+                atomicOr(S.error, 1);
+                break;
+            }
+            stack[sp++] = __float_as_int(nd.hi.w);
+            stack[sp++] = __float_as_int(nd.lo.w);
+            continue;
+        }
+        const int e2 = __float_as_int(nd.lo.w);
+        const int4 b = S.edges[e2];
+        if (S.eMesh[e2] == m1) {
+            if (e2 == e1) continue;
+            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) continue;
+        }
+        const V3 q1 = ld(S.pos, b.x), q2 = ld(S.pos, b.y);
+        V3 c1, c2;
+        float mua, mub;
+        segment_closest_points(p1, p2, q1, q2, c1, c2, mua, mub);
+        const float d = dist(c1, c2);
+        emin = fminf(d, emin);
+        if (!(d < r)) continue;
+        const V3 ev2 = q2 - q1;
+        bool ok = false;
+        if (dot(c2 - p1, ev1) >= 0.f && dot(p2 - c2, ev1) >= 0.f)
+            ok = true;
+        else if (dot(c1 - q1, ev2) >= 0.f && dot(q2 - c1, ev2) >= 0.f)
+            ok = true;
+        if (!ok) continue;
+        if (RECORD) {
+            if (count >= countCap) {
+                atomicOr(S.error, 4);
+                break;
+            }
+            S.recEE[base + count] = make_int2(e1, e2);
+            S.recEEmu[base + count] = make_float2(mua, mub);
+            S.recEEd[base + count] = d;
+        } else {
+            // keep the eeCap contacts with the lowest other-edge ids (order-free selection)
+            int slot = stored;
+            if (stored == S.eeCap) {
+                overflow = 1;
+                slot = 0;
+                for (int k = 1; k < stored; ++k)
+                    if (cE[k] > cE[slot]) slot = k;
+                if (e2 > cE[slot]) slot = -1;
+            } else {
+                ++stored;
+            }
+            if (slot >= 0) {
+                cE[slot] = e2;
+                cMu1[slot] = mua;
+                cMu2[slot] = mub;
+                cD[slot] = d;
+            }
+        }
+        ++count;
+    }
+    if (RECORD) return;
+    S.eeMin[e1] = emin;
+    S.eeCount[e1] = count;
+    S.eeStored[e1] = stored;
+    S.eeOverflow[e1] = overflow;
+    for (int k = 0; k < stored; ++k) S.eeRec[(int64_t)e1 * S.eeCap + k] = EEContactRec{cE[k], cMu1[k], cMu2[k], cD[k]};
+}
+
+// VBDClothPhysics.cpp:775-812 per vertex
+__global__ __launch_bounds__(256) void bounds_kernel(const float* vfMin, const int* faceMin, const float* eeMin,
+                                                     const int* vfBegin, const int* vfFace, const int* veBegin,
+                                                     const int* veEdge, int nv, float r, float thickness,
+                                                     float relaxation, float* out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    float dMin = r;
+    dMin = fminf(dMin, vfMin[v]);
+    for (int k = vfBegin[v]; k < vfBegin[v + 1]; ++k) dMin = fminf(dMin, __int_as_float(faceMin[vfFace[k]]));
+    for (int k = veBegin[v]; k < veBegin[v + 1]; ++k) dMin = fminf(dMin, eeMin[veEdge[k]]);
+    dMin = fmaxf(0.f, dMin - thickness);
+    out[v] = dMin * relaxation;
+}
+
+// ---- build / refit -------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned int order_bits(float f) {
+    const unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unorder_bits(unsigned int u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__device__ __forceinline__ V3 prim_centroid(const CDView& S, int kind, int i) {
+    if (kind == 0) {
+        const int4 f = S.faces[i];
+        const V3 a = ld(S.pos, f.x), b = ld(S.pos, f.y), c = ld(S.pos, f.z);
+        return (1.f / 3.f) * ((a + b) + c);
+    }
+    const int4 e = S.edges[i];
+    return 0.5f * (ld(S.pos, e.x) + ld(S.pos, e.y));
+}
+
+__global__ __launch_bounds__(256) void box_reset_kernel(unsigned int* box) {
+    if (threadIdx.x < 3) box[threadIdx.x] = 0xFFFFFFFFu;
+    else if (threadIdx.x < 6) box[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void centroid_box_kernel(CDView S, int kind, int n, unsigned int* box) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    if (i < n) {
+        const V3 c = prim_centroid(S, kind, i);
+        const float cc[3] = {c.x, c.y, c.z};
+        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = order_bits(cc[d]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        unsigned int a = lo[d], b = hi[d];
+        for (int o = 32; o > 0; o >>= 1) {
+            a = min(a, (unsigned int)__shfl_xor((int)a, o, 64));
+            b = max(b, (unsigned int)__shfl_xor((int)b, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(box + d, a);
+            atomicMax(box + 3 + d, b);
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned int expand_bits10(unsigned int v) {
+    // spread the low 10 bits of v to every third bit
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(CDView S, int kind, int n, const unsigned int* box,
+                                                     uint64_t* keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 c = prim_centroid(S, kind, i);
+    const float cc[3] = {c.x, c.y, c.z};
+    unsigned int q[3];
+    for (int d = 0; d < 3; ++d) {
+        const float lo = unorder_bits(box[d]), hi = unorder_bits(box[3 + d]);
+        const float ext = hi - lo;
+        float u = ext > 0.f ? (cc[d] - lo) / ext : 0.f;
+        u = fminf(fmaxf(u * 1024.f, 0.f), 1023.f);
+        q[d] = (unsigned int)u;
+    }
+    const unsigned int code = expand_bits10(q[0]) * 4u + expand_bits10(q[1]) * 2u + expand_bits10(q[2]);
+    keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+}
+
+__device__ __forceinline__ int common_prefix(const uint64_t* keys, int n, int i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    return __clzll(keys[i] ^ keys[j]);
+}
+
+// Binary radix tree over sorted unique keys (Karras 2012): internal node i covers a key range
+// starting or ending at i; its split is the highest differing bit in the range.
+__global__ __launch_bounds__(256) void build_tree_kernel(const uint64_t* keys, int n, BvhNode* nodes, int* parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {  // leaves
+        BvhNode& L = nodes[n - 1 + i];
+        L.lo.w = __int_as_float((int)(keys[i] & 0xFFFFFFFFu));
+        L.hi.w = __int_as_float(-1);
+    }
+    if (i == 0) parent[0] = -1;
+    if (i >= n - 1) return;
+    const int d = (common_prefix(keys, n, i, i + 1) - common_prefix(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = common_prefix(keys, n, i, (int64_t)i - d);
+    int64_t lmax = 2;
+    while (common_prefix(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (common_prefix(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = common_prefix(keys, n, i, j);
+    int64_t s = 0;
+    for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
+        if (common_prefix(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+    }
+    const int gamma = (int)(i + s * d + min(d, 0));
+    const int lo = (int)min((int64_t)i, j), hi = (int)max((int64_t)i, j);
+    const int left = lo == gamma ? n - 1 + gamma : gamma;
+    const int right = hi == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
+    nodes[i].lo.w = __int_as_float(left);
+    nodes[i].hi.w = __int_as_float(right);
+    parent[left] = i;
+    parent[right] = i;
+}
+
+__global__ __launch_bounds__(256) void depth_kernel(const int* parent, int nInternal, int* depth, uint64_t* keys,
+                                                    int* hist, int* error) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nInternal) return;
+    int dpt = 0;
+    for (int p = parent[i]; p >= 0 && dpt < 4 * kMaxDepth; p = parent[p]) ++dpt;
+    if (dpt >= kMaxDepth) {
+        atomicOr(error, 2);
+        dpt = kMaxDepth - 1;
+    }
+    depth[i] = dpt;
+    // deepest first: ascending (kMaxDepth - depth, node)
+    keys[i] = ((uint64_t)(kMaxDepth - dpt) << 32) | (uint32_t)i;
+    atomicAdd(hist + dpt, 1);
+}
+
+__global__ __launch_bounds__(256) void level_nodes_kernel(const uint64_t* keys, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int)(keys[i] & 0xFFFFFFFFu);
+}
+
+__global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNode* nodes, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    BvhNode& L = nodes[n - 1 + k];
+    const int prim = __float_as_int(L.lo.w);
+    V3 lo, hi;
+    if (kind == 0) {
+        const int4 f = S.faces[prim];
+        const V3 a = ld(S.pos, f.x), b = ld(S.pos, f.y), c = ld(S.pos, f.z);
+        lo = {fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y), fminf(fminf(a.z, b.z), c.z)};
+        hi = {fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y), fmaxf(fmaxf(a.z, b.z), c.z)};
+    } else {
+        const int4 e = S.edges[prim];
+        const V3 a = ld(S.pos, e.x), b = ld(S.pos, e.y);
+        lo = {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)};
+        hi = {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)};
+    }
+    L.lo = make_float4(lo.x, lo.y, lo.z, L.lo.w);
+    L.hi = make_float4(hi.x, hi.y, hi.z, L.hi.w);
+}
+
+__global__ __launch_bounds__(256) void internal_box_kernel(const int* levelNodes, int begin, int end,
+                                                           BvhNode* nodes) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const int i = levelNodes[k];
+    BvhNode& N = nodes[i];
+    const BvhNode A = nodes[__float_as_int(N.lo.w)];
+    const BvhNode B = nodes[__float_as_int(N.hi.w)];
+    N.lo = make_float4(fminf(A.lo.x, B.lo.x), fminf(A.lo.y, B.lo.y), fminf(A.lo.z, B.lo.z), N.lo.w);
+    N.hi = make_float4(fmaxf(A.hi.x, B.hi.x), fmaxf(A.hi.y, B.hi.y), fmaxf(A.hi.z, B.hi.z), N.hi.w);
+}
+
+__global__ __launch_bounds__(256) void fill_int_kernel(int* p, int64_t n, int value) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// ---- workload generator: x = rest + step * drift + sigma * N(0, 1) --------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float gaussian(uint64_t seed, int step, int64_t idx) {
+    const uint64_t z = mix64(seed ^ mix64((uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)idx));
+    const float u1 = ((float)(z >> 40) + 0.5f) * (1.0f / 16777216.0f);
+    const float u2 = ((float)((z >> 16) & 0xFFFFFFull) + 0.5f) * (1.0f / 16777216.0f);
+    return sqrtf(-2.f * logf(u1)) * cosf(6.28318530718f * u2);
+}
+
+__global__ __launch_bounds__(256) void perturb_kernel(const float4* rest, float4* pos, const int* vMesh,
+                                                      const float* drift, int nv, uint64_t seed, int step,
+                                                      float sigma) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const float4 r = rest[v];
+    const int m = vMesh[v];
+    const float fs = (float)step;
+    pos[v] = make_float4(r.x + fs * drift[3 * m] + sigma * gaussian(seed, step, 3 * (int64_t)v),
+                         r.y + fs * drift[3 * m + 1] + sigma * gaussian(seed, step, 3 * (int64_t)v + 1),
+                         r.z + fs * drift[3 * m + 2] + sigma * gaussian(seed, step, 3 * (int64_t)v + 2), 0.f);
+}
+
+inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+
+template <class T> T* ContactDetector::dalloc(size_t n) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(GVBD_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes for the contact detector failed");
+    }
+    allocs_.push_back(p);
+    bytes_ += bytes;
+    return (T*)p;
+}
+
+ContactDetector::ContactDetector(int device, hipStream_t stream, float maxQueryDis, int vfCap, int eeCap)
+    : device_(device), stream_(stream), r_(maxQueryDis), vfCap_(vfCap), eeCap_(eeCap) {
+    require(maxQueryDis > 0.f && std::isfinite(maxQueryDis), "max_query_distance must be positive");
+    require(vfCap >= 1 && vfCap <= kMaxCap && eeCap >= 1 && eeCap <= kMaxCap,
+            "contact capacities must be in [1, 16]");
+    meshV_.push_back(0);
+    meshF_.push_back(0);
+    meshE_.push_back(0);
+}
+
+ContactDetector::~ContactDetector() { freeAll(); }
+
+void ContactDetector::freeAll() {
+    for (void* p : allocs_) (void)hipFree(p);
+    allocs_.clear();
+    bytes_ = 0;
+}
+
+int ContactDetector::addMesh(int nv, const float* pos, int nf, const int* faces) {
+    if (ready_) fail(GVBD_ERR_STATE, "cannot add meshes after finalize");
+    require(nv > 0 && nf >= 0 && pos != nullptr && (nf == 0 || faces != nullptr), "bad mesh arguments");
+    std::vector<TriEdge> edges;
+    std::vector<int32_t> faceEdges;
+    std::string err;
+    if (!build_tri_edges(nv, nf, faces, edges, &faceEdges, err)) fail(GVBD_ERR_INVALID, err);
+    const int m = (int)meshV_.size() - 1;
+    const int vo = nv_, fo = nf_, eo = ne_;
+    hostPos_.insert(hostPos_.end(), pos, pos + 3 * (size_t)nv);
+    for (int i = 0; i < nv; ++i) hVMesh_.push_back(m);
+    for (int f = 0; f < nf; ++f) {
+        for (int k = 0; k < 3; ++k) hFaces_.push_back(vo + faces[3 * (size_t)f + k]);
+        hFaces_.push_back(m);
+        for (int k = 0; k < 3; ++k) hFaceEdges_.push_back(eo + faceEdges[3 * (size_t)f + k]);
+        hFaceEdges_.push_back(0);
+    }
+    for (const TriEdge& e : edges) {
+        hEdges_.push_back(vo + e.v1);
+        hEdges_.push_back(vo + e.v2);
+        hEdges_.push_back(e.v12Next < 0 ? -1 : vo + e.v12Next);
+        hEdges_.push_back(e.v21Next < 0 ? -1 : vo + e.v21Next);
+        hEMesh_.push_back(m);
+    }
+    nv_ += nv;
+    nf_ += nf;
+    ne_ += (int)edges.size();
+    meshV_.push_back(nv_);
+    meshF_.push_back(nf_);
+    meshE_.push_back(ne_);
+    return m;
+}
+
+void ContactDetector::meshOffsets(int mesh, int* v, int* f, int* e) const {
+    require(mesh >= 0 && mesh < numMeshes(), "mesh index out of range");
+    if (v) *v = meshV_[mesh];
+    if (f) *f = meshF_[mesh];
+    if (e) *e = meshE_[mesh];
+}
+
+void ContactDetector::topology(int32_t* edges, int32_t* faceEdges) const {
+    if (edges) std::memcpy(edges, hEdges_.data(), hEdges_.size() * sizeof(int32_t));
+    if (faceEdges)
+        for (int f = 0; f < nf_; ++f)
+            for (int k = 0; k < 3; ++k) faceEdges[3 * f + k] = hFaceEdges_[4 * (size_t)f + k];
+}
+
+void ContactDetector::finalize() {
+    if (ready_) fail(GVBD_ERR_STATE, "detector already finalized");
+    if (nv_ == 0) fail(GVBD_ERR_STATE, "detector has no meshes");
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    // CSR adjacency: vertex -> neighbour vertices, incident faces, incident edges
+    auto csr = [&](const std::vector<std::pair<int, int>>& items, std::vector<int>& begin, std::vector<int>& out) {
+        begin.assign(nv_ + 1, 0);
+        for (auto& it : items) begin[it.first + 1]++;
+        for (int i = 0; i < nv_; ++i) begin[i + 1] += begin[i];
+        out.assign(begin[nv_], 0);
+        std::vector<int> fill(begin.begin(), begin.end() - 1);
+        for (auto& it : items) out[fill[it.first]++] = it.second;
+    };
+    std::vector<std::pair<int, int>> vv, vf, ve;
+    for (int e = 0; e < ne_; ++e) {
+        const int a = hEdges_[4 * (size_t)e], b = hEdges_[4 * (size_t)e + 1];
+        vv.push_back({a, b});
+        vv.push_back({b, a});
+        ve.push_back({a, e});
+        ve.push_back({b, e});
+    }
+    for (int f = 0; f < nf_; ++f)
+        for (int k = 0; k < 3; ++k) vf.push_back({hFaces_[4 * (size_t)f + k], f});
+    std::vector<int> vvB, vvN, vfB, vfF, veB, veE;
+    csr(vv, vvB, vvN);
+    csr(vf, vfB, vfF);
+    csr(ve, veB, veE);
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "detector upload");
+    };
+    std::vector<float4> p4(nv_);
+    for (int i = 0; i < nv_; ++i) p4[i] = make_float4(hostPos_[3 * i], hostPos_[3 * i + 1], hostPos_[3 * i + 2], 0.f);
+    pos_ = dalloc<float4>(nv_);
+    rest_ = dalloc<float4>(nv_);
+    up(pos_, p4.data(), sizeof(float4) * nv_);
+    up(rest_, p4.data(), sizeof(float4) * nv_);
+    vMesh_ = dalloc<int>(nv_);
+    up(vMesh_, hVMesh_.data(), sizeof(int) * nv_);
+    faces_ = dalloc<int4>(nf_);
+    up(faces_, hFaces_.data(), sizeof(int4) * nf_);
+    faceEdges_ = dalloc<int4>(nf_);
+    up(faceEdges_, hFaceEdges_.data(), sizeof(int4) * nf_);
+    edges_ = dalloc<int4>(ne_);
+    up(edges_, hEdges_.data(), sizeof(int4) * ne_);
+    eMesh_ = dalloc<int>(ne_);
+    up(eMesh_, hEMesh_.data(), sizeof(int) * ne_);
+    vvBegin_ = dalloc<int>(nv_ + 1);
+    up(vvBegin_, vvB.data(), sizeof(int) * (nv_ + 1));
+    vvNei_ = dalloc<int>(vvN.size());
+    up(vvNei_, vvN.data(), sizeof(int) * vvN.size());
+    vfBegin_ = dalloc<int>(nv_ + 1);
+    up(vfBegin_, vfB.data(), sizeof(int) * (nv_ + 1));
+    vfFace_ = dalloc<int>(vfF.size());
+    up(vfFace_, vfF.data(), sizeof(int) * vfF.size());
+    veBegin_ = dalloc<int>(nv_ + 1);
+    up(veBegin_, veB.data(), sizeof(int) * (nv_ + 1));
+    veEdge_ = dalloc<int>(veE.size());
+    up(veEdge_, veE.data(), sizeof(int) * veE.size());
+    const int M = numMeshes();
+    drift_ = dalloc<float>(3 * (size_t)M);
+    sceneBox_ = dalloc<float>(8);
+    vfMin_ = dalloc<float>(nv_);
+    eeMin_ = dalloc<float>(ne_);
+    bounds_ = dalloc<float>(nv_);
+    faceMin_ = dalloc<int>(nf_);
+    vfPairCount_ = dalloc<int>(nv_ + 1);  // one extra zero: the scans' totals
+    vfStored_ = dalloc<int>(nv_);
+    vfOverflow_ = dalloc<int>(nv_);
+    eeCount_ = dalloc<int>(ne_ + 1);
+    eeStored_ = dalloc<int>(ne_);
+    eeOverflow_ = dalloc<int>(ne_);
+    vfRec_ = dalloc<VFContactRec>((size_t)nv_ * vfCap_);
+    eeRec_ = dalloc<EEContactRec>((size_t)ne_ * eeCap_);
+    vfOffset_ = dalloc<int64_t>(nv_ + 1);
+    eeOffset_ = dalloc<int64_t>(ne_ + 1);
+    error_ = dalloc<int>(64);
+    depthHist_ = dalloc<int>(kMaxDepth);
+    hip_check(hipMemsetAsync(error_, 0, 64 * sizeof(int), stream_), "detector clear");
+    hip_check(hipMemsetAsync(vfPairCount_, 0, sizeof(int) * (nv_ + 1), stream_), "detector clear");
+    hip_check(hipMemsetAsync(eeCount_, 0, sizeof(int) * (ne_ + 1), stream_), "detector clear");
+    // scan temp storage (sized for the larger of the two scans)
+    size_t a = 0, b = 0;
+    hip_check(rocprim::exclusive_scan(nullptr, a, vfPairCount_, vfOffset_, (int64_t)0, (size_t)nv_ + 1,
+                                      rocprim::plus<int64_t>(), stream_),
+              "scan size");
+    hip_check(rocprim::exclusive_scan(nullptr, b, eeCount_, eeOffset_, (int64_t)0, (size_t)ne_ + 1,
+                                      rocprim::plus<int64_t>(), stream_),
+              "scan size");
+    scanTempBytes_ = std::max(a, b);
+    scanTemp_ = dalloc<char>(scanTempBytes_);
+    // BVH storage
+    for (int kind = 0; kind < 2; ++kind) {
+        DeviceBvh& B = kind == 0 ? fb_ : eb_;
+        B.n = kind == 0 ? nf_ : ne_;
+        if (B.n == 0) continue;
+        B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
+        B.parent = dalloc<int>(2 * (size_t)B.n - 1);
+        B.keys = dalloc<uint64_t>(B.n);
+        B.keysAlt = dalloc<uint64_t>(B.n);
+        B.depthKeys = dalloc<uint64_t>(std::max(B.n - 1, 1));
+        B.depthKeysAlt = dalloc<uint64_t>(std::max(B.n - 1, 1));
+        B.depth = dalloc<int>(std::max(B.n - 1, 1));
+        B.levelNodes = dalloc<int>(std::max(B.n - 1, 1));
+        size_t s1 = 0, s2 = 0;
+        hip_check(rocprim::radix_sort_keys(nullptr, s1, B.keys, B.keysAlt, (size_t)B.n, 0, 62, stream_), "sort size");
+        hip_check(rocprim::radix_sort_keys(nullptr, s2, B.depthKeys, B.depthKeysAlt, (size_t)std::max(B.n - 1, 1), 0,
+                                           40, stream_),
+                  "sort size");
+        B.sortTempBytes = std::max(s1, s2);
+        B.sortTemp = dalloc<char>(B.sortTempBytes);
+    }
+    hip_check(hipStreamSynchronize(stream_), "detector upload sync");
+    ready_ = true;
+    updateBvh(true);
+}
+
+namespace {
+CDView make_view(float4* pos, int nv, int nf, int ne) {
+    CDView v{};
+    v.pos = pos;
+    v.nv = nv;
+    v.nf = nf;
+    v.ne = ne;
+    return v;
+}
+}  // namespace
+
+void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
+    if (B.n == 0) return;
+    CDView S = make_view(pos_, nv_, nf_, ne_);
+    S.faces = faces_;
+    S.edges = edges_;
+    const int n = B.n;
+    unsigned int* box = (unsigned int*)sceneBox_;
+    hipLaunchKernelGGL(box_reset_kernel, dim3(1), dim3(64), 0, stream_, box);
+    hipLaunchKernelGGL(centroid_box_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, box);
+    hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, box, B.keys);
+    size_t tb = B.sortTempBytes;
+    hip_check(rocprim::radix_sort_keys(B.sortTemp, tb, B.keys, B.keysAlt, (size_t)n, 0, 62, stream_), "morton sort");
+    hipLaunchKernelGGL(build_tree_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, B.keysAlt, n, B.nodes, B.parent);
+    hip_check(hipGetLastError(), "bvh build launch");
+    // refit schedule: internal nodes by depth, deepest first
+    B.levelBegin.clear();
+    const int ni = n - 1;
+    if (ni > 0) {
+        std::vector<int> h(kMaxDepth, 0);
+        int* dh = depthHist_;
+        hip_check(hipMemsetAsync(dh, 0, kMaxDepth * sizeof(int), stream_), "hist clear");
+        hipLaunchKernelGGL(depth_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, B.parent, ni, B.depth,
+                           B.depthKeys, dh, error_);
+        size_t tb2 = B.sortTempBytes;
+        hip_check(rocprim::radix_sort_keys(B.sortTemp, tb2, B.depthKeys, B.depthKeysAlt, (size_t)ni, 0, 40, stream_),
+                  "depth sort");
+        hipLaunchKernelGGL(level_nodes_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, B.depthKeysAlt, ni,
+                           B.levelNodes);
+        hip_check(hipMemcpyAsync(h.data(), dh, kMaxDepth * sizeof(int), hipMemcpyDeviceToHost, stream_), "hist read");
+        hip_check(hipStreamSynchronize(stream_), "bvh build sync");
+        int at = 0;
+        for (int d = kMaxDepth - 1; d >= 0; --d) {
+            if (h[d] == 0) continue;
+            B.levelBegin.push_back(at);
+            at += h[d];
+        }
+        B.levelBegin.push_back(at);
+    }
+    refitBvh(B, kind);
+}
+
+void ContactDetector::refitBvh(DeviceBvh& B, int kind) {
+    if (B.n == 0) return;
+    CDView S = make_view(pos_, nv_, nf_, ne_);
+    S.faces = faces_;
+    S.edges = edges_;
+    hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream_, S, kind, B.nodes, B.n);
+    for (size_t l = 0; l + 1 < B.levelBegin.size(); ++l) {
+        const int b = B.levelBegin[l], e = B.levelBegin[l + 1];
+        hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_, B.levelNodes, b, e,
+                           B.nodes);
+    }
+    hip_check(hipGetLastError(), "bvh refit launch");
+}
+
+void ContactDetector::updateBvh(bool rebuild) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    if (rebuild) {
+        buildBvh(fb_, 0);
+        buildBvh(eb_, 1);
+    } else {
+        refitBvh(fb_, 0);
+        refitBvh(eb_, 1);
+    }
+}
+
+void ContactDetector::setPositions(int mesh, const float* pos) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    require(mesh >= 0 && mesh < numMeshes(), "mesh index out of range");
+    const int off = meshV_[mesh], n = meshV_[mesh + 1] - off;
+    std::vector<float4> p4(n);
+    for (int i = 0; i < n; ++i) p4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+    hip_check(hipMemcpyAsync(pos_ + off, p4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream_), "positions");
+    hip_check(hipStreamSynchronize(stream_), "positions sync");
+}
+
+void ContactDetector::getPositions(int mesh, float* pos) const {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    require(mesh >= 0 && mesh < numMeshes(), "mesh index out of range");
+    const int off = meshV_[mesh], n = meshV_[mesh + 1] - off;
+    std::vector<float4> p4(n);
+    hip_check(hipMemcpyAsync(p4.data(), pos_ + off, sizeof(float4) * n, hipMemcpyDeviceToHost, stream_), "positions");
+    hip_check(hipStreamSynchronize(stream_), "positions sync");
+    for (int i = 0; i < n; ++i) {
+        pos[3 * i] = p4[i].x;
+        pos[3 * i + 1] = p4[i].y;
+        pos[3 * i + 2] = p4[i].z;
+    }
+}
+
+void ContactDetector::perturb(uint64_t seed, int step, float sigma, const float* drift) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    const int M = numMeshes();
+    std::vector<float> d(3 * (size_t)M, 0.f);
+    if (drift) std::memcpy(d.data(), drift, d.size() * sizeof(float));
+    hip_check(hipMemcpyAsync(drift_, d.data(), d.size() * sizeof(float), hipMemcpyHostToDevice, stream_), "drift");
+    hipLaunchKernelGGL(perturb_kernel, dim3(blocks_for(nv_)), dim3(256), 0, stream_, rest_, pos_, vMesh_, drift_, nv_,
+                       seed, step, sigma);
+    hip_check(hipGetLastError(), "perturb launch");
+    hip_check(hipStreamSynchronize(stream_), "perturb sync");
+}
+
+void ContactDetector::query(bool record) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    CDView S = make_view(pos_, nv_, nf_, ne_);
+    S.vMesh = vMesh_;
+    S.faces = faces_;
+    S.faceEdges = faceEdges_;
+    S.edges = edges_;
+    S.eMesh = eMesh_;
+    S.vvBegin = vvBegin_;
+    S.vvNei = vvNei_;
+    S.fNodes = fb_.nodes;
+    S.fN = fb_.n;
+    S.eNodes = eb_.nodes;
+    S.eN = eb_.n;
+    S.r = r_;
+    S.vfCap = vfCap_;
+    S.eeCap = eeCap_;
+    S.vfMin = vfMin_;
+    S.faceMin = faceMin_;
+    S.eeMin = eeMin_;
+    S.vfPairCount = vfPairCount_;
+    S.vfStored = vfStored_;
+    S.vfOverflow = vfOverflow_;
+    S.eeCount = eeCount_;
+    S.eeStored = eeStored_;
+    S.eeOverflow = eeOverflow_;
+    S.vfRec = vfRec_;
+    S.eeRec = eeRec_;
+    S.error = error_;
+    float rr = r_;
+    int rbits;
+    std::memcpy(&rbits, &rr, sizeof(int));
+    hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nf_), 4096)), dim3(256), 0, stream_, faceMin_,
+                       (int64_t)nf_, rbits);
+    hipLaunchKernelGGL((vf_query_kernel<false>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
+    hipLaunchKernelGGL((ee_query_kernel<false>), dim3(blocks_for(ne_)), dim3(256), 0, stream_, S);
+    hip_check(hipGetLastError(), "query launch");
+    recorded_ = false;
+    if (!record) return;
+    // record pass: offsets from the counts, then the uncapped lists
+    size_t tb = scanTempBytes_;
+    hip_check(rocprim::exclusive_scan(scanTemp_, tb, vfPairCount_, vfOffset_, (int64_t)0, (size_t)nv_ + 1,
+                                      rocprim::plus<int64_t>(), stream_),
+              "pair scan");
+    tb = scanTempBytes_;
+    hip_check(rocprim::exclusive_scan(scanTemp_, tb, eeCount_, eeOffset_, (int64_t)0, (size_t)ne_ + 1,
+                                      rocprim::plus<int64_t>(), stream_),
+              "ee scan");
+    hip_check(hipMemcpyAsync(&nRecPairs_, vfOffset_ + nv_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_), "total");
+    hip_check(hipMemcpyAsync(&nRecEE_, eeOffset_ + ne_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_), "total");
+    hip_check(hipStreamSynchronize(stream_), "record sync");
+    if (nRecPairs_ > capPairs_) {
+        capPairs_ = std::max<int64_t>(nRecPairs_, 2 * capPairs_);
+        recVF_ = dalloc<int2>(capPairs_);
+        recVFd_ = dalloc<float>(capPairs_);
+        recKeys_ = dalloc<int4>(capPairs_);
+        recKeysD_ = dalloc<float>(capPairs_);
+    }
+    if (nRecEE_ > capEE_) {
+        capEE_ = std::max<int64_t>(nRecEE_, 2 * capEE_);
+        recEE_ = dalloc<int2>(capEE_);
+        recEEmu_ = dalloc<float2>(capEE_);
+        recEEd_ = dalloc<float>(capEE_);
+    }
+    S.vfOffset = vfOffset_;
+    S.eeOffset = eeOffset_;
+    S.recVF = recVF_;
+    S.recVFd = recVFd_;
+    S.recKeys = recKeys_;
+    S.recKeysD = recKeysD_;
+    S.recEE = recEE_;
+    S.recEEmu = recEEmu_;
+    S.recEEd = recEEd_;
+    hipLaunchKernelGGL((vf_query_kernel<true>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
+    hipLaunchKernelGGL((ee_query_kernel<true>), dim3(blocks_for(ne_)), dim3(256), 0, stream_, S);
+    hip_check(hipGetLastError(), "record query launch");
+    hip_check(hipStreamSynchronize(stream_), "record query sync");
+    recorded_ = true;
+}
+
+void ContactDetector::bounds(float thickness, float relaxation) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nv_)), dim3(256), 0, stream_, vfMin_, faceMin_, eeMin_,
+                       vfBegin_, vfFace_, veBegin_, veEdge_, nv_, r_, thickness, relaxation, bounds_);
+    hip_check(hipGetLastError(), "bounds launch");
+}
+
+namespace {
+template <class T> std::vector<T> download(const T* d, size_t n, hipStream_t s) {
+    std::vector<T> h(n);
+    if (n) hip_check(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "download sync");
+    return h;
+}
+}  // namespace
+
+DetectorResultCounts ContactDetector::counts() {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    const auto err = download(error_, 1, stream_);
+    if (err[0] & 1) fail(GVBD_ERR_INTERNAL, "BVH traversal stack overflow");
+    if (err[0] & 2) fail(GVBD_ERR_INTERNAL, "BVH deeper than the refit schedule supports");
+    if (err[0] & 4) fail(GVBD_ERR_INTERNAL, "record pass found more contacts than the counting pass");
+    const auto pc = download(vfPairCount_, nv_, stream_);
+    const auto st = download(vfStored_, nv_, stream_);
+    const auto vo = download(vfOverflow_, nv_, stream_);
+    const auto ec = download(eeCount_, ne_, stream_);
+    const auto eo = download(eeOverflow_, ne_, stream_);
+    DetectorResultCounts c{};
+    for (int i = 0; i < nv_; ++i) {
+        c.vfPairs += pc[i];
+        c.vfContacts += st[i];
+        c.vfOverflow += vo[i];
+    }
+    for (int i = 0; i < ne_; ++i) {
+        c.eeContacts += ec[i];
+        c.eeOverflow += eo[i];
+    }
+    if (recorded_) {
+        // exact distinct-key count from the record pass
+        const auto keys = download(recKeys_, nRecPairs_, stream_);
+        c.vfContacts = 0;
+        for (const int4& k : keys) c.vfContacts += k.y >= 0;
+    }
+    return c;
+}
+
+void ContactDetector::vfPairs(int32_t* vf, float* d) {
+    if (!recorded_) fail(GVBD_ERR_STATE, "no recorded query (call query with record = 1)");
+    const auto p = download(recVF_, nRecPairs_, stream_);
+    const auto dd = download(recVFd_, nRecPairs_, stream_);
+    for (int64_t i = 0; i < nRecPairs_; ++i) {
+        if (vf) {
+            vf[2 * i] = p[i].x;
+            vf[2 * i + 1] = p[i].y;
+        }
+        if (d) d[i] = dd[i];
+    }
+}
+
+void ContactDetector::vfContactsAll(int32_t* rec, float* d) {
+    if (!recorded_) fail(GVBD_ERR_STATE, "no recorded query (call query with record = 1)");
+    const auto k = download(recKeys_, nRecPairs_, stream_);
+    const auto dd = download(recKeysD_, nRecPairs_, stream_);
+    int64_t o = 0;
+    for (int64_t i = 0; i < nRecPairs_; ++i) {
+        if (k[i].y < 0) continue;
+        if (rec) {
+            rec[4 * o] = k[i].x;
+            rec[4 * o + 1] = k[i].y;
+            rec[4 * o + 2] = k[i].z;
+            rec[4 * o + 3] = k[i].w;
+        }
+        if (d) d[o] = dd[i];
+        ++o;
+    }
+}
+
+void ContactDetector::eeContactsAll(int32_t* ee, float* mu, float* d) {
+    if (!recorded_) fail(GVBD_ERR_STATE, "no recorded query (call query with record = 1)");
+    const auto p = download(recEE_, nRecEE_, stream_);
+    const auto m = download(recEEmu_, nRecEE_, stream_);
+    const auto dd = download(recEEd_, nRecEE_, stream_);
+    for (int64_t i = 0; i < nRecEE_; ++i) {
+        if (ee) {
+            ee[2 * i] = p[i].x;
+            ee[2 * i + 1] = p[i].y;
+        }
+        if (mu) {
+            mu[2 * i] = m[i].x;
+            mu[2 * i + 1] = m[i].y;
+        }
+        if (d) d[i] = dd[i];
+    }
+}
+
+void ContactDetector::vfContactsCapped(int32_t* count, VFContactRec* rec) {
+    if (count) {
+        const auto c = download(vfStored_, nv_, stream_);
+        std::memcpy(count, c.data(), sizeof(int) * nv_);
+    }
+    if (rec) {
+        const auto r = download(vfRec_, (size_t)nv_ * vfCap_, stream_);
+        std::memcpy(rec, r.data(), sizeof(VFContactRec) * r.size());
+    }
+}
+
+void ContactDetector::eeContactsCapped(int32_t* count, EEContactRec* rec) {
+    if (count) {
+        const auto c = download(eeStored_, ne_, stream_);
+        std::memcpy(count, c.data(), sizeof(int) * ne_);
+    }
+    if (rec) {
+        const auto r = download(eeRec_, (size_t)ne_ * eeCap_, stream_);
+        std::memcpy(rec, r.data(), sizeof(EEContactRec) * r.size());
+    }
+}
+
+void ContactDetector::minDistances(float* vf, float* face, float* edge) {
+    if (vf) {
+        const auto a = download(vfMin_, nv_, stream_);
+        std::memcpy(vf, a.data(), sizeof(float) * nv_);
+    }
+    if (face) {
+        const auto a = download(faceMin_, nf_, stream_);
+        std::memcpy(face, a.data(), sizeof(float) * nf_);
+    }
+    if (edge) {
+        const auto a = download(eeMin_, ne_, stream_);
+        std::memcpy(edge, a.data(), sizeof(float) * ne_);
+    }
+}
+
+void ContactDetector::boundsOut(float* b) {
+    const auto a = download(bounds_, nv_, stream_);
+    std::memcpy(b, a.data(), sizeof(float) * nv_);
+}
+
+}  // namespace gvbd

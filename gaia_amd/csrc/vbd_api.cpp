@@ -109,7 +109,8 @@ struct gvbd_context {
     std::map<int32_t, std::unique_ptr<gvbd::SceneObj>> scenes;
     std::map<int32_t, std::unique_ptr<gvbd::PlanObj>> plans;
     std::map<int32_t, int32_t> batches;  // batch id -> plan id
-    int32_t nextScene = 0, nextPlan = 0, nextBatch = 0;
+    std::map<int32_t, std::unique_ptr<gvbd::ContactDetector>> detectors;
+    int32_t nextScene = 0, nextPlan = 0, nextBatch = 0, nextDetector = 0;
 };
 
 namespace gvbd {
@@ -773,6 +774,7 @@ int gvbd_context_destroy(gvbd_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->plans) free_plan(*kv.second);
+    ctx->detectors.clear();
     for (auto& kv : ctx->scenes)
         if (kv.second->dev.arena) (void)hipFree(kv.second->dev.arena);
     for (auto& e : ctx->events)
@@ -799,6 +801,7 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes) {
         int64_t b = 0;
         for (auto& kv : ctx->scenes) b += (int64_t)kv.second->dev.bytes;
         for (auto& kv : ctx->plans) b += (int64_t)kv.second->bytes;
+        for (auto& kv : ctx->detectors) b += (int64_t)kv.second->deviceBytes();
         *bytes = b;
     });
 }
@@ -1339,6 +1342,138 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
         if (bytes) *bytes = b;
         if (vertices) *vertices = nvx;
     });
+}
+
+// ---- contact detector ----
+
+static ContactDetector& detector_of(gvbd_context* ctx, int32_t cd) {
+    require(ctx != nullptr, "ctx is NULL");
+    auto it = ctx->detectors.find(cd);
+    if (it == ctx->detectors.end()) fail(GVBD_ERR_INVALID, "unknown detector id " + std::to_string(cd));
+    return *it->second;
+}
+
+int gvbd_cd_create(gvbd_context* ctx, float max_query_distance, int32_t vf_capacity, int32_t ee_capacity,
+                   int32_t* cd_out) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr && cd_out != nullptr, "NULL argument");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        auto d = std::make_unique<ContactDetector>(ctx->device, ctx->stream, max_query_distance, vf_capacity,
+                                                   ee_capacity);
+        const int32_t id = ctx->nextDetector++;
+        ctx->detectors[id] = std::move(d);
+        *cd_out = id;
+    });
+}
+
+int gvbd_cd_destroy(gvbd_context* ctx, int32_t cd) {
+    return guard(ctx, [&] {
+        detector_of(ctx, cd);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        ctx->detectors.erase(cd);
+    });
+}
+
+int gvbd_cd_add_mesh(gvbd_context* ctx, int32_t cd, int32_t num_vertices, const float* positions, int32_t num_faces,
+                     const int32_t* faces, int32_t* mesh_out) {
+    return guard(ctx, [&] {
+        const int32_t m = detector_of(ctx, cd).addMesh(num_vertices, positions, num_faces, faces);
+        if (mesh_out) *mesh_out = m;
+    });
+}
+
+int gvbd_cd_finalize(gvbd_context* ctx, int32_t cd) {
+    return guard(ctx, [&] { detector_of(ctx, cd).finalize(); });
+}
+
+int gvbd_cd_sizes(gvbd_context* ctx, int32_t cd, int32_t* num_vertices, int32_t* num_faces, int32_t* num_edges) {
+    return guard(ctx, [&] {
+        ContactDetector& d = detector_of(ctx, cd);
+        if (num_vertices) *num_vertices = d.numVertices();
+        if (num_faces) *num_faces = d.numFaces();
+        if (num_edges) *num_edges = d.numEdges();
+    });
+}
+
+int gvbd_cd_mesh_offsets(gvbd_context* ctx, int32_t cd, int32_t mesh, int32_t* vertex, int32_t* face, int32_t* edge) {
+    return guard(ctx, [&] { detector_of(ctx, cd).meshOffsets(mesh, vertex, face, edge); });
+}
+
+int gvbd_cd_topology(gvbd_context* ctx, int32_t cd, int32_t* edges, int32_t* face_edges) {
+    return guard(ctx, [&] { detector_of(ctx, cd).topology(edges, face_edges); });
+}
+
+int gvbd_cd_set_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, const float* positions) {
+    return guard(ctx, [&] {
+        require(positions != nullptr, "positions is NULL");
+        detector_of(ctx, cd).setPositions(mesh, positions);
+    });
+}
+
+int gvbd_cd_get_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, float* positions) {
+    return guard(ctx, [&] {
+        require(positions != nullptr, "positions is NULL");
+        detector_of(ctx, cd).getPositions(mesh, positions);
+    });
+}
+
+int gvbd_cd_perturb(gvbd_context* ctx, int32_t cd, uint64_t seed, int32_t step, float sigma, const float* drift) {
+    return guard(ctx, [&] { detector_of(ctx, cd).perturb(seed, step, sigma, drift); });
+}
+
+int gvbd_cd_update_bvh(gvbd_context* ctx, int32_t cd, int32_t rebuild) {
+    return guard(ctx, [&] { detector_of(ctx, cd).updateBvh(rebuild != 0); });
+}
+
+int gvbd_cd_query(gvbd_context* ctx, int32_t cd, int32_t record) {
+    return guard(ctx, [&] { detector_of(ctx, cd).query(record != 0); });
+}
+
+int gvbd_cd_bounds(gvbd_context* ctx, int32_t cd, float thickness, float relaxation, float* bounds) {
+    return guard(ctx, [&] {
+        ContactDetector& d = detector_of(ctx, cd);
+        d.bounds(thickness, relaxation);
+        if (bounds) d.boundsOut(bounds);
+    });
+}
+
+int gvbd_cd_counts(gvbd_context* ctx, int32_t cd, int64_t* counts) {
+    return guard(ctx, [&] {
+        require(counts != nullptr, "counts is NULL");
+        const DetectorResultCounts c = detector_of(ctx, cd).counts();
+        counts[0] = c.vfPairs;
+        counts[1] = c.vfContacts;
+        counts[2] = c.eeContacts;
+        counts[3] = c.vfOverflow;
+        counts[4] = c.eeOverflow;
+    });
+}
+
+int gvbd_cd_vf_pairs(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* d) {
+    return guard(ctx, [&] { detector_of(ctx, cd).vfPairs(pairs, d); });
+}
+
+int gvbd_cd_vf_contacts(gvbd_context* ctx, int32_t cd, int32_t* records, float* d) {
+    return guard(ctx, [&] { detector_of(ctx, cd).vfContactsAll(records, d); });
+}
+
+int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu, float* d) {
+    return guard(ctx, [&] { detector_of(ctx, cd).eeContactsAll(pairs, mu, d); });
+}
+
+static_assert(sizeof(gvbd_vf_contact) == sizeof(VFContactRec), "gvbd_vf_contact layout");
+static_assert(sizeof(gvbd_ee_contact) == sizeof(EEContactRec), "gvbd_ee_contact layout");
+
+int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records) {
+    return guard(ctx, [&] { detector_of(ctx, cd).vfContactsCapped(count, (VFContactRec*)records); });
+}
+
+int gvbd_cd_ee_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_ee_contact* records) {
+    return guard(ctx, [&] { detector_of(ctx, cd).eeContactsCapped(count, (EEContactRec*)records); });
+}
+
+int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex, float* face, float* edge) {
+    return guard(ctx, [&] { detector_of(ctx, cd).minDistances(vertex, face, edge); });
 }
 
 int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms) {

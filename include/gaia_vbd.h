@@ -226,6 +226,67 @@ int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n)
 /* algorithmic bytes of one solve launch of `group` (gather model; see DESIGN) and its vertex count */
 int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
 
+/* ---- contact detector (GPU LBVH) ----
+ * Replaces struct ClothContactDetector (CollisionDetector/ClothContactDetector.h:133-182): Embree
+ * scenes of faces and edges become linear BVHs on the device. Ids are global over the meshes in
+ * the order they were added (gvbd_cd_mesh_offsets gives each mesh's first vertex / face / edge);
+ * edges are numbered per mesh by first appearance over its faces (eV1 -> eV2 the direction in the
+ * first face). Contact records (16 bytes each): */
+typedef struct gvbd_vf_contact {
+    int32_t face;  /* representative face: lowest face id whose closest point is this primitive */
+    int32_t type;  /* primitive of the closest point: 0 vertex, 1 edge, 2 face interior */
+    int32_t prim;  /* vertex, edge or face id */
+    float d;       /* distance */
+} gvbd_vf_contact;
+typedef struct gvbd_ee_contact {
+    int32_t edge;  /* the other edge */
+    float mu1;     /* closest point on the query edge: lerp(eV1, eV2, mu1) */
+    float mu2;     /* closest point on the other edge */
+    float d;
+} gvbd_ee_contact;
+
+/* ClothContactDetector(params) with params.maxQueryDis = max_query_distance; at most
+ * vf_capacity (1..16) contacts are kept per vertex and ee_capacity per edge (the reference's
+ * VF_CONTACT_PREALLOCATE / EE_CONTACT_PREALLOCATE = 8; more are counted as overflow) */
+int gvbd_cd_create(gvbd_context* ctx, float max_query_distance, int32_t vf_capacity, int32_t ee_capacity,
+                   int32_t* cd_out);
+int gvbd_cd_destroy(gvbd_context* ctx, int32_t cd);
+/* ClothContactDetector::initialize(meshes): one call per mesh, then finalize (uploads, builds) */
+int gvbd_cd_add_mesh(gvbd_context* ctx, int32_t cd, int32_t num_vertices, const float* positions, int32_t num_faces,
+                     const int32_t* faces, int32_t* mesh_out);
+int gvbd_cd_finalize(gvbd_context* ctx, int32_t cd);
+int gvbd_cd_sizes(gvbd_context* ctx, int32_t cd, int32_t* num_vertices, int32_t* num_faces, int32_t* num_edges);
+int gvbd_cd_mesh_offsets(gvbd_context* ctx, int32_t cd, int32_t mesh, int32_t* vertex, int32_t* face, int32_t* edge);
+/* edges 4 per edge (eV1, eV2, eV12Next, eV21Next; -1 none), face_edges 3 per face (AB, BC, CA) */
+int gvbd_cd_topology(gvbd_context* ctx, int32_t cd, int32_t* edges, int32_t* face_edges);
+int gvbd_cd_set_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, const float* positions);
+int gvbd_cd_get_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, float* positions);
+/* workload generator (no reference counterpart): x = x_initial + step * drift[mesh] + sigma * N(0,1),
+ * the noise drawn per (seed, step, vertex, axis); drift holds 3 floats per mesh (may be NULL) */
+int gvbd_cd_perturb(gvbd_context* ctx, int32_t cd, uint64_t seed, int32_t step, float sigma, const float* drift);
+/* ClothContactDetector::updateBVH: rebuild = 1 re-sorts and rebuilds (RTC_BUILD_QUALITY_LOW),
+ * 0 refits the boxes (RTC_BUILD_QUALITY_REFIT) (ClothContactDetector.cpp:1137-1181) */
+int gvbd_cd_update_bvh(gvbd_context* ctx, int32_t cd, int32_t rebuild);
+/* contactQueryVF for every vertex and contactQueryEE for every edge, with resetFaceContactInfo
+ * (VBDClothPhysics.cpp:735-757); record = 1 also keeps the uncapped lists (gvbd_cd_vf_pairs,
+ * gvbd_cd_vf_contacts, gvbd_cd_ee_contacts) */
+int gvbd_cd_query(gvbd_context* ctx, int32_t cd, int32_t record);
+/* conservative bounds of applyCollisionDetection (VBDClothPhysics.cpp:775-812); bounds may be NULL */
+int gvbd_cd_bounds(gvbd_context* ctx, int32_t cd, float thickness, float relaxation, float* bounds);
+/* counts[5]: VF candidate pairs, VF contacts, EE contacts, vertices and edges that overflowed */
+int gvbd_cd_counts(gvbd_context* ctx, int32_t cd, int64_t* counts);
+/* recorded lists (after gvbd_cd_query(record = 1)): (vertex, face) pairs with d < r; VF contacts
+ * as (vertex, type, prim, face); EE contacts as (edge, other edge) with mu (2 each) */
+int gvbd_cd_vf_pairs(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* d);
+int gvbd_cd_vf_contacts(gvbd_context* ctx, int32_t cd, int32_t* records, float* d);
+int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu, float* d);
+/* capped per-primitive results of the last query: count[nv] and vf_capacity records per vertex;
+ * count[ne] and ee_capacity records per edge */
+int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records);
+int gvbd_cd_ee_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_ee_contact* records);
+/* minDisToPrimitives of every vertex (VF) and edge (EE), faceMinDisToVertices of every face */
+int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex, float* face, float* edge);
+
 /* ---- timing on the context stream ---- */
 /* copies `bytes` (multiple of 16) between two device buffers `reps` times with a plain
  * streaming kernel and returns the average kernel time: the measured copy bandwidth and
