@@ -51,6 +51,8 @@ class PhysicsParams(ctypes.Structure):
         ("thickness", ctypes.c_float),
         ("conservative_step_relaxation", ctypes.c_float),
         ("contact_energy_type", ctypes.c_int32),
+        ("contact_query_radius", ctypes.c_float),
+        ("contact_bvh_rebuild_steps", ctypes.c_int32),
     ]
 
 
@@ -103,6 +105,8 @@ class ClothMeshDesc(ctypes.Structure):
         ("fixed_points", c_int32_p),
         ("vertex_colors", c_int32_p),
         ("coloring_algorithm", ctypes.c_int32),
+        ("friction_dynamic", ctypes.c_float),
+        ("friction_eps_v", ctypes.c_float),
     ]
 
 
@@ -154,6 +158,7 @@ _SIGNATURES = {
     "gvbd_scene_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_scene_run_iteration": [_P, _I, _I],
     "gvbd_scene_update_velocities": [_P, _I],
+    "gvbd_scene_collision_detection": [_P, _I],
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
     "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],

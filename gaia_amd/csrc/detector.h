@@ -52,6 +52,23 @@ struct DetectorResultCounts {
     int64_t vfPairs, vfContacts, eeContacts, vfOverflow, eeOverflow;
 };
 
+// Device pointers a solver reads after a detection (all sized by the detector's counts).
+struct DetectorDeviceView {
+    const int4* faces;
+    const int4* edges;
+    const int* vfStored;
+    const VFContactRec* vfRec;
+    int vfCap;
+    const int* eeStored;
+    const EEContactRec* eeRec;
+    int eeCap;
+    const int2* fvRange;
+    const uint64_t* fvKeys;
+    const float* bounds;
+    const float4* xAtDetection;
+    int* flag;
+};
+
 class ContactDetector {
   public:
     ContactDetector(int device, hipStream_t stream, float maxQueryDis, int vfCap, int eeCap);
@@ -60,7 +77,16 @@ class ContactDetector {
     ContactDetector& operator=(const ContactDetector&) = delete;
 
     int addMesh(int nv, const float* pos, int nf, const int* faces);
+    // Queries read (and refits follow) positions owned by the caller, e.g. a scene's x array in
+    // the same vertex order; call before finalize.
+    void useExternalPositions(float4* pos);
     void finalize();
+    // Face back-links of the stored VF contacts (ClothContactDetector::faceContactInfos): per
+    // face the lowest `cap` vertex ids whose stored contacts name that face.
+    void buildFaceLinks(int cap);
+    // positionsAtPrevCollisionDetection = x; clears the re-detection flag
+    void snapshotPositions();
+    DetectorDeviceView deviceView() const;
     bool finalized() const { return ready_; }
 
     void setPositions(int mesh, const float* pos);
@@ -140,6 +166,14 @@ class ContactDetector {
     bool recorded_ = false;
     int* error_ = nullptr;      // bit 0: traversal stack overflow, bit 1: tree too deep
     int* depthHist_ = nullptr;  // per-depth internal node counts (build)
+    float4* external_ = nullptr;
+    // face back-links and the solver-side state
+    uint64_t *fvKeys_ = nullptr, *fvKeysAlt_ = nullptr;
+    int2* fvRange_ = nullptr;
+    void* fvSortTemp_ = nullptr;
+    size_t fvSortTempBytes_ = 0;
+    float4* xAtCD_ = nullptr;
+    int* flag_ = nullptr;
 };
 
 }  // namespace gvbd

@@ -692,9 +692,66 @@ __global__ __launch_bounds__(256) void perturb_kernel(const float4* rest, float4
                          r.z + fs * drift[3 * m + 2] + sigma * gaussian(seed, step, 3 * (int64_t)v + 2), 0.f);
 }
 
+// face back-links: one key (face << 32 | vertex) per stored VF contact, all-ones for empty slots
+__global__ __launch_bounds__(256) void fv_keys_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
+                                                      uint64_t* keys) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nv * cap) return;
+    const int v = (int)(i / cap), k = (int)(i % cap);
+    keys[i] = k < stored[v] ? (((uint64_t)rec[i].face << 32) | (uint32_t)v) : ~0ull;
+}
+
+__device__ __forceinline__ int64_t lower_bound_key(const uint64_t* keys, int64_t n, uint64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void fv_range_kernel(const uint64_t* keys, int64_t n, int nf, int cap,
+                                                       int2* range) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int64_t b = lower_bound_key(keys, n, (uint64_t)f << 32);
+    const int64_t e = lower_bound_key(keys, n, (uint64_t)(f + 1) << 32);
+    range[f] = make_int2((int)b, (int)std::min<int64_t>(e, b + cap));
+}
+
 inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
 
 }  // namespace
+
+void ContactDetector::useExternalPositions(float4* pos) {
+    if (ready_) fail(GVBD_ERR_STATE, "detector already finalized");
+    external_ = pos;
+}
+
+void ContactDetector::buildFaceLinks(int cap) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    const int64_t n = (int64_t)nv_ * vfCap_;
+    hipLaunchKernelGGL(fv_keys_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
+                       fvKeysAlt_);
+    size_t tb = fvSortTempBytes_;
+    hip_check(rocprim::radix_sort_keys(fvSortTemp_, tb, fvKeysAlt_, fvKeys_, (size_t)n, 0, 64, stream_),
+              "face link sort");
+    hipLaunchKernelGGL(fv_range_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, fvKeys_, n, nf_, cap,
+                       fvRange_);
+    hip_check(hipGetLastError(), "face link launch");
+}
+
+void ContactDetector::snapshotPositions() {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    hip_check(hipMemcpyAsync(xAtCD_, pos_, sizeof(float4) * nv_, hipMemcpyDeviceToDevice, stream_), "snapshot");
+    hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
+}
+
+DetectorDeviceView ContactDetector::deviceView() const {
+    return DetectorDeviceView{faces_, edges_,  vfStored_, vfRec_, vfCap_, eeStored_, eeRec_,
+                              eeCap_, fvRange_, fvKeys_,  bounds_, xAtCD_, flag_};
+}
 
 // ---------------------------------------------------------------------------------------------
 // host side
@@ -809,10 +866,25 @@ void ContactDetector::finalize() {
     };
     std::vector<float4> p4(nv_);
     for (int i = 0; i < nv_; ++i) p4[i] = make_float4(hostPos_[3 * i], hostPos_[3 * i + 1], hostPos_[3 * i + 2], 0.f);
-    pos_ = dalloc<float4>(nv_);
-    rest_ = dalloc<float4>(nv_);
-    up(pos_, p4.data(), sizeof(float4) * nv_);
-    up(rest_, p4.data(), sizeof(float4) * nv_);
+    if (external_) {
+        pos_ = external_;  // the owner keeps it current
+    } else {
+        pos_ = dalloc<float4>(nv_);
+        rest_ = dalloc<float4>(nv_);
+        up(pos_, p4.data(), sizeof(float4) * nv_);
+        up(rest_, p4.data(), sizeof(float4) * nv_);
+    }
+    // face back-links and solver-side state
+    fvKeys_ = dalloc<uint64_t>((size_t)nv_ * vfCap_);
+    fvKeysAlt_ = dalloc<uint64_t>((size_t)nv_ * vfCap_);
+    fvRange_ = dalloc<int2>(nf_);
+    xAtCD_ = dalloc<float4>(nv_);
+    flag_ = dalloc<int>(1);
+    hip_check(rocprim::radix_sort_keys(nullptr, fvSortTempBytes_, fvKeysAlt_, fvKeys_, (size_t)nv_ * vfCap_, 0, 64,
+                                       stream_),
+              "sort size");
+    fvSortTemp_ = dalloc<char>(fvSortTempBytes_);
+    hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
     vMesh_ = dalloc<int>(nv_);
     up(vMesh_, hVMesh_.data(), sizeof(int) * nv_);
     faces_ = dalloc<int4>(nf_);
@@ -996,6 +1068,7 @@ void ContactDetector::getPositions(int mesh, float* pos) const {
 
 void ContactDetector::perturb(uint64_t seed, int step, float sigma, const float* drift) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    if (!rest_) fail(GVBD_ERR_STATE, "perturb needs a detector that owns its positions");
     const int M = numMeshes();
     std::vector<float> d(3 * (size_t)M, 0.f);
     if (drift) std::memcpy(d.data(), drift, d.size() * sizeof(float));

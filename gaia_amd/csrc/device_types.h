@@ -59,7 +59,8 @@ struct alignas(16) BendInc {
 };
 static_assert(sizeof(BendInc) == 32, "BendInc must be 32 bytes");
 
-// Cloth work slot: (vertex, face begin, face end, flags), (bending begin, bending end, -, -).
+// Cloth work slot: (vertex, face begin, face end, flags), (bending begin, bending end, edge begin,
+// edge end); the edge range indexes ContactView::edgeRef (incident edges, contact scenes only).
 struct alignas(16) ClothSlot {
     int4 a;
     int4 b;
@@ -92,6 +93,32 @@ struct SolveParams {
     float clothFrictionEpsU;       // boundaryFrictionEpsValue * dt
     float degenerateThresSq;       // degenerateTriangleThres^2
     float clothStepSize;
+    // cloth contact (VBDClothPhysicsParameters.h:60-71)
+    float contactStiffness;
+    float contactRadius;
+    float thickness;
+    int32_t contactEnergyType;
+};
+
+// Contact data of a cloth scene with collision handling: the detector's per-primitive lists
+// (scene vertex / face / edge ids), face back-links and conservative bounds.
+struct ContactView {
+    const int4* faces;        // v0, v1, v2, mesh
+    const int4* edges;        // eV1, eV2, eV12Next, eV21Next
+    const float2* friction;   // per vertex: (frictionDynamic, frictionEpsV) of its mesh
+    const int* vfCount;       // per vertex: stored VF contacts
+    const int4* vfRec;        // per vertex vfCap records (face, type, prim, d bits)
+    int vfCap;
+    int eeCap;
+    const int* eeCount;       // per edge
+    const float4* eeRec;      // per edge eeCap records (edge bits, mu1, mu2, d)
+    const int2* fvRange;      // per face: [begin, end) into fvKeys (at most 32)
+    const uint64_t* fvKeys;   // sorted (face << 32 | vertex)
+    const int* faceRef;       // per FaceInc record: face id | order << 30
+    const int* edgeRef;       // per cloth-slot edge record: edge id | order << 30
+    const float* bound;       // per vertex conservative bound
+    const float4* xAtCD;      // positions at the last detection
+    int* flag;                // set when a vertex is truncated: detect before the next iteration
 };
 
 // Device view of one scene, referenced by work items of a (batched) launch.
@@ -110,8 +137,10 @@ struct SceneView {
     const FaceInc* cface;
     const BendInc* cbend;
     int32_t nv;
-    int32_t pad;
+    int32_t hasContact;
     SolveParams sp;
+    ContactView cv;
+    const float* initBound;  // per vertex bound of initialisation type 3 (nullptr: unbounded)
 };
 
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`

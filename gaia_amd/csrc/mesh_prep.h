@@ -79,6 +79,8 @@ struct ClothMaterial {
     float exponentialVelDamping = 1.f;  // applied per substep as given (cloth does not rescale by dt)
     float constantVelDamping = 0.f;
     int32_t initializationType = 2;     // VBDObjectParamsTriMesh default: previous position
+    float frictionDynamic = 0.1f;       // TriMesh.h:41-42 (contact friction)
+    float frictionEpsV = 0.1f;
 };
 
 // Edge of a triangle mesh (TriMesh/TriMesh.h:51-60). Edges are numbered in order of first

@@ -139,6 +139,8 @@ gvbd_physics_params default_physics_params() {
     p.thickness = 0.f;
     p.conservative_step_relaxation = 0.45f;
     p.contact_energy_type = 1;
+    p.contact_query_radius = 0.4f;  // ClothContactDetectorParameters() sets maxQueryDis = 0.4
+    p.contact_bvh_rebuild_steps = 32;
     return p;
 }
 
@@ -178,6 +180,9 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         get_num(*pp, "contactRadius", p.contact_radius);
         get_num(*pp, "thickness", p.thickness);
         get_num(*pp, "conservativeStepRelaxation", p.conservative_step_relaxation);
+        get_int(*pp, "contactBVHReconstructionSteps", p.contact_bvh_rebuild_steps);
+        // BaseClothPhsicsFramework::parseRunningParameters (Framework/BaseClothSimPhsicsFramework.cpp:17-28)
+        if (const json::Value* cdp = pp->get("ContactDetectorParams")) get_num(*cdp, "maxQueryDis", p.contact_query_radius);
         int32_t useBowl = 0;
         get_int(*pp, "useBowlGround", useBowl);
         if (useBowl) {
@@ -251,6 +256,8 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
             get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
             get_int(mj, "initializationType", m.mat.initializationType);
+            get_num(mj, "frictionDynamic", m.mat.frictionDynamic);  // TriMesh.cpp:459-460
+            get_num(mj, "frictionEpsV", m.mat.frictionEpsV);
             const int t = m.mat.initializationType;
             if (!(t == 0 || t == 2 || t == 3 || t == 4)) {
                 err = "cloth initializationType must be 0, 2, 3 or 4 in this build";

@@ -207,7 +207,246 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
 }
 
 // ---------------------------------------------------------------------------
-// Cloth vertex block (VBDClothPhysics.cpp:1026-1154, no contacts)
+// Cloth contact (VBDClothPhysics.cpp:1248-1399, 1500-1573): every evaluation recomputes the
+// closest points from the current positions (the reference's updateContactInfo = true).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 normalized(f3 v) {
+    const float n = sqrtf(dot(v, v));
+    return n > 0.f ? (1.f / n) * v : v;
+}
+
+// CollisionGeometry.h:90-157 with barycentrics
+__device__ void closest_point_bary(f3 p, f3 a, f3 b, f3 c, f3& cp, f3& bary) {
+    const f3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.f && d2 <= 0.f) {
+        cp = a;
+        bary = mk(1.f, 0.f, 0.f);
+        return;
+    }
+    const f3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.f && d4 <= d3) {
+        cp = b;
+        bary = mk(0.f, 1.f, 0.f);
+        return;
+    }
+    const f3 cpv = p - c;
+    const float d5 = dot(ab, cpv), d6 = dot(ac, cpv);
+    if (d6 >= 0.f && d5 <= d6) {
+        cp = c;
+        bary = mk(0.f, 0.f, 1.f);
+        return;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+        const float v = d1 / (d1 - d3);
+        cp = a + v * ab;
+        bary = mk(1.f - v, v, 0.f);
+        return;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+        const float v = d2 / (d2 - d6);
+        cp = a + v * ac;
+        bary = mk(1.f - v, 0.f, v);
+        return;
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+        const float v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        cp = b + v * (c - b);
+        bary = mk(0.f, 1.f - v, v);
+        return;
+    }
+    const float denom = 1.f / (va + vb + vc);
+    const float v = vb * denom, w = vc * denom;
+    cp = (a + v * ab) + w * ac;
+    bary = mk(1.f - v - w, v, w);
+}
+
+// CollisionGeometry.h:160-267 (absolute CMP_EPSILON on the determinant, as the reference)
+__device__ void segment_closest(f3 p0, f3 p1, f3 q0, f3 q1, f3& c1, f3& c2, float& s, float& t) {
+    const f3 p = p1 - p0, q = q1 - q0, r = p0 - q0;
+    const float a = dot(p, p), b = dot(p, q), c = dot(q, q), d = dot(p, r), e = dot(q, r);
+    s = 0.f;
+    t = 0.f;
+    const float det = a * c - b * b;
+    if (det > 0.00001f) {
+        const float bte = b * e, ctd = c * d;
+        if (bte <= ctd) {
+            if (e <= 0.f) {
+                s = (-d >= a ? 1.f : (-d > 0.f ? -d / a : 0.f));
+            } else if (e < c) {
+                t = e / c;
+            } else {
+                s = (b - d >= a ? 1.f : (b - d > 0.f ? (b - d) / a : 0.f));
+                t = 1.f;
+            }
+        } else {
+            s = bte - ctd;
+            if (s >= det) {
+                if (b + e <= 0.f) {
+                    s = (-d <= 0.f ? 0.f : (-d < a ? -d / a : 1.f));
+                    t = 0.f;
+                } else if (b + e < c) {
+                    s = 1.f;
+                    t = (b + e) / c;
+                } else {
+                    s = (b - d <= 0.f ? 0.f : (b - d < a ? (b - d) / a : 1.f));
+                    t = 1.f;
+                }
+            } else {
+                const float ate = a * e, btd = b * d;
+                if (ate <= btd) {
+                    s = (-d <= 0.f ? 0.f : (-d >= a ? 1.f : -d / a));
+                    t = 0.f;
+                } else {
+                    t = ate - btd;
+                    if (t >= det) {
+                        s = (b - d <= 0.f ? 0.f : (b - d >= a ? 1.f : (b - d) / a));
+                        t = 1.f;
+                    } else {
+                        s /= det;
+                        t /= det;
+                    }
+                }
+            }
+        }
+    } else {
+        if (e <= 0.f) {
+            s = (-d <= 0.f ? 0.f : (-d >= a ? 1.f : -d / a));
+        } else if (e >= c) {
+            s = (b - d <= 0.f ? 0.f : (b - d >= a ? 1.f : (b - d) / a));
+            t = 1.f;
+        } else {
+            t = e / c;
+        }
+    }
+    c1 = (1.f - s) * p0 + s * p1;
+    c2 = (1.f - t) * q0 + t * q1;
+}
+
+// VBDClothPhysics.cpp:1533-1573 (0 quadratic, 1 quadratic + log below radius / 2, 2 IPC-like)
+__device__ void repulsive(const SolveParams& sp, float dis, float& dEdD, float& d2E) {
+    const float adj = dis - sp.thickness;
+    const float pen = sp.contactRadius - adj;
+    const float k = sp.contactStiffness;
+    if (sp.contactEnergyType == 1) {
+        const float tau = sp.contactRadius * 0.5f;
+        if (adj < tau && adj > 0.00001f) {
+            const float k2 = 0.5f * tau * tau * k;
+            dEdD = -k2 / adj;
+            d2E = k2 / (adj * adj);
+        } else {
+            dEdD = -k * pen;
+            d2E = k;
+        }
+    } else if (sp.contactEnergyType == 2) {
+        const float dr = adj - sp.contactRadius;
+        const float lg = logf(adj / sp.contactRadius);
+        dEdD = -k * dr * dr / adj - 2.f * k * lg * dr;
+        d2E = k * (dr * dr / (adj * adj) - 4.f * dr / adj - 2.f * lg);
+    } else {
+        dEdD = -k * pen;
+        d2E = k;
+    }
+}
+
+// repulsion b lambda n, Hessian d2E b^2 n n^T, and IPC-mollified friction in the tangent frame
+// (t0 normalised, (t0 x n) normalised), VBDClothPhysics.cpp:1500-1531
+__device__ void contact_apply(float dt, float b, float lambda, float d2E, f3 n, f3 dx, f3 t0, float mu, float epsV,
+                              f3& f, Sym3& h) {
+    f = f + (b * lambda) * n;
+    const float cn = d2E * b * b;
+    h.xx += cn * n.x * n.x; h.xy += cn * n.x * n.y; h.xz += cn * n.x * n.z;
+    h.yy += cn * n.y * n.y; h.yz += cn * n.y * n.z; h.zz += cn * n.z * n.z;
+    const f3 e0 = normalized(t0);
+    const f3 e1 = normalized(cross(e0, n));
+    const float u0 = dot(e0, dx), u1 = dot(e1, dx);
+    const float uNorm = sqrtf(u0 * u0 + u1 * u1);
+    if (!(uNorm > 0.f)) return;
+    const float epsU = epsV * dt;
+    const float f1 = uNorm > epsU ? 1.f / uNorm : (-uNorm / epsU + 2.f) / epsU;
+    const float c = mu * lambda * f1;
+    f = f - (b * c) * (u0 * e0 + u1 * e1);
+    const float ch = b * b * c;
+    h.xx += ch * (e0.x * e0.x + e1.x * e1.x); h.xy += ch * (e0.x * e0.y + e1.x * e1.y);
+    h.xz += ch * (e0.x * e0.z + e1.x * e1.z); h.yy += ch * (e0.y * e0.y + e1.y * e1.y);
+    h.yz += ch * (e0.y * e0.z + e1.y * e1.z); h.zz += ch * (e0.z * e0.z + e1.z * e1.z);
+}
+
+// VF contact of vertex u with face `face`, on the vertex of order `order` (3: u itself,
+// 0..2: face vertex; weights 1 and -bary_order), VBDClothPhysics.cpp:1248-1321
+__device__ void vf_contact(const SceneView& S, int u, int face, int order, f3& f, Sym3& h) {
+    const ContactView& C = S.cv;
+    const int4 fv = C.faces[face];
+    const f3 x = xyz(S.pos[u]);
+    const f3 a = xyz(S.pos[fv.x]), b = xyz(S.pos[fv.y]), c = xyz(S.pos[fv.z]);
+    f3 cp, bary;
+    closest_point_bary(x, a, b, c, cp, bary);
+    const f3 diff = x - cp;
+    const f3 n = normalized(diff);
+    const float dis = dot(diff, n);
+    if (!(dis < S.sp.contactRadius + S.sp.thickness)) return;
+    const float w = order == 3 ? 1.f : -(order == 0 ? bary.x : (order == 1 ? bary.y : bary.z));
+    float dEdD, d2E;
+    repulsive(S.sp, dis, dEdD, d2E);
+    const f3 cpPrev = bary.x * xyz(S.prev[fv.x]) + bary.y * xyz(S.prev[fv.y]) + bary.z * xyz(S.prev[fv.z]);
+    const f3 dx = (x - xyz(S.prev[u])) - (cp - cpPrev);
+    const float2 fa = C.friction[fv.x], fb = C.friction[u];
+    contact_apply(S.sp.dt, w, -dEdD, d2E, n, dx, b - a, (fa.x + fb.x) * 0.5f, (fa.y + fb.y) * 0.5f, f, h);
+}
+
+// EE contact of edge e1 with e2, on the vertex of order `order` of e1 (weights 1 - mu1, mu1),
+// VBDClothPhysics.cpp:1323-1399
+__device__ void ee_contact(const SceneView& S, int e1, int e2, int order, f3& f, Sym3& h) {
+    const ContactView& C = S.cv;
+    const int4 A = C.edges[e1], B = C.edges[e2];
+    const f3 p1 = xyz(S.pos[A.x]), p2 = xyz(S.pos[A.y]), q1 = xyz(S.pos[B.x]), q2 = xyz(S.pos[B.y]);
+    f3 c1, c2;
+    float mu1, mu2;
+    segment_closest(p1, p2, q1, q2, c1, c2, mu1, mu2);
+    const f3 diff = c1 - c2;
+    const float dis = sqrtf(dot(diff, diff));
+    const f3 v1 = p2 - p1, v2 = q2 - q1;
+    const f3 cr = cross(v1, v2);
+    const float parallelEps = dot(cr, cr) / (sqrtf(dot(v1, v1)) * sqrtf(dot(v2, v2)));
+    if (!(dis < S.sp.contactRadius + S.sp.thickness && parallelEps > 0.00001f)) return;
+    const f3 n = (1.f / dis) * diff;
+    const float w = order == 0 ? 1.f - mu1 : mu1;
+    float dEdD, d2E;
+    repulsive(S.sp, dis, dEdD, d2E);
+    const f3 c1Prev = (1.f - mu1) * xyz(S.prev[A.x]) + mu1 * xyz(S.prev[A.y]);
+    const f3 c2Prev = (1.f - mu2) * xyz(S.prev[B.x]) + mu2 * xyz(S.prev[B.y]);
+    const f3 dx = (c1 - c1Prev) - (c2 - c2Prev);
+    const float2 fa = C.friction[A.x], fb = C.friction[B.x];
+    contact_apply(S.sp.dt, w, -dEdD, d2E, n, dx, v1, (fa.x + fb.x) * 0.5f, (fa.y + fb.y) * 0.5f, f, h);
+}
+
+// VBDClothPhysics.cpp:1033-1094: the vertex's own VF contacts, the FV contacts of its incident
+// faces and the EE contacts of its incident edges
+__device__ void cloth_contacts(const SceneView& S, int v, int4 sa, int4 sb, f3& f, Sym3& h) {
+    const ContactView& C = S.cv;
+    const int nvf = C.vfCount[v];
+    for (int k = 0; k < nvf; ++k) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, f, h);
+    for (int r = sa.y; r < sa.z; ++r) {
+        const int ref = C.faceRef[r];
+        const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
+        const int2 range = C.fvRange[face];
+        for (int i = range.x; i < range.y; ++i) vf_contact(S, (int)(C.fvKeys[i] & 0xFFFFFFFFu), face, order, f, h);
+    }
+    for (int r = sb.z; r < sb.w; ++r) {
+        const int ref = C.edgeRef[r];
+        const int e = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
+        const int n = C.eeCount[e];
+        for (int k = 0; k < min(n, C.eeCap); ++k)
+            ee_contact(S, e, __float_as_int(C.eeRec[(int64_t)e * C.eeCap + k].x), order, f, h);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cloth vertex block (VBDClothPhysics.cpp:1026-1154)
 // ---------------------------------------------------------------------------
 template <int LPV>
 __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& it) {
@@ -329,9 +568,24 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     h.yy += hi;
     h.zz += hi;
     if (sp.usePlaneGround) boundary_force(sp, sp.clothBoundaryStiffness, sp.clothFrictionEpsU, x, xp, true, f, h);
+    if (S.hasContact) cloth_contacts(S, v, sa, sb, f, h);
     f3 dx;
     f3 xn = x;
-    if (solve_sym3(h, f, dx)) xn = x + sp.clothStepSize * dx;
+    if (solve_sym3(h, f, dx)) {
+        xn = x + sp.clothStepSize * dx;
+        if (S.hasContact) {
+            // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246); any truncation
+            // requests detection before the next iteration
+            const f3 x0 = xyz(S.cv.xAtCD[v]);
+            const f3 acc = xn - x0;
+            const float size = sqrtf(dot(acc, acc));
+            const float b = S.cv.bound[v];
+            if (size > b) {
+                xn = x0 + (b / size) * acc;
+                atomicOr(S.cv.flag, 1);
+            }
+        }
+    }
     float4* dst = (sa.w & SLOT_INPLACE) ? S.pos : S.pnext;
     dst[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
 }
@@ -458,7 +712,8 @@ __global__ __launch_bounds__(256) void counter_reset_kernel(int* __restrict__ co
 
 // VBD_BaseMaterial.h:244-259 applyInitialStep (computeInertia + forwardStep) and
 // VBDBaseTriMesh.cpp:163-178; the per-vertex initialisation code is
-// 0: x = y, 1: keep x, 2: x + dt v_prev, 3: (x + y)/2, 4: x + dt v_prev + dt^2 r g.
+// 0: x = y, 1: keep x, 2: x + dt v_prev, 3: (x + y)/2, 4: x + dt v_prev + dt^2 r g,
+// 5: x + (y - x) shortened to the vertex's conservative bound (cloth type 3).
 __global__ __launch_bounds__(256) void initial_step_kernel(const SceneView* __restrict__ scenes,
                                                            const WorkItem* __restrict__ items) {
     const WorkItem it = items[blockIdx.x];
@@ -490,6 +745,17 @@ __global__ __launch_bounds__(256) void initial_step_kernel(const SceneView* __re
     case 4: {
         const float c = dt * dt * sp.initRatioG;
         xn = (x + dt * xyz(vel)) + mk(c * sp.gravity[0], c * sp.gravity[1], c * sp.gravity[2]);
+        break;
+    }
+    case 5: {
+        // cloth inertia within the conservative bounds (VBDBaseTriMesh.cpp:137-152)
+        f3 step = y - x;
+        const float size = sqrtf(dot(step, step));
+        if (S.initBound) {
+            const float b = S.initBound[v];
+            if (size > b && size != 0.f) step = (b / size) * step;
+        }
+        xn = x + step;
         break;
     }
     default: xn = x; break;

@@ -37,6 +37,9 @@ struct DeviceScene {
     ClothSlot* cslots = nullptr;
     FaceInc* cface = nullptr;
     BendInc* cbend = nullptr;
+    int* faceRef = nullptr;     // contact scenes: per face record, face id | order << 30
+    int* edgeRef = nullptr;     // contact scenes: per slot incident edge, edge id | order << 30
+    float2* friction = nullptr; // contact scenes: per vertex (frictionDynamic, frictionEpsV)
 };
 
 enum MeshKind { MESH_TET = 0, MESH_CLOTH = 1 };
@@ -58,6 +61,9 @@ struct SceneObj {
     DeviceScene dev;
     SceneView view{};
     int32_t plan = -1;  // the scene's own single-scene plan
+    // cloth collision handling (handle_collision): detector over the scene's positions
+    std::unique_ptr<ContactDetector> cd;
+    int64_t frame = 0;
 
     int32_t meshVertices(int32_t m) const {
         return meshes[m].first == MESH_TET ? tets[meshes[m].second].nv : cloths[meshes[m].second].nv;
@@ -94,6 +100,7 @@ struct PlanObj {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
+    bool contact = false;  // single cloth scene with collision handling: host-driven iterations
 };
 
 }  // namespace
@@ -157,6 +164,10 @@ SolveParams make_solve_params(const gvbd_physics_params& p, const SceneObj& s) {
     sp.clothFrictionEpsU = p.cloth_boundary_friction_eps * sp.dt;
     sp.degenerateThresSq = p.degenerate_triangle_thres * p.degenerate_triangle_thres;
     sp.clothStepSize = p.cloth_step_size;
+    sp.contactStiffness = p.contact_stiffness;
+    sp.contactRadius = p.contact_radius;
+    sp.thickness = p.thickness;
+    sp.contactEnergyType = p.contact_energy_type;
     return sp;
 }
 
@@ -232,13 +243,24 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
 }
 
 // Colour-major cloth slot, face and bending records of one vertex.
+// Contact references of a cloth vertex (scene-global face / edge ids with the vertex's order
+// in the primitive); only built for scenes with collision handling.
+struct ClothContactRefs {
+    int32_t faceOff = 0, edgeOff = 0;
+    const std::vector<std::vector<int32_t>>* vertexEdges = nullptr;  // per local vertex, ascending
+    std::vector<int32_t>* faceRef = nullptr;
+    std::vector<int32_t>* edgeRef = nullptr;
+};
+
 void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, bool inplace, std::vector<ClothSlot>& slots,
-                     std::vector<FaceInc>& faces, std::vector<BendInc>& bends, std::vector<int64_t>& work) {
+                     std::vector<FaceInc>& faces, std::vector<BendInc>& bends, std::vector<int64_t>& work,
+                     const ClothContactRefs* refs) {
     ClothSlot cs{};
     cs.a.x = off + lv;
     cs.a.y = (int32_t)faces.size();
     for (int32_t k = m.vfBegin[lv]; k < m.vfBegin[lv + 1]; ++k) {
         const int32_t f = m.vfFace[k], o = m.vfOrder[k];
+        if (refs) refs->faceRef->push_back((refs->faceOff + f) | (o << 30));
         const float* D = &m.dminv[4 * (size_t)f];  // (0,0) (1,0) (0,1) (1,1)
         // shape row of face vertex j: j = 0 -> minus the sum of DmInv rows, j >= 1 -> row j-1
         float B[3][2];
@@ -282,6 +304,12 @@ void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, bool inpla
         bends.push_back(rec);
     }
     cs.b.y = (int32_t)bends.size();
+    if (refs) {
+        cs.b.z = (int32_t)refs->edgeRef->size();
+        for (int32_t e : (*refs->vertexEdges)[lv])
+            refs->edgeRef->push_back((refs->edgeOff + e) | ((m.edges[e].v1 == lv ? 0 : 1) << 30));
+        cs.b.w = (int32_t)refs->edgeRef->size();
+    }
     slots.push_back(cs);
     // gather model: 60 B per vertex + 56 B per face + 116 B per bending stencil
     work.push_back(60 + 56 * (int64_t)(cs.a.z - cs.a.y) + 116 * (int64_t)(cs.b.y - cs.b.x));
@@ -306,6 +334,26 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
                                                         : s.cloths[s.meshes[im].second].fixed;
         for (int32_t f : fx) fixedMask[im][f] = 1;
     }
+    // contact references (collision handling): scene-global face and edge ids follow the mesh
+    // order, as the detector numbers them
+    const bool contact = s.params.handle_collision != 0 && s.tets.empty();
+    std::vector<int32_t> faceRef, edgeRef;
+    std::vector<ClothContactRefs> refs(s.meshes.size());
+    std::vector<std::vector<std::vector<int32_t>>> vertexEdges(s.meshes.size());
+    if (contact) {
+        int32_t fo = 0, eo = 0;
+        for (size_t im = 0; im < s.meshes.size(); ++im) {
+            const ClothMeshHost& m = s.cloths[s.meshes[im].second];
+            vertexEdges[im].assign(m.nv, {});
+            for (int32_t e = 0; e < (int32_t)m.edges.size(); ++e) {
+                vertexEdges[im][m.edges[e].v1].push_back(e);
+                vertexEdges[im][m.edges[e].v2].push_back(e);
+            }
+            refs[im] = ClothContactRefs{fo, eo, &vertexEdges[im], &faceRef, &edgeRef};
+            fo += m.nf;
+            eo += (int32_t)m.edges.size();
+        }
+    }
     s.clothSlotInplace.clear();
     for (int32_t g = 0; g < G; ++g) {
         const auto& grp = s.groups[g];
@@ -329,7 +377,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
                 emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork);
             } else {
                 const bool inplace = !multiColor[im] && !s.params.handle_collision;
-                emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, inplace, cslots, cface, cbend, clothWork);
+                emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, inplace, cslots, cface, cbend, clothWork,
+                                contact ? &refs[im] : nullptr);
                 s.clothSlotInplace.push_back(inplace ? 1 : 0);
             }
         }
@@ -342,6 +391,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     const int32_t nv = s.nv;
     std::vector<float4> pos(nv), inertia(nv), vel(nv), vparam(nv);
     std::vector<uint8_t> vflag(nv, 0);
+    std::vector<float2> friction(contact ? nv : 0);
     for (size_t im = 0; im < s.meshes.size(); ++im) {
         const int32_t off = s.vertexOffset[im];
         const std::vector<float>* mp;
@@ -367,9 +417,12 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
                              m.mat.dampingBending);
             // cloth initialisation types (VBDBaseTriMesh.cpp:72-153) as per-vertex codes:
             // 0 inertia -> x = y; 2 previous position, 4 none -> keep x;
-            // 3 inertia within conservative bounds -> x = y (bounds are infinite without contact)
+            // 3 inertia within the conservative bounds -> x + (y - x) truncated (code 5)
             const int t = m.mat.initializationType;
-            code = (uint8_t)((t == 0 || t == 3) ? 0 : 1);
+            code = (uint8_t)(t == 0 ? 0 : (t == 3 ? 5 : 1));
+            if (contact)
+                for (int32_t i = 0; i < m.nv; ++i)
+                    friction[s.vertexOffset[im] + i] = make_float2(m.mat.frictionDynamic, m.mat.frictionEpsV);
         }
         const int32_t n = s.meshVertices((int32_t)im);
         for (int32_t i = 0; i < n; ++i) {
@@ -392,7 +445,10 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     const size_t bCS = align(sizeof(ClothSlot) * std::max<size_t>(cslots.size(), 1));
     const size_t bCF = align(sizeof(FaceInc) * std::max<size_t>(cface.size(), 1));
     const size_t bCB = align(sizeof(BendInc) * std::max<size_t>(cbend.size(), 1));
-    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB;
+    const size_t bFR = contact ? align(sizeof(int32_t) * std::max<size_t>(faceRef.size(), 1)) : 0;
+    const size_t bER = contact ? align(sizeof(int32_t) * std::max<size_t>(edgeRef.size(), 1)) : 0;
+    const size_t bFr = contact ? align(sizeof(float2) * (size_t)std::max(nv, 1)) : 0;
+    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr;
     DeviceScene& d = s.dev;
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (hipMalloc(&d.arena, total) != hipSuccess) {
@@ -420,6 +476,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     d.cslots = (ClothSlot*)take(bCS);
     d.cface = (FaceInc*)take(bCF);
     d.cbend = (BendInc*)take(bCB);
+    if (contact) {
+        d.faceRef = (int*)take(bFR);
+        d.edgeRef = (int*)take(bER);
+        d.friction = (float2*)take(bFr);
+    }
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "upload");
     };
@@ -435,6 +496,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     up(d.cslots, cslots.data(), sizeof(ClothSlot) * cslots.size());
     up(d.cface, cface.data(), sizeof(FaceInc) * cface.size());
     up(d.cbend, cbend.data(), sizeof(BendInc) * cbend.size());
+    if (contact) {
+        up(d.faceRef, faceRef.data(), sizeof(int32_t) * faceRef.size());
+        up(d.edgeRef, edgeRef.data(), sizeof(int32_t) * edgeRef.size());
+        up(d.friction, friction.data(), sizeof(float2) * friction.size());
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     SceneView& v = s.view;
     v.pos = d.pos;
@@ -605,9 +671,39 @@ void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
     }
 }
 
+void scene_collision_detection(SceneObj& s, bool firstIteration);
+
+// One frame of a cloth scene with collision handling (VBDClothPhysics.cpp:389-518): detection
+// before iteration 0 (ahead of the initial guess) and before every iteration after one in which
+// a vertex was truncated to its conservative bound; the host reads that flag per iteration.
+void plan_contact_frame(gvbd_context* ctx, PlanObj& p) {
+    SceneObj& s = scene_of(ctx, p.scenes[0]);
+    const DetectorDeviceView dv = s.cd->deviceView();
+    for (int sub = 0; sub < p.substeps; ++sub) {
+        bool required = true;
+        for (int it = 0; it < p.iterations; ++it) {
+            if (it == 0 || required) scene_collision_detection(s, it == 0);
+            if (it == 0) plan_initial_step(ctx, p);
+            plan_iteration(ctx, p, it, it);
+            int flag = 0;
+            hip_check(hipMemcpyAsync(&flag, dv.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "flag read");
+            hip_check(hipStreamSynchronize(ctx->stream), "flag sync");
+            required = flag != 0;
+        }
+        if (p.iterations == 0) plan_initial_step(ctx, p);
+        plan_velocity(ctx, p);
+    }
+    ++s.frame;
+}
+
 // One frame: substeps x (initial step, iterations x groups, velocity update)
 // (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
+    if (p.contact) {
+        plan_contact_frame(ctx, p);
+        p.epoch = p.iterations;
+        return;
+    }
     for (int sub = 0; sub < p.substeps; ++sub) {
         plan_initial_step(ctx, p);
         for (int it = 0; it < p.iterations; ++it) plan_iteration(ctx, p, it, it);
@@ -618,6 +714,9 @@ void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
 
 void plan_capture(gvbd_context* ctx, PlanObj& p) {
     free_plan_graph(p);
+    if (p.contact)
+        fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling: whether "
+                               "an iteration runs collision detection is decided on the host");
     hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     try {
         plan_frame_launches(ctx, p);
@@ -654,6 +753,11 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
         if (!s.initialized) fail(GVBD_ERR_STATE, "scene " + std::to_string(sid) + " is not initialised");
         if (s.params.iterations != p->iterations || s.params.num_substeps != p->substeps)
             fail(GVBD_ERR_INVALID, "scenes of one batch must share iterations and substeps");
+        if (s.cd) {
+            if (scenes.size() > 1)
+                fail(GVBD_ERR_INVALID, "scenes with collision handling run alone (gvbd_scene_run_step), not in a batch");
+            p->contact = true;
+        }
     }
     const int32_t id = ctx->nextPlan++;
     ctx->plans[id] = std::move(p);
@@ -663,8 +767,11 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
 void initialize_scene(gvbd_context* ctx, SceneObj& s) {
     if (s.initialized) fail(GVBD_ERR_STATE, "scene already initialised");
     if (s.meshes.empty()) fail(GVBD_ERR_STATE, "scene has no meshes");
-    if (!s.cloths.empty() && s.params.handle_collision)
-        fail(GVBD_ERR_INVALID, "handle_collision (cloth self-contact) is not supported by this build; set it to 0");
+    const bool contact = !s.cloths.empty() && s.params.handle_collision;
+    if (contact && !s.tets.empty())
+        fail(GVBD_ERR_INVALID, "handle_collision (cloth contact) needs a scene of cloth meshes only");
+    if (contact && !(s.params.contact_query_radius > 0.f))
+        fail(GVBD_ERR_INVALID, "contact_query_radius must be positive");
     s.sp = make_solve_params(s.params, s);
     std::vector<const std::vector<std::vector<int32_t>>*> cats;
     std::vector<bool> removeUsed;
@@ -679,7 +786,51 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s) {
     }
     s.groups = merge_parallel_groups(cats, removeUsed);
     upload_scene(ctx, s);
+    if (contact) {
+        // ClothContactDetector over the scene's own positions (VBDClothPhysics initialize ->
+        // ClothContactDetector::initialize, ClothContactDetector.cpp:944-1050)
+        s.cd = std::make_unique<ContactDetector>(ctx->device, ctx->stream, s.params.contact_query_radius, 8, 8);
+        s.cd->useExternalPositions(s.dev.pos);
+        for (auto& mk : s.meshes) {
+            const ClothMeshHost& m = s.cloths[mk.second];
+            s.cd->addMesh(m.nv, m.pos.data(), m.nf, m.faces.data());
+        }
+        s.cd->finalize();
+        const DetectorDeviceView dv = s.cd->deviceView();
+        ContactView& cv = s.view.cv;
+        cv.faces = dv.faces;
+        cv.edges = dv.edges;
+        cv.friction = s.dev.friction;
+        cv.vfCount = dv.vfStored;
+        cv.vfRec = (const int4*)dv.vfRec;
+        cv.vfCap = dv.vfCap;
+        cv.eeCap = dv.eeCap;
+        cv.eeCount = dv.eeStored;
+        cv.eeRec = (const float4*)dv.eeRec;
+        cv.fvRange = dv.fvRange;
+        cv.fvKeys = dv.fvKeys;
+        cv.faceRef = s.dev.faceRef;
+        cv.edgeRef = s.dev.edgeRef;
+        cv.bound = dv.bounds;
+        cv.xAtCD = dv.xAtDetection;
+        cv.flag = dv.flag;
+        s.view.hasContact = 1;
+        s.view.initBound = dv.bounds;
+    }
     s.initialized = true;
+}
+
+// applyCollisionDetection (VBDClothPhysics.cpp:719-819): BVH refit (full rebuild every
+// contact_bvh_rebuild_steps frames at the first iteration), VF/EE queries, conservative bounds,
+// face back-links (FV_CONTACT_PREALLOCATE = 32 per face), positions at detection.
+void scene_collision_detection(SceneObj& s, bool firstIteration) {
+    const int steps = s.params.contact_bvh_rebuild_steps;
+    const bool rebuild = firstIteration && steps > 0 && s.frame > 0 && s.frame % steps == 0;
+    s.cd->updateBvh(rebuild);
+    s.cd->query(false);
+    s.cd->bounds(s.params.thickness, s.params.conservative_step_relaxation);
+    s.cd->buildFaceLinks(32);
+    s.cd->snapshotPositions();
 }
 
 template <class F> int guard(gvbd_context* ctx, F&& f) {
@@ -799,7 +950,10 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes) {
     return guard(ctx, [&] {
         require(ctx && bytes, "NULL argument");
         int64_t b = 0;
-        for (auto& kv : ctx->scenes) b += (int64_t)kv.second->dev.bytes;
+        for (auto& kv : ctx->scenes) {
+            b += (int64_t)kv.second->dev.bytes;
+            if (kv.second->cd) b += (int64_t)kv.second->cd->deviceBytes();
+        }
         for (auto& kv : ctx->plans) b += (int64_t)kv.second->bytes;
         for (auto& kv : ctx->detectors) b += (int64_t)kv.second->deviceBytes();
         *bytes = b;
@@ -832,6 +986,7 @@ int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
             free_plan(plan_of(ctx, s.plan));
             ctx->plans.erase(s.plan);
         }
+        s.cd.reset();  // reads the arena's positions: released first
         if (s.dev.arena) hip_check(hipFree(s.dev.arena), "hipFree");
         ctx->scenes.erase(scene);
     });
@@ -910,6 +1065,8 @@ int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth
         m.mat.exponentialVelDamping = desc->exponential_vel_damping;
         m.mat.constantVelDamping = desc->constant_vel_damping;
         m.mat.initializationType = t;
+        m.mat.frictionDynamic = desc->friction_dynamic;
+        m.mat.frictionEpsV = desc->friction_eps_v;
         m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
         for (int32_t v : m.faces) require(v >= 0 && v < m.nv, "face vertex index out of range");
         if (desc->vertex_colors) {
@@ -1142,12 +1299,24 @@ int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene) {
     });
 }
 
+int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.cd) fail(GVBD_ERR_STATE, "scene has no collision handling");
+        scene_collision_detection(s, true);
+    });
+}
+
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
         SceneObj& s = scene_of(ctx, scene);
         if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
         PlanObj& p = plan_of(ctx, s.plan);
+        if (enable && p.contact)
+            fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling: "
+                                   "whether an iteration runs collision detection is decided on the host");
         p.wantGraph = enable != 0;
         free_plan_graph(p);
         if (p.wantGraph) {

@@ -31,8 +31,8 @@ DEFAULT_PHYSICS = dict(
     step_size_gd=1e-3,
     friction_start_iter=0,
     associate_gravity_with_inertia=1,
-    # cloth (VBDCloth/VBDClothPhysicsParameters.h:15-81); self-contact is not supported yet, so
-    # handle_collision defaults to 0 here (the reference's JSON default is 1, see gvbd_scene_load_json)
+    # cloth (VBDCloth/VBDClothPhysicsParameters.h:15-81); handle_collision defaults to 0 here (the
+    # reference's JSON default is 1, which gvbd_scene_load_json keeps)
     cloth_boundary_contact_stiffness=1e4,
     cloth_boundary_friction_eps=1e-2,
     degenerate_triangle_thres=1e-6,
@@ -43,6 +43,8 @@ DEFAULT_PHYSICS = dict(
     thickness=0.0,
     conservative_step_relaxation=0.45,
     contact_energy_type=1,
+    contact_query_radius=0.4,
+    contact_bvh_rebuild_steps=32,
 )
 
 # VBDObjectParamsTriMeshStVk defaults (VBDCloth/VBDTriMeshStVK.h:17-20, VBDCloth/VBDBaseTriMesh.h:12-16,
@@ -57,6 +59,8 @@ DEFAULT_CLOTH_MATERIAL = dict(
     exponential_vel_damping=1.0,
     constant_vel_damping=0.0,
     initialization_type=2,
+    friction_dynamic=0.1,
+    friction_eps_v=0.1,
 )
 
 # ObjectParametersVBDNeoHookean defaults (VBD/VBD_NeoHookean.h:25-29, VBD/VBD_BaseMaterial.h:29-45,
@@ -324,7 +328,7 @@ class VBDPhysics:
         d.num_faces = faces.size // 3
         d.faces = iptr(faces)
         for k in ("density", "mu", "lmbd", "bending_stiffness", "damping_stvk", "damping_bending",
-                  "exponential_vel_damping", "constant_vel_damping"):
+                  "exponential_vel_damping", "constant_vel_damping", "friction_dynamic", "friction_eps_v"):
             setattr(d, k, float(mat[k]))
         d.initialization_type = int(mat["initialization_type"])
         d.num_fixed = fixed.size
@@ -355,6 +359,10 @@ class VBDPhysics:
 
     def updateVelocities(self):
         self.ctx.check(lib().gvbd_scene_update_velocities(self.ctx.handle, self.scene))
+
+    def applyCollisionDetection(self):
+        """VBDClothSimulationFramework::applyCollisionDetection (cloth scenes with handle_collision)."""
+        self.ctx.check(lib().gvbd_scene_collision_detection(self.ctx.handle, self.scene))
 
     def recordVBDSolveGraph(self, enable=True):
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))

@@ -76,12 +76,14 @@ typedef struct gvbd_physics_params {
     float cloth_boundary_friction_eps;      /* boundaryFrictionEpsValue (1e-2) */
     float degenerate_triangle_thres;        /* degenerateTriangleThres (1e-6) */
     float cloth_step_size;                  /* stepSize (1) */
-    int32_t handle_collision;               /* VF/EE self-contact of cloth; must be 0 in this build */
+    int32_t handle_collision;               /* VF/EE contact between cloth meshes (cloth-only scenes) */
     float contact_stiffness;                /* contactStiffness (1e4) */
     float contact_radius;                   /* contactRadius (1) */
     float thickness;                        /* thickness (0) */
     float conservative_step_relaxation;     /* conservativeStepRelaxation (0.45) */
     int32_t contact_energy_type;            /* contactEnergyTypes (1) */
+    float contact_query_radius;             /* ContactDetectorParams.maxQueryDis (0.4) */
+    int32_t contact_bvh_rebuild_steps;      /* contactBVHReconstructionSteps (32): full BVH rebuild period in frames */
 } gvbd_physics_params;
 
 /* One tet mesh (ObjectParametersVBDNeoHookean, VBD/VBD_NeoHookean.cpp:1613-1625,
@@ -127,11 +129,14 @@ typedef struct gvbd_cloth_mesh_desc {
     float damping_bending;
     float exponential_vel_damping; /* applied per substep as given (cloth does not rescale by dt) */
     float constant_vel_damping;
-    int32_t initialization_type;   /* 0 inertia, 2 previous position (default), 3 (= 0 without contact), 4 none */
+    int32_t initialization_type;   /* 0 inertia, 2 previous position (default), 3 inertia within the conservative
+                                      bounds, 4 none */
     int32_t num_fixed;
     const int32_t* fixed_points;
     const int32_t* vertex_colors;  /* colour per vertex, or NULL to colour the vertex graph with bending edges */
     int32_t coloring_algorithm;
+    float friction_dynamic;        /* frictionDynamic (0.1): contact friction, averaged over the two meshes */
+    float friction_eps_v;          /* frictionEpsV (0.1) */
 } gvbd_cloth_mesh_desc;
 
 #define GVBD_MODE_GROUP_LAUNCHES 0 /* one kernel launch per parallel group (the reference's structure) */
@@ -197,6 +202,10 @@ int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t gr
  * (the iteration body of VBDPhysics::runStepGPU / runStep, VBD/VBDPhysics.cpp:879-945, 1605-1627) */
 int gvbd_scene_run_iteration(gvbd_context* ctx, int32_t scene, int32_t iteration);
 int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
+/* cloth scenes with handle_collision: VBDClothSimulationFramework::applyCollisionDetection
+ * (VBDClothPhysics.cpp:735-819) at the current positions; run_step calls it itself before
+ * iteration 0 and after any iteration in which a vertex was truncated to its bound */
+int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);

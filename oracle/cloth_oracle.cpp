@@ -22,8 +22,10 @@
 // A failed 3x3 solve leaves the vertex where it is (the reference copies a stale
 // positionsNext, VBDClothPhysics.cpp:1141-1153 and :483-493).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <utility>
@@ -31,6 +33,17 @@
 
 #include "oracle.h"
 #include "pool.h"
+
+// contact detector oracle (detector_oracle.cpp)
+extern "C" {
+void* oracle_cd_create(int nmesh, const int* mesh_nv, const int* mesh_nf, const int* faces);
+void oracle_cd_destroy(void* h);
+void oracle_cd_detect(void* h, const float* pos, float radius, int mode, int threads);
+void oracle_cd_counts(void* h, int64_t* counts);
+void oracle_cd_vf_contacts(void* h, int* rec, float* d);
+void oracle_cd_ee_contacts(void* h, int* ee, float* mu, float* d);
+void oracle_cd_bounds(void* h, float thickness, float relaxation, float* bounds);
+}
 
 namespace {
 
@@ -47,7 +60,25 @@ template <class R> struct ClothMesh {
     std::vector<int32_t> fixedList;
     std::vector<std::vector<int32_t>> categories;
     R mu, lmbd, kb, dampS, dampB, expVel, constVel, density;
+    R frictionMu = R(0.1), frictionEpsV = R(0.1);  // contact friction (TriMesh.h:41-42 defaults)
     int initType = 2;
+    std::vector<int32_t> veBegin, veEdge;             // vertex -> incident edges (ascending)
+};
+
+// Contact state of a scene with handle_collision (global ids: meshes concatenated in order).
+struct ContactState {
+    void* cd = nullptr;                      // detector oracle (detector_oracle.cpp)
+    std::vector<int> vOff, fOff, eOff;       // per mesh, + total
+    std::vector<int> vMesh;                  // global vertex -> mesh
+    std::vector<std::vector<int>> vfFaces;   // global vertex -> its contact faces (lowest 8 face ids)
+    std::vector<std::vector<int>> fvVerts;   // global face -> vertices in contact with it (lowest 32 ids)
+    std::vector<std::vector<int>> eeOther;   // global edge -> other edges in contact (lowest 8 ids)
+    std::vector<float> bound;                // conservative bound per global vertex
+    std::vector<double> xAtCD;               // positions at the last detection (3 per global vertex)
+    std::atomic<int> required{0};            // a vertex left its bound: detect before the next iteration
+    ~ContactState() {
+        if (cd) oracle_cd_destroy(cd);
+    }
 };
 
 template <class R> struct ClothScene {
@@ -55,6 +86,8 @@ template <class R> struct ClothScene {
     R dt, dtSqrRecip;
     std::vector<ClothMesh<R>> meshes;
     std::vector<std::vector<int32_t>> groups;
+    std::unique_ptr<ContactState> contact;   // set by finalize when OP_HANDLE_COLLISION != 0
+    int threads = 1;
 };
 
 template <class R> inline void cross3(const R* a, const R* b, R* c) {
@@ -143,6 +176,12 @@ template <class R> int tri_topology(ClothMesh<R>& m) {
     for (int e = 0; e < m.ne; ++e) {
         vEdges[m.edges[6 * e]].push_back(e);
         vEdges[m.edges[6 * e + 1]].push_back(e);
+    }
+    m.veBegin.assign(m.nv + 1, 0);
+    m.veEdge.clear();
+    for (int v = 0; v < m.nv; ++v) {
+        m.veEdge.insert(m.veEdge.end(), vEdges[v].begin(), vEdges[v].end());
+        m.veBegin[v + 1] = (int)m.veEdge.size();
     }
     // face -> edge opposite each local vertex
     auto edge_id = [&](int a, int b) { return edgeOf[std::make_pair(std::min(a, b), std::max(a, b))]; };
@@ -375,10 +414,327 @@ template <class R> void friction(R mu, R lambda, const R* T, const R* u, R epsU,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cloth contact (VBDClothPhysics.cpp:1248-1399, 1500-1573), evaluated from the current positions
+// as the reference does (updateContactInfo = true recomputes closest points every evaluation).
+// ---------------------------------------------------------------------------
+
+// CollisionGeometry.h:90-157 in precision R; bary receives the closest point's barycentrics
+template <class R> void closest_point_tri(const R* p, const R* a, const R* b, const R* c, R* cp, R* bary) {
+    R ab[3], ac[3], ap[3], bp[3], q[3];
+    for (int d = 0; d < 3; ++d) {
+        ab[d] = b[d] - a[d];
+        ac[d] = c[d] - a[d];
+        ap[d] = p[d] - a[d];
+    }
+    const R d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    auto set = [&](const R* x, R b0, R b1, R b2) {
+        for (int d = 0; d < 3; ++d) cp[d] = x[d];
+        bary[0] = b0;
+        bary[1] = b1;
+        bary[2] = b2;
+    };
+    if (d1 <= 0 && d2 <= 0) return set(a, 1, 0, 0);
+    for (int d = 0; d < 3; ++d) bp[d] = p[d] - b[d];
+    const R d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return set(b, 0, 1, 0);
+    R cpv[3];
+    for (int d = 0; d < 3; ++d) cpv[d] = p[d] - c[d];
+    const R d5 = dot3(ab, cpv), d6 = dot3(ac, cpv);
+    if (d6 >= 0 && d5 <= d6) return set(c, 0, 0, 1);
+    const R vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        const R v = d1 / (d1 - d3);
+        for (int d = 0; d < 3; ++d) q[d] = a[d] + v * ab[d];
+        return set(q, R(1) - v, v, 0);
+    }
+    const R vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        const R v = d2 / (d2 - d6);
+        for (int d = 0; d < 3; ++d) q[d] = a[d] + v * ac[d];
+        return set(q, R(1) - v, 0, v);
+    }
+    const R va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        const R v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int d = 0; d < 3; ++d) q[d] = b[d] + v * (c[d] - b[d]);
+        return set(q, 0, R(1) - v, v);
+    }
+    const R denom = R(1) / (va + vb + vc);
+    const R v = vb * denom, w = vc * denom;
+    for (int d = 0; d < 3; ++d) q[d] = (a[d] + v * ab[d]) + w * ac[d];
+    return set(q, R(1) - v - w, v, w);
+}
+
+// CollisionGeometry.h:160-267 in precision R (absolute CMP_EPSILON as in the reference)
+template <class R> void segment_closest(const R* p0, const R* p1, const R* q0, const R* q1, R* c1, R* c2, R& s, R& t) {
+    R p[3], q[3], r[3];
+    for (int d = 0; d < 3; ++d) {
+        p[d] = p1[d] - p0[d];
+        q[d] = q1[d] - q0[d];
+        r[d] = p0[d] - q0[d];
+    }
+    const R a = dot3(p, p), b = dot3(p, q), c = dot3(q, q), d = dot3(p, r), e = dot3(q, r);
+    s = 0;
+    t = 0;
+    const R det = a * c - b * b;
+    if (det > R(0.00001)) {
+        const R bte = b * e, ctd = c * d;
+        if (bte <= ctd) {
+            if (e <= 0) {
+                s = (-d >= a ? R(1) : (-d > 0 ? -d / a : R(0)));
+                t = 0;
+            } else if (e < c) {
+                s = 0;
+                t = e / c;
+            } else {
+                s = (b - d >= a ? R(1) : (b - d > 0 ? (b - d) / a : R(0)));
+                t = 1;
+            }
+        } else {
+            s = bte - ctd;
+            if (s >= det) {
+                if (b + e <= 0) {
+                    s = (-d <= 0 ? R(0) : (-d < a ? -d / a : R(1)));
+                    t = 0;
+                } else if (b + e < c) {
+                    s = 1;
+                    t = (b + e) / c;
+                } else {
+                    s = (b - d <= 0 ? R(0) : (b - d < a ? (b - d) / a : R(1)));
+                    t = 1;
+                }
+            } else {
+                const R ate = a * e, btd = b * d;
+                if (ate <= btd) {
+                    s = (-d <= 0 ? R(0) : (-d >= a ? R(1) : -d / a));
+                    t = 0;
+                } else {
+                    t = ate - btd;
+                    if (t >= det) {
+                        s = (b - d <= 0 ? R(0) : (b - d >= a ? R(1) : (b - d) / a));
+                        t = 1;
+                    } else {
+                        s /= det;
+                        t /= det;
+                    }
+                }
+            }
+        }
+    } else {
+        if (e <= 0) {
+            s = (-d <= 0 ? R(0) : (-d >= a ? R(1) : -d / a));
+            t = 0;
+        } else if (e >= c) {
+            s = (b - d <= 0 ? R(0) : (b - d >= a ? R(1) : (b - d) / a));
+            t = 1;
+        } else {
+            s = 0;
+            t = e / c;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        c1[k] = (R(1) - s) * p0[k] + s * p1[k];
+        c2[k] = (R(1) - t) * q0[k] + t * q1[k];
+    }
+}
+
+// VBDClothPhysics.cpp:1533-1573 (energy types 0 quadratic, 1 quadratic-log two-stage, 2 IPC-like)
+template <class R> void repulsive(const ClothScene<R>& s, R dis, R& dEdD, R& d2E) {
+    const R adj = dis - (R)s.p[OP_THICKNESS];
+    const R radius = (R)s.p[OP_CONTACT_RADIUS];
+    const R pen = radius - adj;
+    const R k = (R)s.p[OP_CONTACT_STIFFNESS];
+    switch ((int)s.p[OP_CONTACT_ENERGY_TYPE]) {
+    case 0:
+        dEdD = -k * pen;
+        d2E = k;
+        break;
+    case 1: {
+        const R tau = radius * R(0.5);
+        if (adj < tau && adj > R(0.00001)) {
+            const R k2 = R(0.5) * tau * tau * k;
+            dEdD = -k2 / adj;
+            d2E = k2 / (adj * adj);
+        } else {
+            dEdD = -k * pen;
+            d2E = k;
+        }
+        break;
+    }
+    case 2:
+        dEdD = -k * (adj - radius) * (adj - radius) / adj - R(2) * k * std::log(adj / radius) * (adj - radius);
+        d2E = k * ((adj - radius) * (adj - radius) / (adj * adj) - (R(4) * adj - R(4) * radius) / adj -
+                   R(2) * std::log(adj / radius));
+        break;
+    default:
+        dEdD = 0;
+        d2E = 0;
+    }
+}
+
+template <class R> inline const R* gx(ClothScene<R>& s, int gv) {
+    const ContactState& C = *s.contact;
+    const int m = C.vMesh[gv];
+    return &s.meshes[m].x[3 * (size_t)(gv - C.vOff[m])];
+}
+template <class R> inline const R* gxp(ClothScene<R>& s, int gv) {
+    const ContactState& C = *s.contact;
+    const int m = C.vMesh[gv];
+    return &s.meshes[m].xprev[3 * (size_t)(gv - C.vOff[m])];
+}
+inline int mesh_of(const std::vector<int>& off, int g) {
+    return (int)(std::upper_bound(off.begin(), off.end(), g) - off.begin()) - 1;
+}
+template <class R> inline void normalized(const R* v, R* out) {
+    const R n = std::sqrt(dot3(v, v));
+    for (int d = 0; d < 3; ++d) out[d] = n > 0 ? v[d] / n : v[d];
+}
+
+// contact force / Hessian on the vertex of weight b (1 for the vertex side, -bary_o for face vertex o)
+template <class R> void contact_apply(ClothScene<R>& s, R b, R lambda, R d2E, const R* n, const R* dx, const R* t0,
+                                      R mu, R epsV, R* f, R* h) {
+    for (int r = 0; r < 3; ++r) {
+        f[r] += b * lambda * n[r];
+        for (int c = 0; c < 3; ++c) h[r + 3 * c] += d2E * b * b * n[r] * n[c];
+    }
+    R e0[3], e1[3], cr[3];
+    normalized(t0, e0);
+    cross3(e0, n, cr);
+    normalized(cr, e1);
+    const R T[6] = {e0[0], e0[1], e0[2], e1[0], e1[1], e1[2]};
+    const R u[2] = {dot3(e0, dx), dot3(e1, dx)};
+    R ff[3], fh[9];
+    friction<R>(mu, lambda, T, u, epsV * s.dt, ff, fh);
+    for (int r = 0; r < 3; ++r) f[r] += b * ff[r];
+    for (int i = 0; i < 9; ++i) h[i] += b * b * fh[i];
+}
+
+// VF contact of vertex gu with face gf, seen from the vertex of order `order` (3 = the vertex side)
+template <class R> void vf_contact(ClothScene<R>& s, int gu, int gf, int order, R* f, R* h) {
+    const ContactState& C = *s.contact;
+    const int mf = mesh_of(C.fOff, gf), mu_ = C.vMesh[gu];
+    const ClothMesh<R>& F = s.meshes[mf];
+    const int32_t* fv = &F.faces[3 * (size_t)(gf - C.fOff[mf])];
+    const int g0 = C.vOff[mf] + fv[0], g1 = C.vOff[mf] + fv[1], g2 = C.vOff[mf] + fv[2];
+    const R* x = gx(s, gu);
+    R cp[3], bary[3];
+    closest_point_tri(x, gx(s, g0), gx(s, g1), gx(s, g2), cp, bary);
+    R diff[3] = {x[0] - cp[0], x[1] - cp[1], x[2] - cp[2]}, n[3];
+    normalized(diff, n);
+    const R dis = dot3(diff, n);
+    if (!(dis < (R)(s.p[OP_CONTACT_RADIUS] + s.p[OP_THICKNESS]))) return;
+    const R bs[4] = {-bary[0], -bary[1], -bary[2], R(1)};
+    R dEdD, d2E;
+    repulsive(s, dis, dEdD, d2E);
+    const R *p0 = gxp(s, g0), *p1 = gxp(s, g1), *p2 = gxp(s, g2), *xp = gxp(s, gu);
+    R dx[3], t0[3];
+    for (int d = 0; d < 3; ++d) {
+        const R cpPrev = bary[0] * p0[d] + bary[1] * p1[d] + bary[2] * p2[d];
+        dx[d] = (x[d] - xp[d]) - (cp[d] - cpPrev);
+        t0[d] = gx(s, g1)[d] - gx(s, g0)[d];
+    }
+    const R mu = (F.frictionMu + s.meshes[mu_].frictionMu) * R(0.5);
+    const R epsV = (F.frictionEpsV + s.meshes[mu_].frictionEpsV) * R(0.5);
+    contact_apply(s, bs[order], -dEdD, d2E, n, dx, t0, mu, epsV, f, h);
+}
+
+// EE contact of edge ge1 with ge2, seen from the vertex of order `order` (0 eV1, 1 eV2 of ge1)
+template <class R> void ee_contact(ClothScene<R>& s, int ge1, int ge2, int order, R* f, R* h) {
+    const ContactState& C = *s.contact;
+    const int m1 = mesh_of(C.eOff, ge1), m2 = mesh_of(C.eOff, ge2);
+    const int32_t* E1 = &s.meshes[m1].edges[6 * (size_t)(ge1 - C.eOff[m1])];
+    const int32_t* E2 = &s.meshes[m2].edges[6 * (size_t)(ge2 - C.eOff[m2])];
+    const int a0 = C.vOff[m1] + E1[0], a1 = C.vOff[m1] + E1[1], b0 = C.vOff[m2] + E2[0], b1 = C.vOff[m2] + E2[1];
+    R c1[3], c2[3], mu1, mu2;
+    segment_closest(gx(s, a0), gx(s, a1), gx(s, b0), gx(s, b1), c1, c2, mu1, mu2);
+    R diff[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
+    const R dis = std::sqrt(dot3(diff, diff));
+    R v1[3], v2[3], cr[3];
+    for (int d = 0; d < 3; ++d) {
+        v1[d] = gx(s, a1)[d] - gx(s, a0)[d];
+        v2[d] = gx(s, b1)[d] - gx(s, b0)[d];
+    }
+    cross3(v1, v2, cr);
+    const R parallelEps = dot3(cr, cr) / (std::sqrt(dot3(v1, v1)) * std::sqrt(dot3(v2, v2)));
+    if (!(dis < (R)(s.p[OP_CONTACT_RADIUS] + s.p[OP_THICKNESS]) && parallelEps > R(0.00001))) return;
+    R n[3] = {diff[0] / dis, diff[1] / dis, diff[2] / dis};
+    const R bs[4] = {R(1) - mu1, mu1, R(-1) + mu2, -mu2};
+    R dEdD, d2E;
+    repulsive(s, dis, dEdD, d2E);
+    R dx[3];
+    for (int d = 0; d < 3; ++d) {
+        const R c1Prev = (R(1) - mu1) * gxp(s, a0)[d] + mu1 * gxp(s, a1)[d];
+        const R c2Prev = (R(1) - mu2) * gxp(s, b0)[d] + mu2 * gxp(s, b1)[d];
+        dx[d] = (c1[d] - c1Prev) - (c2[d] - c2Prev);
+    }
+    const R mu = (s.meshes[m1].frictionMu + s.meshes[m2].frictionMu) * R(0.5);
+    const R epsV = (s.meshes[m1].frictionEpsV + s.meshes[m2].frictionEpsV) * R(0.5);
+    contact_apply(s, bs[order], -dEdD, d2E, n, dx, v1, mu, epsV, f, h);
+}
+
+// applyCollisionDetection (VBDClothPhysics.cpp:735-819): detection at the current positions,
+// per-primitive contact lists (lowest ids kept: 8 VF per vertex, 32 FV per face, 8 EE per edge),
+// conservative bounds and the positions they are measured from.
+template <class R> void cloth_detect(ClothScene<R>& s) {
+    ContactState& C = *s.contact;
+    const int nv = C.vOff.back(), nf = C.fOff.back(), ne = C.eOff.back();
+    std::vector<float> pos(3 * (size_t)nv);
+    for (int gv = 0; gv < nv; ++gv)
+        for (int d = 0; d < 3; ++d) pos[3 * (size_t)gv + d] = (float)gx(s, gv)[d];
+    oracle_cd_detect(C.cd, pos.data(), (float)s.p[OP_MAX_QUERY_DIS], 1, s.threads);
+    int64_t cnt[3];
+    oracle_cd_counts(C.cd, cnt);
+    std::vector<int> vf(4 * (size_t)cnt[1]), ee(2 * (size_t)cnt[2]);
+    oracle_cd_vf_contacts(C.cd, vf.data(), nullptr);
+    oracle_cd_ee_contacts(C.cd, ee.data(), nullptr, nullptr);
+    C.vfFaces.assign(nv, {});
+    C.fvVerts.assign(nf, {});
+    C.eeOther.assign(ne, {});
+    for (int64_t i = 0; i < cnt[1]; ++i) C.vfFaces[vf[4 * i]].push_back(vf[4 * i + 3]);
+    for (int gv = 0; gv < nv; ++gv) {
+        auto& L = C.vfFaces[gv];
+        std::sort(L.begin(), L.end());
+        if (L.size() > 8) L.resize(8);
+        for (int f : L) C.fvVerts[f].push_back(gv);
+    }
+    for (auto& L : C.fvVerts)
+        if (L.size() > 32) L.resize(32);  // vertices were appended in ascending order
+    for (int64_t i = 0; i < cnt[2]; ++i) C.eeOther[ee[2 * i]].push_back(ee[2 * i + 1]);
+    for (auto& L : C.eeOther) {
+        std::sort(L.begin(), L.end());
+        if (L.size() > 8) L.resize(8);
+    }
+    C.bound.assign(nv, 0.f);
+    oracle_cd_bounds(C.cd, (float)s.p[OP_THICKNESS], (float)s.p[OP_CONSERVATIVE_RELAXATION], C.bound.data());
+    C.xAtCD.resize(3 * (size_t)nv);
+    for (int gv = 0; gv < nv; ++gv)
+        for (int d = 0; d < 3; ++d) C.xAtCD[3 * (size_t)gv + d] = (double)gx(s, gv)[d];
+    C.required = 0;
+}
+
 template <class R> void cloth_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid) {
     R h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     R f[3] = {0, 0, 0};
     const R* xv = &m.x[3 * vid];
+    const int mi = (int)(&m - s.meshes.data());
+    if (s.contact) {
+        // VBDClothPhysics.cpp:1033-1094: own VF contacts, FV contacts of incident faces, EE
+        // contacts of incident edges
+        const ContactState& C = *s.contact;
+        const int gv = C.vOff[mi] + vid;
+        for (int gf : C.vfFaces[gv]) vf_contact(s, gv, gf, 3, f, h);
+        for (int k = m.vfBegin[vid]; k < m.vfBegin[vid + 1]; ++k) {
+            const int gf = C.fOff[mi] + m.vfFace[k];
+            for (int gu : C.fvVerts[gf]) vf_contact(s, gu, gf, m.vfOrder[k], f, h);
+        }
+        for (int k = m.veBegin[vid]; k < m.veBegin[vid + 1]; ++k) {
+            const int e = m.veEdge[k];
+            const int ge = C.eOff[mi] + e;
+            const int order = m.edges[6 * e] == vid ? 0 : 1;
+            for (int ge2 : C.eeOther[ge]) ee_contact(s, ge, ge2, order, f, h);
+        }
+    }
     // inertia (VBDBaseTriMesh.cpp:311-318)
     for (int d = 0; d < 3; ++d) f[d] += m.mass[vid] * (m.y[3 * vid + d] - xv[d]) * s.dtSqrRecip;
     const R hi = m.mass[vid] * s.dtSqrRecip;
@@ -418,6 +774,21 @@ template <class R> void cloth_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid)
     if (solve3<R>(h, f, dx)) {
         const R step = (R)s.p[OP_STEP_SIZE];
         for (int d = 0; d < 3; ++d) xn[d] = xv[d] + step * dx[d];
+        if (s.contact) {
+            // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246). The reference
+            // assigns the shared re-detection flag from every vertex (a race); any truncation
+            // requests detection here.
+            ContactState& C = *s.contact;
+            const int gv = C.vOff[mi] + vid;
+            R acc[3];
+            for (int d = 0; d < 3; ++d) acc[d] = xn[d] - (R)C.xAtCD[3 * (size_t)gv + d];
+            const R size = std::sqrt(dot3(acc, acc));
+            const R b = (R)C.bound[gv];
+            if (size > b) {
+                for (int d = 0; d < 3; ++d) xn[d] = (R)C.xAtCD[3 * (size_t)gv + d] + acc[d] * (b / size);
+                C.required = 1;
+            }
+        }
     } else {
         for (int d = 0; d < 3; ++d) xn[d] = xv[d];
     }
@@ -435,7 +806,23 @@ template <class R> void cloth_initial_step(ClothScene<R>& s, ClothMesh<R>& m) {
     for (int f : m.fixedList)
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
     for (int i = 0; i < 3 * m.nv; ++i) m.y[i] = m.x[i] + dt * m.v[i];
-    if (m.initType == 0 || m.initType == 3) m.x = m.y;  // 3: inertia within (infinite) conservative bounds
+    if (m.initType == 0) m.x = m.y;
+    if (m.initType == 3) {
+        // inertia within the conservative bounds (VBDBaseTriMesh.cpp:137-152; bounds infinite
+        // without collision handling)
+        const int mi = (int)(&m - s.meshes.data());
+        for (int i = 0; i < m.nv; ++i) {
+            R step[3];
+            for (int d = 0; d < 3; ++d) step[d] = m.y[3 * i + d] - m.xprev[3 * i + d];
+            const R size = std::sqrt(dot3(step, step));
+            if (s.contact) {
+                const R b = (R)s.contact->bound[s.contact->vOff[mi] + i];
+                if (size > b && size != 0)
+                    for (int d = 0; d < 3; ++d) step[d] = step[d] * (b / size);
+            }
+            for (int d = 0; d < 3; ++d) m.x[3 * i + d] = m.xprev[3 * i + d] + step[d];
+        }
+    }
     // 2: previous position (x unchanged), 4: nothing
 }
 
@@ -460,6 +847,9 @@ template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
     const int nsub = (int)s.p[OP_NUM_SUBSTEPS], iters = (int)s.p[OP_ITERATIONS];
     for (int sub = 0; sub < nsub; ++sub) {
         for (int it = 0; it < iters; ++it) {
+            // detection before the iteration's sweep: always at iteration 0 (before the initial
+            // guess), later when a vertex left its bound (VBDClothPhysics.cpp:418-434)
+            if (s.contact && (it == 0 || s.contact->required)) cloth_detect(s);
             if (it == 0)
                 for (auto& m : s.meshes) cloth_initial_step(s, m);
             for (auto& grp : s.groups) {
@@ -523,6 +913,8 @@ static int cloth_add(ClothScene<R>& s, int nv, const double* pos, const double* 
     m.expVel = (R)mat[OMC_EXP_VEL_DAMPING];
     m.constVel = (R)mat[OMC_CONST_VEL_DAMPING];
     m.initType = (int)mat[OMC_INIT_TYPE];
+    m.frictionMu = (R)mat[OMC_FRICTION_MU];
+    m.frictionEpsV = (R)mat[OMC_FRICTION_EPS_V];
     if (tri_prepare(m) || tri_topology(m)) return -1;
     bending_forms(m);
     m.fixed.assign(nv, 0);
@@ -649,9 +1041,30 @@ int oracle_cloth_scene_add_mesh(OracleClothScene* sc, int nv, const double* pos,
 }
 
 int oracle_cloth_scene_finalize(OracleClothScene* sc) {
-    if (sc->precision == 64) cloth_groups(sc->s64);
-    else cloth_groups(sc->s32);
-    return 0;
+    auto fin = [&](auto& s) -> int {
+        cloth_groups(s);
+        if (s.p[OP_HANDLE_COLLISION] == 0) return 0;
+        auto C = std::make_unique<ContactState>();
+        std::vector<int> nv, nf, faces;
+        C->vOff.push_back(0);
+        C->fOff.push_back(0);
+        C->eOff.push_back(0);
+        for (size_t im = 0; im < s.meshes.size(); ++im) {
+            const auto& m = s.meshes[im];
+            nv.push_back(m.nv);
+            nf.push_back(m.nf);
+            faces.insert(faces.end(), m.faces.begin(), m.faces.end());
+            C->vOff.push_back(C->vOff.back() + m.nv);
+            C->fOff.push_back(C->fOff.back() + m.nf);
+            C->eOff.push_back(C->eOff.back() + m.ne);
+            for (int i = 0; i < m.nv; ++i) C->vMesh.push_back((int)im);
+        }
+        C->cd = oracle_cd_create((int)s.meshes.size(), nv.data(), nf.data(), faces.data());
+        if (!C->cd) return -1;
+        s.contact = std::move(C);
+        return 0;
+    };
+    return sc->precision == 64 ? fin(sc->s64) : fin(sc->s32);
 }
 
 int oracle_cloth_scene_num_groups(OracleClothScene* sc) {
@@ -668,6 +1081,7 @@ int oracle_cloth_scene_group(OracleClothScene* sc, int g, int32_t* out) {
 int oracle_cloth_scene_step(OracleClothScene* sc, int nsteps, int nthreads) {
     if (nthreads > 1 && (!sc->pool || sc->pool->size() != nthreads)) sc->pool.reset(new oracle::ThreadPool(nthreads));
     oracle::ThreadPool* pool = nthreads > 1 ? sc->pool.get() : nullptr;
+    sc->s64.threads = sc->s32.threads = std::max(1, nthreads);
     for (int i = 0; i < nsteps; ++i) {
         if (sc->precision == 64) cloth_step(sc->s64, pool);
         else cloth_step(sc->s32, pool);

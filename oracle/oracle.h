@@ -35,6 +35,14 @@ enum OraclePhysicsParam {
     OP_CLOTH_BOUNDARY_STIFFNESS, /* VBDClothPhysicsParameters::boundaryContactStiffness */
     OP_CLOTH_BOUNDARY_FRICTION_EPS, /* boundaryFrictionEpsValue */
     OP_DEGENERATE_THRES,         /* degenerateTriangleThres */
+    /* cloth contact (VBDClothPhysicsParameters.h:60-71, ClothContactDetectorParameters::maxQueryDis) */
+    OP_HANDLE_COLLISION,
+    OP_CONTACT_STIFFNESS,
+    OP_CONTACT_RADIUS,
+    OP_THICKNESS,
+    OP_CONSERVATIVE_RELAXATION,
+    OP_CONTACT_ENERGY_TYPE,
+    OP_MAX_QUERY_DIS,
     OP_COUNT = 32
 };
 
@@ -44,7 +52,9 @@ enum OracleClothMaterialParam {
     OMC_DAMP_STVK, OMC_DAMP_BENDING,
     OMC_EXP_VEL_DAMPING,         /* applied per substep as given (no pow(., dt) for cloth) */
     OMC_CONST_VEL_DAMPING,
-    OMC_INIT_TYPE,               /* VBDObjectParamsTriMesh::initializationType: 0, 2, 3 (no contact), 4 */
+    OMC_INIT_TYPE,               /* VBDObjectParamsTriMesh::initializationType: 0, 2, 3, 4 */
+    OMC_FRICTION_MU,             /* TriMeshParams::frictionDynamic (contact friction, averaged per pair) */
+    OMC_FRICTION_EPS_V,          /* TriMeshParams::frictionEpsV */
     OMC_COUNT = 16
 };
 

@@ -228,9 +228,11 @@ class TetScene:
 
 
 # ---------------------------------------------------------------- cloth
-OP.update(CLOTH_BOUNDARY_STIFFNESS=22, CLOTH_BOUNDARY_FRICTION_EPS=23, DEGENERATE_THRES=24)
+OP.update(CLOTH_BOUNDARY_STIFFNESS=22, CLOTH_BOUNDARY_FRICTION_EPS=23, DEGENERATE_THRES=24, HANDLE_COLLISION=25,
+          CONTACT_STIFFNESS=26, CONTACT_RADIUS=27, THICKNESS=28, CONSERVATIVE_RELAXATION=29, CONTACT_ENERGY_TYPE=30,
+          MAX_QUERY_DIS=31)
 OMC = dict(DENSITY=0, MU=1, LAMBDA=2, BENDING=3, DAMP_STVK=4, DAMP_BENDING=5, EXP_VEL_DAMPING=6,
-           CONST_VEL_DAMPING=7, INIT_TYPE=8)
+           CONST_VEL_DAMPING=7, INIT_TYPE=8, FRICTION_MU=9, FRICTION_EPS_V=10)
 OMC_COUNT = 16
 
 
@@ -265,6 +267,13 @@ def cloth_physics_vector(p):
     v[OP["CLOTH_BOUNDARY_FRICTION_EPS"]] = p.get("cloth_boundary_friction_eps", 1e-2)
     v[OP["DEGENERATE_THRES"]] = p.get("degenerate_triangle_thres", 1e-6)
     v[OP["STEP_SIZE"]] = p.get("cloth_step_size", 1.0)
+    v[OP["HANDLE_COLLISION"]] = p.get("handle_collision", 0)
+    v[OP["CONTACT_STIFFNESS"]] = p.get("contact_stiffness", 1e4)
+    v[OP["CONTACT_RADIUS"]] = p.get("contact_radius", 1.0)
+    v[OP["THICKNESS"]] = p.get("thickness", 0.0)
+    v[OP["CONSERVATIVE_RELAXATION"]] = p.get("conservative_step_relaxation", 0.45)
+    v[OP["CONTACT_ENERGY_TYPE"]] = p.get("contact_energy_type", 1)
+    v[OP["MAX_QUERY_DIS"]] = p.get("contact_query_radius", 0.4)
     return v
 
 
@@ -279,6 +288,8 @@ def cloth_material_vector(m):
     v[OMC["EXP_VEL_DAMPING"]] = m.get("exponential_vel_damping", 1.0)
     v[OMC["CONST_VEL_DAMPING"]] = m.get("constant_vel_damping", 0.0)
     v[OMC["INIT_TYPE"]] = m.get("initialization_type", 2)
+    v[OMC["FRICTION_MU"]] = m.get("friction_dynamic", 0.1)
+    v[OMC["FRICTION_EPS_V"]] = m.get("friction_eps_v", 0.1)
     return v
 
 
@@ -347,7 +358,8 @@ class ClothScene:
         return rc
 
     def finalize(self):
-        _cloth_lib().oracle_cloth_scene_finalize(self._h)
+        if _cloth_lib().oracle_cloth_scene_finalize(self._h) != 0:
+            raise ValueError("oracle: contact detector setup failed")
 
     def groups(self):
         L = _cloth_lib()

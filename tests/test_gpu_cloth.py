@@ -158,9 +158,10 @@ def test_cloth_json_loader(gpu_ctx, tmp_path, oracle):
     b.destroy()
 
 
-def test_cloth_json_default_handle_collision_rejected(gpu_ctx, tmp_path):
-    """The reference's cloth default handleCollision = true needs self-contact, which this
-    build does not provide yet: loading must fail loudly, not silently drop contact."""
+def test_cloth_json_default_handle_collision(gpu_ctx, tmp_path):
+    """The reference's cloth default handleCollision = true (VBDClothPhysicsParameters.h:60-62:
+    contact stiffness 1e4, contact radius 1, query radius 0.4) turns collision handling on: the
+    loaded scene runs the contact path and stays within its conservative bounds."""
     import json
 
     pos, faces = scenes.cloth_sheet(4, 0.5, 1.0)
@@ -174,5 +175,11 @@ def test_cloth_json_default_handle_collision_rejected(gpu_ctx, tmp_path):
     json.dump({"PhysicsParams": {"iterations": 2}}, open(tmp_path / "P.json", "w"))
     a = VBDPhysics(gpu_ctx)
     a.loadRunningparameters(str(tmp_path / "M.json"), str(tmp_path / "P.json"))
-    with pytest.raises(gaia_amd.VBDError, match="handle_collision"):
-        a.initialize()
+    a.initialize()
+    with pytest.raises(gaia_amd.VBDError, match="collision handling"):
+        a.recordVBDSolveGraph(True)  # the contact path is host-driven: no graph
+    a.recordVBDSolveGraph(False)
+    a.runSteps(2)
+    x, _ = a.getState(0)
+    assert np.all(np.isfinite(x))
+    a.destroy()

@@ -125,3 +125,74 @@ def build_oracle(oracle, spec, precision=32):
         sc.add_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
     sc.finalize()
     return sc
+
+
+def contact_drape_spec(steps=20, energy=1, density=1.0, stiffness=30.0, friction=0.2, iterations=8):
+    """Cloth contact (VF/FV/EE repulsion + IPC friction + conservative bounds): a free sheet
+    falling with a sideways drift onto a fixed, slightly tilted sheet 1.2 contact radii below."""
+    lo, flo = scenes.cloth_sheet(12, 0.6, 0.0, origin=(-0.3, -0.3))
+    lo[:, 1] = 0.02 * lo[:, 0]  # tilt: the upper sheet slides under friction
+    up, fup = scenes.cloth_sheet(10, 0.4, 0.0, origin=(-0.2, -0.2))
+    up[:, 1] = 0.02 * up[:, 0] + 0.012
+    mu, lm = scenes.stvk_lame(1e4, 0.3)
+    m_lo = dict(density=density, mu=mu, lmbd=lm, bending_stiffness=1e-4, friction_dynamic=friction,
+                friction_eps_v=0.01)
+    m_up = dict(m_lo, initialization_type=0)
+    vel = np.tile([0.05, -0.3, 0.0], (len(up), 1))
+    physics = physics_dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=iterations,
+                           world_lower=(-1e7, -1e7, -1e7), handle_collision=1, contact_stiffness=stiffness,
+                           contact_radius=0.01, thickness=0.0, conservative_step_relaxation=0.45,
+                           contact_energy_type=energy, contact_query_radius=0.02, contact_bvh_rebuild_steps=4)
+    return dict(physics=physics, steps=steps,
+                meshes=[dict(pos=lo, faces=flo, material=m_lo, fixed=list(range(len(lo))), vel=None, coloring="mcs"),
+                        dict(pos=up, faces=fup, material=m_up, fixed=[], vel=vel, coloring="mcs")])
+
+
+def single_mesh(spec):
+    """The same two sheets as one mesh of two components (self-contact within a mesh)."""
+    lo, up = spec["meshes"]
+    pos = np.concatenate([lo["pos"], up["pos"]])
+    faces = np.concatenate([lo["faces"], up["faces"] + len(lo["pos"])])
+    vel = np.concatenate([np.zeros_like(lo["pos"]), up["vel"]])
+    spec["meshes"] = [dict(pos=pos, faces=faces, material=up["material"], fixed=list(lo["fixed"]), vel=vel,
+                           coloring="mcs")]
+    return spec
+
+
+def folded_sheet(rows=24, cols=8, h=0.02, gap=0.016, fold_at=0.2):
+    """Strip of rows x cols vertices (spacing h), folded over itself along its length: the
+    lower layer lies at y = 0, a half circle of diameter gap turns it, and the upper layer runs
+    back at y = gap. Row r sits at arc length r h."""
+    u = np.arange(rows) * h
+    arc = np.pi * gap / 2
+    x = np.where(u <= fold_at, u, np.where(u <= fold_at + arc,
+                                           fold_at + 0.5 * gap * np.cos((u - fold_at) / (0.5 * gap) - np.pi / 2),
+                                           fold_at - (u - fold_at - arc)))
+    y = np.where(u <= fold_at, 0.0, np.where(u <= fold_at + arc,
+                                             0.5 * gap + 0.5 * gap * np.sin((u - fold_at) / (0.5 * gap) - np.pi / 2),
+                                             gap))
+    ii, jj = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    pos = np.stack([x[ii.ravel()], y[ii.ravel()], jj.ravel() * h], axis=1)
+    ci, cj = np.meshgrid(np.arange(rows - 1), np.arange(cols - 1), indexing="ij")
+    v = (cj + ci * cols).ravel()
+    f1 = np.stack([v, v + cols + 1, v + 1], axis=1)
+    f2 = np.stack([v, v + cols, v + cols + 1], axis=1)
+    faces = np.stack([f1, f2], axis=1).reshape(-1, 3).astype(np.int32)
+    return pos, faces
+
+
+def self_contact_spec(steps=20, energy=1, iterations=8):
+    """Self-contact of one mesh: a folded strip whose upper layer falls onto its lower layer,
+    which rests on the ground (boundary penalty at y = -0.002)."""
+    pos, faces = folded_sheet()
+    mu, lm = scenes.stvk_lame(1e4, 0.3)
+    material = dict(density=0.2, mu=mu, lmbd=lm, bending_stiffness=1e-5, friction_dynamic=0.1, friction_eps_v=0.01,
+                    initialization_type=0)
+    vel = np.zeros_like(pos)
+    vel[pos[:, 1] > 0.015, 1] = -0.2
+    physics = physics_dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=iterations,
+                           world_lower=(-1e7, -0.002, -1e7), handle_collision=1, contact_stiffness=1e3,
+                           contact_radius=0.01, thickness=0.0, conservative_step_relaxation=0.45,
+                           contact_energy_type=energy, contact_query_radius=0.02, contact_bvh_rebuild_steps=8)
+    return dict(physics=physics, steps=steps,
+                meshes=[dict(pos=pos, faces=faces, material=material, fixed=[], vel=vel, coloring="mcs")])
