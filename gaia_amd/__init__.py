@@ -9,6 +9,7 @@ from ._lib import LIB_PATH, VBDError, header_functions, lib  # noqa: F401
 from .vbd import (  # noqa: F401
     Batch,
     Context,
+    Server,
     VBDPhysics,
     cloth_mesh_constants,
     color_graph,

@@ -127,6 +127,34 @@ class Schedule(ctypes.Structure):
     ]
 
 
+class ServerStats(ctypes.Structure):
+    """gvbd_server_stats."""
+    _fields_ = [
+        ("budget_bytes", ctypes.c_int64),
+        ("resident_bytes", ctypes.c_int64),
+        ("vertex_solves", ctypes.c_int64),
+        ("served", ctypes.c_int64),
+        ("admissions", ctypes.c_int64),
+        ("evictions", ctypes.c_int64),
+        ("ticks", ctypes.c_int64),
+        ("resident", ctypes.c_int32),
+        ("pending", ctypes.c_int32),
+        ("waiting", ctypes.c_int32),
+        ("last_stepped", ctypes.c_int32),
+    ]
+
+
+class ServedStep(ctypes.Structure):
+    """gvbd_served_step."""
+    _fields_ = [
+        ("tag", ctypes.c_int64),
+        ("scene", ctypes.c_int32),
+        ("frame", ctypes.c_int32),
+        ("latency_ms", ctypes.c_double),
+        ("queue_ms", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 _SCHED = ctypes.POINTER(Schedule)
@@ -176,6 +204,16 @@ _SIGNATURES = {
     "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
     "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],
+    "gvbd_server_create": [_P, ctypes.c_int64, c_int32_p],
+    "gvbd_server_destroy": [_P, _I],
+    "gvbd_server_set_schedule": [_P, _I, _SCHED],
+    "gvbd_server_add_scene": [_P, _I, _I],
+    "gvbd_server_remove_scene": [_P, _I, _I],
+    "gvbd_server_request_step": [_P, _I, _I, ctypes.c_int64],
+    "gvbd_server_tick": [_P, _I, ctypes.POINTER(ServerStats)],
+    "gvbd_server_poll": [_P, _I, ctypes.POINTER(ServedStep), _I, c_int32_p],
+    "gvbd_server_get_stats": [_P, _I, ctypes.POINTER(ServerStats)],
+    "gvbd_server_scene_resident": [_P, _I, _I, c_int32_p],
     "gvbd_cd_create": [_P, ctypes.c_float, _I, _I, c_int32_p],
     "gvbd_cd_destroy": [_P, _I],
     "gvbd_cd_add_mesh": [_P, _I, _I, c_float_p, _I, c_int32_p, c_int32_p],

@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -64,6 +67,11 @@ struct SceneObj {
     // cloth collision handling (handle_collision): detector over the scene's positions
     std::unique_ptr<ContactDetector> cd;
     int64_t frame = 0;
+    // serving loop residency: the server that owns it (-1: none) and, while copied out to host
+    // memory, the arena's bytes and the device address they were copied from
+    int32_t server = -1;
+    std::vector<char> hostCopy;
+    uintptr_t hostCopyBase = 0;
 
     int32_t meshVertices(int32_t m) const {
         return meshes[m].first == MESH_TET ? tets[meshes[m].second].nv : cloths[meshes[m].second].nv;
@@ -103,6 +111,29 @@ struct PlanObj {
     bool contact = false;  // single cloth scene with collision handling: host-driven iterations
 };
 
+// Serving loop: registered scenes, their queued step requests, and the plan of one tick.
+struct ServerObj {
+    using Clock = std::chrono::steady_clock;
+    struct Request {
+        int64_t tag;
+        Clock::time_point at;
+    };
+    struct Entry {
+        std::deque<Request> pending;
+        int64_t lastUse = -1;  // tick of the scene's last frame: idle scenes leave HBM in this order
+        int32_t frames = 0;
+        int64_t needBytes = 0;  // device bytes, once known
+    };
+    int64_t budget = 0;
+    int iterations = -1, substeps = -1;
+    PlanObj plan;  // the scenes stepped by the current tick
+    int64_t layout = 0;       // bumped whenever a scene enters or leaves HBM (device addresses move)
+    int64_t planLayout = -1;  // `layout` when the plan's device views were built
+    std::map<int32_t, Entry> scenes;
+    std::deque<gvbd_served_step> done;
+    gvbd_server_stats stats{};
+};
+
 }  // namespace
 }  // namespace gvbd
 
@@ -117,7 +148,8 @@ struct gvbd_context {
     std::map<int32_t, std::unique_ptr<gvbd::PlanObj>> plans;
     std::map<int32_t, int32_t> batches;  // batch id -> plan id
     std::map<int32_t, std::unique_ptr<gvbd::ContactDetector>> detectors;
-    int32_t nextScene = 0, nextPlan = 0, nextBatch = 0, nextDetector = 0;
+    std::map<int32_t, std::unique_ptr<gvbd::ServerObj>> servers;
+    int32_t nextScene = 0, nextPlan = 0, nextBatch = 0, nextDetector = 0, nextServer = 0;
 };
 
 namespace gvbd {
@@ -315,8 +347,9 @@ void emit_cloth_slot(const ClothMeshHost& m, int32_t off, int32_t lv, bool inpla
     work.push_back(60 + 56 * (int64_t)(cs.a.z - cs.a.y) + 116 * (int64_t)(cs.b.y - cs.b.x));
 }
 
-// Builds the device arrays of an initialised scene.
-void upload_scene(gvbd_context* ctx, SceneObj& s) {
+// Builds the device arrays of an initialised scene. `reserve` (when set) sees the arena's size
+// before it is allocated and may refuse it by throwing.
+void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_t)>& reserve) {
     const int32_t G = (int32_t)s.groups.size();
     std::vector<int4> slots;
     std::vector<TetInc> tinc;
@@ -450,6 +483,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s) {
     const size_t bFr = contact ? align(sizeof(float2) * (size_t)std::max(nv, 1)) : 0;
     const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr;
     DeviceScene& d = s.dev;
+    if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (hipMalloc(&d.arena, total) != hipSuccess) {
         (void)hipGetLastError();
@@ -751,6 +785,8 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
     for (int32_t sid : scenes) {
         SceneObj& s = scene_of(ctx, sid);
         if (!s.initialized) fail(GVBD_ERR_STATE, "scene " + std::to_string(sid) + " is not initialised");
+        if (s.server >= 0 && scenes.size() > 1)
+            fail(GVBD_ERR_STATE, "scene " + std::to_string(sid) + " is registered with a server");
         if (s.params.iterations != p->iterations || s.params.num_substeps != p->substeps)
             fail(GVBD_ERR_INVALID, "scenes of one batch must share iterations and substeps");
         if (s.cd) {
@@ -764,7 +800,7 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
     return id;
 }
 
-void initialize_scene(gvbd_context* ctx, SceneObj& s) {
+void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_t)>& reserve = {}) {
     if (s.initialized) fail(GVBD_ERR_STATE, "scene already initialised");
     if (s.meshes.empty()) fail(GVBD_ERR_STATE, "scene has no meshes");
     const bool contact = !s.cloths.empty() && s.params.handle_collision;
@@ -785,7 +821,7 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s) {
         }
     }
     s.groups = merge_parallel_groups(cats, removeUsed);
-    upload_scene(ctx, s);
+    upload_scene(ctx, s, reserve);
     if (contact) {
         // ClothContactDetector over the scene's own positions (VBDClothPhysics initialize ->
         // ClothContactDetector::initialize, ClothContactDetector.cpp:944-1050)
@@ -925,6 +961,7 @@ int gvbd_context_destroy(gvbd_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->plans) free_plan(*kv.second);
+    for (auto& kv : ctx->servers) free_plan(kv.second->plan);
     ctx->detectors.clear();
     for (auto& kv : ctx->scenes)
         if (kv.second->dev.arena) (void)hipFree(kv.second->dev.arena);
@@ -951,10 +988,11 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes) {
         require(ctx && bytes, "NULL argument");
         int64_t b = 0;
         for (auto& kv : ctx->scenes) {
-            b += (int64_t)kv.second->dev.bytes;
+            if (kv.second->dev.arena) b += (int64_t)kv.second->dev.bytes;
             if (kv.second->cd) b += (int64_t)kv.second->cd->deviceBytes();
         }
         for (auto& kv : ctx->plans) b += (int64_t)kv.second->bytes;
+        for (auto& kv : ctx->servers) b += (int64_t)kv.second->plan.bytes;
         for (auto& kv : ctx->detectors) b += (int64_t)kv.second->deviceBytes();
         *bytes = b;
     });
@@ -977,6 +1015,7 @@ int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
         SceneObj& s = scene_of(ctx, scene);
+        if (s.server >= 0) fail(GVBD_ERR_STATE, "scene is registered with a server; remove it from the server first");
         for (auto& kv : ctx->plans)
             for (int32_t sid : kv.second->scenes)
                 if (sid == scene && kv.first != s.plan)
@@ -1186,6 +1225,7 @@ static PlanObj& scene_plan(gvbd_context* ctx, int32_t scene) {
     require(ctx != nullptr, "ctx is NULL");
     SceneObj& s = scene_of(ctx, scene);
     if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
+    if (!s.dev.arena) fail(GVBD_ERR_STATE, "scene is held in host memory by its server (not resident)");
     return plan_of(ctx, s.plan);
 }
 
@@ -1261,10 +1301,8 @@ int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene) { return gvbd_scene_ru
 
 int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n) {
     return guard(ctx, [&] {
-        require(ctx != nullptr && n >= 0, "bad argument");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        plan_run(ctx, plan_of(ctx, s.plan), n);
+        require(n >= 0, "bad argument");
+        plan_run(ctx, scene_plan(ctx, scene), n);
     });
 }
 
@@ -1278,10 +1316,7 @@ int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene) {
 
 int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t iteration) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        PlanObj& p = plan_of(ctx, s.plan);
+        PlanObj& p = scene_plan(ctx, scene);
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
         plan_group(ctx, p, group, iteration);
@@ -1290,10 +1325,7 @@ int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t gr
 
 int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        PlanObj& p = plan_of(ctx, s.plan);
+        PlanObj& p = scene_plan(ctx, scene);
         build_plan(ctx, p);
         plan_velocity(ctx, p);
     });
@@ -1310,10 +1342,7 @@ int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene) {
 
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable) {
     return guard(ctx, [&] {
-        require(ctx != nullptr, "ctx is NULL");
-        SceneObj& s = scene_of(ctx, scene);
-        if (s.plan < 0) fail(GVBD_ERR_STATE, "scene not initialised");
-        PlanObj& p = plan_of(ctx, s.plan);
+        PlanObj& p = scene_plan(ctx, scene);
         if (enable && p.contact)
             fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling: "
                                    "whether an iteration runs collision detection is decided on the host");
@@ -1337,10 +1366,16 @@ static void mesh_state_io(gvbd_context* ctx, int32_t scene, int32_t mesh, float*
     check_queue_status(ctx);
     float4* dev[2] = {s.dev.pos + off, s.dev.vel + off};
     float* host[2] = {pos, vel};
+    // a scene its server holds in host memory is read and written there
+    const bool resident = s.dev.arena != nullptr;
+    auto held = [&](float4* d) { return s.hostCopy.data() + ((uintptr_t)d - s.hostCopyBase); };
     for (int a = 0; a < 2; ++a) {
         if (!host[a]) continue;
         if (get) {
-            hip_check(hipMemcpy(tmp.data(), dev[a], sizeof(float4) * n, hipMemcpyDeviceToHost), "download");
+            if (resident)
+                hip_check(hipMemcpy(tmp.data(), dev[a], sizeof(float4) * n, hipMemcpyDeviceToHost), "download");
+            else
+                std::memcpy(tmp.data(), held(dev[a]), sizeof(float4) * n);
             for (int32_t i = 0; i < n; ++i) {
                 host[a][3 * i] = tmp[i].x;
                 host[a][3 * i + 1] = tmp[i].y;
@@ -1348,7 +1383,10 @@ static void mesh_state_io(gvbd_context* ctx, int32_t scene, int32_t mesh, float*
             }
         } else {
             for (int32_t i = 0; i < n; ++i) tmp[i] = make_float4(host[a][3 * i], host[a][3 * i + 1], host[a][3 * i + 2], 0.f);
-            hip_check(hipMemcpy(dev[a], tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
+            if (resident)
+                hip_check(hipMemcpy(dev[a], tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
+            else
+                std::memcpy(held(dev[a]), tmp.data(), sizeof(float4) * n);
         }
     }
 }
@@ -1513,6 +1551,349 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
     });
 }
 
+// ---- serving loop ----
+
+static ServerObj& server_of(gvbd_context* ctx, int32_t id) {
+    require(ctx != nullptr, "ctx is NULL");
+    auto it = ctx->servers.find(id);
+    if (it == ctx->servers.end()) fail(GVBD_ERR_INVALID, "unknown server id " + std::to_string(id));
+    return *it->second;
+}
+
+// Moves a scene's device pointers from one arena base to another. The arena's arrays hold
+// indices, never addresses, so a copied arena is valid at any base.
+static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
+    auto mv = [&](auto*& p) {
+        using P = std::remove_reference_t<decltype(p)>;
+        if (p) p = reinterpret_cast<P>(to + (reinterpret_cast<uintptr_t>(p) - from));
+    };
+    DeviceScene& d = s.dev;
+    mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
+    mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
+    SceneView& v = s.view;
+    mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
+    mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend);
+}
+
+// Copies an idle scene's arena out to host memory and frees it.
+static void server_evict(gvbd_context* ctx, SceneObj& s) {
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    s.hostCopy.resize(s.dev.bytes);
+    hip_check(hipMemcpy(s.hostCopy.data(), s.dev.arena, s.dev.bytes, hipMemcpyDeviceToHost), "copy out");
+    s.hostCopyBase = reinterpret_cast<uintptr_t>(s.dev.arena);
+    hip_check(hipFree(s.dev.arena), "hipFree");
+    s.dev.arena = nullptr;
+    if (s.plan >= 0) free_plan(plan_of(ctx, s.plan));  // its device views hold the old addresses
+}
+
+// Copies a scene held in host memory back into a new arena.
+static void server_restore(gvbd_context* ctx, SceneObj& s) {
+    void* arena = nullptr;
+    if (hipMalloc(&arena, s.dev.bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(GVBD_ERR_NOMEM, "hipMalloc of " + std::to_string(s.dev.bytes) + " bytes for a served scene failed");
+    }
+    if (hipMemcpy(arena, s.hostCopy.data(), s.dev.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(arena);
+        fail(GVBD_ERR_HIP, "copy back of a served scene failed");
+    }
+    rebase_scene(s, s.hostCopyBase, reinterpret_cast<uintptr_t>(arena));
+    s.dev.arena = arena;
+    std::vector<char>().swap(s.hostCopy);
+    s.hostCopyBase = 0;
+    if (s.plan >= 0) free_plan(plan_of(ctx, s.plan));
+}
+
+static int64_t server_resident_bytes(gvbd_context* ctx, const ServerObj& S, int32_t* count) {
+    int64_t b = 0;
+    int32_t n = 0;
+    for (auto& kv : S.scenes) {
+        const SceneObj& s = scene_of(ctx, kv.first);
+        if (s.dev.arena) b += (int64_t)s.dev.bytes, ++n;
+    }
+    if (count) *count = n;
+    return b;
+}
+
+// Makes room for `need` more device bytes by copying idle resident scenes (no pending request)
+// out, least recently stepped first. False, with nothing copied out, when even every idle
+// scene would not make room.
+static bool server_make_room(gvbd_context* ctx, ServerObj& S, int64_t need, int32_t self) {
+    int64_t resident = server_resident_bytes(ctx, S, nullptr);
+    if (resident + need <= S.budget) return true;
+    std::vector<std::pair<int64_t, int32_t>> idle;
+    int64_t freeable = 0;
+    for (auto& kv : S.scenes) {
+        const SceneObj& s = scene_of(ctx, kv.first);
+        if (kv.first == self || !s.dev.arena || !kv.second.pending.empty()) continue;
+        idle.push_back({kv.second.lastUse, kv.first});
+        freeable += (int64_t)s.dev.bytes;
+    }
+    if (resident - freeable + need > S.budget) return false;
+    std::sort(idle.begin(), idle.end());
+    for (auto& c : idle) {
+        if (resident + need <= S.budget) break;
+        SceneObj& s = scene_of(ctx, c.second);
+        resident -= (int64_t)s.dev.bytes;
+        server_evict(ctx, s);
+        ++S.stats.evictions;
+        ++S.layout;
+    }
+    return true;
+}
+
+struct ServeBudgetWait {};
+
+// Makes a registered scene resident (first upload, or copy back). False when it does not fit
+// yet; a scene larger than the whole budget fails with GVBD_ERR_NOMEM.
+static bool server_admit(gvbd_context* ctx, ServerObj& S, int32_t sid, ServerObj::Entry& e) {
+    SceneObj& s = scene_of(ctx, sid);
+    auto too_large = [&](int64_t need) {
+        if (need > S.budget)
+            fail(GVBD_ERR_NOMEM, "scene " + std::to_string(sid) + " needs " + std::to_string(need) +
+                                     " device bytes, more than the server's budget");
+    };
+    if (s.initialized) {
+        if (!server_make_room(ctx, S, (int64_t)s.dev.bytes, sid)) return false;
+        server_restore(ctx, s);
+    } else {
+        if (e.needBytes > 0 && !server_make_room(ctx, S, e.needBytes, sid)) return false;
+        try {
+            initialize_scene(ctx, s, [&](size_t total) {
+                e.needBytes = (int64_t)total;
+                too_large(e.needBytes);
+                if (!server_make_room(ctx, S, (int64_t)total, sid)) throw ServeBudgetWait{};
+            });
+        } catch (const ServeBudgetWait&) {
+            return false;
+        }
+        s.plan = new_plan(ctx, {sid});
+    }
+    ++S.stats.admissions;
+    ++S.layout;
+    return true;
+}
+
+static int64_t free_vertices(const SceneObj& s) {
+    return (int64_t)s.tetGroupOffset.back() + (int64_t)s.clothGroupOffset.back();
+}
+
+static void server_tick(gvbd_context* ctx, ServerObj& S) {
+    using Clock = ServerObj::Clock;
+    // one frame of every resident scene with a pending request
+    std::vector<int32_t> ready;
+    for (auto& kv : S.scenes)
+        if (!kv.second.pending.empty() && scene_of(ctx, kv.first).dev.arena) ready.push_back(kv.first);
+    Clock::time_point launched = Clock::now();
+    if (!ready.empty()) {
+        if (ready != S.plan.scenes || S.layout != S.planLayout) {
+            free_plan(S.plan);
+            S.plan.scenes = ready;
+            S.planLayout = S.layout;
+        }
+        build_plan(ctx, S.plan);
+        launched = Clock::now();
+        plan_frame_launches(ctx, S.plan);
+    }
+    // admissions while that frame runs, oldest request first and without overtaking: a scene
+    // that does not fit holds back the ones behind it
+    std::vector<std::pair<Clock::time_point, int32_t>> waiting;
+    for (auto& kv : S.scenes)
+        if (!kv.second.pending.empty() && !scene_of(ctx, kv.first).dev.arena)
+            waiting.push_back({kv.second.pending.front().at, kv.first});
+    std::sort(waiting.begin(), waiting.end());
+    // A failed admission is reported after the launched frame is completed, so that no scene is
+    // left stepped without its request being completed. A scene that can never fit has its
+    // requests completed as rejected (frame -1, latency -1).
+    Failure admitError{GVBD_OK, ""};
+    for (auto& w : waiting) {
+        ServerObj::Entry& e = S.scenes[w.second];
+        try {
+            if (!server_admit(ctx, S, w.second, e)) break;
+        } catch (const Failure& f) {
+            admitError = f;
+            if (f.code == GVBD_ERR_NOMEM && e.needBytes > S.budget) {
+                for (const ServerObj::Request& r : e.pending)
+                    S.done.push_back({r.tag, w.second, -1, -1.0, -1.0});
+                e.pending.clear();
+                continue;
+            }
+            break;
+        }
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "tick sync");
+    check_queue_status(ctx);
+    const Clock::time_point finished = Clock::now();
+    for (int32_t sid : ready) {
+        ServerObj::Entry& e = S.scenes[sid];
+        const ServerObj::Request r = e.pending.front();
+        e.pending.pop_front();
+        e.lastUse = S.stats.ticks;
+        ++e.frames;
+        gvbd_served_step d{};
+        d.tag = r.tag;
+        d.scene = sid;
+        d.frame = e.frames;
+        d.latency_ms = std::chrono::duration<double, std::milli>(finished - r.at).count();
+        d.queue_ms = std::chrono::duration<double, std::milli>(launched - r.at).count();
+        S.done.push_back(d);
+        ++S.stats.served;
+        S.stats.vertex_solves += free_vertices(scene_of(ctx, sid)) * S.iterations * S.substeps;
+    }
+    S.stats.last_stepped = (int32_t)ready.size();
+    ++S.stats.ticks;
+    if (admitError.code != GVBD_OK) throw admitError;
+}
+
+static void server_fill_stats(gvbd_context* ctx, const ServerObj& S, gvbd_server_stats* out) {
+    gvbd_server_stats st = S.stats;
+    st.budget_bytes = S.budget;
+    st.resident_bytes = server_resident_bytes(ctx, S, &st.resident);
+    st.pending = 0;
+    st.waiting = 0;
+    for (auto& kv : S.scenes) {
+        st.pending += (int32_t)kv.second.pending.size();
+        if (!kv.second.pending.empty() && !scene_of(ctx, kv.first).dev.arena) ++st.waiting;
+    }
+    *out = st;
+}
+
+int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_out) {
+    return guard(ctx, [&] {
+        require(ctx && server_out, "NULL argument");
+        require(budget_bytes > 0, "budget_bytes must be positive");
+        size_t freeBytes = 0, totalBytes = 0;
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        hip_check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
+        require((uint64_t)budget_bytes <= (uint64_t)totalBytes, "budget_bytes exceeds the device's memory");
+        auto S = std::make_unique<ServerObj>();
+        S->budget = budget_bytes;
+        const int32_t id = ctx->nextServer++;
+        ctx->servers[id] = std::move(S);
+        *server_out = id;
+    });
+}
+
+int gvbd_server_destroy(gvbd_context* ctx, int32_t server) {
+    return guard(ctx, [&] {
+        ServerObj& S = server_of(ctx, server);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        for (auto& kv : S.scenes) {
+            SceneObj& s = scene_of(ctx, kv.first);
+            if (s.initialized && !s.dev.arena) server_restore(ctx, s);
+        }
+        for (auto& kv : S.scenes) scene_of(ctx, kv.first).server = -1;
+        free_plan(S.plan);
+        ctx->servers.erase(server);
+    });
+}
+
+int gvbd_server_set_schedule(gvbd_context* ctx, int32_t server, const gvbd_schedule* schedule) {
+    return guard(ctx, [&] {
+        require(schedule != nullptr, "schedule is NULL");
+        ServerObj& S = server_of(ctx, server);
+        free_plan(S.plan);
+        set_schedule(S.plan, *schedule);
+    });
+}
+
+int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
+    return guard(ctx, [&] {
+        ServerObj& S = server_of(ctx, server);
+        SceneObj& s = scene_of(ctx, scene);
+        if (s.server >= 0) fail(GVBD_ERR_STATE, "scene is already registered with a server");
+        require(!s.meshes.empty(), "scene has no meshes");
+        if (s.params.handle_collision && !s.cloths.empty())
+            fail(GVBD_ERR_INVALID, "scenes with collision handling are stepped alone (gvbd_scene_run_step), not served");
+        for (auto& kv : ctx->plans)
+            if (kv.first != s.plan)
+                for (int32_t sid : kv.second->scenes)
+                    if (sid == scene) fail(GVBD_ERR_STATE, "scene is part of a live batch; destroy the batch first");
+        if (S.iterations < 0) {
+            S.iterations = s.params.iterations;
+            S.substeps = s.params.num_substeps;
+            S.plan.iterations = S.iterations;
+            S.plan.substeps = S.substeps;
+        } else if (s.params.iterations != S.iterations || s.params.num_substeps != S.substeps) {
+            fail(GVBD_ERR_INVALID, "scenes of one server must share iterations and substeps");
+        }
+        ServerObj::Entry e;
+        if (s.initialized) {
+            e.needBytes = (int64_t)s.dev.bytes;
+            if (!server_make_room(ctx, S, e.needBytes, scene))
+                fail(GVBD_ERR_NOMEM, "the resident scene does not fit the server's budget");
+        }
+        S.scenes[scene] = e;
+        s.server = server;
+        ++S.layout;
+    });
+}
+
+int gvbd_server_remove_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
+    return guard(ctx, [&] {
+        ServerObj& S = server_of(ctx, server);
+        auto it = S.scenes.find(scene);
+        if (it == S.scenes.end()) fail(GVBD_ERR_INVALID, "scene is not registered with this server");
+        if (!it->second.pending.empty()) fail(GVBD_ERR_STATE, "scene has pending step requests");
+        SceneObj& s = scene_of(ctx, scene);
+        if (s.initialized && !s.dev.arena) server_restore(ctx, s);
+        s.server = -1;
+        S.scenes.erase(it);
+        free_plan(S.plan);
+        S.plan.scenes.clear();
+        ++S.layout;
+    });
+}
+
+int gvbd_server_request_step(gvbd_context* ctx, int32_t server, int32_t scene, int64_t tag) {
+    return guard(ctx, [&] {
+        ServerObj& S = server_of(ctx, server);
+        auto it = S.scenes.find(scene);
+        if (it == S.scenes.end()) fail(GVBD_ERR_INVALID, "scene is not registered with this server");
+        if (it->second.needBytes > S.budget)
+            fail(GVBD_ERR_NOMEM, "scene needs more device bytes than the server's budget");
+        it->second.pending.push_back({tag, ServerObj::Clock::now()});
+    });
+}
+
+int gvbd_server_tick(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats) {
+    return guard(ctx, [&] {
+        ServerObj& S = server_of(ctx, server);
+        server_tick(ctx, S);
+        if (stats) server_fill_stats(ctx, S, stats);
+    });
+}
+
+int gvbd_server_poll(gvbd_context* ctx, int32_t server, gvbd_served_step* out, int32_t capacity, int32_t* count) {
+    return guard(ctx, [&] {
+        require(count != nullptr && capacity >= 0 && (out || capacity == 0), "bad argument");
+        ServerObj& S = server_of(ctx, server);
+        int32_t n = 0;
+        while (n < capacity && !S.done.empty()) {
+            out[n++] = S.done.front();
+            S.done.pop_front();
+        }
+        *count = n;
+    });
+}
+
+int gvbd_server_get_stats(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats) {
+    return guard(ctx, [&] {
+        require(stats != nullptr, "stats is NULL");
+        server_fill_stats(ctx, server_of(ctx, server), stats);
+    });
+}
+
+int gvbd_server_scene_resident(gvbd_context* ctx, int32_t server, int32_t scene, int32_t* resident) {
+    return guard(ctx, [&] {
+        require(resident != nullptr, "resident is NULL");
+        ServerObj& S = server_of(ctx, server);
+        if (!S.scenes.count(scene)) fail(GVBD_ERR_INVALID, "scene is not registered with this server");
+        *resident = scene_of(ctx, scene).dev.arena ? 1 : 0;
+    });
+}
+
 // ---- contact detector ----
 
 static ContactDetector& detector_of(gvbd_context* ctx, int32_t cd) {
@@ -1631,6 +2012,7 @@ int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu
 }
 
 static_assert(sizeof(gvbd_vf_contact) == sizeof(VFContactRec), "gvbd_vf_contact layout");
+static_assert(sizeof(gvbd_server_stats) == 72 && sizeof(gvbd_served_step) == 32, "server record layout");
 static_assert(sizeof(gvbd_ee_contact) == sizeof(EEContactRec), "gvbd_ee_contact layout");
 
 int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records) {

@@ -12,8 +12,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (MODE_GROUP_LAUNCHES, MODE_WORK_QUEUE, ClothMeshDesc, PhysicsParams, Schedule, TetMeshDesc, check,
-                   fptr, f32, i32, iptr, lib)
+from ._lib import (MODE_GROUP_LAUNCHES, MODE_WORK_QUEUE, ClothMeshDesc, PhysicsParams, Schedule, ServedStep,
+                   ServerStats, TetMeshDesc, check, fptr, f32, i32, iptr, lib)
 
 # Reference defaults (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
 DEFAULT_PHYSICS = dict(
@@ -246,7 +246,9 @@ class VBDPhysics:
         self._pending.append(("cloth", positions, faces, fixed_points, vertex_colors, coloring, velocities,
                               initial_velocity, material))
 
-    def initialize(self):
+    def initialize(self, defer_upload=False):
+        """Creates the scene and uploads it. With defer_upload the scene is only created with
+        its meshes (host side): a Server uploads it when its first step is requested."""
         h = self.ctx.handle
         sid = ctypes.c_int32()
         if self._json is not None:
@@ -264,7 +266,8 @@ class VBDPhysics:
                     self._add_tet(pos, elems, fixed, colors, coloring, vel, v0, material)
                 else:
                     self._add_cloth(pos, elems, fixed, colors, coloring, vel, v0, material)
-            self.ctx.check(lib().gvbd_scene_initialize(h, self.scene))
+            if not defer_upload:
+                self.ctx.check(lib().gvbd_scene_initialize(h, self.scene))
         except Exception:
             lib().gvbd_scene_destroy(h, self.scene)
             self.scene = None
@@ -436,7 +439,7 @@ class VBDPhysics:
 
     def destroy(self):
         if self.scene is not None and self.ctx._h:
-            lib().gvbd_scene_destroy(self.ctx.handle, self.scene)
+            self.ctx.check(lib().gvbd_scene_destroy(self.ctx.handle, self.scene))
         self.scene = None
 
 
@@ -516,3 +519,71 @@ class Batch:
         if self.batch is not None and self.ctx._h:
             lib().gvbd_batch_destroy(self.ctx.handle, self.batch)
         self.batch = None
+
+
+SERVED_STEP_DTYPE = np.dtype([("tag", "<i8"), ("scene", "<i4"), ("frame", "<i4"), ("latency_ms", "<f8"),
+                              ("queue_ms", "<f8")])
+
+
+def _scene_id(s):
+    return s.scene if isinstance(s, VBDPhysics) else int(s)
+
+
+class Server:
+    """Serving loop: step requests for many scenes, continuously batched into one frame launch
+    sequence per tick, with scenes admitted to HBM under a byte budget (idle ones copied out to
+    host memory, least recently stepped first)."""
+
+    def __init__(self, ctx, budget_bytes):
+        self.ctx = ctx
+        sid = ctypes.c_int32()
+        ctx.check(lib().gvbd_server_create(ctx.handle, int(budget_bytes), ctypes.byref(sid)))
+        self.server = sid.value
+
+    def set_schedule(self, lanes_per_vertex, block, mode="group", vertices_per_entry=0, blocks_per_cu=0,
+                     spin_limit=0):
+        sc = make_schedule(lanes_per_vertex, block, mode, vertices_per_entry, blocks_per_cu, spin_limit)
+        self.ctx.check(lib().gvbd_server_set_schedule(self.ctx.handle, self.server, ctypes.byref(sc)))
+
+    def add_scene(self, scene):
+        self.ctx.check(lib().gvbd_server_add_scene(self.ctx.handle, self.server, _scene_id(scene)))
+
+    def remove_scene(self, scene):
+        self.ctx.check(lib().gvbd_server_remove_scene(self.ctx.handle, self.server, _scene_id(scene)))
+
+    def request_step(self, scene, tag=0):
+        self.ctx.check(lib().gvbd_server_request_step(self.ctx.handle, self.server, _scene_id(scene), int(tag)))
+
+    def tick(self):
+        st = ServerStats()
+        self.ctx.check(lib().gvbd_server_tick(self.ctx.handle, self.server, ctypes.byref(st)))
+        return _stats_dict(st)
+
+    def poll(self, capacity=4096):
+        """Completed requests (structured array: tag, scene, frame, latency_ms, queue_ms)."""
+        out = np.zeros(capacity, dtype=SERVED_STEP_DTYPE)
+        n = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_server_poll(self.ctx.handle, self.server,
+                                              out.ctypes.data_as(ctypes.POINTER(ServedStep)), capacity,
+                                              ctypes.byref(n)))
+        return out[: n.value]
+
+    def stats(self):
+        st = ServerStats()
+        self.ctx.check(lib().gvbd_server_get_stats(self.ctx.handle, self.server, ctypes.byref(st)))
+        return _stats_dict(st)
+
+    def resident(self, scene):
+        r = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_server_scene_resident(self.ctx.handle, self.server, _scene_id(scene),
+                                                        ctypes.byref(r)))
+        return bool(r.value)
+
+    def destroy(self):
+        if self.server is not None and self.ctx._h:
+            self.ctx.check(lib().gvbd_server_destroy(self.ctx.handle, self.server))
+        self.server = None
+
+
+def _stats_dict(st):
+    return {name: getattr(st, name) for name, _ in ServerStats._fields_}

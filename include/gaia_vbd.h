@@ -232,8 +232,60 @@ int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch);
 /* launch only the vertex solve of parallel group `group` (timing of the dominant kernel) */
 int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
 int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);
-/* algorithmic bytes of one solve launch of `group` (gather model; see DESIGN) and its vertex count */
+/* algorithmic bytes of one solve launch of `group` and its vertex count (gather model: per tet
+ * vertex 60 B + 96 B per incident tet, per cloth vertex 60 B + 56 B per face + 116 B per bending
+ * stencil) */
 int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
+
+/* ---- serving loop: continuous batching of step requests ----
+ * No reference counterpart (the reference steps one scene per process). Scenes registered with a
+ * server are made resident in HBM on their first step request when their device bytes fit the
+ * budget; idle resident scenes (no pending request) are copied out to host memory, least recently
+ * used first, to make room, and copied back when asked for again. Each tick advances every
+ * resident scene with a pending request by one frame in one batched launch sequence (the batch
+ * plan's kernels), then admits waiting scenes while that frame runs, then waits for the frame and
+ * completes one request per stepped scene. Results equal those of the scene stepped alone. */
+typedef struct gvbd_server_stats {
+    int64_t budget_bytes;
+    int64_t resident_bytes;  /* device bytes of the resident scenes */
+    int64_t vertex_solves;   /* free vertices x iterations x substeps, over all completed requests */
+    int64_t served;          /* completed step requests */
+    int64_t admissions;      /* scenes made resident (first upload or copy back) */
+    int64_t evictions;       /* idle scenes copied out to host memory */
+    int64_t ticks;
+    int32_t resident;        /* resident scenes */
+    int32_t pending;         /* step requests not completed yet */
+    int32_t waiting;         /* scenes with pending requests that are not resident */
+    int32_t last_stepped;    /* scenes advanced by the last tick */
+} gvbd_server_stats;
+
+typedef struct gvbd_served_step {
+    int64_t tag;       /* the caller's tag of the request */
+    int32_t scene;
+    int32_t frame;     /* frames this scene has completed under the server, this one included */
+    double latency_ms; /* request to completion: the frame's kernels have finished */
+    double queue_ms;   /* request to the launch of the frame that served it */
+} gvbd_served_step;
+
+/* budget_bytes: device bytes the server's scenes may hold at once (at most the device's memory) */
+int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_out);
+/* scenes stay alive and resident or not as they are; evicted scenes are copied back first */
+int gvbd_server_destroy(gvbd_context* ctx, int32_t server);
+int gvbd_server_set_schedule(gvbd_context* ctx, int32_t server, const gvbd_schedule* schedule);
+/* registers a created scene (meshes added; initialised or not) without collision handling; all
+ * scenes of a server share iterations and substeps */
+int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene);
+/* hands a scene back to the caller (resident again); it must have no pending request */
+int gvbd_server_remove_scene(gvbd_context* ctx, int32_t server, int32_t scene);
+/* queues one frame of `scene`; requests of one scene are served in order, one per tick */
+int gvbd_server_request_step(gvbd_context* ctx, int32_t server, int32_t scene, int64_t tag);
+/* one tick (see above); stats may be NULL */
+int gvbd_server_tick(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
+/* drains up to `capacity` completed requests in completion order */
+int gvbd_server_poll(gvbd_context* ctx, int32_t server, gvbd_served_step* out, int32_t capacity, int32_t* count);
+int gvbd_server_get_stats(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
+/* whether a registered scene is resident in HBM (1) or held in host memory / not uploaded yet (0) */
+int gvbd_server_scene_resident(gvbd_context* ctx, int32_t server, int32_t scene, int32_t* resident);
 
 /* ---- contact detector (GPU LBVH) ----
  * Replaces struct ClothContactDetector (CollisionDetector/ClothContactDetector.h:133-182): Embree
