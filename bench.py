@@ -22,6 +22,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+C4_ITERATIONS = 10
+SERVE_SCHEDULE = "group:8x128"  # the batched workload's autotuned schedule
 
 
 def log(msg):
@@ -204,6 +206,135 @@ def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
     ctx.close()
 
 
+def serve_bench(args):
+    """Served VBD: Poisson-arriving step requests over a Zipf-sized population of scenes,
+    continuously batched by the native serving loop (one frame of every ready scene per tick)."""
+    import gaia_amd
+    from gaia_amd import Server, VBDPhysics, scenes
+    from gaia_amd.autotune import parse_schedule_key, schedule_key
+
+    t0 = time.perf_counter()
+    kinds = scenes.c5_scene_kinds(args.serve_scenes, seed=args.seed)
+    templates = {}
+    for key in sorted(set(kinds)):
+        spec = scenes.c4_template(*key)
+        if spec["kind"] == "cloth":
+            colors = gaia_amd.color_tri_mesh(len(spec["positions"]), spec["faces"], spec["coloring"])
+        else:
+            colors = gaia_amd.color_tet_mesh(len(spec["positions"]), spec["tets"], spec["coloring"])
+        templates[key] = (spec, colors)
+    ctx = gaia_amd.Context(0)
+    # one prepared (not uploaded) scene per mesh; every served scene is a clone of one of them
+    protos = {}
+    for key, (spec, colors) in templates.items():
+        sim = VBDPhysics(ctx)
+        sim.setPhysicsParams(**spec["physics"])
+        if spec["kind"] == "cloth":
+            sim.addClothMesh(spec["positions"], spec["faces"], fixed_points=spec["fixed"], vertex_colors=colors,
+                             **spec["material"])
+        else:
+            sim.addTetMesh(spec["positions"], spec["tets"], vertex_colors=colors, **spec["material"])
+        sim.initialize(defer_upload=True)
+        protos[key] = sim
+    sims = [protos[key].clone() for key in kinds]
+    free = np.array([free_vertices(templates[k][0]) for k in kinds], dtype=np.int64)
+    iters = C4_ITERATIONS
+    log(f"registered {len(sims)} scenes ({len(templates)} meshes), {free.sum()} free vertices, "
+        f"in {time.perf_counter() - t0:.1f}s")
+    budget = int(args.budget_gb * 1e9)
+    srv = Server(ctx, budget)
+    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(args.schedule or SERVE_SCHEDULE)}
+    srv.set_schedule(sched["lanes_per_vertex"], sched["block"], sched["mode"], sched["vertices_per_entry"],
+                     sched["blocks_per_cu"])
+    for sim in sims:
+        srv.add_scene(sim)
+    # warm-up (untimed): one frame of every scene, which makes each resident on its first request
+    t = time.perf_counter()
+    for i, sim in enumerate(sims):
+        srv.request_step(sim, -1 - i)
+    while srv.stats()["pending"]:
+        srv.tick()
+    srv.poll(capacity=len(sims) + 16)
+    warm = srv.stats()
+    log(f"warm-up: {len(sims)} scenes admitted and stepped in {time.perf_counter() - t:.1f}s; resident "
+        f"{warm['resident_bytes'] / 1e9:.2f} GB of a {warm['budget_bytes'] / 1e9:.1f} GB budget, "
+        f"{warm['evictions']} evictions")
+
+    rng = np.random.default_rng(args.seed + 7)
+    R = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=R))
+    targets = rng.integers(0, len(sims), size=R)
+    finished = np.full(R, np.nan)
+    native = np.full(R, np.nan)
+    stepped = []
+    submitted = completed = 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    while completed < R:
+        now = time.perf_counter() - t0
+        while submitted < R and arrivals[submitted] <= now:
+            srv.request_step(sims[targets[submitted]], submitted)
+            submitted += 1
+        if submitted > completed:
+            st = srv.tick()
+            end = time.perf_counter() - t0
+            stepped.append(st["last_stepped"])
+            recs = srv.poll(capacity=R)
+            finished[recs["tag"]] = end
+            native[recs["tag"]] = recs["latency_ms"]
+            completed += len(recs)
+        else:
+            time.sleep(max(0.0, arrivals[submitted] - now))
+    wall = time.perf_counter() - t0
+    lat = (finished - arrivals) * 1e3
+    solves = int(free[targets].sum()) * iters
+    rate = solves / wall
+    stats = srv.stats()
+    pct = lambda a: {"p50": round(float(np.percentile(a, 50)), 3), "p99": round(float(np.percentile(a, 99)), 3),
+                     "mean": round(float(a.mean()), 3), "max": round(float(a.max()), 3)}
+    offered = float(free[targets].mean()) * iters * args.rate
+    out = {
+        "metric": "served VBD vertex-solves/sec at 10 iters; step latency p50/p99 ms",
+        "value": round(rate, 1),
+        "unit": "vertex-solves/s",
+        "n_gpus": 1,
+        "steps": R,
+        "warmup": 1,
+        "ms_per_step": round(wall / max(len(stepped), 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"served VBD: {len(sims)} scenes of 1k-200k vertices (32 log-spaced size classes, Zipf(1.2) "
+                        f"over class rank, seed {args.seed}), half neo-Hookean tet cubes and half StVK+bending "
+                        f"sheets, dt=1/60, 10 iterations; {R} step requests, Poisson at {args.rate:g}/s, uniform "
+                        f"over scenes; one batched frame of every ready scene per tick",
+            "scenes": len(sims),
+            "vertices": int(free.sum()),
+            "request_rate_per_s": args.rate,
+            "requests": R,
+            "budget_gb": round(stats["budget_bytes"] / 1e9, 1),
+            "schedule": dict(sched, key=schedule_key(sched)),
+        },
+        "latency_ms": pct(lat),
+        "latency_from_submission_ms": pct(native),
+        "offered_vertex_solves_per_s": round(offered, 1),
+        "ticks": len(stepped),
+        "scenes_per_tick_mean": round(float(np.mean(stepped)), 2),
+        "resident_gb": round(stats["resident_bytes"] / 1e9, 3),
+        "admissions": stats["admissions"],
+        "evictions": stats["evictions"],
+    }
+    log(f"serve: {out}")
+    print(json.dumps(out), flush=True)
+    srv.destroy()
+    for sim in sims + list(protos.values()):
+        sim.destroy()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,8 +350,13 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth calibration")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (profiler runs)")
-    ap.add_argument("--workload", choices=["batched", "detector"], default="batched",
-                    help="batched VBD (default) or the contact detector alone on 64 tori")
+    ap.add_argument("--workload", choices=["batched", "detector", "serve"], default="batched",
+                    help="batched VBD (default), the contact detector alone on 64 tori, or served VBD")
+    ap.add_argument("--serve-scenes", type=int, default=2000)
+    ap.add_argument("--requests", type=int, default=20000, help="served VBD: step requests in the timed window")
+    ap.add_argument("--rate", type=float, default=20000.0, help="served VBD: Poisson request rate per second")
+    ap.add_argument("--budget-gb", type=float, default=288.0,
+                    help="served VBD: device-memory budget in GB (clamped to the device's memory)")
     args = ap.parse_args()
     use_graph = not args.no_graph
     if args.gpus != 1:
@@ -230,6 +366,9 @@ def main():
         sys.exit(2)
     if args.workload == "detector":
         detector_bench(args)
+        return
+    if args.workload == "serve":
+        serve_bench(args)
         return
 
     import gaia_amd

@@ -204,6 +204,7 @@ _SIGNATURES = {
     "gvbd_batch_solve_parallel_group": [_P, _I, _I, _I],
     "gvbd_batch_num_parallel_groups": [_P, _I, c_int32_p],
     "gvbd_batch_group_work": [_P, _I, _I, c_int64_p, c_int64_p],
+    "gvbd_scene_clone": [_P, _I, c_int32_p],
     "gvbd_server_create": [_P, ctypes.c_int64, c_int32_p],
     "gvbd_server_destroy": [_P, _I],
     "gvbd_server_set_schedule": [_P, _I, _SCHED],

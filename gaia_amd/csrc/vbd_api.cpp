@@ -1011,6 +1011,23 @@ int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int3
     });
 }
 
+int gvbd_scene_clone(gvbd_context* ctx, int32_t scene, int32_t* scene_out) {
+    return guard(ctx, [&] {
+        require(ctx && scene_out, "NULL argument");
+        const SceneObj& src = scene_of(ctx, scene);
+        auto s = std::make_unique<SceneObj>();
+        s->params = src.params;
+        s->tets = src.tets;
+        s->cloths = src.cloths;
+        s->meshes = src.meshes;
+        s->vertexOffset = src.vertexOffset;
+        s->nv = src.nv;
+        const int32_t id = ctx->nextScene++;
+        ctx->scenes[id] = std::move(s);
+        *scene_out = id;
+    });
+}
+
 int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
@@ -1766,9 +1783,8 @@ int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_
         size_t freeBytes = 0, totalBytes = 0;
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         hip_check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
-        require((uint64_t)budget_bytes <= (uint64_t)totalBytes, "budget_bytes exceeds the device's memory");
         auto S = std::make_unique<ServerObj>();
-        S->budget = budget_bytes;
+        S->budget = std::min<int64_t>(budget_bytes, (int64_t)totalBytes);
         const int32_t id = ctx->nextServer++;
         ctx->servers[id] = std::move(S);
         *server_out = id;

@@ -136,6 +136,46 @@ def c4_scenes(seed=0, num_scenes=256, vmin=500, vmax=50000, cloth_fraction=0.5, 
     return out
 
 
+def zipf_size_classes(num_scenes, vmin=1000, vmax=200000, exponent=1.2, classes=32, seed=0):
+    """Vertex-count targets of a served population: 32 size classes spaced log-uniformly from
+    vmin to vmax; class rank k (1 = smallest) drawn with probability proportional to k^-exponent
+    (seeded Zipf over the ranks). The mean is about 9.3k vertices for the defaults."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, classes + 1)
+    w = k ** -float(exponent)
+    ranks = rng.choice(classes, size=num_scenes, p=w / w.sum())
+    sizes = vmin * (vmax / vmin) ** (np.arange(classes) / (classes - 1))
+    return sizes[ranks]
+
+
+def c5_scene_kinds(num_scenes=2000, vmin=1000, vmax=200000, exponent=1.2, cloth_fraction=0.5, seed=0):
+    """Served-VBD population: (kind, grid size) per scene, with the batched workload's generators
+    (tet cube of n^3 cells, or an N x N cloth sheet pinned at two corners) at Zipf-drawn vertex
+    targets. Scenes of one (kind, grid size) share their mesh; each has its own state."""
+    sizes = zipf_size_classes(num_scenes, vmin, vmax, exponent, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    cloth = rng.random(num_scenes) < cloth_fraction
+    out = []
+    for target, is_cloth in zip(sizes, cloth):
+        if is_cloth:
+            out.append(("cloth", max(3, int(round(np.sqrt(target))))))
+        else:
+            out.append(("tet", max(2, int(round(target ** (1.0 / 3.0))) - 1)))
+    return out
+
+
+def c4_template(kind, n):
+    """One scene of the batched workload's generators (see c4_scenes) for a given grid size."""
+    mu_c, lm_c = stvk_lame(1e5, 0.3)
+    if kind == "cloth":
+        pos, faces = cloth_sheet(n, 1.0, 1.5, dtype=np.float32)
+        return dict(kind="cloth", positions=pos, faces=faces, physics=dict(C4_PHYSICS), fixed=[0, n - 1],
+                    material=dict(density=0.2, mu=mu_c, lmbd=lm_c, bending_stiffness=1e-4), coloring="mcs")
+    pos, tets = cube_tet_mesh(n, 0.05, origin=(0.0, 0.1, 0.0), dtype=np.float32)
+    return dict(kind="tet", positions=pos, tets=tets, physics=dict(C4_PHYSICS), fixed=[],
+                material=dict(density=1000.0, mu=1e5, lmbd=1e6), coloring="greedy")
+
+
 def c4_tet_scene_sizes(seed=0, num_scenes=256, vmin=500, vmax=50000):
     """Vertex-count targets drawn log-uniform in [vmin, vmax] (seeded)."""
     rng = np.random.default_rng(seed)

@@ -437,6 +437,17 @@ class VBDPhysics:
                                                        fptr(x) if x is not None else None,
                                                        fptr(v) if v is not None else None))
 
+    def clone(self, defer_upload=True):
+        """A new scene with this one's parameters and meshes as added (host preparation copied);
+        with defer_upload it is not uploaded yet (see initialize)."""
+        out = VBDPhysics(self.ctx)
+        sid = ctypes.c_int32()
+        self.ctx.check(lib().gvbd_scene_clone(self.ctx.handle, self.scene, ctypes.byref(sid)))
+        out.scene = sid.value
+        if not defer_upload:
+            self.ctx.check(lib().gvbd_scene_initialize(self.ctx.handle, out.scene))
+        return out
+
     def destroy(self):
         if self.scene is not None and self.ctx._h:
             self.ctx.check(lib().gvbd_scene_destroy(self.ctx.handle, self.scene))

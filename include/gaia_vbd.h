@@ -169,6 +169,10 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
 /* ---- scenes ---- */
 int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
 int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene);
+/* a new, not yet initialised scene with the parameters and meshes of `scene` (positions and
+ * velocities as they were added): the host-side mesh preparation is copied, not redone. No
+ * reference counterpart; it instances one mesh many times (served populations). */
+int gvbd_scene_clone(gvbd_context* ctx, int32_t scene, int32_t* scene_out);
 int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mesh_desc* desc, int32_t* mesh_out);
 /* VBDClothSimulationFramework mesh load + TriMeshFEM::initialize (Framework/BaseClothSimPhsicsFramework.cpp,
  * TriMesh/TriMesh.cpp:15-111) */
@@ -267,7 +271,8 @@ typedef struct gvbd_served_step {
     double queue_ms;   /* request to the launch of the frame that served it */
 } gvbd_served_step;
 
-/* budget_bytes: device bytes the server's scenes may hold at once (at most the device's memory) */
+/* budget_bytes: device bytes the server's scenes may hold at once (clamped to the device's memory;
+ * gvbd_server_get_stats reports the budget in force) */
 int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_out);
 /* scenes stay alive and resident or not as they are; evicted scenes are copied back first */
 int gvbd_server_destroy(gvbd_context* ctx, int32_t server);

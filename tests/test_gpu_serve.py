@@ -177,3 +177,19 @@ def test_scene_larger_than_the_budget_is_rejected(gpu_ctx):
     srv.destroy()
     big.destroy()
     ok.destroy()
+
+
+def test_clone_equals_original(gpu_ctx):
+    """A clone (host preparation copied) steps exactly like a scene built from the same input."""
+    sp = scenes.c4_scenes(seed=9, num_scenes=2, vmin=800, vmax=1500)
+    for spec in sp:
+        proto = build(gpu_ctx, spec, defer_upload=True)
+        a = proto.clone(defer_upload=False)
+        b = build(gpu_ctx, spec)
+        a.runSteps(3)
+        b.runSteps(3)
+        xa, va = a.getState(0)
+        xb, vb = b.getState(0)
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        for s in (proto, a, b):
+            s.destroy()
