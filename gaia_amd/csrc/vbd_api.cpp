@@ -119,16 +119,30 @@ struct ServerObj {
         Clock::time_point at;
     };
     struct Entry {
-        std::deque<Request> pending;
+        std::deque<Request> pending;  // the first `launched` have a frame in flight
+        int32_t launched = 0;
         int64_t lastUse = -1;  // tick of the scene's last frame: idle scenes leave HBM in this order
         int32_t frames = 0;
         int64_t needBytes = 0;  // device bytes, once known
     };
+    // A launched frame: its scenes, the plan slot whose buffers its kernels read, launch time.
+    struct Frame {
+        std::vector<int32_t> scenes;
+        int slot;
+        Clock::time_point launched;
+    };
     int64_t budget = 0;
     int iterations = -1, substeps = -1;
-    PlanObj plan;  // the scenes stepped by the current tick
-    int64_t layout = 0;       // bumped whenever a scene enters or leaves HBM (device addresses move)
-    int64_t planLayout = -1;  // `layout` when the plan's device views were built
+    // Two plans used alternately, so that one tick's frame can be built and queued while the
+    // previous one still runs; each slot has the completion event of its last frame.
+    PlanObj plans[2];
+    int64_t planLayout[2] = {-1, -1};  // `layout` when each plan's device views were built
+    hipEvent_t frameDone[2] = {nullptr, nullptr};
+    int nextSlot = 0;
+    std::deque<Frame> inflight;
+    hipEvent_t ref = nullptr;  // completed at creation; frame completion times are measured from it
+    Clock::time_point refTime;
+    int64_t layout = 0;  // bumped whenever a scene enters or leaves HBM (device addresses move)
     std::map<int32_t, Entry> scenes;
     std::deque<gvbd_served_step> done;
     gvbd_server_stats stats{};
@@ -961,7 +975,12 @@ int gvbd_context_destroy(gvbd_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->plans) free_plan(*kv.second);
-    for (auto& kv : ctx->servers) free_plan(kv.second->plan);
+    for (auto& kv : ctx->servers) {
+        for (gvbd::PlanObj& P : kv.second->plans) free_plan(P);
+        for (hipEvent_t e : kv.second->frameDone)
+            if (e) (void)hipEventDestroy(e);
+        if (kv.second->ref) (void)hipEventDestroy(kv.second->ref);
+    }
     ctx->detectors.clear();
     for (auto& kv : ctx->scenes)
         if (kv.second->dev.arena) (void)hipFree(kv.second->dev.arena);
@@ -992,7 +1011,8 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes) {
             if (kv.second->cd) b += (int64_t)kv.second->cd->deviceBytes();
         }
         for (auto& kv : ctx->plans) b += (int64_t)kv.second->bytes;
-        for (auto& kv : ctx->servers) b += (int64_t)kv.second->plan.bytes;
+        for (auto& kv : ctx->servers)
+            for (const PlanObj& P : kv.second->plans) b += (int64_t)P.bytes;
         for (auto& kv : ctx->detectors) b += (int64_t)kv.second->deviceBytes();
         *bytes = b;
     });
@@ -1570,6 +1590,14 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
 
 // ---- serving loop ----
 
+// Frees a server's device resources (its frames must be complete).
+static void release_server(ServerObj& S) {
+    for (PlanObj& P : S.plans) free_plan(P);
+    for (hipEvent_t& e : S.frameDone)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (S.ref) (void)hipEventDestroy(S.ref), S.ref = nullptr;
+}
+
 static ServerObj& server_of(gvbd_context* ctx, int32_t id) {
     require(ctx != nullptr, "ctx is NULL");
     auto it = ctx->servers.find(id);
@@ -1696,33 +1724,74 @@ static int64_t free_vertices(const SceneObj& s) {
     return (int64_t)s.tetGroupOffset.back() + (int64_t)s.clothGroupOffset.back();
 }
 
+// Completes the oldest frame in flight: one request of each of its scenes.
+static void server_complete_oldest(gvbd_context* ctx, ServerObj& S) {
+    using Clock = ServerObj::Clock;
+    ServerObj::Frame f = std::move(S.inflight.front());
+    S.inflight.pop_front();
+    hip_check(hipEventSynchronize(S.frameDone[f.slot]), "frame wait");
+    check_queue_status(ctx);
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, S.ref, S.frameDone[f.slot]), "frame time");
+    const Clock::time_point finished =
+        S.refTime + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::milli>(ms));
+    for (int32_t sid : f.scenes) {
+        ServerObj::Entry& e = S.scenes[sid];
+        const ServerObj::Request r = e.pending.front();
+        e.pending.pop_front();
+        --e.launched;
+        e.lastUse = S.stats.ticks;
+        ++e.frames;
+        gvbd_served_step d{};
+        d.tag = r.tag;
+        d.scene = sid;
+        d.frame = e.frames;
+        d.latency_ms = std::chrono::duration<double, std::milli>(finished - r.at).count();
+        d.queue_ms = std::chrono::duration<double, std::milli>(f.launched - r.at).count();
+        S.done.push_back(d);
+        ++S.stats.served;
+        S.stats.vertex_solves += free_vertices(scene_of(ctx, sid)) * S.iterations * S.substeps;
+    }
+}
+
+static void server_drain(gvbd_context* ctx, ServerObj& S) {
+    while (!S.inflight.empty()) server_complete_oldest(ctx, S);
+}
+
 static void server_tick(gvbd_context* ctx, ServerObj& S) {
     using Clock = ServerObj::Clock;
-    // one frame of every resident scene with a pending request
+    // one frame of every resident scene with a request not launched yet; a scene whose previous
+    // frame is still in flight is stepped again behind it (one stream orders the two)
     std::vector<int32_t> ready;
     for (auto& kv : S.scenes)
-        if (!kv.second.pending.empty() && scene_of(ctx, kv.first).dev.arena) ready.push_back(kv.first);
-    Clock::time_point launched = Clock::now();
+        if (kv.second.pending.size() > (size_t)kv.second.launched && scene_of(ctx, kv.first).dev.arena)
+            ready.push_back(kv.first);
     if (!ready.empty()) {
-        if (ready != S.plan.scenes || S.layout != S.planLayout) {
-            free_plan(S.plan);
-            S.plan.scenes = ready;
-            S.planLayout = S.layout;
+        const int slot = S.nextSlot;
+        while (!S.inflight.empty() && S.inflight.front().slot == slot) server_complete_oldest(ctx, S);
+        PlanObj& P = S.plans[slot];
+        if (ready != P.scenes || S.layout != S.planLayout[slot]) {
+            free_plan(P);
+            P.scenes = ready;
+            S.planLayout[slot] = S.layout;
         }
-        build_plan(ctx, S.plan);
-        launched = Clock::now();
-        plan_frame_launches(ctx, S.plan);
+        build_plan(ctx, P);
+        const Clock::time_point launched = Clock::now();
+        plan_frame_launches(ctx, P);
+        hip_check(hipEventRecord(S.frameDone[slot], ctx->stream), "frame event");
+        for (int32_t sid : ready) ++S.scenes[sid].launched;
+        S.inflight.push_back({ready, slot, launched});
+        S.nextSlot = slot ^ 1;
     }
     // admissions while that frame runs, oldest request first and without overtaking: a scene
-    // that does not fit holds back the ones behind it
+    // that does not fit holds back the ones behind it. A failed admission is reported after the
+    // bookkeeping below; a scene that can never fit has its requests completed as rejected
+    // (frame -1, latency -1).
     std::vector<std::pair<Clock::time_point, int32_t>> waiting;
     for (auto& kv : S.scenes)
         if (!kv.second.pending.empty() && !scene_of(ctx, kv.first).dev.arena)
             waiting.push_back({kv.second.pending.front().at, kv.first});
     std::sort(waiting.begin(), waiting.end());
-    // A failed admission is reported after the launched frame is completed, so that no scene is
-    // left stepped without its request being completed. A scene that can never fit has its
-    // requests completed as rejected (frame -1, latency -1).
     Failure admitError{GVBD_OK, ""};
     for (auto& w : waiting) {
         ServerObj::Entry& e = S.scenes[w.second];
@@ -1731,32 +1800,18 @@ static void server_tick(gvbd_context* ctx, ServerObj& S) {
         } catch (const Failure& f) {
             admitError = f;
             if (f.code == GVBD_ERR_NOMEM && e.needBytes > S.budget) {
-                for (const ServerObj::Request& r : e.pending)
-                    S.done.push_back({r.tag, w.second, -1, -1.0, -1.0});
+                for (const ServerObj::Request& r : e.pending) S.done.push_back({r.tag, w.second, -1, -1.0, -1.0});
                 e.pending.clear();
                 continue;
             }
             break;
         }
     }
-    hip_check(hipStreamSynchronize(ctx->stream), "tick sync");
-    check_queue_status(ctx);
-    const Clock::time_point finished = Clock::now();
-    for (int32_t sid : ready) {
-        ServerObj::Entry& e = S.scenes[sid];
-        const ServerObj::Request r = e.pending.front();
-        e.pending.pop_front();
-        e.lastUse = S.stats.ticks;
-        ++e.frames;
-        gvbd_served_step d{};
-        d.tag = r.tag;
-        d.scene = sid;
-        d.frame = e.frames;
-        d.latency_ms = std::chrono::duration<double, std::milli>(finished - r.at).count();
-        d.queue_ms = std::chrono::duration<double, std::milli>(launched - r.at).count();
-        S.done.push_back(d);
-        ++S.stats.served;
-        S.stats.vertex_solves += free_vertices(scene_of(ctx, sid)) * S.iterations * S.substeps;
+    // keep at most the frame just launched in flight; with nothing launched, finish everything
+    if (!ready.empty()) {
+        while (S.inflight.size() > 1) server_complete_oldest(ctx, S);
+    } else {
+        server_drain(ctx, S);
     }
     S.stats.last_stepped = (int32_t)ready.size();
     ++S.stats.ticks;
@@ -1785,6 +1840,11 @@ int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_
         hip_check(hipMemGetInfo(&freeBytes, &totalBytes), "hipMemGetInfo");
         auto S = std::make_unique<ServerObj>();
         S->budget = std::min<int64_t>(budget_bytes, (int64_t)totalBytes);
+        for (hipEvent_t& e : S->frameDone) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventCreate(&S->ref), "hipEventCreate");
+        hip_check(hipEventRecord(S->ref, ctx->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(S->ref), "hipEventSynchronize");
+        S->refTime = ServerObj::Clock::now();
         const int32_t id = ctx->nextServer++;
         ctx->servers[id] = std::move(S);
         *server_out = id;
@@ -1794,13 +1854,13 @@ int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_
 int gvbd_server_destroy(gvbd_context* ctx, int32_t server) {
     return guard(ctx, [&] {
         ServerObj& S = server_of(ctx, server);
-        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        server_drain(ctx, S);
         for (auto& kv : S.scenes) {
             SceneObj& s = scene_of(ctx, kv.first);
             if (s.initialized && !s.dev.arena) server_restore(ctx, s);
         }
         for (auto& kv : S.scenes) scene_of(ctx, kv.first).server = -1;
-        free_plan(S.plan);
+        release_server(S);
         ctx->servers.erase(server);
     });
 }
@@ -1809,8 +1869,11 @@ int gvbd_server_set_schedule(gvbd_context* ctx, int32_t server, const gvbd_sched
     return guard(ctx, [&] {
         require(schedule != nullptr, "schedule is NULL");
         ServerObj& S = server_of(ctx, server);
-        free_plan(S.plan);
-        set_schedule(S.plan, *schedule);
+        server_drain(ctx, S);
+        for (PlanObj& P : S.plans) {
+            free_plan(P);
+            set_schedule(P, *schedule);
+        }
     });
 }
 
@@ -1829,8 +1892,10 @@ int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
         if (S.iterations < 0) {
             S.iterations = s.params.iterations;
             S.substeps = s.params.num_substeps;
-            S.plan.iterations = S.iterations;
-            S.plan.substeps = S.substeps;
+            for (PlanObj& P : S.plans) {
+                P.iterations = S.iterations;
+                P.substeps = S.substeps;
+            }
         } else if (s.params.iterations != S.iterations || s.params.num_substeps != S.substeps) {
             fail(GVBD_ERR_INVALID, "scenes of one server must share iterations and substeps");
         }
@@ -1856,8 +1921,10 @@ int gvbd_server_remove_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
         if (s.initialized && !s.dev.arena) server_restore(ctx, s);
         s.server = -1;
         S.scenes.erase(it);
-        free_plan(S.plan);
-        S.plan.scenes.clear();
+        for (PlanObj& P : S.plans) {
+            free_plan(P);
+            P.scenes.clear();
+        }
         ++S.layout;
     });
 }

@@ -20,8 +20,15 @@
  *   gvbd_scene_get/set_mesh_state PhysicsState positions/velocities (checkpoint) (Framework/BasePhysicsFramework.h:145-319)
  *   gvbd_color_*                GraphColoring Mcs / OrderedGreedy                (GraphColoring/mcs.cpp:14-168,
  *                                                                                 GraphColoring/gready.cpp:15-117)
+ *   gvbd_scene_add_cloth_mesh   TriMeshFEM / VBDBaseTriMesh initialize           (TriMesh/TriMesh.cpp:15-111,481-753,
+ *                                                                                 VBDCloth/VBDTriMeshStVK.cpp:50-88)
+ *   gvbd_scene_collision_detection VBDClothSimulationFramework::applyCollisionDetection
+ *                                                                                (VBDCloth/VBDClothPhysics.cpp:719-819)
+ *   gvbd_cd_*                   ClothContactDetector (initialize, updateBVH, contactQueryVF/EE)
+ *                                                                                (CollisionDetector/ClothContactDetector.h:133-182)
  *   gvbd_batch_*                (no reference counterpart: many scenes per launch; the reference's
  *                                nearest form is the multi-mesh parallel-group merge, VBD/VBDPhysics.cpp:235-274)
+ *   gvbd_server_*, gvbd_scene_clone  (no reference counterpart: serving loop over many scenes)
  *
  * Conventions: every function returns GVBD_OK (0) or a negative status; the
  * message is available from gvbd_last_error(ctx) (or gvbd_last_error(NULL) for

@@ -91,6 +91,33 @@ def test_served_equals_alone_bitwise(gpu_ctx, budget_fraction):
         s.destroy()
 
 
+def test_frames_overlap_two_deep(gpu_ctx):
+    """A tick queues its frame behind the one still running and completes only the older one:
+    a scene with two requests gets both frames launched in consecutive ticks."""
+    specs = scenes.c4_scenes(seed=3, num_scenes=2, vmin=500, vmax=800)
+    sims = [build(gpu_ctx, sp) for sp in specs]  # uploaded: resident from registration
+    srv = Server(gpu_ctx, 1 << 34)
+    for s in sims:
+        srv.add_scene(s)
+    srv.request_step(sims[0], 0)
+    srv.request_step(sims[0], 1)
+    srv.request_step(sims[1], 2)
+    st = srv.tick()
+    assert st["last_stepped"] == 2 and st["pending"] == 3 and len(srv.poll()) == 0
+    st = srv.tick()  # scene 0's second frame queued, the first tick's frame completed
+    assert st["last_stepped"] == 1 and st["pending"] == 1
+    assert sorted(srv.poll()["tag"].tolist()) == [0, 2]
+    st = srv.tick()  # nothing new: drains
+    assert st["last_stepped"] == 0 and st["pending"] == 0
+    assert srv.poll()["tag"].tolist() == [1]
+    alone = build(gpu_ctx, specs[0])
+    alone.runSteps(2)
+    assert np.array_equal(alone.getState(0)[0], sims[0].getState(0)[0])
+    srv.destroy()
+    for s in sims + [alone]:
+        s.destroy()
+
+
 def test_requests_of_a_scene_complete_in_order_one_per_tick(gpu_ctx):
     specs = scenes.c4_scenes(seed=2, num_scenes=4, vmin=300, vmax=2000)
     reqs = [[0, 0, 0, 1], [2], [], [0, 3, 3]]
@@ -117,6 +144,8 @@ def test_least_recently_stepped_idle_scene_leaves_first(gpu_ctx):
     srv.request_step(sims[2], 99)
     srv.tick()  # admits scene 2: scene 0 (stepped least recently) is copied out
     assert not srv.resident(sims[0]) and srv.resident(sims[1]) and srv.resident(sims[2])
+    srv.tick()  # launches scene 2's frame
+    assert srv.stats()["pending"] == 1  # in flight until a later tick completes it
     srv.tick()
     assert srv.stats()["evictions"] == 1 and srv.stats()["pending"] == 0
     # a scene held in host memory cannot be stepped outside the server, but can be read
@@ -168,7 +197,8 @@ def test_scene_larger_than_the_budget_is_rejected(gpu_ctx):
     srv.request_step(ok, 2)
     with pytest.raises(gaia_amd.VBDError, match="budget"):
         srv.tick()
-    srv.tick()
+    while srv.stats()["pending"]:
+        srv.tick()
     done = srv.poll()
     rec = {int(t): int(f) for t, f in zip(done["tag"], done["frame"])}
     assert rec == {1: -1, 2: 1}
