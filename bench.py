@@ -6,9 +6,12 @@ resident in HBM. `value` is the batched rate: all scenes advanced together, eith
 by one persistent work-queue launch per iteration or by one launch per parallel
 group (the autotuner times both and keeps the faster), the frame captured in a HIP
 graph. The plain rate (one scene per launch sequence, each scene its own graph)
-is reported beside it.
+is reported beside it, and a served leg ("served": 2000 scenes of 1k-200k vertices,
+Poisson step requests continuously batched by the serving loop) reports served
+throughput with p50/p99 step latency.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 256] [--seed 0] [--schedule KEY]
+       python bench.py --workload detector | serve   (the detector alone; the served leg alone)
 """
 import argparse
 import json
@@ -206,9 +209,10 @@ def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
     ctx.close()
 
 
-def serve_bench(args):
+def serve_bench(args, schedule=None):
     """Served VBD: Poisson-arriving step requests over a Zipf-sized population of scenes,
-    continuously batched by the native serving loop (one frame of every ready scene per tick)."""
+    continuously batched by the native serving loop (one frame of every ready scene per tick).
+    Returns the result line (a dict)."""
     import gaia_amd
     from gaia_amd import Server, VBDPhysics, scenes
     from gaia_amd.autotune import parse_schedule_key, schedule_key
@@ -243,9 +247,12 @@ def serve_bench(args):
         f"in {time.perf_counter() - t0:.1f}s")
     budget = int(args.budget_gb * 1e9)
     srv = Server(ctx, budget)
-    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(args.schedule or SERVE_SCHEDULE)}
+    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0,
+             **(schedule or parse_schedule_key(args.schedule or SERVE_SCHEDULE))}
+    if sched["mode"] != "group":  # the served plan changes every tick: group launches
+        sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(SERVE_SCHEDULE)}
     srv.set_schedule(sched["lanes_per_vertex"], sched["block"], sched["mode"], sched["vertices_per_entry"],
-                     sched["blocks_per_cu"])
+                     sched["blocks_per_cu"], stage_records=sched.get("stage_records", 0))
     for sim in sims:
         srv.add_scene(sim)
     # warm-up (untimed): one frame of every scene, which makes each resident on its first request
@@ -330,11 +337,11 @@ def serve_bench(args):
         "evictions": stats["evictions"],
     }
     log(f"serve: {out}")
-    print(json.dumps(out), flush=True)
     srv.destroy()
     for sim in sims + list(protos.values()):
         sim.destroy()
     ctx.close()
+    return out
 
 
 def main():
@@ -346,6 +353,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg of the default run")
     ap.add_argument("--schedule", default=None,
                     help="fixed schedule instead of autotuning: group:LxB or queue:LxB:eN[:gM] "
                          "(lanes per vertex x block, N vertices per queue entry, M blocks per CU)")
@@ -370,7 +378,7 @@ def main():
         detector_bench(args)
         return
     if args.workload == "serve":
-        serve_bench(args)
+        print(json.dumps(serve_bench(args)), flush=True)
         return
 
     import gaia_amd
@@ -405,7 +413,8 @@ def main():
     plain_rate = None
     if not args.no_plain:
         for s in sims:
-            s.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
+            s.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group",
+                          stage_records=sched.get("stage_records", 0))
             s.recordVBDSolveGraph(use_graph)
         for _ in range(args.warmup):
             for s in sims:
@@ -447,7 +456,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     out = {
-        "metric": "VBD vertex-solves/sec at 10 iters (batched)",
+        "metric": "VBD vertex-solves/sec at 10 iters (batched vs plain); served step p50/p99 ms",
         "value": round(rate, 1),
         "unit": "vertex-solves/s",
         "n_gpus": 1,
@@ -478,11 +487,18 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     batch.destroy()
     for s in sims:
         s.destroy()
     ctx.close()
+    if not args.no_serve:
+        # served leg: the served population under Poisson step requests, batched schedule
+        sv = serve_bench(args, schedule=sched)
+        out["served"] = {k: sv[k] for k in ("value", "unit", "latency_ms", "offered_vertex_solves_per_s",
+                                            "scenes_per_tick_mean", "ticks", "resident_gb")}
+        out["served"]["workload"] = sv["config"]["workload"]
+        out["served"]["schedule"] = sv["config"]["schedule"]["key"]
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

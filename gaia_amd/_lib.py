@@ -123,7 +123,8 @@ class Schedule(ctypes.Structure):
         ("vertices_per_entry", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
         ("spin_limit", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("stage_records", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

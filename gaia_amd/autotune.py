@@ -2,8 +2,9 @@
 (HIP events on the library's stream) for each candidate schedule and keeps the fastest.
 
 Knobs: launch mode (one launch per parallel group, or one persistent work-queue launch per
-iteration), lanes per vertex (vertices per wavefront = 64 / lanes), block size, and for the
-work queue the entry size (vertices per entry) and the persistent grid per CU.
+iteration), lanes per vertex (vertices per wavefront = 64 / lanes), block size, for group
+launches the LDS staging depth (incidence records staged per chunk, 0 = read from HBM by the
+lanes), and for the work queue the entry size (vertices per entry) and the persistent grid per CU.
 """
 import itertools
 
@@ -14,6 +15,8 @@ def default_candidates(modes=("group", "queue")):
     out = []
     if "group" in modes:
         out += [dict(lanes_per_vertex=l, block=b, mode="group") for l, b in DEFAULT_GROUP]
+        for (l, b), st in itertools.product([(4, 128), (8, 128), (8, 256), (16, 256)], [32, 64]):
+            out.append(dict(lanes_per_vertex=l, block=b, mode="group", stage_records=st))
     if "queue" in modes:
         for (l, b), passes in itertools.product([(4, 256), (8, 256), (8, 128)], [4, 16, 64]):
             out.append(dict(lanes_per_vertex=l, block=b, mode="queue", vertices_per_entry=passes * (b // l)))
@@ -22,6 +25,8 @@ def default_candidates(modes=("group", "queue")):
 
 def schedule_key(s):
     key = f"{s['mode']}:{s['lanes_per_vertex']}x{s['block']}"
+    if s["mode"] == "group" and s.get("stage_records", 0):
+        key += f":s{s['stage_records']}"
     if s["mode"] == "queue":
         key += f":e{s.get('vertices_per_entry', 0)}"
         if s.get("blocks_per_cu", 0):
@@ -30,8 +35,8 @@ def schedule_key(s):
 
 
 def parse_schedule_key(key):
-    """Inverse of schedule_key: "group:8x128", "queue:8x128:e64", "queue:8x128:e64:g2"; a bare
-    "8x128" means group launches."""
+    """Inverse of schedule_key: "group:8x128", "group:8x128:s64" (LDS staging, 64 records per
+    chunk), "queue:8x128:e64", "queue:8x128:e64:g2"; a bare "8x128" means group launches."""
     parts = key.split(":")
     if len(parts) == 1:
         parts = ["group"] + parts
@@ -45,6 +50,8 @@ def parse_schedule_key(key):
             s["vertices_per_entry"] = int(p[1:])
         elif p[0] == "g":
             s["blocks_per_cu"] = int(p[1:])
+        elif p[0] == "s":
+            s["stage_records"] = int(p[1:])
         else:
             raise ValueError(f"bad schedule {key!r}")
     return s
@@ -52,7 +59,8 @@ def parse_schedule_key(key):
 
 def apply_schedule(batch, s, use_graph=True):
     batch.set_schedule(s["lanes_per_vertex"], s["block"], mode=s["mode"],
-                       vertices_per_entry=s.get("vertices_per_entry", 0), blocks_per_cu=s.get("blocks_per_cu", 0))
+                       vertices_per_entry=s.get("vertices_per_entry", 0), blocks_per_cu=s.get("blocks_per_cu", 0),
+                       stage_records=s.get("stage_records", 0))
     batch.record_graph(use_graph)
 
 

@@ -9,8 +9,9 @@
 
 namespace gvbd {
 
-// One BVH node: box lo/hi; internal nodes keep their children in lo.w / hi.w (int bits),
-// leaves their primitive id in lo.w. Internal nodes occupy [0, n-1), leaves [n-1, 2n-1).
+// One BVH node: box lo/hi; lo.w (int bits) holds an internal node's left child or a leaf's
+// primitive id, hi.w the node's skip link (next node in depth-first order after its subtree,
+// -1 at the end). Internal nodes occupy [0, n-1), leaves [n-1, 2n-1).
 struct alignas(16) BvhNode {
     float4 lo;
     float4 hi;
@@ -37,6 +38,7 @@ struct DeviceBvh {
     int n = 0;                       // primitives
     BvhNode* nodes = nullptr;        // 2n - 1
     int* parent = nullptr;           // 2n - 1
+    int* right = nullptr;            // n - 1: right child of each internal node (refit)
     int* levelNodes = nullptr;       // internal nodes ordered by depth, deepest first
     std::vector<int> levelBegin;     // host: ranges of levelNodes per depth (deepest first)
     uint64_t* keys = nullptr;        // n (sorted Morton keys)

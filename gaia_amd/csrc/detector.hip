@@ -37,7 +37,9 @@ namespace gvbd {
 namespace {
 
 constexpr int kMaxCap = 16;       // largest per-query contact capacity
-constexpr int kStack = 64;        // traversal stack (tree depth <= 62 for 62-bit unique keys)
+// Traversal is stackless: every node's hi.w holds its skip link (the next node in depth-first
+// order after its subtree, -1 at the end), so a query visits left children first and follows
+// skip links; no per-thread stack lives in scratch memory.
 constexpr int kMaxDepth = 128;    // depth histogram size
 constexpr float kCmpEpsilon = 0.00001f;
 constexpr float kCmpEpsilon2 = kCmpEpsilon * kCmpEpsilon;
@@ -268,23 +270,24 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
         pairBase = S.vfOffset[v];
         keyBase = pairBase;  // the key segment has the pair segment's size (an upper bound)
     }
-    int stack[kStack];
-    int sp = 0;
-    if (S.fN > 0) stack[sp++] = 0;
     const int firstLeaf = S.fN - 1;
-    while (sp > 0) {
-        const int node = stack[--sp];
+    int node = S.fN > 0 ? 0 : -1;
+    for (int visits = 0; node >= 0; ++visits) {
+        if (visits > 2 * S.fN) {  // a depth-first walk visits every node at most once
+            atomicOr(S.error, 1);
+            break;
+        }
         const BvhNode nd = S.fNodes[node];
-        if (!box_overlap(qlo, qhi, nd)) continue;
-        if (node < firstLeaf) {
-            if (sp + 2 > kStack) {
-                atomicOr(S.error, 1);
-                break;
-            }
-            stack[sp++] = __float_as_int(nd.hi.w);
-            stack[sp++] = __float_as_int(nd.lo.w);
+        const int skip = __float_as_int(nd.hi.w);
+        if (!box_overlap(qlo, qhi, nd)) {
+            node = skip;
             continue;
         }
+        if (node < firstLeaf) {
+            node = __float_as_int(nd.lo.w);
+            continue;
+        }
+        node = skip;
         const int f = __float_as_int(nd.lo.w);
         const int4 fv = S.faces[f];
         if (fv.w == mesh && (fv.x == v || fv.y == v || fv.z == v)) continue;
@@ -402,24 +405,25 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     float cMu1[kMaxCap], cMu2[kMaxCap], cD[kMaxCap];
     const int64_t base = RECORD ? S.eeOffset[e1] : 0;
     const int countCap = RECORD ? S.eeCount[e1] : 0;
-    int stack[kStack];
-    int sp = 0;
-    if (S.eN > 0) stack[sp++] = 0;
     const int firstLeaf = S.eN - 1;
     const V3 ev1 = p2 - p1;
-    while (sp > 0) {
-        const int node = stack[--sp];
+    int node = S.eN > 0 ? 0 : -1;
+    for (int visits = 0; node >= 0; ++visits) {
+        if (visits > 2 * S.eN) {
+            atomicOr(S.error, 1);
+            break;
+        }
         const BvhNode nd = S.eNodes[node];
-        if (!box_overlap(qlo, qhi, nd)) continue;
-        if (node < firstLeaf) {
-            if (sp + 2 > kStack) {
-                atomicOr(S.error, 1);
-                break;
-            }
-            stack[sp++] = __float_as_int(nd.hi.w);
-            stack[sp++] = __float_as_int(nd.lo.w);
+        const int skip = __float_as_int(nd.hi.w);
+        if (!box_overlap(qlo, qhi, nd)) {
+            node = skip;
             continue;
         }
+        if (node < firstLeaf) {
+            node = __float_as_int(nd.lo.w);
+            continue;
+        }
+        node = skip;
         const int e2 = __float_as_int(nd.lo.w);
         const int4 b = S.edges[e2];
         if (S.eMesh[e2] == m1) {
@@ -572,12 +576,12 @@ __device__ __forceinline__ int common_prefix(const uint64_t* keys, int n, int i,
 
 // Binary radix tree over sorted unique keys (Karras 2012): internal node i covers a key range
 // starting or ending at i; its split is the highest differing bit in the range.
-__global__ __launch_bounds__(256) void build_tree_kernel(const uint64_t* keys, int n, BvhNode* nodes, int* parent) {
+__global__ __launch_bounds__(256) void build_tree_kernel(const uint64_t* keys, int n, BvhNode* nodes, int* parent,
+                                                         int* rightChild) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {  // leaves
         BvhNode& L = nodes[n - 1 + i];
         L.lo.w = __int_as_float((int)(keys[i] & 0xFFFFFFFFu));
-        L.hi.w = __int_as_float(-1);
     }
     if (i == 0) parent[0] = -1;
     if (i >= n - 1) return;
@@ -600,9 +604,32 @@ __global__ __launch_bounds__(256) void build_tree_kernel(const uint64_t* keys, i
     const int left = lo == gamma ? n - 1 + gamma : gamma;
     const int right = hi == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
     nodes[i].lo.w = __int_as_float(left);
-    nodes[i].hi.w = __int_as_float(right);
+    rightChild[i] = right;
     parent[left] = i;
     parent[right] = i;
+}
+
+// Skip link of every node (written to hi.w): the right sibling of the nearest node on the path
+// to the root (itself included) that is a left child; -1 past the last subtree.
+__global__ __launch_bounds__(256) void skip_kernel(BvhNode* nodes, const int* parent, const int* rightChild,
+                                                   int total, int* error) {
+    const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c0 >= total) return;
+    int c = c0, skip = -1;
+    for (int steps = 0;; ++steps) {
+        const int p = parent[c];
+        if (p < 0) break;
+        if (__float_as_int(nodes[p].lo.w) == c) {
+            skip = rightChild[p];
+            break;
+        }
+        if (steps >= 4 * kMaxDepth) {
+            atomicOr(error, 2);
+            break;
+        }
+        c = p;
+    }
+    nodes[c0].hi.w = __int_as_float(skip);
 }
 
 __global__ __launch_bounds__(256) void depth_kernel(const int* parent, int nInternal, int* depth, uint64_t* keys,
@@ -648,13 +675,13 @@ __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNo
 }
 
 __global__ __launch_bounds__(256) void internal_box_kernel(const int* levelNodes, int begin, int end,
-                                                           BvhNode* nodes) {
+                                                           BvhNode* nodes, const int* rightChild) {
     const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= end) return;
     const int i = levelNodes[k];
     BvhNode& N = nodes[i];
     const BvhNode A = nodes[__float_as_int(N.lo.w)];
-    const BvhNode B = nodes[__float_as_int(N.hi.w)];
+    const BvhNode B = nodes[rightChild[i]];
     N.lo = make_float4(fminf(A.lo.x, B.lo.x), fminf(A.lo.y, B.lo.y), fminf(A.lo.z, B.lo.z), N.lo.w);
     N.hi = make_float4(fmaxf(A.hi.x, B.hi.x), fmaxf(A.hi.y, B.hi.y), fmaxf(A.hi.z, B.hi.z), N.hi.w);
 }
@@ -946,6 +973,7 @@ void ContactDetector::finalize() {
         if (B.n == 0) continue;
         B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
         B.parent = dalloc<int>(2 * (size_t)B.n - 1);
+        B.right = dalloc<int>(std::max(B.n - 1, 1));
         B.keys = dalloc<uint64_t>(B.n);
         B.keysAlt = dalloc<uint64_t>(B.n);
         B.depthKeys = dalloc<uint64_t>(std::max(B.n - 1, 1));
@@ -988,7 +1016,10 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
     hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, box, B.keys);
     size_t tb = B.sortTempBytes;
     hip_check(rocprim::radix_sort_keys(B.sortTemp, tb, B.keys, B.keysAlt, (size_t)n, 0, 62, stream_), "morton sort");
-    hipLaunchKernelGGL(build_tree_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, B.keysAlt, n, B.nodes, B.parent);
+    hipLaunchKernelGGL(build_tree_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, B.keysAlt, n, B.nodes, B.parent,
+                       B.right);
+    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(2 * (int64_t)n - 1)), dim3(256), 0, stream_, B.nodes, B.parent,
+                       B.right, 2 * n - 1, error_);
     hip_check(hipGetLastError(), "bvh build launch");
     // refit schedule: internal nodes by depth, deepest first
     B.levelBegin.clear();
@@ -1026,7 +1057,7 @@ void ContactDetector::refitBvh(DeviceBvh& B, int kind) {
     for (size_t l = 0; l + 1 < B.levelBegin.size(); ++l) {
         const int b = B.levelBegin[l], e = B.levelBegin[l + 1];
         hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_, B.levelNodes, b, e,
-                           B.nodes);
+                           B.nodes, B.right);
     }
     hip_check(hipGetLastError(), "bvh refit launch");
 }
@@ -1178,7 +1209,7 @@ template <class T> std::vector<T> download(const T* d, size_t n, hipStream_t s) 
 DetectorResultCounts ContactDetector::counts() {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     const auto err = download(error_, 1, stream_);
-    if (err[0] & 1) fail(GVBD_ERR_INTERNAL, "BVH traversal stack overflow");
+    if (err[0] & 1) fail(GVBD_ERR_INTERNAL, "BVH traversal visited more nodes than the tree holds");
     if (err[0] & 2) fail(GVBD_ERR_INTERNAL, "BVH deeper than the refit schedule supports");
     if (err[0] & 4) fail(GVBD_ERR_INTERNAL, "record pass found more contacts than the counting pass");
     const auto pc = download(vfPairCount_, nv_, stream_);

@@ -8,12 +8,14 @@ namespace gvbd {
 
 // Lanes per vertex (1, 2, 4, 8, 16) and block size (64, 128, 256) of the group solve.
 bool valid_schedule(int lpv, int block);
+// whether an LDS-staged solve variant exists for (lpv, block, records per chunk); 0 is always valid
+bool valid_stage(int lpv, int block, int stage);
 
 // One parallel group: item i covers `count` consecutive colour-major slots of one scene,
 // tet or cloth by item kind. Tet friction applies when iteration >= frictionStartIter
 // (VBDPhysics.cpp:1610).
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, hipStream_t stream);
+                              int block, int stage, hipStream_t stream);
 // Device arguments of the persistent work-queue solve of a plan.
 struct QueueArgs {
     const SceneView* scenes;

@@ -92,10 +92,39 @@ __device__ __forceinline__ f3 sym_mul(const Sym3& h, f3 v) {
 }
 
 // ---------------------------------------------------------------------------
+// Incidence records: read straight from HBM by the lanes that use them (STAGE = 0), or staged
+// through LDS in chunks of whole-block coalesced loads (STAGE records of 64 B per chunk). A
+// lane handles the same records in the same order either way, so the results are identical.
+// ---------------------------------------------------------------------------
+
+// Calls body(r) for the records k = first, first + LPV, ... < end of one lane, where r points
+// at record k's `Q` float4s (in HBM, or in the LDS copy of the chunk that holds it). The block's
+// records are [blockBegin, blockEnd); every thread of the block must call this (barriers).
+template <int LPV, int STAGE, int Q, class Body>
+__device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs, int blockBegin, int blockEnd,
+                                                 int first, int end, float4* lds, Body&& body) {
+    if constexpr (STAGE == 0) {
+        for (int k = first; k < end; k += LPV) body(recs + (int64_t)Q * k);
+    } else {
+        for (int c0 = blockBegin; c0 < blockEnd; c0 += STAGE) {
+            const int n = min(STAGE, blockEnd - c0);
+            const float4* src = recs + (int64_t)Q * c0;
+            for (int i = threadIdx.x; i < Q * n; i += blockDim.x) lds[i] = src[i];
+            __syncthreads();
+            int k = first;
+            if (k < c0) k += ((c0 - k + LPV - 1) / LPV) * LPV;
+            const int hi = min(end, c0 + n);
+            for (; k < hi; k += LPV) body(lds + Q * (k - c0));
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Tet vertex block (VBDPhysics.cpp:2402-2483)
 // ---------------------------------------------------------------------------
-template <int LPV>
-__device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it, int iteration) {
+template <int LPV, int STAGE>
+__device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it, int iteration, float4* lds) {
     const int local = threadIdx.x / LPV;
     const int sub = threadIdx.x % LPV;
     const bool active = local < it.count;
@@ -112,9 +141,13 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
     float cmu = 0.f;
 
     const float4* __restrict__ pos = S.pos;
-    const TetInc* __restrict__ tinc = S.tinc;
-    for (int k = slot.y + sub; k < slot.z; k += LPV) {
-        const float4* r = reinterpret_cast<const float4*>(tinc + k);
+    int blockBegin = 0, blockEnd = 0;
+    if (STAGE > 0) {
+        blockBegin = S.slots[it.slotBegin].y;
+        blockEnd = S.slots[it.slotBegin + it.count - 1].z;
+    }
+    for_lane_records<LPV, STAGE, 4>(reinterpret_cast<const float4*>(S.tinc), blockBegin, blockEnd, slot.y + sub,
+                                    slot.z, lds, [&](const float4* r) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const float4 r2 = r[2];
@@ -153,7 +186,7 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         gyz += ag.y * g.z;
         gzz += ag.z * g.z;
         cmu += Amu * dot(mv, mv);
-    }
+    });
     if (LPV > 1) {
         f0 = lane_sum<LPV>(f0);
         f1 = lane_sum<LPV>(f1);
@@ -448,8 +481,8 @@ __device__ void cloth_contacts(const SceneView& S, int v, int4 sa, int4 sb, f3& 
 // ---------------------------------------------------------------------------
 // Cloth vertex block (VBDClothPhysics.cpp:1026-1154)
 // ---------------------------------------------------------------------------
-template <int LPV>
-__device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& it) {
+template <int LPV, int STAGE>
+__device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& it, float4* lds) {
     const int local = threadIdx.x / LPV;
     const int sub = threadIdx.x % LPV;
     const bool active = local < it.count;
@@ -465,11 +498,17 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     if (active) x = xyz(S.pos[v]);
 
     const float4* __restrict__ pos = S.pos;
+    int4 first = make_int4(0, 0, 0, 0), last = make_int4(0, 0, 0, 0);
+    if (STAGE > 0) {
+        const ClothSlot c0 = S.cslots[it.slotBegin], c1 = S.cslots[it.slotBegin + it.count - 1];
+        first = make_int4(c0.a.y, c0.b.x, 0, 0);
+        last = make_int4(c1.a.z, c1.b.y, 0, 0);
+    }
     // membrane force and Hessian (kept apart for the StVK damping term)
     float fm0 = 0.f, fm1 = 0.f, fm2 = 0.f;
     float hxx = 0.f, hxy = 0.f, hxz = 0.f, hyy = 0.f, hyz = 0.f, hzz = 0.f;
-    for (int k = sa.y + sub; k < sa.z; k += LPV) {
-        const float4* r = reinterpret_cast<const float4*>(S.cface + k);
+    for_lane_records<LPV, 2 * STAGE, 2>(reinterpret_cast<const float4*>(S.cface), first.x, last.x, sa.y + sub, sa.z,
+                                        lds, [&](const float4* r) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const int n0 = __float_as_int(r0.x), n1 = __float_as_int(r0.y);
@@ -498,12 +537,12 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
         hyy += mSm + cm * (c0.y * c0.y + c1.y * c1.y) + cf * Fm.y * Fm.y;
         hyz += cm * (c0.y * c0.z + c1.y * c1.z) + cf * Fm.y * Fm.z;
         hzz += mSm + cm * (c0.z * c0.z + c1.z * c1.z) + cf * Fm.z * Fm.z;
-    }
+    });
     // bending: f -= sum_j kQ(o,j) X_j, H += kQ(o,o) I, skipping degenerate stencils
     float fb0 = 0.f, fb1 = 0.f, fb2 = 0.f, hb = 0.f;
     const float thr2 = S.sp.degenerateThresSq;
-    for (int k = sb.x + sub; k < sb.y; k += LPV) {
-        const float4* r = reinterpret_cast<const float4*>(S.cbend + k);
+    for_lane_records<LPV, 2 * STAGE, 2>(reinterpret_cast<const float4*>(S.cbend), first.y, last.y, sb.x + sub, sb.y,
+                                        lds, [&](const float4* r) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const int o = __float_as_int(r0.w);
@@ -517,13 +556,13 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
         const f3 X3 = o == 3 ? x : y2;
         const f3 n1 = cross(X1 - X0, X2 - X0);
         const f3 n2 = cross(X2 - X3, X1 - X3);
-        if (dot(n1, n1) < thr2 || dot(n2, n2) < thr2) continue;
+        if (dot(n1, n1) < thr2 || dot(n2, n2) < thr2) return;
         const f3 q = r1.x * y0 + r1.y * y1 + r1.z * y2 + r1.w * x;
         fb0 -= q.x;
         fb1 -= q.y;
         fb2 -= q.z;
         hb += r1.w;
-    }
+    });
     if (LPV > 1) {
         fm0 = lane_sum<LPV>(fm0);
         fm1 = lane_sum<LPV>(fm1);
@@ -594,15 +633,18 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
 
 // One launch = one parallel group of every scene referenced by `items`; the block's
 // work kind is uniform, so the branch does not diverge.
-template <int LPV, int BLOCK>
+// STAGE > 0 stages the incidence records through LDS, STAGE 64-byte records (or 2 * STAGE cloth
+// records of 32 bytes) per chunk.
+template <int LPV, int BLOCK, int STAGE>
 __global__ __launch_bounds__(BLOCK) void group_solve_kernel(const SceneView* __restrict__ scenes,
                                                            const WorkItem* __restrict__ items, int iteration) {
+    __shared__ float4 lds[STAGE > 0 ? 4 * STAGE : 1];
     const WorkItem it = items[blockIdx.x];
     const SceneView& S = scenes[it.scene];
     if (it.kind == WORK_TET)
-        tet_block<LPV>(S, it, iteration);
+        tet_block<LPV, STAGE>(S, it, iteration, lds);
     else
-        cloth_block<LPV>(S, it);
+        cloth_block<LPV, STAGE>(S, it, lds);
 }
 
 // After a group's cloth solves: x = positionsNext for the group's cloth vertices
@@ -688,9 +730,9 @@ __global__ __launch_bounds__(BLOCK) void queue_solve_kernel(const SceneView* __r
             for (int p = 0; p < e.count; p += VPB) {
                 const WorkItem it{e.scene, e.slotBegin + p, min(VPB, e.count - p), e.kind};
                 if (e.kind == WORK_TET)
-                    tet_block<LPV>(S, it, iteration);
+                    tet_block<LPV, 0>(S, it, iteration, nullptr);
                 else
-                    cloth_block<LPV>(S, it);
+                    cloth_block<LPV, 0>(S, it, nullptr);
             }
         }
         // every thread's stores happen before lane 0's agent-scope release (the completion count
@@ -792,26 +834,44 @@ __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* _
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int LPV, int BLOCK>
+template <int LPV, int BLOCK, int STAGE>
 static hipError_t launch_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
                                hipStream_t stream) {
-    hipLaunchKernelGGL((group_solve_kernel<LPV, BLOCK>), dim3(nitems), dim3(BLOCK), 0, stream, scenes, items,
+    hipLaunchKernelGGL((group_solve_kernel<LPV, BLOCK, STAGE>), dim3(nitems), dim3(BLOCK), 0, stream, scenes, items,
                        iteration);
     return hipGetLastError();
 }
 
+// LDS-staged variants: the lane/block shapes the autotuner finds competitive, two chunk sizes.
+#define GVBD_STAGED(X) X(4, 128) X(4, 256) X(8, 128) X(8, 256) X(16, 128) X(16, 256)
+
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, hipStream_t stream) {
+                              int block, int stage, hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
 #define GVBD_CASE(L, B) \
-    if (lpv == L && block == B) return launch_solve<L, B>(scenes, items, nitems, iteration, stream);
+    if (stage == 0 && lpv == L && block == B) return launch_solve<L, B, 0>(scenes, items, nitems, iteration, stream);
     GVBD_CASE(1, 64) GVBD_CASE(1, 128) GVBD_CASE(1, 256)
     GVBD_CASE(2, 64) GVBD_CASE(2, 128) GVBD_CASE(2, 256)
     GVBD_CASE(4, 64) GVBD_CASE(4, 128) GVBD_CASE(4, 256)
     GVBD_CASE(8, 64) GVBD_CASE(8, 128) GVBD_CASE(8, 256)
     GVBD_CASE(16, 64) GVBD_CASE(16, 128) GVBD_CASE(16, 256)
 #undef GVBD_CASE
+#define GVBD_CASE(L, B)                                                                                  \
+    if (lpv == L && block == B && stage == 32) return launch_solve<L, B, 32>(scenes, items, nitems, iteration, stream); \
+    if (lpv == L && block == B && stage == 64) return launch_solve<L, B, 64>(scenes, items, nitems, iteration, stream);
+    GVBD_STAGED(GVBD_CASE)
+#undef GVBD_CASE
     return hipErrorInvalidValue;
+}
+
+bool valid_stage(int lpv, int block, int stage) {
+    if (stage == 0) return true;
+    if (stage != 32 && stage != 64) return false;
+    bool ok = false;
+#define GVBD_CASE(L, B) ok = ok || (lpv == L && block == B);
+    GVBD_STAGED(GVBD_CASE)
+#undef GVBD_CASE
+    return ok;
 }
 
 template <int LPV, int BLOCK>

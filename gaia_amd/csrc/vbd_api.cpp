@@ -85,6 +85,7 @@ constexpr int kDefaultSpinLimit = 1 << 20;
 struct PlanObj {
     std::vector<int32_t> scenes;
     int lpv = 4, block = 256;
+    int stage = 0;  // incidence records per LDS staging chunk (0: read straight from HBM)
     int mode = GVBD_MODE_GROUP_LAUNCHES;
     int vertsPerEntry = 0;  // work-queue entry size (0: 4 passes of block / lpv vertices)
     int blocksPerCu = 0;    // persistent grid per CU (0: resident maximum)
@@ -688,7 +689,7 @@ void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
 
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
     const int b = p.groupItemBegin[g], e = p.groupItemBegin[g + 1];
-    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, ctx->stream),
+    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, p.stage, ctx->stream),
               "group solve launch");
     const int cb = p.groupCommitBegin[g], ce = p.groupCommitBegin[g + 1];
     if (ce > cb) hip_check(launch_cloth_commit(p.dViews, p.dCommit + cb, ce - cb, ctx->stream), "commit launch");
@@ -1275,7 +1276,12 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
             "vertices_per_entry must be a multiple of block / lanes_per_vertex (at most 65536)");
     require(sc.blocks_per_cu >= 0 && sc.blocks_per_cu <= 32, "blocks_per_cu must be in [0, 32]");
     require(sc.spin_limit >= 0, "spin_limit must be non-negative");
+    require(valid_stage(sc.lanes_per_vertex, sc.block, sc.stage_records),
+            "stage_records must be 0, or 32 or 64 with lanes_per_vertex in {4,8,16} and block in {128,256}");
+    require(sc.stage_records == 0 || sc.mode == GVBD_MODE_GROUP_LAUNCHES,
+            "LDS staging applies to group launches only");
     p.lpv = sc.lanes_per_vertex;
+    p.stage = sc.stage_records;
     p.block = sc.block;
     p.mode = sc.mode;
     p.vertsPerEntry = sc.vertices_per_entry;
@@ -1289,6 +1295,7 @@ static void get_schedule(const PlanObj& p, gvbd_schedule* sc) {
     *sc = gvbd_schedule{};
     sc->lanes_per_vertex = p.lpv;
     sc->block = p.block;
+    sc->stage_records = p.stage;
     sc->mode = p.mode;
     sc->vertices_per_entry = p.vertsPerEntry;
     sc->blocks_per_cu = p.blocksPerCu;
@@ -1553,7 +1560,7 @@ int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t gr
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
         const int b = p.groupItemBegin[group], e = p.groupItemBegin[group + 1];
-        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, ctx->stream),
+        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, p.stage, ctx->stream),
                   "group solve launch");
     });
 }

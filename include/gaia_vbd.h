@@ -158,7 +158,10 @@ typedef struct gvbd_schedule {
     int32_t vertices_per_entry; /* work-queue entry size, a multiple of block / lanes_per_vertex (0: 4x that) */
     int32_t blocks_per_cu;      /* persistent grid per compute unit (0: the resident maximum) */
     int32_t spin_limit;         /* polls before a dependency wait is reported as an error (0: default) */
-    int32_t reserved[2];
+    int32_t stage_records;      /* group launches: incidence records staged through LDS per chunk by the whole
+                                   block (32 or 64, with lanes_per_vertex 4/8/16 and block 128/256), or 0 to
+                                   read them straight from HBM; results are identical either way */
+    int32_t reserved;
 } gvbd_schedule;
 
 /* sizes of the ABI structs as compiled into the library (binding self-check) */
