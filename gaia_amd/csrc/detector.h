@@ -39,6 +39,7 @@ struct DeviceBvh {
     BvhNode* nodes = nullptr;        // 2n - 1
     int* parent = nullptr;           // 2n - 1
     int* right = nullptr;            // n - 1: right child of each internal node (refit)
+    int* jump = nullptr;             // 4 (n - 1): depth / ancestor buffers of the depth computation
     int* levelNodes = nullptr;       // internal nodes ordered by depth, deepest first
     std::vector<int> levelBegin;     // host: ranges of levelNodes per depth (deepest first)
     uint64_t* keys = nullptr;        // n (sorted Morton keys)
@@ -167,12 +168,17 @@ class ContactDetector {
     float* recEEd_ = nullptr;
     bool recorded_ = false;
     int* error_ = nullptr;      // bit 0: traversal stack overflow, bit 1: tree too deep
-    int* depthHist_ = nullptr;  // per-depth internal node counts (build)
+    int* depthHist_ = nullptr;  // per depth: first index of its run in the refit order (build)
+    unsigned int* boxPartial_ = nullptr;  // per-block centroid bounds (build)
     float4* external_ = nullptr;
     // face back-links and the solver-side state
     uint64_t *fvKeys_ = nullptr, *fvKeysAlt_ = nullptr;
     int2* fvRange_ = nullptr;
     void* fvSortTemp_ = nullptr;
+    uint64_t* vKeys_ = nullptr;     // vertex Morton keys (VF query order)
+    uint64_t* vKeysAlt_ = nullptr;  // sorted
+    void* vSortTemp_ = nullptr;
+    size_t vSortTempBytes_ = 0;
     size_t fvSortTempBytes_ = 0;
     float4* xAtCD_ = nullptr;
     int* flag_ = nullptr;

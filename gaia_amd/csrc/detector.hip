@@ -210,6 +210,8 @@ struct CDView {
     int2* recEE;
     float2* recEEmu;
     float* recEEd;
+    const uint64_t* eOrder;  // edge tree's sorted Morton keys: query edges in leaf order (or NULL)
+    const uint64_t* vOrder;  // vertices' sorted Morton keys: query vertices in that order (or NULL)
     int* error;
 };
 
@@ -252,8 +254,10 @@ __device__ bool edge_feasible(const CDView& S, V3 p, int e) {
 // VF query of one vertex (ClothContactDetector.cpp:144-288 per candidate face)
 template <bool RECORD>
 __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= S.nv) return;
+    // vertices in Morton order: a wavefront's queries are spatial neighbours
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.nv) return;
+    const int v = S.vOrder ? (int)(S.vOrder[t] & 0xFFFFFFFFu) : t;
     const V3 p = ld(S.pos, v);
     const int mesh = S.vMesh[v];
     const float r = S.r;
@@ -389,16 +393,21 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
 // EE query of one edge (ClothContactDetector.cpp:731-813 per candidate edge, :1106-1135 setup)
 template <bool RECORD>
 __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
-    const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e1 >= S.ne) return;
+    // neighbouring threads take edges that are neighbours in the tree's Morton order, so a
+    // wavefront's traversals share their paths (and cache lines)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.ne) return;
+    const int e1 = S.eOrder ? (int)(S.eOrder[t] & 0xFFFFFFFFu) : t;
     const int4 a = S.edges[e1];
     const int m1 = S.eMesh[e1];
     const V3 p1 = ld(S.pos, a.x), p2 = ld(S.pos, a.y);
     const float r = S.r;
-    const float len = dist(p1, p2);
-    const V3 mid = 0.5f * (p1 + p2);
-    const float rq = (r + 0.5f * len) * 1.0001f + 1e-7f;
-    const V3 qlo = {mid.x - rq, mid.y - rq, mid.z - rq}, qhi = {mid.x + rq, mid.y + rq, mid.z + rq};
+    // Every edge within r of this one overlaps the segment's box grown by r: a tighter region
+    // than the box of the reference's sphere (midpoint, r + len / 2), with the same contacts and
+    // minimum distances (both are decided by the exact tests below).
+    const float rq = r * 1.0001f + 1e-7f;
+    const V3 qlo = {fminf(p1.x, p2.x) - rq, fminf(p1.y, p2.y) - rq, fminf(p1.z, p2.z) - rq};
+    const V3 qhi = {fmaxf(p1.x, p2.x) + rq, fmaxf(p1.y, p2.y) + rq, fmaxf(p1.z, p2.z) + rq};
     float emin = r;
     int count = 0, stored = 0, overflow = 0;
     int cE[kMaxCap];
@@ -512,34 +521,59 @@ __device__ __forceinline__ V3 prim_centroid(const CDView& S, int kind, int i) {
         const V3 a = ld(S.pos, f.x), b = ld(S.pos, f.y), c = ld(S.pos, f.z);
         return (1.f / 3.f) * ((a + b) + c);
     }
-    const int4 e = S.edges[i];
-    return 0.5f * (ld(S.pos, e.x) + ld(S.pos, e.y));
+    if (kind == 1) {
+        const int4 e = S.edges[i];
+        return 0.5f * (ld(S.pos, e.x) + ld(S.pos, e.y));
+    }
+    return ld(S.pos, i);  // kind 2: vertices
 }
 
-__global__ __launch_bounds__(256) void box_reset_kernel(unsigned int* box) {
-    if (threadIdx.x < 3) box[threadIdx.x] = 0xFFFFFFFFu;
-    else if (threadIdx.x < 6) box[threadIdx.x] = 0u;
-}
-
-__global__ __launch_bounds__(256) void centroid_box_kernel(CDView S, int kind, int n, unsigned int* box) {
+// Centroid bounds in two passes without atomics: each 256-thread block writes its 6 ordered
+// bounds to partial[6 * block], then one block reduces the partials into box.
+__global__ __launch_bounds__(256) void centroid_box_kernel(CDView S, int kind, int n, unsigned int* partial) {
+    __shared__ unsigned int red[6][4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    unsigned int v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     if (i < n) {
         const V3 c = prim_centroid(S, kind, i);
         const float cc[3] = {c.x, c.y, c.z};
-        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = order_bits(cc[d]);
+        for (int d = 0; d < 3; ++d) v[d] = v[3 + d] = order_bits(cc[d]);
     }
-    for (int d = 0; d < 3; ++d) {
-        unsigned int a = lo[d], b = hi[d];
+    for (int d = 0; d < 6; ++d) {
+        unsigned int a = v[d];
         for (int o = 32; o > 0; o >>= 1) {
-            a = min(a, (unsigned int)__shfl_xor((int)a, o, 64));
-            b = max(b, (unsigned int)__shfl_xor((int)b, o, 64));
+            const unsigned int b = (unsigned int)__shfl_xor((int)a, o, 64);
+            a = d < 3 ? min(a, b) : max(a, b);
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(box + d, a);
-            atomicMax(box + 3 + d, b);
-        }
+        if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = a;
     }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        unsigned int a = red[d][0];
+        for (int w = 1; w < 4; ++w) a = d < 3 ? min(a, red[d][w]) : max(a, red[d][w]);
+        partial[6 * (int64_t)blockIdx.x + d] = a;
+    }
+}
+
+__global__ __launch_bounds__(256) void box_final_kernel(const unsigned int* partial, int blocks, unsigned int* box) {
+    __shared__ unsigned int red[6][256];
+    unsigned int v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+        for (int d = 0; d < 6; ++d) {
+            const unsigned int x = partial[6 * (int64_t)b + d];
+            v[d] = d < 3 ? min(v[d], x) : max(v[d], x);
+        }
+    for (int d = 0; d < 6; ++d) red[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int d = 0; d < 6; ++d)
+                red[d][threadIdx.x] = d < 3 ? min(red[d][threadIdx.x], red[d][threadIdx.x + s])
+                                            : max(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) box[threadIdx.x] = red[threadIdx.x][0];
 }
 
 __device__ __forceinline__ unsigned int expand_bits10(unsigned int v) {
@@ -632,25 +666,46 @@ __global__ __launch_bounds__(256) void skip_kernel(BvhNode* nodes, const int* pa
     nodes[c0].hi.w = __int_as_float(skip);
 }
 
-__global__ __launch_bounds__(256) void depth_kernel(const int* parent, int nInternal, int* depth, uint64_t* keys,
-                                                    int* hist, int* error) {
+// Depths of the internal nodes by pointer jumping: after round k every node knows its distance
+// to its 2^k-th ancestor (or to the root), so log2(kMaxDepth) rounds give every depth.
+__global__ __launch_bounds__(256) void depth_init_kernel(const int* parent, int nInternal, int* d, int* anc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nInternal) return;
-    int dpt = 0;
-    for (int p = parent[i]; p >= 0 && dpt < 4 * kMaxDepth; p = parent[p]) ++dpt;
-    if (dpt >= kMaxDepth) {
+    const int p = parent[i];
+    d[i] = p >= 0 ? 1 : 0;
+    anc[i] = p;
+}
+
+__global__ __launch_bounds__(256) void depth_jump_kernel(const int* dIn, const int* ancIn, int* dOut, int* ancOut,
+                                                         int nInternal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nInternal) return;
+    const int a = ancIn[i];
+    dOut[i] = a >= 0 ? dIn[i] + dIn[a] : dIn[i];
+    ancOut[i] = a >= 0 ? ancIn[a] : -1;
+}
+
+// sort keys, deepest first: ascending (kMaxDepth - depth, node)
+__global__ __launch_bounds__(256) void depth_keys_kernel(const int* d, const int* anc, int nInternal, int* depth,
+                                                         uint64_t* keys, int* error) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nInternal) return;
+    int dpt = d[i];
+    if (anc[i] >= 0 || dpt >= kMaxDepth) {
         atomicOr(error, 2);
         dpt = kMaxDepth - 1;
     }
     depth[i] = dpt;
-    // deepest first: ascending (kMaxDepth - depth, node)
     keys[i] = ((uint64_t)(kMaxDepth - dpt) << 32) | (uint32_t)i;
-    atomicAdd(hist + dpt, 1);
 }
 
-__global__ __launch_bounds__(256) void level_nodes_kernel(const uint64_t* keys, int n, int* out) {
+// node ids in refit order, and the first index of each depth's run (starts[depth], -1 if none)
+__global__ __launch_bounds__(256) void level_nodes_kernel(const uint64_t* keys, int n, int* out, int* starts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (int)(keys[i] & 0xFFFFFFFFu);
+    if (i >= n) return;
+    out[i] = (int)(keys[i] & 0xFFFFFFFFu);
+    const int dk = kMaxDepth - (int)(keys[i] >> 32);
+    if (i == 0 || (int)(keys[i - 1] >> 32) != (int)(keys[i] >> 32)) starts[dk] = i;
 }
 
 __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNode* nodes, int n) {
@@ -953,6 +1008,7 @@ void ContactDetector::finalize() {
     eeOffset_ = dalloc<int64_t>(ne_ + 1);
     error_ = dalloc<int>(64);
     depthHist_ = dalloc<int>(kMaxDepth);
+    boxPartial_ = dalloc<unsigned int>(6 * (size_t)blocks_for(std::max(nf_, ne_)));
     hip_check(hipMemsetAsync(error_, 0, 64 * sizeof(int), stream_), "detector clear");
     hip_check(hipMemsetAsync(vfPairCount_, 0, sizeof(int) * (nv_ + 1), stream_), "detector clear");
     hip_check(hipMemsetAsync(eeCount_, 0, sizeof(int) * (ne_ + 1), stream_), "detector clear");
@@ -966,6 +1022,14 @@ void ContactDetector::finalize() {
               "scan size");
     scanTempBytes_ = std::max(a, b);
     scanTemp_ = dalloc<char>(scanTempBytes_);
+    // vertex query order (sorted Morton keys, rebuilt with the face tree)
+    if (nv_ > 0) {
+        vKeys_ = dalloc<uint64_t>(nv_);
+        vKeysAlt_ = dalloc<uint64_t>(nv_);
+        hip_check(rocprim::radix_sort_keys(nullptr, vSortTempBytes_, vKeys_, vKeysAlt_, (size_t)nv_, 0, 62, stream_),
+                  "sort size");
+        vSortTemp_ = dalloc<char>(vSortTempBytes_);
+    }
     // BVH storage
     for (int kind = 0; kind < 2; ++kind) {
         DeviceBvh& B = kind == 0 ? fb_ : eb_;
@@ -974,6 +1038,7 @@ void ContactDetector::finalize() {
         B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
         B.parent = dalloc<int>(2 * (size_t)B.n - 1);
         B.right = dalloc<int>(std::max(B.n - 1, 1));
+        B.jump = dalloc<int>(4 * (size_t)std::max(B.n - 1, 1));
         B.keys = dalloc<uint64_t>(B.n);
         B.keysAlt = dalloc<uint64_t>(B.n);
         B.depthKeys = dalloc<uint64_t>(std::max(B.n - 1, 1));
@@ -1011,11 +1076,18 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
     S.edges = edges_;
     const int n = B.n;
     unsigned int* box = (unsigned int*)sceneBox_;
-    hipLaunchKernelGGL(box_reset_kernel, dim3(1), dim3(64), 0, stream_, box);
-    hipLaunchKernelGGL(centroid_box_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, box);
+    hipLaunchKernelGGL(centroid_box_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, boxPartial_);
+    hipLaunchKernelGGL(box_final_kernel, dim3(1), dim3(256), 0, stream_, boxPartial_, blocks_for(n), box);
     hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, box, B.keys);
     size_t tb = B.sortTempBytes;
     hip_check(rocprim::radix_sort_keys(B.sortTemp, tb, B.keys, B.keysAlt, (size_t)n, 0, 62, stream_), "morton sort");
+    if (kind == 0 && nv_ > 0) {
+        // VF query order: vertices sorted by Morton code in the face tree's box
+        hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(nv_)), dim3(256), 0, stream_, S, 2, nv_, box, vKeys_);
+        size_t tv = vSortTempBytes_;
+        hip_check(rocprim::radix_sort_keys(vSortTemp_, tv, vKeys_, vKeysAlt_, (size_t)nv_, 0, 62, stream_),
+                  "vertex order sort");
+    }
     hipLaunchKernelGGL(build_tree_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, B.keysAlt, n, B.nodes, B.parent,
                        B.right);
     hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(2 * (int64_t)n - 1)), dim3(256), 0, stream_, B.nodes, B.parent,
@@ -1025,25 +1097,32 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
     B.levelBegin.clear();
     const int ni = n - 1;
     if (ni > 0) {
-        std::vector<int> h(kMaxDepth, 0);
-        int* dh = depthHist_;
-        hip_check(hipMemsetAsync(dh, 0, kMaxDepth * sizeof(int), stream_), "hist clear");
-        hipLaunchKernelGGL(depth_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, B.parent, ni, B.depth,
-                           B.depthKeys, dh, error_);
+        int* d0 = B.jump;
+        int* d1 = B.jump + ni;
+        int* a0 = B.jump + 2 * (size_t)ni;
+        int* a1 = B.jump + 3 * (size_t)ni;
+        hipLaunchKernelGGL(depth_init_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, B.parent, ni, d0, a0);
+        for (int round = 0; (1 << round) < kMaxDepth; ++round) {
+            hipLaunchKernelGGL(depth_jump_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, d0, a0, d1, a1, ni);
+            std::swap(d0, d1);
+            std::swap(a0, a1);
+        }
+        hipLaunchKernelGGL(depth_keys_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, d0, a0, ni, B.depth,
+                           B.depthKeys, error_);
         size_t tb2 = B.sortTempBytes;
         hip_check(rocprim::radix_sort_keys(B.sortTemp, tb2, B.depthKeys, B.depthKeysAlt, (size_t)ni, 0, 40, stream_),
                   "depth sort");
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, depthHist_, (int64_t)kMaxDepth, -1);
         hipLaunchKernelGGL(level_nodes_kernel, dim3(blocks_for(ni)), dim3(256), 0, stream_, B.depthKeysAlt, ni,
-                           B.levelNodes);
-        hip_check(hipMemcpyAsync(h.data(), dh, kMaxDepth * sizeof(int), hipMemcpyDeviceToHost, stream_), "hist read");
+                           B.levelNodes, depthHist_);
+        std::vector<int> starts(kMaxDepth, -1);
+        hip_check(hipMemcpyAsync(starts.data(), depthHist_, kMaxDepth * sizeof(int), hipMemcpyDeviceToHost, stream_),
+                  "level read");
         hip_check(hipStreamSynchronize(stream_), "bvh build sync");
-        int at = 0;
-        for (int d = kMaxDepth - 1; d >= 0; --d) {
-            if (h[d] == 0) continue;
-            B.levelBegin.push_back(at);
-            at += h[d];
-        }
-        B.levelBegin.push_back(at);
+        // runs are stored deepest first
+        for (int d = kMaxDepth - 1; d >= 0; --d)
+            if (starts[d] >= 0) B.levelBegin.push_back(starts[d]);
+        B.levelBegin.push_back(ni);
     }
     refitBvh(B, kind);
 }
@@ -1124,6 +1203,8 @@ void ContactDetector::query(bool record) {
     S.fN = fb_.n;
     S.eNodes = eb_.nodes;
     S.eN = eb_.n;
+    S.eOrder = eb_.n > 0 ? eb_.keysAlt : nullptr;
+    S.vOrder = fb_.n > 0 ? vKeysAlt_ : nullptr;  // computed with the face tree
     S.r = r_;
     S.vfCap = vfCap_;
     S.eeCap = eeCap_;
