@@ -445,8 +445,8 @@ def main():
     # workload and schedule, from the committed profile summary
     side = os.path.join(REPO, "profiles", f"{roof['kernel']}_batched_256_scenes.traffic.json")
     if os.path.exists(side) and args.scenes == 256 and args.seed == 0:
-        t = json.load(open(side))
-        if t.get("schedule") == key:
+        t = json.load(open(side)).get("schedules", {}).get(key)
+        if t:
             roof["traffic"] = t["hbm_bytes_per_launch"]
     copy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5)
     roof["measured_copy_gbs"] = round(2 * args.copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
