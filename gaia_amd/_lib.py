@@ -79,6 +79,8 @@ class TetMeshDesc(ctypes.Structure):
         ("fixed_points", c_int32_p),
         ("vertex_colors", c_int32_p),
         ("coloring_algorithm", ctypes.c_int32),
+        ("friction_dynamic", ctypes.c_float),
+        ("friction_eps_v", ctypes.c_float),
     ]
 
 

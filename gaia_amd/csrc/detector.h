@@ -79,7 +79,9 @@ class ContactDetector {
     ContactDetector(const ContactDetector&) = delete;
     ContactDetector& operator=(const ContactDetector&) = delete;
 
-    int addMesh(int nv, const float* pos, int nf, const int* faces);
+    // queryMask (optional, per vertex): 0 = the vertex takes no VF queries and gets an unbounded
+    // conservative bound (interior vertices of a tet mesh whose faces are its boundary surface)
+    int addMesh(int nv, const float* pos, int nf, const int* faces, const uint8_t* queryMask = nullptr);
     // Queries read (and refits follow) positions owned by the caller, e.g. a scene's x array in
     // the same vertex order; call before finalize.
     void useExternalPositions(float4* pos);
@@ -138,6 +140,9 @@ class ContactDetector {
     std::vector<int> meshV_, meshF_, meshE_;  // offsets per mesh (size M + 1 after finalize)
     std::vector<float> hostPos_;
     std::vector<int> hFaces_, hFaceEdges_, hEdges_, hVMesh_, hEMesh_;
+    std::vector<uint8_t> hQuery_;  // per vertex: takes VF queries
+    bool masked_ = false;          // some mesh came with a query mask
+    uint8_t* vQuery_ = nullptr;    // device copy of hQuery_ when masked_
     // device
     float4 *pos_ = nullptr, *rest_ = nullptr;
     int *vMesh_ = nullptr, *eMesh_ = nullptr;

@@ -23,6 +23,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -212,6 +213,7 @@ struct CDView {
     float* recEEd;
     const uint64_t* eOrder;  // edge tree's sorted Morton keys: query edges in leaf order (or NULL)
     const uint64_t* vOrder;  // vertices' sorted Morton keys: query vertices in that order (or NULL)
+    const uint8_t* vQuery;   // per vertex: takes VF queries (NULL: every vertex)
     int* error;
 };
 
@@ -258,6 +260,15 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= S.nv) return;
     const int v = S.vOrder ? (int)(S.vOrder[t] & 0xFFFFFFFFu) : t;
+    if (S.vQuery && !S.vQuery[v]) {  // not a surface vertex: no VF contacts
+        if (!RECORD) {
+            S.vfMin[v] = S.r;
+            S.vfPairCount[v] = 0;
+            S.vfStored[v] = 0;
+            S.vfOverflow[v] = 0;
+        }
+        return;
+    }
     const V3 p = ld(S.pos, v);
     const int mesh = S.vMesh[v];
     const float r = S.r;
@@ -493,10 +504,14 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
 // VBDClothPhysics.cpp:775-812 per vertex
 __global__ __launch_bounds__(256) void bounds_kernel(const float* vfMin, const int* faceMin, const float* eeMin,
                                                      const int* vfBegin, const int* vfFace, const int* veBegin,
-                                                     const int* veEdge, int nv, float r, float thickness,
-                                                     float relaxation, float* out) {
+                                                     const int* veEdge, const uint8_t* vQuery, int nv, float r,
+                                                     float thickness, float relaxation, float* out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
+    if (vQuery && !vQuery[v]) {  // interior vertices of tet meshes: unbounded
+        out[v] = FLT_MAX;
+        return;
+    }
     float dMin = r;
     dMin = fminf(dMin, vfMin[v]);
     for (int k = vfBegin[v]; k < vfBegin[v + 1]; ++k) dMin = fminf(dMin, __int_as_float(faceMin[vfFace[k]]));
@@ -869,7 +884,7 @@ void ContactDetector::freeAll() {
     bytes_ = 0;
 }
 
-int ContactDetector::addMesh(int nv, const float* pos, int nf, const int* faces) {
+int ContactDetector::addMesh(int nv, const float* pos, int nf, const int* faces, const uint8_t* queryMask) {
     if (ready_) fail(GVBD_ERR_STATE, "cannot add meshes after finalize");
     require(nv > 0 && nf >= 0 && pos != nullptr && (nf == 0 || faces != nullptr), "bad mesh arguments");
     std::vector<TriEdge> edges;
@@ -880,6 +895,8 @@ int ContactDetector::addMesh(int nv, const float* pos, int nf, const int* faces)
     const int vo = nv_, fo = nf_, eo = ne_;
     hostPos_.insert(hostPos_.end(), pos, pos + 3 * (size_t)nv);
     for (int i = 0; i < nv; ++i) hVMesh_.push_back(m);
+    if (queryMask) masked_ = true;
+    for (int i = 0; i < nv; ++i) hQuery_.push_back(queryMask ? (uint8_t)(queryMask[i] != 0) : (uint8_t)1);
     for (int f = 0; f < nf; ++f) {
         for (int k = 0; k < 3; ++k) hFaces_.push_back(vo + faces[3 * (size_t)f + k]);
         hFaces_.push_back(m);
@@ -969,6 +986,10 @@ void ContactDetector::finalize() {
     hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
     vMesh_ = dalloc<int>(nv_);
     up(vMesh_, hVMesh_.data(), sizeof(int) * nv_);
+    if (masked_) {
+        vQuery_ = dalloc<uint8_t>(nv_);
+        up(vQuery_, hQuery_.data(), nv_);
+    }
     faces_ = dalloc<int4>(nf_);
     up(faces_, hFaces_.data(), sizeof(int4) * nf_);
     faceEdges_ = dalloc<int4>(nf_);
@@ -1193,6 +1214,7 @@ void ContactDetector::query(bool record) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.vMesh = vMesh_;
+    S.vQuery = vQuery_;
     S.faces = faces_;
     S.faceEdges = faceEdges_;
     S.edges = edges_;
@@ -1274,7 +1296,7 @@ void ContactDetector::query(bool record) {
 void ContactDetector::bounds(float thickness, float relaxation) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nv_)), dim3(256), 0, stream_, vfMin_, faceMin_, eeMin_,
-                       vfBegin_, vfFace_, veBegin_, veEdge_, nv_, r_, thickness, relaxation, bounds_);
+                       vfBegin_, vfFace_, veBegin_, veEdge_, vQuery_, nv_, r_, thickness, relaxation, bounds_);
     hip_check(hipGetLastError(), "bounds launch");
 }
 

@@ -35,6 +35,7 @@ static_assert(sizeof(TetInc) == 64, "TetInc must be 64 bytes");
 enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
     SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)
+    SLOT_JACOBI = 4,   // tet (scenes with collision handling): write positionsNext, committed per group
 };
 
 // One incident triangle of a cloth vertex v, relative to v:
@@ -116,6 +117,8 @@ struct ContactView {
     const uint64_t* fvKeys;   // sorted (face << 32 | vertex)
     const int* faceRef;       // per FaceInc record: face id | order << 30
     const int* edgeRef;       // per cloth-slot edge record: edge id | order << 30
+    const int* tFaceRef;      // tet slots: incident boundary faces, face id | order << 30
+    const int* tEdgeRef;      // tet slots: incident boundary edges, edge id | order << 30
     const float* bound;       // per vertex conservative bound
     const float4* xAtCD;      // positions at the last detection
     int* flag;                // set when a vertex is truncated: detect before the next iteration
@@ -141,6 +144,7 @@ struct SceneView {
     SolveParams sp;
     ContactView cv;
     const float* initBound;  // per vertex bound of initialisation type 3 (nullptr: unbounded)
+    const int4* tcontact;    // contact scenes, per tet slot: [tFaceRef range), [tEdgeRef range)
 };
 
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`

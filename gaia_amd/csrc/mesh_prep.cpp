@@ -360,6 +360,64 @@ bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<T
     return true;
 }
 
+bool build_tet_surface(int32_t nv, int32_t nt, const int32_t* tets, const float* pos, std::vector<int32_t>& faces,
+                       std::string& err) {
+    static const int kOpp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    // (sorted vertex triple, 4 t + k) for every tet face; a triple seen once is on the boundary
+    struct Key {
+        int32_t a, b, c;
+        int64_t at;
+    };
+    std::vector<Key> keys;
+    keys.reserve(4 * (size_t)nt);
+    for (int32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+            int32_t v[3] = {tets[4 * (size_t)t + kOpp[k][0]], tets[4 * (size_t)t + kOpp[k][1]],
+                            tets[4 * (size_t)t + kOpp[k][2]]};
+            for (int32_t x : v)
+                if (x < 0 || x >= nv) {
+                    err = "tet mesh: vertex index out of range";
+                    return false;
+                }
+            std::sort(v, v + 3);
+            keys.push_back({v[0], v[1], v[2], 4 * (int64_t)t + k});
+        }
+    std::sort(keys.begin(), keys.end(), [](const Key& p, const Key& q) {
+        if (p.a != q.a) return p.a < q.a;
+        if (p.b != q.b) return p.b < q.b;
+        if (p.c != q.c) return p.c < q.c;
+        return p.at < q.at;
+    });
+    std::vector<int64_t> boundary;
+    for (size_t i = 0; i < keys.size();) {
+        size_t j = i + 1;
+        while (j < keys.size() && keys[j].a == keys[i].a && keys[j].b == keys[i].b && keys[j].c == keys[i].c) ++j;
+        if (j - i == 1) boundary.push_back(keys[i].at);
+        i = j;
+    }
+    std::sort(boundary.begin(), boundary.end());
+    faces.clear();
+    faces.reserve(3 * boundary.size());
+    for (int64_t at : boundary) {
+        const int64_t t = at / 4;
+        const int k = (int)(at % 4);
+        int32_t a = tets[4 * t + kOpp[k][0]], b = tets[4 * t + kOpp[k][1]], c = tets[4 * t + kOpp[k][2]];
+        const int32_t o = tets[4 * t + k];
+        float e1[3], e2[3], w[3], n[3];
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = pos[3 * (size_t)b + d] - pos[3 * (size_t)a + d];
+            e2[d] = pos[3 * (size_t)c + d] - pos[3 * (size_t)a + d];
+            w[d] = pos[3 * (size_t)o + d] - pos[3 * (size_t)a + d];
+        }
+        cross3f(e1, e2, n);
+        if (dot3f(n, w) > 0.f) std::swap(b, c);
+        faces.push_back(a);
+        faces.push_back(b);
+        faces.push_back(c);
+    }
+    return true;
+}
+
 bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err) {
     if ((int64_t)m.pos.size() != 3ll * m.nv || (int64_t)m.faces.size() != 3ll * m.nf) {
         err = "cloth mesh: array sizes do not match vertex/face counts";

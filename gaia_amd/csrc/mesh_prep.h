@@ -20,6 +20,8 @@ struct TetMaterial {
     float constantVelDamping = 0.f;
     int32_t initializationType = 0;
     float initRatioG = 0.5f;
+    float frictionDynamic = 0.1f;  // contact friction of the boundary surface (scenes with collision handling)
+    float frictionEpsV = 0.1f;
 };
 
 struct TetMeshHost {
@@ -122,6 +124,12 @@ bool prepare_cloth_mesh(ClothMeshHost& m, std::string& err);
 // Fails on out-of-range indices and non-manifold or inconsistently oriented meshes.
 bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<TriEdge>& edges,
                      std::vector<int32_t>* faceEdges, std::string& err);
+
+// Boundary surface of a tet mesh: the faces of exactly one tet, in (tet, face) order (face k of
+// a tet leaves out its vertex k), each oriented with its normal away from that vertex. Used as
+// the mesh's contact surface in scenes with collision handling.
+bool build_tet_surface(int32_t nv, int32_t nt, const int32_t* tets, const float* pos, std::vector<int32_t>& faces,
+                       std::string& err);
 // TriMeshVertexGraph.h:20-45: mesh edges; with bending also the two vertices opposite each
 // interior edge.
 Graph tri_vertex_graph(int32_t nv, int32_t nf, const int32_t* faces, bool withBending);

@@ -150,6 +150,7 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
     if (!read_json(modelsPath, models, err) || !read_json(paramsPath, params, err)) return false;
     out.params = default_physics_params();
     gvbd_physics_params& p = out.params;
+    bool handleCollisionGiven = false;
     if (const json::Value* pp = params.get("PhysicsParams")) {
         get_vec3(*pp, "gravity", p.gravity);
         get_num(*pp, "timeStep", p.time_step);
@@ -175,6 +176,7 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         // cloth (VBDClothPhysicsParameters::fromJson, VBDCloth/VBDClothPhysicsParameters.h:83-130)
         get_num(*pp, "stepSize", p.cloth_step_size);
         get_num(*pp, "boundaryFrictionEpsValue", p.cloth_boundary_friction_eps);
+        handleCollisionGiven = pp->get("handleCollision") != nullptr;
         get_int(*pp, "handleCollision", p.handle_collision);
         get_num(*pp, "contactStiffness", p.contact_stiffness);
         get_num(*pp, "contactRadius", p.contact_radius);
@@ -221,6 +223,8 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             get_num(mj, "exponentialVelDamping", m.mat.exponentialVelDamping);
             get_num(mj, "constantVelDamping", m.mat.constantVelDamping);
             get_int(mj, "initializationType", m.mat.initializationType);
+            get_num(mj, "frictionDynamic", m.mat.frictionDynamic);  // boundary-surface contact friction
+            get_num(mj, "frictionEpsV", m.mat.frictionEpsV);
             if (m.mat.initializationType < 0 || m.mat.initializationType > 4) {
                 err = "initializationType must be 0..4 for tet meshes in this build";
                 return false;
@@ -276,6 +280,9 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             return false;
         }
     }
+    // handleCollision defaults to true in the cloth framework (VBDClothPhysicsParameters.h:60) and to
+    // false in the tet framework (VBDPhysicsParameters.h:35)
+    if (!handleCollisionGiven && out.cloths.empty()) p.handle_collision = 0;
     return true;
 }
 

@@ -120,6 +120,9 @@ __device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs
     }
 }
 
+__device__ void contact_forces(const SceneView& S, int v, const int* faceRef, int fb, int fe, const int* edgeRef,
+                               int eb, int ee, f3& f, Sym3& h);
+
 // ---------------------------------------------------------------------------
 // Tet vertex block (VBDPhysics.cpp:2402-2483)
 // ---------------------------------------------------------------------------
@@ -226,17 +229,42 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
     h.zz += hi;
     if (sp.usePlaneGround)
         boundary_force(sp, sp.boundaryStiffness, sp.frictionEpsU, x, xp, iteration >= sp.frictionStartIter, f, h);
+    // collision handling: cloth-model contact on the mesh's boundary surface (interior vertices
+    // have empty reference ranges and an unbounded step)
+    int4 tc = make_int4(0, 0, 0, 0);
+    if (S.hasContact) {
+        tc = S.tcontact[it.slotBegin + local];
+        contact_forces(S, v, S.cv.tFaceRef, tc.x, tc.y, S.cv.tEdgeRef, tc.z, tc.w, f, h);
+    }
 
-    if (dot(f, f) > 1e-10f) {
+    f3 xn = x;
+    const bool moved = dot(f, f) > 1e-10f;
+    if (moved) {
         f3 dx;
         float step = sp.stepSize;
         if (!solve_sym3(h, f, dx)) {
             step = sp.stepSizeGD;
             dx = f;
         }
-        x = x + step * dx;
-        S.pos[v] = make_float4(x.x, x.y, x.z, 0.f);
+        xn = x + step * dx;
     }
+    if (S.hasContact) {
+        // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246). The tet initial
+        // guess is not bounded, so every vertex is truncated, moved or not (interior vertices
+        // have an unbounded step).
+        const f3 x0 = xyz(S.cv.xAtCD[v]);
+        const f3 acc = xn - x0;
+        const float size = sqrtf(dot(acc, acc));
+        const float b = S.cv.bound[v];
+        if (size > b) {
+            xn = x0 + (b / size) * acc;
+            atomicOr(S.cv.flag, 1);
+        }
+    }
+    if (slot.w & SLOT_JACOBI)
+        S.pnext[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
+    else if (moved)
+        S.pos[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,19 +486,20 @@ __device__ void ee_contact(const SceneView& S, int e1, int e2, int order, f3& f,
 }
 
 // VBDClothPhysics.cpp:1033-1094: the vertex's own VF contacts, the FV contacts of its incident
-// faces and the EE contacts of its incident edges
-__device__ void cloth_contacts(const SceneView& S, int v, int4 sa, int4 sb, f3& f, Sym3& h) {
+// faces (faceRef[fb, fe)) and the EE contacts of its incident edges (edgeRef[eb, ee))
+__device__ void contact_forces(const SceneView& S, int v, const int* faceRef, int fb, int fe, const int* edgeRef,
+                               int eb, int ee, f3& f, Sym3& h) {
     const ContactView& C = S.cv;
     const int nvf = C.vfCount[v];
     for (int k = 0; k < nvf; ++k) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, f, h);
-    for (int r = sa.y; r < sa.z; ++r) {
-        const int ref = C.faceRef[r];
+    for (int r = fb; r < fe; ++r) {
+        const int ref = faceRef[r];
         const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int2 range = C.fvRange[face];
         for (int i = range.x; i < range.y; ++i) vf_contact(S, (int)(C.fvKeys[i] & 0xFFFFFFFFu), face, order, f, h);
     }
-    for (int r = sb.z; r < sb.w; ++r) {
-        const int ref = C.edgeRef[r];
+    for (int r = eb; r < ee; ++r) {
+        const int ref = edgeRef[r];
         const int e = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int n = C.eeCount[e];
         for (int k = 0; k < min(n, C.eeCap); ++k)
@@ -607,7 +636,7 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     h.yy += hi;
     h.zz += hi;
     if (sp.usePlaneGround) boundary_force(sp, sp.clothBoundaryStiffness, sp.clothFrictionEpsU, x, xp, true, f, h);
-    if (S.hasContact) cloth_contacts(S, v, sa, sb, f, h);
+    if (S.hasContact) contact_forces(S, v, S.cv.faceRef, sa.y, sa.z, S.cv.edgeRef, sb.z, sb.w, f, h);
     f3 dx;
     f3 xn = x;
     if (solve_sym3(h, f, dx)) {
@@ -654,7 +683,7 @@ __global__ __launch_bounds__(256) void cloth_commit_kernel(const SceneView* __re
     const WorkItem it = items[blockIdx.x];
     const SceneView& S = scenes[it.scene];
     for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
-        const int v = S.cslots[it.slotBegin + i].a.x;
+        const int v = it.kind == WORK_TET ? S.slots[it.slotBegin + i].x : S.cslots[it.slotBegin + i].a.x;
         S.pos[v] = S.pnext[v];
     }
 }

@@ -43,6 +43,20 @@ struct DeviceScene {
     int* faceRef = nullptr;     // contact scenes: per face record, face id | order << 30
     int* edgeRef = nullptr;     // contact scenes: per slot incident edge, edge id | order << 30
     float2* friction = nullptr; // contact scenes: per vertex (frictionDynamic, frictionEpsV)
+    int* tFaceRef = nullptr;    // contact scenes: per tet slot incident boundary face, face id | order << 30
+    int* tEdgeRef = nullptr;    // contact scenes: per tet slot incident boundary edge, edge id | order << 30
+    int4* tcontact = nullptr;   // contact scenes: per tet slot, its tFaceRef and tEdgeRef ranges
+};
+
+// Boundary surface of a tet mesh in a scene with collision handling: its contact faces, their
+// edges (numbered as the detector numbers them) and, per vertex, the incident ones with the
+// vertex's order in each.
+struct TetSurface {
+    std::vector<int32_t> faces;  // 3 per face, outward
+    std::vector<TriEdge> edges;
+    std::vector<int32_t> vfBegin, vfRef;  // vertex -> (face | order << 30)
+    std::vector<int32_t> veBegin, veRef;  // vertex -> (edge | order << 30), ascending edge id
+    std::vector<uint8_t> surface;         // vertex on the boundary
 };
 
 enum MeshKind { MESH_TET = 0, MESH_CLOTH = 1 };
@@ -61,6 +75,8 @@ struct SceneObj {
     std::vector<int32_t> clothGroupOffset;     // G + 1, into cloth slots
     std::vector<int64_t> slotWorkBytes;        // algorithmic bytes per tet slot then per cloth slot
     std::vector<uint8_t> clothSlotInplace;     // per cloth slot: solved in place (no commit pass)
+    std::vector<TetSurface> tetSurfaces;       // per tet mesh, scenes with collision handling
+    bool tetJacobi = false;                    // tet slots write positionsNext (collision handling)
     DeviceScene dev;
     SceneView view{};
     int32_t plan = -1;  // the scene's own single-scene plan
@@ -252,8 +268,18 @@ void check_queue_status(gvbd_context* ctx) {
 }
 
 // Colour-major tet slot and incidence records of one vertex.
+// Contact references of a tet vertex (scenes with collision handling): its incident boundary
+// faces and edges as scene-global ids with the vertex's order, and the slot's ranges of them.
+struct TetContactRefs {
+    int32_t faceOff = 0, edgeOff = 0;
+    const TetSurface* surf = nullptr;
+    std::vector<int32_t>* faceRef = nullptr;
+    std::vector<int32_t>* edgeRef = nullptr;
+    std::vector<int4>* ranges = nullptr;
+};
+
 void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<int4>& slots,
-                   std::vector<TetInc>& tinc, std::vector<int64_t>& work) {
+                   std::vector<TetInc>& tinc, std::vector<int64_t>& work, const TetContactRefs* refs) {
     int4 slot;
     slot.x = off + lv;
     slot.y = (int32_t)tinc.size();
@@ -284,6 +310,25 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
     slot.z = (int32_t)tinc.size();
     const bool damping = m.mat.dampingHydrostatic != 0.f || m.mat.dampingDeviatoric != 0.f;
     slot.w = damping ? SLOT_DAMPING : 0;
+    if (refs) {
+        // contact pairs may join vertices of one colour: solve the group Jacobi-style
+        slot.w |= SLOT_JACOBI;
+        const TetSurface& S = *refs->surf;
+        int4 r;
+        r.x = (int32_t)refs->faceRef->size();
+        for (int32_t k = S.vfBegin[lv]; k < S.vfBegin[lv + 1]; ++k) {
+            const int32_t ref = S.vfRef[k];
+            refs->faceRef->push_back((refs->faceOff + (ref & 0x3FFFFFFF)) | (ref & ~0x3FFFFFFF));
+        }
+        r.y = (int32_t)refs->faceRef->size();
+        r.z = (int32_t)refs->edgeRef->size();
+        for (int32_t k = S.veBegin[lv]; k < S.veBegin[lv + 1]; ++k) {
+            const int32_t ref = S.veRef[k];
+            refs->edgeRef->push_back((refs->edgeOff + (ref & 0x3FFFFFFF)) | (ref & ~0x3FFFFFFF));
+        }
+        r.w = (int32_t)refs->edgeRef->size();
+        refs->ranges->push_back(r);
+    }
     slots.push_back(slot);
     // gather model: 60 B per vertex + 96 B per incident tet
     work.push_back(60 + 96 * (int64_t)(slot.z - slot.y));
@@ -384,13 +429,22 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     }
     // contact references (collision handling): scene-global face and edge ids follow the mesh
     // order, as the detector numbers them
-    const bool contact = s.params.handle_collision != 0 && s.tets.empty();
-    std::vector<int32_t> faceRef, edgeRef;
+    const bool contact = s.params.handle_collision != 0;
+    std::vector<int32_t> faceRef, edgeRef, tFaceRef, tEdgeRef;
+    std::vector<int4> tcontact;
     std::vector<ClothContactRefs> refs(s.meshes.size());
+    std::vector<TetContactRefs> trefs(s.meshes.size());
     std::vector<std::vector<std::vector<int32_t>>> vertexEdges(s.meshes.size());
     if (contact) {
         int32_t fo = 0, eo = 0;
         for (size_t im = 0; im < s.meshes.size(); ++im) {
+            if (s.meshes[im].first == MESH_TET) {
+                const TetSurface& S = s.tetSurfaces[s.meshes[im].second];
+                trefs[im] = TetContactRefs{fo, eo, &S, &tFaceRef, &tEdgeRef, &tcontact};
+                fo += (int32_t)S.faces.size() / 3;
+                eo += (int32_t)S.edges.size();
+                continue;
+            }
             const ClothMeshHost& m = s.cloths[s.meshes[im].second];
             vertexEdges[im].assign(m.nv, {});
             for (int32_t e = 0; e < (int32_t)m.edges.size(); ++e) {
@@ -402,6 +456,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             eo += (int32_t)m.edges.size();
         }
     }
+    s.tetJacobi = contact && !s.tets.empty();
     s.clothSlotInplace.clear();
     for (int32_t g = 0; g < G; ++g) {
         const auto& grp = s.groups[g];
@@ -422,7 +477,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             if (fixedMask[im][lv]) continue;
             const int32_t off = s.vertexOffset[im];
             if (s.meshes[im].first == MESH_TET) {
-                emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork);
+                emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork,
+                              contact ? &trefs[im] : nullptr);
             } else {
                 const bool inplace = !multiColor[im] && !s.params.handle_collision;
                 emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, inplace, cslots, cface, cbend, clothWork,
@@ -456,6 +512,9 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             vp = make_float4(std::pow(m.mat.exponentialVelDamping, s.sp.dt), m.mat.constantVelDamping * s.sp.dt,
                              m.mat.dampingHydrostatic, m.mat.dampingDeviatoric);
             code = (uint8_t)(m.mat.initializationType & 7);
+            if (contact)
+                for (int32_t i = 0; i < m.nv; ++i)
+                    friction[s.vertexOffset[im] + i] = make_float2(m.mat.frictionDynamic, m.mat.frictionEpsV);
         } else {
             const ClothMeshHost& m = s.cloths[s.meshes[im].second];
             mp = &m.pos;
@@ -496,7 +555,10 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bFR = contact ? align(sizeof(int32_t) * std::max<size_t>(faceRef.size(), 1)) : 0;
     const size_t bER = contact ? align(sizeof(int32_t) * std::max<size_t>(edgeRef.size(), 1)) : 0;
     const size_t bFr = contact ? align(sizeof(float2) * (size_t)std::max(nv, 1)) : 0;
-    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr;
+    const size_t bTF = contact ? align(sizeof(int32_t) * std::max<size_t>(tFaceRef.size(), 1)) : 0;
+    const size_t bTE = contact ? align(sizeof(int32_t) * std::max<size_t>(tEdgeRef.size(), 1)) : 0;
+    const size_t bTC = contact ? align(sizeof(int4) * std::max<size_t>(tcontact.size(), 1)) : 0;
+    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -529,6 +591,9 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         d.faceRef = (int*)take(bFR);
         d.edgeRef = (int*)take(bER);
         d.friction = (float2*)take(bFr);
+        d.tFaceRef = (int*)take(bTF);
+        d.tEdgeRef = (int*)take(bTE);
+        d.tcontact = (int4*)take(bTC);
     }
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "upload");
@@ -549,6 +614,9 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         up(d.faceRef, faceRef.data(), sizeof(int32_t) * faceRef.size());
         up(d.edgeRef, edgeRef.data(), sizeof(int32_t) * edgeRef.size());
         up(d.friction, friction.data(), sizeof(float2) * friction.size());
+        up(d.tFaceRef, tFaceRef.data(), sizeof(int32_t) * tFaceRef.size());
+        up(d.tEdgeRef, tEdgeRef.data(), sizeof(int32_t) * tEdgeRef.size());
+        up(d.tcontact, tcontact.data(), sizeof(int4) * tcontact.size());
     }
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     SceneView& v = s.view;
@@ -564,6 +632,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     v.cslots = d.cslots;
     v.cface = d.cface;
     v.cbend = d.cbend;
+    v.tcontact = d.tcontact;
     v.nv = nv;
     v.sp = s.sp;
 }
@@ -607,6 +676,9 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
                 items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
             commit_runs(s, g, kVertexChunk,
                         [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
+            if (s.tetJacobi)  // tet slots of contact scenes write positionsNext
+                for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += kVertexChunk)
+                    commits.push_back({(int32_t)si, k, std::min(kVertexChunk, e - k), WORK_TET});
         }
         p.groupItemBegin[g + 1] = (int32_t)items.size();
         p.groupCommitBegin[g + 1] = (int32_t)commits.size();
@@ -818,12 +890,44 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
 void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_t)>& reserve = {}) {
     if (s.initialized) fail(GVBD_ERR_STATE, "scene already initialised");
     if (s.meshes.empty()) fail(GVBD_ERR_STATE, "scene has no meshes");
-    const bool contact = !s.cloths.empty() && s.params.handle_collision;
-    if (contact && !s.tets.empty())
-        fail(GVBD_ERR_INVALID, "handle_collision (cloth contact) needs a scene of cloth meshes only");
+    const bool contact = s.params.handle_collision != 0;
     if (contact && !(s.params.contact_query_radius > 0.f))
         fail(GVBD_ERR_INVALID, "contact_query_radius must be positive");
     s.sp = make_solve_params(s.params, s);
+    // tet meshes take part in contact through their boundary surface
+    s.tetSurfaces.clear();
+    if (contact) {
+        for (const TetMeshHost& m : s.tets) {
+            TetSurface S;
+            std::string err;
+            if (!build_tet_surface(m.nv, m.nt, m.tets.data(), m.pos.data(), S.faces, err) ||
+                !build_tri_edges(m.nv, (int32_t)S.faces.size() / 3, S.faces.data(), S.edges, nullptr, err))
+                fail(GVBD_ERR_INVALID, "tet mesh boundary surface: " + err);
+            const int32_t nf = (int32_t)S.faces.size() / 3;
+            S.surface.assign(m.nv, 0);
+            for (int32_t v : S.faces) S.surface[v] = 1;
+            S.vfBegin.assign(m.nv + 1, 0);
+            for (int32_t v : S.faces) S.vfBegin[v + 1]++;
+            for (int32_t i = 0; i < m.nv; ++i) S.vfBegin[i + 1] += S.vfBegin[i];
+            S.vfRef.assign(S.vfBegin[m.nv], 0);
+            std::vector<int32_t> fill(S.vfBegin.begin(), S.vfBegin.end() - 1);
+            for (int32_t f = 0; f < nf; ++f)
+                for (int k = 0; k < 3; ++k) S.vfRef[fill[S.faces[3 * (size_t)f + k]]++] = f | (k << 30);
+            S.veBegin.assign(m.nv + 1, 0);
+            for (const TriEdge& e : S.edges) {
+                S.veBegin[e.v1 + 1]++;
+                S.veBegin[e.v2 + 1]++;
+            }
+            for (int32_t i = 0; i < m.nv; ++i) S.veBegin[i + 1] += S.veBegin[i];
+            S.veRef.assign(S.veBegin[m.nv], 0);
+            fill.assign(S.veBegin.begin(), S.veBegin.end() - 1);
+            for (int32_t e = 0; e < (int32_t)S.edges.size(); ++e) {
+                S.veRef[fill[S.edges[e].v1]++] = e;
+                S.veRef[fill[S.edges[e].v2]++] = e | (1 << 30);
+            }
+            s.tetSurfaces.push_back(std::move(S));
+        }
+    }
     std::vector<const std::vector<std::vector<int32_t>>*> cats;
     std::vector<bool> removeUsed;
     for (auto& mk : s.meshes) {
@@ -843,8 +947,14 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         s.cd = std::make_unique<ContactDetector>(ctx->device, ctx->stream, s.params.contact_query_radius, 8, 8);
         s.cd->useExternalPositions(s.dev.pos);
         for (auto& mk : s.meshes) {
-            const ClothMeshHost& m = s.cloths[mk.second];
-            s.cd->addMesh(m.nv, m.pos.data(), m.nf, m.faces.data());
+            if (mk.first == MESH_TET) {
+                const TetMeshHost& m = s.tets[mk.second];
+                const TetSurface& S = s.tetSurfaces[mk.second];
+                s.cd->addMesh(m.nv, m.pos.data(), (int)S.faces.size() / 3, S.faces.data(), S.surface.data());
+            } else {
+                const ClothMeshHost& m = s.cloths[mk.second];
+                s.cd->addMesh(m.nv, m.pos.data(), m.nf, m.faces.data());
+            }
         }
         s.cd->finalize();
         const DetectorDeviceView dv = s.cd->deviceView();
@@ -862,6 +972,8 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         cv.fvKeys = dv.fvKeys;
         cv.faceRef = s.dev.faceRef;
         cv.edgeRef = s.dev.edgeRef;
+        cv.tFaceRef = s.dev.tFaceRef;
+        cv.tEdgeRef = s.dev.tEdgeRef;
         cv.bound = dv.bounds;
         cv.xAtCD = dv.xAtDetection;
         cv.flag = dv.flag;
@@ -1097,6 +1209,8 @@ int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mes
         m.mat.constantVelDamping = desc->constant_vel_damping;
         m.mat.initializationType = desc->initialization_type;
         m.mat.initRatioG = desc->init_ratio_g;
+        m.mat.frictionDynamic = desc->friction_dynamic;
+        m.mat.frictionEpsV = desc->friction_eps_v;
         m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
         for (int32_t t : m.tets) require(t >= 0 && t < m.nv, "tet vertex index out of range");
         Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
@@ -1280,6 +1394,8 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
             "stage_records must be 0, or 32 or 64 with lanes_per_vertex in {4,8,16} and block in {128,256}");
     require(sc.stage_records == 0 || sc.mode == GVBD_MODE_GROUP_LAUNCHES,
             "LDS staging applies to group launches only");
+    require(!(p.contact && sc.mode == GVBD_MODE_WORK_QUEUE),
+            "scenes with collision handling run with group launches (the host decides each iteration's detection)");
     p.lpv = sc.lanes_per_vertex;
     p.stage = sc.stage_records;
     p.block = sc.block;
@@ -1622,9 +1738,10 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     DeviceScene& d = s.dev;
     mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
     mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
+    mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
-    mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend);
+    mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact);
 }
 
 // Copies an idle scene's arena out to host memory and frees it.
@@ -1890,7 +2007,7 @@ int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
         SceneObj& s = scene_of(ctx, scene);
         if (s.server >= 0) fail(GVBD_ERR_STATE, "scene is already registered with a server");
         require(!s.meshes.empty(), "scene has no meshes");
-        if (s.params.handle_collision && !s.cloths.empty())
+        if (s.params.handle_collision)
             fail(GVBD_ERR_INVALID, "scenes with collision handling are stepped alone (gvbd_scene_run_step), not served");
         for (auto& kv : ctx->plans)
             if (kv.first != s.plan)

@@ -75,6 +75,9 @@ DEFAULT_TET_MATERIAL = dict(
     constant_vel_damping=0.0,
     initialization_type=0,
     init_ratio_g=0.5,
+    # boundary-surface contact friction in scenes with collision handling (as TriMesh.h:41-42)
+    friction_dynamic=0.1,
+    friction_eps_v=0.1,
 )
 
 
@@ -295,7 +298,7 @@ class VBDPhysics:
         d.num_tets = tets.size // 4
         d.tets = iptr(tets)
         for k in ("density", "mu", "lmbd", "damping_hydrostatic", "damping_deviatoric", "exponential_vel_damping",
-                  "constant_vel_damping", "init_ratio_g"):
+                  "constant_vel_damping", "init_ratio_g", "friction_dynamic", "friction_eps_v"):
             setattr(d, k, float(mat[k]))
         d.initialization_type = int(mat["initialization_type"])
         d.num_fixed = fixed.size

@@ -116,6 +116,8 @@ typedef struct gvbd_tet_mesh_desc {
     const int32_t* fixed_points;
     const int32_t* vertex_colors;  /* colour per vertex, or NULL to colour here */
     int32_t coloring_algorithm;    /* GVBD_COLOR_MCS / GVBD_COLOR_GREEDY when vertex_colors is NULL */
+    float friction_dynamic;        /* contact friction of the boundary surface (collision handling; 0.1) */
+    float friction_eps_v;          /* (0.1) */
 } gvbd_tet_mesh_desc;
 
 /* One cloth (triangle) mesh with StVK membrane + quadratic bending (VBDObjectParamsTriMeshStVk,

@@ -22,6 +22,8 @@
 // A failed 3x3 solve leaves the vertex where it is (the reference copies a stale
 // positionsNext, VBDClothPhysics.cpp:1141-1153 and :483-493).
 #include <algorithm>
+#include <array>
+#include <map>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -32,6 +34,7 @@
 #include <vector>
 
 #include "oracle.h"
+#include "tet_core.h"
 #include "pool.h"
 
 // contact detector oracle (detector_oracle.cpp)
@@ -43,6 +46,7 @@ void oracle_cd_counts(void* h, int64_t* counts);
 void oracle_cd_vf_contacts(void* h, int* rec, float* d);
 void oracle_cd_ee_contacts(void* h, int* ee, float* mu, float* d);
 void oracle_cd_bounds(void* h, float thickness, float relaxation, float* bounds);
+void oracle_cd_set_query_mask(void* h, const uint8_t* mask);
 }
 
 namespace {
@@ -63,6 +67,14 @@ template <class R> struct ClothMesh {
     R frictionMu = R(0.1), frictionEpsV = R(0.1);  // contact friction (TriMesh.h:41-42 defaults)
     int initType = 2;
     std::vector<int32_t> veBegin, veEdge;             // vertex -> incident edges (ascending)
+    // tet meshes in a contact scene: neo-Hookean elements; faces/edges above are the boundary
+    // surface, which takes part in contact
+    bool tet = false;
+    int nt = 0;
+    std::vector<int32_t> tets, incBegin, incTet, incOrder;
+    std::vector<R> tdminv, tvol, vprev;
+    std::vector<uint8_t> surface;                     // vertex lies on the boundary surface
+    R dampH = 0, dampD = 0;
 };
 
 // Contact state of a scene with handle_collision (global ids: meshes concatenated in order).
@@ -843,6 +855,193 @@ template <class R> void cloth_velocity(ClothScene<R>& s, ClothMesh<R>& m) {
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Tet meshes with cloth-model contact on their boundary surface. The reference's tet solver
+// uses its own penetration (DCD/CCD) contact; this composition keeps each tet vertex's solve as
+// the reference's (VBDPhysics.cpp:2402-2483) and adds, for boundary vertices, the cloth VF/EE
+// contact energies, friction and conservative-bound truncation (VBDClothPhysics.cpp:1033-1094,
+// 1223-1246). Vertices of one colour are solved Jacobi-style (positionsNext) as cloth is, since
+// contact pairs may share a colour.
+// ---------------------------------------------------------------------------
+
+// Boundary faces: faces of exactly one tet, visited in (tet, face) order, each oriented with
+// its normal away from the tet's fourth vertex.
+template <class R> void tet_surface(int nt, const int32_t* tets, const R* x, std::vector<int32_t>& faces) {
+    static const int opp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    std::map<std::array<int32_t, 3>, int> count;
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+            std::array<int32_t, 3> key = {tets[4 * t + opp[k][0]], tets[4 * t + opp[k][1]], tets[4 * t + opp[k][2]]};
+            std::sort(key.begin(), key.end());
+            count[key]++;
+        }
+    faces.clear();
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 4; ++k) {
+            int32_t a = tets[4 * t + opp[k][0]], b = tets[4 * t + opp[k][1]], c = tets[4 * t + opp[k][2]];
+            std::array<int32_t, 3> key = {a, b, c};
+            std::sort(key.begin(), key.end());
+            if (count[key] != 1) continue;
+            const int32_t o = tets[4 * t + k];
+            R e1[3], e2[3], n[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = x[3 * b + d] - x[3 * a + d];
+                e2[d] = x[3 * c + d] - x[3 * a + d];
+                w[d] = x[3 * o + d] - x[3 * a + d];
+            }
+            cross3(e1, e2, n);
+            if (dot3(n, w) > 0) std::swap(b, c);
+            faces.push_back(a);
+            faces.push_back(b);
+            faces.push_back(c);
+        }
+}
+
+// the contacts of global vertex gv of mesh mi (own VF, FV of incident faces, EE of incident edges)
+template <class R> void vertex_contacts(ClothScene<R>& s, ClothMesh<R>& m, int mi, int vid, R* f, R* h) {
+    const ContactState& C = *s.contact;
+    const int gv = C.vOff[mi] + vid;
+    for (int gf : C.vfFaces[gv]) vf_contact(s, gv, gf, 3, f, h);
+    for (int k = m.vfBegin[vid]; k < m.vfBegin[vid + 1]; ++k) {
+        const int gf = C.fOff[mi] + m.vfFace[k];
+        for (int gu : C.fvVerts[gf]) vf_contact(s, gu, gf, m.vfOrder[k], f, h);
+    }
+    for (int k = m.veBegin[vid]; k < m.veBegin[vid + 1]; ++k) {
+        const int e = m.veEdge[k];
+        const int ge = C.eOff[mi] + e;
+        const int order = m.edges[6 * e] == vid ? 0 : 1;
+        for (int ge2 : C.eeOther[ge]) ee_contact(s, ge, ge2, order, f, h);
+    }
+}
+
+// VBDPhysics.cpp:2402-2483 for one tet vertex, plus surface contact; writes positionsNext
+template <class R> void tet_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid, bool applyFriction) {
+    R h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    R f[3] = {0, 0, 0};
+    const R* xv = &m.x[3 * vid];
+    const int mi = (int)(&m - s.meshes.data());
+    const R disp[3] = {xv[0] - m.xprev[3 * vid], xv[1] - m.xprev[3 * vid + 1], xv[2] - m.xprev[3 * vid + 2]};
+    for (int k = m.incBegin[vid]; k < m.incBegin[vid + 1]; ++k) {
+        const int t = m.incTet[k];
+        R x4[12];
+        for (int j = 0; j < 4; ++j)
+            for (int d = 0; d < 3; ++d) x4[3 * j + d] = m.x[3 * m.tets[4 * t + j] + d];
+        tetcore::neohookean_vertex<R>(x4, &m.tdminv[9 * t], m.tvol[t], m.mu, m.lmbd, m.incOrder[k], m.dampH, m.dampD,
+                                      s.dt, disp, f, h);
+    }
+    const R mass = m.mass[vid];
+    for (int d = 0; d < 3; ++d) f[d] += mass * (m.y[3 * vid + d] - xv[d]) * s.dtSqrRecip;
+    const R hi = mass * s.dtSqrRecip;
+    h[0] += hi; h[4] += hi; h[8] += hi;
+    if (s.p[OP_USE_PLANE_GROUND] != 0) {
+        const R k = (R)s.p[OP_BOUNDARY_STIFFNESS];
+        for (int dim = 0; dim < 3; ++dim) {
+            const R lower = (R)s.p[OP_LOWER_X + dim], upper = (R)s.p[OP_UPPER_X + dim];
+            R pen;
+            if (xv[dim] < lower) {
+                pen = lower - xv[dim];
+                f[dim] += pen * k;
+            } else if (xv[dim] > upper) {
+                pen = xv[dim] - upper;
+                f[dim] -= pen * k;
+            } else {
+                continue;
+            }
+            h[dim + 3 * dim] += k;
+            if (applyFriction) {
+                R T[6] = {0, 0, 0, 0, 0, 0};
+                T[(dim + 1) % 3] = 1;
+                T[3 + (dim + 2) % 3] = 1;
+                R u[2] = {T[0] * disp[0] + T[1] * disp[1] + T[2] * disp[2],
+                          T[3] * disp[0] + T[4] * disp[1] + T[5] * disp[2]};
+                R ff[3], fh[9];
+                friction<R>((R)s.p[OP_BOUNDARY_FRICTION_MU], pen * k, T, u,
+                            (R)s.p[OP_BOUNDARY_FRICTION_EPS_V] * s.dt, ff, fh);
+                for (int d = 0; d < 3; ++d) f[d] += ff[d];
+                for (int i = 0; i < 9; ++i) h[i] += fh[i];
+            }
+        }
+    }
+    if (s.contact && m.surface[vid]) vertex_contacts(s, m, mi, vid, f, h);
+    R* xn = &m.xnext[3 * vid];
+    for (int d = 0; d < 3; ++d) xn[d] = xv[d];
+    if (f[0] * f[0] + f[1] * f[1] + f[2] * f[2] > R(1e-10)) {
+        R dx[3];
+        R step = (R)s.p[OP_STEP_SIZE];
+        if (!solve3<R>(h, f, dx)) {
+            step = (R)s.p[OP_STEP_SIZE_GD];
+            dx[0] = f[0]; dx[1] = f[1]; dx[2] = f[2];
+        }
+        for (int d = 0; d < 3; ++d) xn[d] = xv[d] + step * dx[d];
+    }
+    // The tet initial guess is not bounded, so the truncation applies to every boundary vertex,
+    // moved or not: a vertex whose force is under the update threshold still returns within its
+    // conservative bound (otherwise it could stay past it, and whether it is truncated would hang
+    // on rounding of a near-zero force).
+    if (s.contact && m.surface[vid]) {
+        ContactState& C = *s.contact;
+        const int gv = C.vOff[mi] + vid;
+        R acc[3];
+        for (int d = 0; d < 3; ++d) acc[d] = xn[d] - (R)C.xAtCD[3 * (size_t)gv + d];
+        const R size = std::sqrt(dot3(acc, acc));
+        const R b = (R)C.bound[gv];
+        if (size > b) {
+            for (int d = 0; d < 3; ++d) xn[d] = (R)C.xAtCD[3 * (size_t)gv + d] + acc[d] * (b / size);
+            C.required = 1;
+        }
+    }
+}
+
+// VBD_BaseMaterial.h:244-259, VBD_BaseMaterial.cpp:7-50 (initialisation types 0-4)
+template <class R> void tet_initial_step(ClothScene<R>& s, ClothMesh<R>& m) {
+    const R dt = s.dt;
+    const R g[3] = {(R)s.p[OP_GRAVITY_X], (R)s.p[OP_GRAVITY_Y], (R)s.p[OP_GRAVITY_Z]};
+    m.xprev = m.x;
+    m.vprev = m.v;
+    if (s.p[OP_ASSOCIATE_GRAVITY] != 0) {
+        const R dg[3] = {dt * g[0], dt * g[1], dt * g[2]};
+        for (int i = 0; i < m.nv; ++i)
+            for (int d = 0; d < 3; ++d) m.v[3 * i + d] += dg[d];
+    }
+    for (int f : m.fixedList)
+        for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
+    for (int i = 0; i < 3 * m.nv; ++i) m.y[i] = m.x[i] + dt * m.v[i];
+    switch (m.initType) {
+    case 0: m.x = m.y; break;
+    case 2:
+        for (int i = 0; i < 3 * m.nv; ++i) m.x[i] = m.xprev[i] + dt * m.vprev[i];
+        break;
+    case 3:
+        for (int i = 0; i < 3 * m.nv; ++i) m.x[i] = R(0.5) * m.x[i] + R(0.5) * m.y[i];
+        break;
+    case 4: {
+        const R r = (R)s.p[OP_INIT_RATIO_G];
+        for (int i = 0; i < m.nv; ++i)
+            for (int d = 0; d < 3; ++d) m.x[3 * i + d] = m.x[3 * i + d] + dt * m.vprev[3 * i + d] + dt * dt * r * g[d];
+        break;
+    }
+    default: break;
+    }
+}
+
+// VBDPhysics.cpp:2486-2561 (damping pre-scaled by timeStepEqualize at load)
+template <class R> void tet_velocity(ClothScene<R>& s, ClothMesh<R>& m) {
+    for (int i = 0; i < m.nv; ++i) {
+        R* v = &m.v[3 * i];
+        for (int d = 0; d < 3; ++d) v[d] = (m.x[3 * i + d] - m.xprev[3 * i + d]) / s.dt;
+        R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (vMag > R(1e-6)) {
+            R vNew = vMag * m.expVel - m.constVel;
+            vNew = vNew > R(1e-6) ? vNew : R(0);
+            for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
+        } else {
+            for (int d = 0; d < 3; ++d) v[d] *= R(0);
+        }
+    }
+    for (int f : m.fixedList)
+        for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
+}
+
 template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
     const int nsub = (int)s.p[OP_NUM_SUBSTEPS], iters = (int)s.p[OP_ITERATIONS];
     for (int sub = 0; sub < nsub; ++sub) {
@@ -851,13 +1050,18 @@ template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
             // guess), later when a vertex left its bound (VBDClothPhysics.cpp:418-434)
             if (s.contact && (it == 0 || s.contact->required)) cloth_detect(s);
             if (it == 0)
-                for (auto& m : s.meshes) cloth_initial_step(s, m);
+                for (auto& m : s.meshes) m.tet ? tet_initial_step(s, m) : cloth_initial_step(s, m);
+            const bool friction = it >= (int)s.p[OP_FRICTION_START_ITER];
             for (auto& grp : s.groups) {
                 const int n = (int)grp.size() / 2;
                 auto solve = [&](int i) {
                     ClothMesh<R>& m = s.meshes[grp[2 * i]];
                     int vid = grp[2 * i + 1];
-                    if (!m.fixed[vid]) cloth_vertex(s, m, vid);
+                    if (m.fixed[vid]) return;
+                    if (m.tet)
+                        tet_vertex(s, m, vid, friction);
+                    else
+                        cloth_vertex(s, m, vid);
                 };
                 auto copy = [&](int i) {
                     ClothMesh<R>& m = s.meshes[grp[2 * i]];
@@ -875,8 +1079,8 @@ template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
             }
         }
         if (iters == 0)
-            for (auto& m : s.meshes) cloth_initial_step(s, m);
-        for (auto& m : s.meshes) cloth_velocity(s, m);
+            for (auto& m : s.meshes) m.tet ? tet_initial_step(s, m) : cloth_initial_step(s, m);
+        for (auto& m : s.meshes) m.tet ? tet_velocity(s, m) : cloth_velocity(s, m);
     }
 }
 
@@ -932,6 +1136,61 @@ static int cloth_add(ClothScene<R>& s, int nv, const double* pos, const double* 
     return (int)s.meshes.size() - 1;
 }
 
+// A tet mesh (VBD/VBDPhysics.cpp material setup, TetMeshFEM.cpp:332-443) in a scene with
+// collision handling: its boundary surface takes part in contact.
+template <class R>
+static int tet_add(ClothScene<R>& s, int nv, const double* pos, const double* vel, int nt, const int32_t* tets,
+                   const double* mat, int nfixed, const int32_t* fixed, const int32_t* colors) {
+    ClothMesh<R> m;
+    m.tet = true;
+    m.nv = nv;
+    m.nt = nt;
+    m.x.resize(3 * nv);
+    m.v.assign(3 * nv, 0);
+    for (int i = 0; i < 3 * nv; ++i) m.x[i] = (R)pos[i];
+    if (vel)
+        for (int i = 0; i < 3 * nv; ++i) m.v[i] = (R)vel[i];
+    m.xprev = m.x;
+    m.vprev = m.v;
+    m.y = m.x;
+    m.xnext = m.x;
+    m.tets.assign(tets, tets + 4 * nt);
+    m.density = (R)mat[OM_DENSITY];
+    m.mu = (R)mat[OM_MU];
+    m.lmbd = (R)mat[OM_LAMBDA];
+    m.dampH = (R)mat[OM_DAMP_HYDRO];
+    m.dampD = (R)mat[OM_DAMP_DEV];
+    m.initType = (int)mat[OM_INIT_TYPE];
+    m.expVel = (R)std::pow((R)mat[OM_EXP_VEL_DAMPING], s.dt);  // timeStepEqualize (VBDPhysics.cpp:794-807)
+    m.constVel = (R)mat[OM_CONST_VEL_DAMPING] * s.dt;
+    m.frictionMu = (R)mat[OM_FRICTION_MU];
+    m.frictionEpsV = (R)mat[OM_FRICTION_EPS_V];
+    m.tdminv.resize(9 * (size_t)nt);
+    m.tvol.resize(nt);
+    m.mass.resize(nv);
+    if (tetcore::tet_prepare<R>(nv, m.x.data(), nt, tets, m.density, m.tdminv.data(), m.tvol.data(), m.mass.data()))
+        return -1;
+    tetcore::build_incidence(m);
+    tet_surface(nt, tets, m.x.data(), m.faces);
+    m.nf = (int)m.faces.size() / 3;
+    if (tri_topology(m)) return -1;
+    m.surface.assign(nv, 0);
+    for (int32_t v : m.faces) m.surface[v] = 1;
+    m.fixed.assign(nv, 0);
+    for (int i = 0; i < nfixed; ++i) {
+        m.fixed[fixed[i]] = 1;
+        m.fixedList.push_back(fixed[i]);
+    }
+    int nc = 0;
+    for (int i = 0; i < nv; ++i) nc = std::max(nc, colors[i] + 1);
+    m.categories.assign(nc, {});
+    for (int i = 0; i < nv; ++i) m.categories[colors[i]].push_back(i);
+    std::stable_sort(m.categories.begin(), m.categories.end(),
+                     [](const std::vector<int32_t>& a, const std::vector<int32_t>& b) { return a.size() > b.size(); });
+    s.meshes.push_back(std::move(m));
+    return (int)s.meshes.size() - 1;
+}
+
 // VBDClothPhysics.cpp:72-111 (a mesh's colours may share a group: remove = false)
 template <class R> static void cloth_groups(ClothScene<R>& s) {
     size_t ng = 0;
@@ -940,14 +1199,20 @@ template <class R> static void cloth_groups(ClothScene<R>& s) {
     for (size_t im = 0; im < s.meshes.size(); ++im) {
         auto& m = s.meshes[im];
         const size_t nc = m.categories.size();
+        // tet meshes take each group at most once (VBDPhysics.cpp:235-274), cloth meshes may reuse
+        std::vector<int> avail(nc);
+        for (size_t c = 0; c < nc; ++c) avail[c] = (int)c;
         for (size_t c = 0; c < nc; ++c) {
-            size_t best = s.groups[0].size();
-            int bestId = 0;
-            for (size_t i = 1; i < nc; ++i)
-                if (s.groups[i].size() < best) {
-                    best = s.groups[i].size();
-                    bestId = (int)i;
+            size_t best = s.groups[avail[0]].size();
+            int bestId = avail[0];
+            size_t bestI = 0;
+            for (size_t i = 1; i < avail.size(); ++i)
+                if (s.groups[avail[i]].size() < best) {
+                    best = s.groups[avail[i]].size();
+                    bestId = avail[i];
+                    bestI = i;
                 }
+            if (m.tet) avail.erase(avail.begin() + bestI);
             for (int v : m.categories[c]) {
                 s.groups[bestId].push_back((int32_t)im);
                 s.groups[bestId].push_back(v);
@@ -1040,6 +1305,22 @@ int oracle_cloth_scene_add_mesh(OracleClothScene* sc, int nv, const double* pos,
     return cloth_add(sc->s32, nv, pos, vel, nf, faces, material, nfixed, fixed, colors);
 }
 
+int oracle_cloth_scene_add_tet(OracleClothScene* sc, int nv, const double* pos, const double* vel, int nt,
+                               const int32_t* tets, const double* material, int nfixed, const int32_t* fixed,
+                               const int32_t* colors) {
+    if (sc->precision == 64) return tet_add(sc->s64, nv, pos, vel, nt, tets, material, nfixed, fixed, colors);
+    return tet_add(sc->s32, nv, pos, vel, nt, tets, material, nfixed, fixed, colors);
+}
+
+int oracle_cloth_scene_surface(OracleClothScene* sc, int mesh, int32_t* faces) {
+    auto get = [&](auto& s) -> int {
+        if (mesh < 0 || mesh >= (int)s.meshes.size()) return -1;
+        if (faces) std::memcpy(faces, s.meshes[mesh].faces.data(), s.meshes[mesh].faces.size() * sizeof(int32_t));
+        return s.meshes[mesh].nf;
+    };
+    return sc->precision == 64 ? get(sc->s64) : get(sc->s32);
+}
+
 int oracle_cloth_scene_finalize(OracleClothScene* sc) {
     auto fin = [&](auto& s) -> int {
         cloth_groups(s);
@@ -1061,6 +1342,13 @@ int oracle_cloth_scene_finalize(OracleClothScene* sc) {
         }
         C->cd = oracle_cd_create((int)s.meshes.size(), nv.data(), nf.data(), faces.data());
         if (!C->cd) return -1;
+        bool anyTet = false;
+        std::vector<uint8_t> mask;
+        for (const auto& m : s.meshes) {
+            anyTet = anyTet || m.tet;
+            for (int i = 0; i < m.nv; ++i) mask.push_back(m.tet ? m.surface[i] : 1);
+        }
+        if (anyTet) oracle_cd_set_query_mask(C->cd, mask.data());
         s.contact = std::move(C);
         return 0;
     };

@@ -20,6 +20,7 @@
 // Arithmetic follows Embree's Vec3fa (dot = (x + y) + z, no fused multiply-add; the library is
 // built with -ffp-contract=off); sqrt and division are IEEE.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -184,6 +185,7 @@ struct Detector {
     std::vector<int> faceEdges;  // 3 * nf: edges AB, BC, CA
     std::vector<int> edges;      // 4 * ne: eV1 eV2 eV12Next eV21Next (global ids, -1 none)
     std::vector<int> vfBegin, vfFace, veBegin, veEdge, vvBegin, vvNei;
+    std::vector<uint8_t> query;  // per vertex: take part in VF queries (empty: all vertices)
     // results
     std::vector<V3> pos;
     float r = 0.f;
@@ -384,6 +386,7 @@ void detect(Detector& D, const float* pos, float r, int mode, int threads) {
     std::vector<std::vector<VFContact>> contactsV(D.nv);
     D.vfMin.assign(D.nv, r);
     pool.parallel_for(0, D.nv, [&](int v) {
+        if (!D.query.empty() && !D.query[v]) return;  // not a surface vertex
         const V3 p = D.pos[v];
         std::vector<int> cand;
         if (mode == 1) {
@@ -489,6 +492,12 @@ void* oracle_cd_create(int nmesh, const int* mesh_nv, const int* mesh_nf, const 
 
 void oracle_cd_destroy(void* h) { delete (Detector*)h; }
 
+// Vertices with mask 0 take no VF queries and get an unbounded step (tet interior vertices).
+void oracle_cd_set_query_mask(void* h, const uint8_t* mask) {
+    Detector& D = *(Detector*)h;
+    D.query.assign(mask, mask + D.nv);
+}
+
 void oracle_cd_sizes(void* h, int* nv, int* nf, int* ne) {
     const Detector& D = *(Detector*)h;
     *nv = D.nv;
@@ -559,6 +568,10 @@ void oracle_cd_min_dis(void* h, float* vf, float* face, float* edge) {
 void oracle_cd_bounds(void* h, float thickness, float relaxation, float* bounds) {
     const Detector& D = *(Detector*)h;
     for (int v = 0; v < D.nv; ++v) {
+        if (!D.query.empty() && !D.query[v]) {
+            bounds[v] = FLT_MAX;
+            continue;
+        }
         float dMin = D.r;
         dMin = std::min(dMin, D.vfMin[v]);
         for (int k = D.vfBegin[v]; k < D.vfBegin[v + 1]; ++k) dMin = std::min(dMin, D.faceMin[D.vfFace[k]]);

@@ -65,7 +65,9 @@ enum OracleMaterialParam {
     OM_EXP_VEL_DAMPING,          /* raw JSON value; the oracle applies pow(., dt) */
     OM_CONST_VEL_DAMPING,        /* raw JSON value; the oracle multiplies by dt */
     OM_INIT_TYPE,                /* ObjectParamsVBD::initializationType */
-    OM_COUNT = 8
+    OM_FRICTION_MU,              /* contact friction of the boundary surface (contact scenes) */
+    OM_FRICTION_EPS_V,
+    OM_COUNT = 10
 };
 
 #ifdef __cplusplus
@@ -124,6 +126,12 @@ void oracle_cloth_scene_destroy(OracleClothScene*);
 int oracle_cloth_scene_add_mesh(OracleClothScene*, int nv, const double* pos, const double* vel, int nf,
                                 const int32_t* faces, const double* material /*OMC_COUNT*/, int nfixed,
                                 const int32_t* fixed, const int32_t* colors);
+/* a tet mesh whose boundary surface takes part in contact (scenes with handle_collision) */
+int oracle_cloth_scene_add_tet(OracleClothScene*, int nv, const double* pos, const double* vel, int nt,
+                               const int32_t* tets, const double* material /*OM_COUNT*/, int nfixed,
+                               const int32_t* fixed, const int32_t* colors);
+/* a mesh's contact faces (a tet mesh's boundary surface); returns the face count */
+int oracle_cloth_scene_surface(OracleClothScene*, int mesh, int32_t* faces);
 int oracle_cloth_scene_finalize(OracleClothScene*);
 int oracle_cloth_scene_num_groups(OracleClothScene*);
 int oracle_cloth_scene_group(OracleClothScene*, int g, int32_t* out);

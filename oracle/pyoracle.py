@@ -17,8 +17,9 @@ OP = dict(GRAVITY_X=0, GRAVITY_Y=1, GRAVITY_Z=2, DT=3, NUM_SUBSTEPS=4, ITERATION
           BOUNDARY_FRICTION_MU=14, BOUNDARY_FRICTION_EPS_V=15, STEP_SIZE=16, STEP_SIZE_GD=17,
           FRICTION_START_ITER=18, ASSOCIATE_GRAVITY=19, INIT_RATIO_G=21)
 OP_COUNT = 32
-OM = dict(DENSITY=0, MU=1, LAMBDA=2, DAMP_HYDRO=3, DAMP_DEV=4, EXP_VEL_DAMPING=5, CONST_VEL_DAMPING=6, INIT_TYPE=7)
-OM_COUNT = 8
+OM = dict(DENSITY=0, MU=1, LAMBDA=2, DAMP_HYDRO=3, DAMP_DEV=4, EXP_VEL_DAMPING=5, CONST_VEL_DAMPING=6, INIT_TYPE=7,
+          FRICTION_MU=8, FRICTION_EPS_V=9)
+OM_COUNT = 10
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -99,6 +100,8 @@ def material_vector(m):
     v[OM["EXP_VEL_DAMPING"]] = m.get("exponential_vel_damping", 1.0)
     v[OM["CONST_VEL_DAMPING"]] = m.get("constant_vel_damping", 0.0)
     v[OM["INIT_TYPE"]] = m.get("initialization_type", 0)
+    v[OM["FRICTION_MU"]] = m.get("friction_dynamic", 0.1)
+    v[OM["FRICTION_EPS_V"]] = m.get("friction_eps_v", 0.1)
     return v
 
 
@@ -250,6 +253,9 @@ def _cloth_lib():
         L.oracle_cloth_scene_destroy.restype = None
         L.oracle_cloth_scene_add_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp,
                                                   ctypes.c_int, _ip, _ip]
+        L.oracle_cloth_scene_add_tet.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                                 ctypes.c_int, _ip, _ip]
+        L.oracle_cloth_scene_surface.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
         L.oracle_cloth_scene_finalize.argtypes = [ctypes.c_void_p]
         L.oracle_cloth_scene_num_groups.argtypes = [ctypes.c_void_p]
         L.oracle_cloth_scene_group.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
@@ -356,6 +362,34 @@ class ClothScene:
             raise ValueError("oracle: bad cloth mesh")
         self.nv.append(nv)
         return rc
+
+    def add_tet_mesh(self, pos, tets, material, colors, fixed=(), vel=None):
+        """A neo-Hookean tet mesh whose boundary surface takes part in contact (scenes with
+        collision handling): the tet vertex solve plus cloth-model contact on surface vertices."""
+        L = _cloth_lib()
+        pos = np.asarray(pos).reshape(-1, 3)
+        nv = len(pos)
+        p, pp = _d(pos.reshape(-1))
+        pv = None
+        if vel is not None:
+            v, pv = _d(np.asarray(vel).reshape(-1))
+        t, pt = _i(np.asarray(tets).reshape(-1))
+        m, pm = _d(material_vector(material))
+        fx, pfx = _i(np.asarray(fixed, dtype=np.int32).reshape(-1))
+        c, pc = _i(colors)
+        rc = L.oracle_cloth_scene_add_tet(self._h, nv, pp, pv, t.size // 4, pt, pm, fx.size, pfx, pc)
+        if rc < 0:
+            raise ValueError("oracle: bad tet mesh")
+        self.nv.append(nv)
+        return rc
+
+    def surface(self, mesh):
+        """The mesh's contact faces (a tet mesh's boundary surface, outward oriented)."""
+        L = _cloth_lib()
+        n = L.oracle_cloth_scene_surface(self._h, mesh, None)
+        buf = np.zeros(3 * n, np.int32)
+        L.oracle_cloth_scene_surface(self._h, mesh, buf.ctypes.data_as(_ip))
+        return buf.reshape(-1, 3)
 
     def finalize(self):
         if _cloth_lib().oracle_cloth_scene_finalize(self._h) != 0:

@@ -270,13 +270,17 @@ def test_contact_scene_rejects_graph_and_batch(gpu_ctx):
         s.destroy()
 
 
-def test_contact_with_tet_mesh_rejected(gpu_ctx):
+def test_contact_scene_with_tet_mesh_accepted(gpu_ctx):
+    """Cloth and tet meshes in one scene with collision handling: the tet mesh takes part in
+    contact through its boundary surface (parity in test_gpu_tet_contact.py)."""
     sim = VBDPhysics(gpu_ctx)
-    sim.setPhysicsParams(handle_collision=1)
+    sim.setPhysicsParams(handle_collision=1, num_substeps=1, iterations=2)
     pos, faces = scenes.cloth_sheet(4, 0.5, 1.0)
     sim.addClothMesh(pos, faces)
-    p, t = scenes.cube_tet_mesh(2, 0.1)
+    p, t = scenes.cube_tet_mesh(2, 0.1, origin=(0.0, 0.5, 0.0))
     sim.addTetMesh(p, t)
-    with pytest.raises(gaia_amd.VBDError, match="cloth meshes only"):
-        sim.initialize()
+    sim.initialize()
+    sim.runSteps(2)
+    x, _ = sim.getState(1)
+    assert np.isfinite(x).all()
     sim.destroy()

@@ -1,0 +1,118 @@
+"""GPU parity of tet bodies in contact: the tet vertex solve with cloth-model VF/EE contact,
+friction and conservative-bound truncation on each tet mesh's boundary surface (the composition
+for scenes of tet bodies; the reference's tet solver uses penetration contact instead). The HIP
+path through the C ABI against the FP32 oracle, per step from identical state, 1e-5
+normwise-relative."""
+import numpy as np
+import pytest
+
+from gaia_amd import VBDPhysics, scenes
+from gaia_amd.vbd import physics_dict
+from vbd_cases import build_gpu, build_oracle_mixed, rel_err, tet_drop_spec
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+THREADS = 8
+DT = float(np.float32(1.0 / 60.0))
+
+
+def parity(gpu_ctx, oracle, spec, with_fp64=False):
+    """Per step from identical state: the HIP path against the FP32 oracle (and, when asked, the
+    FP32 oracle against the FP64 one)."""
+    sim = build_gpu(gpu_ctx, spec)
+    o32 = build_oracle_mixed(oracle, spec)
+    o64 = build_oracle_mixed(oracle, spec, 64) if with_fp64 else None
+    nm = len(spec["meshes"])
+    errs, errs64 = [], []
+    for _ in range(spec["steps"]):
+        for m in range(nm):
+            sim.setState(m, *o32.get_state(m))
+            if o64 is not None:
+                o64.set_state(m, *o32.get_state(m))
+        sim.runStep()
+        o32.step(1, threads=THREADS)
+        if o64 is not None:
+            o64.step(1, threads=THREADS)
+            errs64.append(max(rel_err(o32.get_state(m)[0], o64.get_state(m)[0]) for m in range(nm)))
+        errs.append(max(rel_err(sim.getState(m)[0], o32.get_state(m)[0]) for m in range(nm)))
+    sim.destroy()
+    return np.array(errs), np.array(errs64)
+
+
+@pytest.mark.parametrize("energy", [0, 1, 2])
+def test_cube_on_fixed_sheet_per_step_parity(gpu_ctx, oracle, energy):
+    errs, _ = parity(gpu_ctx, oracle, tet_drop_spec(energy=energy))
+    print(f"\n[tet contact, energy {energy}] worst per-step rel err {errs.max():.3e}", flush=True)
+    assert errs.max() < TOL, f"step {errs.argmax()}: {errs.max():.3e}"
+
+
+@pytest.mark.parametrize("energy", [0, 1, 2])
+def test_two_cubes_per_step_parity(gpu_ctx, oracle, energy):
+    """A cube dropped on a cube resting on the sheet: tet-tet contact between the two boundary
+    surfaces, with truncation on the impact steps."""
+    errs, _ = parity(gpu_ctx, oracle, tet_drop_spec(second_cube=True, steps=25, energy=energy))
+    print(f"\n[tet contact, two cubes, energy {energy}] worst per-step rel err {errs.max():.3e}", flush=True)
+    assert errs.max() < TOL, f"step {errs.argmax()}: {errs.max():.3e}"
+
+
+def test_soft_tet_apex_closed_form(gpu_ctx):
+    """The oracle pin on the device: a soft tet's apex at d over a large fixed triangle moves to
+    (k r + M d) / (M + k) in one Newton step."""
+    d, k, r, L = 0.004, 100.0, 0.01, 0.05
+    big = np.array([[-10.0, 0.0, -10.0], [10.0, 0.0, -10.0], [0.0, 0.0, 10.0]])
+    tet_pos = np.array([[0.0, d, 0.0], [L, d + L, 0.0], [-L, d + L, -L], [0.0, d + L, L]])
+    vol = np.linalg.det(np.stack([tet_pos[1] - tet_pos[0], tet_pos[2] - tet_pos[0], tet_pos[3] - tet_pos[0]])) / 6
+    density = 1000.0 * DT ** 2 * 4 / vol
+    sim = VBDPhysics(gpu_ctx)
+    sim.setPhysicsParams(**physics_dict(gravity=(0.0, 0.0, 0.0), time_step=DT, num_substeps=1, iterations=1,
+                                        world_lower=(-1e7, -1e7, -1e7), handle_collision=1, contact_stiffness=k,
+                                        contact_radius=r, thickness=0.0, conservative_step_relaxation=0.45,
+                                        contact_energy_type=0, contact_query_radius=0.02))
+    sim.addClothMesh(big, np.array([[0, 2, 1]], np.int32), fixed_points=[0, 1, 2],
+                     vertex_colors=np.arange(3, dtype=np.int32), density=1.0, mu=0.0, lmbd=0.0)
+    sim.addTetMesh(tet_pos, np.array([[0, 1, 2, 3]], np.int32), vertex_colors=np.arange(4, dtype=np.int32),
+                   density=density, mu=1e-9, lmbd=1e-9)
+    sim.initialize()
+    sim.runStep()
+    x, _ = sim.getState(1)
+    want = tet_pos.copy()
+    want[0, 1] = (k * r + 1000.0 * d) / (1000.0 + k)
+    assert rel_err(x, want) < TOL
+    sim.destroy()
+
+
+def test_contact_scene_without_contacts_equals_plain_scene_bitwise(gpu_ctx):
+    """One vertex per colour, a contact radius below every distance and a query radius that keeps
+    every step inside its bound: the contact path (boundary-surface detection, positionsNext and
+    commit) steps to the same bits as the plain tet path."""
+    pos, tets = scenes.cube_tet_mesh(2, 0.05, origin=(0.0, 0.1, 0.0))
+    pos = pos + np.random.default_rng(3).normal(scale=0.003, size=pos.shape)
+    colors = np.arange(len(pos), dtype=np.int32)
+    mat = dict(density=1000.0, mu=1e5, lmbd=1e6, damping_hydrostatic=0.01, damping_deviatoric=0.02)
+    out = []
+    for contact in (1, 0):
+        sim = VBDPhysics(gpu_ctx)
+        sim.setPhysicsParams(**physics_dict(gravity=(0.0, -9.8, 0.0), time_step=DT, num_substeps=1, iterations=4,
+                                            world_lower=(-1e7, 0.0, -1e7), handle_collision=contact,
+                                            contact_radius=1e-7, contact_query_radius=1.0))
+        sim.addTetMesh(pos, tets, vertex_colors=colors, **mat)
+        sim.initialize()
+        sim.runSteps(3)
+        out.append(sim.getState(0))
+        sim.destroy()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_tet_contact_json_default_off(gpu_ctx, tmp_path):
+    """A tet-only scene loaded without handleCollision keeps the tet framework's default (off):
+    it records a HIP graph, which a scene with collision handling cannot."""
+    pos, tets = scenes.cube_tet_mesh(2, 0.1, origin=(0.0, 0.5, 0.0))
+    models, params = scenes.write_scene_json(str(tmp_path), "cube", pos, tets,
+                                             physics_dict(iterations=2, num_substeps=1),
+                                             dict(density=1000.0, mu=1e5, lmbd=1e6))
+    sim = VBDPhysics(gpu_ctx)
+    sim.loadRunningparameters(models, params, repo_root=str(tmp_path))
+    sim.initialize()
+    sim.recordVBDSolveGraph(True)
+    sim.runStep()
+    sim.destroy()

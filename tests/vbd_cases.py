@@ -196,3 +196,49 @@ def self_contact_spec(steps=20, energy=1, iterations=8):
                            contact_energy_type=energy, contact_query_radius=0.02, contact_bvh_rebuild_steps=8)
     return dict(physics=physics, steps=steps,
                 meshes=[dict(pos=pos, faces=faces, material=material, fixed=[], vel=vel, coloring="mcs")])
+
+
+def tet_drop_spec(steps=20, energy=1, stiffness=300.0, density=30.0, iterations=16, second_cube=False, vy=-0.3):
+    """Tet bodies with contact on their boundary surface: a soft cube (3^3 cells of 4 cm) falling
+    onto a fixed, slightly tilted cloth sheet 1.2 contact radii below; optionally a second cube
+    above it (tet-tet contact). With these defaults the cubes come to rest inside the contact
+    radius without touching (bottom 7-8 mm above the sheet, 2-7 mm between the cubes); a softer
+    contact or fewer iterations leave the per-step solve too far from equilibrium to hold the
+    weight (8 iterations settle a cube at 7.3 mm where 128 reach the 9.9 mm equilibrium)."""
+    lo, flo = scenes.cloth_sheet(12, 0.6, 0.0, origin=(-0.3, -0.3))
+    lo[:, 1] = 0.02 * lo[:, 0]
+    m_lo = dict(density=1.0, mu=1e4, lmbd=1e4, bending_stiffness=1e-4, friction_dynamic=0.2, friction_eps_v=0.01)
+    cp, ct = scenes.cube_tet_mesh(3, 0.04, origin=(-0.06, 0.0, -0.06))
+    cp[:, 1] += 0.02 * cp[:, 0] * (cp[:, 1] < 1e-9) + 0.012  # bottom layer follows the tilt
+    mat = dict(density=density, mu=2e4, lmbd=1e5, friction_dynamic=0.2, friction_eps_v=0.01)
+    meshes = [dict(pos=lo, faces=flo, material=m_lo, fixed=list(range(len(lo))), vel=None, coloring="mcs"),
+              dict(pos=cp, tets=ct, material=mat, fixed=[], vel=np.tile([0.02, vy, 0.0], (len(cp), 1)),
+                   coloring="mcs")]
+    if second_cube:
+        cp2 = cp + np.array([0.01, 0.12 + 0.012, 0.0])
+        meshes.append(dict(pos=cp2, tets=ct, material=mat, fixed=[], vel=np.tile([0.0, vy, 0.0], (len(cp), 1)),
+                           coloring="mcs"))
+    physics = physics_dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=iterations,
+                           world_lower=(-1e7, -1e7, -1e7), handle_collision=1, contact_stiffness=stiffness,
+                           contact_radius=0.01, thickness=0.0, conservative_step_relaxation=0.45,
+                           contact_energy_type=energy, contact_query_radius=0.02, contact_bvh_rebuild_steps=4)
+    return dict(physics=physics, steps=steps, meshes=meshes)
+
+
+def build_oracle_mixed(oracle, spec, precision=32):
+    """Oracle scene of cloth and tet meshes with collision handling."""
+    import gaia_amd
+
+    sc = oracle.ClothScene(spec["physics"], precision=precision)
+    for m in spec["meshes"]:
+        pos = np.asarray(m["pos"], np.float32).astype(np.float64) if precision == 32 else m["pos"]
+        vel = None if m["vel"] is None else (np.asarray(m["vel"], np.float32).astype(np.float64)
+                                             if precision == 32 else m["vel"])
+        if "faces" in m:
+            colors = gaia_amd.color_tri_mesh(len(m["pos"]), m["faces"], m["coloring"])
+            sc.add_mesh(pos, m["faces"], m["material"], colors, fixed=m["fixed"], vel=vel)
+        else:
+            colors = gaia_amd.color_tet_mesh(len(m["pos"]), m["tets"], m["coloring"])
+            sc.add_tet_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
+    sc.finalize()
+    return sc
