@@ -11,7 +11,8 @@ Poisson step requests continuously batched by the serving loop) reports served
 throughput with p50/p99 step latency.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 256] [--seed 0] [--schedule KEY]
-       python bench.py --workload detector | serve   (the detector alone; the served leg alone)
+       python bench.py --workload detector | serve | contact   (the detector alone; the served leg
+                                                               alone; tet bodies in contact)
 """
 import argparse
 import json
@@ -209,6 +210,132 @@ def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
     ctx.close()
 
 
+def contact_bench(args):
+    """Tet bodies in contact: 32 bodies (16 subdivided cubes and 16 tet balls of 8,640 tets each;
+    70,304 vertices, 276,480 tets) falling in a 2 m box, dt = 1/120, 20 iterations, contact
+    radius 2 mm on every body's boundary surface (VF/FV/EE energies, friction, conservative-bound
+    truncation, re-detection after truncating iterations). One step = one frame of the scene."""
+    import gaia_amd
+    from gaia_amd import Batch, VBDPhysics, scenes
+    from gaia_amd.autotune import parse_schedule_key, schedule_key
+
+    t0 = time.perf_counter()
+    bodies = scenes.c2_bodies(seed=args.seed)
+    phys = scenes.c2_physics()
+    ctx = gaia_amd.Context(0)
+    sim = VBDPhysics(ctx)
+    sim.setPhysicsParams(**phys)
+    colors = {}
+    for b in bodies:
+        key = (b["kind"], len(b["tets"]))
+        if key not in colors:
+            colors[key] = gaia_amd.color_tet_mesh(len(b["pos"]), b["tets"], "greedy")
+        sim.addTetMesh(b["pos"], b["tets"], vertex_colors=colors[key], **b["material"])
+    sim.initialize()
+    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(args.schedule or SERVE_SCHEDULE)}
+    sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group", stage_records=sched.get("stage_records", 0))
+    nverts = sum(len(b["pos"]) for b in bodies)
+    ntets = sum(len(b["tets"]) for b in bodies)
+    iters = phys["iterations"]
+    log(f"contact: {len(bodies)} bodies, {nverts} vertices, {ntets} tets, built in {time.perf_counter() - t0:.1f}s; "
+        f"device memory {ctx.memory_bytes() / 2**30:.2f} GiB")
+    for _ in range(args.warmup):
+        sim.runStep()
+    ctx.synchronize()
+    st0 = sim.contactStats()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        sim.runStep()
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    st1 = sim.contactStats()
+    n = args.steps
+    rate = nverts * iters * n / wall
+    detect_ms = (st1["detect_ms"] - st0["detect_ms"]) / n
+    detections = (st1["detections"] - st0["detections"]) / n
+    lowest = min(float(sim.getState(m)[0][:, 1].min()) for m in range(len(bodies)))
+    log(f"contact: {wall / n * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s, {detections:.2f} detections/step "
+        f"({detect_ms:.2f} ms/step), last detection {st1['vf_pairs']} VF / {st1['ee_pairs']} EE candidates")
+
+    # the solve kernel's launches (every parallel group, 5 times each) at the final state, timed with
+    # HIP events on the library stream; algorithmic bytes from the tet gather model (contact
+    # gathers of surface vertices are not counted, so the figure is a lower bound)
+    batch = Batch(ctx, [sim])
+    groups = batch.num_groups()
+    work = [batch.group_work(g)[0] for g in range(groups)]
+    reps = 5
+    tot_ms = 0.0
+    for g in range(groups):
+        sim.solveParallelGroup(g, 0)
+        ctx.synchronize()
+        ctx.event_record(2)
+        for _ in range(reps):
+            sim.solveParallelGroup(g, 0)
+        ctx.event_record(3)
+        tot_ms += ctx.event_elapsed_ms(2, 3)
+    batch.destroy()
+    avg_ms = tot_ms / (groups * reps)
+    avg_bytes = sum(work) / groups
+    gbs = avg_bytes / (avg_ms * 1e-3) / 1e9
+    solve_ms_per_step = avg_ms * groups * iters
+    roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                traffic=None, kernel="group_solve_kernel", avg_launch_us=round(avg_ms * 1e3, 2),
+                algorithmic_bytes_per_launch=int(avg_bytes))
+    out = {
+        "metric": "tet-contact VBD vertex-solves/sec at 20 iters",
+        "value": round(rate, 1),
+        "unit": "vertex-solves/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / n * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tet bodies in contact: 16 cubes + 16 tet balls of 8,640 tets (seed {args.seed}, seeded "
+                        f"rotations, mu log-uniform 1e4-1e6, lambda 10 mu) on a 4x4x2 lattice in a 2 m box, dt=1/120, "
+                        f"20 iterations, contact radius 2 mm, stiffness 1e4 (quadratic-log), friction 0.2, query "
+                        f"radius 2 cm, BVH rebuilt every 32 frames; frames {args.warmup + 1}-{args.warmup + n} of "
+                        f"the fall",
+            "bodies": len(bodies), "vertices": nverts, "tets": ntets, "iterations": iters,
+            "parallel_groups": groups,
+            "schedule": dict(sched, key=schedule_key(sched)),
+            "detections_per_step": round(detections, 3),
+            "detect_ms_per_step": round(detect_ms, 3),
+            "solve_ms_per_step": round(solve_ms_per_step, 3),
+            "vf_candidates_last": st1["vf_pairs"], "ee_candidates_last": st1["ee_pairs"],
+            "lowest_vertex_y": round(lowest, 4),
+        },
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # the oracle (FP32) on the host cores: one frame of the same scene from its initial state
+        from oracle import pyoracle
+
+        from gaia_amd.vbd import physics_dict
+
+        threads = min(16, os.cpu_count() or 1)
+        sc = pyoracle.ClothScene(physics_dict(**phys), precision=32)
+        for b in bodies:
+            sc.add_tet_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
+                            colors[(b["kind"], len(b["tets"]))])
+        sc.finalize()
+        t = time.perf_counter()
+        sc.step(1, threads=threads)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(nverts * iters / cpu_s, 1), "unit": "vertex-solves/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "the first frame of the same scene, FP32 oracle (detection included)"}
+    log(f"contact: {out}")
+    sim.destroy()
+    ctx.close()
+    return out
+
+
 def serve_bench(args, schedule=None):
     """Served VBD: Poisson-arriving step requests over a Zipf-sized population of scenes,
     continuously batched by the native serving loop (one frame of every ready scene per tick).
@@ -360,8 +487,9 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size of the copy-bandwidth calibration")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels directly instead of replaying HIP graphs (profiler runs)")
-    ap.add_argument("--workload", choices=["batched", "detector", "serve"], default="batched",
-                    help="batched VBD (default), the contact detector alone on 64 tori, or served VBD")
+    ap.add_argument("--workload", choices=["batched", "detector", "serve", "contact"], default="batched",
+                    help="batched VBD (default), the contact detector alone on 64 tori, served VBD, or tet "
+                         "bodies in contact (32 bodies of 8,640 tets)")
     ap.add_argument("--serve-scenes", type=int, default=2000)
     ap.add_argument("--requests", type=int, default=20000, help="served VBD: step requests in the timed window")
     ap.add_argument("--rate", type=float, default=20000.0, help="served VBD: Poisson request rate per second")
@@ -379,6 +507,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(serve_bench(args)), flush=True)
+        return
+    if args.workload == "contact":
+        print(json.dumps(contact_bench(args)), flush=True)
         return
 
     import gaia_amd

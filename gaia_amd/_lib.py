@@ -130,6 +130,19 @@ class Schedule(ctypes.Structure):
     ]
 
 
+class ContactStats(ctypes.Structure):
+    """gvbd_contact_stats."""
+    _fields_ = [
+        ("frames", ctypes.c_int64),
+        ("iterations", ctypes.c_int64),
+        ("detections", ctypes.c_int64),
+        ("rebuilds", ctypes.c_int64),
+        ("detect_ms", ctypes.c_double),
+        ("vf_pairs", ctypes.c_int64),
+        ("ee_pairs", ctypes.c_int64),
+    ]
+
+
 class ServerStats(ctypes.Structure):
     """gvbd_server_stats."""
     _fields_ = [
@@ -190,6 +203,7 @@ _SIGNATURES = {
     "gvbd_scene_run_iteration": [_P, _I, _I],
     "gvbd_scene_update_velocities": [_P, _I],
     "gvbd_scene_collision_detection": [_P, _I],
+    "gvbd_scene_contact_stats": [_P, _I, ctypes.POINTER(ContactStats)],
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
     "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],

@@ -13,9 +13,10 @@ bool valid_stage(int lpv, int block, int stage);
 
 // One parallel group: item i covers `count` consecutive colour-major slots of one scene,
 // tet or cloth by item kind. Tet friction applies when iteration >= frictionStartIter
-// (VBDPhysics.cpp:1610).
+// (VBDPhysics.cpp:1610). `contact` selects the variant with contact forces and bound truncation
+// (scenes with collision handling; stage must then be 0).
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, int stage, hipStream_t stream);
+                              int block, int stage, bool contact, hipStream_t stream);
 // Device arguments of the persistent work-queue solve of a plan.
 struct QueueArgs {
     const SceneView* scenes;

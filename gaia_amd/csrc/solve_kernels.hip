@@ -120,13 +120,30 @@ __device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs
     }
 }
 
-__device__ void contact_forces(const SceneView& S, int v, const int* faceRef, int fb, int fe, const int* edgeRef,
-                               int eb, int ee, f3& f, Sym3& h);
+template <int LPV>
+__device__ void contact_forces(const SceneView& S, int v, int sub, const int* faceRef, int fb, int fe,
+                               const int* edgeRef, int eb, int ee, f3& f, Sym3& h);
+
+// Sums a vertex's contact force and Hessian over its LPV lanes (every lane gets the sums).
+template <int LPV>
+__device__ __forceinline__ void lane_sum_contact(f3& f, Sym3& h) {
+    if (LPV > 1) {
+        f.x = lane_sum<LPV>(f.x);
+        f.y = lane_sum<LPV>(f.y);
+        f.z = lane_sum<LPV>(f.z);
+        h.xx = lane_sum<LPV>(h.xx);
+        h.xy = lane_sum<LPV>(h.xy);
+        h.xz = lane_sum<LPV>(h.xz);
+        h.yy = lane_sum<LPV>(h.yy);
+        h.yz = lane_sum<LPV>(h.yz);
+        h.zz = lane_sum<LPV>(h.zz);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Tet vertex block (VBDPhysics.cpp:2402-2483)
 // ---------------------------------------------------------------------------
-template <int LPV, int STAGE>
+template <int LPV, int STAGE, bool CONTACT>
 __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it, int iteration, float4* lds) {
     const int local = threadIdx.x / LPV;
     const int sub = threadIdx.x % LPV;
@@ -202,6 +219,17 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         gzz = lane_sum<LPV>(gzz);
         cmu = lane_sum<LPV>(cmu);
     }
+    // collision handling: cloth-model contact on the mesh's boundary surface, each of the
+    // vertex's lanes taking a share of its contact pairs (interior vertices have empty ranges)
+    f3 cf = mk(0.f, 0.f, 0.f);
+    Sym3 ch{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (CONTACT) {
+        if (active) {
+            const int4 tc = S.tcontact[it.slotBegin + local];
+            contact_forces<LPV>(S, v, sub, S.cv.tFaceRef, tc.x, tc.y, S.cv.tEdgeRef, tc.z, tc.w, cf, ch);
+        }
+        lane_sum_contact<LPV>(cf, ch);
+    }
     if (!active || sub != 0) return;
 
     const SolveParams& sp = S.sp;
@@ -229,12 +257,10 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
     h.zz += hi;
     if (sp.usePlaneGround)
         boundary_force(sp, sp.boundaryStiffness, sp.frictionEpsU, x, xp, iteration >= sp.frictionStartIter, f, h);
-    // collision handling: cloth-model contact on the mesh's boundary surface (interior vertices
-    // have empty reference ranges and an unbounded step)
-    int4 tc = make_int4(0, 0, 0, 0);
-    if (S.hasContact) {
-        tc = S.tcontact[it.slotBegin + local];
-        contact_forces(S, v, S.cv.tFaceRef, tc.x, tc.y, S.cv.tEdgeRef, tc.z, tc.w, f, h);
+    if constexpr (CONTACT) {
+        f = f + cf;
+        h.xx += ch.xx; h.xy += ch.xy; h.xz += ch.xz;
+        h.yy += ch.yy; h.yz += ch.yz; h.zz += ch.zz;
     }
 
     f3 xn = x;
@@ -248,7 +274,7 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         }
         xn = x + step * dx;
     }
-    if (S.hasContact) {
+    if constexpr (CONTACT) {
         // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246). The tet initial
         // guess is not bounded, so every vertex is truncated, moved or not (interior vertices
         // have an unbounded step).
@@ -487,30 +513,42 @@ __device__ void ee_contact(const SceneView& S, int e1, int e2, int order, f3& f,
 
 // VBDClothPhysics.cpp:1033-1094: the vertex's own VF contacts, the FV contacts of its incident
 // faces (faceRef[fb, fe)) and the EE contacts of its incident edges (edgeRef[eb, ee))
-__device__ void contact_forces(const SceneView& S, int v, const int* faceRef, int fb, int fe, const int* edgeRef,
-                               int eb, int ee, f3& f, Sym3& h) {
+// A vertex's contact pairs (VBDClothPhysics.cpp:1033-1094): its own VF list, the vertex lists of
+// its incident faces (FV) and the edge lists of its incident edges (EE). Lane `sub` of the
+// vertex's LPV lanes takes every LPV-th pair of each of the three sequences (the FV and EE
+// sequences run over all incident primitives back to back), so the pairs spread evenly however
+// they fall on the primitives; LPV = 1 is the reference's serial order.
+template <int LPV>
+__device__ void contact_forces(const SceneView& S, int v, int sub, const int* faceRef, int fb, int fe,
+                               const int* edgeRef, int eb, int ee, f3& f, Sym3& h) {
     const ContactView& C = S.cv;
     const int nvf = C.vfCount[v];
-    for (int k = 0; k < nvf; ++k) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, f, h);
+    for (int k = sub; k < nvf; k += LPV) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, f, h);
+    int t = sub, base = 0;
     for (int r = fb; r < fe; ++r) {
         const int ref = faceRef[r];
         const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int2 range = C.fvRange[face];
-        for (int i = range.x; i < range.y; ++i) vf_contact(S, (int)(C.fvKeys[i] & 0xFFFFFFFFu), face, order, f, h);
+        const int end = base + (range.y - range.x);
+        for (; t < end; t += LPV) vf_contact(S, (int)(C.fvKeys[range.x + (t - base)] & 0xFFFFFFFFu), face, order, f, h);
+        base = end;
     }
+    t = sub;
+    base = 0;
     for (int r = eb; r < ee; ++r) {
         const int ref = edgeRef[r];
         const int e = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
-        const int n = C.eeCount[e];
-        for (int k = 0; k < min(n, C.eeCap); ++k)
-            ee_contact(S, e, __float_as_int(C.eeRec[(int64_t)e * C.eeCap + k].x), order, f, h);
+        const int end = base + min(C.eeCount[e], C.eeCap);
+        for (; t < end; t += LPV)
+            ee_contact(S, e, __float_as_int(C.eeRec[(int64_t)e * C.eeCap + (t - base)].x), order, f, h);
+        base = end;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Cloth vertex block (VBDClothPhysics.cpp:1026-1154)
 // ---------------------------------------------------------------------------
-template <int LPV, int STAGE>
+template <int LPV, int STAGE, bool CONTACT>
 __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& it, float4* lds) {
     const int local = threadIdx.x / LPV;
     const int sub = threadIdx.x % LPV;
@@ -607,6 +645,13 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
         fb2 = lane_sum<LPV>(fb2);
         hb = lane_sum<LPV>(hb);
     }
+    // contact pairs, shared over the vertex's lanes
+    f3 cf = mk(0.f, 0.f, 0.f);
+    Sym3 ch{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (CONTACT) {
+        if (active) contact_forces<LPV>(S, v, sub, S.cv.faceRef, sa.y, sa.z, S.cv.edgeRef, sb.z, sb.w, cf, ch);
+        lane_sum_contact<LPV>(cf, ch);
+    }
     if (!active || sub != 0) return;
 
     const SolveParams& sp = S.sp;
@@ -636,12 +681,16 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     h.yy += hi;
     h.zz += hi;
     if (sp.usePlaneGround) boundary_force(sp, sp.clothBoundaryStiffness, sp.clothFrictionEpsU, x, xp, true, f, h);
-    if (S.hasContact) contact_forces(S, v, S.cv.faceRef, sa.y, sa.z, S.cv.edgeRef, sb.z, sb.w, f, h);
+    if constexpr (CONTACT) {
+        f = f + cf;
+        h.xx += ch.xx; h.xy += ch.xy; h.xz += ch.xz;
+        h.yy += ch.yy; h.yz += ch.yz; h.zz += ch.zz;
+    }
     f3 dx;
     f3 xn = x;
     if (solve_sym3(h, f, dx)) {
         xn = x + sp.clothStepSize * dx;
-        if (S.hasContact) {
+        if constexpr (CONTACT) {
             // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246); any truncation
             // requests detection before the next iteration
             const f3 x0 = xyz(S.cv.xAtCD[v]);
@@ -664,16 +713,18 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
 // work kind is uniform, so the branch does not diverge.
 // STAGE > 0 stages the incidence records through LDS, STAGE 64-byte records (or 2 * STAGE cloth
 // records of 32 bytes) per chunk.
-template <int LPV, int BLOCK, int STAGE>
+// CONTACT selects the variant for scenes with collision handling (contact forces, bound
+// truncation); the others carry none of that code, so their register use is unchanged by it.
+template <int LPV, int BLOCK, int STAGE, bool CONTACT>
 __global__ __launch_bounds__(BLOCK) void group_solve_kernel(const SceneView* __restrict__ scenes,
                                                            const WorkItem* __restrict__ items, int iteration) {
     __shared__ float4 lds[STAGE > 0 ? 4 * STAGE : 1];
     const WorkItem it = items[blockIdx.x];
     const SceneView& S = scenes[it.scene];
     if (it.kind == WORK_TET)
-        tet_block<LPV, STAGE>(S, it, iteration, lds);
+        tet_block<LPV, STAGE, CONTACT>(S, it, iteration, lds);
     else
-        cloth_block<LPV, STAGE>(S, it, lds);
+        cloth_block<LPV, STAGE, CONTACT>(S, it, lds);
 }
 
 // After a group's cloth solves: x = positionsNext for the group's cloth vertices
@@ -759,9 +810,9 @@ __global__ __launch_bounds__(BLOCK) void queue_solve_kernel(const SceneView* __r
             for (int p = 0; p < e.count; p += VPB) {
                 const WorkItem it{e.scene, e.slotBegin + p, min(VPB, e.count - p), e.kind};
                 if (e.kind == WORK_TET)
-                    tet_block<LPV, 0>(S, it, iteration, nullptr);
+                    tet_block<LPV, 0, false>(S, it, iteration, nullptr);
                 else
-                    cloth_block<LPV, 0>(S, it, nullptr);
+                    cloth_block<LPV, 0, false>(S, it, nullptr);
             }
         }
         // every thread's stores happen before lane 0's agent-scope release (the completion count
@@ -863,11 +914,11 @@ __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* _
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int LPV, int BLOCK, int STAGE>
+template <int LPV, int BLOCK, int STAGE, bool CONTACT>
 static hipError_t launch_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
                                hipStream_t stream) {
-    hipLaunchKernelGGL((group_solve_kernel<LPV, BLOCK, STAGE>), dim3(nitems), dim3(BLOCK), 0, stream, scenes, items,
-                       iteration);
+    hipLaunchKernelGGL((group_solve_kernel<LPV, BLOCK, STAGE, CONTACT>), dim3(nitems), dim3(BLOCK), 0, stream, scenes,
+                       items, iteration);
     return hipGetLastError();
 }
 
@@ -875,19 +926,22 @@ static hipError_t launch_solve(const SceneView* scenes, const WorkItem* items, i
 #define GVBD_STAGED(X) X(4, 128) X(4, 256) X(8, 128) X(8, 256) X(16, 128) X(16, 256)
 
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, int stage, hipStream_t stream) {
+                              int block, int stage, bool contact, hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
-#define GVBD_CASE(L, B) \
-    if (stage == 0 && lpv == L && block == B) return launch_solve<L, B, 0>(scenes, items, nitems, iteration, stream);
+#define GVBD_CASE(L, B)                                                                                        \
+    if (stage == 0 && lpv == L && block == B)                                                                  \
+        return contact ? launch_solve<L, B, 0, true>(scenes, items, nitems, iteration, stream)                 \
+                       : launch_solve<L, B, 0, false>(scenes, items, nitems, iteration, stream);
     GVBD_CASE(1, 64) GVBD_CASE(1, 128) GVBD_CASE(1, 256)
     GVBD_CASE(2, 64) GVBD_CASE(2, 128) GVBD_CASE(2, 256)
     GVBD_CASE(4, 64) GVBD_CASE(4, 128) GVBD_CASE(4, 256)
     GVBD_CASE(8, 64) GVBD_CASE(8, 128) GVBD_CASE(8, 256)
     GVBD_CASE(16, 64) GVBD_CASE(16, 128) GVBD_CASE(16, 256)
 #undef GVBD_CASE
+    if (contact) return hipErrorInvalidValue;  // staged variants carry no contact code
 #define GVBD_CASE(L, B)                                                                                  \
-    if (lpv == L && block == B && stage == 32) return launch_solve<L, B, 32>(scenes, items, nitems, iteration, stream); \
-    if (lpv == L && block == B && stage == 64) return launch_solve<L, B, 64>(scenes, items, nitems, iteration, stream);
+    if (lpv == L && block == B && stage == 32) return launch_solve<L, B, 32, false>(scenes, items, nitems, iteration, stream); \
+    if (lpv == L && block == B && stage == 64) return launch_solve<L, B, 64, false>(scenes, items, nitems, iteration, stream);
     GVBD_STAGED(GVBD_CASE)
 #undef GVBD_CASE
     return hipErrorInvalidValue;

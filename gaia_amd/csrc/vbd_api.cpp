@@ -83,6 +83,8 @@ struct SceneObj {
     // cloth collision handling (handle_collision): detector over the scene's positions
     std::unique_ptr<ContactDetector> cd;
     int64_t frame = 0;
+    gvbd_contact_stats contactStats{};       // gvbd_scene_contact_stats
+    hipEvent_t cdEvents[2] = {nullptr, nullptr};  // around a frame's detection pass
     // serving loop residency: the server that owns it (-1: none) and, while copied out to host
     // memory, the arena's bytes and the device address they were copied from
     int32_t server = -1;
@@ -761,7 +763,7 @@ void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
 
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
     const int b = p.groupItemBegin[g], e = p.groupItemBegin[g + 1];
-    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, p.stage, ctx->stream),
+    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, p.stage, p.contact, ctx->stream),
               "group solve launch");
     const int cb = p.groupCommitBegin[g], ce = p.groupCommitBegin[g + 1];
     if (ce > cb) hip_check(launch_cloth_commit(p.dViews, p.dCommit + cb, ce - cb, ctx->stream), "commit launch");
@@ -792,7 +794,7 @@ void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
     }
 }
 
-void scene_collision_detection(SceneObj& s, bool firstIteration);
+bool scene_collision_detection(SceneObj& s, bool firstIteration);
 
 // One frame of a cloth scene with collision handling (VBDClothPhysics.cpp:389-518): detection
 // before iteration 0 (ahead of the initial guess) and before every iteration after one in which
@@ -803,16 +805,33 @@ void plan_contact_frame(gvbd_context* ctx, PlanObj& p) {
     for (int sub = 0; sub < p.substeps; ++sub) {
         bool required = true;
         for (int it = 0; it < p.iterations; ++it) {
-            if (it == 0 || required) scene_collision_detection(s, it == 0);
+            const bool detect = it == 0 || required;
+            if (detect) {
+                if (!s.cdEvents[0]) {
+                    hip_check(hipEventCreate(&s.cdEvents[0]), "event create");
+                    hip_check(hipEventCreate(&s.cdEvents[1]), "event create");
+                }
+                hip_check(hipEventRecord(s.cdEvents[0], ctx->stream), "event record");
+                if (scene_collision_detection(s, it == 0)) ++s.contactStats.rebuilds;
+                hip_check(hipEventRecord(s.cdEvents[1], ctx->stream), "event record");
+                ++s.contactStats.detections;
+            }
             if (it == 0) plan_initial_step(ctx, p);
             plan_iteration(ctx, p, it, it);
             int flag = 0;
             hip_check(hipMemcpyAsync(&flag, dv.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "flag read");
             hip_check(hipStreamSynchronize(ctx->stream), "flag sync");
             required = flag != 0;
+            if (detect) {
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, s.cdEvents[0], s.cdEvents[1]), "event time");
+                s.contactStats.detect_ms += ms;
+            }
         }
         if (p.iterations == 0) plan_initial_step(ctx, p);
         plan_velocity(ctx, p);
+        ++s.contactStats.frames;
+        s.contactStats.iterations += p.iterations;
     }
     ++s.frame;
 }
@@ -986,7 +1005,8 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
 // applyCollisionDetection (VBDClothPhysics.cpp:719-819): BVH refit (full rebuild every
 // contact_bvh_rebuild_steps frames at the first iteration), VF/EE queries, conservative bounds,
 // face back-links (FV_CONTACT_PREALLOCATE = 32 per face), positions at detection.
-void scene_collision_detection(SceneObj& s, bool firstIteration) {
+// Returns whether the BVH was rebuilt (a refit otherwise).
+bool scene_collision_detection(SceneObj& s, bool firstIteration) {
     const int steps = s.params.contact_bvh_rebuild_steps;
     const bool rebuild = firstIteration && steps > 0 && s.frame > 0 && s.frame % steps == 0;
     s.cd->updateBvh(rebuild);
@@ -994,6 +1014,7 @@ void scene_collision_detection(SceneObj& s, bool firstIteration) {
     s.cd->bounds(s.params.thickness, s.params.conservative_step_relaxation);
     s.cd->buildFaceLinks(32);
     s.cd->snapshotPositions();
+    return rebuild;
 }
 
 template <class F> int guard(gvbd_context* ctx, F&& f) {
@@ -1176,6 +1197,11 @@ int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
             ctx->plans.erase(s.plan);
         }
         s.cd.reset();  // reads the arena's positions: released first
+        for (hipEvent_t& e : s.cdEvents)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
         if (s.dev.arena) hip_check(hipFree(s.dev.arena), "hipFree");
         ctx->scenes.erase(scene);
     });
@@ -1396,6 +1422,7 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
             "LDS staging applies to group launches only");
     require(!(p.contact && sc.mode == GVBD_MODE_WORK_QUEUE),
             "scenes with collision handling run with group launches (the host decides each iteration's detection)");
+    require(!(p.contact && sc.stage_records != 0), "LDS staging is not available for scenes with collision handling");
     p.lpv = sc.lanes_per_vertex;
     p.stage = sc.stage_records;
     p.block = sc.block;
@@ -1497,6 +1524,22 @@ int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene) {
         SceneObj& s = scene_of(ctx, scene);
         if (!s.cd) fail(GVBD_ERR_STATE, "scene has no collision handling");
         scene_collision_detection(s, true);
+    });
+}
+
+int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out) {
+    return guard(ctx, [&] {
+        require(ctx && out, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.cd) fail(GVBD_ERR_STATE, "scene has no collision handling");
+        *out = s.contactStats;
+        out->vf_pairs = out->ee_pairs = 0;
+        if (s.contactStats.detections > 0) {
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+            const DetectorResultCounts c = s.cd->counts();
+            out->vf_pairs = c.vfPairs;
+            out->ee_pairs = c.eeContacts;
+        }
     });
 }
 
@@ -1676,7 +1719,7 @@ int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t gr
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
         const int b = p.groupItemBegin[group], e = p.groupItemBegin[group + 1];
-        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, p.stage, ctx->stream),
+        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, p.stage, p.contact, ctx->stream),
                   "group solve launch");
     });
 }

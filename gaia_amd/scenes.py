@@ -280,6 +280,56 @@ def random_rotation(rng):
     return rz @ ry @ rx
 
 
+def tet_ball(n=12, radius=0.15, dtype=np.float64):
+    """Tet ball: the n^3-cell tet cube on [-1, 1]^3 mapped onto the ball by
+    (x, y, z) -> (x sqrt(1 - y^2/2 - z^2/2 + y^2 z^2/3), and cyclically), a smooth bijection of the
+    cube onto the ball whose Jacobian stays positive, so every tet keeps a positive volume.
+    (n+1)^3 vertices and 5 n^3 tets; the boundary is the mapped cube surface."""
+    pos, tets = cube_tet_mesh(n, 2.0 / n, origin=(-1.0, -1.0, -1.0))
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    x2, y2, z2 = x * x, y * y, z * z
+    ball = np.stack([x * np.sqrt(1 - y2 / 2 - z2 / 2 + y2 * z2 / 3),
+                     y * np.sqrt(1 - z2 / 2 - x2 / 2 + z2 * x2 / 3),
+                     z * np.sqrt(1 - x2 / 2 - y2 / 2 + x2 * y2 / 3)], axis=1)
+    return (radius * ball).astype(dtype), tets
+
+
+def c2_bodies(seed=0, n=12, cube_size=0.28, ball_radius=0.14, spacing=0.5, layers_y=(0.3, 0.8)):
+    """Tet bodies in contact: 32 bodies on a 4 x 4 x 2 lattice (spacing 0.5 m, layers at 0.3 m and
+    0.8 m) in a 2 m box, alternating 16 subdivided cubes (n^3 cells, edge cube_size) and 16 tet
+    balls (tet_ball(n), radius ball_radius), each with 5 n^3 tets (8,640 at n = 12) and a seeded
+    rotation; shear modulus log-uniform in [1e4, 1e6] Pa, lambda = 10 mu, density 1000 kg/m^3,
+    contact friction 0.2. The sizes keep every rotated body clear of its neighbours.
+    Returns a list of dicts (pos, tets, material)."""
+    rng = np.random.default_rng(seed)
+    cube, cube_tets = cube_tet_mesh(n, cube_size / n, origin=(-cube_size / 2,) * 3)
+    ball, ball_tets = tet_ball(n, ball_radius)
+    bodies = []
+    coords = [(-0.75 + spacing * i, y, -0.75 + spacing * k) for y in layers_y for k in range(4) for i in range(4)]
+    for b, centre in enumerate(coords):
+        i, k, layer = b % 4, (b // 4) % 4, b // 16
+        is_cube = (i + k + layer) % 2 == 0
+        base, tets = (cube, cube_tets) if is_cube else (ball, ball_tets)
+        rot = random_rotation(rng)
+        mu = float(10.0 ** rng.uniform(4.0, 6.0))
+        pos = base @ rot.T + np.asarray(centre)
+        # a rotation is proper, so tet orientation (and positive volume) is kept
+        bodies.append(dict(pos=pos, tets=tets, kind="cube" if is_cube else "ball",
+                           material=dict(density=1000.0, mu=mu, lmbd=10.0 * mu, friction_dynamic=0.2,
+                                         friction_eps_v=0.01)))
+    return bodies
+
+
+def c2_physics(iterations=20):
+    """Physics of the tet-contact workload: dt = 1/120, one substep, contact radius 2 mm,
+    contact stiffness 1e4 (the cloth framework's default) with the quadratic-log energy, query
+    radius 2 cm, BVH rebuilt every 32 steps (refit between), box boundary of 2 m."""
+    return dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 120.0, num_substeps=1, iterations=iterations,
+                world_lower=(-1.0, 0.0, -1.0), world_upper=(1.0, 2.0, 1.0), handle_collision=1,
+                contact_stiffness=1e4, contact_radius=0.002, thickness=0.0, conservative_step_relaxation=0.45,
+                contact_energy_type=1, contact_query_radius=0.02, contact_bvh_rebuild_steps=32)
+
+
 def c3_tori(seed=0, grid=4, nu=128, nv=64, spacing=0.3, major=0.12, minor=0.04, drift=0.005):
     """Collision-detection workload: grid^3 tori (default 64 x 16,384 triangles, edges of about
     4 mm) on a lattice with `spacing` between centres (neighbouring tori of outer radius 0.16 m

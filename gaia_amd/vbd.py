@@ -12,8 +12,8 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import (MODE_GROUP_LAUNCHES, MODE_WORK_QUEUE, ClothMeshDesc, PhysicsParams, Schedule, ServedStep,
-                   ServerStats, TetMeshDesc, check, fptr, f32, i32, iptr, lib)
+from ._lib import (MODE_GROUP_LAUNCHES, MODE_WORK_QUEUE, ClothMeshDesc, ContactStats, PhysicsParams, Schedule,
+                   ServedStep, ServerStats, TetMeshDesc, check, fptr, f32, i32, iptr, lib)
 
 # Reference defaults (VBD/VBDPhysicsParameters.h:8-81, Parameters/PhysicsParameters.h:18-120).
 DEFAULT_PHYSICS = dict(
@@ -369,6 +369,13 @@ class VBDPhysics:
     def applyCollisionDetection(self):
         """VBDClothSimulationFramework::applyCollisionDetection (cloth scenes with handle_collision)."""
         self.ctx.check(lib().gvbd_scene_collision_detection(self.ctx.handle, self.scene))
+
+    def contactStats(self):
+        """Counters of a scene with handle_collision: frames, iterations, detection passes, BVH
+        rebuilds, device time of detection (ms), and the candidate counts of the last detection."""
+        st = ContactStats()
+        self.ctx.check(lib().gvbd_scene_contact_stats(self.ctx.handle, self.scene, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ContactStats._fields_}
 
     def recordVBDSolveGraph(self, enable=True):
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))

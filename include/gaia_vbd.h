@@ -222,6 +222,19 @@ int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
  * (VBDClothPhysics.cpp:735-819) at the current positions; run_step calls it itself before
  * iteration 0 and after any iteration in which a vertex was truncated to its bound */
 int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene);
+/* counters of a scene with handle_collision since its initialisation (no reference counterpart:
+ * the reference prints timings instead, VBDClothPhysics.cpp:389-518). detect_ms is device time of
+ * the detection work (BVH update, queries, bounds, face links), timed with events around it. */
+typedef struct gvbd_contact_stats {
+    int64_t frames;          /* frames (substeps) stepped with collision handling */
+    int64_t iterations;      /* solver iterations over those frames */
+    int64_t detections;      /* detection passes: one per frame plus one after each truncating iteration */
+    int64_t rebuilds;        /* full BVH rebuilds (refits otherwise) */
+    double detect_ms;        /* device time of the detection passes */
+    int64_t vf_pairs;        /* after the last detection: vertex-face candidates within the query radius */
+    int64_t ee_pairs;        /* and edge-edge candidates */
+} gvbd_contact_stats;
+int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);

@@ -55,6 +55,23 @@ def test_two_cubes_per_step_parity(gpu_ctx, oracle, energy):
     assert errs.max() < TOL, f"step {errs.argmax()}: {errs.max():.3e}"
 
 
+def test_many_bodies_error_distribution(gpu_ctx, oracle):
+    """The tet-contact workload at 3^3 cells per body (32 cubes and balls, 2,048 vertices) over 80
+    frames: bodies fall, hit the ground and each other. Stiff contact (k = 1e4 against
+    m / dt^2 of about 5e3) makes the impact steps ill-conditioned (the FP32 and FP64 oracles part
+    by up to ~2e-4 there), so the per-step errors are checked as a distribution: median at most
+    that of FP32 vs FP64, and most steps within the per-step tolerance."""
+    bodies = scenes.c2_bodies(n=3)
+    spec = dict(physics=physics_dict(**scenes.c2_physics()), steps=80,
+                meshes=[dict(pos=b["pos"], tets=b["tets"], material=b["material"], fixed=[], vel=None,
+                             coloring="greedy") for b in bodies])
+    errs, errs64 = parity(gpu_ctx, oracle, spec, with_fp64=True)
+    print(f"\n[tet contact, 32 bodies] gpu-vs-fp32 median {np.median(errs):.2e} max {errs.max():.2e}; "
+          f"fp32-vs-fp64 median {np.median(errs64):.2e} max {errs64.max():.2e}", flush=True)
+    assert np.median(errs) <= max(2.0 * np.median(errs64), 1e-6)
+    assert (errs < TOL).mean() >= 0.8
+
+
 def test_soft_tet_apex_closed_form(gpu_ctx):
     """The oracle pin on the device: a soft tet's apex at d over a large fixed triangle moves to
     (k r + M d) / (M + k) in one Newton step."""
@@ -81,10 +98,12 @@ def test_soft_tet_apex_closed_form(gpu_ctx):
     sim.destroy()
 
 
-def test_contact_scene_without_contacts_equals_plain_scene_bitwise(gpu_ctx):
+def test_contact_scene_without_contacts_matches_plain_scene(gpu_ctx):
     """One vertex per colour, a contact radius below every distance and a query radius that keeps
     every step inside its bound: the contact path (boundary-surface detection, positionsNext and
-    commit) steps to the same bits as the plain tet path."""
+    commit) steps as the plain tet path. The two are separately compiled kernel variants (the
+    plain one carries no contact code), so they agree to rounding, not bit for bit (the oracle's
+    composition equals its tet solver bit for bit)."""
     pos, tets = scenes.cube_tet_mesh(2, 0.05, origin=(0.0, 0.1, 0.0))
     pos = pos + np.random.default_rng(3).normal(scale=0.003, size=pos.shape)
     colors = np.arange(len(pos), dtype=np.int32)
@@ -100,7 +119,9 @@ def test_contact_scene_without_contacts_equals_plain_scene_bitwise(gpu_ctx):
         sim.runSteps(3)
         out.append(sim.getState(0))
         sim.destroy()
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert rel_err(out[0][0], out[1][0]) < 1e-6
+    # velocities are position differences over dt: compared on the positions' scale
+    assert np.abs(out[0][1] - out[1][1]).max() * DT / np.abs(out[1][0]).max() < 1e-6
 
 
 def test_tet_contact_json_default_off(gpu_ctx, tmp_path):

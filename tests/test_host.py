@@ -91,3 +91,24 @@ def test_errors_are_reported_not_raised_in_c():
         gaia_amd.color_tet_mesh(3, np.array([[0, 1, 2, 5]]))
     with pytest.raises(gaia_amd.VBDError, match="degenerate"):
         tet_mesh_constants(np.zeros((4, 3)), np.array([[0, 1, 2, 3]]), 1.0)
+
+
+def test_tet_contact_workload_bodies():
+    """The tet-contact workload's generator: 16 cubes and 16 balls of 8,640 positively oriented
+    tets, inside the 2 m box and at least 3 cm apart from each other (rotations included)."""
+    from scipy.spatial import cKDTree
+
+    bodies = scenes.c2_bodies()
+    assert [b["kind"] for b in bodies].count("cube") == 16 and len(bodies) == 32
+    allp = np.concatenate([b["pos"] for b in bodies])
+    label = np.concatenate([np.full(len(b["pos"]), i) for i, b in enumerate(bodies)])
+    pairs = cKDTree(allp).query_pairs(0.03, output_type="ndarray")
+    assert not np.any(label[pairs[:, 0]] != label[pairs[:, 1]])
+    assert allp[:, 1].min() > 0 and np.abs(allp[:, [0, 2]]).max() < 1.0
+    for b in bodies[:2]:
+        a = b["pos"][b["tets"]]
+        vol = np.einsum("ij,ij->i", np.cross(a[:, 1] - a[:, 0], a[:, 2] - a[:, 0]), a[:, 3] - a[:, 0])
+        assert len(b["tets"]) == 8640 and vol.min() > 0
+    mus = np.array([b["material"]["mu"] for b in bodies])
+    assert mus.min() >= 1e4 and mus.max() <= 1e6
+    assert all(b["material"]["lmbd"] == 10 * b["material"]["mu"] for b in bodies)
