@@ -222,6 +222,8 @@ def contact_bench(args):
     t0 = time.perf_counter()
     bodies = scenes.c2_bodies(seed=args.seed)
     phys = scenes.c2_physics()
+    if args.query_radius is not None:
+        phys["contact_query_radius"] = args.query_radius
     ctx = gaia_amd.Context(0)
     sim = VBDPhysics(ctx)
     sim.setPhysicsParams(**phys)
@@ -298,8 +300,8 @@ def contact_bench(args):
             "workload": f"tet bodies in contact: 16 cubes + 16 tet balls of 8,640 tets (seed {args.seed}, seeded "
                         f"rotations, mu log-uniform 1e4-1e6, lambda 10 mu) on a 4x4x2 lattice in a 2 m box, dt=1/120, "
                         f"20 iterations, contact radius 2 mm, stiffness 1e4 (quadratic-log), friction 0.2, query "
-                        f"radius 2 cm, BVH rebuilt every 32 frames; frames {args.warmup + 1}-{args.warmup + n} of "
-                        f"the fall",
+                        f"radius {phys['contact_query_radius'] * 1000:g} mm, BVH rebuilt every 32 frames; frames "
+                        f"{args.warmup + 1}-{args.warmup + n} of the fall",
             "bodies": len(bodies), "vertices": nverts, "tets": ntets, "iterations": iters,
             "parallel_groups": groups,
             "schedule": dict(sched, key=schedule_key(sched)),
@@ -490,6 +492,8 @@ def main():
     ap.add_argument("--workload", choices=["batched", "detector", "serve", "contact"], default="batched",
                     help="batched VBD (default), the contact detector alone on 64 tori, served VBD, or tet "
                          "bodies in contact (32 bodies of 8,640 tets)")
+    ap.add_argument("--query-radius", type=float, default=None,
+                    help="tet-contact workload: contact query radius in m (default 0.003, 1.5 contact radii)")
     ap.add_argument("--serve-scenes", type=int, default=2000)
     ap.add_argument("--requests", type=int, default=20000, help="served VBD: step requests in the timed window")
     ap.add_argument("--rate", type=float, default=20000.0, help="served VBD: Poisson request rate per second")

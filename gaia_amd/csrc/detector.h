@@ -66,7 +66,7 @@ struct DetectorDeviceView {
     const EEContactRec* eeRec;
     int eeCap;
     const int2* fvRange;
-    const uint64_t* fvKeys;
+    const int* fvVerts;
     const float* bounds;
     const float4* xAtDetection;
     int* flag;
@@ -177,14 +177,16 @@ class ContactDetector {
     unsigned int* boxPartial_ = nullptr;  // per-block centroid bounds (build)
     float4* external_ = nullptr;
     // face back-links and the solver-side state
-    uint64_t *fvKeys_ = nullptr, *fvKeysAlt_ = nullptr;
+    int* fvVerts_ = nullptr;    // nv * vfCap: vertex ids grouped by face (FV lists)
+    int* fvCounts_ = nullptr;   // 2 nf + 1: per-face counts (nf + 1), then scatter cursors (nf)
+    int* fvOffsets_ = nullptr;  // nf + 1: exclusive scan of the counts
     int2* fvRange_ = nullptr;
-    void* fvSortTemp_ = nullptr;
+    void* fvScanTemp_ = nullptr;
     uint64_t* vKeys_ = nullptr;     // vertex Morton keys (VF query order)
     uint64_t* vKeysAlt_ = nullptr;  // sorted
     void* vSortTemp_ = nullptr;
     size_t vSortTempBytes_ = 0;
-    size_t fvSortTempBytes_ = 0;
+    size_t fvScanTempBytes_ = 0;
     float4* xAtCD_ = nullptr;
     int* flag_ = nullptr;
 };

@@ -789,32 +789,44 @@ __global__ __launch_bounds__(256) void perturb_kernel(const float4* rest, float4
                          r.z + fs * drift[3 * m + 2] + sigma * gaussian(seed, step, 3 * (int64_t)v + 2), 0.f);
 }
 
-// face back-links: one key (face << 32 | vertex) per stored VF contact, all-ones for empty slots
-__global__ __launch_bounds__(256) void fv_keys_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
-                                                      uint64_t* keys) {
+// Face back-links (the FV lists): for every face, the vertices whose stored VF list holds it,
+// ascending, at most `cap` of them (the lowest ids). Built from the stored records only:
+// count per face, exclusive scan, scatter, then an in-place selection of each face's lowest ids.
+__global__ __launch_bounds__(256) void fv_count_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
+                                                       int* counts) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nv * cap) return;
     const int v = (int)(i / cap), k = (int)(i % cap);
-    keys[i] = k < stored[v] ? (((uint64_t)rec[i].face << 32) | (uint32_t)v) : ~0ull;
+    if (k < stored[v]) atomicAdd(&counts[rec[i].face], 1);
 }
 
-__device__ __forceinline__ int64_t lower_bound_key(const uint64_t* keys, int64_t n, uint64_t key) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+__global__ __launch_bounds__(256) void fv_scatter_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
+                                                         const int* offsets, int* fill, int* verts) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nv * cap) return;
+    const int v = (int)(i / cap), k = (int)(i % cap);
+    if (k >= stored[v]) return;
+    const int f = rec[i].face;
+    verts[offsets[f] + atomicAdd(&fill[f], 1)] = v;
 }
 
-__global__ __launch_bounds__(256) void fv_range_kernel(const uint64_t* keys, int64_t n, int nf, int cap,
-                                                       int2* range) {
+// selection of the `cap` lowest vertex ids of each face into its first slots, ascending (the
+// scatter order is arbitrary; this makes the lists, and every sum over them, deterministic)
+__global__ __launch_bounds__(256) void fv_select_kernel(const int* offsets, int nf, int cap, int* verts,
+                                                        int2* range) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
-    const int64_t b = lower_bound_key(keys, n, (uint64_t)f << 32);
-    const int64_t e = lower_bound_key(keys, n, (uint64_t)(f + 1) << 32);
-    range[f] = make_int2((int)b, (int)std::min<int64_t>(e, b + cap));
+    const int b = offsets[f], e = offsets[f + 1];
+    const int m = min(e - b, cap);
+    for (int i = b; i < b + m; ++i) {
+        int best = i;
+        for (int j = i + 1; j < e; ++j)
+            if (verts[j] < verts[best]) best = j;
+        const int t = verts[i];
+        verts[i] = verts[best];
+        verts[best] = t;
+    }
+    range[f] = make_int2(b, b + m);
 }
 
 inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
@@ -829,12 +841,19 @@ void ContactDetector::useExternalPositions(float4* pos) {
 void ContactDetector::buildFaceLinks(int cap) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     const int64_t n = (int64_t)nv_ * vfCap_;
-    hipLaunchKernelGGL(fv_keys_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
-                       fvKeysAlt_);
-    size_t tb = fvSortTempBytes_;
-    hip_check(rocprim::radix_sort_keys(fvSortTemp_, tb, fvKeysAlt_, fvKeys_, (size_t)n, 0, 64, stream_),
-              "face link sort");
-    hipLaunchKernelGGL(fv_range_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, fvKeys_, n, nf_, cap,
+    int* counts = fvCounts_;         // nf + 1 (the last stays 0: the scan's total lands at offsets[nf])
+    int* fill = fvCounts_ + nf_ + 1;  // nf
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(2 * (int64_t)nf_ + 1)), dim3(256), 0, stream_, fvCounts_,
+                       2 * (int64_t)nf_ + 1, 0);
+    hipLaunchKernelGGL(fv_count_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
+                       counts);
+    size_t tb = fvScanTempBytes_;
+    hip_check(rocprim::exclusive_scan(fvScanTemp_, tb, counts, fvOffsets_, 0, (size_t)nf_ + 1, rocprim::plus<int>(),
+                                      stream_),
+              "face link scan");
+    hipLaunchKernelGGL(fv_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
+                       fvOffsets_, fill, fvVerts_);
+    hipLaunchKernelGGL(fv_select_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, fvOffsets_, nf_, cap, fvVerts_,
                        fvRange_);
     hip_check(hipGetLastError(), "face link launch");
 }
@@ -847,7 +866,7 @@ void ContactDetector::snapshotPositions() {
 
 DetectorDeviceView ContactDetector::deviceView() const {
     return DetectorDeviceView{faces_, edges_,  vfStored_, vfRec_, vfCap_, eeStored_, eeRec_,
-                              eeCap_, fvRange_, fvKeys_,  bounds_, xAtCD_, flag_};
+                              eeCap_, fvRange_, fvVerts_, bounds_, xAtCD_, flag_};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -974,15 +993,16 @@ void ContactDetector::finalize() {
         up(rest_, p4.data(), sizeof(float4) * nv_);
     }
     // face back-links and solver-side state
-    fvKeys_ = dalloc<uint64_t>((size_t)nv_ * vfCap_);
-    fvKeysAlt_ = dalloc<uint64_t>((size_t)nv_ * vfCap_);
-    fvRange_ = dalloc<int2>(nf_);
+    fvVerts_ = dalloc<int>(std::max<size_t>((size_t)nv_ * vfCap_, 1));
+    fvCounts_ = dalloc<int>(2 * (size_t)nf_ + 1);
+    fvOffsets_ = dalloc<int>((size_t)nf_ + 1);
+    fvRange_ = dalloc<int2>(std::max(nf_, 1));
     xAtCD_ = dalloc<float4>(nv_);
     flag_ = dalloc<int>(1);
-    hip_check(rocprim::radix_sort_keys(nullptr, fvSortTempBytes_, fvKeysAlt_, fvKeys_, (size_t)nv_ * vfCap_, 0, 64,
-                                       stream_),
-              "sort size");
-    fvSortTemp_ = dalloc<char>(fvSortTempBytes_);
+    hip_check(rocprim::exclusive_scan(nullptr, fvScanTempBytes_, fvCounts_, fvOffsets_, 0, (size_t)nf_ + 1,
+                                      rocprim::plus<int>(), stream_),
+              "scan size");
+    fvScanTemp_ = dalloc<char>(std::max<size_t>(fvScanTempBytes_, 1));
     hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
     vMesh_ = dalloc<int>(nv_);
     up(vMesh_, hVMesh_.data(), sizeof(int) * nv_);

@@ -113,8 +113,8 @@ struct ContactView {
     int eeCap;
     const int* eeCount;       // per edge
     const float4* eeRec;      // per edge eeCap records (edge bits, mu1, mu2, d)
-    const int2* fvRange;      // per face: [begin, end) into fvKeys (at most 32)
-    const uint64_t* fvKeys;   // sorted (face << 32 | vertex)
+    const int2* fvRange;      // per face: [begin, end) into fvVerts (at most 32)
+    const int* fvVerts;       // per face, ascending vertex ids of its VF contacts
     const int* faceRef;       // per FaceInc record: face id | order << 30
     const int* edgeRef;       // per cloth-slot edge record: edge id | order << 30
     const int* tFaceRef;      // tet slots: incident boundary faces, face id | order << 30

@@ -530,7 +530,7 @@ __device__ void contact_forces(const SceneView& S, int v, int sub, const int* fa
         const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int2 range = C.fvRange[face];
         const int end = base + (range.y - range.x);
-        for (; t < end; t += LPV) vf_contact(S, (int)(C.fvKeys[range.x + (t - base)] & 0xFFFFFFFFu), face, order, f, h);
+        for (; t < end; t += LPV) vf_contact(S, C.fvVerts[range.x + (t - base)], face, order, f, h);
         base = end;
     }
     t = sub;

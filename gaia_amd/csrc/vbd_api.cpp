@@ -988,7 +988,7 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         cv.eeCount = dv.eeStored;
         cv.eeRec = (const float4*)dv.eeRec;
         cv.fvRange = dv.fvRange;
-        cv.fvKeys = dv.fvKeys;
+        cv.fvVerts = dv.fvVerts;
         cv.faceRef = s.dev.faceRef;
         cv.edgeRef = s.dev.edgeRef;
         cv.tFaceRef = s.dev.tFaceRef;

@@ -320,14 +320,17 @@ def c2_bodies(seed=0, n=12, cube_size=0.28, ball_radius=0.14, spacing=0.5, layer
     return bodies
 
 
-def c2_physics(iterations=20):
+def c2_physics(iterations=20, query_radius=0.003):
     """Physics of the tet-contact workload: dt = 1/120, one substep, contact radius 2 mm,
-    contact stiffness 1e4 (the cloth framework's default) with the quadratic-log energy, query
-    radius 2 cm, BVH rebuilt every 32 steps (refit between), box boundary of 2 m."""
+    contact stiffness 1e4 (the cloth framework's default, VBDClothPhysicsParameters.h:61) with the
+    quadratic-log energy, query radius 1.5 contact radii (the ratio of the reference's cloth
+    scenes, e.g. contactRadius 0.2 / maxQueryDis 0.3 in
+    VBDCloth/ParameterGen/Parameters/S01_Twisting/*/Parameters.json), BVH rebuilt every 32 steps
+    (contactBVHReconstructionSteps' default, refit between), box boundary of 2 m."""
     return dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 120.0, num_substeps=1, iterations=iterations,
                 world_lower=(-1.0, 0.0, -1.0), world_upper=(1.0, 2.0, 1.0), handle_collision=1,
                 contact_stiffness=1e4, contact_radius=0.002, thickness=0.0, conservative_step_relaxation=0.45,
-                contact_energy_type=1, contact_query_radius=0.02, contact_bvh_rebuild_steps=32)
+                contact_energy_type=1, contact_query_radius=query_radius, contact_bvh_rebuild_steps=32)
 
 
 def c3_tori(seed=0, grid=4, nu=128, nv=64, spacing=0.3, major=0.12, minor=0.04, drift=0.005):
