@@ -42,6 +42,8 @@ struct DeviceBvh {
     int* jump = nullptr;             // 4 (n - 1): depth / ancestor buffers of the depth computation
     int* levelNodes = nullptr;       // internal nodes ordered by depth, deepest first
     std::vector<int> levelBegin;     // host: ranges of levelNodes per depth (deepest first)
+    int* dLevelBegin = nullptr;      // the same ranges on the device
+    std::vector<std::pair<int, int>> segments;  // refit launches: [first, last) level runs
     uint64_t* keys = nullptr;        // n (sorted Morton keys)
     uint64_t* keysAlt = nullptr;     // n
     uint64_t* depthKeys = nullptr;   // n - 1

@@ -42,6 +42,7 @@ constexpr int kMaxCap = 16;       // largest per-query contact capacity
 // order after its subtree, -1 at the end), so a query visits left children first and follows
 // skip links; no per-thread stack lives in scratch memory.
 constexpr int kMaxDepth = 128;    // depth histogram size
+constexpr int kBlockLevel = 1024;  // refit levels of at most this many nodes share one-block launches
 constexpr float kCmpEpsilon = 0.00001f;
 constexpr float kCmpEpsilon2 = kCmpEpsilon * kCmpEpsilon;
 constexpr float kAbsoluteRelaxation = 1e-6f;
@@ -756,6 +757,26 @@ __global__ __launch_bounds__(256) void internal_box_kernel(const int* levelNodes
     N.hi = make_float4(fmaxf(A.hi.x, B.hi.x), fmaxf(A.hi.y, B.hi.y), fmaxf(A.hi.z, B.hi.z), N.hi.w);
 }
 
+// Several consecutive refit levels of few nodes each in one block: the levels run in order with a
+// block barrier between them, so a level reads children finished by the previous ones (levels
+// before the segment were finished by earlier launches).
+__global__ __launch_bounds__(1024) void internal_box_levels_kernel(const int* levelNodes, const int* levelBegin,
+                                                                   int l0, int l1, BvhNode* nodes,
+                                                                   const int* rightChild) {
+    for (int l = l0; l < l1; ++l) {
+        const int b = levelBegin[l], e = levelBegin[l + 1];
+        for (int k = b + (int)threadIdx.x; k < e; k += (int)blockDim.x) {
+            const int i = levelNodes[k];
+            BvhNode& N = nodes[i];
+            const BvhNode A = nodes[__float_as_int(N.lo.w)];
+            const BvhNode B = nodes[rightChild[i]];
+            N.lo = make_float4(fminf(A.lo.x, B.lo.x), fminf(A.lo.y, B.lo.y), fminf(A.lo.z, B.lo.z), N.lo.w);
+            N.hi = make_float4(fmaxf(A.hi.x, B.hi.x), fmaxf(A.hi.y, B.hi.y), fmaxf(A.hi.z, B.hi.z), N.hi.w);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_int_kernel(int* p, int64_t n, int value) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = value;
@@ -1086,6 +1107,7 @@ void ContactDetector::finalize() {
         B.depthKeysAlt = dalloc<uint64_t>(std::max(B.n - 1, 1));
         B.depth = dalloc<int>(std::max(B.n - 1, 1));
         B.levelNodes = dalloc<int>(std::max(B.n - 1, 1));
+        B.dLevelBegin = dalloc<int>(kMaxDepth + 1);
         size_t s1 = 0, s2 = 0;
         hip_check(rocprim::radix_sort_keys(nullptr, s1, B.keys, B.keysAlt, (size_t)B.n, 0, 62, stream_), "sort size");
         hip_check(rocprim::radix_sort_keys(nullptr, s2, B.depthKeys, B.depthKeysAlt, (size_t)std::max(B.n - 1, 1), 0,
@@ -1164,6 +1186,24 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
         for (int d = kMaxDepth - 1; d >= 0; --d)
             if (starts[d] >= 0) B.levelBegin.push_back(starts[d]);
         B.levelBegin.push_back(ni);
+        // refit launches: every level of more than kBlockLevel nodes alone, runs of smaller levels
+        // together in one block
+        B.segments.clear();
+        const int nl = (int)B.levelBegin.size() - 1;
+        for (int l = 0; l < nl;) {
+            if (B.levelBegin[l + 1] - B.levelBegin[l] > kBlockLevel) {
+                B.segments.push_back({l, l + 1});
+                ++l;
+                continue;
+            }
+            int e = l;
+            while (e < nl && B.levelBegin[e + 1] - B.levelBegin[e] <= kBlockLevel) ++e;
+            B.segments.push_back({l, e});
+            l = e;
+        }
+        hip_check(hipMemcpyAsync(B.dLevelBegin, B.levelBegin.data(), sizeof(int) * B.levelBegin.size(),
+                                 hipMemcpyHostToDevice, stream_),
+                  "level upload");
     }
     refitBvh(B, kind);
 }
@@ -1174,10 +1214,14 @@ void ContactDetector::refitBvh(DeviceBvh& B, int kind) {
     S.faces = faces_;
     S.edges = edges_;
     hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream_, S, kind, B.nodes, B.n);
-    for (size_t l = 0; l + 1 < B.levelBegin.size(); ++l) {
-        const int b = B.levelBegin[l], e = B.levelBegin[l + 1];
-        hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_, B.levelNodes, b, e,
-                           B.nodes, B.right);
+    for (const auto& seg : B.segments) {
+        const int b = B.levelBegin[seg.first], e = B.levelBegin[seg.first + 1];
+        if (seg.second == seg.first + 1 && e - b > kBlockLevel)
+            hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_, B.levelNodes, b,
+                               e, B.nodes, B.right);
+        else
+            hipLaunchKernelGGL(internal_box_levels_kernel, dim3(1), dim3(kBlockLevel), 0, stream_, B.levelNodes,
+                               B.dLevelBegin, seg.first, seg.second, B.nodes, B.right);
     }
     hip_check(hipGetLastError(), "bvh refit launch");
 }
