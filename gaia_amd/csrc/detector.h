@@ -126,12 +126,18 @@ class ContactDetector {
 
   private:
     void buildBvh(DeviceBvh& b, int kind);
-    void refitBvh(DeviceBvh& b, int kind);
+    void refitBvh(DeviceBvh& b, int kind, hipStream_t stream);
+    // the face-side and edge-side work of a pass are independent: side_ runs the edge side
+    // between fork() (side_ waits for stream_) and join() (stream_ waits for side_)
+    void fork();
+    void join();
     void freeAll();
     template <class T> T* dalloc(size_t n);
 
     int device_;
     hipStream_t stream_;
+    hipStream_t side_ = nullptr;  // default priority, created at finalize
+    hipEvent_t forkEv_ = nullptr, joinEv_ = nullptr;
     float r_;
     int vfCap_, eeCap_;
     bool ready_ = false;

@@ -916,7 +916,25 @@ ContactDetector::ContactDetector(int device, hipStream_t stream, float maxQueryD
     meshE_.push_back(0);
 }
 
-ContactDetector::~ContactDetector() { freeAll(); }
+ContactDetector::~ContactDetector() {
+    if (side_) {
+        (void)hipStreamSynchronize(side_);
+        (void)hipStreamDestroy(side_);
+    }
+    if (forkEv_) (void)hipEventDestroy(forkEv_);
+    if (joinEv_) (void)hipEventDestroy(joinEv_);
+    freeAll();
+}
+
+void ContactDetector::fork() {
+    hip_check(hipEventRecord(forkEv_, stream_), "fork record");
+    hip_check(hipStreamWaitEvent(side_, forkEv_, 0), "fork wait");
+}
+
+void ContactDetector::join() {
+    hip_check(hipEventRecord(joinEv_, side_), "join record");
+    hip_check(hipStreamWaitEvent(stream_, joinEv_, 0), "join wait");
+}
 
 void ContactDetector::freeAll() {
     for (void* p : allocs_) (void)hipFree(p);
@@ -977,6 +995,9 @@ void ContactDetector::finalize() {
     if (ready_) fail(GVBD_ERR_STATE, "detector already finalized");
     if (nv_ == 0) fail(GVBD_ERR_STATE, "detector has no meshes");
     hip_check(hipSetDevice(device_), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "side stream");
+    hip_check(hipEventCreateWithFlags(&forkEv_, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&joinEv_, hipEventDisableTiming), "event");
     // CSR adjacency: vertex -> neighbour vertices, incident faces, incident edges
     auto csr = [&](const std::vector<std::pair<int, int>>& items, std::vector<int>& begin, std::vector<int>& out) {
         begin.assign(nv_ + 1, 0);
@@ -1205,22 +1226,22 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
                                  hipMemcpyHostToDevice, stream_),
                   "level upload");
     }
-    refitBvh(B, kind);
+    refitBvh(B, kind, stream_);
 }
 
-void ContactDetector::refitBvh(DeviceBvh& B, int kind) {
+void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream) {
     if (B.n == 0) return;
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.faces = faces_;
     S.edges = edges_;
-    hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream_, S, kind, B.nodes, B.n);
+    hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream, S, kind, B.nodes, B.n);
     for (const auto& seg : B.segments) {
         const int b = B.levelBegin[seg.first], e = B.levelBegin[seg.first + 1];
         if (seg.second == seg.first + 1 && e - b > kBlockLevel)
-            hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_, B.levelNodes, b,
+            hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream, B.levelNodes, b,
                                e, B.nodes, B.right);
         else
-            hipLaunchKernelGGL(internal_box_levels_kernel, dim3(1), dim3(kBlockLevel), 0, stream_, B.levelNodes,
+            hipLaunchKernelGGL(internal_box_levels_kernel, dim3(1), dim3(kBlockLevel), 0, stream, B.levelNodes,
                                B.dLevelBegin, seg.first, seg.second, B.nodes, B.right);
     }
     hip_check(hipGetLastError(), "bvh refit launch");
@@ -1232,8 +1253,10 @@ void ContactDetector::updateBvh(bool rebuild) {
         buildBvh(fb_, 0);
         buildBvh(eb_, 1);
     } else {
-        refitBvh(fb_, 0);
-        refitBvh(eb_, 1);
+        fork();
+        refitBvh(fb_, 0, stream_);
+        refitBvh(eb_, 1, side_);
+        join();
     }
 }
 
@@ -1311,8 +1334,10 @@ void ContactDetector::query(bool record) {
     std::memcpy(&rbits, &rr, sizeof(int));
     hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nf_), 4096)), dim3(256), 0, stream_, faceMin_,
                        (int64_t)nf_, rbits);
+    fork();
     hipLaunchKernelGGL((vf_query_kernel<false>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
-    hipLaunchKernelGGL((ee_query_kernel<false>), dim3(blocks_for(ne_)), dim3(256), 0, stream_, S);
+    hipLaunchKernelGGL((ee_query_kernel<false>), dim3(blocks_for(ne_)), dim3(256), 0, side_, S);
+    join();
     hip_check(hipGetLastError(), "query launch");
     recorded_ = false;
     if (!record) return;
