@@ -234,8 +234,6 @@ def contact_bench(args):
             colors[key] = gaia_amd.color_tet_mesh(len(b["pos"]), b["tets"], "greedy")
         sim.addTetMesh(b["pos"], b["tets"], vertex_colors=colors[key], **b["material"])
     sim.initialize()
-    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(args.schedule or SERVE_SCHEDULE)}
-    sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group", stage_records=sched.get("stage_records", 0))
     nverts = sum(len(b["pos"]) for b in bodies)
     ntets = sum(len(b["tets"]) for b in bodies)
     iters = phys["iterations"]
@@ -244,6 +242,37 @@ def contact_bench(args):
     for _ in range(args.warmup):
         sim.runStep()
     ctx.synchronize()
+    # schedule: lanes per vertex x block of the group launches, each candidate timed over 3 frames
+    # from the same state (the state is restored after every candidate and before the timed run)
+    saved = [sim.getState(m) for m in range(len(bodies))]
+
+    def restore():
+        for m, (x, v) in enumerate(saved):
+            sim.setState(m, x, v)
+
+    if args.schedule:
+        cands = [args.schedule]
+    else:
+        cands = ["group:4x64", "group:4x128", "group:8x64", "group:8x128", "group:8x256", "group:16x128",
+                 "group:16x256"]
+    tune = {}
+    for key in cands:
+        c = parse_schedule_key(key)
+        sim.setSchedule(c["lanes_per_vertex"], c["block"], mode="group")
+        restore()
+        sim.runStep()
+        restore()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            sim.runStep()
+        ctx.synchronize()
+        tune[key] = round((time.perf_counter() - t) / 3 * 1e3, 3)
+    best = min(tune, key=tune.get)
+    sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(best)}
+    sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
+    restore()
+    log(f"contact autotune (ms/frame): {tune} -> {best}")
     st0 = sim.contactStats()
     t = time.perf_counter()
     for _ in range(args.steps):
@@ -305,6 +334,7 @@ def contact_bench(args):
             "bodies": len(bodies), "vertices": nverts, "tets": ntets, "iterations": iters,
             "parallel_groups": groups,
             "schedule": dict(sched, key=schedule_key(sched)),
+            "autotune_ms_per_frame": tune,
             "detections_per_step": round(detections, 3),
             "detect_ms_per_step": round(detect_ms, 3),
             "solve_ms_per_step": round(solve_ms_per_step, 3),

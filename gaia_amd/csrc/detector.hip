@@ -255,7 +255,10 @@ __device__ bool edge_feasible(const CDView& S, V3 p, int e) {
 }
 
 // VF query of one vertex (ClothContactDetector.cpp:144-288 per candidate face)
-template <bool RECORD>
+// CAP bounds the per-thread candidate list (the runtime vfCap is at most CAP); only the face
+// id and the packed (type, primitive) key are kept per candidate, the distance is recomputed
+// for the stored ones at the end, so the list costs 2 CAP registers.
+template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
     // vertices in Morton order: a wavefront's queries are spatial neighbours
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -277,8 +280,7 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
     const V3 qlo = {p.x - rq, p.y - rq, p.z - rq}, qhi = {p.x + rq, p.y + rq, p.z + rq};
     float vmin = r;
     int pairs = 0, stored = 0, overflow = 0;
-    int kType[kMaxCap], kPrim[kMaxCap], kFace[kMaxCap];
-    float kD[kMaxCap];
+    int kKey[CAP], kFace[CAP];  // key: primitive | type << 30
     int64_t pairBase = 0, keyBase = 0;
     int recKeys = 0, pairCap = 0;
     if (RECORD) {
@@ -338,14 +340,12 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
         // dedup on (type, primitive): keep the lowest face id as the representative
         bool dup = false;
         if (!RECORD) {
-            for (int k = 0; k < stored; ++k)
-                if (kType[k] == ptype && kPrim[k] == prim) {
-                    if (f < kFace[k]) {
-                        kFace[k] = f;
-                        kD[k] = d;
-                    }
+            const int key = prim | (ptype << 30);
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k < stored && !dup && kKey[k] == key) {
+                    if (f < kFace[k]) kFace[k] = f;
                     dup = true;
-                    break;
                 }
         } else {
             for (int k = 0; k < recKeys; ++k) {
@@ -377,18 +377,23 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
             if (stored == S.vfCap) {
                 overflow = 1;
                 slot = 0;
-                for (int k = 1; k < stored; ++k)
-                    if (kFace[k] > kFace[slot]) slot = k;
-                if (f > kFace[slot]) slot = -1;
+                int worst = kFace[0];
+#pragma unroll
+                for (int k = 1; k < CAP; ++k)
+                    if (k < stored && kFace[k] > worst) {
+                        worst = kFace[k];
+                        slot = k;
+                    }
+                if (f > worst) slot = -1;
             } else {
                 ++stored;
             }
-            if (slot >= 0) {
-                kType[slot] = ptype;
-                kPrim[slot] = prim;
-                kFace[slot] = f;
-                kD[slot] = d;
-            }
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k == slot) {
+                    kKey[k] = prim | (ptype << 30);
+                    kFace[k] = f;
+                }
         }
     }
     if (RECORD) {
@@ -399,11 +404,30 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
     S.vfPairCount[v] = pairs;
     S.vfStored[v] = stored;
     S.vfOverflow[v] = overflow;
-    for (int k = 0; k < stored; ++k) S.vfRec[(int64_t)v * S.vfCap + k] = VFContactRec{kFace[k], kType[k], kPrim[k], kD[k]};
+    // stored in ascending face id, whatever the traversal order (so the list, and the solver's sum
+    // over it, does not depend on the tree's shape); representative faces are distinct
+    int last = -1;
+    for (int o = 0; o < stored; ++o) {
+        int bf = INT_MAX, bk = 0;
+#pragma unroll
+        for (int k = 0; k < CAP; ++k)
+            if (k < stored && kFace[k] > last && kFace[k] < bf) {
+                bf = kFace[k];
+                bk = kKey[k];
+            }
+        // the same closest-point computation as the traversal's, so the same distance
+        const int4 fv = S.faces[bf];
+        int type;
+        const float d = dist(p, closest_point_triangle(p, ld(S.pos, fv.x), ld(S.pos, fv.y), ld(S.pos, fv.z), type));
+        S.vfRec[(int64_t)v * S.vfCap + o] = VFContactRec{bf, (bk >> 30) & 3, bk & 0x3FFFFFFF, d};
+        last = bf;
+    }
 }
 
 // EE query of one edge (ClothContactDetector.cpp:731-813 per candidate edge, :1106-1135 setup)
-template <bool RECORD>
+// CAP as for the VF query: only the other edge's id is kept per candidate; the closest points and
+// distance of the stored ones are recomputed at the end.
+template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     // neighbouring threads take edges that are neighbours in the tree's Morton order, so a
     // wavefront's traversals share their paths (and cache lines)
@@ -422,8 +446,7 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     const V3 qhi = {fmaxf(p1.x, p2.x) + rq, fmaxf(p1.y, p2.y) + rq, fmaxf(p1.z, p2.z) + rq};
     float emin = r;
     int count = 0, stored = 0, overflow = 0;
-    int cE[kMaxCap];
-    float cMu1[kMaxCap], cMu2[kMaxCap], cD[kMaxCap];
+    int cE[CAP];
     const int64_t base = RECORD ? S.eeOffset[e1] : 0;
     const int countCap = RECORD ? S.eeCount[e1] : 0;
     const int firstLeaf = S.eN - 1;
@@ -479,18 +502,20 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
             if (stored == S.eeCap) {
                 overflow = 1;
                 slot = 0;
-                for (int k = 1; k < stored; ++k)
-                    if (cE[k] > cE[slot]) slot = k;
-                if (e2 > cE[slot]) slot = -1;
+                int worst = cE[0];
+#pragma unroll
+                for (int k = 1; k < CAP; ++k)
+                    if (k < stored && cE[k] > worst) {
+                        worst = cE[k];
+                        slot = k;
+                    }
+                if (e2 > worst) slot = -1;
             } else {
                 ++stored;
             }
-            if (slot >= 0) {
-                cE[slot] = e2;
-                cMu1[slot] = mua;
-                cMu2[slot] = mub;
-                cD[slot] = d;
-            }
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k == slot) cE[k] = e2;
         }
         ++count;
     }
@@ -499,7 +524,20 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     S.eeCount[e1] = count;
     S.eeStored[e1] = stored;
     S.eeOverflow[e1] = overflow;
-    for (int k = 0; k < stored; ++k) S.eeRec[(int64_t)e1 * S.eeCap + k] = EEContactRec{cE[k], cMu1[k], cMu2[k], cD[k]};
+    // stored in ascending other-edge id (independent of the traversal order)
+    int last = -1;
+    for (int o = 0; o < stored; ++o) {
+        int be = INT_MAX;
+#pragma unroll
+        for (int k = 0; k < CAP; ++k)
+            if (k < stored && cE[k] > last && cE[k] < be) be = cE[k];
+        const int4 b = S.edges[be];
+        V3 c1, c2;
+        float mua, mub;
+        segment_closest_points(p1, p2, ld(S.pos, b.x), ld(S.pos, b.y), c1, c2, mua, mub);
+        S.eeRec[(int64_t)e1 * S.eeCap + o] = EEContactRec{be, mua, mub, dist(c1, c2)};
+        last = be;
+    }
 }
 
 // VBDClothPhysics.cpp:775-812 per vertex
@@ -1335,8 +1373,14 @@ void ContactDetector::query(bool record) {
     hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nf_), 4096)), dim3(256), 0, stream_, faceMin_,
                        (int64_t)nf_, rbits);
     fork();
-    hipLaunchKernelGGL((vf_query_kernel<false>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
-    hipLaunchKernelGGL((ee_query_kernel<false>), dim3(blocks_for(ne_)), dim3(256), 0, side_, S);
+    if (vfCap_ <= 8)
+        hipLaunchKernelGGL((vf_query_kernel<false, 8>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
+    else
+        hipLaunchKernelGGL((vf_query_kernel<false, kMaxCap>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
+    if (eeCap_ <= 8)
+        hipLaunchKernelGGL((ee_query_kernel<false, 8>), dim3(blocks_for(ne_)), dim3(256), 0, side_, S);
+    else
+        hipLaunchKernelGGL((ee_query_kernel<false, kMaxCap>), dim3(blocks_for(ne_)), dim3(256), 0, side_, S);
     join();
     hip_check(hipGetLastError(), "query launch");
     recorded_ = false;
@@ -1375,8 +1419,8 @@ void ContactDetector::query(bool record) {
     S.recEE = recEE_;
     S.recEEmu = recEEmu_;
     S.recEEd = recEEd_;
-    hipLaunchKernelGGL((vf_query_kernel<true>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
-    hipLaunchKernelGGL((ee_query_kernel<true>), dim3(blocks_for(ne_)), dim3(256), 0, stream_, S);
+    hipLaunchKernelGGL((vf_query_kernel<true, 1>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
+    hipLaunchKernelGGL((ee_query_kernel<true, 1>), dim3(blocks_for(ne_)), dim3(256), 0, stream_, S);
     hip_check(hipGetLastError(), "record query launch");
     hip_check(hipStreamSynchronize(stream_), "record query sync");
     recorded_ = true;
