@@ -16,5 +16,8 @@ from .vbd import (  # noqa: F401
     color_tet_mesh,
     color_tri_mesh,
     physics_params,
+    state_read,
+    state_write,
     tet_mesh_constants,
+    write_ply,
 )

@@ -207,6 +207,14 @@ _SIGNATURES = {
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
     "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
+    "gvbd_scene_save_state": [_P, _I, ctypes.c_char_p, _I, ctypes.c_double, _I],
+    "gvbd_scene_load_state": [_P, _I, ctypes.c_char_p, c_int32_p, ctypes.POINTER(ctypes.c_double)],
+    "gvbd_scene_write_ply": [_P, _I, ctypes.c_char_p, _I],
+    "gvbd_scene_write_binary_frame": [_P, _I, ctypes.c_char_p],
+    "gvbd_state_write": [ctypes.c_char_p, _I, ctypes.c_double, _I, _I, c_int32_p, c_float_p, c_float_p],
+    "gvbd_state_read": [ctypes.c_char_p, _I, c_int32_p, c_int32_p, ctypes.c_int64, c_float_p, c_float_p, c_int32_p,
+                        ctypes.POINTER(ctypes.c_double)],
+    "gvbd_write_ply": [ctypes.c_char_p, _I, c_float_p, _I, c_int32_p],
     "gvbd_batch_create": [_P, c_int32_p, _I, c_int32_p],
     "gvbd_batch_destroy": [_P, _I],
     "gvbd_batch_set_schedule": [_P, _I, _I, _I],

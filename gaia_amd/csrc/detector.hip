@@ -988,7 +988,7 @@ int ContactDetector::addMesh(int nv, const float* pos, int nf, const int* faces,
     std::string err;
     if (!build_tri_edges(nv, nf, faces, edges, &faceEdges, err)) fail(GVBD_ERR_INVALID, err);
     const int m = (int)meshV_.size() - 1;
-    const int vo = nv_, fo = nf_, eo = ne_;
+    const int vo = nv_, eo = ne_;
     hostPos_.insert(hostPos_.end(), pos, pos + 3 * (size_t)nv);
     for (int i = 0; i < nv; ++i) hVMesh_.push_back(m);
     if (queryMask) masked_ = true;

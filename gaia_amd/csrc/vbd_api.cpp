@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -19,6 +20,7 @@
 #include "errors.h"
 #include "kernels.h"
 #include "mesh_prep.h"
+#include "scene_io.h"
 #include "scene_loader.h"
 
 namespace gvbd {
@@ -1602,6 +1604,182 @@ int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, co
                               const float* velocities) {
     return guard(ctx, [&] {
         mesh_state_io(ctx, scene, mesh, const_cast<float*>(positions), const_cast<float*>(velocities), false);
+    });
+}
+
+// ---- scene files ----
+
+static std::vector<io::MeshState> scene_states(gvbd_context* ctx, int32_t scene) {
+    SceneObj& s = scene_of(ctx, scene);
+    std::vector<io::MeshState> out(s.meshes.size());
+    for (size_t m = 0; m < s.meshes.size(); ++m) {
+        const size_t n = 3 * (size_t)s.meshVertices((int32_t)m);
+        out[m].position.resize(n);
+        out[m].velocities.resize(n);
+        mesh_state_io(ctx, scene, (int32_t)m, out[m].position.data(), out[m].velocities.data(), true);
+    }
+    return out;
+}
+
+// The surface of one mesh at its current positions: a tet mesh's boundary vertices (ascending
+// tet-vertex id) and boundary faces renumbered to them; a cloth mesh whole.
+static void mesh_surface(SceneObj& s, int32_t m, const std::vector<float>& pos, std::vector<float>& verts,
+                         std::vector<int32_t>& faces) {
+    if (s.meshes[m].first == MESH_CLOTH) {
+        verts = pos;
+        faces = s.cloths[s.meshes[m].second].faces;
+        return;
+    }
+    const TetMeshHost& t = s.tets[s.meshes[m].second];
+    std::vector<int32_t> tf;
+    std::string err;
+    if (!build_tet_surface(t.nv, t.nt, t.tets.data(), t.pos.data(), tf, err)) fail(GVBD_ERR_INVALID, err);
+    std::vector<int32_t> remap(t.nv, -1);
+    for (int32_t v : tf) remap[v] = 0;
+    verts.clear();
+    int32_t k = 0;
+    for (int32_t v = 0; v < t.nv; ++v)
+        if (remap[v] == 0) {
+            remap[v] = k++;
+            verts.insert(verts.end(), {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]});
+        }
+    faces.resize(tf.size());
+    for (size_t i = 0; i < tf.size(); ++i) faces[i] = remap[tf[i]];
+}
+
+int gvbd_scene_save_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t frame_id, double cur_time,
+                          int32_t binary) {
+    return guard(ctx, [&] {
+        require(ctx && json_path, "NULL argument");
+        io::write_state(json_path, frame_id, cur_time, binary != 0, scene_states(ctx, scene));
+    });
+}
+
+int gvbd_scene_load_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t* frame_id,
+                          double* cur_time) {
+    return guard(ctx, [&] {
+        require(ctx && json_path, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
+        int32_t fid = 0;
+        double t = 0.0;
+        std::vector<io::MeshState> st;
+        io::read_state(json_path, fid, t, st);
+        const size_t nm = std::min(st.size(), s.meshes.size());
+        for (size_t m = 0; m < nm; ++m)
+            if (st[m].position.size() != 3 * (size_t)s.meshVertices((int32_t)m))
+                fail(GVBD_ERR_INVALID, "recovery state: mesh " + std::to_string(m) + " holds " +
+                                           std::to_string(st[m].position.size() / 3) + " vertices, the scene's " +
+                                           std::to_string(s.meshVertices((int32_t)m)));
+        for (size_t m = 0; m < nm; ++m)
+            mesh_state_io(ctx, scene, (int32_t)m, st[m].position.data(),
+                          st[m].velocities.empty() ? nullptr : st[m].velocities.data(), false);
+        if (frame_id) *frame_id = fid;
+        if (cur_time) *cur_time = t;
+    });
+}
+
+int gvbd_scene_write_ply(gvbd_context* ctx, int32_t scene, const char* path, int32_t all_in_one) {
+    return guard(ctx, [&] {
+        require(ctx && path, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        const std::vector<io::MeshState> st = scene_states(ctx, scene);
+        std::vector<float> allV, v;
+        std::vector<int32_t> allF, f;
+        const std::string p(path);
+        const size_t slash = p.find_last_of("/\\"), dot = p.find_last_of('.');
+        const bool hasExt = dot != std::string::npos && (slash == std::string::npos || dot > slash);
+        const std::string base = hasExt ? p.substr(0, dot) : p, ext = hasExt ? p.substr(dot) : std::string();
+        for (size_t m = 0; m < s.meshes.size(); ++m) {
+            mesh_surface(s, (int32_t)m, st[m].position, v, f);
+            if (all_in_one) {
+                const int32_t off = (int32_t)(allV.size() / 3);
+                allV.insert(allV.end(), v.begin(), v.end());
+                for (int32_t i : f) allF.push_back(i + off);
+            } else {
+                char num[16];
+                std::snprintf(num, sizeof(num), "%06d", (int)m);
+                io::write_ply(base + "_Model_" + num + ext, v, f);
+            }
+        }
+        if (all_in_one) io::write_ply(p, allV, allF);
+    });
+}
+
+int gvbd_scene_write_binary_frame(gvbd_context* ctx, int32_t scene, const char* path) {
+    return guard(ctx, [&] {
+        require(ctx && path, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        const std::vector<io::MeshState> st = scene_states(ctx, scene);
+        std::vector<float> all, v;
+        std::vector<int32_t> f;
+        for (size_t m = 0; m < s.meshes.size(); ++m) {
+            mesh_surface(s, (int32_t)m, st[m].position, v, f);
+            all.insert(all.end(), v.begin(), v.end());
+        }
+        io::write_binary(path, all);
+    });
+}
+
+int gvbd_state_write(const char* json_path, int32_t frame_id, double cur_time, int32_t binary, int32_t num_meshes,
+                     const int32_t* num_vertices, const float* positions, const float* velocities) {
+    return guard(nullptr, [&] {
+        require(json_path && num_meshes >= 0 && (num_meshes == 0 || (num_vertices && positions)), "bad argument");
+        std::vector<io::MeshState> st(num_meshes);
+        size_t off = 0;
+        for (int32_t m = 0; m < num_meshes; ++m) {
+            require(num_vertices[m] >= 0, "negative vertex count");
+            const size_t n = 3 * (size_t)num_vertices[m];
+            st[m].position.assign(positions + off, positions + off + n);
+            if (velocities) st[m].velocities.assign(velocities + off, velocities + off + n);
+            off += n;
+        }
+        require(velocities != nullptr || !binary, "a binary recovery state needs velocities");
+        io::write_state(json_path, frame_id, cur_time, binary != 0, st);
+    });
+}
+
+int gvbd_state_read(const char* json_path, int32_t max_meshes, int32_t* num_meshes, int32_t* num_vertices,
+                    int64_t max_floats, float* positions, float* velocities, int32_t* frame_id, double* cur_time) {
+    return guard(nullptr, [&] {
+        require(json_path && num_meshes, "bad argument");
+        int32_t fid = 0;
+        double t = 0.0;
+        std::vector<io::MeshState> st;
+        io::read_state(json_path, fid, t, st);
+        *num_meshes = (int32_t)st.size();
+        if (frame_id) *frame_id = fid;
+        if (cur_time) *cur_time = t;
+        size_t total = 0;
+        for (size_t m = 0; m < st.size(); ++m) {
+            if (num_vertices && (int32_t)m < max_meshes) num_vertices[m] = (int32_t)(st[m].position.size() / 3);
+            total += st[m].position.size();
+        }
+        if (!positions) return;
+        require((int64_t)st.size() <= max_meshes && (int64_t)total <= max_floats, "buffers too small");
+        size_t off = 0;
+        for (const auto& m : st) {
+            std::memcpy(positions + off, m.position.data(), sizeof(float) * m.position.size());
+            if (velocities) {
+                if (m.velocities.empty())
+                    std::fill(velocities + off, velocities + off + m.position.size(), 0.f);
+                else
+                    std::memcpy(velocities + off, m.velocities.data(), sizeof(float) * m.velocities.size());
+            }
+            off += m.position.size();
+        }
+    });
+}
+
+int gvbd_write_ply(const char* path, int32_t num_vertices, const float* positions, int32_t num_faces,
+                   const int32_t* faces) {
+    return guard(nullptr, [&] {
+        require(path && num_vertices >= 0 && num_faces >= 0 && (num_vertices == 0 || positions) &&
+                    (num_faces == 0 || faces),
+                "bad argument");
+        for (int32_t i = 0; i < 3 * num_faces; ++i) require(faces[i] >= 0 && faces[i] < num_vertices, "face index out of range");
+        io::write_ply(path, std::vector<float>(positions, positions + 3 * (size_t)num_vertices),
+                      std::vector<int32_t>(faces, faces + 3 * (size_t)num_faces));
     });
 }
 

@@ -159,6 +159,45 @@ class Context:
         return ms.value
 
 
+def state_write(json_path, positions, velocities=None, frame_id=0, cur_time=0.0, binary=False):
+    """Recovery-state file from per-mesh (N, 3) arrays (no device)."""
+    counts = i32([len(np.asarray(p).reshape(-1, 3)) for p in positions])
+    pos = f32(np.concatenate([np.asarray(p, np.float32).reshape(-1) for p in positions])) if len(positions) \
+        else np.zeros(0, np.float32)
+    vel = None
+    if velocities is not None:
+        vel = f32(np.concatenate([np.asarray(v, np.float32).reshape(-1) for v in velocities])) if len(velocities) \
+            else np.zeros(0, np.float32)
+    check(lib().gvbd_state_write(os.fsencode(json_path), int(frame_id), float(cur_time), 1 if binary else 0,
+                                 len(counts), iptr(counts), fptr(pos), fptr(vel) if vel is not None else None))
+
+
+def state_read(json_path):
+    """Reads a recovery-state file: (frameId, curTime, [positions (N, 3)], [velocities (N, 3)])."""
+    nm = ctypes.c_int32()
+    check(lib().gvbd_state_read(os.fsencode(json_path), 0, ctypes.byref(nm), None, 0, None, None, None, None))
+    counts = np.zeros(max(nm.value, 1), np.int32)
+    check(lib().gvbd_state_read(os.fsencode(json_path), nm.value, ctypes.byref(nm), iptr(counts), 0, None, None,
+                                None, None))
+    total = 3 * int(counts[:nm.value].sum())
+    pos = np.zeros(max(total, 1), np.float32)
+    vel = np.zeros(max(total, 1), np.float32)
+    fid = ctypes.c_int32()
+    t = ctypes.c_double()
+    check(lib().gvbd_state_read(os.fsencode(json_path), nm.value, ctypes.byref(nm), iptr(counts), total, fptr(pos),
+                                fptr(vel), ctypes.byref(fid), ctypes.byref(t)))
+    splits = np.cumsum(3 * counts[:nm.value])[:-1]
+    return (fid.value, t.value, [p.reshape(-1, 3) for p in np.split(pos[:total], splits)],
+            [v.reshape(-1, 3) for v in np.split(vel[:total], splits)])
+
+
+def write_ply(path, positions, faces):
+    """ASCII PLY of one surface (GAIA::saveAsPly's format)."""
+    pos = f32(positions).reshape(-1)
+    fc = i32(faces).reshape(-1)
+    check(lib().gvbd_write_ply(os.fsencode(path), len(pos) // 3, fptr(pos), len(fc) // 3, iptr(fc)))
+
+
 def color_tet_mesh(num_vertices, tets, algorithm="mcs"):
     """Vertex colouring of a tet mesh's vertex graph (GraphColoring mcs/greedy)."""
     tets = i32(tets).reshape(-1)
@@ -449,6 +488,57 @@ class VBDPhysics:
         self.ctx.check(lib().gvbd_scene_set_mesh_state(self.ctx.handle, self.scene, mesh,
                                                        fptr(x) if x is not None else None,
                                                        fptr(v) if v is not None else None))
+
+    # ---- scene files (BasePhysicsFramework.h:145-319, BasePhysicsFramework.cpp:264-281,441-503) ----
+    def saveState(self, json_path, frame_id=0, cur_time=0.0, binary=False):
+        """PhysicsState of every mesh (positions and velocities) as a recovery-state JSON; binary
+        writes <stem>.pos / <stem>.vel beside it."""
+        self.ctx.check(lib().gvbd_scene_save_state(self.ctx.handle, self.scene, os.fsencode(json_path), int(frame_id),
+                                                   float(cur_time), 1 if binary else 0))
+
+    def recoverFromState(self, state_file):
+        """BasePhysicFramework::recoverFromState: loads a recovery state (text or binary) into the
+        meshes; returns (frameId, curTime)."""
+        fid = ctypes.c_int32()
+        t = ctypes.c_double()
+        self.ctx.check(lib().gvbd_scene_load_state(self.ctx.handle, self.scene, os.fsencode(state_file),
+                                                   ctypes.byref(fid), ctypes.byref(t)))
+        self.frameId, self.curTime = fid.value, t.value
+        return fid.value, t.value
+
+    def writePLY(self, path, all_in_one=True):
+        """Surfaces of all meshes as ASCII PLY (one file, or <stem>_Model_NNNNNN<ext> per mesh)."""
+        self.ctx.check(lib().gvbd_scene_write_ply(self.ctx.handle, self.scene, os.fsencode(path), 1 if all_in_one else 0))
+
+    def writeBinaryFrame(self, path):
+        """Float32 surface positions of all meshes back to back (the reference's .bin frame)."""
+        self.ctx.check(lib().gvbd_scene_write_binary_frame(self.ctx.handle, self.scene, os.fsencode(path)))
+
+    def writeOutputs(self, out_folder, frame_id, output_ext="ply", save_all_models_together=True,
+                     output_recovery_state=False, recovery_state_step=1, recovery_state_binary=False,
+                     binary_visualization_steps=10, cur_time=0.0):
+        """BasePhysicFramework::writeOutputs (BasePhysicsFramework.cpp:441-503): A<frame, 8 digits>.<ext>
+        in out_folder ("ply", or "bin" with TemplateMesh/TemplateMesh.ply at frame 0 and a PLY every
+        binary_visualization_steps frames), and RecoveryStates/A<frame>.json every
+        recovery_state_step frames when asked."""
+        num = f"{int(frame_id):08d}"
+        os.makedirs(out_folder, exist_ok=True)
+        if output_ext == "ply":
+            self.writePLY(os.path.join(out_folder, f"A{num}.ply"), save_all_models_together)
+        elif output_ext == "bin":
+            if frame_id == 0:
+                tdir = os.path.join(out_folder, "TemplateMesh")
+                os.makedirs(tdir, exist_ok=True)
+                self.writePLY(os.path.join(tdir, "TemplateMesh.ply"), all_in_one=False)
+            self.writeBinaryFrame(os.path.join(out_folder, f"A{num}.bin"))
+            if frame_id % binary_visualization_steps == 0:
+                self.writePLY(os.path.join(out_folder, f"A{num}.ply"), save_all_models_together)
+        else:
+            raise ValueError(f"unsupported output extension {output_ext!r}")
+        if output_recovery_state and frame_id % recovery_state_step == 0:
+            sdir = os.path.join(out_folder, "RecoveryStates")
+            os.makedirs(sdir, exist_ok=True)
+            self.saveState(os.path.join(sdir, f"A{num}.json"), frame_id, cur_time, recovery_state_binary)
 
     def clone(self, defer_upload=True):
         """A new scene with this one's parameters and meshes as added (host preparation copied);

@@ -244,6 +244,35 @@ int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, fl
 int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, const float* positions,
                               const float* velocities);
 
+/* ---- scene files (synchronous) ----
+ * Recovery states: PhysicsState::toJson / fromJson + BasePhysicFramework::recoverFromState
+ * (Framework/BasePhysicsFramework.h:145-319, BasePhysicsFramework.cpp:264-281; the cloth framework's
+ * ClothPhysicsState, BaseClothSimPhsicsFramework.h:65-120, has the same layout). Every mesh of the
+ * scene in addition order; binary = 1 writes "<stem>.pos" / "<stem>.vel" (float32 xyz) beside
+ * the JSON. Loading sets the meshes the file holds (a shorter file leaves the rest as they are,
+ * as the reference does); a mesh stored without velocities keeps its current ones. */
+int gvbd_scene_save_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t frame_id, double cur_time,
+                          int32_t binary);
+int gvbd_scene_load_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t* frame_id,
+                          double* cur_time);
+/* Frame outputs: the surface of every mesh (a tet mesh's boundary vertices, ascending, and its
+ * boundary faces; every vertex and face of a cloth mesh) as ASCII PLY (GAIA::saveAsPly /
+ * writeAllToPLY, IO/FileIO.cpp:8-50,119-254): all meshes in one file (all_in_one = 1), or one file
+ * per mesh named "<dir>/<stem>_Model_<6-digit mesh index><ext>"; and the float32 surface
+ * positions of all meshes back to back (writeAllToBinary, IO/FileIO.cpp:52-117). */
+int gvbd_scene_write_ply(gvbd_context* ctx, int32_t scene, const char* path, int32_t all_in_one);
+int gvbd_scene_write_binary_frame(gvbd_context* ctx, int32_t scene, const char* path);
+/* The same file formats from host arrays (no device): num_meshes meshes of num_vertices[m]
+ * vertices, positions / velocities xyz back to back. gvbd_state_read fills num_meshes and
+ * num_vertices (capacity max_meshes) and, when positions is not NULL, positions and velocities
+ * (capacity max_floats floats each; a mesh stored without velocities reads as zeros). */
+int gvbd_state_write(const char* json_path, int32_t frame_id, double cur_time, int32_t binary, int32_t num_meshes,
+                     const int32_t* num_vertices, const float* positions, const float* velocities);
+int gvbd_state_read(const char* json_path, int32_t max_meshes, int32_t* num_meshes, int32_t* num_vertices,
+                    int64_t max_floats, float* positions, float* velocities, int32_t* frame_id, double* cur_time);
+int gvbd_write_ply(const char* path, int32_t num_vertices, const float* positions, int32_t num_faces,
+                   const int32_t* faces);
+
 /* ---- batched stepping: many scenes per launch ---- */
 int gvbd_batch_create(gvbd_context* ctx, const int32_t* scenes, int32_t n, int32_t* batch_out);
 int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch);

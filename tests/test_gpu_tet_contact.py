@@ -137,3 +137,28 @@ def test_tet_contact_json_default_off(gpu_ctx, tmp_path):
     sim.recordVBDSolveGraph(True)
     sim.runStep()
     sim.destroy()
+
+
+def test_contact_stats_count_frames_and_detections(gpu_ctx):
+    """gvbd_scene_contact_stats: frames and iterations stepped with collision handling, at least
+    one detection pass per frame, detection time measured; scenes without collision handling
+    have no such counters."""
+    import gaia_amd
+
+    spec = tet_drop_spec(steps=6)
+    sim = build_gpu(gpu_ctx, spec)
+    sim.runSteps(6)
+    st = sim.contactStats()
+    iters = spec["physics"]["iterations"]
+    assert st["frames"] == 6 and st["iterations"] == 6 * iters
+    assert 6 <= st["detections"] <= 6 * iters and st["rebuilds"] >= 1
+    assert st["detect_ms"] > 0 and st["vf_pairs"] >= 0 and st["ee_pairs"] >= 0
+    sim.destroy()
+    pos, tets = scenes.cube_tet_mesh(2, 0.1, origin=(0.0, 0.5, 0.0))
+    plain = VBDPhysics(gpu_ctx)
+    plain.setPhysicsParams(handle_collision=0)
+    plain.addTetMesh(pos, tets)
+    plain.initialize()
+    with pytest.raises(gaia_amd.VBDError, match="no collision handling"):
+        plain.contactStats()
+    plain.destroy()
