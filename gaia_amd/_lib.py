@@ -53,6 +53,8 @@ class PhysicsParams(ctypes.Structure):
         ("contact_energy_type", ctypes.c_int32),
         ("contact_query_radius", ctypes.c_float),
         ("contact_bvh_rebuild_steps", ctypes.c_int32),
+        ("use_accelerator", ctypes.c_int32),
+        ("accelerator_rho", ctypes.c_float),
     ]
 
 

@@ -145,7 +145,17 @@ struct SceneView {
     ContactView cv;
     const float* initBound;  // per vertex bound of initialisation type 3 (nullptr: unbounded)
     const int4* tcontact;    // contact scenes, per tet slot: [tFaceRef range), [tEdgeRef range)
+    // Chebyshev acceleration (scenes with use_accelerator; nullptr otherwise): positions two and
+    // one iterations back, and omega per iteration
+    float4* accPrevPrev;
+    float4* accPrevIter;
+    const float* accOmega;
 };
+
+// vflag bits: 1 fixed, (code << 1) initialisation type, 16 accelerated (tet vertex of a scene
+// with use_accelerator)
+constexpr uint8_t VFLAG_FIXED = 1;
+constexpr uint8_t VFLAG_ACCEL = 16;
 
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`
 // (tet slots or cloth slots by `kind`; vertex ranges for the per-vertex kernels).

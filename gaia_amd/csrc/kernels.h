@@ -40,6 +40,11 @@ hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, i
 // Items cover vertex ranges [slotBegin, slotBegin + count), count <= 256.
 hipError_t launch_initial_step(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 hipError_t launch_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
+// Chebyshev acceleration over the per-vertex items: history init after the initial step, and the
+// accelerated update after iteration `iteration` (scenes without use_accelerator are skipped).
+hipError_t launch_accel_init(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
+hipError_t launch_accel_apply(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
+                              hipStream_t stream);
 
 // dst[i] = src[i] for n float4 elements (bandwidth / profiler calibration).
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream);

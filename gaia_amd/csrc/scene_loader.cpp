@@ -141,6 +141,8 @@ gvbd_physics_params default_physics_params() {
     p.contact_energy_type = 1;
     p.contact_query_radius = 0.4f;  // ClothContactDetectorParameters() sets maxQueryDis = 0.4
     p.contact_bvh_rebuild_steps = 32;
+    p.use_accelerator = 0;
+    p.accelerator_rho = 0.93f;
     return p;
 }
 
@@ -183,6 +185,8 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         get_num(*pp, "thickness", p.thickness);
         get_num(*pp, "conservativeStepRelaxation", p.conservative_step_relaxation);
         get_int(*pp, "contactBVHReconstructionSteps", p.contact_bvh_rebuild_steps);
+        get_int(*pp, "useAccelerator", p.use_accelerator);
+        get_num(*pp, "acceleratorRho", p.accelerator_rho);
         // BaseClothPhsicsFramework::parseRunningParameters (Framework/BaseClothSimPhsicsFramework.cpp:17-28)
         if (const json::Value* cdp = pp->get("ContactDetectorParams")) get_num(*cdp, "maxQueryDis", p.contact_query_radius);
         int32_t useBowl = 0;

@@ -886,6 +886,45 @@ __global__ __launch_bounds__(256) void initial_step_kernel(const SceneView* __re
     S.vel[v] = make_float4(vn.x, vn.y, vn.z, 0.f);
 }
 
+// Chebyshev acceleration (runStepGPU, VBDPhysics.cpp:872-906): after the initial step both
+// history buffers take the current positions (recordInitialPositionForAccelerator, :3577-3592).
+__global__ __launch_bounds__(256) void accel_init_kernel(const SceneView* __restrict__ scenes,
+                                                         const WorkItem* __restrict__ items) {
+    const WorkItem it = items[blockIdx.x];
+    const int i = threadIdx.x;
+    if (i >= it.count) return;
+    const SceneView& S = scenes[it.scene];
+    if (!S.accOmega) return;
+    const int v = it.slotBegin + i;
+    if (!(S.vflag[v] & VFLAG_ACCEL)) return;
+    const float4 p = S.pos[v];
+    S.accPrevPrev[v] = p;
+    S.accPrevIter[v] = p;
+}
+
+// After iteration k's sweep (VBDSolveParallelGroup_applyAcceleration_kernel,
+// VBDPhysicsCompute.cu:910-944): x <- y_{k-1} + omega_k (x - y_{k-1}); y_{k-1} <- y_k (the positions
+// before this sweep); and the accelerated positions become the next sweep's y_k
+// (recordPrevIterPositionsAccelerator, VBDPhysics.cpp:3594-3606). Fixed vertices are left alone.
+__global__ __launch_bounds__(256) void accel_apply_kernel(const SceneView* __restrict__ scenes,
+                                                          const WorkItem* __restrict__ items, int iteration) {
+    const WorkItem it = items[blockIdx.x];
+    const int i = threadIdx.x;
+    if (i >= it.count) return;
+    const SceneView& S = scenes[it.scene];
+    if (!S.accOmega) return;
+    const int v = it.slotBegin + i;
+    const uint8_t flag = S.vflag[v];
+    if (!(flag & VFLAG_ACCEL) || (flag & VFLAG_FIXED)) return;
+    const float omega = S.accOmega[iteration];
+    const f3 x = xyz(S.pos[v]);
+    const f3 pp = xyz(S.accPrevPrev[v]);
+    const f3 xn = pp + omega * (x - pp);
+    S.pos[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
+    S.accPrevPrev[v] = S.accPrevIter[v];
+    S.accPrevIter[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
+}
+
 // VBDPhysics.cpp:2486-2561 updateVelocities + dampVelocities (cloth: VBDClothPhysics.cpp:2318-2360).
 __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* __restrict__ scenes,
                                                               const WorkItem* __restrict__ items) {
@@ -1008,6 +1047,19 @@ hipError_t launch_counter_reset(int* counters, int n, hipStream_t stream) {
 hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(cloth_commit_kernel, dim3(nitems), dim3(256), 0, stream, scenes, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel_init(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(accel_init_kernel, dim3(nitems), dim3(256), 0, stream, scenes, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel_apply(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
+                              hipStream_t stream) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(accel_apply_kernel, dim3(nitems), dim3(256), 0, stream, scenes, items, iteration);
     return hipGetLastError();
 }
 

@@ -48,6 +48,8 @@ struct DeviceScene {
     int* tFaceRef = nullptr;    // contact scenes: per tet slot incident boundary face, face id | order << 30
     int* tEdgeRef = nullptr;    // contact scenes: per tet slot incident boundary edge, edge id | order << 30
     int4* tcontact = nullptr;   // contact scenes: per tet slot, its tFaceRef and tEdgeRef ranges
+    float4 *accPrevPrev = nullptr, *accPrevIter = nullptr;  // use_accelerator: iteration history
+    float* accOmega = nullptr;                              // use_accelerator: omega per iteration
 };
 
 // Boundary surface of a tet mesh in a scene with collision handling: its contact faces, their
@@ -130,6 +132,7 @@ struct PlanObj {
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
     bool contact = false;  // single cloth scene with collision handling: host-driven iterations
+    bool accel = false;    // some scene uses the Chebyshev accelerator
 };
 
 // Serving loop: registered scenes, their queued step requests, and the plan of one tick.
@@ -434,6 +437,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     // contact references (collision handling): scene-global face and edge ids follow the mesh
     // order, as the detector numbers them
     const bool contact = s.params.handle_collision != 0;
+    const bool accel = s.params.use_accelerator != 0 && !s.tets.empty();
     std::vector<int32_t> faceRef, edgeRef, tFaceRef, tEdgeRef;
     std::vector<int4> tcontact;
     std::vector<ClothContactRefs> refs(s.meshes.size());
@@ -545,7 +549,19 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             vflag[off + i] = (uint8_t)(code << 1);
         }
         for (int32_t i = 0; i < n; ++i)
-            if (fixedMask[im][i]) vflag[off + i] |= 1;
+            if (fixedMask[im][i]) vflag[off + i] |= VFLAG_FIXED;
+        if (accel && s.meshes[im].first == MESH_TET)
+            for (int32_t i = 0; i < n; ++i) vflag[off + i] |= VFLAG_ACCEL;
+    }
+    // omega per iteration (getAcceleratorOmega, VBDPhysics.cpp:3558-3575), in float as there
+    std::vector<float> omega;
+    if (accel) {
+        const float rho2 = s.params.accelerator_rho * s.params.accelerator_rho;
+        float w = 1.f;
+        for (int k = 1; k <= std::max(s.params.iterations, 1); ++k) {
+            w = k == 1 ? 1.f : (k == 2 ? 2.f / (2.f - rho2) : 4.f / (4.f - rho2 * w));
+            omega.push_back(w);
+        }
     }
     // one arena
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -562,7 +578,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bTF = contact ? align(sizeof(int32_t) * std::max<size_t>(tFaceRef.size(), 1)) : 0;
     const size_t bTE = contact ? align(sizeof(int32_t) * std::max<size_t>(tEdgeRef.size(), 1)) : 0;
     const size_t bTC = contact ? align(sizeof(int4) * std::max<size_t>(tcontact.size(), 1)) : 0;
-    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC;
+    const size_t bA = accel ? bV : 0, bW = accel ? align(sizeof(float) * omega.size()) : 0;
+    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -599,6 +616,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         d.tEdgeRef = (int*)take(bTE);
         d.tcontact = (int4*)take(bTC);
     }
+    if (accel) {
+        d.accPrevPrev = (float4*)take(bA);
+        d.accPrevIter = (float4*)take(bA);
+        d.accOmega = (float*)take(bW);
+    }
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "upload");
     };
@@ -622,6 +644,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         up(d.tEdgeRef, tEdgeRef.data(), sizeof(int32_t) * tEdgeRef.size());
         up(d.tcontact, tcontact.data(), sizeof(int4) * tcontact.size());
     }
+    if (accel) {
+        up(d.accPrevPrev, pos.data(), sizeof(float4) * nv);
+        up(d.accPrevIter, pos.data(), sizeof(float4) * nv);
+        up(d.accOmega, omega.data(), sizeof(float) * omega.size());
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     SceneView& v = s.view;
     v.pos = d.pos;
@@ -637,6 +664,9 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     v.cface = d.cface;
     v.cbend = d.cbend;
     v.tcontact = d.tcontact;
+    v.accPrevPrev = d.accPrevPrev;
+    v.accPrevIter = d.accPrevIter;
+    v.accOmega = d.accOmega;
     v.nv = nv;
     v.sp = s.sp;
 }
@@ -648,10 +678,12 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     const int vpb = p.block / p.lpv;
     std::vector<SceneView> views;
     p.numGroups = 0;
+    p.accel = false;
     for (int32_t sid : p.scenes) {
         SceneObj& s = scene_of(ctx, sid);
         views.push_back(s.view);
         p.numGroups = std::max(p.numGroups, (int)s.groups.size());
+        if (s.view.accOmega) p.accel = true;
     }
     std::vector<WorkItem> items, commits, vitems;
     p.groupItemBegin.assign(p.numGroups + 1, 0);
@@ -760,6 +792,7 @@ void plan_reset_counters(gvbd_context* ctx, PlanObj& p);
 // Initial step of a substep; in work-queue mode it also starts a new counter epoch.
 void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
     hip_check(launch_initial_step(p.dViews, p.dVItems, p.nVItems, ctx->stream), "initial step launch");
+    if (p.accel) hip_check(launch_accel_init(p.dViews, p.dVItems, p.nVItems, ctx->stream), "accelerator launch");
     if (p.mode == GVBD_MODE_WORK_QUEUE) plan_reset_counters(ctx, p);
 }
 
@@ -794,6 +827,12 @@ void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
     } else {
         for (int g = 0; g < p.numGroups; ++g) plan_group(ctx, p, g, it);
     }
+    // Chebyshev acceleration of the sweep (omega of iteration it; the per-iteration hook may be
+    // called past the configured count, which keeps the last omega)
+    if (p.accel)
+        hip_check(launch_accel_apply(p.dViews, p.dVItems, p.nVItems, std::max(0, std::min(it, p.iterations - 1)),
+                                     ctx->stream),
+                  "accelerator launch");
 }
 
 bool scene_collision_detection(SceneObj& s, bool firstIteration);
@@ -914,6 +953,13 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
     const bool contact = s.params.handle_collision != 0;
     if (contact && !(s.params.contact_query_radius > 0.f))
         fail(GVBD_ERR_INVALID, "contact_query_radius must be positive");
+    if (s.params.use_accelerator) {
+        if (contact)
+            fail(GVBD_ERR_INVALID, "use_accelerator applies to tet scenes without collision handling (the "
+                                   "reference's accelerator is part of its tet GPU path)");
+        if (!(s.params.accelerator_rho >= 0.f && s.params.accelerator_rho < 1.f))
+            fail(GVBD_ERR_INVALID, "accelerator_rho must be in [0, 1)");
+    }
     s.sp = make_solve_params(s.params, s);
     // tet meshes take part in contact through their boundary surface
     s.tetSurfaces.clear();
@@ -1959,10 +2005,11 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     DeviceScene& d = s.dev;
     mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
     mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
-    mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact);
+    mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact), mv(d.accPrevPrev), mv(d.accPrevIter), mv(d.accOmega);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
-    mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact);
+    mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact), mv(v.accPrevPrev), mv(v.accPrevIter);
+    mv(v.accOmega);
 }
 
 // Copies an idle scene's arena out to host memory and frees it.

@@ -45,6 +45,9 @@ DEFAULT_PHYSICS = dict(
     contact_energy_type=1,
     contact_query_radius=0.4,
     contact_bvh_rebuild_steps=32,
+    # Chebyshev acceleration of the tet iterations (VBD/VBDPhysicsParameters.h:64-65)
+    use_accelerator=0,
+    accelerator_rho=0.93,
 )
 
 # VBDObjectParamsTriMeshStVk defaults (VBDCloth/VBDTriMeshStVK.h:17-20, VBDCloth/VBDBaseTriMesh.h:12-16,

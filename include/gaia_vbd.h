@@ -91,6 +91,10 @@ typedef struct gvbd_physics_params {
     int32_t contact_energy_type;            /* contactEnergyTypes (1) */
     float contact_query_radius;             /* ContactDetectorParams.maxQueryDis (0.4) */
     int32_t contact_bvh_rebuild_steps;      /* contactBVHReconstructionSteps (32): full BVH rebuild period in frames */
+    /* Chebyshev acceleration of the tet iterations (useAccelerator, acceleratorRho 0.93,
+     * VBD/VBDPhysicsParameters.h:64-65; runStepGPU, VBD/VBDPhysics.cpp:872-906) */
+    int32_t use_accelerator;
+    float accelerator_rho;                  /* in [0, 1) */
 } gvbd_physics_params;
 
 /* One tet mesh (ObjectParametersVBDNeoHookean, VBD/VBD_NeoHookean.cpp:1613-1625,

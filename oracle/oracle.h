@@ -43,7 +43,10 @@ enum OraclePhysicsParam {
     OP_CONSERVATIVE_RELAXATION,
     OP_CONTACT_ENERGY_TYPE,
     OP_MAX_QUERY_DIS,
-    OP_COUNT = 32
+    /* Chebyshev acceleration of the tet iterations (VBDPhysicsParameters.h:64-65) */
+    OP_USE_ACCELERATOR,
+    OP_ACCELERATOR_RHO,
+    OP_COUNT = 34
 };
 
 enum OracleClothMaterialParam {

@@ -15,8 +15,8 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 OP = dict(GRAVITY_X=0, GRAVITY_Y=1, GRAVITY_Z=2, DT=3, NUM_SUBSTEPS=4, ITERATIONS=5, LOWER_X=6, LOWER_Y=7,
           LOWER_Z=8, UPPER_X=9, UPPER_Y=10, UPPER_Z=11, USE_PLANE_GROUND=12, BOUNDARY_STIFFNESS=13,
           BOUNDARY_FRICTION_MU=14, BOUNDARY_FRICTION_EPS_V=15, STEP_SIZE=16, STEP_SIZE_GD=17,
-          FRICTION_START_ITER=18, ASSOCIATE_GRAVITY=19, INIT_RATIO_G=21)
-OP_COUNT = 32
+          FRICTION_START_ITER=18, ASSOCIATE_GRAVITY=19, INIT_RATIO_G=21, USE_ACCELERATOR=32, ACCELERATOR_RHO=33)
+OP_COUNT = 34
 OM = dict(DENSITY=0, MU=1, LAMBDA=2, DAMP_HYDRO=3, DAMP_DEV=4, EXP_VEL_DAMPING=5, CONST_VEL_DAMPING=6, INIT_TYPE=7,
           FRICTION_MU=8, FRICTION_EPS_V=9)
 OM_COUNT = 10
@@ -87,6 +87,8 @@ def physics_vector(p):
     v[OP["FRICTION_START_ITER"]] = p["friction_start_iter"]
     v[OP["ASSOCIATE_GRAVITY"]] = p["associate_gravity_with_inertia"]
     v[OP["INIT_RATIO_G"]] = p.get("init_ratio_g", 0.5)
+    v[OP["USE_ACCELERATOR"]] = p.get("use_accelerator", 0)
+    v[OP["ACCELERATOR_RHO"]] = p.get("accelerator_rho", 0.93)
     return v
 
 

@@ -83,6 +83,7 @@ inline void vertex_friction(R mu, R lambda, const R* T, const R* u, R epsU, R* f
 template <class R> struct Mesh {
     int nv = 0, nt = 0;
     std::vector<R> x, xprev, v, vprev, y, dminv, vol, mass;
+    std::vector<R> accPrevPrev, accPrevIter;  // Chebyshev acceleration history
     std::vector<int32_t> tets;
     std::vector<uint8_t> fixed;
     std::vector<int32_t> fixedList;
@@ -216,13 +217,28 @@ template <class R> void update_velocities(Scene<R>& s, Mesh<R>& m) {
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
 }
 
+// VBDPhysics::getAcceleratorOmega (VBDPhysics.cpp:3558-3575), order from 1
+template <class R> R accelerator_omega(int order, R rho, R prevOmega) {
+    if (order == 1) return R(1);
+    if (order == 2) return R(2) / (R(2) - rho * rho);
+    return R(4) / (R(4) - rho * rho * prevOmega);
+}
+
 template <class R> void scene_step(Scene<R>& s, int nthreads) {
     const int nsub = (int)s.p[OP_NUM_SUBSTEPS];
     const int iters = (int)s.p[OP_ITERATIONS];
     const int fsi = (int)s.p[OP_FRICTION_START_ITER];
+    const bool accel = s.p[OP_USE_ACCELERATOR] != 0;
+    const R rho = (R)s.p[OP_ACCELERATOR_RHO];
     for (int sub = 0; sub < nsub; ++sub) {
         for (auto& m : s.meshes) apply_initial_step(s, m);
+        // Chebyshev acceleration (runStepGPU, VBDPhysics.cpp:872-906): both histories start at
+        // the initial guess
+        R omega = R(1);
+        if (accel)
+            for (auto& m : s.meshes) m.accPrevPrev = m.accPrevIter = m.x;
         for (int it = 0; it < iters; ++it) {
+            if (accel) omega = accelerator_omega<R>(it + 1, rho, omega);
             const bool fr = it >= fsi;
             for (auto& grp : s.groups) {
                 const int n = (int)grp.size() / 2;
@@ -236,6 +252,21 @@ template <class R> void scene_step(Scene<R>& s, int nthreads) {
                 else
                     for (int i = 0; i < n; ++i) body(i);
             }
+            // VBDSolveParallelGroup_applyAcceleration_kernel (VBDPhysicsCompute.cu:910-944):
+            // x <- y_{k-1} + omega (x - y_{k-1}), y_{k-1} <- y_k; the next sweep's y_k is the
+            // accelerated x (recordPrevIterPositionsAccelerator, VBDPhysics.cpp:3594-3606)
+            if (accel)
+                for (auto& m : s.meshes)
+                    for (int v = 0; v < m.nv; ++v) {
+                        if (m.fixed[v]) continue;
+                        for (int d = 0; d < 3; ++d) {
+                            const size_t k = 3 * (size_t)v + d;
+                            const R pp = m.accPrevPrev[k];
+                            m.x[k] = pp + omega * (m.x[k] - pp);
+                            m.accPrevPrev[k] = m.accPrevIter[k];
+                            m.accPrevIter[k] = m.x[k];
+                        }
+                    }
         }
         for (auto& m : s.meshes) update_velocities(s, m);
     }
