@@ -32,6 +32,14 @@ struct alignas(16) TetInc {
 };
 static_assert(sizeof(TetInc) == 64, "TetInc must be 64 bytes");
 
+// One tet for energy evaluation: scene vertex ids, DmInv (column-major), rest volume, moduli.
+struct alignas(16) TetEnergyRec {
+    int4 v;
+    float dminv[9];
+    float A, mu, lmbd;
+};
+static_assert(sizeof(TetEnergyRec) == 64, "TetEnergyRec must be 64 bytes");
+
 enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
     SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)

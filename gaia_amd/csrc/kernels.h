@@ -46,6 +46,14 @@ hipError_t launch_accel_init(const SceneView* scenes, const WorkItem* items, int
 hipError_t launch_accel_apply(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
                               hipStream_t stream);
 
+// Convergence evaluation of one scene (VBDPhysics::evaluateConvergenceGPU): per-block partial
+// sums of |force| over the tet slots, of m |y - x|^2 over the tet vertices `tetVerts` and of the
+// elastic energy over `tets`; convergence_blocks(n) blocks of partials each.
+int convergence_blocks(int n);
+hipError_t launch_convergence(const SceneView& view, int nslots, const TetEnergyRec* tets, int nt,
+                              const int* tetVerts, int ntv, double* forcePartial, double* inertiaPartial,
+                              double* energyPartial, hipStream_t stream);
+
 // dst[i] = src[i] for n float4 elements (bandwidth / profiler calibration).
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 

@@ -86,11 +86,6 @@ __device__ __forceinline__ void boundary_force(const SolveParams& sp, float k, f
     h.zz = hd[2];
 }
 
-__device__ __forceinline__ f3 sym_mul(const Sym3& h, f3 v) {
-    return mk(h.xx * v.x + h.xy * v.y + h.xz * v.z, h.xy * v.x + h.yy * v.y + h.yz * v.z,
-              h.xz * v.x + h.yz * v.y + h.zz * v.z);
-}
-
 // ---------------------------------------------------------------------------
 // Incidence records: read straight from HBM by the lanes that use them (STAGE = 0), or staged
 // through LDS in chunks of whole-block coalesced loads (STAGE records of 64 B per chunk). A

@@ -88,6 +88,8 @@ struct SceneObj {
     std::unique_ptr<ContactDetector> cd;
     int64_t frame = 0;
     gvbd_contact_stats contactStats{};       // gvbd_scene_contact_stats
+    void* convBuf = nullptr;                 // convergence evaluation: tet records + partial sums
+    size_t convBytes = 0;
     hipEvent_t cdEvents[2] = {nullptr, nullptr};  // around a frame's detection pass
     // serving loop residency: the server that owns it (-1: none) and, while copied out to host
     // memory, the arena's bytes and the device address they were copied from
@@ -1245,6 +1247,8 @@ int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
             ctx->plans.erase(s.plan);
         }
         s.cd.reset();  // reads the arena's positions: released first
+        if (s.convBuf) (void)hipFree(s.convBuf);
+        s.convBuf = nullptr;
         for (hipEvent_t& e : s.cdEvents)
             if (e) {
                 (void)hipEventDestroy(e);
@@ -1588,6 +1592,87 @@ int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stat
             out->vf_pairs = c.vfPairs;
             out->ee_pairs = c.eeContacts;
         }
+    });
+}
+
+int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
+                                    double* elastic_energy, double* avg_force_norm) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
+        if (!s.dev.arena) fail(GVBD_ERR_STATE, "scene is held in host memory by its server (not resident)");
+        int64_t ntet = 0, ntetVerts = 0;
+        for (const TetMeshHost& m : s.tets) {
+            ntet += m.nt;
+            ntetVerts += m.nv;
+        }
+        const int nslots = (int)s.numTetSlots();
+        const int bs = convergence_blocks(std::max(nslots, 1)), bt = convergence_blocks((int)std::max<int64_t>(ntet, 1));
+        const int bv = convergence_blocks((int)std::max<int64_t>(ntetVerts, 1));
+        const size_t recBytes = sizeof(TetEnergyRec) * (size_t)std::max<int64_t>(ntet, 1);
+        const size_t vertBytes = (sizeof(int32_t) * (size_t)std::max<int64_t>(ntetVerts, 1) + 15) & ~(size_t)15;
+        const size_t need = recBytes + vertBytes + sizeof(double) * (size_t)(bs + bv + bt);
+        if (!s.convBuf) {
+            // the tets of every tet mesh with scene vertex ids (built once)
+            std::vector<TetEnergyRec> recs;
+            std::vector<int32_t> verts;
+            recs.reserve((size_t)ntet);
+            for (size_t im = 0; im < s.meshes.size(); ++im) {
+                if (s.meshes[im].first != MESH_TET) continue;
+                const TetMeshHost& m = s.tets[s.meshes[im].second];
+                const int32_t off = s.vertexOffset[im];
+                for (int32_t i = 0; i < m.nv; ++i) verts.push_back(off + i);
+                for (int32_t t = 0; t < m.nt; ++t) {
+                    TetEnergyRec r{};
+                    r.v = make_int4(off + m.tets[4 * (size_t)t], off + m.tets[4 * (size_t)t + 1],
+                                    off + m.tets[4 * (size_t)t + 2], off + m.tets[4 * (size_t)t + 3]);
+                    for (int k = 0; k < 9; ++k) r.dminv[k] = m.dminv[9 * (size_t)t + k];
+                    r.A = m.vol[t];
+                    r.mu = m.mat.mu;
+                    r.lmbd = m.mat.lambda;
+                    recs.push_back(r);
+                }
+            }
+            hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+            if (hipMalloc(&s.convBuf, need) != hipSuccess) {
+                (void)hipGetLastError();
+                s.convBuf = nullptr;
+                fail(GVBD_ERR_NOMEM, "hipMalloc for convergence evaluation failed");
+            }
+            s.convBytes = need;
+            if (!recs.empty())
+                hip_check(hipMemcpyAsync(s.convBuf, recs.data(), sizeof(TetEnergyRec) * recs.size(),
+                                         hipMemcpyHostToDevice, ctx->stream),
+                          "upload");
+            if (!verts.empty())
+                hip_check(hipMemcpyAsync((char*)s.convBuf + recBytes, verts.data(), sizeof(int32_t) * verts.size(),
+                                         hipMemcpyHostToDevice, ctx->stream),
+                          "upload");
+            hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
+        }
+        auto* recs = (const TetEnergyRec*)s.convBuf;
+        auto* tverts = (const int*)((char*)s.convBuf + recBytes);
+        auto* part = (double*)((char*)s.convBuf + recBytes + vertBytes);
+        hip_check(launch_convergence(s.view, nslots, recs, (int)ntet, tverts, (int)ntetVerts, part, part + bs,
+                                     part + bs + bv, ctx->stream),
+                  "convergence launch");
+        std::vector<double> h((size_t)(bs + bv + bt), 0.0);
+        hip_check(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream),
+                  "download");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        double fsum = 0.0, isum = 0.0, esum = 0.0;
+        if (nslots > 0)
+            for (int b = 0; b < bs; ++b) fsum += h[b];
+        if (ntetVerts > 0)
+            for (int b = 0; b < bv; ++b) isum += h[bs + b];
+        if (ntet > 0)
+            for (int b = 0; b < bt; ++b) esum += h[bs + bv + b];
+        const double eIn = 0.5 * (double)s.sp.dtSqrRecip * isum;
+        if (inertia_energy) *inertia_energy = eIn;
+        if (elastic_energy) *elastic_energy = esum;
+        if (energy) *energy = eIn + esum;
+        if (avg_force_norm) *avg_force_norm = ntetVerts ? fsum / (double)ntetVerts : 0.0;
     });
 }
 

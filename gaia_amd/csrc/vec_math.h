@@ -23,6 +23,11 @@ struct Sym3 {
     float xx, xy, xz, yy, yz, zz;
 };
 
+__device__ __forceinline__ f3 sym_mul(const Sym3& h, f3 v) {
+    return mk(h.xx * v.x + h.xy * v.y + h.xz * v.z, h.xy * v.x + h.yy * v.y + h.yz * v.z,
+              h.xz * v.x + h.yz * v.y + h.zz * v.z);
+}
+
 // Solves H dx = b for a symmetric 3x3 block by adjugate / determinant.
 // Stands in for CuMatrix::solve3x3_psd_stable (VBDPhysics.cpp:2449,
 // VBDPhysicsCompute.cu:896), whose source is not available: failure iff the

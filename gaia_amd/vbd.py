@@ -419,6 +419,13 @@ class VBDPhysics:
         self.ctx.check(lib().gvbd_scene_contact_stats(self.ctx.handle, self.scene, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in ContactStats._fields_}
 
+    def evaluateConvergence(self):
+        """VBDPhysics::evaluateConvergenceGPU: merit energy, its inertia and elastic parts, and the
+        average force norm over the tet vertices, at the current positions."""
+        out = [ctypes.c_double() for _ in range(4)]
+        self.ctx.check(lib().gvbd_scene_evaluate_convergence(self.ctx.handle, self.scene, *[ctypes.byref(o) for o in out]))
+        return dict(zip(("energy", "inertia_energy", "elastic_energy", "avg_force_norm"), (o.value for o in out)))
+
     def recordVBDSolveGraph(self, enable=True):
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))
 

@@ -239,6 +239,14 @@ typedef struct gvbd_contact_stats {
     int64_t ee_pairs;        /* and edge-edge candidates */
 } gvbd_contact_stats;
 int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out);
+/* VBDPhysics::evaluateConvergenceGPU (VBD/VBDPhysics.cpp:3510-3556): at the current positions and
+ * inertia targets, the merit energy (inertia 1/(2 dt^2) sum m |y - x|^2 + stable neo-Hookean
+ * elastic energy), its two parts, and the average over the tet vertices of |material + damping +
+ * inertia force| (fixed vertices count with zero force). Synchronous; any NULL output is skipped.
+ * Between frames it measures how far the last frame's iterate is from the implicit-Euler
+ * solution; with the per-iteration hooks it reproduces the reference's per-iteration log. */
+int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
+                                    double* elastic_energy, double* avg_force_norm);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);

@@ -113,6 +113,8 @@ int oracle_tet_scene_group(OracleTetScene*, int g, int32_t* out);
 int oracle_tet_scene_step(OracleTetScene*, int nsteps, int nthreads);
 int oracle_tet_scene_get_state(OracleTetScene*, int mesh, double* pos, double* vel);
 int oracle_tet_scene_set_state(OracleTetScene*, int mesh, const double* pos, const double* vel);
+/* VBDPhysics::evaluateConvergence: out[4] = {energy, inertia energy, elastic energy, avg force norm} */
+int oracle_tet_scene_evaluate_convergence(OracleTetScene*, double* out);
 
 /* --- cloth: TriMesh precompute + topology + bending forms (TriMesh.cpp, VBDTriMeshStVK.cpp:50-88) ---
  * dminv 4F col-major, area F, mass N, edges 6E (eV1 eV2 eV12Next eV21Next fId1 fId2), Q 16E row-major;

@@ -54,6 +54,7 @@ def lib():
         L.oracle_tet_scene_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.oracle_tet_scene_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.oracle_tet_scene_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_tet_scene_evaluate_convergence.argtypes = [ctypes.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -207,6 +208,11 @@ class TetScene:
 
     def step(self, n=1, threads=1):
         lib().oracle_tet_scene_step(self._h, n, threads)
+
+    def evaluate_convergence(self):
+        out = np.zeros(4)
+        lib().oracle_tet_scene_evaluate_convergence(self._h, out.ctypes.data_as(_dp))
+        return dict(zip(("energy", "inertia_energy", "elastic_energy", "avg_force_norm"), out.tolist()))
 
     def get_state(self, mesh=0):
         nv = self.nv[mesh]

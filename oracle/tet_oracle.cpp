@@ -272,6 +272,65 @@ template <class R> void scene_step(Scene<R>& s, int nthreads) {
     }
 }
 
+// VBDPhysics::evaluateConvergence (VBDPhysics.cpp:3459-3508) with evaluateMeritEnergy
+// (:3784-3798), computeInertiaEnergy (VBD_BaseMaterial.h:223-240) and computeElasticEnergy
+// (VBD_NeoHookean.h:64-84): out = {energy, inertia energy, elastic energy, average force norm}.
+template <class R> void evaluate_convergence(Scene<R>& s, double* out) {
+    double eIn = 0.0, eEl = 0.0, fsum = 0.0;
+    int64_t nAll = 0;
+    for (auto& m : s.meshes) {
+        nAll += m.nv;
+        const bool started = m.y.size() == m.x.size();
+        for (int t = 0; t < m.nt; ++t) {
+            R x4[12];
+            for (int j = 0; j < 4; ++j)
+                for (int d = 0; d < 3; ++d) x4[3 * j + d] = m.x[3 * m.tets[4 * t + j] + d];
+            const R* DmInv = &m.dminv[9 * t];
+            R F[9];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) {
+                    R acc = 0;
+                    for (int k = 0; k < 3; ++k) acc += (x4[3 * (k + 1) + r] - x4[r]) * DmInv[k + 3 * c];
+                    F[r + 3 * c] = acc;
+                }
+            const R a = R(1) + m.mu / m.lmbd;
+            const R A = m.vol[t];
+            R fn = 0;
+            for (int i = 0; i < 9; ++i) fn += F[i] * F[i];
+            const R k = det3(F) - a;
+            R energy = R(0.5) * A * (m.lmbd * k * k + m.mu * (fn - R(3)));
+            energy -= R(0.5) * A * m.lmbd * (R(1) - a) * (R(1) - a);
+            eEl += (double)energy;
+        }
+        for (int v = 0; v < m.nv; ++v) {
+            const R* xv = &m.x[3 * v];
+            const R* yv = started ? &m.y[3 * v] : xv;
+            R dy[3] = {yv[0] - xv[0], yv[1] - xv[1], yv[2] - xv[2]};
+            eIn += (double)(m.mass[v] * (dy[0] * dy[0] + dy[1] * dy[1] + dy[2] * dy[2]));
+            if (m.fixed[v]) continue;
+            R f[3] = {0, 0, 0}, h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            R disp[3] = {0, 0, 0};
+            if (started)
+                for (int d = 0; d < 3; ++d) disp[d] = xv[d] - m.xprev[3 * v + d];
+            for (int k = m.incBegin[v]; k < m.incBegin[v + 1]; ++k) {
+                const int t = m.incTet[k];
+                R x4[12];
+                for (int j = 0; j < 4; ++j)
+                    for (int d = 0; d < 3; ++d) x4[3 * j + d] = m.x[3 * m.tets[4 * t + j] + d];
+                neohookean_vertex<R>(x4, &m.dminv[9 * t], m.vol[t], m.mu, m.lmbd, m.incOrder[k], m.dampH, m.dampD,
+                                     s.dt, disp, f, h);
+            }
+            for (int d = 0; d < 3; ++d) f[d] += m.mass[v] * dy[d] * s.dtSqrRecip;
+            fsum += std::sqrt((double)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
+        }
+    }
+    eIn *= 0.5 * (double)s.dtSqrRecip;
+    out[0] = eIn + eEl;
+    out[1] = eIn;
+    out[2] = eEl;
+    out[3] = nAll ? fsum / (double)nAll : 0.0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -450,6 +509,13 @@ int oracle_tet_scene_step(OracleTetScene* sc, int nsteps, int nthreads) {
 
 int oracle_tet_scene_get_state(OracleTetScene* sc, int mesh, double* pos, double* vel) {
     return sc->precision == 64 ? get_state(sc->s64, mesh, pos, vel) : get_state(sc->s32, mesh, pos, vel);
+}
+int oracle_tet_scene_evaluate_convergence(OracleTetScene* sc, double* out) {
+    if (sc->precision == 64)
+        evaluate_convergence(sc->s64, out);
+    else
+        evaluate_convergence(sc->s32, out);
+    return 0;
 }
 int oracle_tet_scene_set_state(OracleTetScene* sc, int mesh, const double* pos, const double* vel) {
     return sc->precision == 64 ? set_state(sc->s64, mesh, pos, vel) : set_state(sc->s32, mesh, pos, vel);
