@@ -145,6 +145,35 @@ class ContactStats(ctypes.Structure):
     ]
 
 
+class DeformerDesc(ctypes.Structure):
+    """gvbd_deformer_desc."""
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("center", ctypes.c_float * 3),
+        ("axis", ctypes.c_float * 3),
+        ("speed", ctypes.c_float * 3),
+        ("start_time", ctypes.c_float),
+        ("angular_velocity", ctypes.c_double),
+        ("end_time", ctypes.c_double),
+        ("num_meshes", ctypes.c_int32),
+        ("meshes", ctypes.POINTER(ctypes.c_int32)),
+        ("num_vertices", ctypes.POINTER(ctypes.c_int32)),
+        ("vertices", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
+class AttachmentDesc(ctypes.Structure):
+    """gvbd_attachment_desc."""
+    _fields_ = [
+        ("mesh", ctypes.c_int32),
+        ("vertex", ctypes.c_int32),
+        ("target_mesh", ctypes.c_int32),
+        ("face", ctypes.c_int32 * 3),
+        ("bary", ctypes.c_float * 3),
+        ("stiffness", ctypes.c_float),
+    ]
+
+
 class ServerStats(ctypes.Structure):
     """gvbd_server_stats."""
     _fields_ = [
@@ -206,6 +235,10 @@ _SIGNATURES = {
     "gvbd_scene_update_velocities": [_P, _I],
     "gvbd_scene_collision_detection": [_P, _I],
     "gvbd_scene_contact_stats": [_P, _I, ctypes.POINTER(ContactStats)],
+    "gvbd_scene_add_deformer": [_P, _I, ctypes.POINTER(DeformerDesc)],
+    "gvbd_scene_add_attachments": [_P, _I, ctypes.POINTER(AttachmentDesc), _I],
+    "gvbd_scene_get_time": [_P, _I, ctypes.POINTER(ctypes.c_float)],
+    "gvbd_scene_set_time": [_P, _I, ctypes.c_float],
     "gvbd_scene_evaluate_convergence": [_P, _I] + [ctypes.POINTER(ctypes.c_double)] * 4,
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],

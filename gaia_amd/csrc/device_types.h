@@ -44,7 +44,41 @@ enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
     SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)
     SLOT_JACOBI = 4,   // tet (scenes with collision handling): write positionsNext, committed per group
+    SLOT_ATTACH = 8,   // cloth: the vertex has attachment records (SceneView::attBegin)
 };
+
+// One deformed vertex (VBD/VBDDeformer.h, VBDCloth/VBDClothDeformer.h): scene vertex v moved by
+// deformer d of its scene; a = root (Rotator) or initial position (tet Translator), b = RootP.
+struct alignas(16) DeformVert {
+    int32_t v;
+    int32_t d;
+    float a[3];
+    float b[3];
+};
+static_assert(sizeof(DeformVert) == 32, "DeformVert must be 32 bytes");
+
+// A deformer's parameters: kind 0 Rotator (axis, angular velocity omega, end time), 1 Translator
+// (speed, start and end time).
+struct alignas(16) DeformParams {
+    int32_t kind;
+    float axis[3];
+    float speed[3];
+    float startTime;
+    double omega;
+    double endTime;
+};
+static_assert(sizeof(DeformParams) == 48, "DeformParams must be 48 bytes");
+
+// One vertex-face attachment record of a cloth vertex (VBDClothPhysics.cpp:1465-1498):
+// force -= k b (x[read] - target), H += k b^2 I, target = p.x x[f0] + p.y x[f1] + p.z x[f2]
+// (f0 >= 0) or the point p.xyz (f0 < 0). ids = (f0, f1, f2, read) in scene vertex ids; p.w = k b.
+struct alignas(16) AttachRec {
+    int4 ids;
+    float4 p;
+    float b;
+    float pad[3];
+};
+static_assert(sizeof(AttachRec) == 48, "AttachRec must be 48 bytes");
 
 // One incident triangle of a cloth vertex v, relative to v:
 //   F (3x2) = (x_nb0 - x_v) m0^T + (x_nb1 - x_v) m1^T,  m_v = -(m0 + m1)
@@ -158,6 +192,14 @@ struct SceneView {
     float4* accPrevPrev;
     float4* accPrevIter;
     const float* accOmega;
+    // scene time (curTime, float as in the reference), advanced by the initial step
+    float* clock;
+    // deformers: deformed vertices (cloth meshes' first, then tet meshes'), per-deformer parameters
+    const DeformVert* dverts;
+    const DeformParams* dparams;
+    // cloth attachments: records of scene vertex v are att[attBegin[v] .. attBegin[v + 1])
+    const int* attBegin;
+    const AttachRec* att;
 };
 
 // vflag bits: 1 fixed, (code << 1) initialisation type, 16 accelerated (tet vertex of a scene

@@ -23,5 +23,8 @@ inline void hip_check(hipError_t e, const char* what) {
 inline void require(bool cond, const char* msg) {
     if (!cond) fail(GVBD_ERR_INVALID, msg);
 }
+inline void require(bool cond, const std::string& msg) {
+    if (!cond) fail(GVBD_ERR_INVALID, msg);
+}
 
 }  // namespace gvbd

@@ -39,6 +39,9 @@ hipError_t launch_counter_reset(int* counters, int n, hipStream_t stream);
 hipError_t launch_cloth_commit(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 // Items cover vertex ranges [slotBegin, slotBegin + count), count <= 256.
 hipError_t launch_initial_step(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
+// deformed vertices of one phase (items of at most 256 DeformVert each): cloth meshes' before the
+// initial step, tet meshes' after it
+hipError_t launch_deform(const SceneView* scenes, const WorkItem* items, int nitems, bool cloth, hipStream_t stream);
 hipError_t launch_update_velocity(const SceneView* scenes, const WorkItem* items, int nitems, hipStream_t stream);
 // Chebyshev acceleration over the per-vertex items: history init after the initial step, and the
 // accelerated update after iteration `iteration` (scenes without use_accelerator are skipped).

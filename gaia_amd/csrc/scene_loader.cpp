@@ -1,6 +1,7 @@
 // Scene JSON loading; see scene_loader.h.
 #include "scene_loader.h"
 
+#include <array>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -194,6 +195,80 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         if (useBowl) {
             err = "useBowlGround is not supported by this build (plane ground only)";
             return false;
+        }
+    }
+    // vertex-face attachments (VertexFaceAttachmentConstraintParams::fromJson,
+    // VBDCloth/VBDTriMeshConstraints.h:13-19; loadConstraints, VBDClothPhysics.cpp:148-224)
+    if (const json::Value* pp = params.get("PhysicsParams")) {
+        if (const json::Value* ac = pp->get("VertexFaceAttachmentConstraintParams")) {
+            auto ints = [&](const char* k) {
+                std::vector<int32_t> out;
+                if (const json::Value* v = ac->get(k))
+                    for (const json::Value& e : v->arr) out.push_back((int32_t)e.num);
+                return out;
+            };
+            const std::vector<int32_t> vi = ints("vertexConstraintVertexInfos");
+            const std::vector<int32_t> tf = ints("vertexConstraintTargetFacesInfos");
+            std::vector<std::array<float, 3>> bary;
+            if (const json::Value* v = ac->get("vertexConstraintBarys"))
+                for (const json::Value& e : v->arr)
+                    bary.push_back({(float)e.arr.at(0).num, (float)e.arr.at(1).num, (float)e.arr.at(2).num});
+            std::vector<float> k;
+            if (const json::Value* v = ac->get("vertexConstraintStiffness"))
+                for (const json::Value& e : v->arr) k.push_back((float)e.num);
+            const size_t n = vi.size() / 2;
+            if (vi.size() % 2 || tf.size() != 4 * n || bary.size() != n || (!k.empty() && k.size() != n)) {
+                err = "VertexFaceAttachmentConstraintParams: array lengths disagree";
+                return false;
+            }
+            for (size_t i = 0; i < n; ++i) {
+                gvbd_attachment_desc a{};
+                a.mesh = vi[2 * i];
+                a.vertex = vi[2 * i + 1];
+                a.target_mesh = tf[4 * i];
+                for (int j = 0; j < 3; ++j) {
+                    a.face[j] = tf[4 * i + 1 + j];
+                    a.bary[j] = bary[i][j];
+                }
+                a.stiffness = k.empty() ? 1e4f : k[i];  // VBDTriMeshConstraints.h:32
+                out.attachments.push_back(a);
+            }
+        }
+    }
+    // deformers (loadDeformers, VBD/VBDDeformer.h:181-222, VBDCloth/VBDClothDeformer.h:199-243)
+    if (const json::Value* dl = params.get("Deformers")) {
+        for (const json::Value& dj : dl->arr) {
+            LoadedDeformer d;
+            const std::string name = get_str(dj, "DeformerName");
+            if (name == "Rotator") {
+                d.desc.kind = GVBD_DEFORMER_ROTATOR;
+                get_vec3(dj, "center", d.desc.center);
+                get_vec3(dj, "axis", d.desc.axis);
+                if (const json::Value* v = dj.get("angularVelocity")) d.desc.angular_velocity = v->num;
+                d.desc.end_time = -1.0;
+                if (const json::Value* v = dj.get("rotationEndTime")) d.desc.end_time = v->num;
+            } else if (name == "Translator") {
+                d.desc.kind = GVBD_DEFORMER_TRANSLATOR;
+                get_vec3(dj, "speed", d.desc.speed);
+                d.desc.end_time = -1.0;
+                if (const json::Value* v = dj.get("deformationEndTime")) d.desc.end_time = v->num;
+                get_num(dj, "deformationStartTime", d.desc.start_time);
+            } else {
+                err = "unrecognised deformer \"" + name + "\" (Rotator, Translator)";
+                return false;
+            }
+            if (const json::Value* v = dj.get("selectedMeshes"))
+                for (const json::Value& e : v->arr) d.meshes.push_back((int32_t)e.num);
+            if (const json::Value* v = dj.get("selectedVertices"))
+                for (const json::Value& l : v->arr) {
+                    d.vertices.emplace_back();
+                    for (const json::Value& e : l.arr) d.vertices.back().push_back((int32_t)e.num);
+                }
+            if (d.vertices.size() != d.meshes.size()) {
+                err = "deformer \"" + name + "\": selectedVertices needs one list per selected mesh";
+                return false;
+            }
+            out.deformers.push_back(std::move(d));
         }
     }
     const json::Value* list = models.get("Models");

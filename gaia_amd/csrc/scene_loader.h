@@ -10,7 +10,16 @@
 
 namespace gvbd {
 
+// One entry of the parameter file's "Deformers" array (loadDeformers, VBD/VBDDeformer.h:181-222).
+struct LoadedDeformer {
+    gvbd_deformer_desc desc{};
+    std::vector<int32_t> meshes;
+    std::vector<std::vector<int32_t>> vertices;
+};
+
 struct LoadedScene {
+    std::vector<LoadedDeformer> deformers;
+    std::vector<gvbd_attachment_desc> attachments;  // PhysicsParams.VertexFaceAttachmentConstraintParams
     gvbd_physics_params params{};
     std::vector<TetMeshHost> tets;
     std::vector<ClothMeshHost> cloths;

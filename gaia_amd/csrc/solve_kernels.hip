@@ -681,6 +681,21 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
         h.xx += ch.xx; h.xy += ch.xy; h.xz += ch.xz;
         h.yy += ch.yy; h.yz += ch.yz; h.zz += ch.zz;
     }
+    if (sa.w & SLOT_ATTACH) {
+        // vertex attachments (VBDClothPhysics.cpp:1465-1498)
+        for (int r = S.attBegin[v]; r < S.attBegin[v + 1]; ++r) {
+            const AttachRec A = S.att[r];
+            const f3 xr = xyz(S.pos[A.ids.w]);
+            f3 tgt = mk(A.p.x, A.p.y, A.p.z);
+            if (A.ids.x >= 0)
+                tgt = A.p.x * xyz(S.pos[A.ids.x]) + A.p.y * xyz(S.pos[A.ids.y]) + A.p.z * xyz(S.pos[A.ids.z]);
+            f = f - A.p.w * (xr - tgt);
+            const float hk = A.p.w * A.b;
+            h.xx += hk;
+            h.yy += hk;
+            h.zz += hk;
+        }
+    }
     f3 dx;
     f3 xn = x;
     if (solve_sym3(h, f, dx)) {
@@ -844,6 +859,8 @@ __global__ __launch_bounds__(256) void initial_step_kernel(const SceneView* __re
     const uint8_t flag = S.vflag[v];
     const float dt = sp.dt;
     S.prev[v] = p;
+    // curTime += dt at the start of the substep (one thread per scene)
+    if (v == 0) S.clock[0] = S.clock[0] + dt;
     f3 vn = xyz(vel);
     if (sp.associateGravity) {
         const f3 dg = mk(dt * sp.gravity[0], dt * sp.gravity[1], dt * sp.gravity[2]);
@@ -879,6 +896,55 @@ __global__ __launch_bounds__(256) void initial_step_kernel(const SceneView* __re
     }
     S.pos[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
     S.vel[v] = make_float4(vn.x, vn.y, vn.z, 0.f);
+}
+
+// Deformers (VBD/VBDDeformer.h:45-178, VBDCloth/VBDClothDeformer.h:44-196). Items cover one
+// phase's deformed vertices: cloth = 1 for cloth meshes' (launched before the substep's initial
+// step, so the substep's time is clock + dt), 0 for tet meshes' (after it: clock). Rotator:
+// p = root + R(theta) RootP, theta = min(t, end) omega (end > 0), R's entries in double rounded
+// to float; tet Translator: p = speed min(t, end) + p0; cloth Translator: p += speed dt while
+// start <= t and (t < end or end < 0). Products and sums unfused, as the reference's float code.
+__global__ __launch_bounds__(256) void deform_kernel(const SceneView* __restrict__ scenes,
+                                                     const WorkItem* __restrict__ items, int cloth) {
+    const WorkItem it = items[blockIdx.x];
+    const int i = threadIdx.x;
+    if (i >= it.count) return;
+    const SceneView& S = scenes[it.scene];
+    const DeformVert r = S.dverts[it.slotBegin + i];
+    const DeformParams P = S.dparams[r.d];
+    const float dt = S.sp.dt;
+    const float t = cloth ? __fadd_rn(S.clock[0], dt) : S.clock[0];
+    float p[3];
+    if (P.kind == 0) {
+        double tc = t;
+        if (tc >= P.endTime && P.endTime > 0) tc = cloth ? (double)(float)P.endTime : P.endTime;
+        const double theta = tc * P.omega;
+        const double c = cos(theta), s = sin(theta);
+        const double ux = P.axis[0], uy = P.axis[1], uz = P.axis[2];
+        const float R[9] = {(float)(c + ux * ux * (1 - c)), (float)(ux * uy * (1 - c) - uz * s),
+                            (float)(ux * uz * (1 - c) + uy * s), (float)(uy * ux * (1 - c) + uz * s),
+                            (float)(c + uy * uy * (1 - c)),      (float)(uy * uz * (1 - c) - ux * s),
+                            (float)(uz * ux * (1 - c) - uy * s), (float)(uz * uy * (1 - c) + ux * s),
+                            (float)(c + uz * uz * (1 - c))};
+        for (int k = 0; k < 3; ++k) {
+            const float q = __fadd_rn(__fadd_rn(__fmul_rn(R[3 * k], r.b[0]), __fmul_rn(R[3 * k + 1], r.b[1])),
+                                      __fmul_rn(R[3 * k + 2], r.b[2]));
+            p[k] = __fadd_rn(r.a[k], q);
+        }
+    } else if (!cloth) {
+        float tc = t;
+        const float end = (float)P.endTime;
+        if (tc >= end && end > 0.f) tc = end;
+        for (int k = 0; k < 3; ++k) p[k] = __fadd_rn(__fmul_rn(P.speed[k], tc), r.a[k]);
+    } else {
+        const float end = (float)P.endTime;
+        if (!(t >= P.startTime && (t < end || end < 0.f))) return;
+        const float4 x = S.pos[r.v];
+        p[0] = __fadd_rn(x.x, __fmul_rn(P.speed[0], dt));
+        p[1] = __fadd_rn(x.y, __fmul_rn(P.speed[1], dt));
+        p[2] = __fadd_rn(x.z, __fmul_rn(P.speed[2], dt));
+    }
+    S.pos[r.v] = make_float4(p[0], p[1], p[2], 0.f);
 }
 
 // Chebyshev acceleration (runStepGPU, VBDPhysics.cpp:872-906): after the initial step both
@@ -1055,6 +1121,12 @@ hipError_t launch_accel_apply(const SceneView* scenes, const WorkItem* items, in
                               hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(accel_apply_kernel, dim3(nitems), dim3(256), 0, stream, scenes, items, iteration);
+    return hipGetLastError();
+}
+
+hipError_t launch_deform(const SceneView* scenes, const WorkItem* items, int nitems, bool cloth, hipStream_t stream) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(deform_kernel, dim3(nitems), dim3(256), 0, stream, scenes, items, cloth ? 1 : 0);
     return hipGetLastError();
 }
 

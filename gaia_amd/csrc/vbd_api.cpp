@@ -50,6 +50,20 @@ struct DeviceScene {
     int4* tcontact = nullptr;   // contact scenes: per tet slot, its tFaceRef and tEdgeRef ranges
     float4 *accPrevPrev = nullptr, *accPrevIter = nullptr;  // use_accelerator: iteration history
     float* accOmega = nullptr;                              // use_accelerator: omega per iteration
+    float* clock = nullptr;                                 // scene time
+    DeformVert* dverts = nullptr;                           // deformed vertices (deformers)
+    DeformParams* dparams = nullptr;
+    int* attBegin = nullptr;                                // cloth attachments: per vertex CSR
+    AttachRec* att = nullptr;
+};
+
+// A deformer as added (gvbd_scene_add_deformer): parameters and, per selected mesh, its vertices
+// with the constants taken at construction (root / initial position, RootP).
+struct DeformerHost {
+    DeformParams prm{};
+    std::vector<int32_t> meshes;
+    std::vector<std::vector<int32_t>> verts;
+    std::vector<std::vector<float>> a, b;  // 3 per vertex
 };
 
 // Boundary surface of a tet mesh in a scene with collision handling: its contact faces, their
@@ -96,6 +110,10 @@ struct SceneObj {
     int32_t server = -1;
     std::vector<char> hostCopy;
     uintptr_t hostCopyBase = 0;
+    std::vector<DeformerHost> deformers;            // gvbd_scene_add_deformer
+    std::vector<gvbd_attachment_desc> attachments;  // gvbd_scene_add_attachments
+    float time0 = 0.f;                              // scene time at initialisation
+    int32_t nDeformPre = 0, nDeformPost = 0;        // deformed vertices of cloth / tet meshes
 
     int32_t meshVertices(int32_t m) const {
         return meshes[m].first == MESH_TET ? tets[meshes[m].second].nv : cloths[meshes[m].second].nv;
@@ -122,6 +140,9 @@ struct PlanObj {
     std::vector<int32_t> groupItemBegin;    // G + 1
     std::vector<int32_t> groupCommitBegin;  // G + 1
     int32_t nVItems = 0;
+    WorkItem* dDeformPre = nullptr;   // deformed vertices of cloth meshes (before the initial step)
+    WorkItem* dDeformPost = nullptr;  // deformed vertices of tet meshes (after it)
+    int32_t nDeformPre = 0, nDeformPost = 0;
     // work queue (mode GVBD_MODE_WORK_QUEUE)
     QueueEntry* dQueue = nullptr;
     int* dCounters = nullptr;  // tickets[iterations] then progress[scenes]
@@ -253,8 +274,10 @@ void free_plan_graph(PlanObj& p) {
 void free_plan(PlanObj& p) {
     free_plan_graph(p);
     for (void* ptr : {(void*)p.dViews, (void*)p.dItems, (void*)p.dCommit, (void*)p.dVItems, (void*)p.dQueue,
-                      (void*)p.dCounters})
+                      (void*)p.dCounters, (void*)p.dDeformPre, (void*)p.dDeformPost})
         if (ptr) (void)hipFree(ptr);
+    p.dDeformPre = p.dDeformPost = nullptr;
+    p.nDeformPre = p.nDeformPost = 0;
     p.dViews = nullptr;
     p.dItems = nullptr;
     p.dCommit = nullptr;
@@ -468,6 +491,44 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     }
     s.tetJacobi = contact && !s.tets.empty();
     s.clothSlotInplace.clear();
+    // vertex-face attachments (loadConstraints, VBDClothPhysics.cpp:148-224): the vertex side gets
+    // b = 1, each face vertex with a non-zero barycentric b = -bary (its position read from the
+    // vertex-side mesh at the face vertex's own index, VBDClothPhysics.cpp:1471); records of one
+    // vertex keep the constraint order
+    std::vector<std::vector<AttachRec>> attOf;
+    if (!s.attachments.empty()) {
+        attOf.assign(s.nv, {});
+        for (const gvbd_attachment_desc& A : s.attachments) {
+            const int32_t offV = s.vertexOffset[A.mesh];
+            AttachRec r{};
+            r.ids = make_int4(-1, -1, -1, offV + A.vertex);
+            if (A.target_mesh >= 0) {
+                const int32_t offT = s.vertexOffset[A.target_mesh];
+                r.ids = make_int4(offT + A.face[0], offT + A.face[1], offT + A.face[2], offV + A.vertex);
+            }
+            r.p = make_float4(A.bary[0], A.bary[1], A.bary[2], A.stiffness * 1.f);
+            r.b = 1.f;
+            attOf[offV + A.vertex].push_back(r);
+            if (A.target_mesh < 0) continue;
+            for (int j = 0; j < 3; ++j) {
+                if (A.bary[j] == 0.f) continue;
+                AttachRec q = r;
+                q.ids.w = offV + A.face[j];
+                q.b = -A.bary[j];
+                q.p.w = A.stiffness * q.b;
+                attOf[s.vertexOffset[A.target_mesh] + A.face[j]].push_back(q);
+            }
+        }
+    }
+    std::vector<int32_t> attBegin;
+    std::vector<AttachRec> att;
+    if (!attOf.empty()) {
+        attBegin.push_back(0);
+        for (const auto& l : attOf) {
+            att.insert(att.end(), l.begin(), l.end());
+            attBegin.push_back((int32_t)att.size());
+        }
+    }
     for (int32_t g = 0; g < G; ++g) {
         const auto& grp = s.groups[g];
         // A cloth mesh with a single colour in this group can be updated in place: its solved
@@ -490,9 +551,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
                 emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork,
                               contact ? &trefs[im] : nullptr);
             } else {
-                const bool inplace = !multiColor[im] && !s.params.handle_collision;
+                // attachments read other vertices of the group: positionsNext keeps them Jacobi
+                const bool inplace = !multiColor[im] && !s.params.handle_collision && attOf.empty();
                 emit_cloth_slot(s.cloths[s.meshes[im].second], off, lv, inplace, cslots, cface, cbend, clothWork,
                                 contact ? &refs[im] : nullptr);
+                if (!attOf.empty() && !attOf[off + lv].empty()) cslots.back().a.w |= SLOT_ATTACH;
                 s.clothSlotInplace.push_back(inplace ? 1 : 0);
             }
         }
@@ -565,6 +628,31 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             omega.push_back(w);
         }
     }
+    // deformed vertices: cloth meshes' first, then tet meshes' (the two launch phases)
+    std::vector<DeformVert> dverts;
+    std::vector<DeformParams> dparams;
+    for (int phase = 0; phase < 2; ++phase)
+        for (size_t di = 0; di < s.deformers.size(); ++di) {
+            const DeformerHost& D = s.deformers[di];
+            for (size_t k = 0; k < D.meshes.size(); ++k) {
+                const bool cloth = s.meshes[D.meshes[k]].first == MESH_CLOTH;
+                if (cloth != (phase == 0)) continue;
+                for (size_t i = 0; i < D.verts[k].size(); ++i) {
+                    DeformVert r{};
+                    r.v = s.vertexOffset[D.meshes[k]] + D.verts[k][i];
+                    r.d = (int32_t)di;
+                    for (int c = 0; c < 3; ++c) {
+                        r.a[c] = D.a[k][3 * i + c];
+                        r.b[c] = D.b[k][3 * i + c];
+                    }
+                    dverts.push_back(r);
+                }
+            }
+            if (phase == 0) s.nDeformPre = (int32_t)dverts.size();
+        }
+    if (s.deformers.empty()) s.nDeformPre = 0;
+    s.nDeformPost = (int32_t)dverts.size() - s.nDeformPre;
+    for (const DeformerHost& D : s.deformers) dparams.push_back(D.prm);
     // one arena
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t bV = align(sizeof(float4) * (size_t)std::max(nv, 1));
@@ -581,7 +669,13 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bTE = contact ? align(sizeof(int32_t) * std::max<size_t>(tEdgeRef.size(), 1)) : 0;
     const size_t bTC = contact ? align(sizeof(int4) * std::max<size_t>(tcontact.size(), 1)) : 0;
     const size_t bA = accel ? bV : 0, bW = accel ? align(sizeof(float) * omega.size()) : 0;
-    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW;
+    const size_t bClk = 256;
+    const size_t bDV = dverts.empty() ? 0 : align(sizeof(DeformVert) * dverts.size());
+    const size_t bDP = dparams.empty() ? 0 : align(sizeof(DeformParams) * dparams.size());
+    const size_t bAB = attBegin.empty() ? 0 : align(sizeof(int32_t) * attBegin.size());
+    const size_t bAR = att.empty() ? 0 : align(sizeof(AttachRec) * att.size());
+    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW +
+                         bClk + bDV + bDP + bAB + bAR;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -623,6 +717,15 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         d.accPrevIter = (float4*)take(bA);
         d.accOmega = (float*)take(bW);
     }
+    d.clock = (float*)take(bClk);
+    if (bDV) {
+        d.dverts = (DeformVert*)take(bDV);
+        d.dparams = (DeformParams*)take(bDP);
+    }
+    if (bAB) {
+        d.attBegin = (int*)take(bAB);
+        d.att = (AttachRec*)take(bAR);
+    }
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->stream), "upload");
     };
@@ -651,6 +754,15 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         up(d.accPrevIter, pos.data(), sizeof(float4) * nv);
         up(d.accOmega, omega.data(), sizeof(float) * omega.size());
     }
+    up(d.clock, &s.time0, sizeof(float));
+    if (bDV) {
+        up(d.dverts, dverts.data(), sizeof(DeformVert) * dverts.size());
+        up(d.dparams, dparams.data(), sizeof(DeformParams) * dparams.size());
+    }
+    if (bAB) {
+        up(d.attBegin, attBegin.data(), sizeof(int32_t) * attBegin.size());
+        up(d.att, att.data(), sizeof(AttachRec) * att.size());
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
     SceneView& v = s.view;
     v.pos = d.pos;
@@ -669,6 +781,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     v.accPrevPrev = d.accPrevPrev;
     v.accPrevIter = d.accPrevIter;
     v.accOmega = d.accOmega;
+    v.clock = d.clock;
+    v.dverts = d.dverts;
+    v.dparams = d.dparams;
+    v.attBegin = d.attBegin;
+    v.att = d.att;
     v.nv = nv;
     v.sp = s.sp;
 }
@@ -767,6 +884,16 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
             vitems.push_back({(int32_t)si, k, std::min(kVertexChunk, s.nv - k), 0});
     }
     p.nVItems = (int32_t)vitems.size();
+    std::vector<WorkItem> dpre, dpost;
+    for (size_t si = 0; si < p.scenes.size(); ++si) {
+        SceneObj& s = scene_of(ctx, p.scenes[si]);
+        for (int32_t k = 0; k < s.nDeformPre; k += 256)
+            dpre.push_back({(int32_t)si, k, std::min(256, s.nDeformPre - k), 0});
+        const int32_t e = s.nDeformPre + s.nDeformPost;
+        for (int32_t k = s.nDeformPre; k < e; k += 256) dpost.push_back({(int32_t)si, k, std::min(256, e - k), 0});
+    }
+    p.nDeformPre = (int32_t)dpre.size();
+    p.nDeformPost = (int32_t)dpost.size();
     auto alloc_up = [&](void** dst, const void* src, size_t n) {
         if (hipMalloc(dst, std::max<size_t>(n, 16)) != hipSuccess) {
             (void)hipGetLastError();
@@ -780,6 +907,8 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     alloc_up((void**)&p.dItems, items.data(), sizeof(WorkItem) * items.size());
     alloc_up((void**)&p.dCommit, commits.data(), sizeof(WorkItem) * commits.size());
     alloc_up((void**)&p.dVItems, vitems.data(), sizeof(WorkItem) * vitems.size());
+    if (p.nDeformPre) alloc_up((void**)&p.dDeformPre, dpre.data(), sizeof(WorkItem) * dpre.size());
+    if (p.nDeformPost) alloc_up((void**)&p.dDeformPost, dpost.data(), sizeof(WorkItem) * dpost.size());
     if (p.mode == GVBD_MODE_WORK_QUEUE) {
         alloc_up((void**)&p.dQueue, queue.data(), sizeof(QueueEntry) * queue.size());
         std::vector<int> zeros(p.nCounters, 0);
@@ -791,9 +920,21 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
 
 void plan_reset_counters(gvbd_context* ctx, PlanObj& p);
 
-// Initial step of a substep; in work-queue mode it also starts a new counter epoch.
-void plan_initial_step(gvbd_context* ctx, PlanObj& p) {
+// Deformers of cloth meshes (VBDClothPhysics.cpp:408-412: at iteration 0, before detection and
+// the initial guess).
+void plan_deform_cloth(gvbd_context* ctx, PlanObj& p) {
+    if (p.nDeformPre)
+        hip_check(launch_deform(p.dViews, p.dDeformPre, p.nDeformPre, true, ctx->stream), "deformer launch");
+}
+
+// Initial step of a substep, preceded by the cloth meshes' deformers unless the caller already
+// ran them and followed by the tet meshes' (VBDPhysics.cpp:849-862: after the initial step); in
+// work-queue mode it also starts a new counter epoch.
+void plan_initial_step(gvbd_context* ctx, PlanObj& p, bool clothDeformed = false) {
+    if (!clothDeformed) plan_deform_cloth(ctx, p);
     hip_check(launch_initial_step(p.dViews, p.dVItems, p.nVItems, ctx->stream), "initial step launch");
+    if (p.nDeformPost)
+        hip_check(launch_deform(p.dViews, p.dDeformPost, p.nDeformPost, false, ctx->stream), "deformer launch");
     if (p.accel) hip_check(launch_accel_init(p.dViews, p.dVItems, p.nVItems, ctx->stream), "accelerator launch");
     if (p.mode == GVBD_MODE_WORK_QUEUE) plan_reset_counters(ctx, p);
 }
@@ -849,6 +990,7 @@ void plan_contact_frame(gvbd_context* ctx, PlanObj& p) {
         bool required = true;
         for (int it = 0; it < p.iterations; ++it) {
             const bool detect = it == 0 || required;
+            if (it == 0) plan_deform_cloth(ctx, p);
             if (detect) {
                 if (!s.cdEvents[0]) {
                     hip_check(hipEventCreate(&s.cdEvents[0]), "event create");
@@ -859,7 +1001,7 @@ void plan_contact_frame(gvbd_context* ctx, PlanObj& p) {
                 hip_check(hipEventRecord(s.cdEvents[1], ctx->stream), "event record");
                 ++s.contactStats.detections;
             }
-            if (it == 0) plan_initial_step(ctx, p);
+            if (it == 0) plan_initial_step(ctx, p, true);
             plan_iteration(ctx, p, it, it);
             int flag = 0;
             hip_check(hipMemcpyAsync(&flag, dv.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "flag read");
@@ -1226,6 +1368,9 @@ int gvbd_scene_clone(gvbd_context* ctx, int32_t scene, int32_t* scene_out) {
         s->meshes = src.meshes;
         s->vertexOffset = src.vertexOffset;
         s->nv = src.nv;
+        s->deformers = src.deformers;
+        s->attachments = src.attachments;
+        s->time0 = src.time0;
         const int32_t id = ctx->nextScene++;
         ctx->scenes[id] = std::move(s);
         *scene_out = id;
@@ -1350,6 +1495,144 @@ int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth
     });
 }
 
+namespace gvbd {
+namespace {
+
+std::vector<int32_t>& mesh_fixed(SceneObj& s, int32_t m) {
+    return s.meshes[m].first == MESH_TET ? s.tets[s.meshes[m].second].fixed : s.cloths[s.meshes[m].second].fixed;
+}
+const std::vector<float>& mesh_positions(const SceneObj& s, int32_t m) {
+    return s.meshes[m].first == MESH_TET ? s.tets[s.meshes[m].second].pos : s.cloths[s.meshes[m].second].pos;
+}
+
+// The scene time: the device copy once initialised (resident, or in the served scene's host copy).
+float* scene_clock_host(SceneObj& s) {
+    if (!s.hostCopy.empty() && s.dev.clock)
+        return reinterpret_cast<float*>(s.hostCopy.data() + (reinterpret_cast<uintptr_t>(s.dev.clock) - s.hostCopyBase));
+    return nullptr;
+}
+
+void set_scene_time(gvbd_context* ctx, SceneObj& s, float t) {
+    if (!s.initialized) {
+        s.time0 = t;
+        return;
+    }
+    if (float* h = scene_clock_host(s)) {
+        *h = t;
+        return;
+    }
+    hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    hip_check(hipMemcpy(s.dev.clock, &t, sizeof(float), hipMemcpyHostToDevice), "time upload");
+}
+
+}  // namespace
+}  // namespace gvbd
+
+int gvbd_scene_add_deformer(gvbd_context* ctx, int32_t scene, const gvbd_deformer_desc* desc) {
+    return guard(ctx, [&] {
+        require(ctx && desc, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (s.initialized) fail(GVBD_ERR_STATE, "deformers are added before the scene is initialised");
+        require(desc->kind == GVBD_DEFORMER_ROTATOR || desc->kind == GVBD_DEFORMER_TRANSLATOR,
+                "unrecognised deformer kind");
+        require(desc->num_meshes >= 0 && (desc->num_meshes == 0 || (desc->meshes && desc->num_vertices)),
+                "deformer meshes are NULL");
+        DeformerHost D;
+        D.prm.kind = desc->kind;
+        for (int c = 0; c < 3; ++c) {
+            D.prm.axis[c] = desc->axis[c];
+            D.prm.speed[c] = desc->speed[c];
+        }
+        D.prm.startTime = desc->start_time;
+        D.prm.omega = desc->angular_velocity;
+        D.prm.endTime = desc->end_time;
+        size_t at = 0;
+        for (int32_t k = 0; k < desc->num_meshes; ++k) {
+            const int32_t m = desc->meshes[k];
+            require(m >= 0 && m < (int32_t)s.meshes.size(), "deformer mesh index out of range");
+            require(desc->num_vertices[k] >= 0, "negative vertex count");
+            require(desc->num_vertices[k] == 0 || desc->vertices, "deformer vertices are NULL");
+            const std::vector<float>& P = mesh_positions(s, m);
+            D.meshes.push_back(m);
+            D.verts.emplace_back();
+            D.a.emplace_back();
+            D.b.emplace_back();
+            for (int32_t i = 0; i < desc->num_vertices[k]; ++i) {
+                const int32_t v = desc->vertices[at++];
+                require(v >= 0 && v < s.meshVertices(m), "deformer vertex index out of range");
+                D.verts.back().push_back(v);
+                // Rotator: root = ((P - o) . axis) axis, RootP = P - root; Translator: P
+                // (VBDDeformer.h:52-67, 139-148), in float
+                const float p[3] = {P[3 * (size_t)v], P[3 * (size_t)v + 1], P[3 * (size_t)v + 2]};
+                float op[3], root[3];
+                for (int c = 0; c < 3; ++c) op[c] = p[c] - desc->center[c];
+                const float dot = op[0] * desc->axis[0] + op[1] * desc->axis[1] + op[2] * desc->axis[2];
+                for (int c = 0; c < 3; ++c) root[c] = dot * desc->axis[c];
+                for (int c = 0; c < 3; ++c) {
+                    D.a.back().push_back(desc->kind == GVBD_DEFORMER_ROTATOR ? root[c] : p[c]);
+                    D.b.back().push_back(desc->kind == GVBD_DEFORMER_ROTATOR ? p[c] - root[c] : 0.f);
+                }
+            }
+        }
+        // setFixed / setFixedPointCloth (VBDDeformer.h:20-43, VBDClothDeformer.h:5-26)
+        for (size_t k = 0; k < D.meshes.size(); ++k) {
+            std::vector<int32_t>& fx = mesh_fixed(s, D.meshes[k]);
+            for (int32_t v : D.verts[k])
+                if (std::find(fx.begin(), fx.end(), v) == fx.end()) fx.push_back(v);
+        }
+        s.deformers.push_back(std::move(D));
+    });
+}
+
+int gvbd_scene_add_attachments(gvbd_context* ctx, int32_t scene, const gvbd_attachment_desc* list, int32_t n) {
+    return guard(ctx, [&] {
+        require(ctx && (n == 0 || list) && n >= 0, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (s.initialized) fail(GVBD_ERR_STATE, "attachments are added before the scene is initialised");
+        auto cloth = [&](int32_t m) { return m >= 0 && m < (int32_t)s.meshes.size() && s.meshes[m].first == MESH_CLOTH; };
+        for (int32_t i = 0; i < n; ++i) {
+            const gvbd_attachment_desc& A = list[i];
+            require(cloth(A.mesh), "attachment " + std::to_string(i) + ": mesh must be a cloth mesh of the scene");
+            require(A.vertex >= 0 && A.vertex < s.meshVertices(A.mesh),
+                    "attachment " + std::to_string(i) + ": vertex index out of range");
+            if (A.target_mesh == -1) continue;
+            require(cloth(A.target_mesh),
+                    "attachment " + std::to_string(i) + ": target mesh must be -1 or a cloth mesh of the scene");
+            for (int j = 0; j < 3; ++j) {
+                require(A.face[j] >= 0 && A.face[j] < s.meshVertices(A.target_mesh),
+                        "attachment " + std::to_string(i) + ": face vertex index out of range");
+                // the face side reads the vertex-side mesh at the face vertex's index
+                require(A.bary[j] == 0.f || A.face[j] < s.meshVertices(A.mesh),
+                        "attachment " + std::to_string(i) +
+                            ": face vertex index exceeds the vertex-side mesh (its face-side force reads it)");
+            }
+        }
+        s.attachments.insert(s.attachments.end(), list, list + n);
+    });
+}
+
+int gvbd_scene_get_time(gvbd_context* ctx, int32_t scene, float* cur_time) {
+    return guard(ctx, [&] {
+        require(ctx && cur_time, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.initialized) {
+            *cur_time = s.time0;
+        } else if (float* h = scene_clock_host(s)) {
+            *cur_time = *h;
+        } else {
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+            hip_check(hipMemcpy(cur_time, s.dev.clock, sizeof(float), hipMemcpyDeviceToHost), "time read");
+        }
+    });
+}
+
+int gvbd_scene_set_time(gvbd_context* ctx, int32_t scene, float cur_time) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        set_scene_time(ctx, scene_of(ctx, scene), cur_time);
+    });
+}
+
 int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
@@ -1375,6 +1658,25 @@ int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char*
                     add_tet_mesh_host(ctx, sid, std::move(ls.tets[mk.second]));
                 else
                     add_cloth_mesh_host(ctx, sid, std::move(ls.cloths[mk.second]));
+            }
+            for (const LoadedDeformer& d : ls.deformers) {
+                gvbd_deformer_desc dd = d.desc;
+                std::vector<int32_t> counts;
+                std::vector<int32_t> verts;
+                for (const auto& l : d.vertices) {
+                    counts.push_back((int32_t)l.size());
+                    verts.insert(verts.end(), l.begin(), l.end());
+                }
+                dd.num_meshes = (int32_t)d.meshes.size();
+                dd.meshes = d.meshes.data();
+                dd.num_vertices = counts.data();
+                dd.vertices = verts.data();
+                rc = gvbd_scene_add_deformer(ctx, sid, &dd);
+                if (rc != GVBD_OK) fail(rc, ctx->lastError);
+            }
+            if (!ls.attachments.empty()) {
+                rc = gvbd_scene_add_attachments(ctx, sid, ls.attachments.data(), (int32_t)ls.attachments.size());
+                if (rc != GVBD_OK) fail(rc, ctx->lastError);
             }
             SceneObj& s = scene_of(ctx, sid);
             initialize_scene(ctx, s);
@@ -1805,6 +2107,7 @@ int gvbd_scene_load_state(gvbd_context* ctx, int32_t scene, const char* json_pat
         for (size_t m = 0; m < nm; ++m)
             mesh_state_io(ctx, scene, (int32_t)m, st[m].position.data(),
                           st[m].velocities.empty() ? nullptr : st[m].velocities.data(), false);
+        set_scene_time(ctx, s, (float)t);  // PhysicsState::toPhysics sets curTime (BasePhysicsFramework.h:178)
         if (frame_id) *frame_id = fid;
         if (cur_time) *cur_time = t;
     });
@@ -2091,10 +2394,11 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
     mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
     mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact), mv(d.accPrevPrev), mv(d.accPrevIter), mv(d.accOmega);
+    mv(d.clock), mv(d.dverts), mv(d.dparams), mv(d.attBegin), mv(d.att);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
     mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact), mv(v.accPrevPrev), mv(v.accPrevIter);
-    mv(v.accOmega);
+    mv(v.accOmega), mv(v.clock), mv(v.dverts), mv(v.dparams), mv(v.attBegin), mv(v.att);
 }
 
 // Copies an idle scene's arena out to host memory and frees it.

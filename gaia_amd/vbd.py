@@ -269,6 +269,8 @@ class VBDPhysics:
         self.ctx = ctx if ctx is not None else Context(device)
         self.scene = None
         self._pending = []
+        self._deformers = []
+        self._attachments = []
         self._params = None
         self._json = None
 
@@ -291,6 +293,73 @@ class VBDPhysics:
         self._pending.append(("cloth", positions, faces, fixed_points, vertex_colors, coloring, velocities,
                               initial_velocity, material))
 
+    def addDeformer(self, deformer):
+        """A deformer given as the reference's JSON object (loadDeformers, VBD/VBDDeformer.h:181-222):
+        DeformerName "Rotator" (center, axis, angularVelocity, rotationEndTime) or "Translator"
+        (speed, deformationEndTime, deformationStartTime), selectedMeshes, selectedVertices. Its
+        vertices become fixed points moved along the prescribed path."""
+        self._deformers.append(dict(deformer))
+
+    def addAttachments(self, params):
+        """Vertex-face attachments given as the reference's VertexFaceAttachmentConstraintParams
+        (vertexConstraintVertexInfos, vertexConstraintTargetFacesInfos, vertexConstraintBarys,
+        vertexConstraintStiffness; VBDCloth/VBDClothPhysics.cpp:148-224)."""
+        self._attachments.append(dict(params))
+
+    @property
+    def curTime(self):
+        """The scene time (float, + dt per substep)."""
+        t = ctypes.c_float()
+        self.ctx.check(lib().gvbd_scene_get_time(self.ctx.handle, self.scene, ctypes.byref(t)))
+        return t.value
+
+    @curTime.setter
+    def curTime(self, t):
+        self.ctx.check(lib().gvbd_scene_set_time(self.ctx.handle, self.scene, float(t)))
+
+    def _add_deformer(self, d):
+        name = d.get("DeformerName")
+        if name not in ("Rotator", "Translator"):
+            raise ValueError(f"unrecognised deformer {name!r} (Rotator, Translator)")
+        desc = _lib.DeformerDesc()
+        rot = name == "Rotator"
+        desc.kind = 0 if rot else 1
+        desc.center[:] = [float(c) for c in d.get("center", (0, 0, 0))]
+        desc.axis[:] = [float(c) for c in d.get("axis", (0, 0, 0))]
+        desc.speed[:] = [float(c) for c in d.get("speed", (0, 0, 0))]
+        desc.start_time = float(d.get("deformationStartTime", 0.0))
+        desc.angular_velocity = float(d.get("angularVelocity", 0.0))
+        desc.end_time = float(d.get("rotationEndTime" if rot else "deformationEndTime", -1.0))
+        meshes = i32(np.asarray(d["selectedMeshes"], dtype=np.int32).reshape(-1))
+        sel = [np.asarray(v, dtype=np.int32).reshape(-1) for v in d["selectedVertices"]]
+        if len(sel) != len(meshes):
+            raise ValueError("selectedVertices needs one list per selected mesh")
+        counts = i32(np.array([len(v) for v in sel], dtype=np.int32))
+        verts = i32(np.concatenate(sel) if sel else np.zeros(0, np.int32))
+        desc.num_meshes = len(meshes)
+        desc.meshes = iptr(meshes)
+        desc.num_vertices = iptr(counts)
+        desc.vertices = iptr(verts)
+        self.ctx.check(lib().gvbd_scene_add_deformer(self.ctx.handle, self.scene, ctypes.byref(desc)))
+
+    def _add_attachments(self, params):
+        vi = np.asarray(params["vertexConstraintVertexInfos"], dtype=np.int32).reshape(-1, 2)
+        tf = np.asarray(params["vertexConstraintTargetFacesInfos"], dtype=np.int32).reshape(-1, 4)
+        bary = np.asarray(params["vertexConstraintBarys"], dtype=np.float64).reshape(-1, 3)
+        k = np.asarray(params.get("vertexConstraintStiffness", []), dtype=np.float64).reshape(-1)
+        n = len(vi)
+        if len(tf) != n or len(bary) != n or (len(k) and len(k) != n):
+            raise ValueError("attachment arrays disagree in length")
+        arr = (_lib.AttachmentDesc * max(n, 1))()
+        for i in range(n):
+            a = arr[i]
+            a.mesh, a.vertex = int(vi[i, 0]), int(vi[i, 1])
+            a.target_mesh = int(tf[i, 0])
+            a.face[:] = [int(c) for c in tf[i, 1:]]
+            a.bary[:] = [float(c) for c in bary[i]]
+            a.stiffness = float(k[i]) if len(k) else 1e4
+        self.ctx.check(lib().gvbd_scene_add_attachments(self.ctx.handle, self.scene, arr, n))
+
     def initialize(self, defer_upload=False):
         """Creates the scene and uploads it. With defer_upload the scene is only created with
         its meshes (host side): a Server uploads it when its first step is requested."""
@@ -311,6 +380,10 @@ class VBDPhysics:
                     self._add_tet(pos, elems, fixed, colors, coloring, vel, v0, material)
                 else:
                     self._add_cloth(pos, elems, fixed, colors, coloring, vel, v0, material)
+            for d in self._deformers:
+                self._add_deformer(d)
+            for a in self._attachments:
+                self._add_attachments(a)
             if not defer_upload:
                 self.ctx.check(lib().gvbd_scene_initialize(h, self.scene))
         except Exception:
@@ -500,9 +573,11 @@ class VBDPhysics:
                                                        fptr(v) if v is not None else None))
 
     # ---- scene files (BasePhysicsFramework.h:145-319, BasePhysicsFramework.cpp:264-281,441-503) ----
-    def saveState(self, json_path, frame_id=0, cur_time=0.0, binary=False):
+    def saveState(self, json_path, frame_id=0, cur_time=None, binary=False):
         """PhysicsState of every mesh (positions and velocities) as a recovery-state JSON; binary
-        writes <stem>.pos / <stem>.vel beside it."""
+        writes <stem>.pos / <stem>.vel beside it. cur_time defaults to the scene time."""
+        if cur_time is None:
+            cur_time = self.curTime
         self.ctx.check(lib().gvbd_scene_save_state(self.ctx.handle, self.scene, os.fsencode(json_path), int(frame_id),
                                                    float(cur_time), 1 if binary else 0))
 

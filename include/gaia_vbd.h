@@ -22,6 +22,10 @@
  *                                                                                 GraphColoring/gready.cpp:15-117)
  *   gvbd_scene_add_cloth_mesh   TriMeshFEM / VBDBaseTriMesh initialize           (TriMesh/TriMesh.cpp:15-111,481-753,
  *                                                                                 VBDCloth/VBDTriMeshStVK.cpp:50-88)
+ *   gvbd_scene_add_deformer     loadDeformers + DeformerRotator / DeformerTranslater (VBD/VBDDeformer.h:6-222,
+ *                               ClothDeformer*, VBDCloth/VBDClothDeformer.h:6-246)
+ *   gvbd_scene_add_attachments  VBDClothSimulationFramework::loadConstraints       (VBDCloth/VBDClothPhysics.cpp:148-224)
+ *   gvbd_scene_get/set_time     BasePhysicFramework::curTime                     (Framework/BasePhysicsFramework.h:100)
  *   gvbd_scene_collision_detection VBDClothSimulationFramework::applyCollisionDetection
  *                                                                                (VBDCloth/VBDClothPhysics.cpp:719-819)
  *   gvbd_cd_*                   ClothContactDetector (initialize, updateBVH, contactQueryVF/EE)
@@ -194,6 +198,61 @@ int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mes
  * TriMesh/TriMesh.cpp:15-111) */
 int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth_mesh_desc* desc, int32_t* mesh_out);
 int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene);
+
+/* Deformers: prescribed motion of selected vertices, which join the mesh's fixed points
+ * (VBDBaseDeformer / VBDClothBaseDeformer; loadDeformers, VBD/VBDDeformer.h:181-222 and
+ * VBDCloth/VBDClothDeformer.h:199-243, read from the "Deformers" array of a scene's parameter
+ * file). Added before gvbd_scene_initialize; the constants of each vertex are taken from the
+ * mesh's positions as added. Applied once per substep at the scene time t (float, advanced by dt
+ * at the start of every substep): on tet meshes after the initial step, on cloth meshes before
+ * detection and the initial guess, as the two reference frameworks do.
+ *   Rotator (DeformerRotator, VBDDeformer.h:45-131; ClothDeformerRotator,
+ *   VBDClothDeformer.h:44-128): p = root + R(theta) RootP, root = ((P - center) . axis) axis,
+ *   RootP = P - root, theta = angular_velocity * min(t, end_time) (end_time > 0); the axis is
+ *   used as given.
+ *   Translator on tet meshes (DeformerTranslater, VBDDeformer.h:134-178): p = P + speed min(t,
+ *   end_time) (end_time > 0); on cloth meshes (ClothDeformerTranslater, VBDClothDeformer.h:
+ *   130-196): p += speed dt in every substep with start_time <= t and (t < end_time or
+ *   end_time < 0). */
+#define GVBD_DEFORMER_ROTATOR 0
+#define GVBD_DEFORMER_TRANSLATOR 1
+typedef struct gvbd_deformer_desc {
+    int32_t kind;                 /* GVBD_DEFORMER_ROTATOR / GVBD_DEFORMER_TRANSLATOR */
+    float center[3];              /* Rotator */
+    float axis[3];                /* Rotator */
+    float speed[3];               /* Translator */
+    float start_time;             /* Translator on cloth meshes (deformationStartTime) */
+    double angular_velocity;      /* Rotator */
+    double end_time;              /* rotationEndTime / deformationEndTime; -1: none */
+    int32_t num_meshes;           /* selectedMeshes (scene mesh indices) */
+    const int32_t* meshes;
+    const int32_t* num_vertices;  /* per selected mesh */
+    const int32_t* vertices;      /* selectedVertices, concatenated */
+} gvbd_deformer_desc;
+int gvbd_scene_add_deformer(gvbd_context* ctx, int32_t scene, const gvbd_deformer_desc* desc);
+
+/* Vertex-face attachments of cloth vertices (VertexFaceAttachmentConstraint,
+ * VBDCloth/VBDTriMeshConstraints.h:31-49; loadConstraints, VBDCloth/VBDClothPhysics.cpp:148-224,
+ * read from PhysicsParams.VertexFaceAttachmentConstraintParams). Each ties vertex `vertex` of
+ * cloth mesh `mesh` to the point bary[0] x(face[0]) + bary[1] x(face[1]) + bary[2] x(face[2]) of
+ * cloth mesh `target_mesh`, or, with target_mesh = -1, to the point `bary`. Its solve adds
+ * f -= k b (x_read - target), H += k b^2 I (accumulateConstraintForceAndHessian,
+ * VBDClothPhysics.cpp:1465-1498): b = 1 on the vertex; each face vertex j with bary[j] != 0 gets
+ * b = -bary[j] and reads x_read from the vertex-side mesh at its own index, as the reference
+ * does. Added before gvbd_scene_initialize. */
+typedef struct gvbd_attachment_desc {
+    int32_t mesh, vertex;  /* vertexConstraintVertexInfos pair */
+    int32_t target_mesh;   /* vertexConstraintTargetFacesInfos: mesh (-1: fixed point), face */
+    int32_t face[3];
+    float bary[3];         /* vertexConstraintBarys */
+    float stiffness;       /* vertexConstraintStiffness (1e4 when the file gives none) */
+} gvbd_attachment_desc;
+int gvbd_scene_add_attachments(gvbd_context* ctx, int32_t scene, const gvbd_attachment_desc* list, int32_t n);
+
+/* Scene time (BasePhysicFramework::curTime, Framework/BasePhysicsFramework.h:100): 0 at
+ * creation, + dt per substep; gvbd_scene_load_state sets it from the recovery state. */
+int gvbd_scene_get_time(gvbd_context* ctx, int32_t scene, float* cur_time);
+int gvbd_scene_set_time(gvbd_context* ctx, int32_t scene, float cur_time);
 /* Models.json + Parameters.json; "${REPO_ROOT}" in paths is replaced by repo_root (may be NULL). */
 int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char* params_json,
                          const char* repo_root, int32_t* scene_out);

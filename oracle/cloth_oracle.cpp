@@ -35,6 +35,7 @@
 
 #include "oracle.h"
 #include "tet_core.h"
+#include "deform.h"
 #include "pool.h"
 
 // contact detector oracle (detector_oracle.cpp)
@@ -75,6 +76,7 @@ template <class R> struct ClothMesh {
     std::vector<R> tdminv, tvol, vprev;
     std::vector<uint8_t> surface;                     // vertex lies on the boundary surface
     R dampH = 0, dampD = 0;
+    std::vector<std::vector<oracle::Attachment>> att; // per vertex attachment records (cloth)
 };
 
 // Contact state of a scene with handle_collision (global ids: meshes concatenated in order).
@@ -100,6 +102,8 @@ template <class R> struct ClothScene {
     std::vector<std::vector<int32_t>> groups;
     std::unique_ptr<ContactState> contact;   // set by finalize when OP_HANDLE_COLLISION != 0
     int threads = 1;
+    std::vector<oracle::Deformer> deformers;
+    float clock = 0.f;                       // curTime (float in the reference)
 };
 
 template <class R> inline void cross3(const R* a, const R* b, R* c) {
@@ -781,6 +785,26 @@ template <class R> void cloth_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid)
             for (int i = 0; i < 9; ++i) h[i] += fh[i];
         }
     }
+    // vertex attachments (VBDClothPhysics.cpp:1465-1498)
+    if (!m.att.empty())
+        for (const oracle::Attachment& A : m.att[vid]) {
+            const R* xr = &s.meshes[A.readMesh].x[3 * (size_t)A.readVert];
+            R tgt[3];
+            if (A.tgtMesh >= 0) {
+                const std::vector<R>& tx = s.meshes[A.tgtMesh].x;
+                for (int d = 0; d < 3; ++d)
+                    tgt[d] = (R)A.p[0] * tx[3 * (size_t)A.f[0] + d] + (R)A.p[1] * tx[3 * (size_t)A.f[1] + d] +
+                             (R)A.p[2] * tx[3 * (size_t)A.f[2] + d];
+            } else {
+                for (int d = 0; d < 3; ++d) tgt[d] = (R)A.p[d];
+            }
+            const R kb = (R)A.k * (R)A.b;
+            for (int d = 0; d < 3; ++d) f[d] -= kb * (xr[d] - tgt[d]);
+            const R hk = kb * (R)A.b;
+            h[0] += hk;
+            h[4] += hk;
+            h[8] += hk;
+        }
     R dx[3];
     R* xn = &m.xnext[3 * vid];
     if (solve3<R>(h, f, dx)) {
@@ -1044,13 +1068,29 @@ template <class R> void tet_velocity(ClothScene<R>& s, ClothMesh<R>& m) {
 
 template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
     const int nsub = (int)s.p[OP_NUM_SUBSTEPS], iters = (int)s.p[OP_ITERATIONS];
+    // deformers (VBDClothPhysics.cpp:408-412): cloth meshes' deformed vertices move at iteration 0
+    // before detection and the initial guess; tet meshes' after the initial step (as the tet
+    // framework does, VBDPhysics.cpp:849-862)
+    auto deform = [&](bool clothPhase) {
+        for (const auto& D : s.deformers)
+            for (size_t k = 0; k < D.meshes.size(); ++k) {
+                ClothMesh<R>& m = s.meshes[D.meshes[k]];
+                if (m.tet == clothPhase) continue;
+                oracle::deformer_apply<R>(D, (int)k, clothPhase, s.clock, (float)s.dt, m.x.data());
+            }
+    };
+    auto initial = [&]() {
+        for (auto& m : s.meshes) m.tet ? tet_initial_step(s, m) : cloth_initial_step(s, m);
+        deform(false);
+    };
     for (int sub = 0; sub < nsub; ++sub) {
+        s.clock += (float)s.dt;
+        deform(true);
         for (int it = 0; it < iters; ++it) {
             // detection before the iteration's sweep: always at iteration 0 (before the initial
             // guess), later when a vertex left its bound (VBDClothPhysics.cpp:418-434)
             if (s.contact && (it == 0 || s.contact->required)) cloth_detect(s);
-            if (it == 0)
-                for (auto& m : s.meshes) m.tet ? tet_initial_step(s, m) : cloth_initial_step(s, m);
+            if (it == 0) initial();
             const bool friction = it >= (int)s.p[OP_FRICTION_START_ITER];
             for (auto& grp : s.groups) {
                 const int n = (int)grp.size() / 2;
@@ -1078,8 +1118,7 @@ template <class R> void cloth_step(ClothScene<R>& s, oracle::ThreadPool* pool) {
                 }
             }
         }
-        if (iters == 0)
-            for (auto& m : s.meshes) m.tet ? tet_initial_step(s, m) : cloth_initial_step(s, m);
+        if (iters == 0) initial();
         for (auto& m : s.meshes) m.tet ? tet_velocity(s, m) : cloth_velocity(s, m);
     }
 }
@@ -1236,6 +1275,50 @@ template <class R> static int cloth_state(ClothScene<R>& s, int mesh, double* po
     return 0;
 }
 
+template <class R> static int attachments_add(ClothScene<R>& s, int n, const int32_t* ints, const double* vals) {
+    const int nm = (int)s.meshes.size();
+    auto cloth = [&](int mi) { return mi >= 0 && mi < nm && !s.meshes[mi].tet; };
+    for (int i = 0; i < n; ++i) {
+        const int32_t* q = ints + 6 * i;
+        const double* w = vals + 4 * i;
+        const int mv = q[0], v = q[1], tg = q[2];
+        if (!cloth(mv) || v < 0 || v >= s.meshes[mv].nv) return -1;
+        if (tg >= 0) {
+            if (!cloth(tg)) return -1;
+            for (int k = 0; k < 3; ++k)
+                if (q[3 + k] < 0 || q[3 + k] >= s.meshes[tg].nv) return -1;
+        } else if (tg != -1) {
+            return -1;
+        }
+        oracle::Attachment A;
+        A.readMesh = mv;
+        A.readVert = v;
+        A.tgtMesh = tg;
+        for (int k = 0; k < 3; ++k) {
+            A.f[k] = q[3 + k];
+            A.p[k] = (double)(float)w[k];
+        }
+        A.k = (float)w[3];
+        A.b = 1.f;
+        auto push = [&](int mi, int vi, const oracle::Attachment& r) {
+            ClothMesh<R>& m = s.meshes[mi];
+            if (m.att.empty()) m.att.resize(m.nv);
+            m.att[vi].push_back(r);
+        };
+        push(mv, v, A);
+        if (tg >= 0)
+            for (int k = 0; k < 3; ++k) {
+                if ((float)A.p[k] == 0.f) continue;
+                oracle::Attachment F = A;
+                F.readVert = A.f[k];  // the vertex-side mesh at the face vertex's index
+                if (F.readVert >= s.meshes[mv].nv) return -1;
+                F.b = -(float)A.p[k];
+                push(tg, A.f[k], F);
+            }
+    }
+    return 0;
+}
+
 extern "C" {
 
 int oracle_cloth_prepare(int precision, int nv, const double* pos, int nf, const int32_t* faces, double density,
@@ -1375,6 +1458,22 @@ int oracle_cloth_scene_step(OracleClothScene* sc, int nsteps, int nthreads) {
         else cloth_step(sc->s32, pool);
     }
     return 0;
+}
+
+int oracle_cloth_scene_add_deformer(OracleClothScene* sc, int kind, const double* params, int nmeshes,
+                                    const int32_t* meshes, const int32_t* counts, const int32_t* verts) {
+    if (sc->precision == 64)
+        return oracle::deformer_add(sc->s64.deformers, sc->s64.meshes, kind, params, nmeshes, meshes, counts, verts);
+    return oracle::deformer_add(sc->s32.deformers, sc->s32.meshes, kind, params, nmeshes, meshes, counts, verts);
+}
+double oracle_cloth_scene_time(OracleClothScene* sc) { return sc->precision == 64 ? sc->s64.clock : sc->s32.clock; }
+void oracle_cloth_scene_set_time(OracleClothScene* sc, double t) { sc->s64.clock = sc->s32.clock = (float)t; }
+
+// n attachments: ints (mesh, vertex, target mesh, face v0, v1, v2) and values (bary or point,
+// stiffness) per record (loadConstraints, VBDClothPhysics.cpp:148-224).
+int oracle_cloth_scene_add_attachments(OracleClothScene* sc, int n, const int32_t* ints, const double* vals) {
+    if (sc->precision == 64) return attachments_add(sc->s64, n, ints, vals);
+    return attachments_add(sc->s32, n, ints, vals);
 }
 
 int oracle_cloth_scene_get_state(OracleClothScene* sc, int mesh, double* pos, double* vel) {

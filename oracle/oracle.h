@@ -113,6 +113,12 @@ int oracle_tet_scene_group(OracleTetScene*, int g, int32_t* out);
 int oracle_tet_scene_step(OracleTetScene*, int nsteps, int nthreads);
 int oracle_tet_scene_get_state(OracleTetScene*, int mesh, double* pos, double* vel);
 int oracle_tet_scene_set_state(OracleTetScene*, int mesh, const double* pos, const double* vel);
+// Deformers (VBD/VBDDeformer.h): kind 0 rotator, 1 translator; params o[3], axis[3],
+// angularVelocity, endTime, speed[3], startTime; before finalize. Scene time (curTime).
+int oracle_tet_scene_add_deformer(OracleTetScene*, int kind, const double* params, int nmeshes, const int32_t* meshes,
+                                  const int32_t* counts, const int32_t* verts);
+double oracle_tet_scene_time(OracleTetScene*);
+void oracle_tet_scene_set_time(OracleTetScene*, double t);
 /* VBDPhysics::evaluateConvergence: out[4] = {energy, inertia energy, elastic energy, avg force norm} */
 int oracle_tet_scene_evaluate_convergence(OracleTetScene*, double* out);
 
@@ -143,6 +149,11 @@ int oracle_cloth_scene_group(OracleClothScene*, int g, int32_t* out);
 int oracle_cloth_scene_step(OracleClothScene*, int nsteps, int nthreads);
 int oracle_cloth_scene_get_state(OracleClothScene*, int mesh, double* pos, double* vel);
 int oracle_cloth_scene_set_state(OracleClothScene*, int mesh, const double* pos, const double* vel);
+int oracle_cloth_scene_add_deformer(OracleClothScene*, int kind, const double* params, int nmeshes,
+                                    const int32_t* meshes, const int32_t* counts, const int32_t* verts);
+double oracle_cloth_scene_time(OracleClothScene*);
+void oracle_cloth_scene_set_time(OracleClothScene*, double t);
+int oracle_cloth_scene_add_attachments(OracleClothScene*, int n, const int32_t* ints, const double* vals);
 
 /* --- cloth vertex graph: mesh edges plus, with bending, the cross edge of every interior edge
  * (TriMeshVertexGraph.h:20-45) --- */

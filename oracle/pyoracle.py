@@ -55,6 +55,11 @@ def lib():
         L.oracle_tet_scene_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.oracle_tet_scene_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.oracle_tet_scene_evaluate_convergence.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_tet_scene_add_deformer.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, _ip]
+        L.oracle_tet_scene_time.argtypes = [ctypes.c_void_p]
+        L.oracle_tet_scene_time.restype = ctypes.c_double
+        L.oracle_tet_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oracle_tet_scene_set_time.restype = None
         _lib = L
     return _lib
 
@@ -166,6 +171,37 @@ def color_graph(nv, edges, algorithm="mcs"):
     return colors
 
 
+def deformer_args(d):
+    """(kind, params, meshes, counts, vertices) of a deformer given as the reference's JSON object
+    (VBD/VBDDeformer.h:181-222): DeformerName "Rotator" (center, axis, angularVelocity,
+    rotationEndTime) or "Translator" (speed, deformationEndTime, deformationStartTime), with
+    selectedMeshes and selectedVertices."""
+    name = d["DeformerName"]
+    if name not in ("Rotator", "Translator"):
+        raise ValueError(f"unrecognised deformer {name!r}")
+    rot = name == "Rotator"
+    end = d.get("rotationEndTime" if rot else "deformationEndTime", -1)
+    prm = np.array([*d.get("center", (0, 0, 0)), *d.get("axis", (0, 0, 0)), d.get("angularVelocity", 0.0), end,
+                    *d.get("speed", (0, 0, 0)), d.get("deformationStartTime", 0.0)], dtype=np.float64)
+    meshes = np.asarray(d["selectedMeshes"], dtype=np.int32)
+    sel = [np.asarray(v, dtype=np.int32).reshape(-1) for v in d["selectedVertices"]]
+    if len(sel) != len(meshes):
+        raise ValueError("selectedVertices needs one list per selected mesh")
+    counts = np.array([len(v) for v in sel], dtype=np.int32)
+    verts = np.concatenate(sel) if sel else np.zeros(0, np.int32)
+    return 0 if rot else 1, prm, meshes, counts, verts.astype(np.int32)
+
+
+def _add_deformer(fn, h, d):
+    kind, prm, meshes, counts, verts = deformer_args(d)
+    a, pa = _d(prm)
+    m, pm = _i(meshes)
+    c, pc = _i(counts)
+    v, pv = _i(verts)
+    if fn(h, kind, pa, len(meshes), pm, pc, pv) < 0:
+        raise ValueError("oracle: bad deformer")
+
+
 class TetScene:
     """Oracle tet VBD scene (VBDPhysics::runStep_serialCollisionHandling restated)."""
 
@@ -208,6 +244,18 @@ class TetScene:
 
     def step(self, n=1, threads=1):
         lib().oracle_tet_scene_step(self._h, n, threads)
+
+    def add_deformer(self, d):
+        """A deformer as the reference's JSON object (before finalize)."""
+        _add_deformer(lib().oracle_tet_scene_add_deformer, self._h, d)
+
+    @property
+    def time(self):
+        return lib().oracle_tet_scene_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        lib().oracle_tet_scene_set_time(self._h, float(t))
 
     def evaluate_convergence(self):
         out = np.zeros(4)
@@ -270,6 +318,12 @@ def _cloth_lib():
         L.oracle_cloth_scene_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.oracle_cloth_scene_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.oracle_cloth_scene_set_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_cloth_scene_add_deformer.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, _ip]
+        L.oracle_cloth_scene_time.argtypes = [ctypes.c_void_p]
+        L.oracle_cloth_scene_time.restype = ctypes.c_double
+        L.oracle_cloth_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oracle_cloth_scene_set_time.restype = None
+        L.oracle_cloth_scene_add_attachments.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp]
         L.oracle_color_tri_mesh.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip]
         L._cloth_bound = True
     return L
@@ -344,6 +398,22 @@ def color_tri_mesh(nv, faces, algorithm="mcs", with_bending=True):
     return colors
 
 
+def attachment_arrays(params):
+    """Rows (mesh, vertex, target mesh, f0, f1, f2) and (b0, b1, b2, stiffness) from the
+    reference's VertexFaceAttachmentConstraintParams (VBDClothPhysics.cpp:148-224; stiffness 1e4
+    when vertexConstraintStiffness is empty)."""
+    vi = np.asarray(params["vertexConstraintVertexInfos"], dtype=np.int32).reshape(-1, 2)
+    tf = np.asarray(params["vertexConstraintTargetFacesInfos"], dtype=np.int32).reshape(-1, 4)
+    bary = np.asarray(params["vertexConstraintBarys"], dtype=np.float64).reshape(-1, 3)
+    k = np.asarray(params.get("vertexConstraintStiffness", []), dtype=np.float64).reshape(-1)
+    n = len(vi)
+    if len(tf) != n or len(bary) != n or (len(k) and len(k) != n):
+        raise ValueError("attachment arrays disagree in length")
+    if not len(k):
+        k = np.full(n, 1e4)
+    return np.concatenate([vi, tf], axis=1).astype(np.int32), np.concatenate([bary, k[:, None]], axis=1)
+
+
 class ClothScene:
     """Oracle cloth VBD scene (VBDClothSimulationFramework::runStep_CDEveryIter, no contact)."""
 
@@ -415,6 +485,27 @@ class ClothScene:
 
     def step(self, n=1, threads=1):
         _cloth_lib().oracle_cloth_scene_step(self._h, n, threads)
+
+    def add_deformer(self, d):
+        """A deformer as the reference's JSON object (before finalize)."""
+        _add_deformer(_cloth_lib().oracle_cloth_scene_add_deformer, self._h, d)
+
+    def add_attachments(self, params):
+        """Vertex-face attachments given as the reference's VertexFaceAttachmentConstraintParams
+        object (VBDCloth/VBDTriMeshConstraints.h:6-29)."""
+        ints, vals = attachment_arrays(params)
+        i, pi = _i(ints.reshape(-1))
+        v, pv = _d(vals.reshape(-1))
+        if _cloth_lib().oracle_cloth_scene_add_attachments(self._h, len(ints), pi, pv) != 0:
+            raise ValueError("oracle: bad attachment")
+
+    @property
+    def time(self):
+        return _cloth_lib().oracle_cloth_scene_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        _cloth_lib().oracle_cloth_scene_set_time(self._h, float(t))
 
     def get_state(self, mesh=0):
         nv = self.nv[mesh]

@@ -16,6 +16,7 @@
 //   VBD/VBDPhysics.cpp:2821-2954,2563-2592 plane-ground boundary penalty + IPC friction
 //   VBD/VBDPhysics.cpp:2486-2561,794-807  velocity update + damping (step-invariant scaling)
 #include "oracle.h"
+#include "deform.h"
 #include "pool.h"
 #include "tet_core.h"
 
@@ -99,6 +100,8 @@ template <class R> struct Scene {
     std::vector<Mesh<R>> meshes;
     std::vector<std::vector<int32_t>> groups;  // interleaved (mesh, vertex)
     oracle::ThreadPool* pool = nullptr;
+    std::vector<oracle::Deformer> deformers;
+    float clock = 0.f;  // curTime (float in the reference)
 };
 
 // VBD_BaseMaterial.h:244-259, 201-221; VBD_BaseMaterial.cpp:7-50.
@@ -231,7 +234,13 @@ template <class R> void scene_step(Scene<R>& s, int nthreads) {
     const bool accel = s.p[OP_USE_ACCELERATOR] != 0;
     const R rho = (R)s.p[OP_ACCELERATOR_RHO];
     for (int sub = 0; sub < nsub; ++sub) {
+        s.clock += (float)s.dt;
         for (auto& m : s.meshes) apply_initial_step(s, m);
+        // runStepGPU (VBDPhysics.cpp:839-862): deformers move their (fixed) vertices after the
+        // initial step
+        for (const auto& D : s.deformers)
+            for (size_t k = 0; k < D.meshes.size(); ++k)
+                oracle::deformer_apply<R>(D, (int)k, false, s.clock, (float)s.dt, s.meshes[D.meshes[k]].x.data());
         // Chebyshev acceleration (runStepGPU, VBDPhysics.cpp:872-906): both histories start at
         // the initial guess
         R omega = R(1);
@@ -506,6 +515,15 @@ int oracle_tet_scene_step(OracleTetScene* sc, int nsteps, int nthreads) {
     }
     return 0;
 }
+
+int oracle_tet_scene_add_deformer(OracleTetScene* sc, int kind, const double* params, int nmeshes,
+                                  const int32_t* meshes, const int32_t* counts, const int32_t* verts) {
+    if (sc->precision == 64)
+        return oracle::deformer_add(sc->s64.deformers, sc->s64.meshes, kind, params, nmeshes, meshes, counts, verts);
+    return oracle::deformer_add(sc->s32.deformers, sc->s32.meshes, kind, params, nmeshes, meshes, counts, verts);
+}
+double oracle_tet_scene_time(OracleTetScene* sc) { return sc->precision == 64 ? sc->s64.clock : sc->s32.clock; }
+void oracle_tet_scene_set_time(OracleTetScene* sc, double t) { sc->s64.clock = sc->s32.clock = (float)t; }
 
 int oracle_tet_scene_get_state(OracleTetScene* sc, int mesh, double* pos, double* vel) {
     return sc->precision == 64 ? get_state(sc->s64, mesh, pos, vel) : get_state(sc->s32, mesh, pos, vel);

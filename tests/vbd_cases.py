@@ -51,6 +51,10 @@ def build_gpu(ctx, spec):
             colors = gaia_amd.color_tet_mesh(len(m["pos"]), m["tets"], m["coloring"])
             sim.addTetMesh(m["pos"], m["tets"], fixed_points=m["fixed"], vertex_colors=colors, velocities=m["vel"],
                            **m["material"])
+    for d in spec.get("deformers", []):
+        sim.addDeformer(d)
+    if "attachments" in spec:
+        sim.addAttachments(spec["attachments"])
     sim.initialize()
     return sim
 
@@ -65,6 +69,10 @@ def build_oracle_cloth(oracle, spec, precision=32):
         vel = None if m["vel"] is None else (np.asarray(m["vel"], np.float32).astype(np.float64)
                                              if precision == 32 else m["vel"])
         sc.add_mesh(pos, m["faces"], m["material"], colors, fixed=m["fixed"], vel=vel)
+    for d in spec.get("deformers", []):
+        sc.add_deformer(d)
+    if "attachments" in spec:
+        sc.add_attachments(spec["attachments"])
     sc.finalize()
     return sc
 
@@ -123,6 +131,8 @@ def build_oracle(oracle, spec, precision=32):
         vel = None if m["vel"] is None else (np.asarray(m["vel"], np.float32).astype(np.float64)
                                              if precision == 32 else m["vel"])
         sc.add_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
+    for d in spec.get("deformers", []):
+        sc.add_deformer(d)
     sc.finalize()
     return sc
 
@@ -242,3 +252,22 @@ def build_oracle_mixed(oracle, spec, precision=32):
             sc.add_tet_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
     sc.finalize()
     return sc
+
+
+def twist_spec(n=16, steps=30, iterations=10):
+    """The twisting experiment's set-up at small size (S01_Twisting, Parameters.json "Deformers"):
+    a horizontal sheet whose two opposite edges are turned by Rotators about the x axis in
+    opposite directions (angularVelocity pi/4, rotationEndTime 60)."""
+    pos, faces = scenes.cloth_sheet(n, 0.5, 0.0, origin=(-0.25, -0.25))
+    left = [i * n for i in range(n)]
+    right = [i * n + n - 1 for i in range(n)]
+    mu, lm = scenes.stvk_lame(1e5, 0.3)
+    material = dict(density=0.2, mu=mu, lmbd=lm, bending_stiffness=1e-4, initialization_type=2)
+    physics = physics_dict(gravity=(0.0, 0.0, 0.0), time_step=1.0 / 60.0, num_substeps=2, iterations=iterations,
+                           world_lower=(-1e7, -1e7, -1e7))
+    rot = lambda axis, sel: dict(DeformerName="Rotator", center=[0, 0, 0], axis=axis,
+                                 angularVelocity=np.pi / 4, selectedMeshes=[0], selectedVertices=[sel],
+                                 rotationEndTime=60)
+    return dict(physics=physics, steps=steps,
+                meshes=[dict(pos=pos, faces=faces, material=material, fixed=[], vel=None, coloring="mcs")],
+                deformers=[rot([1, 0, 0], right), rot([-1, 0, 0], left)])
