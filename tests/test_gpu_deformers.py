@@ -317,3 +317,55 @@ def test_argument_errors(gpu_ctx):
         sim.addAttachments(bad_att)
         with pytest.raises(gaia_amd.VBDError, match="attachment"):
             sim.initialize()
+
+
+def test_served_deformer_scenes_equal_alone(gpu_ctx):
+    """Scenes with deformers in the serving loop under a budget that forces copy-out between
+    frames: the scene time and the deformed vertices travel with the arena, so each served scene
+    equals the scene stepped alone."""
+    from gaia_amd import Server
+
+    rot = dict(DeformerName="Rotator", center=[0, 0, 0], axis=[0, 1, 0], angularVelocity=1.5, selectedMeshes=[0],
+               selectedVertices=[[0, 1, 2, 3]])
+    ds = [rot, None, dict(rot, axis=[1, 0, 0]), None]
+
+    def make(i, upload):
+        pos, tets = scenes.cube_tet_mesh(4 + i, 0.05, origin=(0.0, 0.3, 0.0))
+        sim = VBDPhysics(gpu_ctx)
+        sim.setPhysicsParams(**physics_dict(gravity=(0.0, -9.8, 0.0), time_step=DT, num_substeps=1, iterations=5,
+                                            world_lower=(-1e7, 0.0, -1e7)))
+        sim.addTetMesh(pos, tets, density=1000.0, mu=1e5, lmbd=1e6)
+        if ds[i]:
+            sim.addDeformer(ds[i])
+        sim.initialize(defer_upload=not upload)
+        return sim
+
+    before = gpu_ctx.memory_bytes()
+    probe = make(3, True)
+    biggest = gpu_ctx.memory_bytes() - before
+    probe.destroy()
+    sims = [make(i, False) for i in range(4)]
+    srv = Server(gpu_ctx, int(1.5 * biggest))
+    for s in sims:
+        srv.add_scene(s)
+    rng = np.random.default_rng(2)
+    counts = np.zeros(4, int)
+    for _ in range(12):
+        for i in rng.integers(0, 4, size=2):
+            srv.request_step(sims[i])
+            counts[i] += 1
+        srv.tick()
+    while srv.stats()["pending"]:
+        srv.tick()
+    assert srv.stats()["evictions"] > 0
+    for i in range(4):
+        alone = make(i, True)
+        alone.runSteps(int(counts[i]))
+        xa, va = alone.getState(0)
+        xs, vs = sims[i].getState(0)
+        assert np.array_equal(xa, xs) and np.array_equal(va, vs), f"scene {i}"
+        assert alone.curTime == sims[i].curTime == clock(int(counts[i]))
+        alone.destroy()
+    srv.destroy()
+    for s in sims:
+        s.destroy()
