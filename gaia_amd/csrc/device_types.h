@@ -43,7 +43,6 @@ static_assert(sizeof(TetEnergyRec) == 64, "TetEnergyRec must be 64 bytes");
 enum SlotFlags : int32_t {
     SLOT_DAMPING = 1,  // the vertex's mesh has non-zero material damping
     SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)
-    SLOT_JACOBI = 4,   // tet (scenes with collision handling): write positionsNext, committed per group
     SLOT_ATTACH = 8,   // cloth: the vertex has attachment records (SceneView::attBegin)
 };
 
@@ -164,6 +163,12 @@ struct ContactView {
     const float* bound;       // per vertex conservative bound
     const float4* xAtCD;      // positions at the last detection
     int* flag;                // set when a vertex is truncated: detect before the next iteration
+    // Tet vertices of contact scenes are updated in place; a contact pair may join two vertices
+    // of one parallel group, so contact partners of the solving vertex's own group are read from
+    // the iteration-start snapshot (their value at the start of the group, as the reference's
+    // positionsNext + copy gives); vgroup: per vertex parallel group (-1: not solved)
+    const float4* snap;
+    const int* vgroup;
 };
 
 // Device view of one scene, referenced by work items of a (batched) launch.

@@ -282,10 +282,7 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
             atomicOr(S.cv.flag, 1);
         }
     }
-    if (slot.w & SLOT_JACOBI)
-        S.pnext[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
-    else if (moved)
-        S.pos[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
+    if (CONTACT || moved) S.pos[v] = make_float4(xn.x, xn.y, xn.z, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,13 +455,19 @@ __device__ void contact_apply(float dt, float b, float lambda, float d2E, f3 n, 
     h.yz += ch * (e0.y * e0.z + e1.y * e1.z); h.zz += ch * (e0.z * e0.z + e1.z * e1.z);
 }
 
+// Position of contact partner q for a vertex of parallel group g (see ContactView::snap).
+__device__ __forceinline__ f3 cpos(const SceneView& S, int q, int g) {
+    return xyz(S.cv.vgroup[q] == g ? S.cv.snap[q] : S.pos[q]);
+}
+
 // VF contact of vertex u with face `face`, on the vertex of order `order` (3: u itself,
-// 0..2: face vertex; weights 1 and -bary_order), VBDClothPhysics.cpp:1248-1321
-__device__ void vf_contact(const SceneView& S, int u, int face, int order, f3& f, Sym3& h) {
+// 0..2: face vertex; weights 1 and -bary_order), VBDClothPhysics.cpp:1248-1321; g is the
+// solving vertex's parallel group
+__device__ void vf_contact(const SceneView& S, int u, int face, int order, int g, f3& f, Sym3& h) {
     const ContactView& C = S.cv;
     const int4 fv = C.faces[face];
-    const f3 x = xyz(S.pos[u]);
-    const f3 a = xyz(S.pos[fv.x]), b = xyz(S.pos[fv.y]), c = xyz(S.pos[fv.z]);
+    const f3 x = cpos(S, u, g);
+    const f3 a = cpos(S, fv.x, g), b = cpos(S, fv.y, g), c = cpos(S, fv.z, g);
     f3 cp, bary;
     closest_point_bary(x, a, b, c, cp, bary);
     const f3 diff = x - cp;
@@ -482,10 +485,10 @@ __device__ void vf_contact(const SceneView& S, int u, int face, int order, f3& f
 
 // EE contact of edge e1 with e2, on the vertex of order `order` of e1 (weights 1 - mu1, mu1),
 // VBDClothPhysics.cpp:1323-1399
-__device__ void ee_contact(const SceneView& S, int e1, int e2, int order, f3& f, Sym3& h) {
+__device__ void ee_contact(const SceneView& S, int e1, int e2, int order, int g, f3& f, Sym3& h) {
     const ContactView& C = S.cv;
     const int4 A = C.edges[e1], B = C.edges[e2];
-    const f3 p1 = xyz(S.pos[A.x]), p2 = xyz(S.pos[A.y]), q1 = xyz(S.pos[B.x]), q2 = xyz(S.pos[B.y]);
+    const f3 p1 = cpos(S, A.x, g), p2 = cpos(S, A.y, g), q1 = cpos(S, B.x, g), q2 = cpos(S, B.y, g);
     f3 c1, c2;
     float mu1, mu2;
     segment_closest(p1, p2, q1, q2, c1, c2, mu1, mu2);
@@ -517,15 +520,16 @@ template <int LPV>
 __device__ void contact_forces(const SceneView& S, int v, int sub, const int* faceRef, int fb, int fe,
                                const int* edgeRef, int eb, int ee, f3& f, Sym3& h) {
     const ContactView& C = S.cv;
+    const int g = C.vgroup[v];
     const int nvf = C.vfCount[v];
-    for (int k = sub; k < nvf; k += LPV) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, f, h);
+    for (int k = sub; k < nvf; k += LPV) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, g, f, h);
     int t = sub, base = 0;
     for (int r = fb; r < fe; ++r) {
         const int ref = faceRef[r];
         const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int2 range = C.fvRange[face];
         const int end = base + (range.y - range.x);
-        for (; t < end; t += LPV) vf_contact(S, C.fvVerts[range.x + (t - base)], face, order, f, h);
+        for (; t < end; t += LPV) vf_contact(S, C.fvVerts[range.x + (t - base)], face, order, g, f, h);
         base = end;
     }
     t = sub;
@@ -535,7 +539,7 @@ __device__ void contact_forces(const SceneView& S, int v, int sub, const int* fa
         const int e = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
         const int end = base + min(C.eeCount[e], C.eeCap);
         for (; t < end; t += LPV)
-            ee_contact(S, e, __float_as_int(C.eeRec[(int64_t)e * C.eeCap + (t - base)].x), order, f, h);
+            ee_contact(S, e, __float_as_int(C.eeRec[(int64_t)e * C.eeCap + (t - base)].x), order, g, f, h);
         base = end;
     }
 }
@@ -744,7 +748,7 @@ __global__ __launch_bounds__(256) void cloth_commit_kernel(const SceneView* __re
     const WorkItem it = items[blockIdx.x];
     const SceneView& S = scenes[it.scene];
     for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
-        const int v = it.kind == WORK_TET ? S.slots[it.slotBegin + i].x : S.cslots[it.slotBegin + i].a.x;
+        const int v = S.cslots[it.slotBegin + i].a.x;
         S.pos[v] = S.pnext[v];
     }
 }

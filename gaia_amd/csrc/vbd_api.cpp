@@ -48,6 +48,8 @@ struct DeviceScene {
     int* tFaceRef = nullptr;    // contact scenes: per tet slot incident boundary face, face id | order << 30
     int* tEdgeRef = nullptr;    // contact scenes: per tet slot incident boundary edge, edge id | order << 30
     int4* tcontact = nullptr;   // contact scenes: per tet slot, its tFaceRef and tEdgeRef ranges
+    float4* snap = nullptr;     // contact scenes: positions at the start of the iteration
+    int* vgroup = nullptr;      // contact scenes: per vertex parallel group (-1: not solved)
     float4 *accPrevPrev = nullptr, *accPrevIter = nullptr;  // use_accelerator: iteration history
     float* accOmega = nullptr;                              // use_accelerator: omega per iteration
     float* clock = nullptr;                                 // scene time
@@ -94,7 +96,6 @@ struct SceneObj {
     std::vector<int64_t> slotWorkBytes;        // algorithmic bytes per tet slot then per cloth slot
     std::vector<uint8_t> clothSlotInplace;     // per cloth slot: solved in place (no commit pass)
     std::vector<TetSurface> tetSurfaces;       // per tet mesh, scenes with collision handling
-    bool tetJacobi = false;                    // tet slots write positionsNext (collision handling)
     DeviceScene dev;
     SceneView view{};
     int32_t plan = -1;  // the scene's own single-scene plan
@@ -343,8 +344,6 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
     const bool damping = m.mat.dampingHydrostatic != 0.f || m.mat.dampingDeviatoric != 0.f;
     slot.w = damping ? SLOT_DAMPING : 0;
     if (refs) {
-        // contact pairs may join vertices of one colour: solve the group Jacobi-style
-        slot.w |= SLOT_JACOBI;
         const TetSurface& S = *refs->surf;
         int4 r;
         r.x = (int32_t)refs->faceRef->size();
@@ -489,7 +488,6 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             eo += (int32_t)m.edges.size();
         }
     }
-    s.tetJacobi = contact && !s.tets.empty();
     s.clothSlotInplace.clear();
     // vertex-face attachments (loadConstraints, VBDClothPhysics.cpp:148-224): the vertex side gets
     // b = 1, each face vertex with a non-zero barycentric b = -bary (its position read from the
@@ -668,6 +666,16 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bTF = contact ? align(sizeof(int32_t) * std::max<size_t>(tFaceRef.size(), 1)) : 0;
     const size_t bTE = contact ? align(sizeof(int32_t) * std::max<size_t>(tEdgeRef.size(), 1)) : 0;
     const size_t bTC = contact ? align(sizeof(int4) * std::max<size_t>(tcontact.size(), 1)) : 0;
+    const size_t bSn = contact ? bV : 0, bVG = contact ? align(sizeof(int32_t) * (size_t)std::max(nv, 1)) : 0;
+    std::vector<int32_t> vgroup;
+    if (contact) {
+        vgroup.assign(nv, -1);
+        for (int32_t g = 0; g < G; ++g)
+            for (size_t i = 0; i + 1 < s.groups[g].size(); i += 2) {
+                const int32_t im = s.groups[g][i], lv = s.groups[g][i + 1];
+                if (!fixedMask[im][lv]) vgroup[s.vertexOffset[im] + lv] = g;
+            }
+    }
     const size_t bA = accel ? bV : 0, bW = accel ? align(sizeof(float) * omega.size()) : 0;
     const size_t bClk = 256;
     const size_t bDV = dverts.empty() ? 0 : align(sizeof(DeformVert) * dverts.size());
@@ -675,7 +683,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bAB = attBegin.empty() ? 0 : align(sizeof(int32_t) * attBegin.size());
     const size_t bAR = att.empty() ? 0 : align(sizeof(AttachRec) * att.size());
     const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW +
-                         bClk + bDV + bDP + bAB + bAR;
+                         bClk + bDV + bDP + bAB + bAR + bSn + bVG;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -711,6 +719,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         d.tFaceRef = (int*)take(bTF);
         d.tEdgeRef = (int*)take(bTE);
         d.tcontact = (int4*)take(bTC);
+        d.snap = (float4*)take(bSn);
+        d.vgroup = (int*)take(bVG);
     }
     if (accel) {
         d.accPrevPrev = (float4*)take(bA);
@@ -748,6 +758,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         up(d.tFaceRef, tFaceRef.data(), sizeof(int32_t) * tFaceRef.size());
         up(d.tEdgeRef, tEdgeRef.data(), sizeof(int32_t) * tEdgeRef.size());
         up(d.tcontact, tcontact.data(), sizeof(int4) * tcontact.size());
+        up(d.snap, pos.data(), sizeof(float4) * nv);
+        up(d.vgroup, vgroup.data(), sizeof(int32_t) * nv);
     }
     if (accel) {
         up(d.accPrevPrev, pos.data(), sizeof(float4) * nv);
@@ -831,9 +843,6 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
                 items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
             commit_runs(s, g, kVertexChunk,
                         [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
-            if (s.tetJacobi)  // tet slots of contact scenes write positionsNext
-                for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += kVertexChunk)
-                    commits.push_back({(int32_t)si, k, std::min(kVertexChunk, e - k), WORK_TET});
         }
         p.groupItemBegin[g + 1] = (int32_t)items.size();
         p.groupCommitBegin[g + 1] = (int32_t)commits.size();
@@ -963,7 +972,14 @@ void plan_reset_counters(gvbd_context* ctx, PlanObj& p) {
 
 // One iteration of every scene: one persistent launch (work queue, the `epoch`-th launch since
 // the counters were reset) or one launch per parallel group.
+// Contact scenes: the iteration-start snapshot read for same-group contact partners.
+void plan_contact_snapshot(gvbd_context* ctx, PlanObj& p) {
+    SceneObj& s = scene_of(ctx, p.scenes[0]);
+    hip_check(launch_stream_copy(s.view.pos, s.dev.snap, s.nv, ctx->stream), "snapshot launch");
+}
+
 void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
+    if (p.contact) plan_contact_snapshot(ctx, p);
     if (p.mode == GVBD_MODE_WORK_QUEUE) {
         hip_check(launch_queue_solve(queue_args(ctx, p), it, epoch, p.queueGrid, p.lpv, p.block, ctx->stream),
                   "work-queue solve launch");
@@ -1188,6 +1204,8 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         cv.bound = dv.bounds;
         cv.xAtCD = dv.xAtDetection;
         cv.flag = dv.flag;
+        cv.snap = s.dev.snap;
+        cv.vgroup = s.dev.vgroup;
         s.view.hasContact = 1;
         s.view.initBound = dv.bounds;
     }
@@ -1860,6 +1878,7 @@ int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t gr
         PlanObj& p = scene_plan(ctx, scene);
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
+        if (p.contact) plan_contact_snapshot(ctx, p);
         plan_group(ctx, p, group, iteration);
     });
 }
@@ -2394,6 +2413,7 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
     mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
     mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact), mv(d.accPrevPrev), mv(d.accPrevIter), mv(d.accOmega);
+    mv(d.snap), mv(d.vgroup);
     mv(d.clock), mv(d.dverts), mv(d.dparams), mv(d.attBegin), mv(d.att);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
