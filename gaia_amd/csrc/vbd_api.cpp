@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2897,6 +2898,11 @@ int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu
 
 static_assert(sizeof(gvbd_vf_contact) == sizeof(VFContactRec), "gvbd_vf_contact layout");
 static_assert(sizeof(gvbd_server_stats) == 72 && sizeof(gvbd_served_step) == 32, "server record layout");
+static_assert(sizeof(gvbd_deformer_desc) == 96 && offsetof(gvbd_deformer_desc, angular_velocity) == 48 &&
+                  offsetof(gvbd_deformer_desc, meshes) == 72,
+              "gvbd_deformer_desc layout");
+static_assert(sizeof(gvbd_attachment_desc) == 40 && offsetof(gvbd_attachment_desc, bary) == 24,
+              "gvbd_attachment_desc layout");
 static_assert(sizeof(gvbd_ee_contact) == sizeof(EEContactRec), "gvbd_ee_contact layout");
 
 int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records) {

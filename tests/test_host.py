@@ -112,3 +112,15 @@ def test_tet_contact_workload_bodies():
     mus = np.array([b["material"]["mu"] for b in bodies])
     assert mus.min() >= 1e4 and mus.max() <= 1e6
     assert all(b["material"]["lmbd"] == 10 * b["material"]["mu"] for b in bodies)
+
+
+def test_deformer_and_attachment_descriptor_layouts():
+    """ctypes mirrors of gvbd_deformer_desc / gvbd_attachment_desc match the C layout the library
+    asserts at compile time (96 and 40 bytes on LP64)."""
+    import ctypes
+
+    from gaia_amd._lib import AttachmentDesc, DeformerDesc
+
+    assert ctypes.sizeof(DeformerDesc) == 96 and DeformerDesc.angular_velocity.offset == 48
+    assert DeformerDesc.meshes.offset == 72
+    assert ctypes.sizeof(AttachmentDesc) == 40 and AttachmentDesc.bary.offset == 24
