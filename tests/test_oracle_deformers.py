@@ -256,3 +256,33 @@ def test_attachment_arguments(oracle):
     with pytest.raises(ValueError):
         lone_vertices(oracle, dict(vertexConstraintVertexInfos=[0, 0, 0, 1], vertexConstraintTargetFacesInfos=[-1, 0, 0, 0],
                                    vertexConstraintBarys=[[0, 0, 0]]))
+
+
+def test_reference_experiment_deformers_parse():
+    """The "Deformers" arrays of the reference's twisting (S01) and knot-pulling (S02) parameter
+    files (fixture written by tests/golden/make_deformer_fixtures.py) read as the set-ups built
+    here: S01 turns the two x-extreme columns of its 50 x 50 sheet about +x and -x at pi/4 rad/s
+    until t = 60 (the layout of twist_spec), S02 pulls two vertex sets apart at 10 m/s."""
+    import json
+    import os
+
+    from oracle.pyoracle import deformer_args
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "deformers", "reference_deformers.json")
+    ref = json.load(open(path))
+    s01 = [deformer_args(d) for d in ref["S01_Twisting"]]
+    assert [k for k, *_ in s01] == [0, 0]
+    n = 50
+    right, left = [i * n + n - 1 for i in range(n)], [i * n for i in range(n)]
+    for (kind, prm, meshes, counts, verts), axis, sel in zip(s01, ([1, 0, 0], [-1, 0, 0]), (right, left)):
+        assert list(meshes) == [0] and list(counts) == [n] and sorted(verts.tolist()) == sel
+        np.testing.assert_allclose(prm[3:6], axis)
+        np.testing.assert_allclose(prm[6], np.pi / 4)
+        assert prm[7] == 60
+    s02 = [deformer_args(d) for d in ref["S02_KnotPulling"]]
+    assert [k for k, *_ in s02] == [1, 1]
+    np.testing.assert_allclose(s02[0][1][8:11], [-10, 0, 0])
+    np.testing.assert_allclose(s02[1][1][8:11], [10, 0, 0])
+    for kind, prm, meshes, counts, verts in s02:
+        assert prm[7] == -1 and prm[11] == 0  # no end time: the pull runs on; starts at 0
+        assert len(verts) == counts.sum() > 0
