@@ -94,16 +94,86 @@ def solve_kernel_roofline(ctx, batch, schedule, reps=5):
                 algorithmic_bytes_per_launch=int(avg_bytes))
 
 
-def cpu_baseline(specs, budget_s=12.0):
+def host_cpus():
+    """The host's CPUs as this process sees them: nproc, the CPUs it may run on (affinity), the
+    cgroup CPU quota when one is set, the CPU model (lscpu), and the thread count the CPU baseline
+    uses: every CPU available to the process, at most 16 (one GPU's share of the GPU host)."""
+    import subprocess
+
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return dict(nproc=os.cpu_count(), affinity=affinity, cgroup_quota_cpus=quota, model=model,
+                threads=min(usable, 16))
+
+
+def c1_leg(ctx, threads, steps=300):
+    """Config 1 (the plain soft cube: 1,331 vertices, 5,000 tets, greedy colouring, ground only):
+    `steps` frames from the initial state on the GPU plain path (one scene, its own HIP graph) and
+    on the FP32 oracle at 1 thread and at `threads`; ms per step and vertex-solves/s each."""
+    from oracle import pyoracle
+
+    import gaia_amd
+    from gaia_amd import VBDPhysics, scenes
+    from gaia_amd.vbd import physics_dict
+
+    c = scenes.c1_soft_cube()
+    colors = gaia_amd.color_tet_mesh(len(c["positions"]), c["tets"], "greedy")
+    solves = len(c["positions"]) * c["physics"]["iterations"] * steps
+    sim = VBDPhysics(ctx)
+    sim.setPhysicsParams(**c["physics"])
+    sim.addTetMesh(c["positions"], c["tets"], vertex_colors=colors, **c["material"])
+    sim.initialize()
+    sim.recordVBDSolveGraph(True)
+    sim.runStep()  # graph capture, untimed
+    x0, v0 = c["positions"], np.zeros_like(c["positions"])
+    sim.setState(0, x0, v0)
+    ctx.synchronize()
+    t = time.perf_counter()
+    sim.runSteps(steps)
+    ctx.synchronize()
+    gpu_s = time.perf_counter() - t
+    sim.destroy()
+    out = dict(workload=f"soft cube 1,331 vertices / 5,000 tets, greedy colouring (8 colours), 10 iterations, "
+                        f"{steps} frames from rest", steps=steps,
+               gpu_plain=dict(ms_per_step=round(gpu_s / steps * 1e3, 4), vertex_solves_per_s=round(solves / gpu_s, 1)))
+    for n in sorted({1, threads}):
+        sc = pyoracle.TetScene(physics_dict(**c["physics"]), precision=32)
+        sc.add_mesh(c["positions"], c["tets"], c["material"], colors)
+        sc.finalize()
+        t = time.perf_counter()
+        sc.step(steps, threads=n)
+        s = time.perf_counter() - t
+        out[f"cpu_{n}_threads"] = dict(ms_per_step=round(s / steps * 1e3, 3), vertex_solves_per_s=round(solves / s, 1))
+    return out
+
+
+def cpu_baseline(specs, threads, budget_s=12.0):
     """The oracle (plain C++ restatement of the reference CPU path, full 9x9
-    assembly per tet) on the host cores, over the first scenes of the workload."""
+    assembly per tet) on `threads` host threads, over the first scenes of the workload."""
     from oracle import pyoracle
 
     from gaia_amd.vbd import physics_dict
 
     import gaia_amd
 
-    threads = min(16, os.cpu_count() or 1)
     solves = 0
     elapsed = 0.0
     used = 0
@@ -126,7 +196,8 @@ def cpu_baseline(specs, budget_s=12.0):
         if elapsed > budget_s:
             break
     return dict(value=round(solves / elapsed, 1), unit="vertex-solves/s", cores=threads, kind="port",
-                sample=f"first {used} scenes of the workload, 1 step each, FP32 oracle on {threads} threads")
+                sample=f"first {used} scenes of the workload, 1 step each, FP32 oracle on {threads} threads",
+                seconds=round(elapsed, 2))
 
 
 def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
@@ -196,7 +267,7 @@ def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
         # the oracle's uniform-grid detector (same predicates) on the host cores, one step
         from oracle import pyoracle
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpus()["threads"]
         D = pyoracle.Detector(meshes)
         pos = cd.positions()
         t = time.perf_counter()
@@ -350,7 +421,7 @@ def contact_bench(args):
 
         from gaia_amd.vbd import physics_dict
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpus()["threads"]
         sc = pyoracle.ClothScene(physics_dict(**phys), precision=32)
         for b in bodies:
             sc.add_tet_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
@@ -616,8 +687,16 @@ def main():
     copy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5)
     roof["measured_copy_gbs"] = round(2 * args.copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
     log(f"roofline: {roof}")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(specs)
-    if cpu:
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the FP32 oracle on the host's cores (every CPU available to the process, at most 16) and on
+        # one thread, over a bounded sample of the workload, and config 1 over its 300 frames
+        host = host_cpus()
+        cpu = cpu_baseline(specs, host["threads"])
+        one = cpu_baseline(specs, 1, budget_s=10.0)
+        cpu["one_thread"] = dict(value=one["value"], sample=one["sample"])
+        cpu["host"] = host
+        cpu["c1"] = c1_leg(ctx, host["threads"])
         log(f"cpu baseline: {cpu}")
 
     out = {

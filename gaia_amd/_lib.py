@@ -83,6 +83,9 @@ class TetMeshDesc(ctypes.Structure):
         ("coloring_algorithm", ctypes.c_int32),
         ("friction_dynamic", ctypes.c_float),
         ("friction_eps_v", ctypes.c_float),
+        ("has_no_grav_zone", ctypes.c_int32),
+        ("no_grav_zone_threshold", ctypes.c_float),
+        ("max_velocity_magnitude", ctypes.c_float),
     ]
 
 
@@ -111,6 +114,9 @@ class ClothMeshDesc(ctypes.Structure):
         ("coloring_algorithm", ctypes.c_int32),
         ("friction_dynamic", ctypes.c_float),
         ("friction_eps_v", ctypes.c_float),
+        ("has_no_grav_zone", ctypes.c_int32),
+        ("no_grav_zone_threshold", ctypes.c_float),
+        ("max_velocity_magnitude", ctypes.c_float),
     ]
 
 
@@ -235,6 +241,7 @@ _SIGNATURES = {
     "gvbd_scene_update_velocities": [_P, _I],
     "gvbd_scene_collision_detection": [_P, _I],
     "gvbd_scene_contact_stats": [_P, _I, ctypes.POINTER(ContactStats)],
+    "gvbd_scene_solve_failures": [_P, _I, _I, ctypes.POINTER(ctypes.c_int64)],
     "gvbd_scene_add_deformer": [_P, _I, ctypes.POINTER(DeformerDesc)],
     "gvbd_scene_add_attachments": [_P, _I, ctypes.POINTER(AttachmentDesc), _I],
     "gvbd_scene_get_time": [_P, _I, ctypes.POINTER(ctypes.c_float)],
@@ -320,7 +327,7 @@ class VBDError(RuntimeError):
 def header_functions():
     """Names of every function declared in include/gaia_vbd.h."""
     text = open(HEADER_PATH).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(gvbd_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:GVBD_API\s+)?(?:int|const char\*)\s+(gvbd_\w+)\s*\(", text, re.M)))
 
 
 def lib():

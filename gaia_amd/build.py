@@ -43,6 +43,12 @@ def build_library(verbose=False):
         "-std=c++17",
         "-fPIC",
         "-shared",
+        # export only the C ABI (GVBD_API in include/gaia_vbd.h), not the C++ internals
+        "-fvisibility=hidden",
+        "-fvisibility-inlines-hidden",
+        # template instantiations of the standard library keep default visibility: the version
+        # script makes the gvbd_* entry points the only exported symbols
+        "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"),
         "-Wall",
         "-Wno-unused-function",
         "-Wno-unused-result",

@@ -205,12 +205,20 @@ struct SceneView {
     // cloth attachments: records of scene vertex v are att[attBegin[v] .. attBegin[v + 1])
     const int* attBegin;
     const AttachRec* att;
+    // no-gravity zone: per vertex (noGravZoneThreshold, maxVelocityMagnitude) of its mesh, read for
+    // vertices flagged VFLAG_NOGRAV_POS / VFLAG_NOGRAV_VEL (nullptr when no mesh has a zone)
+    const float2* vzone;
+    // count of vertex solves whose 3x3 block could not be solved (gradient step or no move)
+    int* solveFail;
 };
 
 // vflag bits: 1 fixed, (code << 1) initialisation type, 16 accelerated (tet vertex of a scene
-// with use_accelerator)
+// with use_accelerator), 32 / 64 no-gravity zone on the position's / the velocity's y (tet / cloth
+// meshes with hasNoGravZone and maxVelocityMagnitude > 0; SceneView::vzone holds the constants)
 constexpr uint8_t VFLAG_FIXED = 1;
 constexpr uint8_t VFLAG_ACCEL = 16;
+constexpr uint8_t VFLAG_NOGRAV_POS = 32;
+constexpr uint8_t VFLAG_NOGRAV_VEL = 64;
 
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`
 // (tet slots or cloth slots by `kind`; vertex ranges for the per-vertex kernels).

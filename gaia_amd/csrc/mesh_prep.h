@@ -22,6 +22,10 @@ struct TetMaterial {
     float initRatioG = 0.5f;
     float frictionDynamic = 0.1f;  // contact friction of the boundary surface (scenes with collision handling)
     float frictionEpsV = 0.1f;
+    // no-gravity zone (Materials.h:34-35): position y above the threshold -> speed capped, no damping
+    int32_t hasNoGravZone = 0;
+    float noGravZoneThreshold = 0.f;
+    float maxVelocityMagnitude = -1.f;
 };
 
 struct TetMeshHost {
@@ -83,6 +87,10 @@ struct ClothMaterial {
     int32_t initializationType = 2;     // VBDObjectParamsTriMesh default: previous position
     float frictionDynamic = 0.1f;       // TriMesh.h:41-42 (contact friction)
     float frictionEpsV = 0.1f;
+    // no-gravity zone (Materials.h:34-35): velocity y above the threshold -> speed capped, no damping
+    int32_t hasNoGravZone = 0;
+    float noGravZoneThreshold = 0.f;
+    float maxVelocityMagnitude = -1.f;
 };
 
 // Edge of a triangle mesh (TriMesh/TriMesh.h:51-60). Edges are numbered in order of first

@@ -304,6 +304,9 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             get_int(mj, "initializationType", m.mat.initializationType);
             get_num(mj, "frictionDynamic", m.mat.frictionDynamic);  // boundary-surface contact friction
             get_num(mj, "frictionEpsV", m.mat.frictionEpsV);
+            get_int(mj, "hasNoGravZone", m.mat.hasNoGravZone);  // Materials.h:62-64
+            get_num(mj, "noGravZoneThreshold", m.mat.noGravZoneThreshold);
+            get_num(mj, "maxVelocityMagnitude", m.mat.maxVelocityMagnitude);
             if (m.mat.initializationType < 0 || m.mat.initializationType > 4) {
                 err = "initializationType must be 0..4 for tet meshes in this build";
                 return false;
@@ -341,6 +344,9 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             get_int(mj, "initializationType", m.mat.initializationType);
             get_num(mj, "frictionDynamic", m.mat.frictionDynamic);  // TriMesh.cpp:459-460
             get_num(mj, "frictionEpsV", m.mat.frictionEpsV);
+            get_int(mj, "hasNoGravZone", m.mat.hasNoGravZone);  // Materials.h:62-64
+            get_num(mj, "noGravZoneThreshold", m.mat.noGravZoneThreshold);
+            get_num(mj, "maxVelocityMagnitude", m.mat.maxVelocityMagnitude);
             const int t = m.mat.initializationType;
             if (!(t == 0 || t == 2 || t == 3 || t == 4)) {
                 err = "cloth initializationType must be 0, 2, 3 or 4 in this build";

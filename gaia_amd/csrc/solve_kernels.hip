@@ -264,8 +264,10 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         f3 dx;
         float step = sp.stepSize;
         if (!solve_sym3(h, f, dx)) {
+            // the 3x3 block is singular: gradient step (VBDPhysics.cpp:2449-2466), counted
             step = sp.stepSizeGD;
             dx = f;
+            atomicAdd(S.solveFail, 1);
         }
         xn = x + step * dx;
     }
@@ -702,7 +704,9 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     }
     f3 dx;
     f3 xn = x;
-    if (solve_sym3(h, f, dx)) {
+    const bool solved = solve_sym3(h, f, dx);
+    if (!solved) atomicAdd(S.solveFail, 1);  // singular block: the vertex keeps its position, counted
+    if (solved) {
         xn = x + sp.clothStepSize * dx;
         if constexpr (CONTACT) {
             // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246); any truncation
@@ -1003,7 +1007,18 @@ __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* _
     const float4 q = S.prev[v];
     f3 vel = mk((p.x - q.x) / dt, (p.y - q.y) / dt, (p.z - q.z) / dt);
     const float vMag = sqrtf(dot(vel, vel));
-    if (vMag > 1e-6f) {
+    const uint8_t flags = S.vflag[v];
+    // no-gravity zone (VBDPhysics.cpp:2534-2542 on the position's y, VBDClothPhysics.cpp:2330-2338
+    // on the velocity's y): the speed is capped at maxVelocityMagnitude, no damping
+    bool zone = false;
+    float2 z = make_float2(0.f, 0.f);
+    if (flags & (VFLAG_NOGRAV_POS | VFLAG_NOGRAV_VEL)) {
+        z = S.vzone[v];
+        zone = ((flags & VFLAG_NOGRAV_POS) ? p.y : vel.y) > z.x;
+    }
+    if (zone) {
+        if (vMag > z.y) vel = (z.y / vMag) * vel;
+    } else if (vMag > 1e-6f) {
         const float4 vp = S.vparam[v];
         float vNew = vMag * vp.x - vp.y;
         vNew = vNew > 1e-6f ? vNew : 0.f;
@@ -1011,7 +1026,7 @@ __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* _
     } else {
         vel = 0.f * vel;
     }
-    if (S.vflag[v] & 1) vel = mk(0.f, 0.f, 0.f);
+    if (flags & VFLAG_FIXED) vel = mk(0.f, 0.f, 0.f);
     S.vel[v] = make_float4(vel.x, vel.y, vel.z, 0.f);
 }
 

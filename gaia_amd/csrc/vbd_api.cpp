@@ -31,6 +31,7 @@ namespace {
 thread_local std::string g_lastError;
 
 constexpr int kVertexChunk = 256;
+constexpr int kSolveFailSlot = 16;  // int index of the solve-failure counter in the clock block
 
 // All device arrays of one scene live in one allocation.
 struct DeviceScene {
@@ -54,6 +55,7 @@ struct DeviceScene {
     float4 *accPrevPrev = nullptr, *accPrevIter = nullptr;  // use_accelerator: iteration history
     float* accOmega = nullptr;                              // use_accelerator: omega per iteration
     float* clock = nullptr;                                 // scene time
+    float2* vzone = nullptr;                                // no-gravity zone constants per vertex
     DeformVert* dverts = nullptr;                           // deformed vertices (deformers)
     DeformParams* dparams = nullptr;
     int* attBegin = nullptr;                                // cloth attachments: per vertex CSR
@@ -568,13 +570,15 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     std::vector<float4> pos(nv), inertia(nv), vel(nv), vparam(nv);
     std::vector<uint8_t> vflag(nv, 0);
     std::vector<float2> friction(contact ? nv : 0);
+    std::vector<float2> vzone;  // (noGravZoneThreshold, maxVelocityMagnitude), when a mesh has a zone
     for (size_t im = 0; im < s.meshes.size(); ++im) {
         const int32_t off = s.vertexOffset[im];
         const std::vector<float>* mp;
         const std::vector<float>* mv;
         const std::vector<float>* mm;
         float4 vp;
-        uint8_t code;
+        uint8_t code, zoneFlag;
+        float2 zone;
         if (s.meshes[im].first == MESH_TET) {
             const TetMeshHost& m = s.tets[s.meshes[im].second];
             mp = &m.pos;
@@ -584,6 +588,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             vp = make_float4(std::pow(m.mat.exponentialVelDamping, s.sp.dt), m.mat.constantVelDamping * s.sp.dt,
                              m.mat.dampingHydrostatic, m.mat.dampingDeviatoric);
             code = (uint8_t)(m.mat.initializationType & 7);
+            zoneFlag = (m.mat.hasNoGravZone && m.mat.maxVelocityMagnitude > 0.f) ? VFLAG_NOGRAV_POS : 0;
+            zone = make_float2(m.mat.noGravZoneThreshold, m.mat.maxVelocityMagnitude);
             if (contact)
                 for (int32_t i = 0; i < m.nv; ++i)
                     friction[s.vertexOffset[im] + i] = make_float2(m.mat.frictionDynamic, m.mat.frictionEpsV);
@@ -599,6 +605,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             // 3 inertia within the conservative bounds -> x + (y - x) truncated (code 5)
             const int t = m.mat.initializationType;
             code = (uint8_t)(t == 0 ? 0 : (t == 3 ? 5 : 1));
+            zoneFlag = (m.mat.hasNoGravZone && m.mat.maxVelocityMagnitude > 0.f) ? VFLAG_NOGRAV_VEL : 0;
+            zone = make_float2(m.mat.noGravZoneThreshold, m.mat.maxVelocityMagnitude);
             if (contact)
                 for (int32_t i = 0; i < m.nv; ++i)
                     friction[s.vertexOffset[im] + i] = make_float2(m.mat.frictionDynamic, m.mat.frictionEpsV);
@@ -614,6 +622,13 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         }
         for (int32_t i = 0; i < n; ++i)
             if (fixedMask[im][i]) vflag[off + i] |= VFLAG_FIXED;
+        if (zoneFlag) {
+            vzone.resize(nv, make_float2(0.f, 0.f));
+            for (int32_t i = 0; i < n; ++i) {
+                vflag[off + i] |= zoneFlag;
+                vzone[off + i] = zone;
+            }
+        }
         if (accel && s.meshes[im].first == MESH_TET)
             for (int32_t i = 0; i < n; ++i) vflag[off + i] |= VFLAG_ACCEL;
     }
@@ -679,12 +694,13 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     }
     const size_t bA = accel ? bV : 0, bW = accel ? align(sizeof(float) * omega.size()) : 0;
     const size_t bClk = 256;
+    const size_t bZ = vzone.empty() ? 0 : align(sizeof(float2) * vzone.size());
     const size_t bDV = dverts.empty() ? 0 : align(sizeof(DeformVert) * dverts.size());
     const size_t bDP = dparams.empty() ? 0 : align(sizeof(DeformParams) * dparams.size());
     const size_t bAB = attBegin.empty() ? 0 : align(sizeof(int32_t) * attBegin.size());
     const size_t bAR = att.empty() ? 0 : align(sizeof(AttachRec) * att.size());
     const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW +
-                         bClk + bDV + bDP + bAB + bAR + bSn + bVG;
+                         bClk + bZ + bDV + bDP + bAB + bAR + bSn + bVG;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -729,6 +745,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         d.accOmega = (float*)take(bW);
     }
     d.clock = (float*)take(bClk);
+    if (bZ) d.vzone = (float2*)take(bZ);
     if (bDV) {
         d.dverts = (DeformVert*)take(bDV);
         d.dparams = (DeformParams*)take(bDP);
@@ -767,7 +784,11 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
         up(d.accPrevIter, pos.data(), sizeof(float4) * nv);
         up(d.accOmega, omega.data(), sizeof(float) * omega.size());
     }
-    up(d.clock, &s.time0, sizeof(float));
+    // scene clock block: the time at [0], the solve-failure counter at int [16]
+    float clockBlock[64] = {};
+    clockBlock[0] = s.time0;
+    up(d.clock, clockBlock, sizeof(clockBlock));
+    if (bZ) up(d.vzone, vzone.data(), sizeof(float2) * vzone.size());
     if (bDV) {
         up(d.dverts, dverts.data(), sizeof(DeformVert) * dverts.size());
         up(d.dparams, dparams.data(), sizeof(DeformParams) * dparams.size());
@@ -799,6 +820,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     v.dparams = d.dparams;
     v.attBegin = d.attBegin;
     v.att = d.att;
+    v.vzone = d.vzone;
+    v.solveFail = reinterpret_cast<int*>(d.clock) + kSolveFailSlot;
     v.nv = nv;
     v.sp = s.sp;
 }
@@ -1114,6 +1137,13 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
     const bool contact = s.params.handle_collision != 0;
     if (contact && !(s.params.contact_query_radius > 0.f))
         fail(GVBD_ERR_INVALID, "contact_query_radius must be positive");
+    if (contact)
+        for (const DeformerHost& D : s.deformers)
+            for (int32_t m : D.meshes)
+                if (s.meshes[m].first == MESH_TET)
+                    fail(GVBD_ERR_INVALID,
+                         "deformers of tet meshes are not supported with collision handling: they move vertices "
+                         "after the frame's first detection, so contact bounds would not see the motion");
     if (s.params.use_accelerator) {
         if (contact)
             fail(GVBD_ERR_INVALID, "use_accelerator applies to tet scenes without collision handling (the "
@@ -1285,7 +1315,8 @@ int32_t add_cloth_mesh_host(gvbd_context* ctx, int32_t scene, ClothMeshHost&& m)
 
 using namespace gvbd;
 
-extern "C" {
+// The gvbd_* entry points below keep the C linkage of their declarations in gaia_vbd.h; the
+// helpers between them are internal C++.
 
 int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
                           int32_t* cloth_mesh_desc_bytes, int32_t* schedule_bytes) {
@@ -1453,6 +1484,9 @@ int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mes
         m.mat.initRatioG = desc->init_ratio_g;
         m.mat.frictionDynamic = desc->friction_dynamic;
         m.mat.frictionEpsV = desc->friction_eps_v;
+        m.mat.hasNoGravZone = desc->has_no_grav_zone;
+        m.mat.noGravZoneThreshold = desc->no_grav_zone_threshold;
+        m.mat.maxVelocityMagnitude = desc->max_velocity_magnitude;
         m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
         for (int32_t t : m.tets) require(t >= 0 && t < m.nv, "tet vertex index out of range");
         Graph g = tet_vertex_graph(m.nv, m.nt, m.tets.data());
@@ -1500,6 +1534,9 @@ int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth
         m.mat.initializationType = t;
         m.mat.frictionDynamic = desc->friction_dynamic;
         m.mat.frictionEpsV = desc->friction_eps_v;
+        m.mat.hasNoGravZone = desc->has_no_grav_zone;
+        m.mat.noGravZoneThreshold = desc->no_grav_zone_threshold;
+        m.mat.maxVelocityMagnitude = desc->max_velocity_magnitude;
         m.fixed.assign(desc->fixed_points, desc->fixed_points + desc->num_fixed);
         for (int32_t v : m.faces) require(v >= 0 && v < m.nv, "face vertex index out of range");
         if (desc->vertex_colors) {
@@ -1579,6 +1616,18 @@ int gvbd_scene_add_deformer(gvbd_context* ctx, int32_t scene, const gvbd_deforme
             for (int32_t i = 0; i < desc->num_vertices[k]; ++i) {
                 const int32_t v = desc->vertices[at++];
                 require(v >= 0 && v < s.meshVertices(m), "deformer vertex index out of range");
+                // the device moves every deformed vertex of a phase in one launch: a vertex selected
+                // twice would be written by two threads (the reference applies deformers in list
+                // order), so each vertex may belong to one deformer only
+                const bool taken = std::any_of(s.deformers.begin(), s.deformers.end(), [&](const DeformerHost& o) {
+                    for (size_t j = 0; j < o.meshes.size(); ++j)
+                        if (o.meshes[j] == m && std::find(o.verts[j].begin(), o.verts[j].end(), v) != o.verts[j].end())
+                            return true;
+                    return false;
+                });
+                if (taken || std::find(D.verts.back().begin(), D.verts.back().end(), v) != D.verts.back().end())
+                    fail(GVBD_ERR_INVALID, "vertex " + std::to_string(v) + " of mesh " + std::to_string(m) +
+                                               " is already selected by a deformer");
                 D.verts.back().push_back(v);
                 // Rotator: root = ((P - o) . axis) axis, RootP = P - root; Translator: P
                 // (VBDDeformer.h:52-67, 139-148), in float
@@ -2049,6 +2098,26 @@ static void mesh_state_io(gvbd_context* ctx, int32_t scene, int32_t mesh, float*
     }
 }
 
+int gvbd_scene_solve_failures(gvbd_context* ctx, int32_t scene, int32_t reset, int64_t* count) {
+    return guard(ctx, [&] {
+        require(ctx && count, "NULL argument");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        int32_t n = 0;
+        int* dev = s.view.solveFail;
+        if (s.dev.arena) {
+            hip_check(hipMemcpy(&n, dev, sizeof(n), hipMemcpyDeviceToHost), "download");
+            if (reset) hip_check(hipMemset(dev, 0, sizeof(n)), "counter reset");
+        } else {  // held in host memory by its server
+            char* held = s.hostCopy.data() + ((uintptr_t)dev - s.hostCopyBase);
+            std::memcpy(&n, held, sizeof(n));
+            if (reset) std::memset(held, 0, sizeof(n));
+        }
+        *count = n;
+    });
+}
+
 int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, float* positions, float* velocities) {
     return guard(ctx, [&] { mesh_state_io(ctx, scene, mesh, positions, velocities, true); });
 }
@@ -2350,9 +2419,10 @@ int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t gr
         PlanObj& p = batch_plan(ctx, batch);
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
-        const int b = p.groupItemBegin[group], e = p.groupItemBegin[group + 1];
-        hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iteration, p.lpv, p.block, p.stage, p.contact, ctx->stream),
-                  "group solve launch");
+        // as the single-scene unit: contact scenes read same-group partners from the snapshot, and
+        // cloth slots solved through positionsNext are committed after the group
+        if (p.contact) plan_contact_snapshot(ctx, p);
+        plan_group(ctx, p, group, iteration);
     });
 }
 
@@ -2415,11 +2485,12 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
     mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact), mv(d.accPrevPrev), mv(d.accPrevIter), mv(d.accOmega);
     mv(d.snap), mv(d.vgroup);
-    mv(d.clock), mv(d.dverts), mv(d.dparams), mv(d.attBegin), mv(d.att);
+    mv(d.clock), mv(d.vzone), mv(d.dverts), mv(d.dparams), mv(d.attBegin), mv(d.att);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
     mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact), mv(v.accPrevPrev), mv(v.accPrevIter);
-    mv(v.accOmega), mv(v.clock), mv(v.dverts), mv(v.dparams), mv(v.attBegin), mv(v.att);
+    mv(v.accOmega), mv(v.clock), mv(v.vzone), mv(v.dverts), mv(v.dparams), mv(v.attBegin), mv(v.att);
+    mv(v.solveFail);
 }
 
 // Copies an idle scene's arena out to host memory and frees it.
@@ -3048,4 +3119,3 @@ int gvbd_color_tri_mesh(int32_t num_vertices, int32_t num_faces, const int32_t* 
     });
 }
 
-}  // extern "C"

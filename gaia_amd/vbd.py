@@ -64,6 +64,11 @@ DEFAULT_CLOTH_MATERIAL = dict(
     initialization_type=2,
     friction_dynamic=0.1,
     friction_eps_v=0.1,
+    # no-gravity zone (Materials/Materials.h:34-35): speed capped instead of damped while the
+    # velocity's y exceeds the threshold (needs max_velocity_magnitude > 0)
+    has_no_grav_zone=0,
+    no_grav_zone_threshold=0.0,
+    max_velocity_magnitude=-1.0,
 )
 
 # ObjectParametersVBDNeoHookean defaults (VBD/VBD_NeoHookean.h:25-29, VBD/VBD_BaseMaterial.h:29-45,
@@ -81,6 +86,11 @@ DEFAULT_TET_MATERIAL = dict(
     # boundary-surface contact friction in scenes with collision handling (as TriMesh.h:41-42)
     friction_dynamic=0.1,
     friction_eps_v=0.1,
+    # no-gravity zone (Materials/Materials.h:34-35): speed capped instead of damped while the
+    # position's y exceeds the threshold (needs max_velocity_magnitude > 0)
+    has_no_grav_zone=0,
+    no_grav_zone_threshold=0.0,
+    max_velocity_magnitude=-1.0,
 )
 
 
@@ -413,9 +423,11 @@ class VBDPhysics:
         d.num_tets = tets.size // 4
         d.tets = iptr(tets)
         for k in ("density", "mu", "lmbd", "damping_hydrostatic", "damping_deviatoric", "exponential_vel_damping",
-                  "constant_vel_damping", "init_ratio_g", "friction_dynamic", "friction_eps_v"):
+                  "constant_vel_damping", "init_ratio_g", "friction_dynamic", "friction_eps_v",
+                  "no_grav_zone_threshold", "max_velocity_magnitude"):
             setattr(d, k, float(mat[k]))
         d.initialization_type = int(mat["initialization_type"])
+        d.has_no_grav_zone = int(bool(mat["has_no_grav_zone"]))
         d.num_fixed = fixed.size
         d.fixed_points = iptr(fixed) if fixed.size else None
         if colors is not None:
@@ -449,9 +461,11 @@ class VBDPhysics:
         d.num_faces = faces.size // 3
         d.faces = iptr(faces)
         for k in ("density", "mu", "lmbd", "bending_stiffness", "damping_stvk", "damping_bending",
-                  "exponential_vel_damping", "constant_vel_damping", "friction_dynamic", "friction_eps_v"):
+                  "exponential_vel_damping", "constant_vel_damping", "friction_dynamic", "friction_eps_v",
+                  "no_grav_zone_threshold", "max_velocity_magnitude"):
             setattr(d, k, float(mat[k]))
         d.initialization_type = int(mat["initialization_type"])
+        d.has_no_grav_zone = int(bool(mat["has_no_grav_zone"]))
         d.num_fixed = fixed.size
         d.fixed_points = iptr(fixed) if fixed.size else None
         if colors is not None:
@@ -491,6 +505,14 @@ class VBDPhysics:
         st = ContactStats()
         self.ctx.check(lib().gvbd_scene_contact_stats(self.ctx.handle, self.scene, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in ContactStats._fields_}
+
+    def solveFailures(self, reset=False):
+        """Vertex solves whose 3x3 block had no solution (gradient step for tet vertices, no move for
+        cloth vertices) since initialisation or the last reset."""
+        n = ctypes.c_int64()
+        self.ctx.check(lib().gvbd_scene_solve_failures(self.ctx.handle, self.scene, 1 if reset else 0,
+                                                       ctypes.byref(n)))
+        return n.value
 
     def evaluateConvergence(self):
         """VBDPhysics::evaluateConvergenceGPU: merit energy, its inertia and elastic parts, and the

@@ -47,6 +47,13 @@
 
 #include <stdint.h>
 
+/* The library is built with hidden symbol visibility; only the entry points below are exported. */
+#if defined(__GNUC__) || defined(__clang__)
+#define GVBD_API __attribute__((visibility("default")))
+#else
+#define GVBD_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -126,6 +133,13 @@ typedef struct gvbd_tet_mesh_desc {
     int32_t coloring_algorithm;    /* GVBD_COLOR_MCS / GVBD_COLOR_GREEDY when vertex_colors is NULL */
     float friction_dynamic;        /* contact friction of the boundary surface (collision handling; 0.1) */
     float friction_eps_v;          /* (0.1) */
+    /* no-gravity zone (ObjectParams::hasNoGravZone / noGravZoneThreshold / maxVelocityMagnitude,
+     * Materials/Materials.h:34-35; dampVelocities, VBD/VBDPhysics.cpp:2529-2555): a vertex whose
+     * position y exceeds the threshold has its speed capped at max_velocity_magnitude and no
+     * damping; the zone needs max_velocity_magnitude > 0 */
+    int32_t has_no_grav_zone;
+    float no_grav_zone_threshold;
+    float max_velocity_magnitude;  /* <= 0: none (-1) */
 } gvbd_tet_mesh_desc;
 
 /* One cloth (triangle) mesh with StVK membrane + quadratic bending (VBDObjectParamsTriMeshStVk,
@@ -154,6 +168,12 @@ typedef struct gvbd_cloth_mesh_desc {
     int32_t coloring_algorithm;
     float friction_dynamic;        /* frictionDynamic (0.1): contact friction, averaged over the two meshes */
     float friction_eps_v;          /* frictionEpsV (0.1) */
+    /* no-gravity zone (Materials/Materials.h:34-35; updateVelocity, VBDCloth/VBDClothPhysics.cpp:2325-2351):
+     * as for tet meshes, except that the reference compares the vertex's velocity y (not its
+     * position) with the threshold */
+    int32_t has_no_grav_zone;
+    float no_grav_zone_threshold;
+    float max_velocity_magnitude;  /* <= 0: none (-1) */
 } gvbd_cloth_mesh_desc;
 
 #define GVBD_MODE_GROUP_LAUNCHES 0 /* one kernel launch per parallel group (the reference's structure) */
@@ -175,29 +195,29 @@ typedef struct gvbd_schedule {
 } gvbd_schedule;
 
 /* sizes of the ABI structs as compiled into the library (binding self-check) */
-int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
+GVBD_API int gvbd_abi_struct_sizes(int32_t* physics_params_bytes, int32_t* tet_mesh_desc_bytes,
                           int32_t* cloth_mesh_desc_bytes, int32_t* schedule_bytes);
 
 /* ---- context ---- */
-int gvbd_context_create(int32_t device, gvbd_context** out);
-int gvbd_context_destroy(gvbd_context* ctx);
-const char* gvbd_last_error(const gvbd_context* ctx);
-int gvbd_synchronize(gvbd_context* ctx);
+GVBD_API int gvbd_context_create(int32_t device, gvbd_context** out);
+GVBD_API int gvbd_context_destroy(gvbd_context* ctx);
+GVBD_API const char* gvbd_last_error(const gvbd_context* ctx);
+GVBD_API int gvbd_synchronize(gvbd_context* ctx);
 /* device bytes currently held by scenes and plans of this context */
-int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
+GVBD_API int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
 
 /* ---- scenes ---- */
-int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
-int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene);
+GVBD_API int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
+GVBD_API int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene);
 /* a new, not yet initialised scene with the parameters and meshes of `scene` (positions and
  * velocities as they were added): the host-side mesh preparation is copied, not redone. No
  * reference counterpart; it instances one mesh many times (served populations). */
-int gvbd_scene_clone(gvbd_context* ctx, int32_t scene, int32_t* scene_out);
-int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mesh_desc* desc, int32_t* mesh_out);
+GVBD_API int gvbd_scene_clone(gvbd_context* ctx, int32_t scene, int32_t* scene_out);
+GVBD_API int gvbd_scene_add_tet_mesh(gvbd_context* ctx, int32_t scene, const gvbd_tet_mesh_desc* desc, int32_t* mesh_out);
 /* VBDClothSimulationFramework mesh load + TriMeshFEM::initialize (Framework/BaseClothSimPhsicsFramework.cpp,
  * TriMesh/TriMesh.cpp:15-111) */
-int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth_mesh_desc* desc, int32_t* mesh_out);
-int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene);
+GVBD_API int gvbd_scene_add_cloth_mesh(gvbd_context* ctx, int32_t scene, const gvbd_cloth_mesh_desc* desc, int32_t* mesh_out);
+GVBD_API int gvbd_scene_initialize(gvbd_context* ctx, int32_t scene);
 
 /* Deformers: prescribed motion of selected vertices, which join the mesh's fixed points
  * (VBDBaseDeformer / VBDClothBaseDeformer; loadDeformers, VBD/VBDDeformer.h:181-222 and
@@ -229,7 +249,7 @@ typedef struct gvbd_deformer_desc {
     const int32_t* num_vertices;  /* per selected mesh */
     const int32_t* vertices;      /* selectedVertices, concatenated */
 } gvbd_deformer_desc;
-int gvbd_scene_add_deformer(gvbd_context* ctx, int32_t scene, const gvbd_deformer_desc* desc);
+GVBD_API int gvbd_scene_add_deformer(gvbd_context* ctx, int32_t scene, const gvbd_deformer_desc* desc);
 
 /* Vertex-face attachments of cloth vertices (VertexFaceAttachmentConstraint,
  * VBDCloth/VBDTriMeshConstraints.h:31-49; loadConstraints, VBDCloth/VBDClothPhysics.cpp:148-224,
@@ -247,44 +267,44 @@ typedef struct gvbd_attachment_desc {
     float bary[3];         /* vertexConstraintBarys */
     float stiffness;       /* vertexConstraintStiffness (1e4 when the file gives none) */
 } gvbd_attachment_desc;
-int gvbd_scene_add_attachments(gvbd_context* ctx, int32_t scene, const gvbd_attachment_desc* list, int32_t n);
+GVBD_API int gvbd_scene_add_attachments(gvbd_context* ctx, int32_t scene, const gvbd_attachment_desc* list, int32_t n);
 
 /* Scene time (BasePhysicFramework::curTime, Framework/BasePhysicsFramework.h:100): 0 at
  * creation, + dt per substep; gvbd_scene_load_state sets it from the recovery state. */
-int gvbd_scene_get_time(gvbd_context* ctx, int32_t scene, float* cur_time);
-int gvbd_scene_set_time(gvbd_context* ctx, int32_t scene, float cur_time);
+GVBD_API int gvbd_scene_get_time(gvbd_context* ctx, int32_t scene, float* cur_time);
+GVBD_API int gvbd_scene_set_time(gvbd_context* ctx, int32_t scene, float cur_time);
 /* Models.json + Parameters.json; "${REPO_ROOT}" in paths is replaced by repo_root (may be NULL). */
-int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char* params_json,
+GVBD_API int gvbd_scene_load_json(gvbd_context* ctx, const char* models_json, const char* params_json,
                          const char* repo_root, int32_t* scene_out);
 
-int gvbd_scene_num_meshes(gvbd_context* ctx, int32_t scene, int32_t* n);
-int gvbd_scene_mesh_info(gvbd_context* ctx, int32_t scene, int32_t mesh, int32_t* num_vertices,
+GVBD_API int gvbd_scene_num_meshes(gvbd_context* ctx, int32_t scene, int32_t* n);
+GVBD_API int gvbd_scene_mesh_info(gvbd_context* ctx, int32_t scene, int32_t mesh, int32_t* num_vertices,
                          int32_t* num_tets);
-int gvbd_scene_num_parallel_groups(gvbd_context* ctx, int32_t scene, int32_t* n);
+GVBD_API int gvbd_scene_num_parallel_groups(gvbd_context* ctx, int32_t scene, int32_t* n);
 /* interleaved (mesh, vertex) pairs; pairs may be NULL to query the count */
-int gvbd_scene_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t* pairs, int32_t* count);
+GVBD_API int gvbd_scene_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t* pairs, int32_t* count);
 /* rest-pose constants of one mesh: dminv 9*T (column-major), volume T, mass N (any may be NULL) */
-int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, float* dminv, float* volume,
+GVBD_API int gvbd_scene_mesh_constants(gvbd_context* ctx, int32_t scene, int32_t mesh, float* dminv, float* volume,
                               float* mass);
 /* lanes per vertex (1,2,4,8,16) and block size (64,128,256) of the vertex solve */
-int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_vertex, int32_t block);
+GVBD_API int gvbd_scene_set_schedule(gvbd_context* ctx, int32_t scene, int32_t lanes_per_vertex, int32_t block);
 /* full schedule (mode, work-queue entry size, persistent grid); drops a recorded graph */
-int gvbd_scene_set_schedule_ex(gvbd_context* ctx, int32_t scene, const gvbd_schedule* schedule);
-int gvbd_scene_get_schedule(gvbd_context* ctx, int32_t scene, gvbd_schedule* schedule);
+GVBD_API int gvbd_scene_set_schedule_ex(gvbd_context* ctx, int32_t scene, const gvbd_schedule* schedule);
+GVBD_API int gvbd_scene_get_schedule(gvbd_context* ctx, int32_t scene, gvbd_schedule* schedule);
 
 /* ---- stepping (asynchronous on the context stream) ---- */
-int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene);               /* all substeps of one frame */
-int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n);
-int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene);
-int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t iteration);
+GVBD_API int gvbd_scene_run_step(gvbd_context* ctx, int32_t scene);               /* all substeps of one frame */
+GVBD_API int gvbd_scene_run_steps(gvbd_context* ctx, int32_t scene, int32_t n);
+GVBD_API int gvbd_scene_apply_initial_step(gvbd_context* ctx, int32_t scene);
+GVBD_API int gvbd_scene_solve_parallel_group(gvbd_context* ctx, int32_t scene, int32_t group, int32_t iteration);
 /* one VBD iteration: every parallel group in order, in the scene's schedule mode
  * (the iteration body of VBDPhysics::runStepGPU / runStep, VBD/VBDPhysics.cpp:879-945, 1605-1627) */
-int gvbd_scene_run_iteration(gvbd_context* ctx, int32_t scene, int32_t iteration);
-int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
+GVBD_API int gvbd_scene_run_iteration(gvbd_context* ctx, int32_t scene, int32_t iteration);
+GVBD_API int gvbd_scene_update_velocities(gvbd_context* ctx, int32_t scene);
 /* cloth scenes with handle_collision: VBDClothSimulationFramework::applyCollisionDetection
  * (VBDClothPhysics.cpp:735-819) at the current positions; run_step calls it itself before
  * iteration 0 and after any iteration in which a vertex was truncated to its bound */
-int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene);
+GVBD_API int gvbd_scene_collision_detection(gvbd_context* ctx, int32_t scene);
 /* counters of a scene with handle_collision since its initialisation (no reference counterpart:
  * the reference prints timings instead, VBDClothPhysics.cpp:389-518). detect_ms is device time of
  * the detection work (BVH update, queries, bounds, face links), timed with events around it. */
@@ -297,22 +317,27 @@ typedef struct gvbd_contact_stats {
     int64_t vf_pairs;        /* after the last detection: vertex-face candidates within the query radius */
     int64_t ee_pairs;        /* and edge-edge candidates */
 } gvbd_contact_stats;
-int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out);
+GVBD_API int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out);
+/* Vertex solves since initialisation (or the last reset) whose 3x3 block had no solution: tet
+ * vertices then take the gradient step (VBD/VBDPhysics.cpp:2449-2466), cloth vertices keep their
+ * position (VBDCloth/VBDClothPhysics.cpp:1126-1150). The reference counts none; this is a diagnostic.
+ * Synchronises the context's stream; reset != 0 clears the counter. */
+GVBD_API int gvbd_scene_solve_failures(gvbd_context* ctx, int32_t scene, int32_t reset, int64_t* count);
 /* VBDPhysics::evaluateConvergenceGPU (VBD/VBDPhysics.cpp:3510-3556): at the current positions and
  * inertia targets, the merit energy (inertia 1/(2 dt^2) sum m |y - x|^2 + stable neo-Hookean
  * elastic energy), its two parts, and the average over the tet vertices of |material + damping +
  * inertia force| (fixed vertices count with zero force). Synchronous; any NULL output is skipped.
  * Between frames it measures how far the last frame's iterate is from the implicit-Euler
  * solution; with the per-iteration hooks it reproduces the reference's per-iteration log. */
-int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
+GVBD_API int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
                                     double* elastic_energy, double* avg_force_norm);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
-int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);
+GVBD_API int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);
 
 /* ---- state I/O (synchronous) ---- */
-int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, float* positions, float* velocities);
-int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, const float* positions,
+GVBD_API int gvbd_scene_get_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, float* positions, float* velocities);
+GVBD_API int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, const float* positions,
                               const float* velocities);
 
 /* ---- scene files (synchronous) ----
@@ -322,49 +347,49 @@ int gvbd_scene_set_mesh_state(gvbd_context* ctx, int32_t scene, int32_t mesh, co
  * scene in addition order; binary = 1 writes "<stem>.pos" / "<stem>.vel" (float32 xyz) beside
  * the JSON. Loading sets the meshes the file holds (a shorter file leaves the rest as they are,
  * as the reference does); a mesh stored without velocities keeps its current ones. */
-int gvbd_scene_save_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t frame_id, double cur_time,
+GVBD_API int gvbd_scene_save_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t frame_id, double cur_time,
                           int32_t binary);
-int gvbd_scene_load_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t* frame_id,
+GVBD_API int gvbd_scene_load_state(gvbd_context* ctx, int32_t scene, const char* json_path, int32_t* frame_id,
                           double* cur_time);
 /* Frame outputs: the surface of every mesh (a tet mesh's boundary vertices, ascending, and its
  * boundary faces; every vertex and face of a cloth mesh) as ASCII PLY (GAIA::saveAsPly /
  * writeAllToPLY, IO/FileIO.cpp:8-50,119-254): all meshes in one file (all_in_one = 1), or one file
  * per mesh named "<dir>/<stem>_Model_<6-digit mesh index><ext>"; and the float32 surface
  * positions of all meshes back to back (writeAllToBinary, IO/FileIO.cpp:52-117). */
-int gvbd_scene_write_ply(gvbd_context* ctx, int32_t scene, const char* path, int32_t all_in_one);
-int gvbd_scene_write_binary_frame(gvbd_context* ctx, int32_t scene, const char* path);
+GVBD_API int gvbd_scene_write_ply(gvbd_context* ctx, int32_t scene, const char* path, int32_t all_in_one);
+GVBD_API int gvbd_scene_write_binary_frame(gvbd_context* ctx, int32_t scene, const char* path);
 /* The same file formats from host arrays (no device): num_meshes meshes of num_vertices[m]
  * vertices, positions / velocities xyz back to back. gvbd_state_read fills num_meshes and
  * num_vertices (capacity max_meshes) and, when positions is not NULL, positions and velocities
  * (capacity max_floats floats each; a mesh stored without velocities reads as zeros). */
-int gvbd_state_write(const char* json_path, int32_t frame_id, double cur_time, int32_t binary, int32_t num_meshes,
+GVBD_API int gvbd_state_write(const char* json_path, int32_t frame_id, double cur_time, int32_t binary, int32_t num_meshes,
                      const int32_t* num_vertices, const float* positions, const float* velocities);
-int gvbd_state_read(const char* json_path, int32_t max_meshes, int32_t* num_meshes, int32_t* num_vertices,
+GVBD_API int gvbd_state_read(const char* json_path, int32_t max_meshes, int32_t* num_meshes, int32_t* num_vertices,
                     int64_t max_floats, float* positions, float* velocities, int32_t* frame_id, double* cur_time);
-int gvbd_write_ply(const char* path, int32_t num_vertices, const float* positions, int32_t num_faces,
+GVBD_API int gvbd_write_ply(const char* path, int32_t num_vertices, const float* positions, int32_t num_faces,
                    const int32_t* faces);
 
 /* ---- batched stepping: many scenes per launch ---- */
-int gvbd_batch_create(gvbd_context* ctx, const int32_t* scenes, int32_t n, int32_t* batch_out);
-int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch);
-int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_vertex, int32_t block);
-int gvbd_batch_set_schedule_ex(gvbd_context* ctx, int32_t batch, const gvbd_schedule* schedule);
-int gvbd_batch_get_schedule(gvbd_context* ctx, int32_t batch, gvbd_schedule* schedule);
+GVBD_API int gvbd_batch_create(gvbd_context* ctx, const int32_t* scenes, int32_t n, int32_t* batch_out);
+GVBD_API int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch);
+GVBD_API int gvbd_batch_set_schedule(gvbd_context* ctx, int32_t batch, int32_t lanes_per_vertex, int32_t block);
+GVBD_API int gvbd_batch_set_schedule_ex(gvbd_context* ctx, int32_t batch, const gvbd_schedule* schedule);
+GVBD_API int gvbd_batch_get_schedule(gvbd_context* ctx, int32_t batch, gvbd_schedule* schedule);
 /* work-queue size of one iteration and the persistent grid (blocks) of the current schedule */
-int gvbd_batch_queue_info(gvbd_context* ctx, int32_t batch, int32_t* entries, int32_t* grid);
-int gvbd_batch_record_graph(gvbd_context* ctx, int32_t batch, int32_t enable);
-int gvbd_batch_run_steps(gvbd_context* ctx, int32_t batch, int32_t n);
+GVBD_API int gvbd_batch_queue_info(gvbd_context* ctx, int32_t batch, int32_t* entries, int32_t* grid);
+GVBD_API int gvbd_batch_record_graph(gvbd_context* ctx, int32_t batch, int32_t enable);
+GVBD_API int gvbd_batch_run_steps(gvbd_context* ctx, int32_t batch, int32_t n);
 /* per-iteration hooks over every scene of the batch (as the gvbd_scene_ forms) */
-int gvbd_batch_apply_initial_step(gvbd_context* ctx, int32_t batch);
-int gvbd_batch_run_iteration(gvbd_context* ctx, int32_t batch, int32_t iteration);
-int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch);
+GVBD_API int gvbd_batch_apply_initial_step(gvbd_context* ctx, int32_t batch);
+GVBD_API int gvbd_batch_run_iteration(gvbd_context* ctx, int32_t batch, int32_t iteration);
+GVBD_API int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch);
 /* launch only the vertex solve of parallel group `group` (timing of the dominant kernel) */
-int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
-int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);
+GVBD_API int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
+GVBD_API int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);
 /* algorithmic bytes of one solve launch of `group` and its vertex count (gather model: per tet
  * vertex 60 B + 96 B per incident tet, per cloth vertex 60 B + 56 B per face + 116 B per bending
  * stencil) */
-int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
+GVBD_API int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
 
 /* ---- serving loop: continuous batching of step requests ----
  * No reference counterpart (the reference steps one scene per process). Scenes registered with a
@@ -398,24 +423,24 @@ typedef struct gvbd_served_step {
 
 /* budget_bytes: device bytes the server's scenes may hold at once (clamped to the device's memory;
  * gvbd_server_get_stats reports the budget in force) */
-int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_out);
+GVBD_API int gvbd_server_create(gvbd_context* ctx, int64_t budget_bytes, int32_t* server_out);
 /* scenes stay alive and resident or not as they are; evicted scenes are copied back first */
-int gvbd_server_destroy(gvbd_context* ctx, int32_t server);
-int gvbd_server_set_schedule(gvbd_context* ctx, int32_t server, const gvbd_schedule* schedule);
+GVBD_API int gvbd_server_destroy(gvbd_context* ctx, int32_t server);
+GVBD_API int gvbd_server_set_schedule(gvbd_context* ctx, int32_t server, const gvbd_schedule* schedule);
 /* registers a created scene (meshes added; initialised or not) without collision handling; all
  * scenes of a server share iterations and substeps */
-int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene);
+GVBD_API int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene);
 /* hands a scene back to the caller (resident again); it must have no pending request */
-int gvbd_server_remove_scene(gvbd_context* ctx, int32_t server, int32_t scene);
+GVBD_API int gvbd_server_remove_scene(gvbd_context* ctx, int32_t server, int32_t scene);
 /* queues one frame of `scene`; requests of one scene are served in order, one per tick */
-int gvbd_server_request_step(gvbd_context* ctx, int32_t server, int32_t scene, int64_t tag);
+GVBD_API int gvbd_server_request_step(gvbd_context* ctx, int32_t server, int32_t scene, int64_t tag);
 /* one tick (see above); stats may be NULL */
-int gvbd_server_tick(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
+GVBD_API int gvbd_server_tick(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
 /* drains up to `capacity` completed requests in completion order */
-int gvbd_server_poll(gvbd_context* ctx, int32_t server, gvbd_served_step* out, int32_t capacity, int32_t* count);
-int gvbd_server_get_stats(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
+GVBD_API int gvbd_server_poll(gvbd_context* ctx, int32_t server, gvbd_served_step* out, int32_t capacity, int32_t* count);
+GVBD_API int gvbd_server_get_stats(gvbd_context* ctx, int32_t server, gvbd_server_stats* stats);
 /* whether a registered scene is resident in HBM (1) or held in host memory / not uploaded yet (0) */
-int gvbd_server_scene_resident(gvbd_context* ctx, int32_t server, int32_t scene, int32_t* resident);
+GVBD_API int gvbd_server_scene_resident(gvbd_context* ctx, int32_t server, int32_t scene, int32_t* resident);
 
 /* ---- contact detector (GPU LBVH) ----
  * Replaces struct ClothContactDetector (CollisionDetector/ClothContactDetector.h:133-182): Embree
@@ -439,74 +464,74 @@ typedef struct gvbd_ee_contact {
 /* ClothContactDetector(params) with params.maxQueryDis = max_query_distance; at most
  * vf_capacity (1..16) contacts are kept per vertex and ee_capacity per edge (the reference's
  * VF_CONTACT_PREALLOCATE / EE_CONTACT_PREALLOCATE = 8; more are counted as overflow) */
-int gvbd_cd_create(gvbd_context* ctx, float max_query_distance, int32_t vf_capacity, int32_t ee_capacity,
+GVBD_API int gvbd_cd_create(gvbd_context* ctx, float max_query_distance, int32_t vf_capacity, int32_t ee_capacity,
                    int32_t* cd_out);
-int gvbd_cd_destroy(gvbd_context* ctx, int32_t cd);
+GVBD_API int gvbd_cd_destroy(gvbd_context* ctx, int32_t cd);
 /* ClothContactDetector::initialize(meshes): one call per mesh, then finalize (uploads, builds) */
-int gvbd_cd_add_mesh(gvbd_context* ctx, int32_t cd, int32_t num_vertices, const float* positions, int32_t num_faces,
+GVBD_API int gvbd_cd_add_mesh(gvbd_context* ctx, int32_t cd, int32_t num_vertices, const float* positions, int32_t num_faces,
                      const int32_t* faces, int32_t* mesh_out);
-int gvbd_cd_finalize(gvbd_context* ctx, int32_t cd);
-int gvbd_cd_sizes(gvbd_context* ctx, int32_t cd, int32_t* num_vertices, int32_t* num_faces, int32_t* num_edges);
-int gvbd_cd_mesh_offsets(gvbd_context* ctx, int32_t cd, int32_t mesh, int32_t* vertex, int32_t* face, int32_t* edge);
+GVBD_API int gvbd_cd_finalize(gvbd_context* ctx, int32_t cd);
+GVBD_API int gvbd_cd_sizes(gvbd_context* ctx, int32_t cd, int32_t* num_vertices, int32_t* num_faces, int32_t* num_edges);
+GVBD_API int gvbd_cd_mesh_offsets(gvbd_context* ctx, int32_t cd, int32_t mesh, int32_t* vertex, int32_t* face, int32_t* edge);
 /* edges 4 per edge (eV1, eV2, eV12Next, eV21Next; -1 none), face_edges 3 per face (AB, BC, CA) */
-int gvbd_cd_topology(gvbd_context* ctx, int32_t cd, int32_t* edges, int32_t* face_edges);
-int gvbd_cd_set_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, const float* positions);
-int gvbd_cd_get_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, float* positions);
+GVBD_API int gvbd_cd_topology(gvbd_context* ctx, int32_t cd, int32_t* edges, int32_t* face_edges);
+GVBD_API int gvbd_cd_set_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, const float* positions);
+GVBD_API int gvbd_cd_get_positions(gvbd_context* ctx, int32_t cd, int32_t mesh, float* positions);
 /* workload generator (no reference counterpart): x = x_initial + step * drift[mesh] + sigma * N(0,1),
  * the noise drawn per (seed, step, vertex, axis); drift holds 3 floats per mesh (may be NULL) */
-int gvbd_cd_perturb(gvbd_context* ctx, int32_t cd, uint64_t seed, int32_t step, float sigma, const float* drift);
+GVBD_API int gvbd_cd_perturb(gvbd_context* ctx, int32_t cd, uint64_t seed, int32_t step, float sigma, const float* drift);
 /* ClothContactDetector::updateBVH: rebuild = 1 re-sorts and rebuilds (RTC_BUILD_QUALITY_LOW),
  * 0 refits the boxes (RTC_BUILD_QUALITY_REFIT) (ClothContactDetector.cpp:1137-1181) */
-int gvbd_cd_update_bvh(gvbd_context* ctx, int32_t cd, int32_t rebuild);
+GVBD_API int gvbd_cd_update_bvh(gvbd_context* ctx, int32_t cd, int32_t rebuild);
 /* contactQueryVF for every vertex and contactQueryEE for every edge, with resetFaceContactInfo
  * (VBDClothPhysics.cpp:735-757); record = 1 also keeps the uncapped lists (gvbd_cd_vf_pairs,
  * gvbd_cd_vf_contacts, gvbd_cd_ee_contacts) */
-int gvbd_cd_query(gvbd_context* ctx, int32_t cd, int32_t record);
+GVBD_API int gvbd_cd_query(gvbd_context* ctx, int32_t cd, int32_t record);
 /* conservative bounds of applyCollisionDetection (VBDClothPhysics.cpp:775-812); bounds may be NULL */
-int gvbd_cd_bounds(gvbd_context* ctx, int32_t cd, float thickness, float relaxation, float* bounds);
+GVBD_API int gvbd_cd_bounds(gvbd_context* ctx, int32_t cd, float thickness, float relaxation, float* bounds);
 /* counts[5]: VF candidate pairs, VF contacts, EE contacts, vertices and edges that overflowed */
-int gvbd_cd_counts(gvbd_context* ctx, int32_t cd, int64_t* counts);
+GVBD_API int gvbd_cd_counts(gvbd_context* ctx, int32_t cd, int64_t* counts);
 /* recorded lists (after gvbd_cd_query(record = 1)): (vertex, face) pairs with d < r; VF contacts
  * as (vertex, type, prim, face); EE contacts as (edge, other edge) with mu (2 each) */
-int gvbd_cd_vf_pairs(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* d);
-int gvbd_cd_vf_contacts(gvbd_context* ctx, int32_t cd, int32_t* records, float* d);
-int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu, float* d);
+GVBD_API int gvbd_cd_vf_pairs(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* d);
+GVBD_API int gvbd_cd_vf_contacts(gvbd_context* ctx, int32_t cd, int32_t* records, float* d);
+GVBD_API int gvbd_cd_ee_contacts(gvbd_context* ctx, int32_t cd, int32_t* pairs, float* mu, float* d);
 /* capped per-primitive results of the last query: count[nv] and vf_capacity records per vertex;
  * count[ne] and ee_capacity records per edge */
-int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records);
-int gvbd_cd_ee_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_ee_contact* records);
+GVBD_API int gvbd_cd_vf_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_vf_contact* records);
+GVBD_API int gvbd_cd_ee_results(gvbd_context* ctx, int32_t cd, int32_t* count, gvbd_ee_contact* records);
 /* minDisToPrimitives of every vertex (VF) and edge (EE), faceMinDisToVertices of every face */
-int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex, float* face, float* edge);
+GVBD_API int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex, float* face, float* edge);
 
 /* ---- timing on the context stream ---- */
 /* copies `bytes` (multiple of 16) between two device buffers `reps` times with a plain
  * streaming kernel and returns the average kernel time: the measured copy bandwidth and
  * the calibration point for profiler byte counters (no reference counterpart) */
-int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms);
-int gvbd_event_record(gvbd_context* ctx, int32_t slot);  /* slot 0..63 */
-int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, float* ms);
+GVBD_API int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms);
+GVBD_API int gvbd_event_record(gvbd_context* ctx, int32_t slot);  /* slot 0..63 */
+GVBD_API int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, float* ms);
 
 /* ---- rest-pose precompute (host; TetMeshFEM::initialize, TetMesh/TetMeshFEM.cpp:409-429) ----
  * dminv 9*T column-major DmInv, volume T signed det(Dm)/6, mass N lumped vol*density/4 */
-int gvbd_tet_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_tets, const int32_t* tets,
+GVBD_API int gvbd_tet_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_tets, const int32_t* tets,
                             float density, float* dminv, float* volume, float* mass);
 
 /* ---- cloth rest pose, topology and bending forms (host; TriMesh/TriMesh.cpp:213-244, 481-753,
  * VBDCloth/VBDTriMeshStVK.cpp:50-88). dminv 4*F (column-major 2x2), area F, mass N; edges 6*E as
  * (eV1, eV2, eV12Next, eV21Next, fId1, fId2) with -1 on boundary edges; Q 16*E (row-major 4x4).
  * Pass NULL for outputs not needed; *num_edges receives E (call once with edges = Q = NULL to size). */
-int gvbd_cloth_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_faces, const int32_t* faces,
+GVBD_API int gvbd_cloth_mesh_constants(int32_t num_vertices, const float* positions, int32_t num_faces, const int32_t* faces,
                               float density, float* dminv, float* area, float* mass, int32_t* num_edges,
                               int32_t* edges, float* Q);
 
 /* ---- colouring (host) ---- */
 /* cloth vertex graph: mesh edges plus (with_bending) the cross pair of every interior edge
  * (GraphColoring/TriMeshVertexGraph.h:20-45) */
-int gvbd_color_tri_mesh(int32_t num_vertices, int32_t num_faces, const int32_t* faces, int32_t with_bending,
+GVBD_API int gvbd_color_tri_mesh(int32_t num_vertices, int32_t num_faces, const int32_t* faces, int32_t with_bending,
                         int32_t algorithm, int32_t* colors, int32_t* num_colors);
-int gvbd_color_graph(int32_t num_vertices, int32_t num_edges, const int32_t* edges, int32_t algorithm,
+GVBD_API int gvbd_color_graph(int32_t num_vertices, int32_t num_edges, const int32_t* edges, int32_t algorithm,
                      int32_t* colors, int32_t* num_colors);
-int gvbd_color_tet_mesh(int32_t num_vertices, int32_t num_tets, const int32_t* tets, int32_t algorithm,
+GVBD_API int gvbd_color_tet_mesh(int32_t num_vertices, int32_t num_tets, const int32_t* tets, int32_t algorithm,
                         int32_t* colors, int32_t* num_colors);
 
 #ifdef __cplusplus

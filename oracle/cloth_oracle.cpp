@@ -67,6 +67,8 @@ template <class R> struct ClothMesh {
     R mu, lmbd, kb, dampS, dampB, expVel, constVel, density;
     R frictionMu = R(0.1), frictionEpsV = R(0.1);  // contact friction (TriMesh.h:41-42 defaults)
     int initType = 2;
+    bool noGravZone = false;                          // hasNoGravZone (Materials.h:34)
+    R noGravThreshold = 0, maxVel = -1;               // noGravZoneThreshold, maxVelocityMagnitude
     std::vector<int32_t> veBegin, veEdge;             // vertex -> incident edges (ascending)
     // tet meshes in a contact scene: neo-Hookean elements; faces/edges above are the boundary
     // surface, which takes part in contact
@@ -104,6 +106,7 @@ template <class R> struct ClothScene {
     int threads = 1;
     std::vector<oracle::Deformer> deformers;
     float clock = 0.f;                       // curTime (float in the reference)
+    std::atomic<long long> solveFailures{0}; // vertex solves whose 3x3 block had no solution
 };
 
 template <class R> inline void cross3(const R* a, const R* b, R* c) {
@@ -827,6 +830,7 @@ template <class R> void cloth_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid)
         }
     } else {
         for (int d = 0; d < 3; ++d) xn[d] = xv[d];
+        ++s.solveFailures;
     }
 }
 
@@ -866,14 +870,7 @@ template <class R> void cloth_velocity(ClothScene<R>& s, ClothMesh<R>& m) {
     for (int i = 0; i < m.nv; ++i) {
         R* v = &m.v[3 * i];
         for (int d = 0; d < 3; ++d) v[d] = (m.x[3 * i + d] - m.xprev[3 * i + d]) / s.dt;
-        R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (vMag > R(1e-6)) {
-            R vNew = vMag * m.expVel - m.constVel;
-            vNew = vNew > R(1e-6) ? vNew : R(0);
-            for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
-        } else {
-            for (int d = 0; d < 3; ++d) v[d] *= R(0);
-        }
+        tetcore::damp_velocity<R>(v, m.noGravZone, v[1], m.noGravThreshold, m.maxVel, m.expVel, m.constVel);
     }
     for (int f : m.fixedList)
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
@@ -995,6 +992,7 @@ template <class R> void tet_vertex(ClothScene<R>& s, ClothMesh<R>& m, int vid, b
         if (!solve3<R>(h, f, dx)) {
             step = (R)s.p[OP_STEP_SIZE_GD];
             dx[0] = f[0]; dx[1] = f[1]; dx[2] = f[2];
+            ++s.solveFailures;
         }
         for (int d = 0; d < 3; ++d) xn[d] = xv[d] + step * dx[d];
     }
@@ -1053,14 +1051,8 @@ template <class R> void tet_velocity(ClothScene<R>& s, ClothMesh<R>& m) {
     for (int i = 0; i < m.nv; ++i) {
         R* v = &m.v[3 * i];
         for (int d = 0; d < 3; ++d) v[d] = (m.x[3 * i + d] - m.xprev[3 * i + d]) / s.dt;
-        R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (vMag > R(1e-6)) {
-            R vNew = vMag * m.expVel - m.constVel;
-            vNew = vNew > R(1e-6) ? vNew : R(0);
-            for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
-        } else {
-            for (int d = 0; d < 3; ++d) v[d] *= R(0);
-        }
+        tetcore::damp_velocity<R>(v, m.noGravZone, m.x[3 * i + 1], m.noGravThreshold, m.maxVel, m.expVel,
+                                  m.constVel);
     }
     for (int f : m.fixedList)
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
@@ -1158,6 +1150,9 @@ static int cloth_add(ClothScene<R>& s, int nv, const double* pos, const double* 
     m.initType = (int)mat[OMC_INIT_TYPE];
     m.frictionMu = (R)mat[OMC_FRICTION_MU];
     m.frictionEpsV = (R)mat[OMC_FRICTION_EPS_V];
+    m.noGravZone = mat[OMC_NO_GRAV_ZONE] != 0;
+    m.noGravThreshold = (R)mat[OMC_NO_GRAV_THRESHOLD];
+    m.maxVel = (R)mat[OMC_MAX_VELOCITY];
     if (tri_prepare(m) || tri_topology(m)) return -1;
     bending_forms(m);
     m.fixed.assign(nv, 0);
@@ -1204,6 +1199,9 @@ static int tet_add(ClothScene<R>& s, int nv, const double* pos, const double* ve
     m.constVel = (R)mat[OM_CONST_VEL_DAMPING] * s.dt;
     m.frictionMu = (R)mat[OM_FRICTION_MU];
     m.frictionEpsV = (R)mat[OM_FRICTION_EPS_V];
+    m.noGravZone = mat[OM_NO_GRAV_ZONE] != 0;
+    m.noGravThreshold = (R)mat[OM_NO_GRAV_THRESHOLD];
+    m.maxVel = (R)mat[OM_MAX_VELOCITY];
     m.tdminv.resize(9 * (size_t)nt);
     m.tvol.resize(nt);
     m.mass.resize(nv);
@@ -1465,6 +1463,9 @@ int oracle_cloth_scene_add_deformer(OracleClothScene* sc, int kind, const double
     if (sc->precision == 64)
         return oracle::deformer_add(sc->s64.deformers, sc->s64.meshes, kind, params, nmeshes, meshes, counts, verts);
     return oracle::deformer_add(sc->s32.deformers, sc->s32.meshes, kind, params, nmeshes, meshes, counts, verts);
+}
+long long oracle_cloth_scene_solve_failures(OracleClothScene* sc) {
+    return sc->precision == 64 ? sc->s64.solveFailures.load() : sc->s32.solveFailures.load();
 }
 double oracle_cloth_scene_time(OracleClothScene* sc) { return sc->precision == 64 ? sc->s64.clock : sc->s32.clock; }
 void oracle_cloth_scene_set_time(OracleClothScene* sc, double t) { sc->s64.clock = sc->s32.clock = (float)t; }

@@ -58,6 +58,9 @@ enum OracleClothMaterialParam {
     OMC_INIT_TYPE,               /* VBDObjectParamsTriMesh::initializationType: 0, 2, 3, 4 */
     OMC_FRICTION_MU,             /* TriMeshParams::frictionDynamic (contact friction, averaged per pair) */
     OMC_FRICTION_EPS_V,          /* TriMeshParams::frictionEpsV */
+    OMC_NO_GRAV_ZONE,            /* ObjectParams::hasNoGravZone (Materials.h:34) */
+    OMC_NO_GRAV_THRESHOLD,       /* noGravZoneThreshold, compared with the velocity's y (VBDClothPhysics.cpp:2330-2332) */
+    OMC_MAX_VELOCITY,            /* maxVelocityMagnitude (Materials.h:35; <= 0: none) */
     OMC_COUNT = 16
 };
 
@@ -70,7 +73,10 @@ enum OracleMaterialParam {
     OM_INIT_TYPE,                /* ObjectParamsVBD::initializationType */
     OM_FRICTION_MU,              /* contact friction of the boundary surface (contact scenes) */
     OM_FRICTION_EPS_V,
-    OM_COUNT = 10
+    OM_NO_GRAV_ZONE,             /* ObjectParams::hasNoGravZone (Materials.h:34) */
+    OM_NO_GRAV_THRESHOLD,        /* noGravZoneThreshold, compared with the position's y (VBDPhysics.cpp:2534-2536) */
+    OM_MAX_VELOCITY,             /* maxVelocityMagnitude (Materials.h:35; <= 0: none) */
+    OM_COUNT = 13
 };
 
 #ifdef __cplusplus
@@ -118,6 +124,8 @@ int oracle_tet_scene_set_state(OracleTetScene*, int mesh, const double* pos, con
 int oracle_tet_scene_add_deformer(OracleTetScene*, int kind, const double* params, int nmeshes, const int32_t* meshes,
                                   const int32_t* counts, const int32_t* verts);
 double oracle_tet_scene_time(OracleTetScene*);
+/* vertex solves whose 3x3 block had no solution (gradient step taken), a diagnostic count */
+long long oracle_tet_scene_solve_failures(OracleTetScene*);
 void oracle_tet_scene_set_time(OracleTetScene*, double t);
 /* VBDPhysics::evaluateConvergence: out[4] = {energy, inertia energy, elastic energy, avg force norm} */
 int oracle_tet_scene_evaluate_convergence(OracleTetScene*, double* out);
@@ -152,6 +160,8 @@ int oracle_cloth_scene_set_state(OracleClothScene*, int mesh, const double* pos,
 int oracle_cloth_scene_add_deformer(OracleClothScene*, int kind, const double* params, int nmeshes,
                                     const int32_t* meshes, const int32_t* counts, const int32_t* verts);
 double oracle_cloth_scene_time(OracleClothScene*);
+/* vertex solves whose 3x3 block had no solution (cloth: no move; tet: gradient step) */
+long long oracle_cloth_scene_solve_failures(OracleClothScene*);
 void oracle_cloth_scene_set_time(OracleClothScene*, double t);
 int oracle_cloth_scene_add_attachments(OracleClothScene*, int n, const int32_t* ints, const double* vals);
 

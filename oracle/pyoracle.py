@@ -18,8 +18,8 @@ OP = dict(GRAVITY_X=0, GRAVITY_Y=1, GRAVITY_Z=2, DT=3, NUM_SUBSTEPS=4, ITERATION
           FRICTION_START_ITER=18, ASSOCIATE_GRAVITY=19, INIT_RATIO_G=21, USE_ACCELERATOR=32, ACCELERATOR_RHO=33)
 OP_COUNT = 34
 OM = dict(DENSITY=0, MU=1, LAMBDA=2, DAMP_HYDRO=3, DAMP_DEV=4, EXP_VEL_DAMPING=5, CONST_VEL_DAMPING=6, INIT_TYPE=7,
-          FRICTION_MU=8, FRICTION_EPS_V=9)
-OM_COUNT = 10
+          FRICTION_MU=8, FRICTION_EPS_V=9, NO_GRAV_ZONE=10, NO_GRAV_THRESHOLD=11, MAX_VELOCITY=12)
+OM_COUNT = 13
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -58,6 +58,8 @@ def lib():
         L.oracle_tet_scene_add_deformer.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, _ip]
         L.oracle_tet_scene_time.argtypes = [ctypes.c_void_p]
         L.oracle_tet_scene_time.restype = ctypes.c_double
+        L.oracle_tet_scene_solve_failures.argtypes = [ctypes.c_void_p]
+        L.oracle_tet_scene_solve_failures.restype = ctypes.c_longlong
         L.oracle_tet_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oracle_tet_scene_set_time.restype = None
         _lib = L
@@ -110,6 +112,9 @@ def material_vector(m):
     v[OM["INIT_TYPE"]] = m.get("initialization_type", 0)
     v[OM["FRICTION_MU"]] = m.get("friction_dynamic", 0.1)
     v[OM["FRICTION_EPS_V"]] = m.get("friction_eps_v", 0.1)
+    v[OM["NO_GRAV_ZONE"]] = m.get("has_no_grav_zone", 0)
+    v[OM["NO_GRAV_THRESHOLD"]] = m.get("no_grav_zone_threshold", 0.0)
+    v[OM["MAX_VELOCITY"]] = m.get("max_velocity_magnitude", -1.0)
     return v
 
 
@@ -253,6 +258,10 @@ class TetScene:
     def time(self):
         return lib().oracle_tet_scene_time(self._h)
 
+    def solve_failures(self):
+        """Vertex solves whose 3x3 block had no solution so far."""
+        return lib().oracle_tet_scene_solve_failures(self._h)
+
     @time.setter
     def time(self, t):
         lib().oracle_tet_scene_set_time(self._h, float(t))
@@ -291,7 +300,8 @@ OP.update(CLOTH_BOUNDARY_STIFFNESS=22, CLOTH_BOUNDARY_FRICTION_EPS=23, DEGENERAT
           CONTACT_STIFFNESS=26, CONTACT_RADIUS=27, THICKNESS=28, CONSERVATIVE_RELAXATION=29, CONTACT_ENERGY_TYPE=30,
           MAX_QUERY_DIS=31)
 OMC = dict(DENSITY=0, MU=1, LAMBDA=2, BENDING=3, DAMP_STVK=4, DAMP_BENDING=5, EXP_VEL_DAMPING=6,
-           CONST_VEL_DAMPING=7, INIT_TYPE=8, FRICTION_MU=9, FRICTION_EPS_V=10)
+           CONST_VEL_DAMPING=7, INIT_TYPE=8, FRICTION_MU=9, FRICTION_EPS_V=10, NO_GRAV_ZONE=11, NO_GRAV_THRESHOLD=12,
+           MAX_VELOCITY=13)
 OMC_COUNT = 16
 
 
@@ -321,6 +331,8 @@ def _cloth_lib():
         L.oracle_cloth_scene_add_deformer.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, _ip]
         L.oracle_cloth_scene_time.argtypes = [ctypes.c_void_p]
         L.oracle_cloth_scene_time.restype = ctypes.c_double
+        L.oracle_cloth_scene_solve_failures.argtypes = [ctypes.c_void_p]
+        L.oracle_cloth_scene_solve_failures.restype = ctypes.c_longlong
         L.oracle_cloth_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oracle_cloth_scene_set_time.restype = None
         L.oracle_cloth_scene_add_attachments.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp]
@@ -358,6 +370,9 @@ def cloth_material_vector(m):
     v[OMC["INIT_TYPE"]] = m.get("initialization_type", 2)
     v[OMC["FRICTION_MU"]] = m.get("friction_dynamic", 0.1)
     v[OMC["FRICTION_EPS_V"]] = m.get("friction_eps_v", 0.1)
+    v[OMC["NO_GRAV_ZONE"]] = m.get("has_no_grav_zone", 0)
+    v[OMC["NO_GRAV_THRESHOLD"]] = m.get("no_grav_zone_threshold", 0.0)
+    v[OMC["MAX_VELOCITY"]] = m.get("max_velocity_magnitude", -1.0)
     return v
 
 
@@ -502,6 +517,10 @@ class ClothScene:
     @property
     def time(self):
         return _cloth_lib().oracle_cloth_scene_time(self._h)
+
+    def solve_failures(self):
+        """Vertex solves whose 3x3 block had no solution so far."""
+        return _cloth_lib().oracle_cloth_scene_solve_failures(self._h)
 
     @time.setter
     def time(self, t):

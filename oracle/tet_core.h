@@ -8,6 +8,29 @@
 
 namespace tetcore {
 
+// Velocity damping of one vertex after v = (x - x_prev) / dt: dampVelocities
+// (VBD/VBDPhysics.cpp:2529-2555, tet meshes, zone coordinate = the position's y) and
+// updateVelocity (VBDCloth/VBDClothPhysics.cpp:2325-2351, cloth meshes, zone coordinate = the
+// velocity's y). In the no-gravity zone (hasNoGravZone, maxVelocityMagnitude > 0 and the zone
+// coordinate above noGravZoneThreshold) the speed is only capped at maxVelocityMagnitude;
+// elsewhere |v| -> max(|v| expDamp - constDamp, 0) with speeds of at most 1e-6 zeroed.
+template <class R>
+inline void damp_velocity(R* v, bool noGravZone, R zoneCoord, R threshold, R maxVel, R expDamp, R constDamp) {
+    const R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (noGravZone && maxVel > R(0) && zoneCoord > threshold) {
+        if (vMag > maxVel) {
+            const R s = maxVel / vMag;
+            for (int d = 0; d < 3; ++d) v[d] *= s;
+        }
+    } else if (vMag > R(1e-6)) {
+        R vNew = vMag * expDamp - constDamp;
+        vNew = vNew > R(1e-6) ? vNew : R(0);
+        for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
+    } else {
+        for (int d = 0; d < 3; ++d) v[d] *= R(0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 3x3 helpers, column-major like the reference's Eigen Mat3 (Types.h:76-127).
 // m[r + 3*c]

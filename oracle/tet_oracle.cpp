@@ -21,6 +21,7 @@
 #include "tet_core.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -92,6 +93,8 @@ template <class R> struct Mesh {
     std::vector<std::vector<int32_t>> categories;
     R mu, lmbd, dampH, dampD, expVelDamp, constVelDamp, density;
     int initType = 0;
+    bool noGravZone = false;        // hasNoGravZone (with maxVelocityMagnitude > 0)
+    R noGravThreshold = 0, maxVel = -1;
 };
 
 template <class R> struct Scene {
@@ -102,6 +105,7 @@ template <class R> struct Scene {
     oracle::ThreadPool* pool = nullptr;
     std::vector<oracle::Deformer> deformers;
     float clock = 0.f;  // curTime (float in the reference)
+    std::atomic<long long> solveFailures{0};  // vertex solves whose 3x3 block had no solution
 };
 
 // VBD_BaseMaterial.h:244-259, 201-221; VBD_BaseMaterial.cpp:7-50.
@@ -197,24 +201,19 @@ template <class R> void vbd_vertex(Scene<R>& s, Mesh<R>& m, int vid, bool applyF
         if (!solve3<R>(h, f, dx)) {
             step = (R)s.p[OP_STEP_SIZE_GD];
             dx[0] = f[0]; dx[1] = f[1]; dx[2] = f[2];
+            ++s.solveFailures;
         }
         for (int d = 0; d < 3; ++d) xv[d] += step * dx[d];
     }
 }
 
-// VBDPhysics.cpp:2486-2561 updateVelocities + dampVelocities (no-gravity-zone branch omitted).
+// VBDPhysics.cpp:2486-2561 updateVelocities + dampVelocities.
 template <class R> void update_velocities(Scene<R>& s, Mesh<R>& m) {
     for (int i = 0; i < m.nv; ++i) {
         R* v = &m.v[3 * i];
         for (int d = 0; d < 3; ++d) v[d] = (m.x[3 * i + d] - m.xprev[3 * i + d]) / s.dt;
-        R vMag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (vMag > R(1e-6)) {
-            R vNew = vMag * m.expVelDamp - m.constVelDamp;
-            vNew = vNew > R(1e-6) ? vNew : R(0);
-            for (int d = 0; d < 3; ++d) v[d] *= vNew / vMag;
-        } else {
-            for (int d = 0; d < 3; ++d) v[d] *= R(0);
-        }
+        tetcore::damp_velocity<R>(v, m.noGravZone, m.x[3 * i + 1], m.noGravThreshold, m.maxVel, m.expVelDamp,
+                                  m.constVelDamp);
     }
     for (int f : m.fixedList)
         for (int d = 0; d < 3; ++d) m.v[3 * f + d] = 0;
@@ -376,6 +375,9 @@ template <class R> static int add_mesh(Scene<R>& s, int nv, const double* pos, c
     // VBDPhysics.cpp:794-807 timeStepEqualize
     m.expVelDamp = (R)std::pow((R)mat[OM_EXP_VEL_DAMPING], s.dt);
     m.constVelDamp = (R)mat[OM_CONST_VEL_DAMPING] * s.dt;
+    m.noGravZone = mat[OM_NO_GRAV_ZONE] != 0;
+    m.noGravThreshold = (R)mat[OM_NO_GRAV_THRESHOLD];
+    m.maxVel = (R)mat[OM_MAX_VELOCITY];
     m.dminv.resize(9 * nt);
     m.vol.resize(nt);
     m.mass.resize(nv);
@@ -521,6 +523,9 @@ int oracle_tet_scene_add_deformer(OracleTetScene* sc, int kind, const double* pa
     if (sc->precision == 64)
         return oracle::deformer_add(sc->s64.deformers, sc->s64.meshes, kind, params, nmeshes, meshes, counts, verts);
     return oracle::deformer_add(sc->s32.deformers, sc->s32.meshes, kind, params, nmeshes, meshes, counts, verts);
+}
+long long oracle_tet_scene_solve_failures(OracleTetScene* sc) {
+    return sc->precision == 64 ? sc->s64.solveFailures.load() : sc->s32.solveFailures.load();
 }
 double oracle_tet_scene_time(OracleTetScene* sc) { return sc->precision == 64 ? sc->s64.clock : sc->s32.clock; }
 void oracle_tet_scene_set_time(OracleTetScene* sc, double t) { sc->s64.clock = sc->s32.clock = (float)t; }

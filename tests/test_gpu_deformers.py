@@ -319,6 +319,30 @@ def test_argument_errors(gpu_ctx):
             sim.initialize()
 
 
+def test_vertex_selected_by_two_deformers_is_rejected(gpu_ctx):
+    """Deformed vertices of one phase move in one launch, so a vertex may belong to one deformer
+    only (the reference applies deformers one after another in list order)."""
+    pos, faces = scenes.cloth_sheet(4, 0.1, 0.0)
+    sim = VBDPhysics(gpu_ctx)
+    sim.addClothMesh(pos, faces)
+    sim.addDeformer(dict(DeformerName="Translator", speed=[1, 0, 0], selectedMeshes=[0], selectedVertices=[[0, 1]]))
+    sim.addDeformer(dict(DeformerName="Translator", speed=[0, 1, 0], selectedMeshes=[0], selectedVertices=[[2, 1]]))
+    with pytest.raises(gaia_amd.VBDError, match="already selected by a deformer"):
+        sim.initialize()
+
+
+def test_tet_deformer_with_collision_handling_is_rejected(gpu_ctx):
+    """A tet deformer moves vertices after the frame's first detection, so contact bounds would not
+    see the motion: such scenes are refused (cloth deformers, which move before detection, are not)."""
+    from vbd_cases import tet_drop_spec
+
+    spec = tet_drop_spec(steps=1)
+    spec["deformers"] = [dict(DeformerName="Rotator", center=[0, 0, 0], axis=[0, 1, 0], angularVelocity=1.0,
+                              selectedMeshes=[1], selectedVertices=[[0, 1]], rotationEndTime=10)]
+    with pytest.raises(gaia_amd.VBDError, match="deformers of tet meshes"):
+        build_gpu(gpu_ctx, spec)
+
+
 def test_served_deformer_scenes_equal_alone(gpu_ctx):
     """Scenes with deformers in the serving loop under a budget that forces copy-out between
     frames: the scene time and the deformed vertices travel with the arena, so each served scene

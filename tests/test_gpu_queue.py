@@ -126,6 +126,34 @@ def test_iteration_hooks_equal_run_step(gpu_ctx, mode):
     b.destroy()
 
 
+def test_batch_per_group_calls_equal_run_iteration(gpu_ctx):
+    """The per-group unit of a batch (gvbd_batch_solve_parallel_group) over every group of one
+    iteration equals the iteration: cloth slots solved through positionsNext are committed after
+    their group, and a contact scene's same-group partners are read from the iteration snapshot."""
+    from vbd_cases import contact_drape_spec
+
+    for spec in (cloth_drop_spec(), contact_drape_spec()):
+        a, b = build_gpu(gpu_ctx, spec), build_gpu(gpu_ctx, spec)
+        ba, bb = Batch(gpu_ctx, [a]), Batch(gpu_ctx, [b])
+        contact = spec["physics"]["handle_collision"] != 0
+        for _ in range(2):
+            for sim, bt in ((a, ba), (b, bb)):
+                if contact:
+                    sim.applyCollisionDetection()
+                bt.apply_initial_step()
+            for it in range(3):
+                ba.run_iteration(it)
+                for g in range(bb.num_groups()):
+                    bb.solve_group(g, it)
+            ba.update_velocities()
+            bb.update_velocities()
+        for m in range(len(spec["meshes"])):
+            (xa, va), (xb, vb) = a.getState(m), b.getState(m)
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb), f"mesh {m}"
+        for x in (ba, bb, a, b):
+            x.destroy()
+
+
 def test_schedule_validation(gpu_ctx):
     spec = multi_spec()
     sim = build_gpu(gpu_ctx, spec)

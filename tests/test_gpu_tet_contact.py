@@ -59,8 +59,9 @@ def test_many_bodies_error_distribution(gpu_ctx, oracle):
     """The tet-contact workload at 3^3 cells per body (32 cubes and balls, 2,048 vertices) over 80
     frames: bodies fall, hit the ground and each other. Stiff contact (k = 1e4 against
     m / dt^2 of about 5e3) makes the impact steps ill-conditioned (the FP32 and FP64 oracles part
-    by up to ~2e-4 there), so the per-step errors are checked as a distribution: median at most
-    that of FP32 vs FP64, and most steps within the per-step tolerance."""
+    by up to ~2e-4 there), so the per-step errors are checked against that spread: the median
+    within 2x of the FP32-vs-FP64 median (or 1e-6), and the worst step within 4x of the worst
+    FP32-vs-FP64 step (or 1e-5)."""
     bodies = scenes.c2_bodies(n=3)
     spec = dict(physics=physics_dict(**scenes.c2_physics()), steps=80,
                 meshes=[dict(pos=b["pos"], tets=b["tets"], material=b["material"], fixed=[], vel=None,
@@ -69,7 +70,7 @@ def test_many_bodies_error_distribution(gpu_ctx, oracle):
     print(f"\n[tet contact, 32 bodies] gpu-vs-fp32 median {np.median(errs):.2e} max {errs.max():.2e}; "
           f"fp32-vs-fp64 median {np.median(errs64):.2e} max {errs64.max():.2e}", flush=True)
     assert np.median(errs) <= max(2.0 * np.median(errs64), 1e-6)
-    assert (errs < TOL).mean() >= 0.8
+    assert errs.max() <= max(4.0 * errs64.max(), TOL)
 
 
 def test_soft_tet_apex_closed_form(gpu_ctx):

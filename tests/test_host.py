@@ -27,6 +27,9 @@ def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", gaia_amd.LIB_PATH], capture_output=True, text=True).stdout
     exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
     assert set(declared) <= exported
+    # nothing else: the C++ internals (and the standard library's instantiations) stay hidden
+    everything = {l.split()[-1] for l in out.splitlines() if len(l.split()) >= 3}
+    assert everything == set(declared), sorted(everything - set(declared))[:5]
     # ctypes binding covers every declared function
     assert sorted(declared) == sorted(gaia_amd._lib._SIGNATURES)
 
