@@ -79,6 +79,25 @@ enum OracleMaterialParam {
     OM_COUNT = 13
 };
 
+/* penetration contact of tet meshes (CollisionDetertionParameters.h:43-80, VBDPhysicsParameters.h:36,70) */
+enum OracleCollisionParam {
+    OC_ALLOW_DCD = 0,
+    OC_ALLOW_CCD,
+    OC_SELF_COLLISION,           /* handleSelfCollision */
+    OC_FEASIBLE_REGION,          /* checkFeasibleRegion */
+    OC_TET_TRAVERSE,             /* checkTetTraverse */
+    OC_TRAVERSE_NON_SELF,        /* tetrahedralTraverseForNonSelfIntersection */
+    OC_SHIFT_TO_CENTER,          /* shiftQueryPointToCenter */
+    OC_STOP_AFTER_PASSING,       /* stopTraversingAfterPassingQueryPoint */
+    OC_STIFFNESS,                /* collisionStiffness */
+    OC_OFF_HEIGHT,               /* collisionOffHeight */
+    OC_FEASIBLE_EPS,             /* feasibleRegionEpsilon */
+    OC_RAY_EPS,                  /* rayTriIntersectionEPSILON */
+    OC_MAX_SEARCH_MULT,          /* maxSearchDistanceMultiplier */
+    OC_CENTER_SHIFT,             /* centerShiftLevel */
+    OC_COUNT = 14
+};
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -129,6 +148,24 @@ long long oracle_tet_scene_solve_failures(OracleTetScene*);
 void oracle_tet_scene_set_time(OracleTetScene*, double t);
 /* VBDPhysics::evaluateConvergence: out[4] = {energy, inertia energy, elastic energy, avg force norm} */
 int oracle_tet_scene_evaluate_convergence(OracleTetScene*, double* out);
+/* penetration contact (DCD + CCD, runStepGPU semantics); before finalize */
+int oracle_tet_scene_set_collision(OracleTetScene*, const double* params /*OC_COUNT*/);
+/* last detection, 6 ints per surface vertex: mesh, vertex, hit mesh, hit face, penetrated, from CCD */
+int oracle_tet_scene_collisions(OracleTetScene*, int32_t* out);
+int oracle_tet_scene_surface(OracleTetScene*, int mesh, int32_t* faces, int32_t* owner);
+/* geometry of the collision model, for pins: strict point-in-tet, the earliest moving point /
+ * moving triangle hit (t, barycentrics; returns 1 on a hit), cubic roots in [0, 1] */
+int oracle_point_in_tet_f32(const float* p, const float* a, const float* b, const float* c, const float* d);
+int oracle_ccd_point_triangle_f64(const double* f0, const double* f1, const double* p0, const double* p1, double* t,
+                                  double* bary);
+int oracle_cubic_roots01_f64(const double* coef, double* roots);
+/* the ray walk of the closest-point check through a tet mesh at positions x (1: target entered) */
+int oracle_tet_walk_f64(int nv, const double* x, int nt, const int32_t* tets, const double* origin, const double* dir,
+                        double maxDis, int startTet, int startLocal, int target);
+/* one penetration contact: x4 = (vertex, face a, b, c) now, xp4 at the step start; order 3 for the
+ * vertex side, 0..2 for face vertex `order`; f (3) and H (9, column-major) are overwritten */
+void oracle_penetration_force_f64(const double* x4, const double* xp4, int order, double k, double mu, double epsV,
+                                  double dt, double* f, double* H);
 
 /* --- cloth: TriMesh precompute + topology + bending forms (TriMesh.cpp, VBDTriMeshStVK.cpp:50-88) ---
  * dminv 4F col-major, area F, mass N, edges 6E (eV1 eV2 eV12Next eV21Next fId1 fId2), Q 16E row-major;

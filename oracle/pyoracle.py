@@ -62,6 +62,16 @@ def lib():
         L.oracle_tet_scene_solve_failures.restype = ctypes.c_longlong
         L.oracle_tet_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oracle_tet_scene_set_time.restype = None
+        L.oracle_tet_scene_set_collision.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_tet_scene_collisions.argtypes = [ctypes.c_void_p, _ip]
+        L.oracle_tet_scene_surface.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip]
+        L.oracle_point_in_tet_f32.argtypes = [_fp] * 5
+        L.oracle_ccd_point_triangle_f64.argtypes = [_dp] * 6
+        L.oracle_cubic_roots01_f64.argtypes = [_dp, _dp]
+        L.oracle_tet_walk_f64.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_penetration_force_f64.argtypes = [_dp, _dp, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp, _dp]
+        L.oracle_penetration_force_f64.restype = None
         _lib = L
     return _lib
 
@@ -98,6 +108,68 @@ def physics_vector(p):
     v[OP["USE_ACCELERATOR"]] = p.get("use_accelerator", 0)
     v[OP["ACCELERATOR_RHO"]] = p.get("accelerator_rho", 0.93)
     return v
+
+
+# penetration contact of tet meshes (oracle.h OracleCollisionParam); defaults of
+# CollisionDetertionParameters.h:43-80 and VBDPhysicsParameters.h:36,70
+OC_KEYS = ("allow_dcd", "allow_ccd", "self_collision", "feasible_region", "tet_traverse", "traverse_non_self",
+           "shift_to_center", "stop_after_passing", "stiffness", "off_height", "feasible_eps", "ray_eps",
+           "max_search_mult", "center_shift")
+OC_DEFAULTS = dict(allow_dcd=1, allow_ccd=1, self_collision=1, feasible_region=1, tet_traverse=1, traverse_non_self=1,
+                   shift_to_center=1, stop_after_passing=1, stiffness=1e5, off_height=1e7, feasible_eps=1e-2,
+                   ray_eps=1e-10, max_search_mult=1.8, center_shift=0.01)
+
+
+def collision_vector(c):
+    full = dict(OC_DEFAULTS, **(c or {}))
+    return np.array([float(full[k]) for k in OC_KEYS])
+
+
+def point_in_tet(p, a, b, c, d):
+    """CollisionGeometry.h:56-88 in float: strictly inside a tet of |6V| >= 1e-5."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (p, a, b, c, d)]
+    return bool(lib().oracle_point_in_tet_f32(*[x.ctypes.data_as(_fp) for x in arrs]))
+
+
+def ccd_point_triangle(f0, f1, p0, p1):
+    """Earliest hit of the point moving p0 -> p1 with the triangle moving f0 -> f1 (3x3 rows):
+    (t, barycentrics) or None."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (f0, f1, p0, p1)]
+    t = np.zeros(1)
+    bary = np.zeros(3)
+    hit = lib().oracle_ccd_point_triangle_f64(*[x.ctypes.data_as(_dp) for x in arrs], t.ctypes.data_as(_dp),
+                                              bary.ctypes.data_as(_dp))
+    return (float(t[0]), bary) if hit else None
+
+
+def cubic_roots01(coef):
+    """Roots in [0, 1] where c0 + c1 x + c2 x^2 + c3 x^3 changes sign, ascending."""
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    r = np.zeros(3)
+    n = lib().oracle_cubic_roots01_f64(c.ctypes.data_as(_dp), r.ctypes.data_as(_dp))
+    return r[:n]
+
+
+def tet_walk(pos, tets, origin, direction, max_dis, start_tet, start_local, target):
+    """The closest-point check's ray walk through the tets (TetMeshFEM::tetrahedralTraverseTo)."""
+    x, px = _d(np.asarray(pos).reshape(-1))
+    t, pt = _i(np.asarray(tets).reshape(-1))
+    o, po = _d(origin)
+    d, pd = _d(direction)
+    return bool(lib().oracle_tet_walk_f64(len(x) // 3, px, len(t) // 4, pt, po, pd, float(max_dis), int(start_tet),
+                                          int(start_local), int(target)))
+
+
+def penetration_force(x4, xp4, order, k, mu=0.0, eps_v=1.0, dt=1.0 / 60.0):
+    """Force and Hessian of one penetration contact on the vertex of `order` (3: the colliding
+    vertex, 0..2: a vertex of its face); x4 rows = vertex, a, b, c."""
+    x, px = _d(np.asarray(x4).reshape(-1))
+    xp, pxp = _d(np.asarray(xp4).reshape(-1))
+    f = np.zeros(3)
+    h = np.zeros(9)
+    lib().oracle_penetration_force_f64(px, pxp, int(order), float(k), float(mu), float(eps_v), float(dt),
+                                       f.ctypes.data_as(_dp), h.ctypes.data_as(_dp))
+    return f, h.reshape(3, 3).T
 
 
 def material_vector(m):
@@ -237,6 +309,28 @@ class TetScene:
 
     def finalize(self):
         lib().oracle_tet_scene_finalize(self._h)
+
+    def set_collision(self, params=None):
+        """Penetration contact (DCD + CCD, runStepGPU's order) with the given parameters over the
+        reference's defaults (keys of OC_KEYS); before finalize."""
+        v, pv = _d(collision_vector(params))
+        lib().oracle_tet_scene_set_collision(self._h, pv)
+
+    def collisions(self):
+        """The last detection, one row per surface vertex: (mesh, vertex, hit mesh, hit face,
+        penetrated, from CCD); hit face -1 = none."""
+        n = lib().oracle_tet_scene_collisions(self._h, None)
+        out = np.zeros((n, 6), np.int32)
+        lib().oracle_tet_scene_collisions(self._h, out.ctypes.data_as(_ip))
+        return out
+
+    def surface(self, mesh=0):
+        """(faces (F, 3), owner (F, 2): owning tet and local face) of the collision model."""
+        nf = lib().oracle_tet_scene_surface(self._h, mesh, None, None)
+        faces = np.zeros((nf, 3), np.int32)
+        owner = np.zeros((nf, 2), np.int32)
+        lib().oracle_tet_scene_surface(self._h, mesh, faces.ctypes.data_as(_ip), owner.ctypes.data_as(_ip))
+        return faces, owner
 
     def groups(self):
         out = []

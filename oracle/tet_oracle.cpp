@@ -15,12 +15,15 @@
 //   VBD/VBD_BaseMaterial.h:359-366        inertia force/Hessian
 //   VBD/VBDPhysics.cpp:2821-2954,2563-2592 plane-ground boundary penalty + IPC friction
 //   VBD/VBDPhysics.cpp:2486-2561,794-807  velocity update + damping (step-invariant scaling)
+//   tet_collision.h                       penetration contact (DCD / CCD) of runStepGPU
 #include "oracle.h"
 #include "deform.h"
 #include "pool.h"
+#include "tet_collision.h"
 #include "tet_core.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -95,6 +98,7 @@ template <class R> struct Mesh {
     int initType = 0;
     bool noGravZone = false;        // hasNoGravZone (with maxVelocityMagnitude > 0)
     R noGravThreshold = 0, maxVel = -1;
+    R frictionMu = 0, frictionEpsV = 0;  // contact friction (frictionDynamic, frictionEpsV)
 };
 
 template <class R> struct Scene {
@@ -106,6 +110,16 @@ template <class R> struct Scene {
     std::vector<oracle::Deformer> deformers;
     float clock = 0.f;  // curTime (float in the reference)
     std::atomic<long long> solveFailures{0};  // vertex solves whose 3x3 block had no solution
+    // penetration contact (tet_collision.h): per mesh its surface topology, per vertex the last
+    // detection (hit mesh and face, whether a tet contained it, whether CCD found it) and the
+    // relations (colliding mesh, colliding vertex, order) of the vertices of hit faces
+    bool collide = false;
+    tetcol::Params col;
+    std::vector<tetcol::Topo> topo;
+    std::vector<std::vector<tetcol::Hit>> hits;
+    std::vector<std::vector<uint8_t>> penetrated, fromCCD;
+    std::vector<std::vector<std::vector<std::array<int, 3>>>> relations;
+    std::vector<std::vector<R>> snap;  // positions at the start of the current parallel group
 };
 
 // VBD_BaseMaterial.h:244-259, 201-221; VBD_BaseMaterial.cpp:7-50.
@@ -143,8 +157,11 @@ template <class R> void apply_initial_step(Scene<R>& s, Mesh<R>& m) {
     }
 }
 
-// VBDPhysics.cpp:2402-2483 VBDStepWithCollision (no contacts, no line search).
-template <class R> void vbd_vertex(Scene<R>& s, Mesh<R>& m, int vid, bool applyFriction) {
+template <class R> void collision_terms(Scene<R>& s, int mi, int vid, R* f, R* h);
+
+// VBDPhysics.cpp:2402-2483 VBDStepWithCollision (no line search); with penetration contact its
+// collision terms follow the boundary terms
+template <class R> void vbd_vertex(Scene<R>& s, Mesh<R>& m, int vid, bool applyFriction, int mi = -1) {
     R h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     R f[3] = {0, 0, 0};
     R* xv = &m.x[3 * vid];
@@ -195,6 +212,7 @@ template <class R> void vbd_vertex(Scene<R>& s, Mesh<R>& m, int vid, bool applyF
             }
         }
     }
+    if (s.collide && mi >= 0) collision_terms(s, mi, vid, f, h);
     if (f[0] * f[0] + f[1] * f[1] + f[2] * f[2] > R(1e-10)) {
         R dx[3];
         R step = (R)s.p[OP_STEP_SIZE];
@@ -226,6 +244,84 @@ template <class R> R accelerator_omega(int order, R rho, R prevOmega) {
     return R(4) / (R(4) - rho * rho * prevOmega);
 }
 
+// The collision terms of vertex vid of mesh mi (VBDPhysicsCompute.cu:1731-1770): its own
+// collision (weight 1) and the collisions whose closest face it belongs to (weight -bary), every
+// position taken at the start of the parallel group
+template <class R> void collision_terms(Scene<R>& s, int mi, int vid, R* f, R* h) {
+    using tetcol::ld;
+    auto contact = [&](int qm, int qv, int order) {
+        const tetcol::Hit& hit = s.hits[qm][qv];
+        const tetcol::Topo& T = s.topo[hit.mesh];
+        const int32_t* fv = &T.faces[3 * hit.face];
+        const R* Xq = s.snap[qm].data();
+        const R* Xf = s.snap[hit.mesh].data();
+        const Mesh<R>& mq = s.meshes[qm];
+        const Mesh<R>& mf = s.meshes[hit.mesh];
+        // contact friction: the average of the two meshes' coefficients
+        const R mu = (mq.frictionMu + mf.frictionMu) * R(0.5), epsV = (mq.frictionEpsV + mf.frictionEpsV) * R(0.5);
+        tetcol::penetration_force<R>(ld(Xq, qv), ld(Xf, fv[0]), ld(Xf, fv[1]), ld(Xf, fv[2]), ld(mq.xprev.data(), qv),
+                                     ld(mf.xprev.data(), fv[0]), ld(mf.xprev.data(), fv[1]),
+                                     ld(mf.xprev.data(), fv[2]), order, (R)s.col.stiffness, mu, epsV, s.dt, f, h);
+    };
+    if (s.hits[mi][vid].face >= 0) contact(mi, vid, 3);
+    for (const auto& r : s.relations[mi][vid]) contact(r[0], r[1], r[2]);
+}
+
+// dcd() / ccd() of runStepGPU (VBDPhysics.cpp:1958-2014, 2066-2102) and prepareCollisionData
+// (:1708-1760): detection at the start of the substep (DCD) and after the initial step (CCD, for
+// surface vertices no tet contained); relations in ascending (mesh, vertex) order
+template <class R> void detect_dcd(Scene<R>& s) {
+    const int nm = (int)s.meshes.size();
+    std::vector<const R*> X(nm);
+    for (int m = 0; m < nm; ++m) X[m] = s.meshes[m].x.data();
+    for (int m = 0; m < nm; ++m) {
+        auto& H = s.hits[m];
+        std::fill(H.begin(), H.end(), tetcol::Hit{});
+        std::fill(s.penetrated[m].begin(), s.penetrated[m].end(), 0);
+        std::fill(s.fromCCD[m].begin(), s.fromCCD[m].end(), 0);
+        if (!s.col.dcd) continue;
+        const tetcol::Topo& T = s.topo[m];
+        auto body = [&](int v) {
+            if (!T.surf[v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
+            bool pen = false;
+            H[v] = tetcol::dcd_vertex<R>(s.topo, X, s.col, m, v, pen);
+            s.penetrated[m][v] = pen;
+        };
+        if (s.pool) s.pool->parallel_for(0, T.nv, body);
+        else for (int v = 0; v < T.nv; ++v) body(v);
+    }
+}
+
+template <class R> void detect_ccd(Scene<R>& s) {
+    const int nm = (int)s.meshes.size();
+    if (s.col.ccd) {
+        std::vector<const R*> X(nm), Xp(nm);
+        for (int m = 0; m < nm; ++m) {
+            X[m] = s.meshes[m].x.data();
+            Xp[m] = s.meshes[m].xprev.data();
+        }
+        for (int m = 0; m < nm; ++m) {
+            const tetcol::Topo& T = s.topo[m];
+            auto body = [&](int v) {
+                if (!T.surf[v] || s.penetrated[m][v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
+                s.hits[m][v] = tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v);
+                s.fromCCD[m][v] = s.hits[m][v].face >= 0;
+            };
+            if (s.pool) s.pool->parallel_for(0, T.nv, body);
+            else for (int v = 0; v < T.nv; ++v) body(v);
+        }
+    }
+    for (int m = 0; m < nm; ++m)
+        for (auto& r : s.relations[m]) r.clear();
+    for (int m = 0; m < nm; ++m)
+        for (int v = 0; v < s.topo[m].nv; ++v) {
+            const tetcol::Hit& hit = s.hits[m][v];
+            if (hit.face < 0) continue;
+            const int32_t* fv = &s.topo[hit.mesh].faces[3 * hit.face];
+            for (int k = 0; k < 3; ++k) s.relations[hit.mesh][fv[k]].push_back({m, v, k});
+        }
+}
+
 template <class R> void scene_step(Scene<R>& s, int nthreads) {
     const int nsub = (int)s.p[OP_NUM_SUBSTEPS];
     const int iters = (int)s.p[OP_ITERATIONS];
@@ -234,12 +330,14 @@ template <class R> void scene_step(Scene<R>& s, int nthreads) {
     const R rho = (R)s.p[OP_ACCELERATOR_RHO];
     for (int sub = 0; sub < nsub; ++sub) {
         s.clock += (float)s.dt;
+        if (s.collide) detect_dcd(s);
         for (auto& m : s.meshes) apply_initial_step(s, m);
         // runStepGPU (VBDPhysics.cpp:839-862): deformers move their (fixed) vertices after the
         // initial step
         for (const auto& D : s.deformers)
             for (size_t k = 0; k < D.meshes.size(); ++k)
                 oracle::deformer_apply<R>(D, (int)k, false, s.clock, (float)s.dt, s.meshes[D.meshes[k]].x.data());
+        if (s.collide) detect_ccd(s);
         // Chebyshev acceleration (runStepGPU, VBDPhysics.cpp:872-906): both histories start at
         // the initial guess
         R omega = R(1);
@@ -250,10 +348,12 @@ template <class R> void scene_step(Scene<R>& s, int nthreads) {
             const bool fr = it >= fsi;
             for (auto& grp : s.groups) {
                 const int n = (int)grp.size() / 2;
+                if (s.collide)
+                    for (size_t m = 0; m < s.meshes.size(); ++m) s.snap[m] = s.meshes[m].x;
                 auto body = [&](int i) {
                     Mesh<R>& m = s.meshes[grp[2 * i]];
                     int vid = grp[2 * i + 1];
-                    if (!m.fixed[vid]) vbd_vertex(s, m, vid, fr);
+                    if (!m.fixed[vid]) vbd_vertex(s, m, vid, fr, grp[2 * i]);
                 };
                 if (s.pool && nthreads > 1)
                     s.pool->parallel_for(0, n, body);
@@ -378,6 +478,8 @@ template <class R> static int add_mesh(Scene<R>& s, int nv, const double* pos, c
     m.noGravZone = mat[OM_NO_GRAV_ZONE] != 0;
     m.noGravThreshold = (R)mat[OM_NO_GRAV_THRESHOLD];
     m.maxVel = (R)mat[OM_MAX_VELOCITY];
+    m.frictionMu = (R)mat[OM_FRICTION_MU];
+    m.frictionEpsV = (R)mat[OM_FRICTION_EPS_V];
     m.dminv.resize(9 * nt);
     m.vol.resize(nt);
     m.mass.resize(nv);
@@ -447,6 +549,26 @@ template <class R> static int set_state(Scene<R>& s, int mesh, const double* pos
     return 0;
 }
 
+template <class R> static void finalize_scene(Scene<R>& s) {
+    build_groups(s);
+    if (!s.collide) return;
+    const size_t nm = s.meshes.size();
+    s.topo.assign(nm, {});
+    s.hits.assign(nm, {});
+    s.penetrated.assign(nm, {});
+    s.fromCCD.assign(nm, {});
+    s.relations.assign(nm, {});
+    s.snap.assign(nm, {});
+    for (size_t m = 0; m < nm; ++m) {
+        Mesh<R>& M = s.meshes[m];
+        tetcol::build_topo<R>(s.topo[m], M.nv, M.nt, M.tets.data(), M.x.data());
+        s.hits[m].assign(M.nv, tetcol::Hit{});
+        s.penetrated[m].assign(M.nv, 0);
+        s.fromCCD[m].assign(M.nv, 0);
+        s.relations[m].assign(M.nv, {});
+    }
+}
+
 extern "C" {
 
 int oracle_tet_prepare_f64(int nv, const double* pos, int nt, const int32_t* tets, double density, double* dminv,
@@ -492,9 +614,75 @@ int oracle_tet_scene_add_mesh(OracleTetScene* sc, int nv, const double* pos, con
 }
 
 int oracle_tet_scene_finalize(OracleTetScene* sc) {
-    if (sc->precision == 64) build_groups(sc->s64);
-    else build_groups(sc->s32);
+    if (sc->precision == 64) finalize_scene(sc->s64);
+    else finalize_scene(sc->s32);
     return 0;
+}
+
+int oracle_tet_scene_set_collision(OracleTetScene* sc, const double* c) {
+    auto set = [&](auto& s) {
+        s.collide = true;
+        tetcol::Params& P = s.col;
+        P.dcd = c[OC_ALLOW_DCD] != 0;
+        P.ccd = c[OC_ALLOW_CCD] != 0;
+        P.selfCollision = c[OC_SELF_COLLISION] != 0;
+        P.feasibleRegion = c[OC_FEASIBLE_REGION] != 0;
+        P.tetTraverse = c[OC_TET_TRAVERSE] != 0;
+        P.traverseNonSelf = c[OC_TRAVERSE_NON_SELF] != 0;
+        P.shiftToCenter = c[OC_SHIFT_TO_CENTER] != 0;
+        P.stopAfterPassing = c[OC_STOP_AFTER_PASSING] != 0;
+        P.stiffness = c[OC_STIFFNESS];
+        P.offHeight = c[OC_OFF_HEIGHT];
+        P.feasibleEps = c[OC_FEASIBLE_EPS];
+        P.rayEps = c[OC_RAY_EPS];
+        P.maxSearchMult = c[OC_MAX_SEARCH_MULT];
+        P.centerShift = c[OC_CENTER_SHIFT];
+    };
+    set(sc->s32);
+    set(sc->s64);
+    return 0;
+}
+
+// the last detection: per surface vertex of every mesh (mesh, vertex, hit mesh, hit face,
+// penetrated, from CCD); returns the number of rows (out may be NULL to count)
+int oracle_tet_scene_collisions(OracleTetScene* sc, int32_t* out) {
+    auto get = [&](auto& s) {
+        int n = 0;
+        for (size_t m = 0; m < s.topo.size(); ++m)
+            for (int v = 0; v < s.topo[m].nv; ++v) {
+                if (!s.topo[m].surf[v]) continue;
+                if (out) {
+                    int32_t* r = out + 6 * n;
+                    r[0] = (int32_t)m;
+                    r[1] = v;
+                    r[2] = s.hits[m][v].mesh;
+                    r[3] = s.hits[m][v].face;
+                    r[4] = s.penetrated[m][v];
+                    r[5] = s.fromCCD[m][v];
+                }
+                ++n;
+            }
+        return n;
+    };
+    return sc->precision == 64 ? get(sc->s64) : get(sc->s32);
+}
+
+// the boundary surface of mesh `mesh` as the collision model sees it: 3 ids per face (out may be
+// NULL), followed in owner (if not NULL) by the owning tet and local face of each face
+int oracle_tet_scene_surface(OracleTetScene* sc, int mesh, int32_t* faces, int32_t* owner) {
+    auto get = [&](auto& s) {
+        if (mesh < 0 || mesh >= (int)s.topo.size()) return -1;
+        const tetcol::Topo& T = s.topo[mesh];
+        const int nf = (int)T.faceTet.size();
+        if (faces) std::memcpy(faces, T.faces.data(), sizeof(int32_t) * 3 * nf);
+        if (owner)
+            for (int f = 0; f < nf; ++f) {
+                owner[2 * f] = T.faceTet[f];
+                owner[2 * f + 1] = T.faceLocal[f];
+            }
+        return nf;
+    };
+    return sc->precision == 64 ? get(sc->s64) : get(sc->s32);
 }
 
 int oracle_tet_scene_num_groups(OracleTetScene* sc) {
@@ -539,6 +727,35 @@ int oracle_tet_scene_evaluate_convergence(OracleTetScene* sc, double* out) {
     else
         evaluate_convergence(sc->s32, out);
     return 0;
+}
+int oracle_point_in_tet_f32(const float* p, const float* a, const float* b, const float* c, const float* d) {
+    using tetcol::ld;
+    return tetcol::point_in_tet<float>(ld(p, 0), ld(a, 0), ld(b, 0), ld(c, 0), ld(d, 0)) ? 1 : 0;
+}
+int oracle_ccd_point_triangle_f64(const double* f0, const double* f1, const double* p0, const double* p1, double* t,
+                                  double* bary) {
+    using tetcol::ld;
+    const tetcol::P3<double> a0[3] = {ld(f0, 0), ld(f0, 1), ld(f0, 2)};
+    const tetcol::P3<double> a1[3] = {ld(f1, 0), ld(f1, 1), ld(f1, 2)};
+    return tetcol::ccd_point_triangle<double>(a0, a1, ld(p0, 0), ld(p1, 0), *t, bary) ? 1 : 0;
+}
+int oracle_tet_walk_f64(int nv, const double* x, int nt, const int32_t* tets, const double* origin, const double* dir,
+                        double maxDis, int startTet, int startLocal, int target) {
+    tetcol::Topo T;
+    tetcol::build_topo<double>(T, nv, nt, tets, x);
+    using tetcol::ld;
+    return tetcol::walk<double>(T, x, ld(origin, 0), ld(dir, 0), maxDis, startTet, startLocal, target, 1e-10) ? 1 : 0;
+}
+void oracle_penetration_force_f64(const double* x4, const double* xp4, int order, double k, double mu, double epsV,
+                                  double dt, double* f, double* H) {
+    using tetcol::ld;
+    for (int i = 0; i < 3; ++i) f[i] = 0;
+    for (int i = 0; i < 9; ++i) H[i] = 0;
+    tetcol::penetration_force<double>(ld(x4, 0), ld(x4, 1), ld(x4, 2), ld(x4, 3), ld(xp4, 0), ld(xp4, 1), ld(xp4, 2),
+                                      ld(xp4, 3), order, k, mu, epsV, dt, f, H);
+}
+int oracle_cubic_roots01_f64(const double* coef, double* roots) {
+    return tetcol::cubic_roots01<double>(coef, roots, 6e-7);
 }
 int oracle_tet_scene_set_state(OracleTetScene* sc, int mesh, const double* pos, const double* vel) {
     return sc->precision == 64 ? set_state(sc->s64, mesh, pos, vel) : set_state(sc->s32, mesh, pos, vel);
