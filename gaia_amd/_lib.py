@@ -247,6 +247,7 @@ _SIGNATURES = {
     "gvbd_scene_get_time": [_P, _I, ctypes.POINTER(ctypes.c_float)],
     "gvbd_scene_set_time": [_P, _I, ctypes.c_float],
     "gvbd_scene_evaluate_convergence": [_P, _I] + [ctypes.POINTER(ctypes.c_double)] * 4,
+    "gvbd_scene_evaluate_convergence_noalloc": [_P, _I] + [ctypes.POINTER(ctypes.c_double)] * 4,
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
     "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],

@@ -5,6 +5,8 @@
 // the material (with damping) + inertia force. Per-block partial sums in double.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "device_types.h"
 #include "kernels.h"
 #include "vec_math.h"
@@ -121,9 +123,53 @@ __global__ __launch_bounds__(kBlock) void tet_energy_kernel(const TetEnergyRec* 
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// out[k] = partial[off[k]] + partial[off[k] + 1] + ... over n[k] entries, k = 0..2, summed in
+// ascending order by one thread (the order of the host-side sum of the allocating entry point,
+// so both give the same doubles)
+__global__ void final_sum_kernel(const double* __restrict__ partial, int3 off, int3 n, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int o[3] = {off.x, off.y, off.z}, c[3] = {n.x, n.y, n.z};
+    for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int b = 0; b < c[k]; ++b) s += partial[o[k] + b];
+        out[k] = s;
+    }
+}
+
+// Scratch of the allocation-free convergence evaluation: per-block partial sums, then the three
+// totals. One copy per GPU, held until the process ends; every call on a GPU shares it.
+static __device__ __align__(256) double scratch[kConvergenceScratchDoubles];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
 int convergence_blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+hipError_t convergence_scratch_init(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = (double*)g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_convergence_final_sum(const double* partial, int off0, int n0, int off1, int n1, int off2, int n2,
+                                        double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(64), 0, stream, partial, make_int3(off0, off1, off2),
+                       make_int3(n0, n1, n2), out);
+    return hipGetLastError();
+}
 
 hipError_t launch_convergence(const SceneView& view, int nslots, const TetEnergyRec* tets, int nt,
                               const int* tetVerts, int ntv, double* forcePartial, double* inertiaPartial,

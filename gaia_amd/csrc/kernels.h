@@ -57,6 +57,17 @@ hipError_t launch_convergence(const SceneView& view, int nslots, const TetEnergy
                               const int* tetVerts, int ntv, double* forcePartial, double* inertiaPartial,
                               double* energyPartial, hipStream_t stream);
 
+// Allocation-free evaluation: partials and totals live in a per-GPU file-scope device array of
+// kConvergenceScratchDoubles doubles (1 MiB). convergence_scratch_init(device) looks up its address
+// on `device`, which must be current, once per device; use only that address on the host side.
+// The last 3 doubles receive the totals of launch_convergence_final_sum.
+constexpr int kConvergenceScratchDoubles = 131072;
+constexpr int kMaxDevices = 64;
+hipError_t convergence_scratch_init(int device, double** scratch);
+// out[k] = sum of partial[off_k .. off_k + n_k) in ascending order, k = 0, 1, 2
+hipError_t launch_convergence_final_sum(const double* partial, int off0, int n0, int off1, int n1, int off2, int n2,
+                                        double* out, hipStream_t stream);
+
 // dst[i] = src[i] for n float4 elements (bandwidth / profiler calibration).
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 

@@ -207,6 +207,7 @@ struct gvbd_context {
     int numCUs = 0;
     hipStream_t stream = nullptr;
     int* dStatus = nullptr;  // set by work-queue kernels when a dependency wait exceeds its limit
+    double* convScratch = nullptr;  // this device's scratch of the allocation-free convergence evaluation
     hipEvent_t events[64] = {};
     std::string lastError;
     std::map<int32_t, std::unique_ptr<gvbd::SceneObj>> scenes;
@@ -1342,6 +1343,7 @@ int gvbd_context_create(int32_t device, gvbd_context** out) {
         for (auto& e : ctx->events) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipMalloc((void**)&ctx->dStatus, 256), "hipMalloc status");
         hip_check(hipMemset(ctx->dStatus, 0, 256), "status clear");
+        hip_check(gvbd::convergence_scratch_init(device, &ctx->convScratch), "convergence scratch address");
         *out = ctx.release();
     });
 }
@@ -1966,29 +1968,64 @@ int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stat
     });
 }
 
+namespace gvbd {
+namespace {
+
+// Sizes of one scene's convergence evaluation: tets, tet vertices, tet slots and the partial-sum
+// blocks of each kernel.
+struct ConvergenceSizes {
+    int64_t ntet = 0, ntetVerts = 0;
+    int nslots = 0, bs = 0, bv = 0, bt = 0;
+    size_t recBytes = 0, vertBytes = 0;
+};
+
+ConvergenceSizes convergence_sizes(const SceneObj& s) {
+    ConvergenceSizes z;
+    for (const TetMeshHost& m : s.tets) {
+        z.ntet += m.nt;
+        z.ntetVerts += m.nv;
+    }
+    z.nslots = (int)s.numTetSlots();
+    z.bs = convergence_blocks(std::max(z.nslots, 1));
+    z.bt = convergence_blocks((int)std::max<int64_t>(z.ntet, 1));
+    z.bv = convergence_blocks((int)std::max<int64_t>(z.ntetVerts, 1));
+    z.recBytes = sizeof(TetEnergyRec) * (size_t)std::max<int64_t>(z.ntet, 1);
+    z.vertBytes = (sizeof(int32_t) * (size_t)std::max<int64_t>(z.ntetVerts, 1) + 15) & ~(size_t)15;
+    return z;
+}
+
+void check_convergence_scene(const SceneObj& s) {
+    if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
+    if (!s.dev.arena) fail(GVBD_ERR_STATE, "scene is held in host memory by its server (not resident)");
+}
+
+// merit energy parts and the average force norm from the three totals (force, inertia, elastic)
+void convergence_outputs(const SceneObj& s, const ConvergenceSizes& z, double fsum, double isum, double esum,
+                         double* energy, double* inertia_energy, double* elastic_energy, double* avg_force_norm) {
+    const double eIn = 0.5 * (double)s.sp.dtSqrRecip * isum;
+    if (inertia_energy) *inertia_energy = eIn;
+    if (elastic_energy) *elastic_energy = esum;
+    if (energy) *energy = eIn + esum;
+    if (avg_force_norm) *avg_force_norm = z.ntetVerts ? fsum / (double)z.ntetVerts : 0.0;
+}
+
+}  // namespace
+}  // namespace gvbd
+
 int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
                                     double* elastic_energy, double* avg_force_norm) {
     return guard(ctx, [&] {
         require(ctx != nullptr, "ctx is NULL");
         SceneObj& s = scene_of(ctx, scene);
-        if (!s.initialized) fail(GVBD_ERR_STATE, "scene not initialised");
-        if (!s.dev.arena) fail(GVBD_ERR_STATE, "scene is held in host memory by its server (not resident)");
-        int64_t ntet = 0, ntetVerts = 0;
-        for (const TetMeshHost& m : s.tets) {
-            ntet += m.nt;
-            ntetVerts += m.nv;
-        }
-        const int nslots = (int)s.numTetSlots();
-        const int bs = convergence_blocks(std::max(nslots, 1)), bt = convergence_blocks((int)std::max<int64_t>(ntet, 1));
-        const int bv = convergence_blocks((int)std::max<int64_t>(ntetVerts, 1));
-        const size_t recBytes = sizeof(TetEnergyRec) * (size_t)std::max<int64_t>(ntet, 1);
-        const size_t vertBytes = (sizeof(int32_t) * (size_t)std::max<int64_t>(ntetVerts, 1) + 15) & ~(size_t)15;
-        const size_t need = recBytes + vertBytes + sizeof(double) * (size_t)(bs + bv + bt);
+        check_convergence_scene(s);
+        const ConvergenceSizes z = convergence_sizes(s);
+        const int bs = z.bs, bv = z.bv, bt = z.bt;
+        const size_t need = z.recBytes + z.vertBytes + sizeof(double) * (size_t)(bs + bv + bt);
         if (!s.convBuf) {
             // the tets of every tet mesh with scene vertex ids (built once)
             std::vector<TetEnergyRec> recs;
             std::vector<int32_t> verts;
-            recs.reserve((size_t)ntet);
+            recs.reserve((size_t)z.ntet);
             for (size_t im = 0; im < s.meshes.size(); ++im) {
                 if (s.meshes[im].first != MESH_TET) continue;
                 const TetMeshHost& m = s.tets[s.meshes[im].second];
@@ -2017,15 +2054,15 @@ int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* en
                                          hipMemcpyHostToDevice, ctx->stream),
                           "upload");
             if (!verts.empty())
-                hip_check(hipMemcpyAsync((char*)s.convBuf + recBytes, verts.data(), sizeof(int32_t) * verts.size(),
+                hip_check(hipMemcpyAsync((char*)s.convBuf + z.recBytes, verts.data(), sizeof(int32_t) * verts.size(),
                                          hipMemcpyHostToDevice, ctx->stream),
                           "upload");
             hip_check(hipStreamSynchronize(ctx->stream), "upload sync");
         }
         auto* recs = (const TetEnergyRec*)s.convBuf;
-        auto* tverts = (const int*)((char*)s.convBuf + recBytes);
-        auto* part = (double*)((char*)s.convBuf + recBytes + vertBytes);
-        hip_check(launch_convergence(s.view, nslots, recs, (int)ntet, tverts, (int)ntetVerts, part, part + bs,
+        auto* tverts = (const int*)((char*)s.convBuf + z.recBytes);
+        auto* part = (double*)((char*)s.convBuf + z.recBytes + z.vertBytes);
+        hip_check(launch_convergence(s.view, z.nslots, recs, (int)z.ntet, tverts, (int)z.ntetVerts, part, part + bs,
                                      part + bs + bv, ctx->stream),
                   "convergence launch");
         std::vector<double> h((size_t)(bs + bv + bt), 0.0);
@@ -2033,17 +2070,50 @@ int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* en
                   "download");
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         double fsum = 0.0, isum = 0.0, esum = 0.0;
-        if (nslots > 0)
+        if (z.nslots > 0)
             for (int b = 0; b < bs; ++b) fsum += h[b];
-        if (ntetVerts > 0)
+        if (z.ntetVerts > 0)
             for (int b = 0; b < bv; ++b) isum += h[bs + b];
-        if (ntet > 0)
+        if (z.ntet > 0)
             for (int b = 0; b < bt; ++b) esum += h[bs + bv + b];
-        const double eIn = 0.5 * (double)s.sp.dtSqrRecip * isum;
-        if (inertia_energy) *inertia_energy = eIn;
-        if (elastic_energy) *elastic_energy = esum;
-        if (energy) *energy = eIn + esum;
-        if (avg_force_norm) *avg_force_norm = ntetVerts ? fsum / (double)ntetVerts : 0.0;
+        convergence_outputs(s, z, fsum, isum, esum, energy, inertia_energy, elastic_energy, avg_force_norm);
+    });
+}
+
+int gvbd_scene_evaluate_convergence_noalloc(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
+                                            double* elastic_energy, double* avg_force_norm) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        SceneObj& s = scene_of(ctx, scene);
+        check_convergence_scene(s);
+        if (!s.convBuf)
+            fail(GVBD_ERR_STATE, "the scene's tet energy records are not resident: call "
+                                 "gvbd_scene_evaluate_convergence once at set-up (it allocates them)");
+        if (!ctx->convScratch) fail(GVBD_ERR_STATE, "convergence scratch address not set up for this device");
+        const ConvergenceSizes z = convergence_sizes(s);
+        // partials of the three kernels, then the three totals, within the fixed scratch
+        const int64_t needDoubles = (int64_t)z.bs + z.bv + z.bt + 3;
+        if (needDoubles > kConvergenceScratchDoubles)
+            fail(GVBD_ERR_INVALID, "scene too large for the allocation-free convergence evaluation: " +
+                                       std::to_string(needDoubles) + " scratch doubles needed, at most " +
+                                       std::to_string(kConvergenceScratchDoubles) + " (about " +
+                                       std::to_string((int64_t)(kConvergenceScratchDoubles - 3) * 256) +
+                                       " tets + tet vertices + free tet vertices)");
+        double* part = ctx->convScratch;
+        double* totals = ctx->convScratch + kConvergenceScratchDoubles - 3;
+        const auto* recs = (const TetEnergyRec*)s.convBuf;
+        const auto* tverts = (const int*)((const char*)s.convBuf + z.recBytes);
+        hip_check(launch_convergence(s.view, z.nslots, recs, (int)z.ntet, tverts, (int)z.ntetVerts, part,
+                                     part + z.bs, part + z.bs + z.bv, ctx->stream),
+                  "convergence launch");
+        // kernels that did not run (no slots, tets or tet vertices) contribute zero partials
+        hip_check(launch_convergence_final_sum(part, 0, z.nslots > 0 ? z.bs : 0, z.bs, z.ntetVerts > 0 ? z.bv : 0,
+                                               z.bs + z.bv, z.ntet > 0 ? z.bt : 0, totals, ctx->stream),
+                  "final sum launch");
+        double h[3] = {0.0, 0.0, 0.0};
+        hip_check(hipMemcpyAsync(h, totals, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "download");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        convergence_outputs(s, z, h[0], h[1], h[2], energy, inertia_energy, elastic_energy, avg_force_norm);
     });
 }
 

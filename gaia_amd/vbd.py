@@ -521,6 +521,14 @@ class VBDPhysics:
         self.ctx.check(lib().gvbd_scene_evaluate_convergence(self.ctx.handle, self.scene, *[ctypes.byref(o) for o in out]))
         return dict(zip(("energy", "inertia_energy", "elastic_energy", "avg_force_norm"), (o.value for o in out)))
 
+    def evaluateConvergenceNoAlloc(self):
+        """The same evaluation without device allocation (the library's fixed per-GPU scratch);
+        evaluateConvergence must have run once on this scene to make its energy records resident."""
+        out = [ctypes.c_double() for _ in range(4)]
+        self.ctx.check(lib().gvbd_scene_evaluate_convergence_noalloc(self.ctx.handle, self.scene,
+                                                                     *[ctypes.byref(o) for o in out]))
+        return dict(zip(("energy", "inertia_energy", "elastic_energy", "avg_force_norm"), (o.value for o in out)))
+
     def recordVBDSolveGraph(self, enable=True):
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))
 

@@ -331,6 +331,18 @@ GVBD_API int gvbd_scene_solve_failures(gvbd_context* ctx, int32_t scene, int32_t
  * solution; with the per-iteration hooks it reproduces the reference's per-iteration log. */
 GVBD_API int gvbd_scene_evaluate_convergence(gvbd_context* ctx, int32_t scene, double* energy, double* inertia_energy,
                                     double* elastic_energy, double* avg_force_norm);
+/* No counterpart in the reference. The same evaluation, with the same results to the bit, for
+ * callers that cannot allocate device memory (an evaluation inside an iteration loop that is
+ * captured or timed, a serving loop): it calls no hipMalloc / hipFree. Its per-block partial sums
+ * and totals live in a fixed per-GPU scratch array of the library (1 MiB, held until the process
+ * ends), so a scene may have at most about 33.5 million tets + tet vertices + free tet vertices
+ * (larger scenes get GVBD_ERR_INVALID). Precondition: the scene's tet energy records are resident,
+ * i.e. gvbd_scene_evaluate_convergence ran once on it at set-up (GVBD_ERR_STATE otherwise).
+ * Calls on one GPU share the scratch: calls that overlap in time on one GPU (several contexts or
+ * threads) race and must be serialised by the caller. */
+GVBD_API int gvbd_scene_evaluate_convergence_noalloc(gvbd_context* ctx, int32_t scene, double* energy,
+                                                     double* inertia_energy, double* elastic_energy,
+                                                     double* avg_force_norm);
 /* capture one frame (substeps x (initial step, iterations x groups, velocities)) into a HIP graph,
  * replayed by run_step(s); enable = 0 drops the graph and launches kernels directly */
 GVBD_API int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable);
