@@ -55,6 +55,22 @@ class PhysicsParams(ctypes.Structure):
         ("contact_bvh_rebuild_steps", ctypes.c_int32),
         ("use_accelerator", ctypes.c_int32),
         ("accelerator_rho", ctypes.c_float),
+        ("tet_contact_model", ctypes.c_int32),
+        ("allow_dcd", ctypes.c_int32),
+        ("allow_ccd", ctypes.c_int32),
+        ("handle_self_collision", ctypes.c_int32),
+        ("collision_stiffness", ctypes.c_float),
+        ("collision_off_height", ctypes.c_float),
+        ("check_feasible_region", ctypes.c_int32),
+        ("check_tet_traverse", ctypes.c_int32),
+        ("tet_traverse_non_self", ctypes.c_int32),
+        ("shift_query_point_to_center", ctypes.c_int32),
+        ("stop_traversing_after_passing", ctypes.c_int32),
+        ("feasible_region_epsilon", ctypes.c_float),
+        ("ray_tri_intersection_epsilon", ctypes.c_float),
+        ("max_search_distance_multiplier", ctypes.c_float),
+        ("center_shift_level", ctypes.c_float),
+        ("dcd_bvh_rebuild_steps", ctypes.c_int32),
     ]
 
 
@@ -248,6 +264,7 @@ _SIGNATURES = {
     "gvbd_scene_set_time": [_P, _I, ctypes.c_float],
     "gvbd_scene_evaluate_convergence": [_P, _I] + [ctypes.POINTER(ctypes.c_double)] * 4,
     "gvbd_scene_evaluate_convergence_noalloc": [_P, _I] + [ctypes.POINTER(ctypes.c_double)] * 4,
+    "gvbd_scene_penetration_results": [_P, _I, c_int32_p, c_int32_p, ctypes.POINTER(ctypes.c_uint8), c_int32_p],
     "gvbd_scene_record_graph": [_P, _I, _I],
     "gvbd_scene_get_mesh_state": [_P, _I, _I, c_float_p, c_float_p],
     "gvbd_scene_set_mesh_state": [_P, _I, _I, c_float_p, c_float_p],

@@ -53,6 +53,42 @@ struct DeviceBvh {
     size_t sortTempBytes = 0;
 };
 
+// Penetration contact of tet meshes (the reference tet solver's DCD and CCD,
+// CollisionDetertionParameters.h:43-80, VBDPhysicsParameters.h:70).
+struct PenetrationParams {
+    int dcd, ccd;              // allowDCD, allowCCD
+    int self;                  // handleSelfCollision
+    int feasible;              // checkFeasibleRegion
+    int traverse;              // checkTetTraverse
+    int traverseNonSelf;       // tetrahedralTraverseForNonSelfIntersection
+    int shiftToCenter;         // shiftQueryPointToCenter
+    int stopAfterPassing;      // stopTraversingAfterPassingQueryPoint
+    float offHeight;           // collisionOffHeight
+    float feasibleEps;         // feasibleRegionEpsilon
+    float rayEps;              // rayTriIntersectionEPSILON
+    float maxSearchMult;       // maxSearchDistanceMultiplier
+    float centerShift;         // centerShiftLevel
+};
+
+// The tets of a scene's tet meshes in scene ids (meshes in order, so a lower tet id means a
+// lower (mesh, tet)), with the boundary surface's topology: the detector's faces are the meshes'
+// boundary faces in the same order.
+struct PenetrationSetup {
+    std::vector<int32_t> tets;       // 4 per tet (scene vertex ids)
+    std::vector<int32_t> tetMesh;    // per tet
+    std::vector<int32_t> tetNbr;     // 4 per tet: scene tet across local face k (opposite vertex k), -1 outside
+    std::vector<int32_t> faceOwner;  // 2 per detector face: scene tet and local face
+    std::vector<int32_t> faceNbr;    // 3 per detector face: face across (v0 v1), (v1 v2), (v2 v0), -1 none
+    std::vector<int32_t> vNbrBegin, vNbr;  // per scene vertex: its neighbours over surface edges
+    std::vector<int32_t> surfVerts;  // boundary-surface vertices, ascending
+    PenetrationParams params{};
+};
+
+// counts of the last detection pass
+struct PenetrationCounts {
+    int dcdHits, penetratedNoFace, ccdHits, walkOverflows;
+};
+
 struct DetectorResultCounts {
     int64_t vfPairs, vfContacts, eeContacts, vfOverflow, eeOverflow;
 };
@@ -93,6 +129,19 @@ class ContactDetector {
     void buildFaceLinks(int cap);
     // positionsAtPrevCollisionDetection = x; clears the re-detection flag
     void snapshotPositions();
+    // Penetration contact of tet meshes instead of the VF / EE proximity queries (call before
+    // finalize). Per pass: penetrationDcd at the start of a substep, then penetrationCcd after the
+    // initial step with the substep's start positions, then buildFaceLinks. Hits are written as
+    // one VF record per vertex (face = the face the vertex is pushed out of), which the face links
+    // and the solve read.
+    void setPenetration(PenetrationSetup&& setup);
+    bool penetration() const { return pen_; }
+    // tree refit (full rebuild when `rebuild`) at the current positions, then DCD
+    void penetrationDcd(bool rebuild);
+    void penetrationCcd(const float4* prev, bool rebuild);
+    PenetrationCounts penetrationCounts();
+    // per vertex: hit face (-1 none) and flags (1 a tet contained it, 2 the hit came from CCD)
+    void penetrationResults(int32_t* face, uint8_t* flags);
     DetectorDeviceView deviceView() const;
     bool finalized() const { return ready_; }
 
@@ -125,8 +174,11 @@ class ContactDetector {
     hipStream_t stream() const { return stream_; }
 
   private:
-    void buildBvh(DeviceBvh& b, int kind);
-    void refitBvh(DeviceBvh& b, int kind, hipStream_t stream);
+    // kind 0 faces, 1 edges, 3 precomputed primitive boxes (`boxes`: lo, hi per primitive)
+    void buildBvh(DeviceBvh& b, int kind, const float4* boxes = nullptr);
+    void refitBvh(DeviceBvh& b, int kind, hipStream_t stream, const float4* boxes = nullptr);
+    void allocBvh(DeviceBvh& b, int n);
+    void setupPenetrationDevice();
     // the face-side and edge-side work of a pass are independent: side_ runs the edge side
     // between fork() (side_ waits for stream_) and join() (stream_ waits for side_)
     void fork();
@@ -197,6 +249,18 @@ class ContactDetector {
     size_t fvScanTempBytes_ = 0;
     float4* xAtCD_ = nullptr;
     int* flag_ = nullptr;
+    // penetration contact
+    bool pen_ = false;
+    PenetrationSetup penHost_;
+    int nt_ = 0, nSurf_ = 0;
+    int4 *penTets_ = nullptr, *penTetNbr_ = nullptr, *penFaceNbr_ = nullptr;
+    int *penTetMesh_ = nullptr, *penVNbrBegin_ = nullptr, *penVNbr_ = nullptr, *penSurf_ = nullptr;
+    int2* penFaceOwner_ = nullptr;
+    float4 *tetBoxes_ = nullptr, *sweptBoxes_ = nullptr;
+    uint8_t* penFlags_ = nullptr;
+    int* penCounts_ = nullptr;  // 4
+    DeviceBvh tb_, sb_;         // tets, faces over their path in a substep
+    bool sweptBuilt_ = false;
 };
 
 }  // namespace gvbd

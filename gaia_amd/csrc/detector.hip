@@ -216,6 +216,7 @@ struct CDView {
     const uint64_t* vOrder;  // vertices' sorted Morton keys: query vertices in that order (or NULL)
     const uint8_t* vQuery;   // per vertex: takes VF queries (NULL: every vertex)
     int* error;
+    const float4* boxes;     // kind 3 primitives: lo, hi per primitive
 };
 
 __device__ __forceinline__ bool box_overlap(V3 qlo, V3 qhi, const BvhNode& n) {
@@ -579,6 +580,10 @@ __device__ __forceinline__ V3 prim_centroid(const CDView& S, int kind, int i) {
         const int4 e = S.edges[i];
         return 0.5f * (ld(S.pos, e.x) + ld(S.pos, e.y));
     }
+    if (kind == 3) {
+        const float4 lo = S.boxes[2 * i], hi = S.boxes[2 * i + 1];
+        return 0.5f * (V3{lo.x, lo.y, lo.z} + V3{hi.x, hi.y, hi.z});
+    }
     return ld(S.pos, i);  // kind 2: vertices
 }
 
@@ -768,7 +773,11 @@ __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNo
     BvhNode& L = nodes[n - 1 + k];
     const int prim = __float_as_int(L.lo.w);
     V3 lo, hi;
-    if (kind == 0) {
+    if (kind == 3) {
+        const float4 a = S.boxes[2 * prim], b = S.boxes[2 * prim + 1];
+        lo = {a.x, a.y, a.z};
+        hi = {b.x, b.y, b.z};
+    } else if (kind == 0) {
         const int4 f = S.faces[prim];
         const V3 a = ld(S.pos, f.x), b = ld(S.pos, f.y), c = ld(S.pos, f.z);
         lo = {fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y), fminf(fminf(a.z, b.z), c.z)};
@@ -886,6 +895,519 @@ __global__ __launch_bounds__(256) void fv_select_kernel(const int* offsets, int 
         verts[best] = t;
     }
     range[f] = make_int2(b, b + m);
+}
+
+// ---- penetration contact of tet meshes (the reference tet solver's DCD and CCD) -------------
+//
+// The same float operations in the same order as the CPU restatement (oracle tet_collision.h),
+// so detections match it bit for bit: DCD finds the lowest tet containing a surface vertex
+// (strictly, |6V| >= CMP_EPSILON; CollisionGeometry.h:56-88), then the closest face of that
+// mesh's surface passing the self-vertex, feasible-region and tet-walk checks
+// (DiscreteCollisionDetector.cpp:97-300,697-815; TetMeshFEM.cpp:503-735); CCD finds the face a
+// surface vertex's path meets first (ContinuousCollisionDetector.cpp:67-190; CCDSolver.h).
+
+constexpr int kPenStack = 64;   // candidate exit faces of a walk
+constexpr int kPenRecent = 32;  // recently entered tets checked for loops
+// local face k of a tet is opposite its vertex k, vertices in this order (TetMeshFEM.cpp:22)
+__constant__ int kTetFace[4][3] = {{1, 2, 3}, {2, 0, 3}, {0, 1, 3}, {1, 0, 2}};
+
+struct PenView {
+    const float4* pos;
+    const float4* prev;      // positions at the start of the step (CCD)
+    const int4* faces;       // v0, v1, v2, mesh (detector faces = the meshes' boundary surfaces)
+    const int4* tets;        // scene vertex ids
+    const int* tetMesh;
+    const int4* tetNbr;      // per local face k, -1 on the surface
+    const int2* faceOwner;   // (tet, local face)
+    const int4* faceNbr;     // neighbour face across (v0 v1), (v1 v2), (v2 v0)
+    const int* vNbrBegin;    // surface-edge neighbours (CSR over scene vertices)
+    const int* vNbr;
+    const int* surfVerts;
+    int nSurf;
+    const int* vMesh;
+    const BvhNode* tNodes;   // tet tree
+    int tN;
+    const BvhNode* fNodes;   // face tree (current positions)
+    int fN;
+    const BvhNode* sNodes;   // swept-face tree (prev and current positions)
+    int sN;
+    PenetrationParams P;
+    int* stored;             // per vertex: 1 with a hit face (the VF record slot's count)
+    VFContactRec* rec;       // per vertex, cap records: rec[v * cap].face = hit face
+    int cap;
+    uint8_t* flags;          // per vertex: 1 a tet contained it, 2 the hit came from CCD
+    int* counts;             // [0] DCD hits, [1] penetrated without a face, [2] CCD hits, [3] walk overflows
+    int* error;
+};
+
+__device__ __forceinline__ V3 vcross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float vnorm(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 vnormalize(V3 a) { return (1.f / vnorm(a)) * a; }
+
+__device__ bool point_in_tet(V3 p, V3 a, V3 b, V3 c, V3 d) {
+    const float vol = dot(b - a, vcross(c - a, d - a));
+    if (fabsf(vol) < kCmpEpsilon) return false;
+    const V3 vs[4] = {a, b, c, d};
+    for (int i = 0; i < 4; ++i) {
+        const V3 v1 = vs[kTetFace[i][0]], v2 = vs[kTetFace[i][1]], v3 = vs[kTetFace[i][2]];
+        if (dot(p - v1, vcross(v2 - v1, v3 - v2)) * vol >= 0.f) return false;
+    }
+    return true;
+}
+
+__device__ V3 face_normal(const PenView& S, int f) {
+    const int4 fv = S.faces[f];
+    const V3 a = ld(S.pos, fv.x), b = ld(S.pos, fv.y), c = ld(S.pos, fv.z);
+    return vnormalize(vcross(b - a, c - a));
+}
+
+// DiscreteCollisionDetector.cpp:697-815 (edge 0 = v0 v1, 1 = v1 v2, 2 = v2 v0; no absolute relaxation)
+__device__ bool pen_feasible(const PenView& S, V3 p, int face, int type) {
+    if (type == AT_INTERIOR) return true;
+    const float eps = S.P.feasibleEps;
+    const int4 fv = S.faces[face];
+    if (type >= AT_AB) {
+        const int e = type == AT_AB ? 0 : (type == AT_BC ? 1 : 2);
+        const int i1 = e == 0 ? fv.x : (e == 1 ? fv.y : fv.z);
+        const int i2 = e == 0 ? fv.y : (e == 1 ? fv.z : fv.x);
+        const int4 nbs = S.faceNbr[face];
+        const int nb = e == 0 ? nbs.x : (e == 1 ? nbs.y : nbs.z);
+        if (nb < 0) return true;
+        const V3 v1 = ld(S.pos, i1), v2 = ld(S.pos, i2);
+        const V3 n1 = -1.f * face_normal(S, face), n2 = -1.f * face_normal(S, nb);
+        const V3 AP = p - v1, BP = p - v2, AB = v2 - v1, BA = -1.f * AB;
+        const float relaxed = -dot(AB, AB) * eps;
+        if (dot(AP, AB) < relaxed) return false;
+        if (dot(BP, BA) < relaxed) return false;
+        if (dot(AP, vcross(n1, AB)) < relaxed) return false;
+        if (dot(AP, vcross(n2, BA)) < relaxed) return false;
+        return true;
+    }
+    const int vid = type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z);
+    const V3 A = ld(S.pos, vid), AP = p - A;
+    for (int k = S.vNbrBegin[vid]; k < S.vNbrBegin[vid + 1]; ++k) {
+        const V3 BA = A - ld(S.pos, S.vNbr[k]);
+        if (dot(AP, BA) < -dot(BA, BA) * eps) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ float cross2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
+
+// The ray walk of the closest-point check (TetMeshFEM.cpp:503-735): from the surface face
+// (startTet, startLocal) along o + s dir through face-adjacent tets whose exit faces the ray may
+// cross (2D projection on a Duff et al. 2017 basis, epsilon rayEps), until the target tet is
+// entered (true), the surface is left, the crossing checked every 5 tets lies beyond maxDis, the
+// candidates run out or the candidate stack overflows (counted).
+__device__ bool pen_walk(const PenView& S, V3 o, V3 dir, float maxDis, int startTet, int startLocal, int target) {
+    if (startTet == target) return true;
+    const float sg = copysignf(1.f, dir.z);
+    const float ca = -1.f / (sg + dir.z);
+    const float cb = dir.x * dir.y * ca;
+    const V3 b1 = {1.f + sg * dir.x * dir.x * ca, sg * cb, -sg * dir.x};
+    const V3 b2 = {cb, sg + dir.y * dir.y * ca, -dir.y};
+    const float eps = S.P.rayEps;
+    float qx[4], qy[4];
+    int ids[4];
+    auto project = [&](int t, int in) {
+        const int4 tv = S.tets[t];
+        const int tvs[4] = {tv.x, tv.y, tv.z, tv.w};
+        for (int i = 0; i < 3; ++i) ids[i] = tvs[kTetFace[in][i]];
+        ids[3] = tvs[in];
+        for (int i = 0; i < 4; ++i) {
+            const V3 d = ld(S.pos, ids[i]) - o;
+            qx[i] = dot(b1, d);
+            qy[i] = dot(b2, d);
+        }
+    };
+    int ex[3];
+    auto exits = [&]() {
+        const float area = cross2(qx[1] - qx[0], qy[1] - qy[0], qx[2] - qx[0], qy[2] - qy[0]);
+        const float s = copysignf(1.f, area);
+        const float d0 = cross2(qx[3], qy[3], qx[0], qy[0]) * s;
+        const float d1 = cross2(qx[3], qy[3], qx[1], qy[1]) * s;
+        const float d2 = cross2(qx[3], qy[3], qx[2], qy[2]) * s;
+        ex[0] = d1 >= -eps && d2 <= eps;
+        ex[1] = d2 >= -eps && d0 <= eps;
+        ex[2] = d0 >= -eps && d1 <= eps;
+    };
+    int stackTet[kPenStack], stackFace[kPenStack], ns = 0;
+    int recent[kPenRecent], nrec = 0, head = 0;
+    project(startTet, startLocal);
+    exits();
+    recent[head] = startTet;
+    head = 1;
+    nrec = 1;
+    for (int i = 0; i < 3; ++i)
+        if (ex[i]) {
+            stackTet[ns] = startTet;
+            stackFace[ns++] = kTetFace[startLocal][i];
+        }
+    int steps = 0;
+    while (ns > 0) {
+        --ns;
+        const int prevT = stackTet[ns], face = stackFace[ns];
+        const int4 pn = S.tetNbr[prevT];
+        const int cur = face == 0 ? pn.x : (face == 1 ? pn.y : (face == 2 ? pn.z : pn.w));
+        if (cur == target) return true;
+        if (cur < 0) return false;
+        bool seen = false;
+        for (int i = 0; i < nrec; ++i) seen = seen || recent[i] == cur;
+        if (seen) continue;
+        recent[head] = cur;
+        head = (head + 1) % kPenRecent;
+        nrec = min(nrec + 1, kPenRecent);
+        const int4 ptv = S.tets[prevT];
+        const int ptvs[4] = {ptv.x, ptv.y, ptv.z, ptv.w};
+        const int fa = ptvs[kTetFace[face][0]], fb = ptvs[kTetFace[face][1]], fc = ptvs[kTetFace[face][2]];
+        const int4 ctv = S.tets[cur];
+        const int ctvs[4] = {ctv.x, ctv.y, ctv.z, ctv.w};
+        int in = 3;
+        for (int i = 0; i < 4; ++i)
+            if (ctvs[i] != fa && ctvs[i] != fb && ctvs[i] != fc) {
+                in = i;
+                break;
+            }
+        project(cur, in);
+        exits();
+        const int4 cn = S.tetNbr[cur];
+        const int cns[4] = {cn.x, cn.y, cn.z, cn.w};
+        for (int i = 0; i < 3; ++i) {
+            if (!ex[i]) continue;
+            const int exitLocal = kTetFace[in][i];
+            if (cns[exitLocal] == target) return true;
+            if (ns == kPenStack) {
+                atomicAdd(S.counts + 3, 1);
+                return false;
+            }
+            stackTet[ns] = cur;
+            stackFace[ns++] = exitLocal;
+            if (steps % 5 == 0 && maxDis > 0.f) {
+                // crossing with exit face i: the origin's barycentrics in its projection
+                const int e0 = i == 0 ? 1 : 0, e1 = i == 2 ? 1 : 2;
+                const float v0x = qx[e1] - qx[e0], v0y = qy[e1] - qy[e0], v1x = qx[3] - qx[e0], v1y = qy[3] - qy[e0];
+                const float v2x = -qx[e0], v2y = -qy[e0];
+                const float den = v0x * v1y - v1x * v0y;
+                const float l1 = (v2x * v1y - v1x * v2y) / den, l2 = (v0x * v2y - v2x * v0y) / den;
+                const float l0 = 1.f - l1 - l2;
+                const V3 hit = (l0 * ld(S.pos, ids[e0]) + l1 * ld(S.pos, ids[e1])) + l2 * ld(S.pos, ids[3]);
+                const V3 d = hit - o;
+                if (dot(d, d) > maxDis * maxDis) return false;
+            }
+        }
+        ++steps;
+    }
+    return false;
+}
+
+__device__ __forceinline__ float box_dist2(V3 p, const BvhNode& n) {
+    const float dx = fmaxf(fmaxf(n.lo.x - p.x, 0.f), p.x - n.hi.x);
+    const float dy = fmaxf(fmaxf(n.lo.y - p.y, 0.f), p.y - n.hi.y);
+    const float dz = fmaxf(fmaxf(n.lo.z - p.z, 0.f), p.z - n.hi.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// DCD of every surface vertex (one thread each) at the current positions
+__global__ __launch_bounds__(64) void pen_dcd_kernel(PenView S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.nSurf) return;
+    const int v = S.surfVerts[i];
+    S.stored[v] = 0;
+    S.flags[v] = 0;
+    const V3 p = ld(S.pos, v);
+    if (!S.P.dcd || !(p.y < S.P.offHeight)) return;
+    const int qm = S.vMesh[v];
+    // the lowest containing tet
+    int ht = -1;
+    {
+        const int firstLeaf = S.tN - 1;
+        int node = S.tN > 0 ? 0 : -1;
+        for (int visits = 0; node >= 0; ++visits) {
+            if (visits > 2 * S.tN) {
+                atomicOr(S.error, 1);
+                break;
+            }
+            const BvhNode nd = S.tNodes[node];
+            const int skip = __float_as_int(nd.hi.w);
+            if (!(p.x >= nd.lo.x && p.x <= nd.hi.x && p.y >= nd.lo.y && p.y <= nd.hi.y && p.z >= nd.lo.z &&
+                  p.z <= nd.hi.z)) {
+                node = skip;
+                continue;
+            }
+            if (node < firstLeaf) {
+                node = __float_as_int(nd.lo.w);
+                continue;
+            }
+            node = skip;
+            const int t = __float_as_int(nd.lo.w);
+            if (ht >= 0 && t > ht) continue;
+            const int m = S.tetMesh[t];
+            if (m == qm && !S.P.self) continue;
+            const int4 tv = S.tets[t];
+            if (m == qm && (tv.x == v || tv.y == v || tv.z == v || tv.w == v)) continue;
+            if (point_in_tet(p, ld(S.pos, tv.x), ld(S.pos, tv.y), ld(S.pos, tv.z), ld(S.pos, tv.w))) ht = t;
+        }
+    }
+    if (ht < 0) return;
+    S.flags[v] = 1;
+    const int hm = S.tetMesh[ht];
+    const float c = S.P.centerShift;
+    // the closest face of mesh hm passing the checks (lower face id on equal distance)
+    float best = FLT_MAX;
+    int bestF = -1;
+    const int firstLeaf = S.fN - 1;
+    int node = S.fN > 0 ? 0 : -1;
+    for (int visits = 0; node >= 0; ++visits) {
+        if (visits > 2 * S.fN) {
+            atomicOr(S.error, 1);
+            break;
+        }
+        const BvhNode nd = S.fNodes[node];
+        const int skip = __float_as_int(nd.hi.w);
+        // prune only boxes surely farther than the best distance (margin for rounding)
+        if (bestF >= 0 && box_dist2(p, nd) > best * best * 1.0001f + 1e-12f) {
+            node = skip;
+            continue;
+        }
+        if (node < firstLeaf) {
+            node = __float_as_int(nd.lo.w);
+            continue;
+        }
+        node = skip;
+        const int f = __float_as_int(nd.lo.w);
+        const int4 fv = S.faces[f];
+        if (fv.w != hm) continue;
+        int type;
+        const V3 a = ld(S.pos, fv.x), b = ld(S.pos, fv.y), cc = ld(S.pos, fv.z);
+        const V3 cp = closest_point_triangle(p, a, b, cc, type);
+        if (hm == qm && type <= AT_C && (type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z)) == v) continue;
+        const float d = vnorm(p - cp);
+        if (bestF >= 0 && !(d < best || (d == best && f < bestF))) continue;
+        if (S.P.feasible && !pen_feasible(S, p, f, type)) continue;
+        if (S.P.traverse && (hm == qm || S.P.traverseNonSelf)) {
+            const V3 from = type == AT_INTERIOR ? cp : (1.f - c) * cp + (c / 3.f) * ((a + b) + cc);
+            V3 to = p;
+            if (S.P.shiftToCenter) {
+                const int4 tv = S.tets[ht];
+                const V3 cen = 0.25f * (((ld(S.pos, tv.x) + ld(S.pos, tv.y)) + ld(S.pos, tv.z)) + ld(S.pos, tv.w));
+                to = (1.f - c) * p + c * cen;
+            }
+            V3 dir = to - from;
+            const float len = vnorm(dir);
+            dir = (1.f / len) * dir;
+            const float maxDis = S.P.stopAfterPassing ? S.P.maxSearchMult * len : -1.f;
+            const int2 own = S.faceOwner[f];
+            if (!pen_walk(S, from, dir, maxDis, own.x, own.y, ht)) continue;
+        }
+        best = d;
+        bestF = f;
+    }
+    if (bestF < 0) {
+        atomicAdd(S.counts + 1, 1);
+        return;
+    }
+    S.stored[v] = 1;
+    S.rec[(int64_t)v * S.cap].face = bestF;
+    atomicAdd(S.counts, 1);
+}
+
+__device__ __forceinline__ float poly3(const float c[4], float x) { return ((c[3] * x + c[2]) * x + c[1]) * x + c[0]; }
+__device__ __forceinline__ bool diff_sign(float a, float b) { return (a < 0.f) != (b < 0.f); }
+
+// cyPolynomial.h:383-460 (see the oracle): Newton from the midpoint, then a safeguarded bracket
+__device__ float bracketed_root(const float c[4], float x0, float x1, float y0, float err) {
+    auto dpoly = [&](float x) { return (3.f * c[3] * x + 2.f * c[2]) * x + c[1]; };
+    const float ep2 = 2.f * err;
+    float xr = (x0 + x1) / 2.f;
+    if (x1 - x0 <= ep2) return xr;
+    const float xr0 = xr;
+    for (int it = 0; it < 16; ++it) {
+        float xn = xr - poly3(c, xr) / dpoly(xr);
+        xn = fminf(fmaxf(xn, x0), x1);
+        if (fabsf(xr - xn) <= err) return xn;
+        xr = xn;
+    }
+    if (!isfinite(xr)) xr = xr0;
+    float yr = poly3(c, xr);
+    float xb0 = x0, xb1 = x1;
+    for (int guard = 0; guard < 200; ++guard) {
+        if (diff_sign(y0, yr)) xb1 = xr;
+        else xb0 = xr;
+        const float xn = xr - yr / dpoly(xr);
+        if (xn > xb0 && xn < xb1) {
+            const float step = fabsf(xr - xn);
+            xr = xn;
+            if (step > err) yr = poly3(c, xr);
+            else break;
+        } else {
+            xr = (xb0 + xb1) / 2.f;
+            if (xr == xb0 || xr == xb1 || xb1 - xb0 <= ep2) break;
+            yr = poly3(c, xr);
+        }
+    }
+    return xr;
+}
+
+__device__ int cubic_roots01(const float c[4], float roots[3], float err) {
+    float pts[4];
+    int np = 0;
+    pts[np++] = 0.f;
+    const float a = 3.f * c[3], b2 = c[2], cc = c[1];
+    const float disc = b2 * b2 - a * cc;
+    if (a != 0.f && disc > 0.f) {
+        const float sq = sqrtf(disc);
+        const float q = -(b2 + copysignf(sq, b2));
+        float r0 = q / a, r1 = cc / q;
+        if (r0 > r1) {
+            const float t = r0;
+            r0 = r1;
+            r1 = t;
+        }
+        if (r0 > 0.f && r0 < 1.f) pts[np++] = r0;
+        if (r1 > 0.f && r1 < 1.f && r1 != r0) pts[np++] = r1;
+    } else if (a == 0.f && b2 != 0.f) {
+        const float r = -cc / (2.f * b2);
+        if (r > 0.f && r < 1.f) pts[np++] = r;
+    }
+    pts[np++] = 1.f;
+    int n = 0;
+    float yprev = poly3(c, pts[0]);
+    for (int i = 1; i < np; ++i) {
+        const float y = poly3(c, pts[i]);
+        if (diff_sign(yprev, y)) roots[n++] = bracketed_root(c, pts[i - 1], pts[i], yprev, err);
+        yprev = y;
+    }
+    return n;
+}
+
+// CCDSolver.h:104-202: the earliest t in [0, 1] at which the moving point lies in the moving
+// triangle's plane and inside it
+__device__ bool ccd_point_triangle(const V3 f0[3], const V3 f1[3], V3 p0, V3 p1, float& tt) {
+    const V3 x0[4] = {f0[0], f0[1], f0[2], p0};
+    const V3 x1[4] = {f1[0], f1[1], f1[2], p1};
+    const V3 dx0 = x1[0] - x0[0], dx1 = x1[1] - x0[1], dx2 = x1[2] - x0[2], dx3 = x1[3] - x0[3];
+    const V3 e1 = x0[1] - x0[0], e2 = x0[2] - x0[0], e3 = x0[3] - x0[0];
+    const V3 de0 = dx3 - dx0, de1 = dx1 - dx0, de2 = dx2 - dx0;
+    const V3 e1xe2 = vcross(e1, e2), de1xde2 = vcross(de1, de2), mixed = vcross(e1, de2) + vcross(de1, e2);
+    const float c[4] = {dot(e3, e1xe2), dot(de0, e1xe2) + dot(e3, mixed), dot(e3, de1xde2) + dot(de0, mixed),
+                        dot(de0, de1xde2)};
+    float ts[3];
+    const int n = cubic_roots01(c, ts, 6e-4f);
+    for (int i = 0; i < n; ++i) {
+        const float t = ts[i];
+        V3 y3[4];
+        for (int k = 0; k < 4; ++k) y3[k] = x0[k] + t * (x1[k] - x0[k]);
+        const V3 nrm = vcross(y3[1] - y3[0], y3[2] - y3[0]);
+        const float ax = fabsf(nrm.x), ay = fabsf(nrm.y), az = fabsf(nrm.z);
+        const int drop = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+        float u[4], w[4];
+        for (int k = 0; k < 4; ++k) {
+            u[k] = drop == 0 ? y3[k].y : y3[k].x;
+            w[k] = drop == 2 ? y3[k].y : y3[k].z;
+        }
+        for (int k = 0; k < 3; ++k) {
+            u[k] -= u[3];
+            w[k] -= w[3];
+        }
+        const bool s01 = signbit(cross2(u[0], w[0], u[1], w[1]));
+        const bool s12 = signbit(cross2(u[1], w[1], u[2], w[2]));
+        const bool s20 = signbit(cross2(u[2], w[2], u[0], w[0]));
+        if (s01 == s12 && s01 == s20) {
+            tt = t;
+            return true;
+        }
+    }
+    return false;
+}
+
+// CCD of every surface vertex that no tet contained, on its path from prev to pos
+__global__ __launch_bounds__(256) void pen_ccd_kernel(PenView S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.nSurf) return;
+    const int v = S.surfVerts[i];
+    if (S.flags[v] & 1) return;
+    const V3 p1 = ld(S.pos, v);
+    if (!(p1.y < S.P.offHeight)) return;
+    const V3 p0 = ld(S.prev, v);
+    const int qm = S.vMesh[v];
+    const float len = vnorm(p1 - p0);
+    const V3 mid = 0.5f * (p0 + p1);
+    // boxes within the half path length of the midpoint (with a margin for the root's tolerance)
+    const float r = 0.5f * len + 1e-3f * len + 1e-6f;
+    const float r2 = r * r;
+    float best = 0.f;
+    int bestF = -1;
+    const int firstLeaf = S.sN - 1;
+    int node = S.sN > 0 ? 0 : -1;
+    for (int visits = 0; node >= 0; ++visits) {
+        if (visits > 2 * S.sN) {
+            atomicOr(S.error, 1);
+            break;
+        }
+        const BvhNode nd = S.sNodes[node];
+        const int skip = __float_as_int(nd.hi.w);
+        if (box_dist2(mid, nd) > r2) {
+            node = skip;
+            continue;
+        }
+        if (node < firstLeaf) {
+            node = __float_as_int(nd.lo.w);
+            continue;
+        }
+        node = skip;
+        const int f = __float_as_int(nd.lo.w);
+        const int4 fv = S.faces[f];
+        if (fv.w == qm && (fv.x == v || fv.y == v || fv.z == v)) continue;
+        const V3 a0[3] = {ld(S.prev, fv.x), ld(S.prev, fv.y), ld(S.prev, fv.z)};
+        const V3 a1[3] = {ld(S.pos, fv.x), ld(S.pos, fv.y), ld(S.pos, fv.z)};
+        float tt;
+        if (!ccd_point_triangle(a0, a1, p0, p1, tt)) continue;
+        const float depth = tt * len;
+        if (bestF < 0 || depth < best || (depth == best && f < bestF)) {
+            best = depth;
+            bestF = f;
+        }
+    }
+    if (bestF < 0) return;
+    S.stored[v] = 1;
+    S.rec[(int64_t)v * S.cap].face = bestF;
+    S.flags[v] = 2;
+    atomicAdd(S.counts + 2, 1);
+}
+
+// boxes of tets (kind 3 primitives): lo, hi per tet
+__global__ __launch_bounds__(256) void tet_box_kernel(const float4* pos, const int4* tets, int n, float4* boxes) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int4 tv = tets[t];
+    const V3 a = ld(pos, tv.x), b = ld(pos, tv.y), c = ld(pos, tv.z), d = ld(pos, tv.w);
+    boxes[2 * t] = make_float4(fminf(fminf(a.x, b.x), fminf(c.x, d.x)), fminf(fminf(a.y, b.y), fminf(c.y, d.y)),
+                               fminf(fminf(a.z, b.z), fminf(c.z, d.z)), 0.f);
+    boxes[2 * t + 1] = make_float4(fmaxf(fmaxf(a.x, b.x), fmaxf(c.x, d.x)), fmaxf(fmaxf(a.y, b.y), fmaxf(c.y, d.y)),
+                                   fmaxf(fmaxf(a.z, b.z), fmaxf(c.z, d.z)), 0.f);
+}
+
+// boxes of faces over their path from prev to pos (movingFaceBoundsFunc, ContinuousCollisionDetector.cpp:19-65)
+__global__ __launch_bounds__(256) void swept_box_kernel(const float4* pos, const float4* prev, const int4* faces, int n,
+                                                        float4* boxes) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const int4 fv = faces[f];
+    const V3 q[6] = {ld(pos, fv.x), ld(pos, fv.y), ld(pos, fv.z), ld(prev, fv.x), ld(prev, fv.y), ld(prev, fv.z)};
+    V3 lo = q[0], hi = q[0];
+    for (int k = 1; k < 6; ++k) {
+        lo = {fminf(lo.x, q[k].x), fminf(lo.y, q[k].y), fminf(lo.z, q[k].z)};
+        hi = {fmaxf(hi.x, q[k].x), fmaxf(hi.y, q[k].y), fmaxf(hi.z, q[k].z)};
+    }
+    boxes[2 * f] = make_float4(lo.x, lo.y, lo.z, 0.f);
+    boxes[2 * f + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+}
+
+__global__ __launch_bounds__(256) void fill_u8_kernel(uint8_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
 }
 
 inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
@@ -1152,32 +1674,36 @@ void ContactDetector::finalize() {
         vSortTemp_ = dalloc<char>(vSortTempBytes_);
     }
     // BVH storage
-    for (int kind = 0; kind < 2; ++kind) {
-        DeviceBvh& B = kind == 0 ? fb_ : eb_;
-        B.n = kind == 0 ? nf_ : ne_;
-        if (B.n == 0) continue;
-        B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
-        B.parent = dalloc<int>(2 * (size_t)B.n - 1);
-        B.right = dalloc<int>(std::max(B.n - 1, 1));
-        B.jump = dalloc<int>(4 * (size_t)std::max(B.n - 1, 1));
-        B.keys = dalloc<uint64_t>(B.n);
-        B.keysAlt = dalloc<uint64_t>(B.n);
-        B.depthKeys = dalloc<uint64_t>(std::max(B.n - 1, 1));
-        B.depthKeysAlt = dalloc<uint64_t>(std::max(B.n - 1, 1));
-        B.depth = dalloc<int>(std::max(B.n - 1, 1));
-        B.levelNodes = dalloc<int>(std::max(B.n - 1, 1));
-        B.dLevelBegin = dalloc<int>(kMaxDepth + 1);
-        size_t s1 = 0, s2 = 0;
-        hip_check(rocprim::radix_sort_keys(nullptr, s1, B.keys, B.keysAlt, (size_t)B.n, 0, 62, stream_), "sort size");
-        hip_check(rocprim::radix_sort_keys(nullptr, s2, B.depthKeys, B.depthKeysAlt, (size_t)std::max(B.n - 1, 1), 0,
-                                           40, stream_),
-                  "sort size");
-        B.sortTempBytes = std::max(s1, s2);
-        B.sortTemp = dalloc<char>(B.sortTempBytes);
-    }
+    allocBvh(fb_, nf_);
+    allocBvh(eb_, ne_);
+    if (pen_) setupPenetrationDevice();
     hip_check(hipStreamSynchronize(stream_), "detector upload sync");
     ready_ = true;
     updateBvh(true);
+    if (pen_) buildBvh(tb_, 3, tetBoxes_);
+}
+
+void ContactDetector::allocBvh(DeviceBvh& B, int n) {
+    B.n = n;
+    if (B.n == 0) return;
+    B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
+    B.parent = dalloc<int>(2 * (size_t)B.n - 1);
+    B.right = dalloc<int>(std::max(B.n - 1, 1));
+    B.jump = dalloc<int>(4 * (size_t)std::max(B.n - 1, 1));
+    B.keys = dalloc<uint64_t>(B.n);
+    B.keysAlt = dalloc<uint64_t>(B.n);
+    B.depthKeys = dalloc<uint64_t>(std::max(B.n - 1, 1));
+    B.depthKeysAlt = dalloc<uint64_t>(std::max(B.n - 1, 1));
+    B.depth = dalloc<int>(std::max(B.n - 1, 1));
+    B.levelNodes = dalloc<int>(std::max(B.n - 1, 1));
+    B.dLevelBegin = dalloc<int>(kMaxDepth + 1);
+    size_t s1 = 0, s2 = 0;
+    hip_check(rocprim::radix_sort_keys(nullptr, s1, B.keys, B.keysAlt, (size_t)B.n, 0, 62, stream_), "sort size");
+    hip_check(rocprim::radix_sort_keys(nullptr, s2, B.depthKeys, B.depthKeysAlt, (size_t)std::max(B.n - 1, 1), 0, 40,
+                                       stream_),
+              "sort size");
+    B.sortTempBytes = std::max(s1, s2);
+    B.sortTemp = dalloc<char>(B.sortTempBytes);
 }
 
 namespace {
@@ -1191,11 +1717,12 @@ CDView make_view(float4* pos, int nv, int nf, int ne) {
 }
 }  // namespace
 
-void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
+void ContactDetector::buildBvh(DeviceBvh& B, int kind, const float4* boxes) {
     if (B.n == 0) return;
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.faces = faces_;
     S.edges = edges_;
+    S.boxes = boxes;
     const int n = B.n;
     unsigned int* box = (unsigned int*)sceneBox_;
     hipLaunchKernelGGL(centroid_box_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, boxPartial_);
@@ -1264,14 +1791,15 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind) {
                                  hipMemcpyHostToDevice, stream_),
                   "level upload");
     }
-    refitBvh(B, kind, stream_);
+    refitBvh(B, kind, stream_, boxes);
 }
 
-void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream) {
+void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream, const float4* boxes) {
     if (B.n == 0) return;
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.faces = faces_;
     S.edges = edges_;
+    S.boxes = boxes;
     hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream, S, kind, B.nodes, B.n);
     for (const auto& seg : B.segments) {
         const int b = B.levelBegin[seg.first], e = B.levelBegin[seg.first + 1];
@@ -1283,6 +1811,160 @@ void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream) {
                                B.dLevelBegin, seg.first, seg.second, B.nodes, B.right);
     }
     hip_check(hipGetLastError(), "bvh refit launch");
+}
+
+void ContactDetector::setPenetration(PenetrationSetup&& setup) {
+    if (ready_) fail(GVBD_ERR_STATE, "detector already finalized");
+    require(setup.tets.size() % 4 == 0 && setup.tetNbr.size() == setup.tets.size() &&
+                setup.tetMesh.size() * 4 == setup.tets.size(),
+            "penetration setup: tet arrays do not match");
+    penHost_ = std::move(setup);
+    pen_ = true;
+}
+
+void ContactDetector::setupPenetrationDevice() {
+    const PenetrationSetup& H = penHost_;
+    nt_ = (int)H.tetMesh.size();
+    nSurf_ = (int)H.surfVerts.size();
+    require((int)H.faceOwner.size() == 2 * nf_ && (int)H.faceNbr.size() == 3 * nf_ &&
+                (int)H.vNbrBegin.size() == nv_ + 1,
+            "penetration setup: surface arrays do not match the detector's faces and vertices");
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "penetration upload");
+    };
+    penTets_ = dalloc<int4>(std::max(nt_, 1));
+    up(penTets_, H.tets.data(), sizeof(int32_t) * H.tets.size());
+    penTetNbr_ = dalloc<int4>(std::max(nt_, 1));
+    up(penTetNbr_, H.tetNbr.data(), sizeof(int32_t) * H.tetNbr.size());
+    penTetMesh_ = dalloc<int>(std::max(nt_, 1));
+    up(penTetMesh_, H.tetMesh.data(), sizeof(int32_t) * H.tetMesh.size());
+    penFaceOwner_ = dalloc<int2>(std::max(nf_, 1));
+    up(penFaceOwner_, H.faceOwner.data(), sizeof(int32_t) * H.faceOwner.size());
+    std::vector<int4> nb(nf_);
+    for (int f = 0; f < nf_; ++f) nb[f] = make_int4(H.faceNbr[3 * f], H.faceNbr[3 * f + 1], H.faceNbr[3 * f + 2], 0);
+    penFaceNbr_ = dalloc<int4>(std::max(nf_, 1));
+    up(penFaceNbr_, nb.data(), sizeof(int4) * nb.size());
+    penVNbrBegin_ = dalloc<int>(nv_ + 1);
+    up(penVNbrBegin_, H.vNbrBegin.data(), sizeof(int32_t) * H.vNbrBegin.size());
+    penVNbr_ = dalloc<int>(std::max<size_t>(H.vNbr.size(), 1));
+    up(penVNbr_, H.vNbr.data(), sizeof(int32_t) * H.vNbr.size());
+    penSurf_ = dalloc<int>(std::max(nSurf_, 1));
+    up(penSurf_, H.surfVerts.data(), sizeof(int32_t) * H.surfVerts.size());
+    tetBoxes_ = dalloc<float4>(2 * (size_t)std::max(nt_, 1));
+    sweptBoxes_ = dalloc<float4>(2 * (size_t)std::max(nf_, 1));
+    penFlags_ = dalloc<uint8_t>(std::max(nv_, 1));
+    penCounts_ = dalloc<int>(4);
+    hipLaunchKernelGGL(fill_u8_kernel, dim3(std::min(blocks_for(nv_), 4096)), dim3(256), 0, stream_, penFlags_,
+                       (int64_t)nv_);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)4, 0);
+    allocBvh(tb_, nt_);
+    allocBvh(sb_, nf_);
+    // tet boxes for the first tree build (at the end of finalize)
+    hipLaunchKernelGGL(tet_box_kernel, dim3(blocks_for(nt_)), dim3(256), 0, stream_, pos_, penTets_, nt_, tetBoxes_);
+    hip_check(hipGetLastError(), "penetration setup launch");
+}
+
+namespace {
+PenView pen_view(float4* pos, const float4* prev, const int4* faces) {
+    PenView v{};
+    v.pos = pos;
+    v.prev = prev;
+    v.faces = faces;
+    return v;
+}
+}  // namespace
+
+void ContactDetector::penetrationDcd(bool rebuild) {
+    if (!ready_ || !pen_) fail(GVBD_ERR_STATE, "detector not finalized for penetration contact");
+    hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nv_), 4096)), dim3(256), 0, stream_, vfStored_,
+                       (int64_t)nv_, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)4, 0);
+    hipLaunchKernelGGL(tet_box_kernel, dim3(blocks_for(nt_)), dim3(256), 0, stream_, pos_, penTets_, nt_, tetBoxes_);
+    if (rebuild) {
+        buildBvh(tb_, 3, tetBoxes_);
+        buildBvh(fb_, 0);
+    } else {
+        refitBvh(tb_, 3, stream_, tetBoxes_);
+        refitBvh(fb_, 0, stream_);
+    }
+    PenView S = pen_view(pos_, nullptr, faces_);
+    S.tets = penTets_;
+    S.tetMesh = penTetMesh_;
+    S.tetNbr = penTetNbr_;
+    S.faceOwner = penFaceOwner_;
+    S.faceNbr = penFaceNbr_;
+    S.vNbrBegin = penVNbrBegin_;
+    S.vNbr = penVNbr_;
+    S.surfVerts = penSurf_;
+    S.nSurf = nSurf_;
+    S.vMesh = vMesh_;
+    S.tNodes = tb_.nodes;
+    S.tN = tb_.n;
+    S.fNodes = fb_.nodes;
+    S.fN = fb_.n;
+    S.P = penHost_.params;
+    S.stored = vfStored_;
+    S.rec = vfRec_;
+    S.cap = vfCap_;
+    S.flags = penFlags_;
+    S.counts = penCounts_;
+    S.error = error_;
+    if (nSurf_ > 0)
+        hipLaunchKernelGGL(pen_dcd_kernel, dim3((nSurf_ + 63) / 64), dim3(64), 0, stream_, S);
+    hip_check(hipGetLastError(), "penetration DCD launch");
+}
+
+void ContactDetector::penetrationCcd(const float4* prev, bool rebuild) {
+    if (!ready_ || !pen_) fail(GVBD_ERR_STATE, "detector not finalized for penetration contact");
+    if (!penHost_.params.ccd || nf_ == 0) return;
+    hipLaunchKernelGGL(swept_box_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, pos_, prev, faces_, nf_,
+                       sweptBoxes_);
+    if (rebuild || !sweptBuilt_) {
+        buildBvh(sb_, 3, sweptBoxes_);
+        sweptBuilt_ = true;
+    } else {
+        refitBvh(sb_, 3, stream_, sweptBoxes_);
+    }
+    PenView S = pen_view(pos_, prev, faces_);
+    S.surfVerts = penSurf_;
+    S.nSurf = nSurf_;
+    S.vMesh = vMesh_;
+    S.sNodes = sb_.nodes;
+    S.sN = sb_.n;
+    S.P = penHost_.params;
+    S.stored = vfStored_;
+    S.rec = vfRec_;
+    S.cap = vfCap_;
+    S.flags = penFlags_;
+    S.counts = penCounts_;
+    S.error = error_;
+    if (nSurf_ > 0) hipLaunchKernelGGL(pen_ccd_kernel, dim3(blocks_for(nSurf_)), dim3(256), 0, stream_, S);
+    hip_check(hipGetLastError(), "penetration CCD launch");
+}
+
+PenetrationCounts ContactDetector::penetrationCounts() {
+    PenetrationCounts c{};
+    if (!pen_) return c;
+    int h[4];
+    hip_check(hipMemcpyAsync(h, penCounts_, sizeof(h), hipMemcpyDeviceToHost, stream_), "counts");
+    hip_check(hipStreamSynchronize(stream_), "counts sync");
+    c.dcdHits = h[0];
+    c.penetratedNoFace = h[1];
+    c.ccdHits = h[2];
+    c.walkOverflows = h[3];
+    return c;
+}
+
+void ContactDetector::penetrationResults(int32_t* face, uint8_t* flags) {
+    if (!pen_) fail(GVBD_ERR_STATE, "detector has no penetration contact");
+    std::vector<int> stored(nv_);
+    std::vector<VFContactRec> rec((size_t)nv_ * vfCap_);
+    hip_check(hipMemcpyAsync(stored.data(), vfStored_, sizeof(int) * nv_, hipMemcpyDeviceToHost, stream_), "download");
+    hip_check(hipMemcpyAsync(rec.data(), vfRec_, sizeof(VFContactRec) * rec.size(), hipMemcpyDeviceToHost, stream_),
+              "download");
+    if (flags) hip_check(hipMemcpyAsync(flags, penFlags_, nv_, hipMemcpyDeviceToHost, stream_), "download");
+    hip_check(hipStreamSynchronize(stream_), "download sync");
+    for (int v = 0; v < nv_; ++v) face[v] = stored[v] ? rec[(size_t)v * vfCap_].face : -1;
 }
 
 void ContactDetector::updateBvh(bool rebuild) {

@@ -169,6 +169,11 @@ struct ContactView {
     // positionsNext + copy gives); vgroup: per vertex parallel group (-1: not solved)
     const float4* snap;
     const int* vgroup;
+    // 0: cloth contact model (VF / FV / EE repulsion, bounds and truncation); 1: penetration
+    // contact of tet meshes (the vertex's own hit face in its first VF record, the hits on its
+    // incident faces in the FV lists; force k d n + friction, no EE, no bounds)
+    int model;
+    float penStiffness;  // collisionStiffness of the penetration model
 };
 
 // Device view of one scene, referenced by work items of a (batched) launch.

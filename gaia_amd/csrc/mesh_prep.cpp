@@ -361,7 +361,7 @@ bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<T
 }
 
 bool build_tet_surface(int32_t nv, int32_t nt, const int32_t* tets, const float* pos, std::vector<int32_t>& faces,
-                       std::string& err) {
+                       std::string& err, std::vector<int32_t>* owner, std::vector<int32_t>* tetNbr) {
     static const int kOpp[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
     // (sorted vertex triple, 4 t + k) for every tet face; a triple seen once is on the boundary
     struct Key {
@@ -389,18 +389,28 @@ bool build_tet_surface(int32_t nv, int32_t nt, const int32_t* tets, const float*
         return p.at < q.at;
     });
     std::vector<int64_t> boundary;
+    if (tetNbr) tetNbr->assign(4 * (size_t)nt, -1);
     for (size_t i = 0; i < keys.size();) {
         size_t j = i + 1;
         while (j < keys.size() && keys[j].a == keys[i].a && keys[j].b == keys[i].b && keys[j].c == keys[i].c) ++j;
         if (j - i == 1) boundary.push_back(keys[i].at);
+        if (j - i == 2 && tetNbr) {
+            (*tetNbr)[keys[i].at] = (int32_t)(keys[i + 1].at / 4);
+            (*tetNbr)[keys[i + 1].at] = (int32_t)(keys[i].at / 4);
+        }
         i = j;
     }
     std::sort(boundary.begin(), boundary.end());
     faces.clear();
     faces.reserve(3 * boundary.size());
+    if (owner) owner->clear();
     for (int64_t at : boundary) {
         const int64_t t = at / 4;
         const int k = (int)(at % 4);
+        if (owner) {
+            owner->push_back((int32_t)t);
+            owner->push_back(k);
+        }
         int32_t a = tets[4 * t + kOpp[k][0]], b = tets[4 * t + kOpp[k][1]], c = tets[4 * t + kOpp[k][2]];
         const int32_t o = tets[4 * t + k];
         float e1[3], e2[3], w[3], n[3];

@@ -135,9 +135,11 @@ bool build_tri_edges(int32_t nv, int32_t nf, const int32_t* faces, std::vector<T
 
 // Boundary surface of a tet mesh: the faces of exactly one tet, in (tet, face) order (face k of
 // a tet leaves out its vertex k), each oriented with its normal away from that vertex. Used as
-// the mesh's contact surface in scenes with collision handling.
+// the mesh's contact surface in scenes with collision handling. Optionally also the owning tet
+// and local face of every boundary face (2 per face) and, per tet and local face k, the tet
+// sharing that face (4 per tet, -1 on the boundary).
 bool build_tet_surface(int32_t nv, int32_t nt, const int32_t* tets, const float* pos, std::vector<int32_t>& faces,
-                       std::string& err);
+                       std::string& err, std::vector<int32_t>* owner = nullptr, std::vector<int32_t>* tetNbr = nullptr);
 // TriMeshVertexGraph.h:20-45: mesh edges; with bending also the two vertices opposite each
 // interior edge.
 Graph tri_vertex_graph(int32_t nv, int32_t nf, const int32_t* faces, bool withBending);

@@ -144,6 +144,24 @@ gvbd_physics_params default_physics_params() {
     p.contact_bvh_rebuild_steps = 32;
     p.use_accelerator = 0;
     p.accelerator_rho = 0.93f;
+    // penetration contact of tet meshes (CollisionDetertionParameters.h:43-80, VBDPhysicsParameters.h:36,70,
+    // PhysicsParameters.h:81)
+    p.tet_contact_model = 0;
+    p.allow_dcd = 1;
+    p.allow_ccd = 1;
+    p.handle_self_collision = 1;
+    p.collision_stiffness = 1e5f;
+    p.collision_off_height = 1e7f;
+    p.check_feasible_region = 1;
+    p.check_tet_traverse = 1;
+    p.tet_traverse_non_self = 1;
+    p.shift_query_point_to_center = 1;
+    p.stop_traversing_after_passing = 1;
+    p.feasible_region_epsilon = 1e-2f;
+    p.ray_tri_intersection_epsilon = 1e-10f;
+    p.max_search_distance_multiplier = 1.8f;
+    p.center_shift_level = 0.01f;
+    p.dcd_bvh_rebuild_steps = 16;
     return p;
 }
 
@@ -194,6 +212,48 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
         get_int(*pp, "useBowlGround", useBowl);
         if (useBowl) {
             err = "useBowlGround is not supported by this build (plane ground only)";
+            return false;
+        }
+    }
+    // the tet framework's collision detection (CollisionDetectionParamters::fromJson,
+    // CollisionDetector/CollisionDetertionParameters.h:82-120; BasePhysicFramework::initializeCollisionDetector,
+    // Framework/BasePhysicsFramework.cpp:251-262) and its solver constants (VBDPhysicsParameters.h:120-147)
+    bool collisionParamsGiven = false;
+    if (const json::Value* cp = params.get("CollisionParams")) {
+        collisionParamsGiven = true;
+        get_int(*cp, "allowDCD", p.allow_dcd);
+        get_int(*cp, "allowCCD", p.allow_ccd);
+        get_int(*cp, "handleSelfCollision", p.handle_self_collision);
+        get_int(*cp, "checkFeasibleRegion", p.check_feasible_region);
+        get_int(*cp, "checkTetTraverse", p.check_tet_traverse);
+        get_int(*cp, "tetrahedralTraverseForNonSelfIntersection", p.tet_traverse_non_self);
+        get_int(*cp, "shiftQueryPointToCenter", p.shift_query_point_to_center);
+        get_int(*cp, "stopTraversingAfterPassingQueryPoint", p.stop_traversing_after_passing);
+        get_num(*cp, "feasibleRegionEpsilon", p.feasible_region_epsilon);
+        get_num(*cp, "rayTriIntersectionEPSILON", p.ray_tri_intersection_epsilon);
+        get_num(*cp, "maxSearchDistanceMultiplier", p.max_search_distance_multiplier);
+        get_num(*cp, "centerShiftLevel", p.center_shift_level);
+        int32_t restPose = 0, loopLess = 0, volumetric = 0, edgeCCD = 0, maxCollisions = 1;
+        get_int(*cp, "restPoseCloestPoint", restPose);
+        get_int(*cp, "loopLessTraverse", loopLess);
+        get_int(*cp, "allowVolumetricCollision", volumetric);
+        get_int(*cp, "doEdgeEdgeCCD", edgeCCD);
+        get_int(*cp, "maxNumberOfCollisions", maxCollisions);
+        if (restPose || loopLess || volumetric || edgeCCD || maxCollisions != 1) {
+            err = "CollisionParams: restPoseCloestPoint, loopLessTraverse, allowVolumetricCollision, doEdgeEdgeCCD "
+                  "and maxNumberOfCollisions other than 1 are not supported by this build";
+            return false;
+        }
+    }
+    if (const json::Value* pp = params.get("PhysicsParams")) {
+        get_num(*pp, "collisionStiffness", p.collision_stiffness);
+        get_num(*pp, "collisionOffHeight", p.collision_off_height);
+        get_int(*pp, "dcdTetMeshSceneBVHRebuildSteps", p.dcd_bvh_rebuild_steps);
+        // collisionSolutionType selects between the CPU paths; the device path (runStepGPU) has one
+        int32_t intermediate = -1;
+        get_int(*pp, "intermediateCollisionIterations", intermediate);
+        if (intermediate > 0) {
+            err = "intermediateCollisionIterations is not supported by this build (detection once per substep)";
             return false;
         }
     }
@@ -365,9 +425,15 @@ bool load_scene_json(const std::string& modelsPath, const std::string& paramsPat
             return false;
         }
     }
-    // handleCollision defaults to true in the cloth framework (VBDClothPhysicsParameters.h:60) and to
-    // false in the tet framework (VBDPhysicsParameters.h:35)
-    if (!handleCollisionGiven && out.cloths.empty()) p.handle_collision = 0;
+    // handleCollision defaults to true in the cloth framework (VBDClothPhysicsParameters.h:60). The
+    // tet framework detects through its CollisionParams (DCD and CCD, both on by default): a tet-only
+    // scene with a CollisionParams section, or with handleCollision set, takes the penetration
+    // contact model; without either it runs without collision detection
+    if (out.cloths.empty()) {
+        if (collisionParamsGiven) p.handle_collision = (p.allow_dcd || p.allow_ccd) ? 1 : 0;
+        else if (!handleCollisionGiven) p.handle_collision = 0;
+        if (p.handle_collision) p.tet_contact_model = 1;
+    }
     return true;
 }
 

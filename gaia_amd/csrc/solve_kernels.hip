@@ -271,7 +271,7 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         }
         xn = x + step * dx;
     }
-    if constexpr (CONTACT) {
+    if (CONTACT && S.cv.model == 0) {
         // applyConservativeBoundTruncation (VBDClothPhysics.cpp:1223-1246). The tet initial
         // guess is not bounded, so every vertex is truncated, moved or not (interior vertices
         // have an unbounded step).
@@ -296,54 +296,55 @@ __device__ __forceinline__ f3 normalized(f3 v) {
     return n > 0.f ? (1.f / n) * v : v;
 }
 
-// CollisionGeometry.h:90-157 with barycentrics
-__device__ void closest_point_bary(f3 p, f3 a, f3 b, f3 c, f3& cp, f3& bary) {
+// CollisionGeometry.h:90-157 with barycentrics; true when the closest point is interior to the face
+__device__ bool closest_point_bary(f3 p, f3 a, f3 b, f3 c, f3& cp, f3& bary) {
     const f3 ab = b - a, ac = c - a, ap = p - a;
     const float d1 = dot(ab, ap), d2 = dot(ac, ap);
     if (d1 <= 0.f && d2 <= 0.f) {
         cp = a;
         bary = mk(1.f, 0.f, 0.f);
-        return;
+        return false;
     }
     const f3 bp = p - b;
     const float d3 = dot(ab, bp), d4 = dot(ac, bp);
     if (d3 >= 0.f && d4 <= d3) {
         cp = b;
         bary = mk(0.f, 1.f, 0.f);
-        return;
+        return false;
     }
     const f3 cpv = p - c;
     const float d5 = dot(ab, cpv), d6 = dot(ac, cpv);
     if (d6 >= 0.f && d5 <= d6) {
         cp = c;
         bary = mk(0.f, 0.f, 1.f);
-        return;
+        return false;
     }
     const float vc = d1 * d4 - d3 * d2;
     if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
         const float v = d1 / (d1 - d3);
         cp = a + v * ab;
         bary = mk(1.f - v, v, 0.f);
-        return;
+        return false;
     }
     const float vb = d5 * d2 - d1 * d6;
     if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
         const float v = d2 / (d2 - d6);
         cp = a + v * ac;
         bary = mk(1.f - v, 0.f, v);
-        return;
+        return false;
     }
     const float va = d3 * d6 - d5 * d4;
     if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
         const float v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         cp = b + v * (c - b);
         bary = mk(0.f, 1.f - v, v);
-        return;
+        return false;
     }
     const float denom = 1.f / (va + vb + vc);
     const float v = vb * denom, w = vc * denom;
     cp = (a + v * ab) + w * ac;
     bary = mk(1.f - v - w, v, w);
+    return true;
 }
 
 // CollisionGeometry.h:160-267 (absolute CMP_EPSILON on the determinant, as the reference)
@@ -511,6 +512,39 @@ __device__ void ee_contact(const SceneView& S, int e1, int e2, int order, int g,
     contact_apply(S.sp.dt, w, -dEdD, d2E, n, dx, v1, (fa.x + fb.x) * 0.5f, (fa.y + fb.y) * 0.5f, f, h);
 }
 
+// One penetration contact of the tet solver (VBDPhysicsCompute.cu:1831-1923): vertex u pushed out
+// of face `face`, evaluated on the vertex of order `order` (3: u, weight 1; 0..2: a face vertex,
+// weight -bary). The closest point is recomputed from the current positions; the normal is the
+// face normal at an interior closest point, else the direction to the closest point turned to
+// the face normal's side; with depth d = (c - x) . n > 0: force k d n and Hessian k n n^T plus
+// friction (the contact_apply frame, t0 = b - a), times the weight (its square for the Hessian).
+__device__ void pen_contact(const SceneView& S, int u, int face, int order, int g, f3& f, Sym3& h) {
+    const ContactView& C = S.cv;
+    const int4 fv = C.faces[face];
+    const f3 x = cpos(S, u, g);
+    const f3 a = cpos(S, fv.x, g), b = cpos(S, fv.y, g), c = cpos(S, fv.z, g);
+    f3 cp, bary;
+    const bool interior = closest_point_bary(x, a, b, c, cp, bary);
+    const f3 fn = normalized(cross(b - a, c - a));
+    const f3 diff = cp - x;
+    f3 n = fn;
+    if (!interior) {
+        n = (1.f / sqrtf(dot(diff, diff))) * diff;
+        if (dot(n, fn) < 0.f) n = -1.f * n;
+    }
+    const float depth = dot(diff, n);
+    if (!(depth > 0.f)) return;
+    const float w = order == 3 ? 1.f : -(order == 0 ? bary.x : (order == 1 ? bary.y : bary.z));
+    const float k = C.penStiffness;
+    // relative displacement over the step as the reference forms it (VBDPhysicsCompute.cu:1870-1887):
+    // from each vertex's own displacement, so that vertices at rest in a direction give exactly zero
+    // there (the friction's Hessian switches on at |u| > 0)
+    const f3 dx = (x - xyz(S.prev[u])) - ((bary.x * (a - xyz(S.prev[fv.x])) + bary.y * (b - xyz(S.prev[fv.y]))) +
+                                          bary.z * (c - xyz(S.prev[fv.z])));
+    const float2 fa = C.friction[fv.x], fb = C.friction[u];
+    contact_apply(S.sp.dt, w, depth * k, k, n, dx, b - a, (fa.x + fb.x) * 0.5f, (fa.y + fb.y) * 0.5f, f, h);
+}
+
 // VBDClothPhysics.cpp:1033-1094: the vertex's own VF contacts, the FV contacts of its incident
 // faces (faceRef[fb, fe)) and the EE contacts of its incident edges (edgeRef[eb, ee))
 // A vertex's contact pairs (VBDClothPhysics.cpp:1033-1094): its own VF list, the vertex lists of
@@ -524,6 +558,20 @@ __device__ void contact_forces(const SceneView& S, int v, int sub, const int* fa
     const ContactView& C = S.cv;
     const int g = C.vgroup[v];
     const int nvf = C.vfCount[v];
+    if (C.model == 1) {
+        // penetration contact: the vertex's own hit, then the hits on its incident faces
+        for (int k = sub; k < nvf; k += LPV) pen_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, g, f, h);
+        int t = sub, base = 0;
+        for (int r = fb; r < fe; ++r) {
+            const int ref = faceRef[r];
+            const int face = ref & 0x3FFFFFFF, order = (ref >> 30) & 3;
+            const int2 range = C.fvRange[face];
+            const int end = base + (range.y - range.x);
+            for (; t < end; t += LPV) pen_contact(S, C.fvVerts[range.x + (t - base)], face, order, g, f, h);
+            base = end;
+        }
+        return;
+    }
     for (int k = sub; k < nvf; k += LPV) vf_contact(S, v, C.vfRec[(int64_t)v * C.vfCap + k].x, 3, g, f, h);
     int t = sub, base = 0;
     for (int r = fb; r < fe; ++r) {

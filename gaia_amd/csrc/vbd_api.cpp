@@ -76,6 +76,7 @@ struct DeformerHost {
 // vertex's order in each.
 struct TetSurface {
     std::vector<int32_t> faces;  // 3 per face, outward
+    std::vector<int32_t> owner, tetNbr;  // penetration contact: owning (tet, local face); tet across each local face
     std::vector<TriEdge> edges;
     std::vector<int32_t> vfBegin, vfRef;  // vertex -> (face | order << 30)
     std::vector<int32_t> veBegin, veRef;  // vertex -> (edge | order << 30), ascending edge id
@@ -108,7 +109,8 @@ struct SceneObj {
     gvbd_contact_stats contactStats{};       // gvbd_scene_contact_stats
     void* convBuf = nullptr;                 // convergence evaluation: tet records + partial sums
     size_t convBytes = 0;
-    hipEvent_t cdEvents[2] = {nullptr, nullptr};  // around a frame's detection pass
+    hipEvent_t cdEvents[4] = {nullptr, nullptr, nullptr, nullptr};  // around a frame's detection passes
+    bool detectPending = false;  // penetration frames: cdEvents hold a detection not yet timed
     // serving loop residency: the server that owns it (-1: none) and, while copied out to host
     // memory, the arena's bytes and the device address they were copied from
     int32_t server = -1;
@@ -1062,11 +1064,57 @@ void plan_contact_frame(gvbd_context* ctx, PlanObj& p) {
     ++s.frame;
 }
 
+// Penetration frames time their detection with events read one substep later (or when the
+// stats are read), so the frame itself has no host wait.
+void collect_detection_time(SceneObj& s) {
+    if (!s.detectPending) return;
+    float a = 0.f, b = 0.f;
+    hip_check(hipEventSynchronize(s.cdEvents[3]), "event sync");
+    hip_check(hipEventElapsedTime(&a, s.cdEvents[0], s.cdEvents[1]), "event time");
+    hip_check(hipEventElapsedTime(&b, s.cdEvents[2], s.cdEvents[3]), "event time");
+    s.contactStats.detect_ms += a + b;
+    s.detectPending = false;
+}
+
+// One frame of a tet scene with penetration contact (runStepGPU, VBDPhysics.cpp:839-969): per
+// substep DCD at its start (the trees rebuilt at substep 0 of every dcd_bvh_rebuild_steps-th
+// frame, VBDPhysics.cpp:1962-1963, refit otherwise), the initial step, CCD over the vertices'
+// paths from their start positions, the face links, then the iterations and the velocities. All
+// on the device: no host reads inside the frame.
+void plan_penetration_frame(gvbd_context* ctx, PlanObj& p) {
+    SceneObj& s = scene_of(ctx, p.scenes[0]);
+    const int steps = s.params.dcd_bvh_rebuild_steps;
+    for (int sub = 0; sub < p.substeps; ++sub) {
+        const bool rebuild = sub == 0 && steps > 0 && s.frame > 0 && s.frame % steps == 0;
+        collect_detection_time(s);
+        for (hipEvent_t& e : s.cdEvents)
+            if (!e) hip_check(hipEventCreate(&e), "event create");
+        hip_check(hipEventRecord(s.cdEvents[0], ctx->stream), "event record");
+        s.cd->penetrationDcd(rebuild);
+        hip_check(hipEventRecord(s.cdEvents[1], ctx->stream), "event record");
+        plan_initial_step(ctx, p);
+        hip_check(hipEventRecord(s.cdEvents[2], ctx->stream), "event record");
+        s.cd->penetrationCcd(s.view.prev, rebuild);
+        s.cd->buildFaceLinks(32);
+        hip_check(hipEventRecord(s.cdEvents[3], ctx->stream), "event record");
+        s.detectPending = true;
+        for (int it = 0; it < p.iterations; ++it) plan_iteration(ctx, p, it, it);
+        plan_velocity(ctx, p);
+        ++s.contactStats.detections;
+        if (rebuild) ++s.contactStats.rebuilds;
+        ++s.contactStats.frames;
+        s.contactStats.iterations += p.iterations;
+    }
+    ++s.frame;
+}
+
 // One frame: substeps x (initial step, iterations x groups, velocity update)
 // (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
     if (p.contact) {
-        plan_contact_frame(ctx, p);
+        SceneObj& s = scene_of(ctx, p.scenes[0]);
+        if (s.cd->penetration()) plan_penetration_frame(ctx, p);
+        else plan_contact_frame(ctx, p);
         p.epoch = p.iterations;
         return;
     }
@@ -1132,6 +1180,70 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
     return id;
 }
 
+// The tets and boundary-surface topology of a penetration-contact scene in scene ids (meshes in
+// order; the detector's faces are the meshes' boundary faces in the same order).
+PenetrationSetup penetration_setup(const SceneObj& s) {
+    PenetrationSetup P;
+    int32_t tetOff = 0, faceOff = 0;
+    std::vector<std::vector<int32_t>> nbr(s.nv);
+    for (size_t im = 0; im < s.meshes.size(); ++im) {
+        const TetMeshHost& m = s.tets[s.meshes[im].second];
+        const TetSurface& S = s.tetSurfaces[s.meshes[im].second];
+        const int32_t vOff = s.vertexOffset[im];
+        for (int32_t t = 0; t < m.nt; ++t) {
+            for (int k = 0; k < 4; ++k) {
+                P.tets.push_back(vOff + m.tets[4 * (size_t)t + k]);
+                const int32_t n = S.tetNbr[4 * (size_t)t + k];
+                P.tetNbr.push_back(n < 0 ? -1 : tetOff + n);
+            }
+            P.tetMesh.push_back((int32_t)im);
+        }
+        const int32_t nf = (int32_t)S.faces.size() / 3;
+        // neighbour faces over directed edges: (u, v) of one face meets (v, u) of another
+        std::map<std::pair<int32_t, int32_t>, int32_t> dir;
+        for (int32_t f = 0; f < nf; ++f)
+            for (int e = 0; e < 3; ++e) dir[{S.faces[3 * (size_t)f + e], S.faces[3 * (size_t)f + (e + 1) % 3]}] = f;
+        for (int32_t f = 0; f < nf; ++f) {
+            P.faceOwner.push_back(tetOff + S.owner[2 * (size_t)f]);
+            P.faceOwner.push_back(S.owner[2 * (size_t)f + 1]);
+            for (int e = 0; e < 3; ++e) {
+                const int32_t u = S.faces[3 * (size_t)f + e], v = S.faces[3 * (size_t)f + (e + 1) % 3];
+                auto it = dir.find({v, u});
+                P.faceNbr.push_back(it == dir.end() ? -1 : faceOff + it->second);
+                nbr[vOff + u].push_back(vOff + v);
+                nbr[vOff + v].push_back(vOff + u);
+            }
+        }
+        for (int32_t v = 0; v < m.nv; ++v)
+            if (S.surface[v]) P.surfVerts.push_back(vOff + v);
+        tetOff += m.nt;
+        faceOff += nf;
+    }
+    P.vNbrBegin.assign(s.nv + 1, 0);
+    for (int32_t v = 0; v < s.nv; ++v) {
+        std::sort(nbr[v].begin(), nbr[v].end());
+        nbr[v].erase(std::unique(nbr[v].begin(), nbr[v].end()), nbr[v].end());
+        P.vNbrBegin[v + 1] = P.vNbrBegin[v] + (int32_t)nbr[v].size();
+        P.vNbr.insert(P.vNbr.end(), nbr[v].begin(), nbr[v].end());
+    }
+    const gvbd_physics_params& q = s.params;
+    PenetrationParams& c = P.params;
+    c.dcd = q.allow_dcd != 0;
+    c.ccd = q.allow_ccd != 0;
+    c.self = q.handle_self_collision != 0;
+    c.feasible = q.check_feasible_region != 0;
+    c.traverse = q.check_tet_traverse != 0;
+    c.traverseNonSelf = q.tet_traverse_non_self != 0;
+    c.shiftToCenter = q.shift_query_point_to_center != 0;
+    c.stopAfterPassing = q.stop_traversing_after_passing != 0;
+    c.offHeight = q.collision_off_height;
+    c.feasibleEps = q.feasible_region_epsilon;
+    c.rayEps = q.ray_tri_intersection_epsilon;
+    c.maxSearchMult = q.max_search_distance_multiplier;
+    c.centerShift = q.center_shift_level;
+    return P;
+}
+
 void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_t)>& reserve = {}) {
     if (s.initialized) fail(GVBD_ERR_STATE, "scene already initialised");
     if (s.meshes.empty()) fail(GVBD_ERR_STATE, "scene has no meshes");
@@ -1152,6 +1264,16 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         if (!(s.params.accelerator_rho >= 0.f && s.params.accelerator_rho < 1.f))
             fail(GVBD_ERR_INVALID, "accelerator_rho must be in [0, 1)");
     }
+    const bool penetration = contact && s.params.tet_contact_model == 1;
+    if (contact && s.params.tet_contact_model != 0 && s.params.tet_contact_model != 1)
+        fail(GVBD_ERR_INVALID, "tet_contact_model must be 0 (cloth-model contact) or 1 (penetration contact)");
+    if (penetration) {
+        if (!s.cloths.empty())
+            fail(GVBD_ERR_INVALID, "the penetration contact model (tet_contact_model 1) is the tet solver's: its scenes "
+                                   "hold tet meshes only");
+        if (!(s.params.collision_stiffness >= 0.f))
+            fail(GVBD_ERR_INVALID, "collision_stiffness must be non-negative");
+    }
     s.sp = make_solve_params(s.params, s);
     // tet meshes take part in contact through their boundary surface
     s.tetSurfaces.clear();
@@ -1159,7 +1281,8 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         for (const TetMeshHost& m : s.tets) {
             TetSurface S;
             std::string err;
-            if (!build_tet_surface(m.nv, m.nt, m.tets.data(), m.pos.data(), S.faces, err) ||
+            if (!build_tet_surface(m.nv, m.nt, m.tets.data(), m.pos.data(), S.faces, err,
+                                   penetration ? &S.owner : nullptr, penetration ? &S.tetNbr : nullptr) ||
                 !build_tri_edges(m.nv, (int32_t)S.faces.size() / 3, S.faces.data(), S.edges, nullptr, err))
                 fail(GVBD_ERR_INVALID, "tet mesh boundary surface: " + err);
             const int32_t nf = (int32_t)S.faces.size() / 3;
@@ -1215,6 +1338,7 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
                 s.cd->addMesh(m.nv, m.pos.data(), m.nf, m.faces.data());
             }
         }
+        if (penetration) s.cd->setPenetration(penetration_setup(s));
         s.cd->finalize();
         const DetectorDeviceView dv = s.cd->deviceView();
         ContactView& cv = s.view.cv;
@@ -1238,6 +1362,8 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
         cv.flag = dv.flag;
         cv.snap = s.dev.snap;
         cv.vgroup = s.dev.vgroup;
+        cv.model = penetration ? 1 : 0;
+        cv.penStiffness = s.params.collision_stiffness;
         s.view.hasContact = 1;
         s.view.initBound = dv.bounds;
     }
@@ -1249,6 +1375,13 @@ void initialize_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(s
 // face back-links (FV_CONTACT_PREALLOCATE = 32 per face), positions at detection.
 // Returns whether the BVH was rebuilt (a refit otherwise).
 bool scene_collision_detection(SceneObj& s, bool firstIteration) {
+    if (s.cd->penetration()) {
+        // the tet solver's DCD at the current positions and the face links (a path for CCD only
+        // exists within a frame)
+        s.cd->penetrationDcd(false);
+        s.cd->buildFaceLinks(32);
+        return false;
+    }
     const int steps = s.params.contact_bvh_rebuild_steps;
     const bool rebuild = firstIteration && steps > 0 && s.frame > 0 && s.frame % steps == 0;
     s.cd->updateBvh(rebuild);
@@ -1957,13 +2090,54 @@ int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stat
         require(ctx && out, "NULL argument");
         SceneObj& s = scene_of(ctx, scene);
         if (!s.cd) fail(GVBD_ERR_STATE, "scene has no collision handling");
+        collect_detection_time(s);
         *out = s.contactStats;
         out->vf_pairs = out->ee_pairs = 0;
         if (s.contactStats.detections > 0) {
             hip_check(hipStreamSynchronize(ctx->stream), "sync");
-            const DetectorResultCounts c = s.cd->counts();
-            out->vf_pairs = c.vfPairs;
-            out->ee_pairs = c.eeContacts;
+            if (s.cd->penetration()) {
+                // penetration contact: vertices with a hit face (DCD + CCD) in vf_pairs
+                const PenetrationCounts c = s.cd->penetrationCounts();
+                out->vf_pairs = c.dcdHits + c.ccdHits;
+            } else {
+                const DetectorResultCounts c = s.cd->counts();
+                out->vf_pairs = c.vfPairs;
+                out->ee_pairs = c.eeContacts;
+            }
+        }
+    });
+}
+
+int gvbd_scene_penetration_results(gvbd_context* ctx, int32_t scene, int32_t* hit_mesh, int32_t* hit_face,
+                                   uint8_t* flags, int32_t* counts) {
+    return guard(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        SceneObj& s = scene_of(ctx, scene);
+        if (!s.cd || !s.cd->penetration()) fail(GVBD_ERR_STATE, "scene has no penetration contact");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        std::vector<int32_t> face(s.nv);
+        std::vector<uint8_t> fl(s.nv);
+        s.cd->penetrationResults(face.data(), fl.data());
+        // scene face ids -> (mesh, boundary face of that mesh)
+        std::vector<int32_t> faceOff(s.meshes.size() + 1, 0);
+        for (size_t im = 0; im < s.meshes.size(); ++im)
+            faceOff[im + 1] = faceOff[im] + (int32_t)s.tetSurfaces[s.meshes[im].second].faces.size() / 3;
+        for (int32_t v = 0; v < s.nv; ++v) {
+            int32_t m = -1, f = -1;
+            if (face[v] >= 0) {
+                m = (int32_t)(std::upper_bound(faceOff.begin(), faceOff.end(), face[v]) - faceOff.begin()) - 1;
+                f = face[v] - faceOff[m];
+            }
+            if (hit_mesh) hit_mesh[v] = m;
+            if (hit_face) hit_face[v] = f;
+            if (flags) flags[v] = fl[v];
+        }
+        if (counts) {
+            const PenetrationCounts c = s.cd->penetrationCounts();
+            counts[0] = c.dcdHits;
+            counts[1] = c.penetratedNoFace;
+            counts[2] = c.ccdHits;
+            counts[3] = c.walkOverflows;
         }
     });
 }

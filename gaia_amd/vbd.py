@@ -48,6 +48,25 @@ DEFAULT_PHYSICS = dict(
     # Chebyshev acceleration of the tet iterations (VBD/VBDPhysicsParameters.h:64-65)
     use_accelerator=0,
     accelerator_rho=0.93,
+    # tet meshes with handle_collision: 0 cloth-model contact on their boundary surface, 1 the
+    # reference tet solver's penetration contact (CollisionDetertionParameters.h:43-80,
+    # VBD/VBDPhysicsParameters.h:36,70; PhysicsParameters.h:81)
+    tet_contact_model=0,
+    allow_dcd=1,
+    allow_ccd=1,
+    handle_self_collision=1,
+    collision_stiffness=1e5,
+    collision_off_height=1e7,
+    check_feasible_region=1,
+    check_tet_traverse=1,
+    tet_traverse_non_self=1,
+    shift_query_point_to_center=1,
+    stop_traversing_after_passing=1,
+    feasible_region_epsilon=1e-2,
+    ray_tri_intersection_epsilon=1e-10,
+    max_search_distance_multiplier=1.8,
+    center_shift_level=0.01,
+    dcd_bvh_rebuild_steps=16,
 )
 
 # VBDObjectParamsTriMeshStVk defaults (VBDCloth/VBDTriMeshStVK.h:17-20, VBDCloth/VBDBaseTriMesh.h:12-16,
@@ -505,6 +524,25 @@ class VBDPhysics:
         st = ContactStats()
         self.ctx.check(lib().gvbd_scene_contact_stats(self.ctx.handle, self.scene, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in ContactStats._fields_}
+
+    def penetrationResults(self):
+        """The last detection of a scene with the penetration contact model (tet_contact_model 1), one
+        row per scene vertex in mesh order: (mesh, vertex, hit mesh, hit face, contained, from CCD),
+        hit face -1 when none (the rows of VBDPhysics::collisionResultsAll, VBD/VBDPhysics.h:154), and
+        the counts (DCD hits, contained vertices without a face, CCD hits, walk stack overflows)."""
+        sizes = [self.meshInfo(m)[0] for m in range(self.numMeshes())]
+        nv = sum(sizes)
+        hm = np.zeros(nv, np.int32)
+        hf = np.zeros(nv, np.int32)
+        fl = np.zeros(nv, np.uint8)
+        counts = np.zeros(4, np.int32)
+        self.ctx.check(lib().gvbd_scene_penetration_results(
+            self.ctx.handle, self.scene, iptr(hm), iptr(hf), fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            iptr(counts)))
+        mesh = np.concatenate([np.full(n, m, np.int32) for m, n in enumerate(sizes)])
+        vert = np.concatenate([np.arange(n, dtype=np.int32) for n in sizes])
+        rows = np.stack([mesh, vert, hm, hf, (fl & 1).astype(np.int32), (fl >> 1 & 1).astype(np.int32)], axis=1)
+        return rows, counts
 
     def solveFailures(self, reset=False):
         """Vertex solves whose 3x3 block had no solution (gradient step for tet vertices, no move for

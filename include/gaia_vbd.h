@@ -106,6 +106,28 @@ typedef struct gvbd_physics_params {
      * VBD/VBDPhysicsParameters.h:64-65; runStepGPU, VBD/VBDPhysics.cpp:872-906) */
     int32_t use_accelerator;
     float accelerator_rho;                  /* in [0, 1) */
+    /* contact model of tet meshes with handle_collision: 0 = their boundary surfaces take part in
+     * the cloth contact model (VF/FV/EE repulsion, friction, conservative bounds); 1 = the reference
+     * tet solver's penetration contact: DCD of surface vertices at the start of a substep and CCD of
+     * their paths after the initial step (VBDPhysics::dcd / ccd, VBD/VBDPhysics.cpp:1958-2116), the
+     * force k d n + friction of each collision in the vertex solve (VBDPhysicsCompute.cu:1731-1923).
+     * Model 1 scenes hold tet meshes only; their frames need no host decisions. */
+    int32_t tet_contact_model;
+    int32_t allow_dcd;                      /* allowDCD (1) */
+    int32_t allow_ccd;                      /* allowCCD (1) */
+    int32_t handle_self_collision;          /* handleSelfCollision (1) */
+    float collision_stiffness;              /* collisionStiffness (1e5) */
+    float collision_off_height;             /* collisionOffHeight (1e7): vertices at or above it are not detected */
+    int32_t check_feasible_region;          /* checkFeasibleRegion (1) */
+    int32_t check_tet_traverse;             /* checkTetTraverse (1) */
+    int32_t tet_traverse_non_self;          /* tetrahedralTraverseForNonSelfIntersection (1) */
+    int32_t shift_query_point_to_center;    /* shiftQueryPointToCenter (1) */
+    int32_t stop_traversing_after_passing;  /* stopTraversingAfterPassingQueryPoint (1) */
+    float feasible_region_epsilon;          /* feasibleRegionEpsilon (1e-2) */
+    float ray_tri_intersection_epsilon;     /* rayTriIntersectionEPSILON (1e-10) */
+    float max_search_distance_multiplier;   /* maxSearchDistanceMultiplier (1.8) */
+    float center_shift_level;               /* centerShiftLevel (0.01) */
+    int32_t dcd_bvh_rebuild_steps;          /* dcdTetMeshSceneBVHRebuildSteps (16): full rebuild period in frames */
 } gvbd_physics_params;
 
 /* One tet mesh (ObjectParametersVBDNeoHookean, VBD/VBD_NeoHookean.cpp:1613-1625,
@@ -318,6 +340,14 @@ typedef struct gvbd_contact_stats {
     int64_t ee_pairs;        /* and edge-edge candidates */
 } gvbd_contact_stats;
 GVBD_API int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stats* out);
+/* penetration contact (tet_contact_model 1), after the last detection: per scene vertex the mesh
+ * and boundary face (in that mesh's boundary-face order, -1 none) it is pushed out of and flags
+ * (1: a tet contained it at the substep's start, 2: the hit came from CCD); each array holds the
+ * scene's vertex count (NULL skips it). counts (4, or NULL): DCD hits, contained vertices with no
+ * face, CCD hits, ray walks stopped by a full candidate stack. Replaces reading
+ * VBDPhysics::collisionResultsAll (VBD/VBDPhysics.h:154). */
+GVBD_API int gvbd_scene_penetration_results(gvbd_context* ctx, int32_t scene, int32_t* hit_mesh, int32_t* hit_face,
+                                            uint8_t* flags, int32_t* counts);
 /* Vertex solves since initialisation (or the last reset) whose 3x3 block had no solution: tet
  * vertices then take the gradient step (VBD/VBDPhysics.cpp:2449-2466), cloth vertices keep their
  * position (VBDCloth/VBDClothPhysics.cpp:1126-1150). The reference counts none; this is a diagnostic.

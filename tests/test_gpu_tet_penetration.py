@@ -1,0 +1,151 @@
+"""GPU parity of the penetration contact of tet meshes (tet_contact_model 1: the reference tet
+solver's DCD / CCD and the collision force in the vertex solve, VBDPhysics.cpp:839-969,1958-2116,
+VBDPhysicsCompute.cu:1731-1923) through the C ABI against the FP32 oracle:
+
+- per step from identical state, the detection (hit mesh and face, contained, from CCD) of every
+  vertex equals the oracle's, and positions are within 1e-5 normwise relative;
+- the closed-form Newton step of an apex pushed out of a fixed tet, a vertex crossing a slab
+  within one step caught by CCD, and bodies that never meet stepping as without the model.
+"""
+import numpy as np
+import pytest
+
+from gaia_amd import VBDPhysics, scenes
+from gaia_amd.vbd import physics_dict
+from vbd_cases import build_gpu, build_oracle, penetration_spec, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+DT = float(np.float32(1.0 / 60.0))
+
+
+def oracle_rows(o32, sizes):
+    """The oracle's detection as per-vertex rows in the device's layout."""
+    rows = []
+    for m, n in enumerate(sizes):
+        r = np.stack([np.full(n, m), np.arange(n), np.full(n, -1), np.full(n, -1), np.zeros(n), np.zeros(n)],
+                     axis=1).astype(np.int32)
+        rows.append(r)
+    rows = np.concatenate(rows)
+    off = np.cumsum([0] + list(sizes))
+    for mesh, v, hm, hf, pen, ccd in o32.collisions():
+        rows[off[mesh] + v, 2:] = (hm, hf, pen, ccd)
+    return rows
+
+
+def lockstep(gpu_ctx, oracle, spec, check_records=True):
+    sim = build_gpu(gpu_ctx, spec)
+    o32 = build_oracle(oracle, spec)
+    nm = len(spec["meshes"])
+    sizes = [len(m["pos"]) for m in spec["meshes"]]
+    errs, hits, mismatched = [], 0, []
+    for step in range(spec["steps"]):
+        for m in range(nm):
+            sim.setState(m, *o32.get_state(m))
+        sim.runStep()
+        o32.step(1, threads=8)
+        errs.append(max(rel_err(sim.getState(m)[0], o32.get_state(m)[0]) for m in range(nm)))
+        rows, counts = sim.penetrationResults()
+        want = oracle_rows(o32, sizes)
+        hits += int((want[:, 3] >= 0).sum())
+        if check_records and not np.array_equal(rows, want):
+            bad = np.nonzero((rows != want).any(axis=1))[0]
+            mismatched.append((step, rows[bad[:4]].tolist(), want[bad[:4]].tolist()))
+        assert counts[3] == 0, "a ray walk overflowed its candidate stack"
+    sim.destroy()
+    return np.array(errs), hits, mismatched
+
+
+def test_falling_cubes_detection_and_per_step_parity(gpu_ctx, oracle):
+    spec = penetration_spec(steps=40, bodies=3)
+    errs, hits, mismatched = lockstep(gpu_ctx, oracle, spec)
+    print(f"\n[penetration, 3 cubes] worst per-step rel err {errs.max():.3e}, {hits} vertex hits over "
+          f"{len(errs)} steps", flush=True)
+    assert hits > 0, "the bodies never collided"
+    assert not mismatched, mismatched[:3]
+    assert errs.max() < TOL, f"step {errs.argmax()}: {errs.max():.3e}"
+
+
+def test_self_collision_off_and_ccd_only(gpu_ctx, oracle):
+    """The same scene with CCD alone (allow_dcd 0) and with DCD alone, no self collision."""
+    for kw in (dict(allow_dcd=0), dict(allow_ccd=0, handle_self_collision=0)):
+        spec = penetration_spec(steps=25, bodies=2, vy=-3.0)
+        spec["physics"].update(kw)
+        errs, hits, mismatched = lockstep(gpu_ctx, oracle, spec)
+        print(f"\n[penetration {kw}] worst per-step rel err {errs.max():.3e}, {hits} hits", flush=True)
+        assert not mismatched, mismatched[:3]
+        assert errs.max() < TOL
+
+
+def test_closed_form_newton_step_out_of_a_fixed_tet(gpu_ctx):
+    """The oracle pin on the device: the free apex of a near-massless-material tet 0.1 inside a fixed
+    tet, under its z = 0 face, moves by -k d / (m / dt^2 + k) in one iteration."""
+    pa = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    apex = np.array([0.2, 0.2, 0.1])
+    pb = np.array([[0.0, 0.0, -0.5], [1.0, 0.0, -0.5], [0.0, 1.0, -0.5], apex])
+    t = np.array([[0, 1, 2, 3]], np.int32)
+    k = 1e5
+    sim = VBDPhysics(gpu_ctx)
+    sim.setPhysicsParams(**physics_dict(gravity=(0.0, 0.0, 0.0), time_step=DT, num_substeps=1, iterations=1,
+                                        world_lower=(-1e7, -1e7, -1e7), handle_collision=1, tet_contact_model=1,
+                                        collision_stiffness=k))
+    sim.addTetMesh(pa, t, fixed_points=[0, 1, 2, 3], vertex_colors=np.arange(4, dtype=np.int32), density=1000.0,
+                   mu=1e5, lmbd=1e6)
+    sim.addTetMesh(pb, t, fixed_points=[0, 1, 2], vertex_colors=np.arange(4, dtype=np.int32), density=1000.0,
+                   mu=1e-9, lmbd=1e-9)
+    sim.initialize()
+    sim.runStep()
+    x, _ = sim.getState(1)
+    vol = abs(np.linalg.det(np.column_stack([pb[1] - pb[0], pb[2] - pb[0], apex - pb[0]]))) / 6
+    M = 1000.0 * vol / 4 / DT ** 2
+    np.testing.assert_allclose(x[3], apex + [0.0, 0.0, -k * 0.1 / (M + k)], rtol=1e-5, atol=1e-6)
+    rows, counts = sim.penetrationResults()
+    assert rows[4 + 3, 2] == 0 and rows[4 + 3, 4] == 1 and counts[0] == 1
+    sim.destroy()
+
+
+def test_ccd_catches_a_vertex_crossing_a_slab(gpu_ctx, oracle):
+    ps, ts = scenes.cube_tet_mesh(2, 0.5)
+    ps = ps * [1.0, 0.2, 1.0]
+    apex = np.array([0.4, 0.25, 0.45])
+    pb = np.array([[0.3, 0.6, 0.4], [0.5, 0.6, 0.4], [0.4, 0.6, 0.55], apex])
+    v = np.zeros((4, 3))
+    v[3] = [0.0, -0.3 / DT, 0.0]
+    mat = dict(density=1000.0, mu=1e3, lmbd=1e4)
+    spec = dict(physics=physics_dict(gravity=(0.0, 0.0, 0.0), time_step=DT, num_substeps=1, iterations=10,
+                                     world_lower=(-1e7, -1e7, -1e7), handle_collision=1, tet_contact_model=1,
+                                     collision_stiffness=1e6),
+                meshes=[dict(pos=ps, tets=ts, material=mat, fixed=list(range(len(ps))), vel=None, coloring="greedy"),
+                        dict(pos=pb, tets=np.array([[0, 1, 2, 3]]), material=mat, fixed=[0, 1, 2], vel=v,
+                             coloring="greedy")],
+                steps=1)
+    errs, hits, mismatched = lockstep(gpu_ctx, oracle, spec)
+    assert not mismatched and hits == 1
+    assert errs.max() < TOL
+    sim = build_gpu(gpu_ctx, spec)
+    sim.runStep()
+    rows, counts = sim.penetrationResults()
+    r = rows[len(ps) + 3]
+    assert r[2] == 0 and r[4] == 0 and r[5] == 1 and counts[2] == 1
+    assert sim.getState(1)[0][3, 1] > 0.0
+    sim.destroy()
+
+
+def test_separated_bodies_step_as_without_the_model(gpu_ctx):
+    """Bodies that never meet: the penetration scene (contact kernel variant, detection every
+    substep) and the plain scene agree to rounding (the two kernel variants are compiled
+    separately)."""
+    spec = penetration_spec(steps=5, bodies=2, vy=0.0)
+    spec["meshes"][1]["pos"] = spec["meshes"][1]["pos"] + [1.0, 0.0, 0.0]
+    out = []
+    for contact in (1, 0):
+        spec["physics"]["handle_collision"] = contact
+        sim = build_gpu(gpu_ctx, spec)
+        sim.runSteps(5)
+        out.append([sim.getState(m)[0] for m in range(2)])
+        if contact:
+            rows, _ = sim.penetrationResults()
+            assert (rows[:, 3] == -1).all()
+        sim.destroy()
+    for a, b in zip(*out):
+        assert rel_err(a, b) < 1e-6

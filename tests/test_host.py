@@ -40,7 +40,8 @@ def test_abi_struct_layout_matches_header():
     a, b, c, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     assert gaia_amd.lib().gvbd_abi_struct_sizes(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                                 ctypes.byref(d)) == 0
-    assert ctypes.sizeof(PhysicsParams) == a.value == 136
+    assert ctypes.sizeof(PhysicsParams) == a.value == 200
+    assert PhysicsParams.tet_contact_model.offset == 136 and PhysicsParams.dcd_bvh_rebuild_steps.offset == 196
     assert ctypes.sizeof(TetMeshDesc) == b.value
     assert ctypes.sizeof(ClothMeshDesc) == c.value
     assert ctypes.sizeof(Schedule) == d.value == 32

@@ -133,8 +133,43 @@ def build_oracle(oracle, spec, precision=32):
         sc.add_mesh(pos, m["tets"], m["material"], colors, fixed=m["fixed"], vel=vel)
     for d in spec.get("deformers", []):
         sc.add_deformer(d)
+    ph = spec["physics"]
+    if ph.get("handle_collision") and ph.get("tet_contact_model") == 1:
+        sc.set_collision(penetration_params(ph))
     sc.finalize()
     return sc
+
+
+def penetration_params(ph):
+    """The oracle's collision parameters from a physics dict with the penetration contact model
+    (values rounded to float as the device takes them)."""
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    return dict(allow_dcd=ph["allow_dcd"], allow_ccd=ph["allow_ccd"], self_collision=ph["handle_self_collision"],
+                feasible_region=ph["check_feasible_region"], tet_traverse=ph["check_tet_traverse"],
+                traverse_non_self=ph["tet_traverse_non_self"], shift_to_center=ph["shift_query_point_to_center"],
+                stop_after_passing=ph["stop_traversing_after_passing"], stiffness=f32(ph["collision_stiffness"]),
+                off_height=f32(ph["collision_off_height"]), feasible_eps=f32(ph["feasible_region_epsilon"]),
+                ray_eps=f32(ph["ray_tri_intersection_epsilon"]),
+                max_search_mult=f32(ph["max_search_distance_multiplier"]), center_shift=f32(ph["center_shift_level"]))
+
+
+def penetration_spec(steps=30, iterations=10, stiffness=1e5, vy=-1.5, bodies=2, n=3, h=0.1, seed=0):
+    """Tet cubes falling onto each other and the ground with the penetration contact model: cube 0
+    rests on the ground, the others start above it with a downward velocity and a seeded lateral
+    offset, so they hit it (and each other) within a few frames."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for b in range(bodies):
+        off = (0.0, 0.0, 0.0) if b == 0 else (rng.uniform(-0.1, 0.1), 0.32 * b, rng.uniform(-0.1, 0.1))
+        pos, tets = scenes.cube_tet_mesh(n, h, origin=off)
+        vel = None if b == 0 else np.tile([0.0, vy, 0.0], (len(pos), 1))
+        meshes.append(dict(pos=pos, tets=tets, material=dict(density=1000.0, mu=2e5, lmbd=1e6, friction_dynamic=0.2,
+                                                             friction_eps_v=0.1),
+                           fixed=(), vel=vel, coloring="greedy"))
+    physics = physics_dict(gravity=(0.0, -9.8, 0.0), time_step=1.0 / 60.0, num_substeps=1, iterations=iterations,
+                           world_lower=(-1e7, 0.0, -1e7), handle_collision=1, tet_contact_model=1,
+                           collision_stiffness=stiffness)
+    return dict(physics=physics, meshes=meshes, steps=steps)
 
 
 def contact_drape_spec(steps=20, energy=1, density=1.0, stiffness=30.0, friction=0.2, iterations=8):
