@@ -1,7 +1,9 @@
 """Parity at the benchmark configurations' own sizes, through the C ABI on the device.
 
 - Batched VBD (256 scenes of log-uniform 500-50k vertices, tet cubes and two-corner sheets,
-  3,274,854 free vertices): every scene's first frame against the FP32 oracle, and the batched
+  3,274,854 free vertices): every tet scene's first frame within 1e-5 of the FP32 oracle, every
+  cloth scene's within the spread of the FP32 and FP64 oracles (an unstressed sheet's first frame
+  is ill-conditioned), and the batched
   frames bitwise equal to each scene stepped alone, under the benchmark's schedule and under a
   persistent work-queue schedule.
 - The two-corner sheet over 60 frames: per-step error distributions of the device against the
@@ -41,9 +43,10 @@ def _states(sims):
 
 
 def test_c4_batched_first_frame_vs_oracle_and_alone_bitwise(gpu_ctx, oracle, c4_specs):
-    """All 256 scenes batched in one graph per frame: the first frame of every scene within 1e-5 of
-    the FP32 oracle (normwise relative), then after 3 frames bitwise equal to the scene alone, for
-    the benchmark's group-launch schedule and a work-queue schedule (same lanes per vertex)."""
+    """All 256 scenes batched in one graph per frame: the first frame of every tet scene within 1e-5
+    of the FP32 oracle (normwise relative) and the cloth scenes within the FP32-vs-FP64 spread, then
+    after 3 frames bitwise equal to the scene alone, for the benchmark's group-launch schedule and a
+    work-queue schedule (same lanes per vertex)."""
     specs = c4_specs
     alone = [build_spec_scene(gpu_ctx, s) for s in specs]
     for s in alone:
@@ -60,16 +63,29 @@ def test_c4_batched_first_frame_vs_oracle_and_alone_bitwise(gpu_ctx, oracle, c4_
     batch.run_steps(1)
     first = _states(sims)
     threads = host_threads()
-    errs = []
+    errs, spread = [], []
     for i, spec in enumerate(specs):
         o = build_spec_oracle(oracle, spec)
         o.step(1, threads=threads)
         errs.append(rel_err(first[i][0], o.get_state(0)[0]))
+        if spec["kind"] == "cloth":
+            o64 = build_spec_oracle(oracle, spec, 64)
+            o64.step(1, threads=threads)
+            spread.append(rel_err(o.get_state(0)[0], o64.get_state(0)[0]))
     errs = np.array(errs)
     kinds = np.array([s["kind"] for s in specs])
-    print(f"\n[C4 first frame vs FP32 oracle] tet max {errs[kinds == 'tet'].max():.2e}, cloth max "
-          f"{errs[kinds == 'cloth'].max():.2e}, median {np.median(errs):.2e}", flush=True)
-    assert errs.max() < TOL, f"scene {errs.argmax()} ({kinds[errs.argmax()]}): {errs.max():.3e}"
+    et, ec, spread = errs[kinds == "tet"], errs[kinds == "cloth"], np.array(spread)
+    print(f"\n[C4 first frame vs FP32 oracle] tet max {et.max():.2e}; cloth median {np.median(ec):.2e} max "
+          f"{ec.max():.2e}, FP32 vs FP64 of the cloth scenes median {np.median(spread):.2e} max {spread.max():.2e}",
+          flush=True)
+    assert et.max() < TOL, f"tet scene: {et.max():.3e}"
+    # An unstressed two-corner sheet's first frame is ill-conditioned (its out-of-plane modes have
+    # no stiffness and the reference's StVK Hessian has no PSD projection): the FP32 and FP64 oracles
+    # of one scene part by up to ~7e-2. Cloth scenes are held to that spread: the device's median
+    # within 2x of the FP32-vs-FP64 median (or 1e-6) and its worst scene within 4x of the worst
+    # FP32-vs-FP64 scene (or 1e-5).
+    assert np.median(ec) <= max(2.0 * np.median(spread), 1e-6)
+    assert ec.max() <= max(4.0 * spread.max(), TOL)
 
     batch.run_steps(2)
     for i, ((xa, va), (xb, vb)) in enumerate(zip(ref, _states(sims))):
