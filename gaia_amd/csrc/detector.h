@@ -123,6 +123,8 @@ class ContactDetector {
     // Queries read (and refits follow) positions owned by the caller, e.g. a scene's x array in
     // the same vertex order; call before finalize.
     void useExternalPositions(float4* pos);
+    // the external positions moved (a scene copied out and back at another address)
+    void rebindPositions(float4* pos);
     void finalize();
     // Face back-links of the stored VF contacts (ClothContactDetector::faceContactInfos): per
     // face the lowest `cap` vertex ids whose stored contacts name that face.

@@ -1419,6 +1419,12 @@ void ContactDetector::useExternalPositions(float4* pos) {
     external_ = pos;
 }
 
+void ContactDetector::rebindPositions(float4* pos) {
+    if (!external_) fail(GVBD_ERR_STATE, "the detector owns its positions");
+    external_ = pos;
+    pos_ = pos;
+}
+
 void ContactDetector::buildFaceLinks(int cap) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     const int64_t n = (int64_t)nv_ * vfCap_;
@@ -1680,7 +1686,15 @@ void ContactDetector::finalize() {
     hip_check(hipStreamSynchronize(stream_), "detector upload sync");
     ready_ = true;
     updateBvh(true);
-    if (pen_) buildBvh(tb_, 3, tetBoxes_);
+    if (pen_) {
+        buildBvh(tb_, 3, tetBoxes_);
+        // the swept-face tree over zero-length paths, so that every later pass refits (or
+        // rebuilds on schedule) without a first-use build
+        hipLaunchKernelGGL(swept_box_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, pos_, pos_, faces_, nf_,
+                           sweptBoxes_);
+        buildBvh(sb_, 3, sweptBoxes_);
+        sweptBuilt_ = true;
+    }
 }
 
 void ContactDetector::allocBvh(DeviceBvh& B, int n) {

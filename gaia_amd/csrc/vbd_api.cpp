@@ -144,6 +144,7 @@ struct PlanObj {
     WorkItem* dCommit = nullptr;  // per group: cloth commit items
     WorkItem* dVItems = nullptr;  // per-vertex items
     std::vector<int32_t> groupItemBegin;    // G + 1
+    std::vector<int32_t> groupContactBegin;  // G: items of scenes with collision handling start here
     std::vector<int32_t> groupCommitBegin;  // G + 1
     int32_t nVItems = 0;
     WorkItem* dDeformPre = nullptr;   // deformed vertices of cloth meshes (before the initial step)
@@ -155,12 +156,21 @@ struct PlanObj {
     int32_t nQueue = 0, nCounters = 0, queueGrid = 0;
     int32_t ticketSlots = 0;  // queue launches allowed between counter resets
     int32_t epoch = 0;        // queue launches since the last reset (host-issued iterations)
-    size_t bytes = 0;
+    size_t bytes = 0;  // device bytes held (the capacity of dBuf)
+    // every device array above lives in one buffer, uploaded in one copy from a pinned host
+    // staging buffer of the same capacity; both grow on demand and are kept across rebuilds, so a
+    // rebuild (a serving tick with a new scene set) allocates and synchronises nothing
+    char* dBuf = nullptr;
+    char* hBuf = nullptr;
+    size_t cap = 0;
+    hipEvent_t uploadDone = nullptr;  // the last copy out of hBuf
     bool wantGraph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
-    bool contact = false;  // single cloth scene with collision handling: host-driven iterations
+    bool capturing = false;  // a frame is being captured into the plan's graph (no events, no host bookkeeping)
+    bool contact = false;      // some scene has collision handling (contact kernel variant for its items)
+    bool clothContact = false;  // a cloth-model contact scene (alone in its plan): host-driven iterations
     bool accel = false;    // some scene uses the Chebyshev accelerator
 };
 
@@ -278,11 +288,10 @@ void free_plan_graph(PlanObj& p) {
     p.graph = nullptr;
 }
 
+// Marks a plan for rebuilding (its scene set, schedule or scene addresses changed); its buffers
+// stay allocated for the rebuild.
 void free_plan(PlanObj& p) {
     free_plan_graph(p);
-    for (void* ptr : {(void*)p.dViews, (void*)p.dItems, (void*)p.dCommit, (void*)p.dVItems, (void*)p.dQueue,
-                      (void*)p.dCounters, (void*)p.dDeformPre, (void*)p.dDeformPost})
-        if (ptr) (void)hipFree(ptr);
     p.dDeformPre = p.dDeformPost = nullptr;
     p.nDeformPre = p.nDeformPost = 0;
     p.dViews = nullptr;
@@ -292,8 +301,18 @@ void free_plan(PlanObj& p) {
     p.dQueue = nullptr;
     p.dCounters = nullptr;
     p.nQueue = p.nCounters = p.queueGrid = p.ticketSlots = p.epoch = 0;
-    p.bytes = 0;
     p.dirty = true;
+}
+
+// Frees a plan's buffers (the plan is being destroyed; its frames must be complete).
+void release_plan(PlanObj& p) {
+    free_plan(p);
+    if (p.dBuf) (void)hipFree(p.dBuf);
+    if (p.hBuf) (void)hipHostFree(p.hBuf);
+    if (p.uploadDone) (void)hipEventDestroy(p.uploadDone);
+    p.dBuf = p.hBuf = nullptr;
+    p.uploadDone = nullptr;
+    p.cap = p.bytes = 0;
 }
 
 // Work-queue status: reports (and clears) a dependency wait that hit its spin limit.
@@ -837,8 +856,13 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     std::vector<SceneView> views;
     p.numGroups = 0;
     p.accel = false;
+    p.contact = p.clothContact = false;
     for (int32_t sid : p.scenes) {
         SceneObj& s = scene_of(ctx, sid);
+        if (s.cd) {
+            p.contact = true;
+            if (!s.cd->penetration()) p.clothContact = true;
+        }
         views.push_back(s.view);
         p.numGroups = std::max(p.numGroups, (int)s.groups.size());
         if (s.view.accOmega) p.accel = true;
@@ -860,16 +884,22 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
             k = r;
         }
     };
+    p.groupContactBegin.assign(p.numGroups, 0);
     for (int g = 0; g < p.numGroups; ++g) {
-        for (size_t si = 0; si < p.scenes.size(); ++si) {
-            SceneObj& s = scene_of(ctx, p.scenes[si]);
-            if (g >= (int)s.groups.size()) continue;
-            for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += vpb)
-                items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_TET});
-            for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += vpb)
-                items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
-            commit_runs(s, g, kVertexChunk,
-                        [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
+        // items of scenes without collision handling first (the plain solve variant), then those
+        // of scenes with it (the contact variant): each scene's work is the same either way
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) p.groupContactBegin[g] = (int32_t)items.size();
+            for (size_t si = 0; si < p.scenes.size(); ++si) {
+                SceneObj& s = scene_of(ctx, p.scenes[si]);
+                if (g >= (int)s.groups.size() || (s.cd != nullptr) != (pass == 1)) continue;
+                for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += vpb)
+                    items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_TET});
+                for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += vpb)
+                    items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_CLOTH});
+                commit_runs(s, g, kVertexChunk,
+                            [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
+            }
         }
         p.groupItemBegin[g + 1] = (int32_t)items.size();
         p.groupCommitBegin[g + 1] = (int32_t)commits.size();
@@ -930,27 +960,61 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
     }
     p.nDeformPre = (int32_t)dpre.size();
     p.nDeformPost = (int32_t)dpost.size();
-    auto alloc_up = [&](void** dst, const void* src, size_t n) {
-        if (hipMalloc(dst, std::max<size_t>(n, 16)) != hipSuccess) {
+    // one buffer: each array at a 256-byte aligned offset
+    struct Part {
+        void** dst;
+        const void* src;
+        size_t n;
+        size_t off;
+    };
+    std::vector<int> zeros;
+    std::vector<Part> parts = {{(void**)&p.dViews, views.data(), sizeof(SceneView) * views.size(), 0},
+                               {(void**)&p.dItems, items.data(), sizeof(WorkItem) * items.size(), 0},
+                               {(void**)&p.dCommit, commits.data(), sizeof(WorkItem) * commits.size(), 0},
+                               {(void**)&p.dVItems, vitems.data(), sizeof(WorkItem) * vitems.size(), 0}};
+    if (p.nDeformPre) parts.push_back({(void**)&p.dDeformPre, dpre.data(), sizeof(WorkItem) * dpre.size(), 0});
+    if (p.nDeformPost) parts.push_back({(void**)&p.dDeformPost, dpost.data(), sizeof(WorkItem) * dpost.size(), 0});
+    if (p.mode == GVBD_MODE_WORK_QUEUE) {
+        zeros.assign(p.nCounters, 0);
+        parts.push_back({(void**)&p.dQueue, queue.data(), sizeof(QueueEntry) * queue.size(), 0});
+        parts.push_back({(void**)&p.dCounters, zeros.data(), sizeof(int) * zeros.size(), 0});
+    }
+    size_t total = 0;
+    for (Part& q : parts) {
+        q.off = total;
+        total += (std::max<size_t>(q.n, 16) + 255) & ~(size_t)255;
+    }
+    if (total > p.cap) {
+        // grow: the old buffers may still be read by frames in flight and by the last upload
+        hip_check(hipStreamSynchronize(ctx->stream), "plan buffer sync");
+        if (p.dBuf) (void)hipFree(p.dBuf);
+        if (p.hBuf) (void)hipHostFree(p.hBuf);
+        p.dBuf = p.hBuf = nullptr;
+        p.cap = p.bytes = 0;
+        const size_t cap = std::max(total, total + total / 2);
+        if (hipMalloc((void**)&p.dBuf, cap) != hipSuccess) {
             (void)hipGetLastError();
-            *dst = nullptr;
+            p.dBuf = nullptr;
             fail(GVBD_ERR_NOMEM, "hipMalloc for plan failed");
         }
-        p.bytes += std::max<size_t>(n, 16);
-        if (n) hip_check(hipMemcpyAsync(*dst, src, n, hipMemcpyHostToDevice, ctx->stream), "plan upload");
-    };
-    alloc_up((void**)&p.dViews, views.data(), sizeof(SceneView) * views.size());
-    alloc_up((void**)&p.dItems, items.data(), sizeof(WorkItem) * items.size());
-    alloc_up((void**)&p.dCommit, commits.data(), sizeof(WorkItem) * commits.size());
-    alloc_up((void**)&p.dVItems, vitems.data(), sizeof(WorkItem) * vitems.size());
-    if (p.nDeformPre) alloc_up((void**)&p.dDeformPre, dpre.data(), sizeof(WorkItem) * dpre.size());
-    if (p.nDeformPost) alloc_up((void**)&p.dDeformPost, dpost.data(), sizeof(WorkItem) * dpost.size());
-    if (p.mode == GVBD_MODE_WORK_QUEUE) {
-        alloc_up((void**)&p.dQueue, queue.data(), sizeof(QueueEntry) * queue.size());
-        std::vector<int> zeros(p.nCounters, 0);
-        alloc_up((void**)&p.dCounters, zeros.data(), sizeof(int) * zeros.size());
+        if (hipHostMalloc((void**)&p.hBuf, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(p.dBuf);
+            p.dBuf = p.hBuf = nullptr;
+            fail(GVBD_ERR_NOMEM, "pinned host allocation for plan staging failed");
+        }
+        p.cap = p.bytes = cap;
+        if (!p.uploadDone) hip_check(hipEventCreateWithFlags(&p.uploadDone, hipEventDisableTiming), "event create");
+    } else {
+        // the staging buffer is rewritten only after its last copy has left it
+        hip_check(hipEventSynchronize(p.uploadDone), "plan staging wait");
     }
-    hip_check(hipStreamSynchronize(ctx->stream), "plan upload sync");
+    for (Part& q : parts) {
+        if (q.n) std::memcpy(p.hBuf + q.off, q.src, q.n);
+        *q.dst = p.dBuf + q.off;
+    }
+    hip_check(hipMemcpyAsync(p.dBuf, p.hBuf, total, hipMemcpyHostToDevice, ctx->stream), "plan upload");
+    hip_check(hipEventRecord(p.uploadDone, ctx->stream), "event record");
     p.dirty = false;
 }
 
@@ -976,9 +1040,13 @@ void plan_initial_step(gvbd_context* ctx, PlanObj& p, bool clothDeformed = false
 }
 
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
-    const int b = p.groupItemBegin[g], e = p.groupItemBegin[g + 1];
-    hip_check(launch_group_solve(p.dViews, p.dItems + b, e - b, iter, p.lpv, p.block, p.stage, p.contact, ctx->stream),
-              "group solve launch");
+    const int b = p.groupItemBegin[g], c = p.groupContactBegin[g], e = p.groupItemBegin[g + 1];
+    if (c > b)
+        hip_check(launch_group_solve(p.dViews, p.dItems + b, c - b, iter, p.lpv, p.block, p.stage, false, ctx->stream),
+                  "group solve launch");
+    if (e > c)
+        hip_check(launch_group_solve(p.dViews, p.dItems + c, e - c, iter, p.lpv, p.block, 0, true, ctx->stream),
+                  "group solve launch");
     const int cb = p.groupCommitBegin[g], ce = p.groupCommitBegin[g + 1];
     if (ce > cb) hip_check(launch_cloth_commit(p.dViews, p.dCommit + cb, ce - cb, ctx->stream), "commit launch");
 }
@@ -1001,8 +1069,10 @@ void plan_reset_counters(gvbd_context* ctx, PlanObj& p) {
 // the counters were reset) or one launch per parallel group.
 // Contact scenes: the iteration-start snapshot read for same-group contact partners.
 void plan_contact_snapshot(gvbd_context* ctx, PlanObj& p) {
-    SceneObj& s = scene_of(ctx, p.scenes[0]);
-    hip_check(launch_stream_copy(s.view.pos, s.dev.snap, s.nv, ctx->stream), "snapshot launch");
+    for (int32_t sid : p.scenes) {
+        SceneObj& s = scene_of(ctx, sid);
+        if (s.cd) hip_check(launch_stream_copy(s.view.pos, s.dev.snap, s.nv, ctx->stream), "snapshot launch");
+    }
 }
 
 void plan_iteration(gvbd_context* ctx, PlanObj& p, int it, int epoch) {
@@ -1076,45 +1146,78 @@ void collect_detection_time(SceneObj& s) {
     s.detectPending = false;
 }
 
-// One frame of a tet scene with penetration contact (runStepGPU, VBDPhysics.cpp:839-969): per
-// substep DCD at its start (the trees rebuilt at substep 0 of every dcd_bvh_rebuild_steps-th
-// frame, VBDPhysics.cpp:1962-1963, refit otherwise), the initial step, CCD over the vertices'
-// paths from their start positions, the face links, then the iterations and the velocities. All
-// on the device: no host reads inside the frame.
-void plan_penetration_frame(gvbd_context* ctx, PlanObj& p) {
-    SceneObj& s = scene_of(ctx, p.scenes[0]);
+// Whether scene s rebuilds its detection trees in the frame it is about to step
+// (dcdTetMeshSceneBVHRebuildSteps, VBDPhysics.cpp:1962-1963: at substep 0 of every such frame
+// but the first).
+bool penetration_rebuild_due(const SceneObj& s) {
     const int steps = s.params.dcd_bvh_rebuild_steps;
+    return steps > 0 && s.frame > 0 && s.frame % steps == 0;
+}
+
+// One frame of the tet scenes with penetration contact of a plan, batched with its other scenes
+// (runStepGPU, VBDPhysics.cpp:839-969): per substep DCD of each contact scene at its start (the
+// trees rebuilt at substep 0 of a rebuild frame, refit otherwise), the batched initial step, CCD
+// of each contact scene over its vertices' paths from their start positions and its face links,
+// then the batched iterations and velocities. No host reads inside the frame; with `timing` the
+// detection passes are timed with events read a substep later.
+void plan_penetration_frame(gvbd_context* ctx, PlanObj& p, bool timing) {
+    std::vector<SceneObj*> cs;
+    for (int32_t sid : p.scenes) {
+        SceneObj& s = scene_of(ctx, sid);
+        if (s.cd) cs.push_back(&s);
+    }
     for (int sub = 0; sub < p.substeps; ++sub) {
-        const bool rebuild = sub == 0 && steps > 0 && s.frame > 0 && s.frame % steps == 0;
-        collect_detection_time(s);
-        for (hipEvent_t& e : s.cdEvents)
-            if (!e) hip_check(hipEventCreate(&e), "event create");
-        hip_check(hipEventRecord(s.cdEvents[0], ctx->stream), "event record");
-        s.cd->penetrationDcd(rebuild);
-        hip_check(hipEventRecord(s.cdEvents[1], ctx->stream), "event record");
+        for (SceneObj* s : cs) {
+            if (timing) {
+                collect_detection_time(*s);
+                for (hipEvent_t& e : s->cdEvents)
+                    if (!e) hip_check(hipEventCreate(&e), "event create");
+                hip_check(hipEventRecord(s->cdEvents[0], ctx->stream), "event record");
+            }
+            s->cd->penetrationDcd(sub == 0 && penetration_rebuild_due(*s));
+            if (timing) hip_check(hipEventRecord(s->cdEvents[1], ctx->stream), "event record");
+        }
         plan_initial_step(ctx, p);
-        hip_check(hipEventRecord(s.cdEvents[2], ctx->stream), "event record");
-        s.cd->penetrationCcd(s.view.prev, rebuild);
-        s.cd->buildFaceLinks(32);
-        hip_check(hipEventRecord(s.cdEvents[3], ctx->stream), "event record");
-        s.detectPending = true;
+        for (SceneObj* s : cs) {
+            if (timing) hip_check(hipEventRecord(s->cdEvents[2], ctx->stream), "event record");
+            s->cd->penetrationCcd(s->view.prev, sub == 0 && penetration_rebuild_due(*s));
+            s->cd->buildFaceLinks(32);
+            if (timing) {
+                hip_check(hipEventRecord(s->cdEvents[3], ctx->stream), "event record");
+                s->detectPending = true;
+            }
+        }
         for (int it = 0; it < p.iterations; ++it) plan_iteration(ctx, p, it, it);
         plan_velocity(ctx, p);
-        ++s.contactStats.detections;
-        if (rebuild) ++s.contactStats.rebuilds;
-        ++s.contactStats.frames;
-        s.contactStats.iterations += p.iterations;
     }
-    ++s.frame;
+}
+
+// Host bookkeeping of one penetration frame of every contact scene of a plan (counters, frame index).
+void penetration_frame_done(PlanObj& p, gvbd_context* ctx) {
+    for (int32_t sid : p.scenes) {
+        SceneObj& s = scene_of(ctx, sid);
+        if (!s.cd) continue;
+        if (penetration_rebuild_due(s)) ++s.contactStats.rebuilds;
+        s.contactStats.detections += p.substeps;
+        s.contactStats.frames += p.substeps;
+        s.contactStats.iterations += (int64_t)p.substeps * p.iterations;
+        ++s.frame;
+    }
 }
 
 // One frame: substeps x (initial step, iterations x groups, velocity update)
 // (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
+    if (p.clothContact) {
+        plan_contact_frame(ctx, p);
+        p.epoch = p.iterations;
+        return;
+    }
     if (p.contact) {
-        SceneObj& s = scene_of(ctx, p.scenes[0]);
-        if (s.cd->penetration()) plan_penetration_frame(ctx, p);
-        else plan_contact_frame(ctx, p);
+        // detection is timed for a scene stepped alone (gvbd_contact_stats.detect_ms); batched and
+        // served frames are not timed, so that no event wait slows their launch
+        plan_penetration_frame(ctx, p, !p.capturing && p.scenes.size() == 1);
+        if (!p.capturing) penetration_frame_done(p, ctx);
         p.epoch = p.iterations;
         return;
     }
@@ -1126,32 +1229,57 @@ void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
     p.epoch = p.iterations;  // device counters after the last substep
 }
 
+// Whether a penetration-contact scene of the plan rebuilds its trees in the next frame (such a
+// frame is launched directly: the rebuild reads level sizes back to the host, and the captured
+// refits of the old trees no longer apply after it).
+bool plan_rebuild_due(gvbd_context* ctx, const PlanObj& p) {
+    if (!p.contact) return false;
+    for (int32_t sid : p.scenes) {
+        const SceneObj& s = scene_of(ctx, sid);
+        if (s.cd && penetration_rebuild_due(s)) return true;
+    }
+    return false;
+}
+
 void plan_capture(gvbd_context* ctx, PlanObj& p) {
     free_plan_graph(p);
-    if (p.contact)
-        fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling: whether "
-                               "an iteration runs collision detection is decided on the host");
+    if (p.clothContact)
+        fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling in the "
+                               "cloth contact model: whether an iteration runs collision detection is decided on "
+                               "the host");
+    if (plan_rebuild_due(ctx, p)) return;  // captured after the rebuild frame (see plan_run)
+    p.capturing = true;
     hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     try {
         plan_frame_launches(ctx, p);
     } catch (...) {
+        p.capturing = false;
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(ctx->stream, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
     }
+    p.capturing = false;
     hip_check(hipStreamEndCapture(ctx->stream, &p.graph), "end capture");
     hip_check(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0), "graph instantiate");
 }
 
 void plan_run(gvbd_context* ctx, PlanObj& p, int n) {
     build_plan(ctx, p);
-    if (p.wantGraph && !p.exec) plan_capture(ctx, p);
     for (int i = 0; i < n; ++i) {
-        if (p.exec)
-            hip_check(hipGraphLaunch(p.exec, ctx->stream), "graph launch");
-        else
+        if (p.wantGraph && plan_rebuild_due(ctx, p)) {
+            // a tree rebuild: launched directly, the graph captured again for the next frames
+            free_plan_graph(p);
             plan_frame_launches(ctx, p);
+            continue;
+        }
+        if (p.wantGraph && !p.exec) plan_capture(ctx, p);
+        if (p.exec) {
+            hip_check(hipGraphLaunch(p.exec, ctx->stream), "graph launch");
+            if (p.contact) penetration_frame_done(p, ctx);
+        } else {
+            plan_frame_launches(ctx, p);
+        }
     }
     p.epoch = p.iterations;
 }
@@ -1170,8 +1298,13 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
         if (s.params.iterations != p->iterations || s.params.num_substeps != p->substeps)
             fail(GVBD_ERR_INVALID, "scenes of one batch must share iterations and substeps");
         if (s.cd) {
-            if (scenes.size() > 1)
-                fail(GVBD_ERR_INVALID, "scenes with collision handling run alone (gvbd_scene_run_step), not in a batch");
+            if (!s.cd->penetration()) {
+                if (scenes.size() > 1)
+                    fail(GVBD_ERR_INVALID, "scenes with collision handling in the cloth contact model run alone "
+                                           "(gvbd_scene_run_step), not in a batch: their re-detection is decided "
+                                           "on the host");
+                p->clothContact = true;
+            }
             p->contact = true;
         }
     }
@@ -1485,9 +1618,9 @@ int gvbd_context_destroy(gvbd_context* ctx) {
     if (!ctx) return GVBD_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->plans) free_plan(*kv.second);
+    for (auto& kv : ctx->plans) release_plan(*kv.second);
     for (auto& kv : ctx->servers) {
-        for (gvbd::PlanObj& P : kv.second->plans) free_plan(P);
+        for (gvbd::PlanObj& P : kv.second->plans) release_plan(P);
         for (hipEvent_t e : kv.second->frameDone)
             if (e) (void)hipEventDestroy(e);
         if (kv.second->ref) (void)hipEventDestroy(kv.second->ref);
@@ -1573,7 +1706,7 @@ int gvbd_scene_destroy(gvbd_context* ctx, int32_t scene) {
                     fail(GVBD_ERR_STATE, "scene is part of a live batch; destroy the batch first");
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         if (s.plan >= 0) {
-            free_plan(plan_of(ctx, s.plan));
+            release_plan(plan_of(ctx, s.plan));
             ctx->plans.erase(s.plan);
         }
         s.cd.reset();  // reads the arena's positions: released first
@@ -1978,7 +2111,7 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
     require(sc.stage_records == 0 || sc.mode == GVBD_MODE_GROUP_LAUNCHES,
             "LDS staging applies to group launches only");
     require(!(p.contact && sc.mode == GVBD_MODE_WORK_QUEUE),
-            "scenes with collision handling run with group launches (the host decides each iteration's detection)");
+            "scenes with collision handling run with group launches (detection runs between the launches)");
     require(!(p.contact && sc.stage_records != 0), "LDS staging is not available for scenes with collision handling");
     p.lpv = sc.lanes_per_vertex;
     p.stage = sc.stage_records;
@@ -2294,9 +2427,10 @@ int gvbd_scene_evaluate_convergence_noalloc(gvbd_context* ctx, int32_t scene, do
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable) {
     return guard(ctx, [&] {
         PlanObj& p = scene_plan(ctx, scene);
-        if (enable && p.contact)
-            fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling: "
-                                   "whether an iteration runs collision detection is decided on the host");
+        if (enable && p.clothContact)
+            fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling in the "
+                                   "cloth contact model: whether an iteration runs collision detection is decided "
+                                   "on the host");
         p.wantGraph = enable != 0;
         free_plan_graph(p);
         if (p.wantGraph) {
@@ -2569,7 +2703,7 @@ int gvbd_batch_destroy(gvbd_context* ctx, int32_t batch) {
         require(ctx != nullptr, "ctx is NULL");
         PlanObj& p = batch_plan(ctx, batch);
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
-        free_plan(p);
+        release_plan(p);
         ctx->plans.erase(ctx->batches[batch]);
         ctx->batches.erase(batch);
     });
@@ -2704,7 +2838,7 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
 
 // Frees a server's device resources (its frames must be complete).
 static void release_server(ServerObj& S) {
-    for (PlanObj& P : S.plans) free_plan(P);
+    for (PlanObj& P : S.plans) release_plan(P);
     for (hipEvent_t& e : S.frameDone)
         if (e) (void)hipEventDestroy(e), e = nullptr;
     if (S.ref) (void)hipEventDestroy(S.ref), S.ref = nullptr;
@@ -2735,6 +2869,12 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact), mv(v.accPrevPrev), mv(v.accPrevIter);
     mv(v.accOmega), mv(v.clock), mv(v.vzone), mv(v.dverts), mv(v.dparams), mv(v.attBegin), mv(v.att);
     mv(v.solveFail);
+    if (s.cd) {
+        // contact arrays in the arena; the detector's own arrays stay where they are
+        ContactView& c = v.cv;
+        mv(c.friction), mv(c.faceRef), mv(c.edgeRef), mv(c.tFaceRef), mv(c.tEdgeRef), mv(c.snap), mv(c.vgroup);
+        s.cd->rebindPositions(d.pos);
+    }
 }
 
 // Copies an idle scene's arena out to host memory and frees it.
@@ -2773,6 +2913,7 @@ static int64_t server_resident_bytes(gvbd_context* ctx, const ServerObj& S, int3
     for (auto& kv : S.scenes) {
         const SceneObj& s = scene_of(ctx, kv.first);
         if (s.dev.arena) b += (int64_t)s.dev.bytes, ++n;
+        if (s.cd) b += (int64_t)s.cd->deviceBytes();  // detection trees and lists stay resident
     }
     if (count) *count = n;
     return b;
@@ -3000,8 +3141,9 @@ int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
         SceneObj& s = scene_of(ctx, scene);
         if (s.server >= 0) fail(GVBD_ERR_STATE, "scene is already registered with a server");
         require(!s.meshes.empty(), "scene has no meshes");
-        if (s.params.handle_collision)
-            fail(GVBD_ERR_INVALID, "scenes with collision handling are stepped alone (gvbd_scene_run_step), not served");
+        if (s.params.handle_collision && !(s.params.tet_contact_model == 1 && s.cloths.empty()))
+            fail(GVBD_ERR_INVALID, "scenes with collision handling in the cloth contact model are stepped alone "
+                                   "(gvbd_scene_run_step), not served: their re-detection is decided on the host");
         for (auto& kv : ctx->plans)
             if (kv.first != s.plan)
                 for (int32_t sid : kv.second->scenes)

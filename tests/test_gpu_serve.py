@@ -173,7 +173,7 @@ def test_server_rejections(gpu_ctx):
     with pytest.raises(gaia_amd.VBDError, match="share iterations"):
         srv.add_scene(b)
     c = build_gpu(gpu_ctx, contact_drape_spec(steps=1))
-    with pytest.raises(gaia_amd.VBDError, match="collision handling"):
+    with pytest.raises(gaia_amd.VBDError, match="collision handling in the cloth contact model are stepped alone"):
         srv.add_scene(c)
     d = build(gpu_ctx, specs[1])
     srv.add_scene(d)

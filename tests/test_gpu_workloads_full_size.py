@@ -133,9 +133,10 @@ def test_c4_sheet_error_distribution(gpu_ctx, oracle):
     e_gpu, e_64 = np.array(e_gpu), np.array(e_64)
     fails = (sim.solveFailures(), o32.solve_failures(), o64.solve_failures())
     print(f"\n[C4 sheet, 60 frames] device vs FP32: median {np.median(e_gpu):.2e} max {e_gpu.max():.2e} "
-          f"(step 1 {e_gpu[0]:.2e}, steps over 1e-5: {(e_gpu > TOL).sum()}); FP32 vs FP64: median "
+          f"(step 1 {e_gpu[0]:.2e}, steps over 1e-5: {(e_gpu > TOL).sum()}); FP32 vs FP64: step 1 {e_64[0]:.2e}, median "
           f"{np.median(e_64):.2e} max {e_64.max():.2e}; 3x3 solve failures device/FP32/FP64 {fails}", flush=True)
-    assert e_gpu[0] < TOL
+    # its first step already parts the FP32 and FP64 oracles by ~1e-5: held to 4x that (or 1e-5)
+    assert e_gpu[0] <= max(4.0 * e_64[0], TOL)
     assert np.median(e_gpu) <= max(2.0 * np.median(e_64), 1e-6)
     assert e_gpu.max() <= max(4.0 * e_64.max(), TOL)
     sim.destroy()
