@@ -283,9 +283,11 @@ def detector_bench(args, rebuild_every=32, sigma=0.001, radius=0.002):
 
 def contact_bench(args):
     """Tet bodies in contact: 32 bodies (16 subdivided cubes and 16 tet balls of 8,640 tets each;
-    70,304 vertices, 276,480 tets) falling in a 2 m box, dt = 1/120, 20 iterations, contact
-    radius 2 mm on every body's boundary surface (VF/FV/EE energies, friction, conservative-bound
-    truncation, re-detection after truncating iterations). One step = one frame of the scene."""
+    70,304 vertices, 276,480 tets) falling in a 2 m box, dt = 1/120, 20 iterations. Contact model
+    "cloth": contact radius 2 mm on every body's boundary surface (VF/FV/EE energies, friction,
+    conservative-bound truncation, re-detection after truncating iterations); "penetration": the
+    reference tet solver's DCD + CCD with the collision force (stiffness 1e5) and friction 0.2 in
+    the solve. One step = one frame of the scene."""
     import gaia_amd
     from gaia_amd import Batch, VBDPhysics, scenes
     from gaia_amd.autotune import parse_schedule_key, schedule_key
@@ -293,6 +295,9 @@ def contact_bench(args):
     t0 = time.perf_counter()
     bodies = scenes.c2_bodies(seed=args.seed)
     phys = scenes.c2_physics()
+    penetration = args.contact_model == "penetration"
+    if penetration:
+        phys.update(tet_contact_model=1)
     if args.query_radius is not None:
         phys["contact_query_radius"] = args.query_radius
     ctx = gaia_amd.Context(0)
@@ -342,8 +347,24 @@ def contact_bench(args):
     best = min(tune, key=tune.get)
     sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(best)}
     sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
+    graph = False
+    if penetration:
+        # the penetration frame needs no host decision: one HIP graph per frame is a candidate too
+        # (detection is then not timed inside the graph; its time comes from the direct frames)
+        sim.recordVBDSolveGraph(True)
+        restore()
+        sim.runStep()
+        restore()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            sim.runStep()
+        ctx.synchronize()
+        tune[best + "+graph"] = round((time.perf_counter() - t) / 3 * 1e3, 3)
+        graph = tune[best + "+graph"] < tune[best]
+        sim.recordVBDSolveGraph(graph)
     restore()
-    log(f"contact autotune (ms/frame): {tune} -> {best}")
+    log(f"contact autotune (ms/frame): {tune} -> {best}{' + graph' if graph else ''}")
     st0 = sim.contactStats()
     t = time.perf_counter()
     for _ in range(args.steps):
@@ -353,11 +374,26 @@ def contact_bench(args):
     st1 = sim.contactStats()
     n = args.steps
     rate = nverts * iters * n / wall
-    detect_ms = (st1["detect_ms"] - st0["detect_ms"]) / n
-    detections = (st1["detections"] - st0["detections"]) / n
+    if graph:
+        # detection time from direct frames after the timed window
+        sim.recordVBDSolveGraph(False)
+        sd = sim.contactStats()
+        for _ in range(3):
+            sim.runStep()
+        se = sim.contactStats()
+        detect_ms = (se["detect_ms"] - sd["detect_ms"]) / 3
+        detections = (se["detections"] - sd["detections"]) / 3
+    else:
+        detect_ms = (st1["detect_ms"] - st0["detect_ms"]) / n
+        detections = (st1["detections"] - st0["detections"]) / n
     lowest = min(float(sim.getState(m)[0][:, 1].min()) for m in range(len(bodies)))
-    log(f"contact: {wall / n * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s, {detections:.2f} detections/step "
-        f"({detect_ms:.2f} ms/step), last detection {st1['vf_pairs']} VF / {st1['ee_pairs']} EE candidates")
+    if penetration:
+        rows, counts = sim.penetrationResults()
+        found = f"last detection {int(counts[0])} DCD / {int(counts[2])} CCD hits"
+    else:
+        found = f"last detection {st1['vf_pairs']} VF / {st1['ee_pairs']} EE candidates"
+    log(f"contact ({args.contact_model}): {wall / n * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s, "
+        f"{detections:.2f} detections/step ({detect_ms:.2f} ms/step), {found}")
 
     # the solve kernel's launches (every parallel group, 5 times each) at the final state, timed with
     # HIP events on the library stream; algorithmic bytes from the tet gather model (contact
@@ -383,6 +419,12 @@ def contact_bench(args):
     roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                 traffic=None, kernel="group_solve_kernel", avg_launch_us=round(avg_ms * 1e3, 2),
                 algorithmic_bytes_per_launch=int(avg_bytes))
+    if penetration:
+        model = ("penetration contact (DCD of surface vertices at each substep's start, CCD of their paths, "
+                 "collision stiffness 1e5, friction 0.2), tet and surface LBVHs rebuilt every 16 frames")
+    else:
+        model = (f"contact radius 2 mm, stiffness 1e4 (quadratic-log), friction 0.2, query radius "
+                 f"{phys['contact_query_radius'] * 1000:g} mm, BVH rebuilt every 32 frames")
     out = {
         "metric": "tet-contact VBD vertex-solves/sec at 20 iters",
         "value": round(rate, 1),
@@ -399,9 +441,8 @@ def contact_bench(args):
         "config": {
             "workload": f"tet bodies in contact: 16 cubes + 16 tet balls of 8,640 tets (seed {args.seed}, seeded "
                         f"rotations, mu log-uniform 1e4-1e6, lambda 10 mu) on a 4x4x2 lattice in a 2 m box, dt=1/120, "
-                        f"20 iterations, contact radius 2 mm, stiffness 1e4 (quadratic-log), friction 0.2, query "
-                        f"radius {phys['contact_query_radius'] * 1000:g} mm, BVH rebuilt every 32 frames; frames "
-                        f"{args.warmup + 1}-{args.warmup + n} of the fall",
+                        f"20 iterations, {model}; frames {args.warmup + 1}-{args.warmup + n} of the fall",
+            "contact_model": args.contact_model,
             "bodies": len(bodies), "vertices": nverts, "tets": ntets, "iterations": iters,
             "parallel_groups": groups,
             "schedule": dict(sched, key=schedule_key(sched)),
@@ -409,7 +450,9 @@ def contact_bench(args):
             "detections_per_step": round(detections, 3),
             "detect_ms_per_step": round(detect_ms, 3),
             "solve_ms_per_step": round(solve_ms_per_step, 3),
-            "vf_candidates_last": st1["vf_pairs"], "ee_candidates_last": st1["ee_pairs"],
+            "hip_graph": graph,
+            **({"dcd_hits_last": int(counts[0]), "ccd_hits_last": int(counts[2])} if penetration else
+               {"vf_candidates_last": st1["vf_pairs"], "ee_candidates_last": st1["ee_pairs"]}),
             "lowest_vertex_y": round(lowest, 4),
         },
         "roofline": roof,
@@ -422,10 +465,17 @@ def contact_bench(args):
         from gaia_amd.vbd import physics_dict
 
         threads = host_cpus()["threads"]
-        sc = pyoracle.ClothScene(physics_dict(**phys), precision=32)
-        for b in bodies:
-            sc.add_tet_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
+        if penetration:
+            sc = pyoracle.TetScene(physics_dict(**phys), precision=32)
+            for b in bodies:
+                sc.add_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
                             colors[(b["kind"], len(b["tets"]))])
+            sc.set_collision(dict(stiffness=float(np.float32(phys.get("collision_stiffness", 1e5)))))
+        else:
+            sc = pyoracle.ClothScene(physics_dict(**phys), precision=32)
+            for b in bodies:
+                sc.add_tet_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
+                                colors[(b["kind"], len(b["tets"]))])
         sc.finalize()
         t = time.perf_counter()
         sc.step(1, threads=threads)
@@ -497,41 +547,56 @@ def serve_bench(args, schedule=None):
         f"{warm['resident_bytes'] / 1e9:.2f} GB of a {warm['budget_bytes'] / 1e9:.1f} GB budget, "
         f"{warm['evictions']} evictions")
 
-    rng = np.random.default_rng(args.seed + 7)
-    R = args.requests
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=R))
-    targets = rng.integers(0, len(sims), size=R)
-    submit = np.full(R, np.nan)
-    native = np.full(R, np.nan)
-    stepped = []
-    submitted = completed = 0
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    while completed < R:
-        now = time.perf_counter() - t0
-        while submitted < R and arrivals[submitted] <= now:
-            srv.request_step(sims[targets[submitted]], submitted)
-            submit[submitted] = time.perf_counter() - t0
-            submitted += 1
-        if submitted > completed:
-            st = srv.tick()
-            if st["last_stepped"]:
-                stepped.append(st["last_stepped"])
-            recs = srv.poll(capacity=R)
-            native[recs["tag"]] = recs["latency_ms"]
-            completed += len(recs)
-        else:
-            time.sleep(max(0.0, arrivals[submitted] - now))
-    wall = time.perf_counter() - t0
-    # from arrival: the wait for the host loop to post the request, then the server's own record
-    # (request to the completion of its frame, timed by the frame's completion event)
-    lat = (submit - arrivals) * 1e3 + native
-    solves = int(free[targets].sum()) * iters
-    rate = solves / wall
-    stats = srv.stats()
+    def timed(req_rate, R, seed):
+        """R Poisson step requests at req_rate per second, served as they arrive: wall time,
+        latencies from arrival and from submission, per-tick scene counts, vertex solves."""
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / req_rate, size=R))
+        targets = rng.integers(0, len(sims), size=R)
+        submit = np.full(R, np.nan)
+        native = np.full(R, np.nan)
+        stepped = []
+        submitted = completed = 0
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        while completed < R:
+            now = time.perf_counter() - t0
+            while submitted < R and arrivals[submitted] <= now:
+                srv.request_step(sims[targets[submitted]], submitted)
+                submit[submitted] = time.perf_counter() - t0
+                submitted += 1
+            if submitted > completed:
+                st = srv.tick()
+                if st["last_stepped"]:
+                    stepped.append(st["last_stepped"])
+                recs = srv.poll(capacity=R)
+                native[recs["tag"]] = recs["latency_ms"]
+                completed += len(recs)
+            else:
+                time.sleep(max(0.0, arrivals[submitted] - now))
+        wall = time.perf_counter() - t0
+        # from arrival: the wait for the host loop to post the request, then the server's own record
+        # (request to the completion of its frame, timed by the frame's completion event)
+        lat = (submit - arrivals) * 1e3 + native
+        return wall, lat, native, stepped, int(free[targets].sum()) * iters, float(free[targets].mean()) * iters
+
     pct = lambda a: {"p50": round(float(np.percentile(a, 50)), 3), "p99": round(float(np.percentile(a, 99)), 3),
                      "mean": round(float(a.mean()), 3), "max": round(float(a.max()), 3)}
-    offered = float(free[targets].mean()) * iters * args.rate
+    R = args.requests
+    wall, lat, native, stepped, solves, mean_solves = timed(args.rate, R, args.seed + 7)
+    rate = solves / wall
+    stats = srv.stats()
+    offered = mean_solves * args.rate
+    # a saturating point: requests offered faster than the device can step them, so the served
+    # rate is the loop's capacity (latency then measures the queue, not the service)
+    sat = None
+    if args.saturation_factor > 0:
+        sat_rate = args.rate * args.saturation_factor
+        sw, slat, _, sstep, ssolves, smean = timed(sat_rate, R, args.seed + 11)
+        sat = {"request_rate_per_s": sat_rate, "offered_vertex_solves_per_s": round(smean * sat_rate, 1),
+               "served_vertex_solves_per_s": round(ssolves / sw, 1), "latency_ms": pct(slat),
+               "scenes_per_tick_mean": round(float(np.mean(sstep)), 2), "ticks": len(sstep)}
+        log(f"serve, saturating: {sat}")
     out = {
         "metric": "served VBD vertex-solves/sec at 10 iters; step latency p50/p99 ms",
         "value": round(rate, 1),
@@ -565,6 +630,7 @@ def serve_bench(args, schedule=None):
         "resident_gb": round(stats["resident_bytes"] / 1e9, 3),
         "admissions": stats["admissions"],
         "evictions": stats["evictions"],
+        "saturation": sat,
     }
     log(f"serve: {out}")
     srv.destroy()
@@ -593,9 +659,14 @@ def main():
     ap.add_argument("--workload", choices=["batched", "detector", "serve", "contact"], default="batched",
                     help="batched VBD (default), the contact detector alone on 64 tori, served VBD, or tet "
                          "bodies in contact (32 bodies of 8,640 tets)")
+    ap.add_argument("--contact-model", choices=["cloth", "penetration"], default="cloth",
+                    help="contact workload: the cloth contact model on the bodies' surfaces, or the reference tet "
+                         "solver's penetration contact (DCD + CCD)")
     ap.add_argument("--query-radius", type=float, default=None,
                     help="tet-contact workload: contact query radius in m (default 0.003, 1.5 contact radii)")
     ap.add_argument("--serve-scenes", type=int, default=2000)
+    ap.add_argument("--saturation-factor", type=float, default=3.0,
+                    help="served VBD: a second timed window at this multiple of --rate (0: none)")
     ap.add_argument("--requests", type=int, default=20000, help="served VBD: step requests in the timed window")
     ap.add_argument("--rate", type=float, default=20000.0, help="served VBD: Poisson request rate per second")
     ap.add_argument("--budget-gb", type=float, default=288.0,
@@ -739,7 +810,7 @@ def main():
         # served leg: the served population under Poisson step requests, batched schedule
         sv = serve_bench(args, schedule=sched)
         out["served"] = {k: sv[k] for k in ("value", "unit", "latency_ms", "offered_vertex_solves_per_s",
-                                            "scenes_per_tick_mean", "ticks", "resident_gb")}
+                                            "scenes_per_tick_mean", "ticks", "resident_gb", "saturation")}
         out["served"]["workload"] = sv["config"]["workload"]
         out["served"]["schedule"] = sv["config"]["schedule"]["key"]
     print(json.dumps(out), flush=True)

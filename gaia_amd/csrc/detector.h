@@ -237,6 +237,7 @@ class ContactDetector {
     int* error_ = nullptr;      // bit 0: traversal stack overflow, bit 1: tree too deep
     int* depthHist_ = nullptr;  // per depth: first index of its run in the refit order (build)
     unsigned int* boxPartial_ = nullptr;  // per-block centroid bounds (build)
+    int boxPartialBlocks_ = 0;            // blocks boxPartial_ holds
     float4* external_ = nullptr;
     // face back-links and the solver-side state
     int* fvVerts_ = nullptr;    // nv * vfCap: vertex ids grouped by face (FV lists)

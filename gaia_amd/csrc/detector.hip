@@ -1657,7 +1657,10 @@ void ContactDetector::finalize() {
     eeOffset_ = dalloc<int64_t>(ne_ + 1);
     error_ = dalloc<int>(64);
     depthHist_ = dalloc<int>(kMaxDepth);
-    boxPartial_ = dalloc<unsigned int>(6 * (size_t)blocks_for(std::max(nf_, ne_)));
+    // per-block partial boxes of a tree build: the largest primitive count of any tree (faces,
+    // edges, and with penetration contact the tets)
+    boxPartialBlocks_ = blocks_for(std::max<int64_t>(std::max(nf_, ne_), pen_ ? (int64_t)penHost_.tetMesh.size() : 0));
+    boxPartial_ = dalloc<unsigned int>(6 * (size_t)boxPartialBlocks_);
     hip_check(hipMemsetAsync(error_, 0, 64 * sizeof(int), stream_), "detector clear");
     hip_check(hipMemsetAsync(vfPairCount_, 0, sizeof(int) * (nv_ + 1), stream_), "detector clear");
     hip_check(hipMemsetAsync(eeCount_, 0, sizeof(int) * (ne_ + 1), stream_), "detector clear");
@@ -1738,6 +1741,8 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind, const float4* boxes) {
     S.edges = edges_;
     S.boxes = boxes;
     const int n = B.n;
+    if (blocks_for(n) > boxPartialBlocks_)
+        fail(GVBD_ERR_STATE, "tree of " + std::to_string(n) + " primitives exceeds the partial-box buffer");
     unsigned int* box = (unsigned int*)sceneBox_;
     hipLaunchKernelGGL(centroid_box_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, S, kind, n, boxPartial_);
     hipLaunchKernelGGL(box_final_kernel, dim3(1), dim3(256), 0, stream_, boxPartial_, blocks_for(n), box);

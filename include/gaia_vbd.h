@@ -28,11 +28,17 @@
  *   gvbd_scene_get/set_time     BasePhysicFramework::curTime                     (Framework/BasePhysicsFramework.h:100)
  *   gvbd_scene_collision_detection VBDClothSimulationFramework::applyCollisionDetection
  *                                                                                (VBDCloth/VBDClothPhysics.cpp:719-819)
+ *   gvbd_scene_penetration_results VBDPhysics::collisionResultsAll after dcd() / ccd() (tet solver
+ *                               penetration contact)                             (VBD/VBDPhysics.h:154,
+ *                                                                                 VBD/VBDPhysics.cpp:1958-2116)
  *   gvbd_cd_*                   ClothContactDetector (initialize, updateBVH, contactQueryVF/EE)
  *                                                                                (CollisionDetector/ClothContactDetector.h:133-182)
  *   gvbd_batch_*                (no reference counterpart: many scenes per launch; the reference's
  *                                nearest form is the multi-mesh parallel-group merge, VBD/VBDPhysics.cpp:235-274)
  *   gvbd_server_*, gvbd_scene_clone  (no reference counterpart: serving loop over many scenes)
+ *   gvbd_scene_evaluate_convergence VBDPhysics::evaluateConvergenceGPU           (VBD/VBDPhysics.cpp:3510-3556)
+ *   gvbd_scene_evaluate_convergence_noalloc (no reference counterpart: allocation-free variant of
+ *                                gvbd_scene_evaluate_convergence for callers that cannot allocate)
  *
  * Conventions: every function returns GVBD_OK (0) or a negative status; the
  * message is available from gvbd_last_error(ctx) (or gvbd_last_error(NULL) for

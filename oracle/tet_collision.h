@@ -391,18 +391,43 @@ struct Hit {
     int mesh = -1, face = -1;
 };
 
+// Axis-aligned box used only to skip meshes and faces that cannot hold a hit (a speed-up of
+// the brute-force queries that leaves their results unchanged: the padding covers rounding in
+// the containment test and the root tolerance of the CCD).
+template <class R> struct Box3 {
+    R lo[3] = {R(1e30), R(1e30), R(1e30)}, hi[3] = {R(-1e30), R(-1e30), R(-1e30)};
+    void add(P3<R> p) {
+        const R c[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], c[k]), hi[k] = std::max(hi[k], c[k]);
+    }
+    void pad(R d) {
+        for (int k = 0; k < 3; ++k) lo[k] -= d, hi[k] += d;
+    }
+    bool overlaps(const Box3& o) const {
+        for (int k = 0; k < 3; ++k)
+            if (o.hi[k] < lo[k] || hi[k] < o.lo[k]) return false;
+        return true;
+    }
+    R extent() const { return std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2])); }
+};
+
 // DCD of surface vertex v of mesh qm (DiscreteCollisionDetector.cpp:514-560 with the callbacks
 // :13-95 and :97-300): the lowest containing tet of any mesh (of its own mesh only with
 // selfCollision, and no tet of its own), then the closest face of that mesh's surface that passes
 // the self-vertex, feasible-region and walk checks. `penetrated` = a containing tet was found.
+// `meshBoxes` (optional): per mesh the box of its positions padded by 1e-3 of its extent; a mesh
+// whose box does not hold the vertex holds no tet that contains it.
 template <class R> Hit dcd_vertex(const std::vector<Topo>& topo, const std::vector<const R*>& X, const Params& P,
-                                  int qm, int v, bool& penetrated) {
+                                  int qm, int v, bool& penetrated, const Box3<R>* meshBoxes = nullptr) {
     Hit h;
     penetrated = false;
     const P3<R> p = ld(X[qm], v);
+    Box3<R> pb;
+    pb.add(p);
     int hm = -1, ht = -1;
     for (int m = 0; m < (int)topo.size() && hm < 0; ++m) {
         if (m == qm && !P.selfCollision) continue;
+        if (meshBoxes && !meshBoxes[m].overlaps(pb)) continue;
         const Topo& T = topo[m];
         for (int t = 0; t < T.nt; ++t) {
             const int32_t* tv = &T.tets[4 * t];
@@ -582,19 +607,39 @@ template <class R> bool ccd_point_triangle(const P3<R> f0[3], const P3<R> f1[3],
 // CCD of surface vertex v of mesh qm (ContinuousCollisionDetector.cpp:67-190, 349-382): the face
 // (of any mesh; of its own mesh only faces without v) that the vertex's path from Xp to X meets
 // first, i.e. with the smallest t |p1 - p0|
+// `sweptBoxes` (optional): per mesh the box of its start and end positions; `maxMove`: per mesh the
+// largest vertex displacement. A face whose swept box, padded by the point's and the face's
+// largest displacement (far above the error of a root within 6e-4 in t), misses the point's path
+// cannot be hit.
 template <class R> Hit ccd_vertex(const std::vector<Topo>& topo, const std::vector<const R*>& X,
-                                  const std::vector<const R*>& Xp, int qm, int v) {
+                                  const std::vector<const R*>& Xp, int qm, int v,
+                                  const Box3<R>* sweptBoxes = nullptr, const R* maxMove = nullptr) {
     Hit h;
     const P3<R> p0 = ld(Xp[qm], v), p1 = ld(X[qm], v);
     const R len = norm(p1 - p0);
+    Box3<R> path;
+    path.add(p0);
+    path.add(p1);
     R best = 0;
     for (int m = 0; m < (int)topo.size(); ++m) {
         const Topo& T = topo[m];
+        const R margin = sweptBoxes ? (len + maxMove[m]) * R(1.01) + R(1e-6) * (R(1) + sweptBoxes[m].extent()) : R(0);
+        if (sweptBoxes) {
+            Box3<R> mb = sweptBoxes[m];
+            mb.pad(margin);
+            if (!mb.overlaps(path)) continue;
+        }
         for (int f = 0; f < (int)T.faceTet.size(); ++f) {
             const int32_t* fv = &T.faces[3 * f];
             if (m == qm && (fv[0] == v || fv[1] == v || fv[2] == v)) continue;
             const P3<R> a0[3] = {ld(Xp[m], fv[0]), ld(Xp[m], fv[1]), ld(Xp[m], fv[2])};
             const P3<R> a1[3] = {ld(X[m], fv[0]), ld(X[m], fv[1]), ld(X[m], fv[2])};
+            if (sweptBoxes) {
+                Box3<R> fb;
+                for (int k = 0; k < 3; ++k) fb.add(a0[k]), fb.add(a1[k]);
+                fb.pad(margin);
+                if (!fb.overlaps(path)) continue;
+            }
             R tt, bary[3];
             if (!ccd_point_triangle(a0, a1, p0, p1, tt, bary)) continue;
             const R depth = tt * len;

@@ -273,7 +273,12 @@ template <class R> void collision_terms(Scene<R>& s, int mi, int vid, R* f, R* h
 template <class R> void detect_dcd(Scene<R>& s) {
     const int nm = (int)s.meshes.size();
     std::vector<const R*> X(nm);
-    for (int m = 0; m < nm; ++m) X[m] = s.meshes[m].x.data();
+    std::vector<tetcol::Box3<R>> boxes(nm);
+    for (int m = 0; m < nm; ++m) {
+        X[m] = s.meshes[m].x.data();
+        for (int v = 0; v < s.meshes[m].nv; ++v) boxes[m].add(tetcol::ld(X[m], v));
+        boxes[m].pad(R(1e-3) * boxes[m].extent() + R(1e-6));
+    }
     for (int m = 0; m < nm; ++m) {
         auto& H = s.hits[m];
         std::fill(H.begin(), H.end(), tetcol::Hit{});
@@ -284,7 +289,7 @@ template <class R> void detect_dcd(Scene<R>& s) {
         auto body = [&](int v) {
             if (!T.surf[v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
             bool pen = false;
-            H[v] = tetcol::dcd_vertex<R>(s.topo, X, s.col, m, v, pen);
+            H[v] = tetcol::dcd_vertex<R>(s.topo, X, s.col, m, v, pen, boxes.data());
             s.penetrated[m][v] = pen;
         };
         if (s.pool) s.pool->parallel_for(0, T.nv, body);
@@ -296,15 +301,23 @@ template <class R> void detect_ccd(Scene<R>& s) {
     const int nm = (int)s.meshes.size();
     if (s.col.ccd) {
         std::vector<const R*> X(nm), Xp(nm);
+        std::vector<tetcol::Box3<R>> swept(nm);
+        std::vector<R> move(nm, R(0));
         for (int m = 0; m < nm; ++m) {
             X[m] = s.meshes[m].x.data();
             Xp[m] = s.meshes[m].xprev.data();
+            for (int v = 0; v < s.meshes[m].nv; ++v) {
+                const tetcol::P3<R> a = tetcol::ld(Xp[m], v), b = tetcol::ld(X[m], v);
+                swept[m].add(a);
+                swept[m].add(b);
+                move[m] = std::max(move[m], tetcol::norm(b - a));
+            }
         }
         for (int m = 0; m < nm; ++m) {
             const tetcol::Topo& T = s.topo[m];
             auto body = [&](int v) {
                 if (!T.surf[v] || s.penetrated[m][v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
-                s.hits[m][v] = tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v);
+                s.hits[m][v] = tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v, swept.data(), move.data());
                 s.fromCCD[m][v] = s.hits[m][v].face >= 0;
             };
             if (s.pool) s.pool->parallel_for(0, T.nv, body);

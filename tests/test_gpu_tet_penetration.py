@@ -33,17 +33,25 @@ def oracle_rows(o32, sizes):
     return rows
 
 
-def lockstep(gpu_ctx, oracle, spec, check_records=True):
+def lockstep(gpu_ctx, oracle, spec, check_records=True, spread=None):
+    """Device and FP32 oracle stepped from identical state; with a list `spread`, the FP64 oracle
+    too (from the FP32 oracle's state), its per-step distance to the FP32 one appended there."""
     sim = build_gpu(gpu_ctx, spec)
     o32 = build_oracle(oracle, spec)
+    o64 = build_oracle(oracle, spec, precision=64) if spread is not None else None
     nm = len(spec["meshes"])
     sizes = [len(m["pos"]) for m in spec["meshes"]]
     errs, hits, mismatched = [], 0, []
     for step in range(spec["steps"]):
         for m in range(nm):
             sim.setState(m, *o32.get_state(m))
+            if o64 is not None:
+                o64.set_state(m, *o32.get_state(m))
         sim.runStep()
         o32.step(1, threads=8)
+        if o64 is not None:
+            o64.step(1, threads=8)
+            spread.append(max(rel_err(o32.get_state(m)[0], o64.get_state(m)[0]) for m in range(nm)))
         errs.append(max(rel_err(sim.getState(m)[0], o32.get_state(m)[0]) for m in range(nm)))
         rows, counts = sim.penetrationResults()
         want = oracle_rows(o32, sizes)
@@ -64,6 +72,36 @@ def test_falling_cubes_detection_and_per_step_parity(gpu_ctx, oracle):
     assert hits > 0, "the bodies never collided"
     assert not mismatched, mismatched[:3]
     assert errs.max() < TOL, f"step {errs.argmax()}: {errs.max():.3e}"
+
+
+def test_fine_cubes_with_more_tets_than_surface_primitives(gpu_ctx, oracle):
+    """Two cubes of 8^3 cells (2,560 tets each): the tet tree has more build blocks of 256
+    primitives than the face and edge trees (20 against 9), the case whose per-block partial
+    boxes must be sized for the tets. Detections equal the FP32 oracle's at every step. The
+    impact frames are ill-conditioned (collision stiffness 1e5 on vertices of about 10 g: the FP32
+    and FP64 oracles stepped from the same state part by up to ~6e-2 and even detect differently
+    there), so positions are held to 1e-5 or, on such a frame, to 4x that frame's FP32-vs-FP64
+    distance."""
+    spec = penetration_spec(steps=12, bodies=2, n=8, h=0.0375, vy=-2.0)
+    nt = sum(len(m["tets"]) for m in spec["meshes"])
+    tets = [np.asarray(m["tets"]) for m in spec["meshes"]]
+    faces = 0
+    for t in tets:
+        f = np.sort(np.concatenate([t[:, [1, 2, 3]], t[:, [0, 2, 3]], t[:, [0, 1, 3]], t[:, [0, 1, 2]]]), axis=1)
+        _, c = np.unique(f, axis=0, return_counts=True)
+        faces += int((c == 1).sum())
+    assert (nt + 255) // 256 > (3 * faces // 2 + 255) // 256
+    spread = []
+    errs, hits, mismatched = lockstep(gpu_ctx, oracle, spec, spread=spread)
+    spread = np.array(spread)
+    print(f"\n[penetration, fine cubes] {nt} tets, {faces} surface faces; per-step rel err median "
+          f"{np.median(errs):.2e} max {errs.max():.2e}; FP32 vs FP64 median {np.median(spread):.2e} max "
+          f"{spread.max():.2e}; {hits} hits", flush=True)
+    assert hits > 0
+    assert not mismatched, mismatched[:3]
+    bound = np.maximum(TOL, 4.0 * spread)
+    assert np.all(errs <= bound), [(i, errs[i], spread[i]) for i in np.nonzero(errs > bound)[0]]
+    assert np.median(errs) < TOL
 
 
 def test_self_collision_off_and_ccd_only(gpu_ctx, oracle):
