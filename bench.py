@@ -755,8 +755,12 @@ def main():
         t = json.load(open(side)).get("schedules", {}).get(key)
         if t:
             roof["traffic"] = t["hbm_bytes_per_launch"]
+    # achievable-bandwidth reference points: the library's streaming copy kernel and the runtime's
+    # device-to-device copy (each moves 2 x copy_bytes per copy)
     copy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5)
+    memcpy_ms = ctx.stream_copy_ms(args.copy_bytes, reps=5, method=1)
     roof["measured_copy_gbs"] = round(2 * args.copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
+    roof["measured_memcpy_gbs"] = round(2 * args.copy_bytes / (memcpy_ms * 1e-3) / 1e9, 1)
     log(f"roofline: {roof}")
     cpu = None
     if not args.no_cpu_baseline:

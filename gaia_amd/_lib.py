@@ -235,6 +235,7 @@ _SIGNATURES = {
     "gvbd_last_error": [_P],
     "gvbd_synchronize": [_P],
     "gvbd_context_memory": [_P, c_int64_p],
+    "gvbd_scene_device_bytes": [_P, _I, c_int64_p, c_int64_p],
     "gvbd_scene_create": [_P, ctypes.POINTER(PhysicsParams), c_int32_p],
     "gvbd_scene_destroy": [_P, _I],
     "gvbd_scene_add_tet_mesh": [_P, _I, ctypes.POINTER(TetMeshDesc), c_int32_p],
@@ -322,6 +323,7 @@ _SIGNATURES = {
     "gvbd_cd_ee_results": [_P, _I, c_int32_p, _P],
     "gvbd_cd_min_distances": [_P, _I, c_float_p, c_float_p, c_float_p],
     "gvbd_stream_copy": [_P, ctypes.c_int64, _I, c_float_p],
+    "gvbd_copy_bandwidth": [_P, ctypes.c_int64, _I, _I, c_float_p],
     "gvbd_event_record": [_P, _I],
     "gvbd_event_elapsed_ms": [_P, _I, _I, c_float_p],
     "gvbd_tet_mesh_constants": [_I, c_float_p, _I, c_int32_p, ctypes.c_float, c_float_p, c_float_p, c_float_p],

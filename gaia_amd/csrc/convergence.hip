@@ -38,8 +38,11 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(SceneView S, int nslot
         f3 f = mk(0.f, 0.f, 0.f);
         Sym3 h{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         float cmu = 0.f;
+        const float4 mat = S.tmat[slot.w >> SLOT_MAT_SHIFT];
         for (int r = slot.y; r < slot.z; ++r) {
             const TetInc rec = S.tinc[r];
+            const float A = S.tvol[r];
+            const float Amu = A * mat.x, Alm = A * mat.y;
             const f3 m0 = mk(rec.m[0], rec.m[1], rec.m[2]);
             const f3 m1 = mk(rec.m[3], rec.m[4], rec.m[5]);
             const f3 m2 = mk(rec.m[6], rec.m[7], rec.m[8]);
@@ -54,16 +57,16 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(SceneView S, int nslot
             const float J = dot(c0, k0);
             const f3 g = mv.x * k0 + mv.y * k1 + mv.z * k2;
             const f3 Fm = mv.x * c0 + mv.y * c1 + mv.z * c2;
-            const float s = rec.Alm * (J - rec.a);
-            f = f - (rec.Amu * Fm + s * g);
-            const f3 ag = rec.Alm * g;
+            const float s = Alm * (J - mat.z);
+            f = f - (Amu * Fm + s * g);
+            const f3 ag = Alm * g;
             h.xx += ag.x * g.x;
             h.xy += ag.x * g.y;
             h.xz += ag.x * g.z;
             h.yy += ag.y * g.y;
             h.yz += ag.y * g.z;
             h.zz += ag.z * g.z;
-            cmu += rec.Amu * dot(mv, mv);
+            cmu += Amu * dot(mv, mv);
         }
         if (slot.w & SLOT_DAMPING) {
             // the solve's damping term (VBD_NeoHookean.cpp:562-574)

@@ -8,7 +8,9 @@
 //   vparam[nv]   float4  (exp vel damping, const vel damping, hydrostatic damping, deviatoric damping)
 //   vflag[nv]    uint8   bit0 fixed, bits1-3 initialisation type
 //   slots[ns]    int4    colour-major work slots: (vertex, incidence begin, incidence end, flags)
-//   tinc[ni]     TetInc  colour-major vertex->tet incidence records (64 B each, streamed)
+//   tinc[ni]     TetInc  colour-major vertex->tet incidence records (48 B each, streamed)
+//   tvol[ni]     float   rest volume of each incidence record's tet (streamed beside tinc)
+//   tmat[nm]     float4  per tet mesh (mu, lambda, a = 1 + mu / lambda, 0), indexed by slot flags >> 8
 // Slots of parallel group g are slots[groupOffset[g] .. groupOffset[g+1]); the incidence
 // records of consecutive slots are consecutive, so a block's records form one range.
 #pragma once
@@ -20,17 +22,14 @@ namespace gvbd {
 // One incident tet of one vertex v, expressed relative to v:
 //   F = sum_k (x_nb[k] - x_v) m_k^T,  m_v = -(m_0 + m_1 + m_2)
 // where m_k is the shape-matrix row of the k-th other vertex (rows of DmInv, or
-// minus their sum for tet vertex 0). Material constants are folded in:
-//   Amu = restVolume * mu, Alm = restVolume * lambda, a = 1 + mu / lambda.
+// minus their sum for tet vertex 0). The tet's rest volume A is the same record's entry of
+// the tvol stream and the material (mu, lambda, a) is its mesh's entry of tmat, so the solve
+// forms A * mu and A * lambda itself (the same float products the host would store).
 struct alignas(16) TetInc {
     int32_t nb[3];
-    float Amu;
     float m[9];  // m_0 (3), m_1 (3), m_2 (3)
-    float Alm;
-    float a;
-    float pad;
 };
-static_assert(sizeof(TetInc) == 64, "TetInc must be 64 bytes");
+static_assert(sizeof(TetInc) == 48, "TetInc must be 48 bytes");
 
 // One tet for energy evaluation: scene vertex ids, DmInv (column-major), rest volume, moduli.
 struct alignas(16) TetEnergyRec {
@@ -45,6 +44,7 @@ enum SlotFlags : int32_t {
     SLOT_INPLACE = 2,  // cloth: write x directly instead of positionsNext (no commit pass)
     SLOT_ATTACH = 8,   // cloth: the vertex has attachment records (SceneView::attBegin)
 };
+constexpr int SLOT_MAT_SHIFT = 8;  // tet slots: flags >> 8 = the mesh's entry of SceneView::tmat
 
 // One deformed vertex (VBD/VBDDeformer.h, VBDCloth/VBDClothDeformer.h): scene vertex v moved by
 // deformer d of its scene; a = root (Rotator) or initial position (tet Translator), b = RootP.
@@ -186,6 +186,8 @@ struct SceneView {
     const uint8_t* vflag;
     const int4* slots;
     const TetInc* tinc;
+    const float* tvol;   // per incidence record: its tet's rest volume
+    const float4* tmat;  // per tet mesh: (mu, lambda, a, 0)
     // cloth
     float4* pnext;  // Jacobi target of cloth solves (positionsNext)
     const ClothSlot* cslots;

@@ -88,18 +88,18 @@ __device__ __forceinline__ void boundary_force(const SolveParams& sp, float k, f
 
 // ---------------------------------------------------------------------------
 // Incidence records: read straight from HBM by the lanes that use them (STAGE = 0), or staged
-// through LDS in chunks of whole-block coalesced loads (STAGE records of 64 B per chunk). A
+// through LDS in chunks of whole-block coalesced loads (STAGE records of Q x 16 B per chunk). A
 // lane handles the same records in the same order either way, so the results are identical.
 // ---------------------------------------------------------------------------
 
-// Calls body(r) for the records k = first, first + LPV, ... < end of one lane, where r points
+// Calls body(r, k) for the records k = first, first + LPV, ... < end of one lane, where r points
 // at record k's `Q` float4s (in HBM, or in the LDS copy of the chunk that holds it). The block's
 // records are [blockBegin, blockEnd); every thread of the block must call this (barriers).
 template <int LPV, int STAGE, int Q, class Body>
 __device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs, int blockBegin, int blockEnd,
                                                  int first, int end, float4* lds, Body&& body) {
     if constexpr (STAGE == 0) {
-        for (int k = first; k < end; k += LPV) body(recs + (int64_t)Q * k);
+        for (int k = first; k < end; k += LPV) body(recs + (int64_t)Q * k, k);
     } else {
         for (int c0 = blockBegin; c0 < blockEnd; c0 += STAGE) {
             const int n = min(STAGE, blockEnd - c0);
@@ -109,7 +109,7 @@ __device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs
             int k = first;
             if (k < c0) k += ((c0 - k + LPV - 1) / LPV) * LPV;
             const int hi = min(end, c0 + n);
-            for (; k < hi; k += LPV) body(lds + Q * (k - c0));
+            for (; k < hi; k += LPV) body(lds + Q * (k - c0), k);
             __syncthreads();
         }
     }
@@ -161,19 +161,24 @@ __device__ __forceinline__ void tet_block(const SceneView& S, const WorkItem& it
         blockBegin = S.slots[it.slotBegin].y;
         blockEnd = S.slots[it.slotBegin + it.count - 1].z;
     }
-    for_lane_records<LPV, STAGE, 4>(reinterpret_cast<const float4*>(S.tinc), blockBegin, blockEnd, slot.y + sub,
-                                    slot.z, lds, [&](const float4* r) {
+    // the mesh's material (mu, lambda, a); A mu and A lambda are formed per record from its tet's
+    // rest volume, the same float products as precomputed ones
+    float4 mat = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) mat = S.tmat[slot.w >> SLOT_MAT_SHIFT];
+    const float* __restrict__ tvol = S.tvol;
+    for_lane_records<LPV, STAGE, 3>(reinterpret_cast<const float4*>(S.tinc), blockBegin, blockEnd, slot.y + sub,
+                                    slot.z, lds, [&](const float4* r, int k) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const float4 r2 = r[2];
-        const float4 r3 = r[3];
         const int n0 = __float_as_int(r0.x), n1 = __float_as_int(r0.y), n2 = __float_as_int(r0.z);
-        const float Amu = r0.w;
-        const f3 m0 = mk(r1.x, r1.y, r1.z);
-        const f3 m1 = mk(r1.w, r2.x, r2.y);
-        const f3 m2 = mk(r2.z, r2.w, r3.x);
-        const float Alm = r3.y;
-        const float a = r3.z;
+        const f3 m0 = mk(r0.w, r1.x, r1.y);
+        const f3 m1 = mk(r1.z, r1.w, r2.x);
+        const f3 m2 = mk(r2.y, r2.z, r2.w);
+        const float A = tvol[k];
+        const float Amu = A * mat.x;
+        const float Alm = A * mat.y;
+        const float a = mat.z;
         const f3 d0 = xyz(pos[n0]) - x;
         const f3 d1 = xyz(pos[n1]) - x;
         const f3 d2 = xyz(pos[n2]) - x;
@@ -624,7 +629,7 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     float fm0 = 0.f, fm1 = 0.f, fm2 = 0.f;
     float hxx = 0.f, hxy = 0.f, hxz = 0.f, hyy = 0.f, hyz = 0.f, hzz = 0.f;
     for_lane_records<LPV, 2 * STAGE, 2>(reinterpret_cast<const float4*>(S.cface), first.x, last.x, sa.y + sub, sa.z,
-                                        lds, [&](const float4* r) {
+                                        lds, [&](const float4* r, int) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const int n0 = __float_as_int(r0.x), n1 = __float_as_int(r0.y);
@@ -658,7 +663,7 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
     float fb0 = 0.f, fb1 = 0.f, fb2 = 0.f, hb = 0.f;
     const float thr2 = S.sp.degenerateThresSq;
     for_lane_records<LPV, 2 * STAGE, 2>(reinterpret_cast<const float4*>(S.cbend), first.y, last.y, sb.x + sub, sb.y,
-                                        lds, [&](const float4* r) {
+                                        lds, [&](const float4* r, int) {
         const float4 r0 = r[0];
         const float4 r1 = r[1];
         const int o = __float_as_int(r0.w);
@@ -1212,15 +1217,26 @@ hipError_t launch_update_velocity(const SceneView* scenes, const WorkItem* items
 // Plain streaming copy (float4 per lane, grid-stride): the reference kernel
 // whose HBM counters calibrate the profiler and whose rate is the measured
 // copy bandwidth of the device.
+// Grid-stride copy with four 16-byte loads in flight per lane before their stores.
 __global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
                                                           int64_t n) {
+    constexpr int U = 4;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    // 256 CUs x 8 blocks of 256 lanes; each lane moves 4 x 16 B per round
+    const int64_t blocks = std::min<int64_t>((n + 1023) / 1024, 256 * 8);
     hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, n);
     return hipGetLastError();
 }

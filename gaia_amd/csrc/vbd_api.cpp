@@ -41,6 +41,8 @@ struct DeviceScene {
     uint8_t* vflag = nullptr;
     int4* slots = nullptr;
     TetInc* tinc = nullptr;
+    float* tvol = nullptr;     // per incidence record: rest volume of its tet
+    float4* tmat = nullptr;    // per tet mesh: (mu, lambda, a, 0)
     ClothSlot* cslots = nullptr;
     FaceInc* cface = nullptr;
     BendInc* cbend = nullptr;
@@ -336,8 +338,9 @@ struct TetContactRefs {
     std::vector<int4>* ranges = nullptr;
 };
 
-void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<int4>& slots,
-                   std::vector<TetInc>& tinc, std::vector<int64_t>& work, const TetContactRefs* refs) {
+void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, int32_t mat, std::vector<int4>& slots,
+                   std::vector<TetInc>& tinc, std::vector<float>& tvol, std::vector<int64_t>& work,
+                   const TetContactRefs* refs) {
     int4 slot;
     slot.x = off + lv;
     slot.y = (int32_t)tinc.size();
@@ -359,15 +362,12 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
             for (int c = 0; c < 3; ++c) rec.m[3 * n + c] = B[j][c];
             ++n;
         }
-        rec.Amu = m.vol[t] * m.mat.mu;
-        rec.Alm = m.vol[t] * m.mat.lambda;
-        rec.a = 1.f + m.mat.mu / m.mat.lambda;
-        rec.pad = 0.f;
         tinc.push_back(rec);
+        tvol.push_back(m.vol[t]);
     }
     slot.z = (int32_t)tinc.size();
     const bool damping = m.mat.dampingHydrostatic != 0.f || m.mat.dampingDeviatoric != 0.f;
-    slot.w = damping ? SLOT_DAMPING : 0;
+    slot.w = (damping ? SLOT_DAMPING : 0) | (mat << SLOT_MAT_SHIFT);
     if (refs) {
         const TetSurface& S = *refs->surf;
         int4 r;
@@ -386,8 +386,9 @@ void emit_tet_slot(const TetMeshHost& m, int32_t off, int32_t lv, std::vector<in
         refs->ranges->push_back(r);
     }
     slots.push_back(slot);
-    // gather model: 60 B per vertex + 96 B per incident tet
-    work.push_back(60 + 96 * (int64_t)(slot.z - slot.y));
+    // gather model: 72 B per vertex (slot, material, own position, previous position, inertia
+    // target, result) + 88 B per incident tet (48 B record, 4 B rest volume, 3 x 12 B neighbours)
+    work.push_back(72 + 88 * (int64_t)(slot.z - slot.y));
 }
 
 // Colour-major cloth slot, face and bending records of one vertex.
@@ -469,6 +470,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const int32_t G = (int32_t)s.groups.size();
     std::vector<int4> slots;
     std::vector<TetInc> tinc;
+    std::vector<float> tvol;
     std::vector<ClothSlot> cslots;
     std::vector<FaceInc> cface;
     std::vector<BendInc> cbend;
@@ -571,8 +573,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
             if (fixedMask[im][lv]) continue;
             const int32_t off = s.vertexOffset[im];
             if (s.meshes[im].first == MESH_TET) {
-                emit_tet_slot(s.tets[s.meshes[im].second], off, lv, slots, tinc, tetWork,
-                              contact ? &trefs[im] : nullptr);
+                emit_tet_slot(s.tets[s.meshes[im].second], off, lv, s.meshes[im].second, slots, tinc, tvol,
+                              tetWork, contact ? &trefs[im] : nullptr);
             } else {
                 // attachments read other vertices of the group: positionsNext keeps them Jacobi
                 const bool inplace = !multiColor[im] && !s.params.handle_collision && attOf.empty();
@@ -695,6 +697,13 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bF = align((size_t)std::max(nv, 1));
     const size_t bS = align(sizeof(int4) * std::max<size_t>(slots.size(), 1));
     const size_t bI = align(sizeof(TetInc) * std::max<size_t>(tinc.size(), 1));
+    const size_t bIV = align(sizeof(float) * std::max<size_t>(tvol.size(), 1));
+    std::vector<float4> tmat(std::max<size_t>(s.tets.size(), 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t t = 0; t < s.tets.size(); ++t) {
+        const auto& mt = s.tets[t].mat;
+        tmat[t] = make_float4(mt.mu, mt.lambda, 1.f + mt.mu / mt.lambda, 0.f);
+    }
+    const size_t bTM = align(sizeof(float4) * tmat.size());
     const size_t bCS = align(sizeof(ClothSlot) * std::max<size_t>(cslots.size(), 1));
     const size_t bCF = align(sizeof(FaceInc) * std::max<size_t>(cface.size(), 1));
     const size_t bCB = align(sizeof(BendInc) * std::max<size_t>(cbend.size(), 1));
@@ -721,7 +730,7 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     const size_t bDP = dparams.empty() ? 0 : align(sizeof(DeformParams) * dparams.size());
     const size_t bAB = attBegin.empty() ? 0 : align(sizeof(int32_t) * attBegin.size());
     const size_t bAR = att.empty() ? 0 : align(sizeof(AttachRec) * att.size());
-    const size_t total = 6 * bV + bF + bS + bI + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW +
+    const size_t total = 6 * bV + bF + bS + bI + bIV + bTM + bCS + bCF + bCB + bFR + bER + bFr + bTF + bTE + bTC + 2 * bA + bW +
                          bClk + bZ + bDV + bDP + bAB + bAR + bSn + bVG;
     DeviceScene& d = s.dev;
     if (reserve) reserve(total);
@@ -748,6 +757,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     d.vflag = (uint8_t*)take(bF);
     d.slots = (int4*)take(bS);
     d.tinc = (TetInc*)take(bI);
+    d.tvol = (float*)take(bIV);
+    d.tmat = (float4*)take(bTM);
     d.cslots = (ClothSlot*)take(bCS);
     d.cface = (FaceInc*)take(bCF);
     d.cbend = (BendInc*)take(bCB);
@@ -788,6 +799,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     up(d.vflag, vflag.data(), nv);
     up(d.slots, slots.data(), sizeof(int4) * slots.size());
     up(d.tinc, tinc.data(), sizeof(TetInc) * tinc.size());
+    up(d.tvol, tvol.data(), sizeof(float) * tvol.size());
+    up(d.tmat, tmat.data(), sizeof(float4) * tmat.size());
     up(d.cslots, cslots.data(), sizeof(ClothSlot) * cslots.size());
     up(d.cface, cface.data(), sizeof(FaceInc) * cface.size());
     up(d.cbend, cbend.data(), sizeof(BendInc) * cbend.size());
@@ -829,6 +842,8 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
     v.vflag = d.vflag;
     v.slots = d.slots;
     v.tinc = d.tinc;
+    v.tvol = d.tvol;
+    v.tmat = d.tmat;
     v.pnext = d.pnext;
     v.cslots = d.cslots;
     v.cface = d.cface;
@@ -1659,6 +1674,15 @@ int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes) {
             for (const PlanObj& P : kv.second->plans) b += (int64_t)P.bytes;
         for (auto& kv : ctx->detectors) b += (int64_t)kv.second->deviceBytes();
         *bytes = b;
+    });
+}
+
+int gvbd_scene_device_bytes(gvbd_context* ctx, int32_t scene, int64_t* arena, int64_t* detector) {
+    return guard(ctx, [&] {
+        require(ctx && arena && detector, "NULL argument");
+        const SceneObj& s = scene_of(ctx, scene);
+        *arena = s.dev.arena ? (int64_t)s.dev.bytes : 0;
+        *detector = s.cd ? (int64_t)s.cd->deviceBytes() : 0;
     });
 }
 
@@ -2860,12 +2884,14 @@ static void rebase_scene(SceneObj& s, uintptr_t from, uintptr_t to) {
     };
     DeviceScene& d = s.dev;
     mv(d.pos), mv(d.prev), mv(d.inertia), mv(d.vel), mv(d.vparam), mv(d.pnext), mv(d.vflag), mv(d.slots);
-    mv(d.tinc), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef), mv(d.friction);
+    mv(d.tinc), mv(d.tvol), mv(d.tmat), mv(d.cslots), mv(d.cface), mv(d.cbend), mv(d.faceRef), mv(d.edgeRef);
+    mv(d.friction);
     mv(d.tFaceRef), mv(d.tEdgeRef), mv(d.tcontact), mv(d.accPrevPrev), mv(d.accPrevIter), mv(d.accOmega);
     mv(d.snap), mv(d.vgroup);
     mv(d.clock), mv(d.vzone), mv(d.dverts), mv(d.dparams), mv(d.attBegin), mv(d.att);
     SceneView& v = s.view;
     mv(v.pos), mv(v.prev), mv(v.inertia), mv(v.vel), mv(v.vparam), mv(v.vflag), mv(v.slots), mv(v.tinc);
+    mv(v.tvol), mv(v.tmat);
     mv(v.pnext), mv(v.cslots), mv(v.cface), mv(v.cbend), mv(v.tcontact), mv(v.accPrevPrev), mv(v.accPrevIter);
     mv(v.accOmega), mv(v.clock), mv(v.vzone), mv(v.dverts), mv(v.dparams), mv(v.attBegin), mv(v.att);
     mv(v.solveFail);
@@ -3375,8 +3401,13 @@ int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex, float* f
 }
 
 int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms) {
+    return gvbd_copy_bandwidth(ctx, bytes, reps, 0, avg_ms);
+}
+
+int gvbd_copy_bandwidth(gvbd_context* ctx, int64_t bytes, int32_t reps, int32_t method, float* avg_ms) {
     return guard(ctx, [&] {
-        require(ctx && avg_ms && bytes > 0 && bytes % 16 == 0 && reps > 0, "bad argument");
+        require(ctx && avg_ms && bytes > 0 && bytes % 16 == 0 && reps > 0 && (method == 0 || method == 1),
+                "bad argument");
         void *a = nullptr, *b = nullptr;
         if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -3386,10 +3417,15 @@ int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_
         try {
             hip_check(hipMemsetAsync(a, 0, bytes, ctx->stream), "memset");
             const int64_t n = bytes / 16;
-            hip_check(launch_stream_copy((const float4*)a, (float4*)b, n, ctx->stream), "copy launch");
+            auto copy = [&] {
+                if (method == 0)
+                    hip_check(launch_stream_copy((const float4*)a, (float4*)b, n, ctx->stream), "copy launch");
+                else
+                    hip_check(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+            };
+            copy();
             hip_check(hipEventRecord(ctx->events[62], ctx->stream), "event");
-            for (int32_t r = 0; r < reps; ++r)
-                hip_check(launch_stream_copy((const float4*)a, (float4*)b, n, ctx->stream), "copy launch");
+            for (int32_t r = 0; r < reps; ++r) copy();
             hip_check(hipEventRecord(ctx->events[63], ctx->stream), "event");
             hip_check(hipEventSynchronize(ctx->events[63]), "event sync");
             float ms = 0.f;

@@ -176,10 +176,11 @@ class Context:
         self.check(lib().gvbd_context_memory(self.handle, ctypes.byref(b)))
         return b.value
 
-    def stream_copy_ms(self, nbytes, reps=5):
-        """Average time of a plain device copy of nbytes (measured copy bandwidth)."""
+    def stream_copy_ms(self, nbytes, reps=5, method=0):
+        """Average time of a device copy of nbytes (measured copy bandwidth): method 0 the
+        library's streaming kernel, 1 the runtime's device-to-device hipMemcpyAsync."""
         ms = ctypes.c_float()
-        self.check(lib().gvbd_stream_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)))
+        self.check(lib().gvbd_copy_bandwidth(self.handle, int(nbytes), reps, int(method), ctypes.byref(ms)))
         return ms.value
 
     def event_record(self, slot):
@@ -524,6 +525,14 @@ class VBDPhysics:
         st = ContactStats()
         self.ctx.check(lib().gvbd_scene_contact_stats(self.ctx.handle, self.scene, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in ContactStats._fields_}
+
+    def deviceBytes(self):
+        """Device bytes of the scene as a Server's budget counts them: {"arena": ..., "detector": ...}
+        (the arena is 0 while the scene is not uploaded or is copied out)."""
+        a = ctypes.c_int64()
+        d = ctypes.c_int64()
+        self.ctx.check(lib().gvbd_scene_device_bytes(self.ctx.handle, self.scene, ctypes.byref(a), ctypes.byref(d)))
+        return {"arena": a.value, "detector": d.value}
 
     def penetrationResults(self):
         """The last detection of a scene with the penetration contact model (tet_contact_model 1), one

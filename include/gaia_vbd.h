@@ -233,6 +233,10 @@ GVBD_API const char* gvbd_last_error(const gvbd_context* ctx);
 GVBD_API int gvbd_synchronize(gvbd_context* ctx);
 /* device bytes currently held by scenes and plans of this context */
 GVBD_API int gvbd_context_memory(gvbd_context* ctx, int64_t* bytes);
+/* device bytes of one scene as a server's memory budget counts them (no reference counterpart):
+ * its arena (0 while not uploaded or copied out) and its contact detector (trees and detection
+ * lists, resident while the scene exists); its launch plans are not included */
+GVBD_API int gvbd_scene_device_bytes(gvbd_context* ctx, int32_t scene, int64_t* arena, int64_t* detector);
 
 /* ---- scenes ---- */
 GVBD_API int gvbd_scene_create(gvbd_context* ctx, const gvbd_physics_params* params, int32_t* scene_out);
@@ -556,6 +560,9 @@ GVBD_API int gvbd_cd_min_distances(gvbd_context* ctx, int32_t cd, float* vertex,
  * streaming kernel and returns the average kernel time: the measured copy bandwidth and
  * the calibration point for profiler byte counters (no reference counterpart) */
 GVBD_API int gvbd_stream_copy(gvbd_context* ctx, int64_t bytes, int32_t reps, float* avg_ms);
+/* the same measurement by method: 0 the library's streaming kernel (gvbd_stream_copy), 1 the
+ * runtime's device-to-device hipMemcpyAsync (no reference counterpart) */
+GVBD_API int gvbd_copy_bandwidth(gvbd_context* ctx, int64_t bytes, int32_t reps, int32_t method, float* avg_ms);
 GVBD_API int gvbd_event_record(gvbd_context* ctx, int32_t slot);  /* slot 0..63 */
 GVBD_API int gvbd_event_elapsed_ms(gvbd_context* ctx, int32_t slot_a, int32_t slot_b, float* ms);
 

@@ -63,6 +63,8 @@ def lib():
         L.oracle_tet_scene_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oracle_tet_scene_set_time.restype = None
         L.oracle_tet_scene_set_collision.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_tet_scene_set_box_prefilter.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_tet_scene_set_box_prefilter.restype = None
         L.oracle_tet_scene_collisions.argtypes = [ctypes.c_void_p, _ip]
         L.oracle_tet_scene_surface.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip]
         L.oracle_point_in_tet_f32.argtypes = [_fp] * 5
@@ -315,6 +317,10 @@ class TetScene:
         reference's defaults (keys of OC_KEYS); before finalize."""
         v, pv = _d(collision_vector(params))
         lib().oracle_tet_scene_set_collision(self._h, pv)
+
+    def set_box_prefilter(self, on):
+        """Box prefilter of the brute-force DCD / CCD (default on; the results are the same)."""
+        lib().oracle_tet_scene_set_box_prefilter(self._h, int(bool(on)))
 
     def collisions(self):
         """The last detection, one row per surface vertex: (mesh, vertex, hit mesh, hit face,

@@ -114,6 +114,7 @@ template <class R> struct Scene {
     // detection (hit mesh and face, whether a tet contained it, whether CCD found it) and the
     // relations (colliding mesh, colliding vertex, order) of the vertices of hit faces
     bool collide = false;
+    bool boxPrefilter = true;  // skip meshes / faces by box in the brute-force queries
     tetcol::Params col;
     std::vector<tetcol::Topo> topo;
     std::vector<std::vector<tetcol::Hit>> hits;
@@ -289,7 +290,7 @@ template <class R> void detect_dcd(Scene<R>& s) {
         auto body = [&](int v) {
             if (!T.surf[v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
             bool pen = false;
-            H[v] = tetcol::dcd_vertex<R>(s.topo, X, s.col, m, v, pen, boxes.data());
+            H[v] = tetcol::dcd_vertex<R>(s.topo, X, s.col, m, v, pen, s.boxPrefilter ? boxes.data() : nullptr);
             s.penetrated[m][v] = pen;
         };
         if (s.pool) s.pool->parallel_for(0, T.nv, body);
@@ -317,7 +318,8 @@ template <class R> void detect_ccd(Scene<R>& s) {
             const tetcol::Topo& T = s.topo[m];
             auto body = [&](int v) {
                 if (!T.surf[v] || s.penetrated[m][v] || !(s.meshes[m].x[3 * v + 1] < (R)s.col.offHeight)) return;
-                s.hits[m][v] = tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v, swept.data(), move.data());
+                s.hits[m][v] = s.boxPrefilter ? tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v, swept.data(), move.data())
+                                              : tetcol::ccd_vertex<R>(s.topo, X, Xp, m, v);
                 s.fromCCD[m][v] = s.hits[m][v].face >= 0;
             };
             if (s.pool) s.pool->parallel_for(0, T.nv, body);
@@ -630,6 +632,13 @@ int oracle_tet_scene_finalize(OracleTetScene* sc) {
     if (sc->precision == 64) finalize_scene(sc->s64);
     else finalize_scene(sc->s32);
     return 0;
+}
+
+// Box prefilter of the brute-force DCD / CCD on (default) or off; a speed-up only, the results are
+// the same either way
+void oracle_tet_scene_set_box_prefilter(OracleTetScene* sc, int on) {
+    sc->s32.boxPrefilter = on != 0;
+    sc->s64.boxPrefilter = on != 0;
 }
 
 int oracle_tet_scene_set_collision(OracleTetScene* sc, const double* c) {

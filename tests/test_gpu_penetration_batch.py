@@ -117,18 +117,15 @@ def test_served_penetration_scenes_equal_alone_under_eviction(gpu_ctx):
                                **m["material"])
         sim.initialize(defer_upload=True)
         sims.append(sim)
-    # the budget: every scene's detector (resident from its admission) plus half of the arenas;
-    # a scene's arena is measured without collision handling, its detector as the difference
-    def device_bytes(spec, contact):
-        sp = dict(spec, physics=dict(spec["physics"], handle_collision=contact))
-        before = gpu_ctx.memory_bytes()
-        probe = build_gpu(gpu_ctx, sp)
-        b = gpu_ctx.memory_bytes() - before
+    # the budget: every scene's detector (resident from its admission) plus half of the arenas,
+    # as the server counts them (a probe of each scene)
+    def device_bytes(spec):
+        probe = build_gpu(gpu_ctx, spec)
+        b = probe.deviceBytes()
         probe.destroy()
-        return b
+        return b["arena"], b["detector"]
 
-    arenas = [device_bytes(sp, 0) for sp in specs]
-    detectors = [device_bytes(sp, sp["physics"]["handle_collision"]) - a for sp, a in zip(specs, arenas)]
+    arenas, detectors = zip(*[device_bytes(sp) for sp in specs])
     budget = int(sum(detectors) + max(0.5 * sum(arenas), 1.1 * max(arenas)))
     print(f"\n[served penetration] arenas {arenas}, detectors {detectors}, budget {budget}", flush=True)
     srv = Server(gpu_ctx, budget)

@@ -341,3 +341,31 @@ def test_ccd_catches_a_vertex_crossing_a_slab(oracle):
     assert np.allclose(ps[faces[r[3]], 1], 0.2)  # the slab's top face
     x, _ = sc.get_state(1)
     assert x[3, 1] > 0.0  # pushed back up from y = -0.05
+
+
+def test_box_prefilter_leaves_detections_and_steps_unchanged(oracle):
+    """The brute-force DCD / CCD skip meshes and faces by box (a speed-up for large scenes): with
+    and without it, fast bodies hitting each other and the ground (CCD hits included) detect and
+    step identically, bit for bit."""
+    import gaia_amd
+    from vbd_cases import penetration_spec
+
+    for seed, bodies, vy in [(1, 3, -3.0), (2, 4, -6.0), (3, 3, -10.0)]:
+        spec = penetration_spec(steps=0, bodies=bodies, vy=vy, seed=seed)
+        runs = []
+        for on in (True, False):
+            sc = oracle.TetScene(spec["physics"], precision=32)
+            for m in spec["meshes"]:
+                sc.add_mesh(np.asarray(m["pos"], np.float32).astype(np.float64), m["tets"], m["material"],
+                            gaia_amd.color_tet_mesh(len(m["pos"]), m["tets"], m["coloring"]), vel=m["vel"])
+            sc.set_collision(dict(stiffness=1e5))
+            sc.set_box_prefilter(on)
+            sc.finalize()
+            cols = []
+            for _ in range(25):
+                sc.step(1, threads=4)
+                cols.append(sc.collisions())
+            runs.append((np.stack(cols), np.concatenate([sc.get_state(m)[0] for m in range(bodies)])))
+        (ca, xa), (cb, xb) = runs
+        assert int(ca[:, :, 5].sum()) > 0 and int((ca[:, :, 3] >= 0).sum()) > 0
+        assert np.array_equal(ca, cb) and np.array_equal(xa, xb), seed
