@@ -51,6 +51,8 @@ struct DeviceBvh {
     int* depth = nullptr;            // n - 1
     void* sortTemp = nullptr;
     size_t sortTempBytes = 0;
+    int* mesh = nullptr;             // 2n - 1 (face trees of penetration contact): the mesh of every
+                                     // face below the node, -1 where faces of several meshes meet
 };
 
 // Penetration contact of tet meshes (the reference tet solver's DCD and CCD,
@@ -86,7 +88,7 @@ struct PenetrationSetup {
 
 // counts of the last detection pass
 struct PenetrationCounts {
-    int dcdHits, penetratedNoFace, ccdHits, walkOverflows;
+    int dcdHits, penetratedNoFace, ccdHits, walkOverflows, fullSearches;
 };
 
 struct DetectorResultCounts {
@@ -261,7 +263,10 @@ class ContactDetector {
     int2* penFaceOwner_ = nullptr;
     float4 *tetBoxes_ = nullptr, *sweptBoxes_ = nullptr;
     uint8_t* penFlags_ = nullptr;
-    int* penCounts_ = nullptr;  // 4
+    int* penCounts_ = nullptr;  // kPenCounts: DCD hits, penetrated without a face, CCD hits, walk
+                                // overflows, contained vertices listed by the DCD's first pass
+    int* penInTet_ = nullptr;   // per vertex: its containing tet (vertices listed by the DCD)
+    int* penList_ = nullptr;    // per surface vertex slot: the contained vertices of the DCD
     DeviceBvh tb_, sb_;         // tets, faces over their path in a substep
     bool sweptBuilt_ = false;
 };

@@ -824,6 +824,39 @@ __global__ __launch_bounds__(1024) void internal_box_levels_kernel(const int* le
     }
 }
 
+// Mesh of every node of a face tree (the faces' mesh ids, w of int4 faces), -1 for nodes whose
+// faces belong to several meshes: leaves, then the internal levels deepest first as the refit.
+__global__ __launch_bounds__(256) void node_mesh_leaf_kernel(const int4* faces, const BvhNode* nodes, int n,
+                                                             int* mesh) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    mesh[n - 1 + k] = faces[__float_as_int(nodes[n - 1 + k].lo.w)].w;
+}
+
+__device__ __forceinline__ int merge_mesh(int a, int b) { return a == b ? a : -1; }
+
+__global__ __launch_bounds__(256) void node_mesh_internal_kernel(const int* levelNodes, int begin, int end,
+                                                                 const BvhNode* nodes, const int* rightChild,
+                                                                 int* mesh) {
+    const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    const int i = levelNodes[k];
+    mesh[i] = merge_mesh(mesh[__float_as_int(nodes[i].lo.w)], mesh[rightChild[i]]);
+}
+
+__global__ __launch_bounds__(1024) void node_mesh_levels_kernel(const int* levelNodes, const int* levelBegin, int l0,
+                                                                int l1, const BvhNode* nodes, const int* rightChild,
+                                                                int* mesh) {
+    for (int l = l0; l < l1; ++l) {
+        const int b = levelBegin[l], e = levelBegin[l + 1];
+        for (int k = b + (int)threadIdx.x; k < e; k += (int)blockDim.x) {
+            const int i = levelNodes[k];
+            mesh[i] = merge_mesh(mesh[__float_as_int(nodes[i].lo.w)], mesh[rightChild[i]]);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_int_kernel(int* p, int64_t n, int value) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = value;
@@ -906,6 +939,8 @@ __global__ __launch_bounds__(256) void fv_select_kernel(const int* offsets, int 
 // (DiscreteCollisionDetector.cpp:97-300,697-815; TetMeshFEM.cpp:503-735); CCD finds the face a
 // surface vertex's path meets first (ContinuousCollisionDetector.cpp:67-190; CCDSolver.h).
 
+constexpr int kPenCounts = 8;   // penetration counters (5 used)
+constexpr int kNearestRounds = 8;  // nearest-face rounds of the DCD before its single-traversal search
 constexpr int kPenStack = 64;   // candidate exit faces of a walk
 constexpr int kPenRecent = 32;  // recently entered tets checked for loops
 // local face k of a tet is opposite its vertex k, vertices in this order (TetMeshFEM.cpp:22)
@@ -928,6 +963,7 @@ struct PenView {
     const BvhNode* tNodes;   // tet tree
     int tN;
     const BvhNode* fNodes;   // face tree (current positions)
+    const int* fMesh;        // per face-tree node: the mesh of its faces, -1 for several
     int fN;
     const BvhNode* sNodes;   // swept-face tree (prev and current positions)
     int sN;
@@ -936,7 +972,11 @@ struct PenView {
     VFContactRec* rec;       // per vertex, cap records: rec[v * cap].face = hit face
     int cap;
     uint8_t* flags;          // per vertex: 1 a tet contained it, 2 the hit came from CCD
-    int* counts;             // [0] DCD hits, [1] penetrated without a face, [2] CCD hits, [3] walk overflows
+    int* counts;             // [0] DCD hits, [1] penetrated without a face, [2] CCD hits, [3] walk overflows,
+                             // [4] contained vertices listed for the closest-face pass, [5] closest-face
+                             // searches past the nearest rounds
+    int* inTet;              // per contained vertex: the lowest tet containing it
+    int* list;               // contained vertices (order of discovery), counts[4] of them
     int* error;
 };
 
@@ -1109,8 +1149,10 @@ __device__ __forceinline__ float box_dist2(V3 p, const BvhNode& n) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// DCD of every surface vertex (one thread each) at the current positions
-__global__ __launch_bounds__(64) void pen_dcd_kernel(PenView S) {
+// DCD pass 1, one thread per surface vertex: the lowest tet containing the vertex (of any mesh;
+// of its own mesh only with self collision, and no tet of its own). Contained vertices are
+// listed for pass 2, so that its waves hold only vertices with a closest-face search to do.
+__global__ __launch_bounds__(64) void pen_contain_kernel(PenView S) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.nSurf) return;
     const int v = S.surfVerts[i];
@@ -1119,55 +1161,18 @@ __global__ __launch_bounds__(64) void pen_dcd_kernel(PenView S) {
     const V3 p = ld(S.pos, v);
     if (!S.P.dcd || !(p.y < S.P.offHeight)) return;
     const int qm = S.vMesh[v];
-    // the lowest containing tet
     int ht = -1;
-    {
-        const int firstLeaf = S.tN - 1;
-        int node = S.tN > 0 ? 0 : -1;
-        for (int visits = 0; node >= 0; ++visits) {
-            if (visits > 2 * S.tN) {
-                atomicOr(S.error, 1);
-                break;
-            }
-            const BvhNode nd = S.tNodes[node];
-            const int skip = __float_as_int(nd.hi.w);
-            if (!(p.x >= nd.lo.x && p.x <= nd.hi.x && p.y >= nd.lo.y && p.y <= nd.hi.y && p.z >= nd.lo.z &&
-                  p.z <= nd.hi.z)) {
-                node = skip;
-                continue;
-            }
-            if (node < firstLeaf) {
-                node = __float_as_int(nd.lo.w);
-                continue;
-            }
-            node = skip;
-            const int t = __float_as_int(nd.lo.w);
-            if (ht >= 0 && t > ht) continue;
-            const int m = S.tetMesh[t];
-            if (m == qm && !S.P.self) continue;
-            const int4 tv = S.tets[t];
-            if (m == qm && (tv.x == v || tv.y == v || tv.z == v || tv.w == v)) continue;
-            if (point_in_tet(p, ld(S.pos, tv.x), ld(S.pos, tv.y), ld(S.pos, tv.z), ld(S.pos, tv.w))) ht = t;
-        }
-    }
-    if (ht < 0) return;
-    S.flags[v] = 1;
-    const int hm = S.tetMesh[ht];
-    const float c = S.P.centerShift;
-    // the closest face of mesh hm passing the checks (lower face id on equal distance)
-    float best = FLT_MAX;
-    int bestF = -1;
-    const int firstLeaf = S.fN - 1;
-    int node = S.fN > 0 ? 0 : -1;
+    const int firstLeaf = S.tN - 1;
+    int node = S.tN > 0 ? 0 : -1;
     for (int visits = 0; node >= 0; ++visits) {
-        if (visits > 2 * S.fN) {
+        if (visits > 2 * S.tN) {
             atomicOr(S.error, 1);
             break;
         }
-        const BvhNode nd = S.fNodes[node];
+        const BvhNode nd = S.tNodes[node];
         const int skip = __float_as_int(nd.hi.w);
-        // prune only boxes surely farther than the best distance (margin for rounding)
-        if (bestF >= 0 && box_dist2(p, nd) > best * best * 1.0001f + 1e-12f) {
+        if (!(p.x >= nd.lo.x && p.x <= nd.hi.x && p.y >= nd.lo.y && p.y <= nd.hi.y && p.z >= nd.lo.z &&
+              p.z <= nd.hi.z)) {
             node = skip;
             continue;
         }
@@ -1176,40 +1181,170 @@ __global__ __launch_bounds__(64) void pen_dcd_kernel(PenView S) {
             continue;
         }
         node = skip;
-        const int f = __float_as_int(nd.lo.w);
-        const int4 fv = S.faces[f];
-        if (fv.w != hm) continue;
-        int type;
-        const V3 a = ld(S.pos, fv.x), b = ld(S.pos, fv.y), cc = ld(S.pos, fv.z);
-        const V3 cp = closest_point_triangle(p, a, b, cc, type);
-        if (hm == qm && type <= AT_C && (type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z)) == v) continue;
-        const float d = vnorm(p - cp);
-        if (bestF >= 0 && !(d < best || (d == best && f < bestF))) continue;
-        if (S.P.feasible && !pen_feasible(S, p, f, type)) continue;
-        if (S.P.traverse && (hm == qm || S.P.traverseNonSelf)) {
-            const V3 from = type == AT_INTERIOR ? cp : (1.f - c) * cp + (c / 3.f) * ((a + b) + cc);
-            V3 to = p;
-            if (S.P.shiftToCenter) {
-                const int4 tv = S.tets[ht];
-                const V3 cen = 0.25f * (((ld(S.pos, tv.x) + ld(S.pos, tv.y)) + ld(S.pos, tv.z)) + ld(S.pos, tv.w));
-                to = (1.f - c) * p + c * cen;
-            }
-            V3 dir = to - from;
-            const float len = vnorm(dir);
-            dir = (1.f / len) * dir;
-            const float maxDis = S.P.stopAfterPassing ? S.P.maxSearchMult * len : -1.f;
-            const int2 own = S.faceOwner[f];
-            if (!pen_walk(S, from, dir, maxDis, own.x, own.y, ht)) continue;
-        }
-        best = d;
-        bestF = f;
+        const int t = __float_as_int(nd.lo.w);
+        if (ht >= 0 && t > ht) continue;
+        const int m = S.tetMesh[t];
+        if (m == qm && !S.P.self) continue;
+        const int4 tv = S.tets[t];
+        if (m == qm && (tv.x == v || tv.y == v || tv.z == v || tv.w == v)) continue;
+        if (point_in_tet(p, ld(S.pos, tv.x), ld(S.pos, tv.y), ld(S.pos, tv.z), ld(S.pos, tv.w))) ht = t;
     }
-    if (bestF < 0) {
+    if (ht < 0) return;
+    S.flags[v] = 1;
+    S.inTet[v] = ht;
+    S.list[atomicAdd(S.counts + 4, 1)] = v;
+}
+
+// Whether surface face f (closest point cp of region `type`) is a valid hit of vertex v (at p,
+// inside tet ht of mesh hm): the feasible-region check and the ray walk from the face to the
+// query point (DiscreteCollisionDetector.cpp:97-300).
+__device__ bool pen_face_valid(const PenView& S, int qm, int hm, int ht, V3 p, int f, int type, V3 cp) {
+    if (S.P.feasible && !pen_feasible(S, p, f, type)) return false;
+    if (S.P.traverse && (hm == qm || S.P.traverseNonSelf)) {
+        const float c = S.P.centerShift;
+        const int4 fv = S.faces[f];
+        const V3 a = ld(S.pos, fv.x), b = ld(S.pos, fv.y), cc = ld(S.pos, fv.z);
+        const V3 from = type == AT_INTERIOR ? cp : (1.f - c) * cp + (c / 3.f) * ((a + b) + cc);
+        V3 to = p;
+        if (S.P.shiftToCenter) {
+            const int4 tv = S.tets[ht];
+            const V3 cen = 0.25f * (((ld(S.pos, tv.x) + ld(S.pos, tv.y)) + ld(S.pos, tv.z)) + ld(S.pos, tv.w));
+            to = (1.f - c) * p + c * cen;
+        }
+        V3 dir = to - from;
+        const float len = vnorm(dir);
+        dir = (1.f / len) * dir;
+        const float maxDis = S.P.stopAfterPassing ? S.P.maxSearchMult * len : -1.f;
+        const int2 own = S.faceOwner[f];
+        if (!pen_walk(S, from, dir, maxDis, own.x, own.y, ht)) return false;
+    }
+    return true;
+}
+
+// Squared distance from p to the farthest point of a node's box.
+__device__ __forceinline__ float box_far2(V3 p, const BvhNode& n) {
+    const float dx = fmaxf(p.x - n.lo.x, n.hi.x - p.x);
+    const float dy = fmaxf(p.y - n.lo.y, n.hi.y - p.y);
+    const float dz = fmaxf(p.z - n.lo.z, n.hi.z - p.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// DCD pass 2, one thread per contained vertex: the closest face of the containing tet's mesh
+// that passes the checks, lower face id on equal distance (DiscreteCollisionDetector.cpp:
+// 514-560). The faces are first taken in ascending (distance, id), the first one passing the
+// checks being the answer: each round is a nearest-face query on the face tree over the faces
+// after the last rejected one, pruning subtrees of other meshes, boxes surely farther than the
+// round's best and boxes surely closer than the rejected face. The nearest face passes for
+// nearly every vertex, so the costly checks run about once per vertex. After kNearestRounds
+// rejected faces the rest is searched in one traversal that checks every face closer than the
+// best valid one so far (each round repeats a traversal, which costs more than the extra checks
+// once many faces are rejected). Both orders give the same answer.
+__global__ __launch_bounds__(64) void pen_closest_kernel(PenView S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.counts[4]) return;
+    const int v = S.list[i];
+    const int ht = S.inTet[v];
+    const V3 p = ld(S.pos, v);
+    const int qm = S.vMesh[v];
+    const int hm = S.tetMesh[ht];
+    float lastD = -1.f;
+    int lastF = -1;
+    int hit = -1;
+    bool exhausted = false;
+    const int firstLeaf = S.fN - 1;
+    for (int round = 0; round < kNearestRounds && hit < 0 && !exhausted; ++round) {
+        float best = FLT_MAX;
+        int bestF = -1, bestType = 0;
+        V3 bestCp = p;
+        const float near2 = lastD * lastD;
+        int node = S.fN > 0 ? 0 : -1;
+        for (int visits = 0; node >= 0; ++visits) {
+            if (visits > 2 * S.fN) {
+                atomicOr(S.error, 1);
+                break;
+            }
+            const BvhNode nd = S.fNodes[node];
+            const int skip = __float_as_int(nd.hi.w);
+            // prune subtrees of other meshes' faces, boxes surely farther than the round's best and
+            // boxes surely closer than the last rejected face (margins for rounding)
+            const int nm = S.fMesh[node];
+            if ((nm >= 0 && nm != hm) || (bestF >= 0 && box_dist2(p, nd) > best * best * 1.0001f + 1e-12f) ||
+                (lastF >= 0 && box_far2(p, nd) * 1.0001f + 1e-12f < near2)) {
+                node = skip;
+                continue;
+            }
+            if (node < firstLeaf) {
+                node = __float_as_int(nd.lo.w);
+                continue;
+            }
+            node = skip;
+            const int f = __float_as_int(nd.lo.w);
+            const int4 fv = S.faces[f];
+            if (fv.w != hm) continue;
+            int type;
+            const V3 cp = closest_point_triangle(p, ld(S.pos, fv.x), ld(S.pos, fv.y), ld(S.pos, fv.z), type);
+            if (hm == qm && type <= AT_C && (type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z)) == v) continue;
+            const float d = vnorm(p - cp);
+            if (!(d > lastD || (d == lastD && f > lastF))) continue;  // taken in an earlier round
+            if (bestF >= 0 && !(d < best || (d == best && f < bestF))) continue;
+            best = d;
+            bestF = f;
+            bestType = type;
+            bestCp = cp;
+        }
+        if (bestF < 0) {
+            exhausted = true;  // no face left
+            break;
+        }
+        if (pen_face_valid(S, qm, hm, ht, p, bestF, bestType, bestCp)) hit = bestF;
+        lastD = best;
+        lastF = bestF;
+    }
+    if (hit < 0 && !exhausted) {
+        // many near faces rejected (a vertex deep in the other body): one traversal over the faces
+        // after the last rejected one, checking each face closer than the best valid face so far
+        atomicAdd(S.counts + 5, 1);
+        float best = FLT_MAX;
+        const float near2 = lastD * lastD;
+        int node = S.fN > 0 ? 0 : -1;
+        for (int visits = 0; node >= 0; ++visits) {
+            if (visits > 2 * S.fN) {
+                atomicOr(S.error, 1);
+                break;
+            }
+            const BvhNode nd = S.fNodes[node];
+            const int skip = __float_as_int(nd.hi.w);
+            const int nm = S.fMesh[node];
+            if ((nm >= 0 && nm != hm) || (hit >= 0 && box_dist2(p, nd) > best * best * 1.0001f + 1e-12f) ||
+                box_far2(p, nd) * 1.0001f + 1e-12f < near2) {
+                node = skip;
+                continue;
+            }
+            if (node < firstLeaf) {
+                node = __float_as_int(nd.lo.w);
+                continue;
+            }
+            node = skip;
+            const int f = __float_as_int(nd.lo.w);
+            const int4 fv = S.faces[f];
+            if (fv.w != hm) continue;
+            int type;
+            const V3 cp = closest_point_triangle(p, ld(S.pos, fv.x), ld(S.pos, fv.y), ld(S.pos, fv.z), type);
+            if (hm == qm && type <= AT_C && (type == AT_A ? fv.x : (type == AT_B ? fv.y : fv.z)) == v) continue;
+            const float d = vnorm(p - cp);
+            if (!(d > lastD || (d == lastD && f > lastF))) continue;
+            if (hit >= 0 && !(d < best || (d == best && f < hit))) continue;
+            if (!pen_face_valid(S, qm, hm, ht, p, f, type, cp)) continue;
+            best = d;
+            hit = f;
+        }
+    }
+    if (hit < 0) {
         atomicAdd(S.counts + 1, 1);
         return;
     }
     S.stored[v] = 1;
-    S.rec[(int64_t)v * S.cap].face = bestF;
+    S.rec[(int64_t)v * S.cap].face = hit;
     atomicAdd(S.counts, 1);
 }
 
@@ -1685,6 +1820,7 @@ void ContactDetector::finalize() {
     // BVH storage
     allocBvh(fb_, nf_);
     allocBvh(eb_, ne_);
+    if (pen_ && nf_ > 0) fb_.mesh = dalloc<int>(2 * (size_t)nf_ - 1);
     if (pen_) setupPenetrationDevice();
     hip_check(hipStreamSynchronize(stream_), "detector upload sync");
     ready_ = true;
@@ -1811,6 +1947,21 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind, const float4* boxes) {
                   "level upload");
     }
     refitBvh(B, kind, stream_, boxes);
+    if (B.mesh && kind == 0) {
+        // mesh membership changes only with the tree's topology: set at every build
+        hipLaunchKernelGGL(node_mesh_leaf_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, faces_, B.nodes, n,
+                           B.mesh);
+        for (const auto& seg : B.segments) {
+            const int b = B.levelBegin[seg.first], e = B.levelBegin[seg.first + 1];
+            if (seg.second == seg.first + 1 && e - b > kBlockLevel)
+                hipLaunchKernelGGL(node_mesh_internal_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream_,
+                                   B.levelNodes, b, e, B.nodes, B.right, B.mesh);
+            else
+                hipLaunchKernelGGL(node_mesh_levels_kernel, dim3(1), dim3(kBlockLevel), 0, stream_, B.levelNodes,
+                                   B.dLevelBegin, seg.first, seg.second, B.nodes, B.right, B.mesh);
+        }
+        hip_check(hipGetLastError(), "node mesh launch");
+    }
 }
 
 void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream, const float4* boxes) {
@@ -1872,10 +2023,12 @@ void ContactDetector::setupPenetrationDevice() {
     tetBoxes_ = dalloc<float4>(2 * (size_t)std::max(nt_, 1));
     sweptBoxes_ = dalloc<float4>(2 * (size_t)std::max(nf_, 1));
     penFlags_ = dalloc<uint8_t>(std::max(nv_, 1));
-    penCounts_ = dalloc<int>(4);
+    penCounts_ = dalloc<int>(kPenCounts);
+    penInTet_ = dalloc<int>(std::max(nv_, 1));
+    penList_ = dalloc<int>(std::max(nSurf_, 1));
     hipLaunchKernelGGL(fill_u8_kernel, dim3(std::min(blocks_for(nv_), 4096)), dim3(256), 0, stream_, penFlags_,
                        (int64_t)nv_);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)4, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)kPenCounts, 0);
     allocBvh(tb_, nt_);
     allocBvh(sb_, nf_);
     // tet boxes for the first tree build (at the end of finalize)
@@ -1897,7 +2050,7 @@ void ContactDetector::penetrationDcd(bool rebuild) {
     if (!ready_ || !pen_) fail(GVBD_ERR_STATE, "detector not finalized for penetration contact");
     hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nv_), 4096)), dim3(256), 0, stream_, vfStored_,
                        (int64_t)nv_, 0);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)4, 0);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, stream_, penCounts_, (int64_t)kPenCounts, 0);
     hipLaunchKernelGGL(tet_box_kernel, dim3(blocks_for(nt_)), dim3(256), 0, stream_, pos_, penTets_, nt_, tetBoxes_);
     if (rebuild) {
         buildBvh(tb_, 3, tetBoxes_);
@@ -1920,6 +2073,7 @@ void ContactDetector::penetrationDcd(bool rebuild) {
     S.tNodes = tb_.nodes;
     S.tN = tb_.n;
     S.fNodes = fb_.nodes;
+    S.fMesh = fb_.mesh;
     S.fN = fb_.n;
     S.P = penHost_.params;
     S.stored = vfStored_;
@@ -1927,9 +2081,13 @@ void ContactDetector::penetrationDcd(bool rebuild) {
     S.cap = vfCap_;
     S.flags = penFlags_;
     S.counts = penCounts_;
+    S.inTet = penInTet_;
+    S.list = penList_;
     S.error = error_;
-    if (nSurf_ > 0)
-        hipLaunchKernelGGL(pen_dcd_kernel, dim3((nSurf_ + 63) / 64), dim3(64), 0, stream_, S);
+    if (nSurf_ > 0) {
+        hipLaunchKernelGGL(pen_contain_kernel, dim3((nSurf_ + 63) / 64), dim3(64), 0, stream_, S);
+        hipLaunchKernelGGL(pen_closest_kernel, dim3((nSurf_ + 63) / 64), dim3(64), 0, stream_, S);
+    }
     hip_check(hipGetLastError(), "penetration DCD launch");
 }
 
@@ -1964,13 +2122,14 @@ void ContactDetector::penetrationCcd(const float4* prev, bool rebuild) {
 PenetrationCounts ContactDetector::penetrationCounts() {
     PenetrationCounts c{};
     if (!pen_) return c;
-    int h[4];
+    int h[kPenCounts];
     hip_check(hipMemcpyAsync(h, penCounts_, sizeof(h), hipMemcpyDeviceToHost, stream_), "counts");
     hip_check(hipStreamSynchronize(stream_), "counts sync");
     c.dcdHits = h[0];
     c.penetratedNoFace = h[1];
     c.ccdHits = h[2];
     c.walkOverflows = h[3];
+    c.fullSearches = h[5];
     return c;
 }
 

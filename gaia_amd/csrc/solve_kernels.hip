@@ -1217,26 +1217,22 @@ hipError_t launch_update_velocity(const SceneView* scenes, const WorkItem* items
 // Plain streaming copy (float4 per lane, grid-stride): the reference kernel
 // whose HBM counters calibrate the profiler and whose rate is the measured
 // copy bandwidth of the device.
-// Grid-stride copy with four 16-byte loads in flight per lane before their stores.
+// Grid-stride copy, one 16-byte non-temporal load and store per lane and step (the fastest of
+// the streaming variants measured on MI355X: 1-4 loads in flight per lane, 1k-64k blocks,
+// temporal or non-temporal accesses).
 __global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
                                                           int64_t n) {
-    constexpr int U = 4;
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v* s = reinterpret_cast<const f4v*>(src);
+    f4v* d = reinterpret_cast<f4v*>(dst);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + (U - 1) * stride < n; i += U * stride) {
-        float4 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
-#pragma unroll
-        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
-    }
-    for (; i < n; i += stride) dst[i] = src[i];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
 }
 
 hipError_t launch_stream_copy(const float4* src, float4* dst, int64_t n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    // 256 CUs x 8 blocks of 256 lanes; each lane moves 4 x 16 B per round
-    const int64_t blocks = std::min<int64_t>((n + 1023) / 1024, 256 * 8);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, n);
     return hipGetLastError();
 }

@@ -2295,6 +2295,7 @@ int gvbd_scene_penetration_results(gvbd_context* ctx, int32_t scene, int32_t* hi
             counts[1] = c.penetratedNoFace;
             counts[2] = c.ccdHits;
             counts[3] = c.walkOverflows;
+            counts[4] = c.fullSearches;
         }
     });
 }

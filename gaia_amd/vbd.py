@@ -538,13 +538,14 @@ class VBDPhysics:
         """The last detection of a scene with the penetration contact model (tet_contact_model 1), one
         row per scene vertex in mesh order: (mesh, vertex, hit mesh, hit face, contained, from CCD),
         hit face -1 when none (the rows of VBDPhysics::collisionResultsAll, VBD/VBDPhysics.h:154), and
-        the counts (DCD hits, contained vertices without a face, CCD hits, walk stack overflows)."""
+        the counts (DCD hits, contained vertices without a face, CCD hits, walk stack overflows, closest-face
+        searches past the nearest 8 faces)."""
         sizes = [self.meshInfo(m)[0] for m in range(self.numMeshes())]
         nv = sum(sizes)
         hm = np.zeros(nv, np.int32)
         hf = np.zeros(nv, np.int32)
         fl = np.zeros(nv, np.uint8)
-        counts = np.zeros(4, np.int32)
+        counts = np.zeros(5, np.int32)
         self.ctx.check(lib().gvbd_scene_penetration_results(
             self.ctx.handle, self.scene, iptr(hm), iptr(hf), fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
             iptr(counts)))

@@ -353,8 +353,9 @@ GVBD_API int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_con
 /* penetration contact (tet_contact_model 1), after the last detection: per scene vertex the mesh
  * and boundary face (in that mesh's boundary-face order, -1 none) it is pushed out of and flags
  * (1: a tet contained it at the substep's start, 2: the hit came from CCD); each array holds the
- * scene's vertex count (NULL skips it). counts (4, or NULL): DCD hits, contained vertices with no
- * face, CCD hits, ray walks stopped by a full candidate stack. Replaces reading
+ * scene's vertex count (NULL skips it). counts (5, or NULL): DCD hits, contained vertices with no
+ * face, CCD hits, ray walks stopped by a full candidate stack, closest-face searches that went
+ * past the nearest 8 faces (all rejected) to a full traversal. Replaces reading
  * VBDPhysics::collisionResultsAll (VBD/VBDPhysics.h:154). */
 GVBD_API int gvbd_scene_penetration_results(gvbd_context* ctx, int32_t scene, int32_t* hit_mesh, int32_t* hit_face,
                                             uint8_t* flags, int32_t* counts);

@@ -11,6 +11,8 @@
 - Tet bodies in contact (32 bodies of 8,640 tets, 70,304 vertices): per-step parity against the
   FP32 oracle from identical state on frames of the fall including impacts, stated against the
   FP32-vs-FP64 spread, and no vertex inside another body over a 60-frame fall.
+- The same bodies with the tet solver's penetration contact, piled up on the floor: detections
+  equal to the FP32 oracle's per vertex on 6 frames from identical state.
 - Served VBD (2,000 scenes of 1k-200k vertices) under a memory budget of a quarter of the scenes'
   bytes: scenes are copied out and back between frames, and a seeded sample of served scenes
   equals the same scenes stepped alone, bitwise.
@@ -219,6 +221,65 @@ def test_c2_full_size_parity_and_no_interpenetration(gpu_ctx, oracle):
     assert np.median(e) <= max(2.0 * np.median(e64), 1e-6)
     assert e.max() <= max(4.0 * e64.max(), TOL)
     sim.destroy()
+
+
+def test_c2_penetration_pileup_detections_equal_oracle(gpu_ctx, oracle):
+    """The same 32 bodies with the tet solver's penetration contact (collision stiffness 1e5),
+    stepped on the device until they pile up on the floor (frame 33), then 6 frames from identical
+    state on the device and the FP32 oracle: every vertex's detection (hit mesh and face,
+    contained, from CCD) equal on every frame, thousands of hits among them, and vertices deep
+    enough in another body that the closest-face search goes past its nearest-face rounds. The
+    positions are stated against the FP32-vs-FP64 spread of the same frames (ill-conditioned)."""
+    bodies = scenes.c2_bodies(seed=0)
+    phys = physics_dict(**dict(scenes.c2_physics(), tet_contact_model=1))
+    sim, colors = _c2_scene(gpu_ctx, bodies, phys)
+    nb = len(bodies)
+    sim.runSteps(33)
+    threads = host_threads()
+
+    def oracle_scene(precision):
+        sc = oracle.TetScene(phys, precision=precision)
+        for b in bodies:
+            sc.add_mesh(np.asarray(b["pos"], np.float32).astype(np.float64), b["tets"], b["material"],
+                        colors[(b["kind"], len(b["tets"]))])
+        sc.set_collision(dict(stiffness=float(np.float32(phys["collision_stiffness"]))))
+        sc.finalize()
+        return sc
+
+    o32, o64 = oracle_scene(32), oracle_scene(64)
+    sizes = [len(b["pos"]) for b in bodies]
+    off = np.cumsum([0] + sizes)
+    hits = full = 0
+    rows_e = []
+    for k in range(6):
+        state = [sim.getState(m) for m in range(nb)]
+        for m, (x, v) in enumerate(state):
+            o32.set_state(m, x, v)
+            o64.set_state(m, x, v)
+        sim.runStep()
+        o32.step(1, threads=threads)
+        o64.step(1, threads=threads)
+        got, counts = sim.penetrationResults()
+        want = got.copy()
+        want[:, 2:] = (-1, -1, 0, 0)
+        for mesh, v, hm, hf, pen, ccd in o32.collisions():
+            want[off[mesh] + v, 2:] = (hm, hf, pen, ccd)
+        bad = np.nonzero((got != want).any(axis=1))[0]
+        assert len(bad) == 0, (k, got[bad[:4]].tolist(), want[bad[:4]].tolist())
+        hits += int((want[:, 3] >= 0).sum())
+        full += int(counts[4])
+        assert counts[3] == 0
+        e = max(rel_err(sim.getState(m)[0], o32.get_state(m)[0]) for m in range(nb))
+        e64 = max(rel_err(o32.get_state(m)[0], o64.get_state(m)[0]) for m in range(nb))
+        rows_e.append((e, e64))
+        # the device continues from the oracle's FP32 state
+        for m in range(nb):
+            sim.setState(m, *o32.get_state(m))
+    e, e64 = np.array(rows_e).T
+    print(f"\n[C2 penetration pile-up] frames 34-39: {hits} hits, {full} searches past the nearest rounds; "
+          f"device vs FP32 max {e.max():.2e}, FP32 vs FP64 max {e64.max():.2e}", flush=True)
+    assert hits > 1000 and full > 0
+    assert np.all(e <= np.maximum(TOL, 4.0 * e64))
 
 
 def test_c5_served_equals_alone_under_binding_budget(gpu_ctx):
