@@ -348,7 +348,7 @@ def contact_bench(args):
     sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(best)}
     sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
     graph = False
-    if penetration:
+    if penetration and not args.no_graph:
         # the penetration frame needs no host decision: one HIP graph per frame is a candidate too
         # (detection is then not timed inside the graph; its time comes from the direct frames)
         sim.recordVBDSolveGraph(True)
