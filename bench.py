@@ -330,11 +330,11 @@ def contact_bench(args):
         cands = [args.schedule]
     else:
         cands = ["group:4x64", "group:4x128", "group:8x64", "group:8x128", "group:8x256", "group:16x128",
-                 "group:16x256"]
+                 "group:16x256", "group:8x128:x", "group:16x256:x"]
     tune = {}
     for key in cands:
         c = parse_schedule_key(key)
-        sim.setSchedule(c["lanes_per_vertex"], c["block"], mode="group")
+        sim.setSchedule(c["lanes_per_vertex"], c["block"], mode="group", xcd_blocks=c.get("xcd_blocks", 0))
         restore()
         sim.runStep()
         restore()
@@ -346,7 +346,7 @@ def contact_bench(args):
         tune[key] = round((time.perf_counter() - t) / 3 * 1e3, 3)
     best = min(tune, key=tune.get)
     sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(best)}
-    sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group")
+    sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group", xcd_blocks=sched.get("xcd_blocks", 0))
     graph = False
     if penetration and not args.no_graph:
         # the penetration frame needs no host decision: one HIP graph per frame is a candidate too
@@ -532,7 +532,8 @@ def serve_bench(args, schedule=None):
     if sched["mode"] != "group":  # the served plan changes every tick: group launches
         sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(SERVE_SCHEDULE)}
     srv.set_schedule(sched["lanes_per_vertex"], sched["block"], sched["mode"], sched["vertices_per_entry"],
-                     sched["blocks_per_cu"], stage_records=sched.get("stage_records", 0))
+                     sched["blocks_per_cu"], stage_records=sched.get("stage_records", 0),
+                     xcd_blocks=sched.get("xcd_blocks", 0))
     for sim in sims:
         srv.add_scene(sim)
     # warm-up (untimed): one frame of every scene, which makes each resident on its first request
@@ -721,7 +722,7 @@ def main():
     if not args.no_plain:
         for s in sims:
             s.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group",
-                          stage_records=sched.get("stage_records", 0))
+                          stage_records=sched.get("stage_records", 0), xcd_blocks=sched.get("xcd_blocks", 0))
             s.recordVBDSolveGraph(use_graph)
         for _ in range(args.warmup):
             for s in sims:

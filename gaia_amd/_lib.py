@@ -150,7 +150,7 @@ class Schedule(ctypes.Structure):
         ("blocks_per_cu", ctypes.c_int32),
         ("spin_limit", ctypes.c_int32),
         ("stage_records", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("xcd_blocks", ctypes.c_int32),
     ]
 
 

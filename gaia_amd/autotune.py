@@ -4,7 +4,8 @@
 Knobs: launch mode (one launch per parallel group, or one persistent work-queue launch per
 iteration), lanes per vertex (vertices per wavefront = 64 / lanes), block size, for group
 launches the LDS staging depth (incidence records staged per chunk, 0 = read from HBM by the
-lanes), and for the work queue the entry size (vertices per entry) and the persistent grid per CU.
+lanes) and the XCD block mapping (each XCD's blocks take one contiguous range of the work
+items), and for the work queue the entry size (vertices per entry) and the persistent grid per CU.
 """
 import itertools
 
@@ -15,6 +16,7 @@ def default_candidates(modes=("group", "queue")):
     out = []
     if "group" in modes:
         out += [dict(lanes_per_vertex=l, block=b, mode="group") for l, b in DEFAULT_GROUP]
+        out += [dict(lanes_per_vertex=l, block=b, mode="group", xcd_blocks=1) for l, b in [(4, 256), (8, 256), (8, 128)]]
         for (l, b), st in itertools.product([(4, 128), (8, 128), (8, 256), (16, 256)], [32, 64]):
             out.append(dict(lanes_per_vertex=l, block=b, mode="group", stage_records=st))
     if "queue" in modes:
@@ -27,6 +29,8 @@ def schedule_key(s):
     key = f"{s['mode']}:{s['lanes_per_vertex']}x{s['block']}"
     if s["mode"] == "group" and s.get("stage_records", 0):
         key += f":s{s['stage_records']}"
+    if s["mode"] == "group" and s.get("xcd_blocks", 0):
+        key += ":x"
     if s["mode"] == "queue":
         key += f":e{s.get('vertices_per_entry', 0)}"
         if s.get("blocks_per_cu", 0):
@@ -36,7 +40,8 @@ def schedule_key(s):
 
 def parse_schedule_key(key):
     """Inverse of schedule_key: "group:8x128", "group:8x128:s64" (LDS staging, 64 records per
-    chunk), "queue:8x128:e64", "queue:8x128:e64:g2"; a bare "8x128" means group launches."""
+    chunk), "group:8x256:x" (XCD block mapping), "queue:8x128:e64", "queue:8x128:e64:g2"; a bare
+    "8x128" means group launches."""
     parts = key.split(":")
     if len(parts) == 1:
         parts = ["group"] + parts
@@ -52,6 +57,8 @@ def parse_schedule_key(key):
             s["blocks_per_cu"] = int(p[1:])
         elif p[0] == "s":
             s["stage_records"] = int(p[1:])
+        elif p == "x":
+            s["xcd_blocks"] = 1
         else:
             raise ValueError(f"bad schedule {key!r}")
     return s
@@ -60,7 +67,7 @@ def parse_schedule_key(key):
 def apply_schedule(batch, s, use_graph=True):
     batch.set_schedule(s["lanes_per_vertex"], s["block"], mode=s["mode"],
                        vertices_per_entry=s.get("vertices_per_entry", 0), blocks_per_cu=s.get("blocks_per_cu", 0),
-                       stage_records=s.get("stage_records", 0))
+                       stage_records=s.get("stage_records", 0), xcd_blocks=s.get("xcd_blocks", 0))
     batch.record_graph(use_graph)
 
 

@@ -261,8 +261,9 @@ __device__ bool edge_feasible(const CDView& S, V3 p, int e) {
 // for the stored ones at the end, so the list costs 2 CAP registers.
 template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
-    // vertices in Morton order: a wavefront's queries are spatial neighbours
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // vertices in Morton order: a wavefront's queries are spatial neighbours, and so are the
+    // blocks of one XCD
+    const int t = xcd_block_index() * blockDim.x + threadIdx.x;
     if (t >= S.nv) return;
     const int v = S.vOrder ? (int)(S.vOrder[t] & 0xFFFFFFFFu) : t;
     if (S.vQuery && !S.vQuery[v]) {  // not a surface vertex: no VF contacts
@@ -431,8 +432,8 @@ __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
 template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     // neighbouring threads take edges that are neighbours in the tree's Morton order, so a
-    // wavefront's traversals share their paths (and cache lines)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // wavefront's traversals share their paths (and cache lines), and an XCD's blocks a range
+    const int t = xcd_block_index() * blockDim.x + threadIdx.x;
     if (t >= S.ne) return;
     const int e1 = S.eOrder ? (int)(S.eOrder[t] & 0xFFFFFFFFu) : t;
     const int4 a = S.edges[e1];

@@ -227,6 +227,16 @@ constexpr uint8_t VFLAG_ACCEL = 16;
 constexpr uint8_t VFLAG_NOGRAV_POS = 32;
 constexpr uint8_t VFLAG_NOGRAV_VEL = 64;
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its
+// own L2. This index maps the blocks of one XCD to one contiguous range of [0, gridDim.x), so the
+// work an XCD has in flight at a time is a run of neighbouring items. A bijection.
+__device__ __forceinline__ int xcd_block_index() {
+    constexpr int kXcds = 8;
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int xcd = b % kXcds, j = b / kXcds, q = nb / kXcds, rem = nb % kXcds;
+    return xcd < rem ? xcd * (q + 1) + j : rem * (q + 1) + (xcd - rem) * q + j;
+}
+
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`
 // (tet slots or cloth slots by `kind`; vertex ranges for the per-vertex kernels).
 struct WorkItem {

@@ -14,9 +14,10 @@ bool valid_stage(int lpv, int block, int stage);
 // One parallel group: item i covers `count` consecutive colour-major slots of one scene,
 // tet or cloth by item kind. Tet friction applies when iteration >= frictionStartIter
 // (VBDPhysics.cpp:1610). `contact` selects the variant with contact forces and bound truncation
-// (scenes with collision handling; stage must then be 0).
+// (scenes with collision handling; stage must then be 0). `xcd` != 0 maps each XCD's blocks to one
+// contiguous range of the items (results unchanged).
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, int stage, bool contact, hipStream_t stream);
+                              int block, int stage, bool contact, int xcd, hipStream_t stream);
 // Device arguments of the persistent work-queue solve of a plan.
 struct QueueArgs {
     const SceneView* scenes;

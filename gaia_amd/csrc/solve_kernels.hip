@@ -788,9 +788,10 @@ __device__ __forceinline__ void cloth_block(const SceneView& S, const WorkItem& 
 // truncation); the others carry none of that code, so their register use is unchanged by it.
 template <int LPV, int BLOCK, int STAGE, bool CONTACT>
 __global__ __launch_bounds__(BLOCK) void group_solve_kernel(const SceneView* __restrict__ scenes,
-                                                           const WorkItem* __restrict__ items, int iteration) {
+                                                           const WorkItem* __restrict__ items, int iteration,
+                                                           int xcdBlocks) {
     __shared__ float4 lds[STAGE > 0 ? 4 * STAGE : 1];
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[xcdBlocks ? xcd_block_index() : (int)blockIdx.x];
     const SceneView& S = scenes[it.scene];
     if (it.kind == WORK_TET)
         tet_block<LPV, STAGE, CONTACT>(S, it, iteration, lds);
@@ -1088,9 +1089,9 @@ __global__ __launch_bounds__(256) void update_velocity_kernel(const SceneView* _
 // ---------------------------------------------------------------------------
 template <int LPV, int BLOCK, int STAGE, bool CONTACT>
 static hipError_t launch_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration,
-                               hipStream_t stream) {
+                               int xcd, hipStream_t stream) {
     hipLaunchKernelGGL((group_solve_kernel<LPV, BLOCK, STAGE, CONTACT>), dim3(nitems), dim3(BLOCK), 0, stream, scenes,
-                       items, iteration);
+                       items, iteration, xcd);
     return hipGetLastError();
 }
 
@@ -1098,12 +1099,12 @@ static hipError_t launch_solve(const SceneView* scenes, const WorkItem* items, i
 #define GVBD_STAGED(X) X(4, 128) X(4, 256) X(8, 128) X(8, 256) X(16, 128) X(16, 256)
 
 hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, int nitems, int iteration, int lpv,
-                              int block, int stage, bool contact, hipStream_t stream) {
+                              int block, int stage, bool contact, int xcd, hipStream_t stream) {
     if (nitems <= 0) return hipSuccess;
 #define GVBD_CASE(L, B)                                                                                        \
     if (stage == 0 && lpv == L && block == B)                                                                  \
-        return contact ? launch_solve<L, B, 0, true>(scenes, items, nitems, iteration, stream)                 \
-                       : launch_solve<L, B, 0, false>(scenes, items, nitems, iteration, stream);
+        return contact ? launch_solve<L, B, 0, true>(scenes, items, nitems, iteration, xcd, stream)            \
+                       : launch_solve<L, B, 0, false>(scenes, items, nitems, iteration, xcd, stream);
     GVBD_CASE(1, 64) GVBD_CASE(1, 128) GVBD_CASE(1, 256)
     GVBD_CASE(2, 64) GVBD_CASE(2, 128) GVBD_CASE(2, 256)
     GVBD_CASE(4, 64) GVBD_CASE(4, 128) GVBD_CASE(4, 256)
@@ -1112,8 +1113,10 @@ hipError_t launch_group_solve(const SceneView* scenes, const WorkItem* items, in
 #undef GVBD_CASE
     if (contact) return hipErrorInvalidValue;  // staged variants carry no contact code
 #define GVBD_CASE(L, B)                                                                                  \
-    if (lpv == L && block == B && stage == 32) return launch_solve<L, B, 32, false>(scenes, items, nitems, iteration, stream); \
-    if (lpv == L && block == B && stage == 64) return launch_solve<L, B, 64, false>(scenes, items, nitems, iteration, stream);
+    if (lpv == L && block == B && stage == 32)                                                           \
+        return launch_solve<L, B, 32, false>(scenes, items, nitems, iteration, xcd, stream);             \
+    if (lpv == L && block == B && stage == 64)                                                           \
+        return launch_solve<L, B, 64, false>(scenes, items, nitems, iteration, xcd, stream);
     GVBD_STAGED(GVBD_CASE)
 #undef GVBD_CASE
     return hipErrorInvalidValue;

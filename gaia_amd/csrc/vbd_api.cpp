@@ -136,6 +136,7 @@ struct PlanObj {
     std::vector<int32_t> scenes;
     int lpv = 4, block = 256;
     int stage = 0;  // incidence records per LDS staging chunk (0: read straight from HBM)
+    int xcd = 0;    // group launches: each XCD's blocks take one contiguous range of the work items
     int mode = GVBD_MODE_GROUP_LAUNCHES;
     int vertsPerEntry = 0;  // work-queue entry size (0: 4 passes of block / lpv vertices)
     int blocksPerCu = 0;    // persistent grid per CU (0: resident maximum)
@@ -1057,10 +1058,11 @@ void plan_initial_step(gvbd_context* ctx, PlanObj& p, bool clothDeformed = false
 void plan_group(gvbd_context* ctx, PlanObj& p, int g, int iter) {
     const int b = p.groupItemBegin[g], c = p.groupContactBegin[g], e = p.groupItemBegin[g + 1];
     if (c > b)
-        hip_check(launch_group_solve(p.dViews, p.dItems + b, c - b, iter, p.lpv, p.block, p.stage, false, ctx->stream),
+        hip_check(launch_group_solve(p.dViews, p.dItems + b, c - b, iter, p.lpv, p.block, p.stage, false, p.xcd,
+                                     ctx->stream),
                   "group solve launch");
     if (e > c)
-        hip_check(launch_group_solve(p.dViews, p.dItems + c, e - c, iter, p.lpv, p.block, 0, true, ctx->stream),
+        hip_check(launch_group_solve(p.dViews, p.dItems + c, e - c, iter, p.lpv, p.block, 0, true, p.xcd, ctx->stream),
                   "group solve launch");
     const int cb = p.groupCommitBegin[g], ce = p.groupCommitBegin[g + 1];
     if (ce > cb) hip_check(launch_cloth_commit(p.dViews, p.dCommit + cb, ce - cb, ctx->stream), "commit launch");
@@ -2137,7 +2139,9 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
     require(!(p.contact && sc.mode == GVBD_MODE_WORK_QUEUE),
             "scenes with collision handling run with group launches (detection runs between the launches)");
     require(!(p.contact && sc.stage_records != 0), "LDS staging is not available for scenes with collision handling");
+    require(sc.xcd_blocks == 0 || sc.xcd_blocks == 1, "xcd_blocks must be 0 or 1");
     p.lpv = sc.lanes_per_vertex;
+    p.xcd = sc.xcd_blocks;
     p.stage = sc.stage_records;
     p.block = sc.block;
     p.mode = sc.mode;
@@ -2153,6 +2157,7 @@ static void get_schedule(const PlanObj& p, gvbd_schedule* sc) {
     sc->lanes_per_vertex = p.lpv;
     sc->block = p.block;
     sc->stage_records = p.stage;
+    sc->xcd_blocks = p.xcd;
     sc->mode = p.mode;
     sc->vertices_per_entry = p.vertsPerEntry;
     sc->blocks_per_cu = p.blocksPerCu;

@@ -581,15 +581,16 @@ class VBDPhysics:
         self.ctx.check(lib().gvbd_scene_record_graph(self.ctx.handle, self.scene, 1 if enable else 0))
 
     def setSchedule(self, lanes_per_vertex, block, mode=None, vertices_per_entry=0, blocks_per_cu=0, spin_limit=0,
-                    stage_records=0):
+                    stage_records=0, xcd_blocks=0):
         """Vertex-solve schedule; `mode` "group" (one launch per parallel group) or "queue" (one
         persistent work-queue launch per iteration); None keeps the current mode. stage_records
-        (group launches): incidence records staged through LDS per chunk (0: none)."""
-        if mode is None and not stage_records:
+        (group launches): incidence records staged through LDS per chunk (0: none); xcd_blocks
+        (group launches): each XCD's blocks take one contiguous range of the work items."""
+        if mode is None and not stage_records and not xcd_blocks:
             self.ctx.check(lib().gvbd_scene_set_schedule(self.ctx.handle, self.scene, lanes_per_vertex, block))
             return
         sc = make_schedule(lanes_per_vertex, block, mode or "group", vertices_per_entry, blocks_per_cu, spin_limit,
-                           stage_records)
+                           stage_records, xcd_blocks)
         self.ctx.check(lib().gvbd_scene_set_schedule_ex(self.ctx.handle, self.scene, ctypes.byref(sc)))
 
     def getSchedule(self):
@@ -724,18 +725,18 @@ _MODES = {"group": MODE_GROUP_LAUNCHES, "queue": MODE_WORK_QUEUE}
 
 
 def make_schedule(lanes_per_vertex, block, mode="group", vertices_per_entry=0, blocks_per_cu=0, spin_limit=0,
-                  stage_records=0):
+                  stage_records=0, xcd_blocks=0):
     if mode not in _MODES:
         raise ValueError(f"mode must be one of {sorted(_MODES)}")
     return Schedule(lanes_per_vertex, block, _MODES[mode], vertices_per_entry, blocks_per_cu, spin_limit,
-                    stage_records)
+                    stage_records, int(bool(xcd_blocks)))
 
 
 def schedule_dict(sc):
     mode = {v: k for k, v in _MODES.items()}[sc.mode]
     return dict(lanes_per_vertex=sc.lanes_per_vertex, block=sc.block, mode=mode,
                 vertices_per_entry=sc.vertices_per_entry, blocks_per_cu=sc.blocks_per_cu, spin_limit=sc.spin_limit,
-                stage_records=sc.stage_records)
+                stage_records=sc.stage_records, xcd_blocks=sc.xcd_blocks)
 
 
 class Batch:
@@ -750,12 +751,12 @@ class Batch:
         self.batch = b.value
 
     def set_schedule(self, lanes_per_vertex, block, mode=None, vertices_per_entry=0, blocks_per_cu=0, spin_limit=0,
-                     stage_records=0):
-        if mode is None and not stage_records:
+                     stage_records=0, xcd_blocks=0):
+        if mode is None and not stage_records and not xcd_blocks:
             self.ctx.check(lib().gvbd_batch_set_schedule(self.ctx.handle, self.batch, lanes_per_vertex, block))
             return
         sc = make_schedule(lanes_per_vertex, block, mode or "group", vertices_per_entry, blocks_per_cu, spin_limit,
-                           stage_records)
+                           stage_records, xcd_blocks)
         self.ctx.check(lib().gvbd_batch_set_schedule_ex(self.ctx.handle, self.batch, ctypes.byref(sc)))
 
     def get_schedule(self):
@@ -823,9 +824,9 @@ class Server:
         self.server = sid.value
 
     def set_schedule(self, lanes_per_vertex, block, mode="group", vertices_per_entry=0, blocks_per_cu=0,
-                     spin_limit=0, stage_records=0):
+                     spin_limit=0, stage_records=0, xcd_blocks=0):
         sc = make_schedule(lanes_per_vertex, block, mode, vertices_per_entry, blocks_per_cu, spin_limit,
-                           stage_records)
+                           stage_records, xcd_blocks)
         self.ctx.check(lib().gvbd_server_set_schedule(self.ctx.handle, self.server, ctypes.byref(sc)))
 
     def add_scene(self, scene):

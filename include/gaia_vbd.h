@@ -219,7 +219,10 @@ typedef struct gvbd_schedule {
     int32_t stage_records;      /* group launches: incidence records staged through LDS per chunk by the whole
                                    block (32 or 64, with lanes_per_vertex 4/8/16 and block 128/256), or 0 to
                                    read them straight from HBM; results are identical either way */
-    int32_t reserved;
+    int32_t xcd_blocks;         /* group launches: 1 maps the blocks one XCD runs (the hardware deals blocks to the
+                                   8 XCDs round-robin) to one contiguous range of the work items, so the vertices
+                                   an XCD solves at a time are neighbours whose gathers share its L2; 0 takes
+                                   work item = block index. Results are identical either way */
 } gvbd_schedule;
 
 /* sizes of the ABI structs as compiled into the library (binding self-check) */

@@ -28,6 +28,30 @@ def states(sims):
     return [s.getState(0) for s in sims]
 
 
+@pytest.mark.parametrize("lpv,block", [(8, 128), (4, 256), (16, 64)])
+def test_xcd_block_mapping_equals_block_order_bitwise(gpu_ctx, lpv, block):
+    """Group launches whose blocks take the work items in XCD-contiguous ranges give bit-identical
+    positions and velocities to block order (each block's work is the same, only where it runs
+    changes), captured or not."""
+    specs = scenes.c4_scenes(seed=6, num_scenes=12, vmin=300, vmax=4000)
+    ref = [build_spec_scene(gpu_ctx, s) for s in specs]
+    xcd = [build_spec_scene(gpu_ctx, s) for s in specs]
+    br = Batch(gpu_ctx, ref)
+    br.set_schedule(lpv, block, mode="group")
+    bx = Batch(gpu_ctx, xcd)
+    bx.set_schedule(lpv, block, mode="group", xcd_blocks=1)
+    assert bx.get_schedule()["xcd_blocks"] == 1
+    bx.record_graph(True)
+    br.run_steps(4)
+    bx.run_steps(4)
+    for (xa, va), (xb, vb) in zip(states(ref), states(xcd)):
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    br.destroy()
+    bx.destroy()
+    for s in ref + xcd:
+        s.destroy()
+
+
 QUEUE_SCHEDULES = [
     dict(lanes_per_vertex=8, block=128, vertices_per_entry=64),
     dict(lanes_per_vertex=4, block=256, vertices_per_entry=64),
