@@ -130,9 +130,19 @@ class ContactDetector {
     void finalize();
     // Face back-links of the stored VF contacts (ClothContactDetector::faceContactInfos): per
     // face the lowest `cap` vertex ids whose stored contacts name that face.
-    void buildFaceLinks(int cap);
+    void buildFaceLinks(int cap, const int* gate = nullptr);
     // positionsAtPrevCollisionDetection = x; clears the re-detection flag
-    void snapshotPositions();
+    void snapshotPositions(const int* gate = nullptr);
+    // One detection pass of the cloth contact model whose run is decided on the device: a gate
+    // set when `first` (a substep's first iteration) or when the re-detection flag is set (a
+    // vertex was truncated in the previous iteration); with the gate clear every kernel of the
+    // pass returns at once and the detection state stays as it was. The same results as calling
+    // updateBvh, query, bounds, buildFaceLinks and snapshotPositions when the host sees the flag
+    // set, without a host read, so the pass can be captured in a graph. `rebuild` (first only)
+    // rebuilds the trees instead of refitting them (a host read: not capturable).
+    void detectGated(bool first, bool rebuild, float thickness, float relaxation);
+    // passes run by detectGated since finalize (a host read)
+    int64_t gatedPasses();
     // Penetration contact of tet meshes instead of the VF / EE proximity queries (call before
     // finalize). Per pass: penetrationDcd at the start of a substep, then penetrationCcd after the
     // initial step with the substep's start positions, then buildFaceLinks. Hits are written as
@@ -154,8 +164,8 @@ class ContactDetector {
     // x = rest + step * drift[mesh] + sigma * N(0, 1) (counter-based, seeded per (seed, step, vertex))
     void perturb(uint64_t seed, int step, float sigma, const float* drift);
     void updateBvh(bool rebuild);
-    void query(bool record);
-    void bounds(float thickness, float relaxation);
+    void query(bool record, const int* gate = nullptr);
+    void bounds(float thickness, float relaxation, const int* gate = nullptr);
 
     DetectorResultCounts counts();
     // record-mode outputs (valid after query(true)); sizes from counts()
@@ -180,7 +190,8 @@ class ContactDetector {
   private:
     // kind 0 faces, 1 edges, 3 precomputed primitive boxes (`boxes`: lo, hi per primitive)
     void buildBvh(DeviceBvh& b, int kind, const float4* boxes = nullptr);
-    void refitBvh(DeviceBvh& b, int kind, hipStream_t stream, const float4* boxes = nullptr);
+    void refitBvh(DeviceBvh& b, int kind, hipStream_t stream, const float4* boxes = nullptr,
+                  const int* gate = nullptr);
     void allocBvh(DeviceBvh& b, int n);
     void setupPenetrationDevice();
     // the face-side and edge-side work of a pass are independent: side_ runs the edge side
@@ -254,6 +265,8 @@ class ContactDetector {
     size_t fvScanTempBytes_ = 0;
     float4* xAtCD_ = nullptr;
     int* flag_ = nullptr;
+    int* gate_ = nullptr;        // detectGated: whether the current pass runs
+    int* gatePasses_ = nullptr;  // detectGated: passes run
     // penetration contact
     bool pen_ = false;
     PenetrationSetup penHost_;

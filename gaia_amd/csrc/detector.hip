@@ -215,6 +215,7 @@ struct CDView {
     const uint64_t* eOrder;  // edge tree's sorted Morton keys: query edges in leaf order (or NULL)
     const uint64_t* vOrder;  // vertices' sorted Morton keys: query vertices in that order (or NULL)
     const uint8_t* vQuery;   // per vertex: takes VF queries (NULL: every vertex)
+    const int* gate;         // device flag: 0 skips the pass's kernels (NULL: always run)
     int* error;
     const float4* boxes;     // kind 3 primitives: lo, hi per primitive
 };
@@ -263,6 +264,7 @@ template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void vf_query_kernel(CDView S) {
     // vertices in Morton order: a wavefront's queries are spatial neighbours, and so are the
     // blocks of one XCD
+    if (S.gate && !*S.gate) return;
     const int t = xcd_block_index() * blockDim.x + threadIdx.x;
     if (t >= S.nv) return;
     const int v = S.vOrder ? (int)(S.vOrder[t] & 0xFFFFFFFFu) : t;
@@ -433,6 +435,7 @@ template <bool RECORD, int CAP>
 __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
     // neighbouring threads take edges that are neighbours in the tree's Morton order, so a
     // wavefront's traversals share their paths (and cache lines), and an XCD's blocks a range
+    if (S.gate && !*S.gate) return;
     const int t = xcd_block_index() * blockDim.x + threadIdx.x;
     if (t >= S.ne) return;
     const int e1 = S.eOrder ? (int)(S.eOrder[t] & 0xFFFFFFFFu) : t;
@@ -546,7 +549,9 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
 __global__ __launch_bounds__(256) void bounds_kernel(const float* vfMin, const int* faceMin, const float* eeMin,
                                                      const int* vfBegin, const int* vfFace, const int* veBegin,
                                                      const int* veEdge, const uint8_t* vQuery, int nv, float r,
-                                                     float thickness, float relaxation, float* out) {
+                                                     float thickness, float relaxation, float* out,
+                                                     const int* gate) {
+    if (gate && !*gate) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     if (vQuery && !vQuery[v]) {  // interior vertices of tet meshes: unbounded
@@ -769,6 +774,7 @@ __global__ __launch_bounds__(256) void level_nodes_kernel(const uint64_t* keys, 
 }
 
 __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNode* nodes, int n) {
+    if (S.gate && !*S.gate) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     BvhNode& L = nodes[n - 1 + k];
@@ -794,7 +800,8 @@ __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNo
 }
 
 __global__ __launch_bounds__(256) void internal_box_kernel(const int* levelNodes, int begin, int end,
-                                                           BvhNode* nodes, const int* rightChild) {
+                                                           BvhNode* nodes, const int* rightChild, const int* gate) {
+    if (gate && !*gate) return;
     const int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= end) return;
     const int i = levelNodes[k];
@@ -810,7 +817,8 @@ __global__ __launch_bounds__(256) void internal_box_kernel(const int* levelNodes
 // before the segment were finished by earlier launches).
 __global__ __launch_bounds__(1024) void internal_box_levels_kernel(const int* levelNodes, const int* levelBegin,
                                                                    int l0, int l1, BvhNode* nodes,
-                                                                   const int* rightChild) {
+                                                                   const int* rightChild, const int* gate) {
+    if (gate && !*gate) return;  // uniform over the block: no barrier is left waiting
     for (int l = l0; l < l1; ++l) {
         const int b = levelBegin[l], e = levelBegin[l + 1];
         for (int k = b + (int)threadIdx.x; k < e; k += (int)blockDim.x) {
@@ -858,6 +866,33 @@ __global__ __launch_bounds__(1024) void node_mesh_levels_kernel(const int* level
     }
 }
 
+__global__ __launch_bounds__(256) void gated_fill_int_kernel(int* p, int64_t n, int value, const int* gate) {
+    if (gate && !*gate) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// Whether this iteration's detection pass runs: always at the first iteration of a substep,
+// afterwards when the previous iteration truncated a vertex to its bound (the re-detection flag,
+// VBDClothPhysics.cpp:430-446); counts the passes that run.
+__global__ void detection_gate_kernel(const int* flag, int* gate, int* passes, int first) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int g = (first || *flag) ? 1 : 0;
+        *gate = g;
+        if (g) ++*passes;
+    }
+}
+
+// positionsAtPrevCollisionDetection = x and the re-detection flag cleared (a gated pass only
+// when its gate is set)
+__global__ __launch_bounds__(256) void snapshot_kernel(const float4* pos, float4* atCD, int n, int* flag,
+                                                       const int* gate) {
+    if (gate && !*gate) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *flag = 0;
+    for (int k = i; k < n; k += gridDim.x * blockDim.x) atCD[k] = pos[k];
+}
+
 __global__ __launch_bounds__(256) void fill_int_kernel(int* p, int64_t n, int value) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = value;
@@ -895,7 +930,8 @@ __global__ __launch_bounds__(256) void perturb_kernel(const float4* rest, float4
 // ascending, at most `cap` of them (the lowest ids). Built from the stored records only:
 // count per face, exclusive scan, scatter, then an in-place selection of each face's lowest ids.
 __global__ __launch_bounds__(256) void fv_count_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
-                                                       int* counts) {
+                                                       int* counts, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nv * cap) return;
     const int v = (int)(i / cap), k = (int)(i % cap);
@@ -903,7 +939,8 @@ __global__ __launch_bounds__(256) void fv_count_kernel(const int* stored, const 
 }
 
 __global__ __launch_bounds__(256) void fv_scatter_kernel(const int* stored, const VFContactRec* rec, int nv, int cap,
-                                                         const int* offsets, int* fill, int* verts) {
+                                                         const int* offsets, int* fill, int* verts, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nv * cap) return;
     const int v = (int)(i / cap), k = (int)(i % cap);
@@ -915,7 +952,8 @@ __global__ __launch_bounds__(256) void fv_scatter_kernel(const int* stored, cons
 // selection of the `cap` lowest vertex ids of each face into its first slots, ascending (the
 // scatter order is arbitrary; this makes the lists, and every sum over them, deterministic)
 __global__ __launch_bounds__(256) void fv_select_kernel(const int* offsets, int nf, int cap, int* verts,
-                                                        int2* range) {
+                                                        int2* range, const int* gate) {
+    if (gate && !*gate) return;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const int b = offsets[f], e = offsets[f + 1];
@@ -1561,30 +1599,60 @@ void ContactDetector::rebindPositions(float4* pos) {
     pos_ = pos;
 }
 
-void ContactDetector::buildFaceLinks(int cap) {
+void ContactDetector::buildFaceLinks(int cap, const int* gate) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     const int64_t n = (int64_t)nv_ * vfCap_;
     int* counts = fvCounts_;         // nf + 1 (the last stays 0: the scan's total lands at offsets[nf])
     int* fill = fvCounts_ + nf_ + 1;  // nf
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(2 * (int64_t)nf_ + 1)), dim3(256), 0, stream_, fvCounts_,
-                       2 * (int64_t)nf_ + 1, 0);
+    // a gated-off pass leaves the counts as they are, so the scan (not gated) rewrites the same
+    // offsets
+    hipLaunchKernelGGL(gated_fill_int_kernel, dim3(blocks_for(2 * (int64_t)nf_ + 1)), dim3(256), 0, stream_,
+                       fvCounts_, 2 * (int64_t)nf_ + 1, 0, gate);
     hipLaunchKernelGGL(fv_count_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
-                       counts);
+                       counts, gate);
     size_t tb = fvScanTempBytes_;
     hip_check(rocprim::exclusive_scan(fvScanTemp_, tb, counts, fvOffsets_, 0, (size_t)nf_ + 1, rocprim::plus<int>(),
                                       stream_),
               "face link scan");
     hipLaunchKernelGGL(fv_scatter_kernel, dim3(blocks_for(n)), dim3(256), 0, stream_, vfStored_, vfRec_, nv_, vfCap_,
-                       fvOffsets_, fill, fvVerts_);
+                       fvOffsets_, fill, fvVerts_, gate);
     hipLaunchKernelGGL(fv_select_kernel, dim3(blocks_for(nf_)), dim3(256), 0, stream_, fvOffsets_, nf_, cap, fvVerts_,
-                       fvRange_);
+                       fvRange_, gate);
     hip_check(hipGetLastError(), "face link launch");
 }
 
-void ContactDetector::snapshotPositions() {
+void ContactDetector::snapshotPositions(const int* gate) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
-    hip_check(hipMemcpyAsync(xAtCD_, pos_, sizeof(float4) * nv_, hipMemcpyDeviceToDevice, stream_), "snapshot");
-    hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
+    hipLaunchKernelGGL(snapshot_kernel, dim3(std::min(blocks_for(nv_), 4096)), dim3(256), 0, stream_, pos_, xAtCD_, nv_,
+                       flag_, gate);
+    hip_check(hipGetLastError(), "snapshot launch");
+}
+
+void ContactDetector::detectGated(bool first, bool rebuild, float thickness, float relaxation) {
+    if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    if (rebuild && !first) fail(GVBD_ERR_STATE, "tree rebuilds belong to a substep's first detection");
+    hipLaunchKernelGGL(detection_gate_kernel, dim3(1), dim3(64), 0, stream_, flag_, gate_, gatePasses_, first ? 1 : 0);
+    if (rebuild) {
+        buildBvh(fb_, 0);
+        buildBvh(eb_, 1);
+    } else {
+        fork();
+        refitBvh(fb_, 0, stream_, nullptr, gate_);
+        refitBvh(eb_, 1, side_, nullptr, gate_);
+        join();
+    }
+    query(false, gate_);
+    bounds(thickness, relaxation, gate_);
+    buildFaceLinks(32, gate_);
+    snapshotPositions(gate_);
+}
+
+int64_t ContactDetector::gatedPasses() {
+    if (!gatePasses_) return 0;
+    int h = 0;
+    hip_check(hipMemcpyAsync(&h, gatePasses_, sizeof(int), hipMemcpyDeviceToHost, stream_), "passes");
+    hip_check(hipStreamSynchronize(stream_), "passes sync");
+    return h;
 }
 
 DetectorDeviceView ContactDetector::deviceView() const {
@@ -1743,11 +1811,15 @@ void ContactDetector::finalize() {
     fvRange_ = dalloc<int2>(std::max(nf_, 1));
     xAtCD_ = dalloc<float4>(nv_);
     flag_ = dalloc<int>(1);
+    gate_ = dalloc<int>(1);
+    gatePasses_ = dalloc<int>(1);
     hip_check(rocprim::exclusive_scan(nullptr, fvScanTempBytes_, fvCounts_, fvOffsets_, 0, (size_t)nf_ + 1,
                                       rocprim::plus<int>(), stream_),
               "scan size");
     fvScanTemp_ = dalloc<char>(std::max<size_t>(fvScanTempBytes_, 1));
     hip_check(hipMemsetAsync(flag_, 0, sizeof(int), stream_), "flag clear");
+    hip_check(hipMemsetAsync(gate_, 0, sizeof(int), stream_), "gate clear");
+    hip_check(hipMemsetAsync(gatePasses_, 0, sizeof(int), stream_), "pass count clear");
     vMesh_ = dalloc<int>(nv_);
     up(vMesh_, hVMesh_.data(), sizeof(int) * nv_);
     if (masked_) {
@@ -1965,21 +2037,22 @@ void ContactDetector::buildBvh(DeviceBvh& B, int kind, const float4* boxes) {
     }
 }
 
-void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream, const float4* boxes) {
+void ContactDetector::refitBvh(DeviceBvh& B, int kind, hipStream_t stream, const float4* boxes, const int* gate) {
     if (B.n == 0) return;
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.faces = faces_;
     S.edges = edges_;
     S.boxes = boxes;
+    S.gate = gate;
     hipLaunchKernelGGL(leaf_box_kernel, dim3(blocks_for(B.n)), dim3(256), 0, stream, S, kind, B.nodes, B.n);
     for (const auto& seg : B.segments) {
         const int b = B.levelBegin[seg.first], e = B.levelBegin[seg.first + 1];
         if (seg.second == seg.first + 1 && e - b > kBlockLevel)
             hipLaunchKernelGGL(internal_box_kernel, dim3(blocks_for(e - b)), dim3(256), 0, stream, B.levelNodes, b,
-                               e, B.nodes, B.right);
+                               e, B.nodes, B.right, gate);
         else
             hipLaunchKernelGGL(internal_box_levels_kernel, dim3(1), dim3(kBlockLevel), 0, stream, B.levelNodes,
-                               B.dLevelBegin, seg.first, seg.second, B.nodes, B.right);
+                               B.dLevelBegin, seg.first, seg.second, B.nodes, B.right, gate);
     }
     hip_check(hipGetLastError(), "bvh refit launch");
 }
@@ -2196,9 +2269,11 @@ void ContactDetector::perturb(uint64_t seed, int step, float sigma, const float*
     hip_check(hipStreamSynchronize(stream_), "perturb sync");
 }
 
-void ContactDetector::query(bool record) {
+void ContactDetector::query(bool record, const int* gate) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    if (record && gate) fail(GVBD_ERR_STATE, "the record pass is not gated");
     CDView S = make_view(pos_, nv_, nf_, ne_);
+    S.gate = gate;
     S.vMesh = vMesh_;
     S.vQuery = vQuery_;
     S.faces = faces_;
@@ -2231,8 +2306,8 @@ void ContactDetector::query(bool record) {
     float rr = r_;
     int rbits;
     std::memcpy(&rbits, &rr, sizeof(int));
-    hipLaunchKernelGGL(fill_int_kernel, dim3(std::min(blocks_for(nf_), 4096)), dim3(256), 0, stream_, faceMin_,
-                       (int64_t)nf_, rbits);
+    hipLaunchKernelGGL(gated_fill_int_kernel, dim3(std::min(blocks_for(nf_), 4096)), dim3(256), 0, stream_, faceMin_,
+                       (int64_t)nf_, rbits, gate);
     fork();
     if (vfCap_ <= 8)
         hipLaunchKernelGGL((vf_query_kernel<false, 8>), dim3(blocks_for(nv_)), dim3(256), 0, stream_, S);
@@ -2287,10 +2362,10 @@ void ContactDetector::query(bool record) {
     recorded_ = true;
 }
 
-void ContactDetector::bounds(float thickness, float relaxation) {
+void ContactDetector::bounds(float thickness, float relaxation, const int* gate) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
     hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nv_)), dim3(256), 0, stream_, vfMin_, faceMin_, eeMin_,
-                       vfBegin_, vfFace_, veBegin_, veEdge_, vQuery_, nv_, r_, thickness, relaxation, bounds_);
+                       vfBegin_, vfFace_, veBegin_, veEdge_, vQuery_, nv_, r_, thickness, relaxation, bounds_, gate);
     hip_check(hipGetLastError(), "bounds launch");
 }
 

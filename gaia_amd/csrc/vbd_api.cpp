@@ -173,7 +173,8 @@ struct PlanObj {
     bool dirty = true;
     bool capturing = false;  // a frame is being captured into the plan's graph (no events, no host bookkeeping)
     bool contact = false;      // some scene has collision handling (contact kernel variant for its items)
-    bool clothContact = false;  // a cloth-model contact scene (alone in its plan): host-driven iterations
+    bool clothContact = false;  // a cloth-model contact scene: re-detection decided per iteration (on the
+                                // host for a scene stepped alone without a graph, on the device otherwise)
     bool accel = false;    // some scene uses the Chebyshev accelerator
 };
 
@@ -1171,20 +1172,30 @@ bool penetration_rebuild_due(const SceneObj& s) {
     return steps > 0 && s.frame > 0 && s.frame % steps == 0;
 }
 
-// One frame of the tet scenes with penetration contact of a plan, batched with its other scenes
-// (runStepGPU, VBDPhysics.cpp:839-969): per substep DCD of each contact scene at its start (the
-// trees rebuilt at substep 0 of a rebuild frame, refit otherwise), the batched initial step, CCD
-// of each contact scene over its vertices' paths from their start positions and its face links,
-// then the batched iterations and velocities. No host reads inside the frame; with `timing` the
-// detection passes are timed with events read a substep later.
-void plan_penetration_frame(gvbd_context* ctx, PlanObj& p, bool timing) {
-    std::vector<SceneObj*> cs;
+bool cloth_rebuild_due(const SceneObj& s);
+
+// One frame of a plan with contact scenes, batched with its other scenes, without host reads
+// (runStepGPU, VBDPhysics.cpp:839-969, and runStep_CDEveryIter, VBDClothPhysics.cpp:389-518).
+// Per substep: DCD of each penetration-contact scene at its start (the trees rebuilt at substep 0
+// of a rebuild frame, refit otherwise); the cloth deformers; the first detection pass of each
+// cloth-model scene; the batched initial step; CCD of each penetration-contact scene over its
+// vertices' paths from their start positions and its face links; then per iteration a detection
+// pass of each cloth-model scene gated on the device by its re-detection flag (iterations after
+// the first) and the batched iteration; the velocities. With `timing` the penetration passes are
+// timed with events read a substep later.
+void plan_device_contact_frame(gvbd_context* ctx, PlanObj& p, bool timing) {
+    std::vector<SceneObj*> pen, cloth;
     for (int32_t sid : p.scenes) {
         SceneObj& s = scene_of(ctx, sid);
-        if (s.cd) cs.push_back(&s);
+        if (s.cd) (s.cd->penetration() ? pen : cloth).push_back(&s);
     }
+    auto gated = [&](bool first) {
+        for (SceneObj* s : cloth)
+            s->cd->detectGated(first, first && cloth_rebuild_due(*s), s->params.thickness,
+                               s->params.conservative_step_relaxation);
+    };
     for (int sub = 0; sub < p.substeps; ++sub) {
-        for (SceneObj* s : cs) {
+        for (SceneObj* s : pen) {
             if (timing) {
                 collect_detection_time(*s);
                 for (hipEvent_t& e : s->cdEvents)
@@ -1194,8 +1205,14 @@ void plan_penetration_frame(gvbd_context* ctx, PlanObj& p, bool timing) {
             s->cd->penetrationDcd(sub == 0 && penetration_rebuild_due(*s));
             if (timing) hip_check(hipEventRecord(s->cdEvents[1], ctx->stream), "event record");
         }
-        plan_initial_step(ctx, p);
-        for (SceneObj* s : cs) {
+        if (p.iterations > 0 && !cloth.empty()) {
+            plan_deform_cloth(ctx, p);
+            gated(true);
+            plan_initial_step(ctx, p, true);
+        } else {
+            plan_initial_step(ctx, p);
+        }
+        for (SceneObj* s : pen) {
             if (timing) hip_check(hipEventRecord(s->cdEvents[2], ctx->stream), "event record");
             s->cd->penetrationCcd(s->view.prev, sub == 0 && penetration_rebuild_due(*s));
             s->cd->buildFaceLinks(32);
@@ -1204,18 +1221,38 @@ void plan_penetration_frame(gvbd_context* ctx, PlanObj& p, bool timing) {
                 s->detectPending = true;
             }
         }
-        for (int it = 0; it < p.iterations; ++it) plan_iteration(ctx, p, it, it);
+        for (int it = 0; it < p.iterations; ++it) {
+            if (it > 0) gated(false);
+            plan_iteration(ctx, p, it, it);
+        }
         plan_velocity(ctx, p);
     }
 }
 
-// Host bookkeeping of one penetration frame of every contact scene of a plan (counters, frame index).
-void penetration_frame_done(PlanObj& p, gvbd_context* ctx) {
+// Whether a cloth-model contact scene rebuilds its trees in the frame it is about to step
+// (contactBVHReconstructionSteps, VBDClothPhysics.cpp:727-735: at the first detection of every
+// substep of such a frame but the first).
+bool cloth_rebuild_due(const SceneObj& s) {
+    const int steps = s.params.contact_bvh_rebuild_steps;
+    return steps > 0 && s.frame > 0 && s.frame % steps == 0;
+}
+
+bool contact_rebuild_due(const SceneObj& s) {
+    return s.cd->penetration() ? penetration_rebuild_due(s) : cloth_rebuild_due(s);
+}
+
+// Host bookkeeping of one device-driven frame of every contact scene of a plan (counters, frame
+// index); the gated detection passes of cloth-model scenes are counted on the device.
+void contact_frame_done(PlanObj& p, gvbd_context* ctx) {
     for (int32_t sid : p.scenes) {
         SceneObj& s = scene_of(ctx, sid);
         if (!s.cd) continue;
-        if (penetration_rebuild_due(s)) ++s.contactStats.rebuilds;
-        s.contactStats.detections += p.substeps;
+        if (s.cd->penetration()) {
+            if (penetration_rebuild_due(s)) ++s.contactStats.rebuilds;
+            s.contactStats.detections += p.substeps;
+        } else if (cloth_rebuild_due(s) && p.iterations > 0) {
+            s.contactStats.rebuilds += p.substeps;
+        }
         s.contactStats.frames += p.substeps;
         s.contactStats.iterations += (int64_t)p.substeps * p.iterations;
         ++s.frame;
@@ -1225,7 +1262,8 @@ void penetration_frame_done(PlanObj& p, gvbd_context* ctx) {
 // One frame: substeps x (initial step, iterations x groups, velocity update)
 // (VBDPhysics.cpp:839-969 / VBDClothPhysics.cpp:389-518 structure, all on the device).
 void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
-    if (p.clothContact) {
+    if (p.clothContact && !p.capturing && p.scenes.size() == 1) {
+        // a cloth-model scene stepped alone: the host reads the re-detection flag per iteration
         plan_contact_frame(ctx, p);
         p.epoch = p.iterations;
         return;
@@ -1233,8 +1271,8 @@ void plan_frame_launches(gvbd_context* ctx, PlanObj& p) {
     if (p.contact) {
         // detection is timed for a scene stepped alone (gvbd_contact_stats.detect_ms); batched and
         // served frames are not timed, so that no event wait slows their launch
-        plan_penetration_frame(ctx, p, !p.capturing && p.scenes.size() == 1);
-        if (!p.capturing) penetration_frame_done(p, ctx);
+        plan_device_contact_frame(ctx, p, !p.capturing && p.scenes.size() == 1);
+        if (!p.capturing) contact_frame_done(p, ctx);
         p.epoch = p.iterations;
         return;
     }
@@ -1253,17 +1291,13 @@ bool plan_rebuild_due(gvbd_context* ctx, const PlanObj& p) {
     if (!p.contact) return false;
     for (int32_t sid : p.scenes) {
         const SceneObj& s = scene_of(ctx, sid);
-        if (s.cd && penetration_rebuild_due(s)) return true;
+        if (s.cd && contact_rebuild_due(s)) return true;
     }
     return false;
 }
 
 void plan_capture(gvbd_context* ctx, PlanObj& p) {
     free_plan_graph(p);
-    if (p.clothContact)
-        fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling in the "
-                               "cloth contact model: whether an iteration runs collision detection is decided on "
-                               "the host");
     if (plan_rebuild_due(ctx, p)) return;  // captured after the rebuild frame (see plan_run)
     p.capturing = true;
     hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -1293,7 +1327,7 @@ void plan_run(gvbd_context* ctx, PlanObj& p, int n) {
         if (p.wantGraph && !p.exec) plan_capture(ctx, p);
         if (p.exec) {
             hip_check(hipGraphLaunch(p.exec, ctx->stream), "graph launch");
-            if (p.contact) penetration_frame_done(p, ctx);
+            if (p.contact) contact_frame_done(p, ctx);
         } else {
             plan_frame_launches(ctx, p);
         }
@@ -1315,13 +1349,7 @@ int32_t new_plan(gvbd_context* ctx, const std::vector<int32_t>& scenes) {
         if (s.params.iterations != p->iterations || s.params.num_substeps != p->substeps)
             fail(GVBD_ERR_INVALID, "scenes of one batch must share iterations and substeps");
         if (s.cd) {
-            if (!s.cd->penetration()) {
-                if (scenes.size() > 1)
-                    fail(GVBD_ERR_INVALID, "scenes with collision handling in the cloth contact model run alone "
-                                           "(gvbd_scene_run_step), not in a batch: their re-detection is decided "
-                                           "on the host");
-                p->clothContact = true;
-            }
+            if (!s.cd->penetration()) p->clothContact = true;
             p->contact = true;
         }
     }
@@ -2254,7 +2282,15 @@ int gvbd_scene_contact_stats(gvbd_context* ctx, int32_t scene, gvbd_contact_stat
         if (!s.cd) fail(GVBD_ERR_STATE, "scene has no collision handling");
         collect_detection_time(s);
         *out = s.contactStats;
+        if (!s.cd->penetration()) out->detections += s.cd->gatedPasses();  // device-decided passes
         out->vf_pairs = out->ee_pairs = 0;
+        if (!s.cd->penetration() && out->detections > 0) {
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+            const DetectorResultCounts c = s.cd->counts();
+            out->vf_pairs = c.vfPairs;
+            out->ee_pairs = c.eeContacts;
+            return;
+        }
         if (s.contactStats.detections > 0) {
             hip_check(hipStreamSynchronize(ctx->stream), "sync");
             if (s.cd->penetration()) {
@@ -2457,10 +2493,6 @@ int gvbd_scene_evaluate_convergence_noalloc(gvbd_context* ctx, int32_t scene, do
 int gvbd_scene_record_graph(gvbd_context* ctx, int32_t scene, int32_t enable) {
     return guard(ctx, [&] {
         PlanObj& p = scene_plan(ctx, scene);
-        if (enable && p.clothContact)
-            fail(GVBD_ERR_INVALID, "HIP graph capture is not available for scenes with collision handling in the "
-                                   "cloth contact model: whether an iteration runs collision detection is decided "
-                                   "on the host");
         p.wantGraph = enable != 0;
         free_plan_graph(p);
         if (p.wantGraph) {
@@ -3173,9 +3205,6 @@ int gvbd_server_add_scene(gvbd_context* ctx, int32_t server, int32_t scene) {
         SceneObj& s = scene_of(ctx, scene);
         if (s.server >= 0) fail(GVBD_ERR_STATE, "scene is already registered with a server");
         require(!s.meshes.empty(), "scene has no meshes");
-        if (s.params.handle_collision && !(s.params.tet_contact_model == 1 && s.cloths.empty()))
-            fail(GVBD_ERR_INVALID, "scenes with collision handling in the cloth contact model are stepped alone "
-                                   "(gvbd_scene_run_step), not served: their re-detection is decided on the host");
         for (auto& kv : ctx->plans)
             if (kv.first != s.plan)
                 for (int32_t sid : kv.second->scenes)

@@ -161,7 +161,7 @@ def test_cloth_json_loader(gpu_ctx, tmp_path, oracle):
 def test_cloth_json_default_handle_collision(gpu_ctx, tmp_path):
     """The reference's cloth default handleCollision = true (VBDClothPhysicsParameters.h:60-62:
     contact stiffness 1e4, contact radius 1, query radius 0.4) turns collision handling on: the
-    loaded scene runs the contact path and stays within its conservative bounds."""
+    loaded scene runs the contact path (contact counters exist only for such scenes)."""
     import json
 
     pos, faces = scenes.cloth_sheet(4, 0.5, 1.0)
@@ -176,10 +176,9 @@ def test_cloth_json_default_handle_collision(gpu_ctx, tmp_path):
     a = VBDPhysics(gpu_ctx)
     a.loadRunningparameters(str(tmp_path / "M.json"), str(tmp_path / "P.json"))
     a.initialize()
-    with pytest.raises(gaia_amd.VBDError, match="collision handling"):
-        a.recordVBDSolveGraph(True)  # the contact path is host-driven: no graph
-    a.recordVBDSolveGraph(False)
     a.runSteps(2)
     x, _ = a.getState(0)
     assert np.all(np.isfinite(x))
+    st = a.contactStats()
+    assert st["frames"] == 2 * 20 and st["detections"] >= st["frames"]  # 20 substeps per frame by default
     a.destroy()

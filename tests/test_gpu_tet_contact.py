@@ -6,6 +6,7 @@ normwise-relative."""
 import numpy as np
 import pytest
 
+import gaia_amd
 from gaia_amd import VBDPhysics, scenes
 from gaia_amd.vbd import physics_dict
 from vbd_cases import build_gpu, build_oracle_mixed, rel_err, tet_drop_spec
@@ -126,8 +127,9 @@ def test_contact_scene_without_contacts_matches_plain_scene(gpu_ctx):
 
 
 def test_tet_contact_json_default_off(gpu_ctx, tmp_path):
-    """A tet-only scene loaded without handleCollision keeps the tet framework's default (off):
-    it records a HIP graph, which a scene with collision handling cannot."""
+    """A tet-only scene loaded without handleCollision keeps the tet framework's default (off): it
+    has no contact counters (gvbd_scene_contact_stats exists only for scenes with collision
+    handling)."""
     pos, tets = scenes.cube_tet_mesh(2, 0.1, origin=(0.0, 0.5, 0.0))
     models, params = scenes.write_scene_json(str(tmp_path), "cube", pos, tets,
                                              physics_dict(iterations=2, num_substeps=1),
@@ -137,6 +139,8 @@ def test_tet_contact_json_default_off(gpu_ctx, tmp_path):
     sim.initialize()
     sim.recordVBDSolveGraph(True)
     sim.runStep()
+    with pytest.raises(gaia_amd.VBDError, match="no collision handling"):
+        sim.contactStats()
     sim.destroy()
 
 
