@@ -348,9 +348,9 @@ def contact_bench(args):
     sched = {"vertices_per_entry": 0, "blocks_per_cu": 0, **parse_schedule_key(best)}
     sim.setSchedule(sched["lanes_per_vertex"], sched["block"], mode="group", xcd_blocks=sched.get("xcd_blocks", 0))
     graph = False
-    if penetration and not args.no_graph:
-        # the penetration frame needs no host decision: one HIP graph per frame is a candidate too
-        # (detection is then not timed inside the graph; its time comes from the direct frames)
+    if not args.no_graph:
+        # one HIP graph per frame is a candidate too (the cloth model's re-detection then decided
+        # on the device; detection is not timed inside the graph, its time comes from direct frames)
         sim.recordVBDSolveGraph(True)
         restore()
         sim.runStep()
@@ -374,18 +374,19 @@ def contact_bench(args):
     st1 = sim.contactStats()
     n = args.steps
     rate = nverts * iters * n / wall
+    detections = (st1["detections"] - st0["detections"]) / n  # counted on the device in graph frames
     if graph:
-        # detection time from direct frames after the timed window
+        # detection time per pass from direct frames after the timed window, times the timed
+        # window's passes per frame
         sim.recordVBDSolveGraph(False)
         sd = sim.contactStats()
         for _ in range(3):
             sim.runStep()
         se = sim.contactStats()
-        detect_ms = (se["detect_ms"] - sd["detect_ms"]) / 3
-        detections = (se["detections"] - sd["detections"]) / 3
+        per_pass = (se["detect_ms"] - sd["detect_ms"]) / max(se["detections"] - sd["detections"], 1)
+        detect_ms = per_pass * detections
     else:
         detect_ms = (st1["detect_ms"] - st0["detect_ms"]) / n
-        detections = (st1["detections"] - st0["detections"]) / n
     lowest = min(float(sim.getState(m)[0][:, 1].min()) for m in range(len(bodies)))
     if penetration:
         rows, counts = sim.penetrationResults()
