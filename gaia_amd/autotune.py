@@ -16,7 +16,8 @@ def default_candidates(modes=("group", "queue")):
     out = []
     if "group" in modes:
         out += [dict(lanes_per_vertex=l, block=b, mode="group") for l, b in DEFAULT_GROUP]
-        out += [dict(lanes_per_vertex=l, block=b, mode="group", xcd_blocks=1) for l, b in [(4, 256), (8, 256), (8, 128)]]
+        out += [dict(lanes_per_vertex=l, block=b, mode="group", xcd_blocks=x)
+                for (l, b), x in itertools.product([(8, 256), (8, 128)], [1, 4, 16, 64])]
         for (l, b), st in itertools.product([(4, 128), (8, 128), (8, 256), (16, 256)], [32, 64]):
             out.append(dict(lanes_per_vertex=l, block=b, mode="group", stage_records=st))
     if "queue" in modes:
@@ -30,7 +31,7 @@ def schedule_key(s):
     if s["mode"] == "group" and s.get("stage_records", 0):
         key += f":s{s['stage_records']}"
     if s["mode"] == "group" and s.get("xcd_blocks", 0):
-        key += ":x"
+        key += ":x" if s["xcd_blocks"] == 1 else f":x{s['xcd_blocks']}"
     if s["mode"] == "queue":
         key += f":e{s.get('vertices_per_entry', 0)}"
         if s.get("blocks_per_cu", 0):
@@ -40,7 +41,8 @@ def schedule_key(s):
 
 def parse_schedule_key(key):
     """Inverse of schedule_key: "group:8x128", "group:8x128:s64" (LDS staging, 64 records per
-    chunk), "group:8x256:x" (XCD block mapping), "queue:8x128:e64", "queue:8x128:e64:g2"; a bare
+    chunk), "group:8x256:x" (XCD-contiguous blocks), "group:8x256:x16" (chunks of 16 items dealt to
+    the XCDs in turn), "queue:8x128:e64", "queue:8x128:e64:g2"; a bare
     "8x128" means group launches."""
     parts = key.split(":")
     if len(parts) == 1:
@@ -57,8 +59,8 @@ def parse_schedule_key(key):
             s["blocks_per_cu"] = int(p[1:])
         elif p[0] == "s":
             s["stage_records"] = int(p[1:])
-        elif p == "x":
-            s["xcd_blocks"] = 1
+        elif p[0] == "x":
+            s["xcd_blocks"] = int(p[1:]) if len(p) > 1 else 1
         else:
             raise ValueError(f"bad schedule {key!r}")
     return s

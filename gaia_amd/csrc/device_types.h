@@ -237,6 +237,19 @@ __device__ __forceinline__ int xcd_block_index() {
     return xcd < rem ? xcd * (q + 1) + j : rem * (q + 1) + (xcd - rem) * q + j;
 }
 
+// The same with the range cut into chunks of `chunk` consecutive items dealt to the XCDs in turn
+// (chunk 16: items 0-15 on XCD 0, 16-31 on XCD 1, ...), so neighbouring items share an L2 while
+// the XCDs' shares of a long range stay balanced; the tail after the last full round of 8 chunks
+// keeps block order. A bijection.
+__device__ __forceinline__ int xcd_chunk_index(int chunk) {
+    constexpr int kXcds = 8;
+    const int b = (int)blockIdx.x;
+    const int round = kXcds * chunk;
+    if (b >= ((int)gridDim.x / round) * round) return b;
+    const int xcd = b % kXcds, j = b / kXcds;
+    return ((j / chunk) * kXcds + xcd) * chunk + j % chunk;
+}
+
 // One block's share of a launch: slots [slotBegin, slotBegin + count) of scene `scene`
 // (tet slots or cloth slots by `kind`; vertex ranges for the per-vertex kernels).
 struct WorkItem {

@@ -791,7 +791,8 @@ __global__ __launch_bounds__(BLOCK) void group_solve_kernel(const SceneView* __r
                                                            const WorkItem* __restrict__ items, int iteration,
                                                            int xcdBlocks) {
     __shared__ float4 lds[STAGE > 0 ? 4 * STAGE : 1];
-    const WorkItem it = items[xcdBlocks ? xcd_block_index() : (int)blockIdx.x];
+    const WorkItem it =
+        items[xcdBlocks == 0 ? (int)blockIdx.x : (xcdBlocks == 1 ? xcd_block_index() : xcd_chunk_index(xcdBlocks))];
     const SceneView& S = scenes[it.scene];
     if (it.kind == WORK_TET)
         tet_block<LPV, STAGE, CONTACT>(S, it, iteration, lds);

@@ -2167,7 +2167,9 @@ static void set_schedule(PlanObj& p, const gvbd_schedule& sc) {
     require(!(p.contact && sc.mode == GVBD_MODE_WORK_QUEUE),
             "scenes with collision handling run with group launches (detection runs between the launches)");
     require(!(p.contact && sc.stage_records != 0), "LDS staging is not available for scenes with collision handling");
-    require(sc.xcd_blocks == 0 || sc.xcd_blocks == 1, "xcd_blocks must be 0 or 1");
+    require(sc.xcd_blocks == 0 || sc.xcd_blocks == 1 || sc.xcd_blocks == 4 || sc.xcd_blocks == 16 ||
+                sc.xcd_blocks == 64,
+            "xcd_blocks must be 0, 1, 4, 16 or 64");
     p.lpv = sc.lanes_per_vertex;
     p.xcd = sc.xcd_blocks;
     p.stage = sc.stage_records;

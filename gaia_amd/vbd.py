@@ -729,7 +729,7 @@ def make_schedule(lanes_per_vertex, block, mode="group", vertices_per_entry=0, b
     if mode not in _MODES:
         raise ValueError(f"mode must be one of {sorted(_MODES)}")
     return Schedule(lanes_per_vertex, block, _MODES[mode], vertices_per_entry, blocks_per_cu, spin_limit,
-                    stage_records, int(bool(xcd_blocks)))
+                    stage_records, int(xcd_blocks))
 
 
 def schedule_dict(sc):

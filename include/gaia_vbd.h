@@ -221,8 +221,9 @@ typedef struct gvbd_schedule {
                                    read them straight from HBM; results are identical either way */
     int32_t xcd_blocks;         /* group launches: 1 maps the blocks one XCD runs (the hardware deals blocks to the
                                    8 XCDs round-robin) to one contiguous range of the work items, so the vertices
-                                   an XCD solves at a time are neighbours whose gathers share its L2; 0 takes
-                                   work item = block index. Results are identical either way */
+                                   an XCD solves at a time are neighbours whose gathers share its L2; 4, 16 or 64
+                                   deal chunks of that many consecutive items to the XCDs in turn; 0 takes work
+                                   item = block index. Results are identical either way */
 } gvbd_schedule;
 
 /* sizes of the ABI structs as compiled into the library (binding self-check) */

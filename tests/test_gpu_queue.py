@@ -28,27 +28,27 @@ def states(sims):
     return [s.getState(0) for s in sims]
 
 
-@pytest.mark.parametrize("lpv,block", [(8, 128), (4, 256), (16, 64)])
-def test_xcd_block_mapping_equals_block_order_bitwise(gpu_ctx, lpv, block):
+@pytest.mark.parametrize("lpv,block,xcd", [(8, 128, 1), (4, 256, 16), (16, 64, 64), (8, 256, 4)])
+def test_xcd_block_mapping_equals_block_order_bitwise(gpu_ctx, lpv, block, xcd):
     """Group launches whose blocks take the work items in XCD-contiguous ranges give bit-identical
     positions and velocities to block order (each block's work is the same, only where it runs
     changes), captured or not."""
     specs = scenes.c4_scenes(seed=6, num_scenes=12, vmin=300, vmax=4000)
     ref = [build_spec_scene(gpu_ctx, s) for s in specs]
-    xcd = [build_spec_scene(gpu_ctx, s) for s in specs]
+    mapped = [build_spec_scene(gpu_ctx, s) for s in specs]
     br = Batch(gpu_ctx, ref)
     br.set_schedule(lpv, block, mode="group")
-    bx = Batch(gpu_ctx, xcd)
-    bx.set_schedule(lpv, block, mode="group", xcd_blocks=1)
-    assert bx.get_schedule()["xcd_blocks"] == 1
+    bx = Batch(gpu_ctx, mapped)
+    bx.set_schedule(lpv, block, mode="group", xcd_blocks=xcd)
+    assert bx.get_schedule()["xcd_blocks"] == xcd
     bx.record_graph(True)
     br.run_steps(4)
     bx.run_steps(4)
-    for (xa, va), (xb, vb) in zip(states(ref), states(xcd)):
+    for (xa, va), (xb, vb) in zip(states(ref), states(mapped)):
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     br.destroy()
     bx.destroy()
-    for s in ref + xcd:
+    for s in ref + mapped:
         s.destroy()
 
 
