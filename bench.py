@@ -59,7 +59,9 @@ def solve_kernel_roofline(ctx, batch, schedule, reps=5):
     """Average duration of the dominant vertex-solve launches (HIP events on the library's
     stream) and their algorithmic bytes (gather model).
 
-    group mode: every parallel group's launch, `reps` times each;
+    group mode: `reps` iterations, each every parallel group's launch in order (as in a frame, so
+    no launch reads data a repeat of itself left in the caches), plus each launch alone repeated
+    `reps` times (warm caches) for the balance of the launches;
     queue mode: `reps` persistent launches, each one whole iteration over every group."""
     groups = batch.num_groups()
     work = [batch.group_work(g)[0] for g in range(groups)]
@@ -75,7 +77,16 @@ def solve_kernel_roofline(ctx, batch, schedule, reps=5):
         avg_bytes = float(sum(work))
         kernel = "queue_solve_kernel"
     else:
-        tot_ms = 0.0
+        for g in range(groups):
+            batch.solve_group(g, 0)
+        ctx.synchronize()
+        ctx.event_record(2)
+        for _ in range(reps):
+            for g in range(groups):
+                batch.solve_group(g, 0)
+        ctx.event_record(3)
+        avg_ms = ctx.event_elapsed_ms(2, 3) / (groups * reps)
+        per = []
         for g in range(groups):
             batch.solve_group(g, 0)
             ctx.synchronize()
@@ -83,15 +94,19 @@ def solve_kernel_roofline(ctx, batch, schedule, reps=5):
             for _ in range(reps):
                 batch.solve_group(g, 0)
             ctx.event_record(3)
-            tot_ms += ctx.event_elapsed_ms(2, 3)
-        avg_ms = tot_ms / (groups * reps)
+            per.append(round(ctx.event_elapsed_ms(2, 3) / reps * 1e3, 1))
         avg_bytes = sum(work) / groups
         kernel = "group_solve_kernel"
     gbs = avg_bytes / (avg_ms * 1e-3) / 1e9
-    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=kernel, avg_launch_us=round(avg_ms * 1e3, 2),
-                algorithmic_bytes_per_launch=int(avg_bytes))
+    out = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+               kernel=kernel, avg_launch_us=round(avg_ms * 1e3, 2),
+               algorithmic_bytes_per_launch=int(avg_bytes))
+    if schedule["mode"] != "queue":
+        # per launch of one iteration: µs when repeated alone (warm caches) and algorithmic MB
+        out["launch_us_repeated"] = per
+        out["launch_mb"] = [round(w / 1e6, 1) for w in work]
+    return out
 
 
 def host_cpus():
@@ -396,22 +411,22 @@ def contact_bench(args):
     log(f"contact ({args.contact_model}): {wall / n * 1e3:.2f} ms/step, {rate:.3e} vertex-solves/s, "
         f"{detections:.2f} detections/step ({detect_ms:.2f} ms/step), {found}")
 
-    # the solve kernel's launches (every parallel group, 5 times each) at the final state, timed with
-    # HIP events on the library stream; algorithmic bytes from the tet gather model (contact
+    # the solve kernel's launches (5 iterations of every parallel group in order) at the final state,
+    # timed with HIP events on the library stream; algorithmic bytes from the tet gather model (contact
     # gathers of surface vertices are not counted, so the figure is a lower bound)
     batch = Batch(ctx, [sim])
     groups = batch.num_groups()
     work = [batch.group_work(g)[0] for g in range(groups)]
     reps = 5
-    tot_ms = 0.0
     for g in range(groups):
         sim.solveParallelGroup(g, 0)
-        ctx.synchronize()
-        ctx.event_record(2)
-        for _ in range(reps):
+    ctx.synchronize()
+    ctx.event_record(2)
+    for _ in range(reps):
+        for g in range(groups):
             sim.solveParallelGroup(g, 0)
-        ctx.event_record(3)
-        tot_ms += ctx.event_elapsed_ms(2, 3)
+    ctx.event_record(3)
+    tot_ms = ctx.event_elapsed_ms(2, 3)
     batch.destroy()
     avg_ms = tot_ms / (groups * reps)
     avg_bytes = sum(work) / groups

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -148,6 +149,7 @@ struct PlanObj {
     WorkItem* dVItems = nullptr;  // per-vertex items
     std::vector<int32_t> groupItemBegin;    // G + 1
     std::vector<int32_t> groupContactBegin;  // G: items of scenes with collision handling start here
+    std::vector<int32_t> groupAt;            // scene x launch: the scene's parallel group solved there, or -1
     std::vector<int32_t> groupCommitBegin;  // G + 1
     int32_t nVItems = 0;
     WorkItem* dDeformPre = nullptr;   // deformed vertices of cloth meshes (before the initial step)
@@ -866,6 +868,80 @@ void upload_scene(gvbd_context* ctx, SceneObj& s, const std::function<void(size_
 }
 
 // (Re)builds the work items of a plan for its current schedule.
+// Launch slot of every parallel group of every scene of a plan (group launches): a scene's
+// groups keep their order (group g in a later launch than group g - 1), which is all a scene's
+// results depend on, so any such assignment gives the same bits. A scene with as many groups as
+// the plan has launches takes them one to one; the others, largest first, take the increasing
+// slots that keep the largest launch smallest (a dynamic program over groups and slots), which
+// moves the groups of scenes with few colours into the launches the many-colour scenes leave
+// small.
+void assign_group_launches(gvbd_context* ctx, PlanObj& p) {
+    const int L = p.numGroups;
+    const size_t nS = p.scenes.size();
+    p.groupAt.assign(nS * (size_t)L, -1);
+    std::vector<std::vector<int64_t>> w(nS);
+    std::vector<int64_t> total(nS, 0);
+    for (size_t si = 0; si < nS; ++si) {
+        const SceneObj& s = scene_of(ctx, p.scenes[si]);
+        const size_t nt = s.numTetSlots();
+        w[si].assign(s.groups.size(), 0);
+        for (size_t g = 0; g < s.groups.size(); ++g) {
+            for (int32_t k = s.tetGroupOffset[g]; k < s.tetGroupOffset[g + 1]; ++k) w[si][g] += s.slotWorkBytes[k];
+            for (int32_t k = s.clothGroupOffset[g]; k < s.clothGroupOffset[g + 1]; ++k)
+                w[si][g] += s.slotWorkBytes[nt + k];
+            total[si] += w[si][g];
+        }
+    }
+    std::vector<size_t> order(nS);
+    for (size_t i = 0; i < nS; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const bool fa = (int)w[a].size() == L, fb = (int)w[b].size() == L;
+        return fa != fb ? fa : total[a] > total[b];
+    });
+    std::vector<int64_t> load(L, 0);
+    const int64_t INF = std::numeric_limits<int64_t>::max();
+    for (size_t si : order) {
+        const int G = (int)w[si].size();
+        if (G == 0) continue;
+        std::vector<int> slot(G);
+        if (G == L || p.mode != GVBD_MODE_GROUP_LAUNCHES) {
+            for (int g = 0; g < G; ++g) slot[g] = g;
+        } else {
+            // best[g][t]: the smallest largest load over groups 0..g with group g in slot t
+            std::vector<int64_t> best((size_t)G * L, INF);
+            std::vector<int> from((size_t)G * L, -1);
+            for (int t = 0; t <= L - G; ++t) best[t] = load[t] + w[si][0];
+            for (int g = 1; g < G; ++g) {
+                int64_t run = INF;
+                int arg = -1;
+                for (int t = g; t <= L - G + g; ++t) {
+                    const int64_t prev = best[(size_t)(g - 1) * L + t - 1];
+                    if (prev < run) {
+                        run = prev;
+                        arg = t - 1;
+                    }
+                    if (arg < 0) continue;
+                    best[(size_t)g * L + t] = std::max(load[t] + w[si][g], run);
+                    from[(size_t)g * L + t] = arg;
+                }
+            }
+            int t = -1;
+            for (int u = G - 1; u <= L - 1; ++u)
+                if (best[(size_t)(G - 1) * L + u] != INF && (t < 0 || best[(size_t)(G - 1) * L + u] <
+                                                                           best[(size_t)(G - 1) * L + t]))
+                    t = u;
+            for (int g = G - 1; g >= 0; --g) {
+                slot[g] = t;
+                t = from[(size_t)g * L + t];
+            }
+        }
+        for (int g = 0; g < G; ++g) {
+            p.groupAt[si * L + slot[g]] = g;
+            load[slot[g]] += w[si][g];
+        }
+    }
+}
+
 void build_plan(gvbd_context* ctx, PlanObj& p) {
     if (!p.dirty) return;
     free_plan(p);
@@ -902,14 +978,16 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
         }
     };
     p.groupContactBegin.assign(p.numGroups, 0);
-    for (int g = 0; g < p.numGroups; ++g) {
+    assign_group_launches(ctx, p);
+    for (int t = 0; t < p.numGroups; ++t) {
         // items of scenes without collision handling first (the plain solve variant), then those
         // of scenes with it (the contact variant): each scene's work is the same either way
         for (int pass = 0; pass < 2; ++pass) {
-            if (pass == 1) p.groupContactBegin[g] = (int32_t)items.size();
+            if (pass == 1) p.groupContactBegin[t] = (int32_t)items.size();
             for (size_t si = 0; si < p.scenes.size(); ++si) {
                 SceneObj& s = scene_of(ctx, p.scenes[si]);
-                if (g >= (int)s.groups.size() || (s.cd != nullptr) != (pass == 1)) continue;
+                const int g = p.groupAt[si * p.numGroups + t];
+                if (g < 0 || (s.cd != nullptr) != (pass == 1)) continue;
                 for (int32_t k = s.tetGroupOffset[g], e = s.tetGroupOffset[g + 1]; k < e; k += vpb)
                     items.push_back({(int32_t)si, k, std::min(vpb, e - k), WORK_TET});
                 for (int32_t k = s.clothGroupOffset[g], e = s.clothGroupOffset[g + 1]; k < e; k += vpb)
@@ -918,8 +996,8 @@ void build_plan(gvbd_context* ctx, PlanObj& p) {
                             [&](int32_t k, int32_t n) { commits.push_back({(int32_t)si, k, n, WORK_CLOTH}); });
             }
         }
-        p.groupItemBegin[g + 1] = (int32_t)items.size();
-        p.groupCommitBegin[g + 1] = (int32_t)commits.size();
+        p.groupItemBegin[t + 1] = (int32_t)items.size();
+        p.groupCommitBegin[t + 1] = (int32_t)commits.size();
     }
     // Work queue of one iteration, ordered by (level, scene): solve entries of level g of every
     // scene, then the level's commit entries. Solve entries of (s, g) need every entry of scene s
@@ -2884,13 +2962,13 @@ int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64
         build_plan(ctx, p);
         require(group >= 0 && group < p.numGroups, "group index out of range");
         int64_t b = 0, nvx = 0;
-        for (int32_t sid : p.scenes) {
-            SceneObj& s = scene_of(ctx, sid);
-            if (group >= (int32_t)s.groups.size()) continue;
+        for (size_t si = 0; si < p.scenes.size(); ++si) {
+            SceneObj& s = scene_of(ctx, p.scenes[si]);
+            const int g = p.groupAt[si * p.numGroups + group];  // the scene's group in this launch
+            if (g < 0) continue;
             const size_t nt = s.numTetSlots();
-            for (int32_t k = s.tetGroupOffset[group]; k < s.tetGroupOffset[group + 1]; ++k, ++nvx)
-                b += s.slotWorkBytes[k];
-            for (int32_t k = s.clothGroupOffset[group]; k < s.clothGroupOffset[group + 1]; ++k, ++nvx)
+            for (int32_t k = s.tetGroupOffset[g]; k < s.tetGroupOffset[g + 1]; ++k, ++nvx) b += s.slotWorkBytes[k];
+            for (int32_t k = s.clothGroupOffset[g]; k < s.clothGroupOffset[g + 1]; ++k, ++nvx)
                 b += s.slotWorkBytes[nt + k];
         }
         if (bytes) *bytes = b;
