@@ -440,11 +440,15 @@ GVBD_API int gvbd_batch_run_steps(gvbd_context* ctx, int32_t batch, int32_t n);
 GVBD_API int gvbd_batch_apply_initial_step(gvbd_context* ctx, int32_t batch);
 GVBD_API int gvbd_batch_run_iteration(gvbd_context* ctx, int32_t batch, int32_t iteration);
 GVBD_API int gvbd_batch_update_velocities(gvbd_context* ctx, int32_t batch);
-/* launch only the vertex solve of parallel group `group` (timing of the dominant kernel) */
+/* launch only the vertex solve of launch `group` of an iteration (timing of the dominant kernel).
+ * With group launches a batch has as many launches per iteration as its scene with the most
+ * parallel groups; each scene's groups sit in increasing launches chosen to balance their work,
+ * so launch g holds group g of a scene with that many groups and some group <= g of the others.
+ * Calling the launches 0 .. n-1 in order is one iteration's solve. */
 GVBD_API int gvbd_batch_solve_parallel_group(gvbd_context* ctx, int32_t batch, int32_t group, int32_t iteration);
 GVBD_API int gvbd_batch_num_parallel_groups(gvbd_context* ctx, int32_t batch, int32_t* n);
-/* algorithmic bytes of one solve launch of `group` and its vertex count (gather model: per tet
- * vertex 60 B + 96 B per incident tet, per cloth vertex 60 B + 56 B per face + 116 B per bending
+/* algorithmic bytes of one solve launch `group` and its vertex count (gather model: per tet
+ * vertex 72 B + 88 B per incident tet, per cloth vertex 60 B + 56 B per face + 116 B per bending
  * stencil) */
 GVBD_API int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t group, int64_t* bytes, int64_t* vertices);
 
