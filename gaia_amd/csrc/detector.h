@@ -11,7 +11,11 @@ namespace gvbd {
 
 // One BVH node: box lo/hi; lo.w (int bits) holds an internal node's left child or a leaf's
 // primitive id, hi.w the node's skip link (next node in depth-first order after its subtree,
-// -1 at the end). Internal nodes occupy [0, n-1), leaves [n-1, 2n-1).
+// -1 at the end). Internal nodes occupy [0, n-1), leaves [n-1, 2n-1). An edge leaf's box is
+// exactly its two endpoints' coordinates, and bits 28-30 of its lo.w say, per axis, whether the
+// edge's first vertex holds the hi coordinate: the leaf alone gives both endpoints, bit for bit.
+constexpr int kLeafPrimBits = 28;
+constexpr int kLeafPrimMask = (1 << kLeafPrimBits) - 1;
 struct alignas(16) BvhNode {
     float4 lo;
     float4 hi;
@@ -208,6 +212,7 @@ class ContactDetector {
     float r_;
     int vfCap_, eeCap_;
     bool ready_ = false;
+    bool treeCurrent_ = false;  // the trees were updated after the last position upload / perturb
     size_t bytes_ = 0;
     std::vector<void*> allocs_;
     // host topology (global ids)

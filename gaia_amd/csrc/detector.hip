@@ -473,19 +473,27 @@ __global__ __launch_bounds__(256) void ee_query_kernel(CDView S) {
             continue;
         }
         node = skip;
-        const int e2 = __float_as_int(nd.lo.w);
-        const int4 b = S.edges[e2];
-        if (S.eMesh[e2] == m1) {
-            if (e2 == e1) continue;
-            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) continue;
-        }
-        const V3 q1 = ld(S.pos, b.x), q2 = ld(S.pos, b.y);
+        // the other edge's endpoints from its leaf (the positions of the last refit, which are
+        // the query's): no edge or position loads for the many leaves beyond r
+        const int tag = __float_as_int(nd.lo.w);
+        const int e2 = tag & kLeafPrimMask;
+        const V3 q1 = {(tag >> kLeafPrimBits) & 1 ? nd.hi.x : nd.lo.x, (tag >> (kLeafPrimBits + 1)) & 1 ? nd.hi.y : nd.lo.y,
+                       (tag >> (kLeafPrimBits + 2)) & 1 ? nd.hi.z : nd.lo.z};
+        const V3 q2 = {(tag >> kLeafPrimBits) & 1 ? nd.lo.x : nd.hi.x, (tag >> (kLeafPrimBits + 1)) & 1 ? nd.lo.y : nd.hi.y,
+                       (tag >> (kLeafPrimBits + 2)) & 1 ? nd.lo.z : nd.hi.z};
         V3 c1, c2;
         float mua, mub;
         segment_closest_points(p1, p2, q1, q2, c1, c2, mua, mub);
         const float d = dist(c1, c2);
-        emin = fminf(d, emin);
+        // emin starts at r, so an edge at d >= r (or NaN) leaves it unchanged whether or not the
+        // topology filter below would have skipped it
         if (!(d < r)) continue;
+        if (S.eMesh[e2] == m1) {
+            if (e2 == e1) continue;
+            const int4 b = S.edges[e2];
+            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) continue;
+        }
+        emin = fminf(d, emin);
         const V3 ev2 = q2 - q1;
         bool ok = false;
         if (dot(c2 - p1, ev1) >= 0.f && dot(p2 - c2, ev1) >= 0.f)
@@ -778,7 +786,8 @@ __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNo
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     BvhNode& L = nodes[n - 1 + k];
-    const int prim = __float_as_int(L.lo.w);
+    const int prim = __float_as_int(L.lo.w) & kLeafPrimMask;
+    int tag = prim;
     V3 lo, hi;
     if (kind == 3) {
         const float4 a = S.boxes[2 * prim], b = S.boxes[2 * prim + 1];
@@ -790,12 +799,16 @@ __global__ __launch_bounds__(256) void leaf_box_kernel(CDView S, int kind, BvhNo
         lo = {fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y), fminf(fminf(a.z, b.z), c.z)};
         hi = {fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y), fmaxf(fmaxf(a.z, b.z), c.z)};
     } else {
+        // the same box as min / max of the endpoints, each bound copied from one endpoint, which
+        // the tag bits name
         const int4 e = S.edges[prim];
         const V3 a = ld(S.pos, e.x), b = ld(S.pos, e.y);
-        lo = {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)};
-        hi = {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)};
+        const bool sx = a.x > b.x, sy = a.y > b.y, sz = a.z > b.z;
+        lo = {sx ? b.x : a.x, sy ? b.y : a.y, sz ? b.z : a.z};
+        hi = {sx ? a.x : b.x, sy ? a.y : b.y, sz ? a.z : b.z};
+        tag = prim | ((int)sx | (int)sy << 1 | (int)sz << 2) << kLeafPrimBits;
     }
-    L.lo = make_float4(lo.x, lo.y, lo.z, L.lo.w);
+    L.lo = make_float4(lo.x, lo.y, lo.z, __int_as_float(tag));
     L.hi = make_float4(hi.x, hi.y, hi.z, L.hi.w);
 }
 
@@ -1910,6 +1923,7 @@ void ContactDetector::finalize() {
 }
 
 void ContactDetector::allocBvh(DeviceBvh& B, int n) {
+    if (n > kLeafPrimMask) fail(GVBD_ERR_INVALID, "a tree of " + std::to_string(n) + " primitives exceeds 2^28 - 1");
     B.n = n;
     if (B.n == 0) return;
     B.nodes = dalloc<BvhNode>(2 * (size_t)B.n - 1);
@@ -2230,6 +2244,7 @@ void ContactDetector::updateBvh(bool rebuild) {
         refitBvh(eb_, 1, side_);
         join();
     }
+    treeCurrent_ = true;
 }
 
 void ContactDetector::setPositions(int mesh, const float* pos) {
@@ -2240,6 +2255,7 @@ void ContactDetector::setPositions(int mesh, const float* pos) {
     for (int i = 0; i < n; ++i) p4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
     hip_check(hipMemcpyAsync(pos_ + off, p4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream_), "positions");
     hip_check(hipStreamSynchronize(stream_), "positions sync");
+    treeCurrent_ = false;
 }
 
 void ContactDetector::getPositions(int mesh, float* pos) const {
@@ -2267,10 +2283,13 @@ void ContactDetector::perturb(uint64_t seed, int step, float sigma, const float*
                        seed, step, sigma);
     hip_check(hipGetLastError(), "perturb launch");
     hip_check(hipStreamSynchronize(stream_), "perturb sync");
+    treeCurrent_ = false;
 }
 
 void ContactDetector::query(bool record, const int* gate) {
     if (!ready_) fail(GVBD_ERR_STATE, "detector not finalized");
+    // the EE query reads the other edge's endpoints from the tree's leaves
+    if (!treeCurrent_) fail(GVBD_ERR_STATE, "positions changed since the last tree update (update the BVH before querying)");
     if (record && gate) fail(GVBD_ERR_STATE, "the record pass is not gated");
     CDView S = make_view(pos_, nv_, nf_, ne_);
     S.gate = gate;

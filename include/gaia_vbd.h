@@ -546,7 +546,8 @@ GVBD_API int gvbd_cd_perturb(gvbd_context* ctx, int32_t cd, uint64_t seed, int32
 GVBD_API int gvbd_cd_update_bvh(gvbd_context* ctx, int32_t cd, int32_t rebuild);
 /* contactQueryVF for every vertex and contactQueryEE for every edge, with resetFaceContactInfo
  * (VBDClothPhysics.cpp:735-757); record = 1 also keeps the uncapped lists (gvbd_cd_vf_pairs,
- * gvbd_cd_vf_contacts, gvbd_cd_ee_contacts) */
+ * gvbd_cd_vf_contacts, gvbd_cd_ee_contacts). The trees must hold the current positions:
+ * GVBD_ERR_STATE after gvbd_cd_set_positions or gvbd_cd_perturb without gvbd_cd_update_bvh */
 GVBD_API int gvbd_cd_query(gvbd_context* ctx, int32_t cd, int32_t record);
 /* conservative bounds of applyCollisionDetection (VBDClothPhysics.cpp:775-812); bounds may be NULL */
 GVBD_API int gvbd_cd_bounds(gvbd_context* ctx, int32_t cd, float thickness, float relaxation, float* bounds);

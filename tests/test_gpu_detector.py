@@ -147,3 +147,26 @@ def test_detector_errors(gpu_ctx):
     bad = np.array([[0, 1, 2], [0, 1, 3]], np.int32)  # edge (0, 1) used twice in the same direction
     with pytest.raises(gaia_amd.VBDError, match="non-manifold"):
         ContactDetector(gpu_ctx, [(np.random.rand(4, 3), bad)], max_query_dis=0.01)
+
+
+def test_query_needs_current_trees(gpu_ctx, oracle):
+    """The EE query reads the other edge's endpoints from the edge tree's leaves, so a query after
+    new positions and before the tree update is refused; after the update the sets are the
+    oracle's again."""
+    import gaia_amd
+    meshes, _ = scenes.c3_tori(seed=2, grid=2, nu=16, nv=8, spacing=0.05, major=0.02, minor=0.008)
+    cd = ContactDetector(gpu_ctx, meshes, max_query_dis=0.004)
+    D = oracle.Detector(meshes)
+    cd.query(record=True)  # trees built at set-up from the same positions
+    compare(cd.recorded(), D.detect(cd.positions(), 0.004, "grid", threads=8))
+    pos0 = cd.positions()[: len(meshes[0][0])]
+    cd.set_positions(0, pos0 + np.float32(0.003))
+    with pytest.raises(gaia_amd.VBDError, match="update the BVH"):
+        cd.query(record=True)
+    cd.updateBVH(False)
+    cd.query(record=True)
+    compare(cd.recorded(), D.detect(cd.positions(), 0.004, "grid", threads=8))
+    cd.perturb(seed=1, step=1, sigma=0.001)
+    with pytest.raises(gaia_amd.VBDError, match="update the BVH"):
+        cd.query(record=False)
+    cd.destroy()
