@@ -88,6 +88,43 @@ struct TetSurface {
 
 enum MeshKind { MESH_TET = 0, MESH_CLOTH = 1 };
 
+// Page-locked host memory (hipHostMalloc) that a served scene's arena is copied out to: copies
+// between it and HBM run at the link's rate without the runtime's staging through a pageable
+// bounce buffer. The allocation is kept when the contents are copied back (size() drops to 0),
+// so a scene that is copied out again reuses it.
+class PinnedBuffer {
+  public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() {
+        if (p_) (void)hipHostFree(p_);
+    }
+    void resize(size_t n) {
+        if (n > cap_) {
+            if (p_) (void)hipHostFree(p_);
+            p_ = nullptr;
+            cap_ = 0;
+            if (hipHostMalloc((void**)&p_, n, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                p_ = nullptr;
+                fail(GVBD_ERR_NOMEM, "page-locked host buffer of " + std::to_string(n) + " bytes");
+            }
+            cap_ = n;
+        }
+        n_ = n;
+    }
+    void clear() { n_ = 0; }
+    bool empty() const { return n_ == 0; }
+    size_t size() const { return n_; }
+    char* data() { return p_; }
+    const char* data() const { return p_; }
+
+  private:
+    char* p_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
+};
+
 struct SceneObj {
     gvbd_physics_params params{};
     SolveParams sp{};
@@ -117,7 +154,7 @@ struct SceneObj {
     // serving loop residency: the server that owns it (-1: none) and, while copied out to host
     // memory, the arena's bytes and the device address they were copied from
     int32_t server = -1;
-    std::vector<char> hostCopy;
+    PinnedBuffer hostCopy;
     uintptr_t hostCopyBase = 0;
     std::vector<DeformerHost> deformers;            // gvbd_scene_add_deformer
     std::vector<gvbd_attachment_desc> attachments;  // gvbd_scene_add_attachments
@@ -3039,6 +3076,8 @@ static void server_restore(gvbd_context* ctx, SceneObj& s) {
         (void)hipGetLastError();
         fail(GVBD_ERR_NOMEM, "hipMalloc of " + std::to_string(s.dev.bytes) + " bytes for a served scene failed");
     }
+    // synchronous on the null stream: the frame in flight on the library's (non-blocking) stream
+    // keeps running meanwhile
     if (hipMemcpy(arena, s.hostCopy.data(), s.dev.bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(arena);
@@ -3046,7 +3085,7 @@ static void server_restore(gvbd_context* ctx, SceneObj& s) {
     }
     rebase_scene(s, s.hostCopyBase, reinterpret_cast<uintptr_t>(arena));
     s.dev.arena = arena;
-    std::vector<char>().swap(s.hostCopy);
+    s.hostCopy.clear();  // the page-locked buffer stays for the next copy-out
     s.hostCopyBase = 0;
     if (s.plan >= 0) free_plan(plan_of(ctx, s.plan));
 }

@@ -456,7 +456,8 @@ GVBD_API int gvbd_batch_group_work(gvbd_context* ctx, int32_t batch, int32_t gro
  * No reference counterpart (the reference steps one scene per process). Scenes registered with a
  * server are made resident in HBM on their first step request when their device bytes fit the
  * budget; idle resident scenes (no pending request) are copied out to host memory, least recently
- * used first, to make room, and copied back when asked for again. Each tick advances every
+ * used first, to make room, and copied back when asked for again (a scene copied out once keeps a
+ * page-locked host buffer of its arena's size until it is destroyed). Each tick advances every
  * resident scene with a pending request by one frame in one batched launch sequence (the batch
  * plan's kernels), then admits waiting scenes while that frame runs, then waits for the frame and
  * completes one request per stepped scene. Results equal those of the scene stepped alone. */
