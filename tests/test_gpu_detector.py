@@ -43,10 +43,17 @@ def run_both(gpu_ctx, oracle, meshes, r, rebuild=True):
     return cd, D, ref, pos
 
 
-@pytest.mark.parametrize("case", ["two_sheets", "crumpled0", "crumpled1", "small_tori"])
+@pytest.mark.parametrize("case", ["two_sheets", "signed_zero_sheets", "crumpled0", "crumpled1", "small_tori"])
 def test_small_configurations_match_oracle(gpu_ctx, oracle, case):
     if case == "two_sheets":
         meshes, r = two_sheets(), 0.002
+    elif case == "signed_zero_sheets":
+        # axis-aligned edges (equal endpoint coordinates on two axes) on a sheet at height 0 with
+        # every other vertex at -0.0: the edge leaves' orientation bits meet ties of both signs
+        meshes, r = two_sheets(), 0.002
+        p0 = meshes[0][0].copy()
+        p0[::2, 1] = np.float32(-0.0)
+        meshes[0] = (p0, meshes[0][1])
     elif case.startswith("crumpled"):
         meshes, r = crumpled(int(case[-1])), 0.002
     else:
