@@ -99,7 +99,21 @@ template <int LPV, int STAGE, int Q, class Body>
 __device__ __forceinline__ void for_lane_records(const float4* __restrict__ recs, int blockBegin, int blockEnd,
                                                  int first, int end, float4* lds, Body&& body) {
     if constexpr (STAGE == 0) {
-        for (int k = first; k < end; k += LPV) body(recs + (int64_t)Q * k, k);
+        // the next record's loads are issued before this record's gathers and arithmetic, so a
+        // lane has two records in flight
+        if (first >= end) return;
+        float4 r[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) r[q] = recs[(int64_t)Q * first + q];
+        for (int k = first; k < end; k += LPV) {
+            float4 nx[Q];
+            const int kn = k + LPV < end ? k + LPV : k;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) nx[q] = recs[(int64_t)Q * kn + q];
+            body(r, k);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) r[q] = nx[q];
+        }
     } else {
         for (int c0 = blockBegin; c0 < blockEnd; c0 += STAGE) {
             const int n = min(STAGE, blockEnd - c0);
