@@ -9,7 +9,7 @@ items), and for the work queue the entry size (vertices per entry) and the persi
 """
 import itertools
 
-DEFAULT_GROUP = [(1, 256), (2, 256), (4, 256), (8, 256), (4, 128), (8, 128)]
+DEFAULT_GROUP = [(4, 256), (8, 256), (16, 256), (4, 128), (8, 128), (16, 128)]
 
 
 def default_candidates(modes=("group", "queue")):
